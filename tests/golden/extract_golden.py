@@ -1,0 +1,63 @@
+"""Write the golden fixtures for the standalone hot path from the reference's own data files.
+
+Reads (as data only) the committed corpus inputs ``corpus/standalone/*.npy`` and the measurement
+rows of ``results/standalone-{cpu-hetzner,hetzner,scale-hetzner}.jsonl`` of the reference tree and
+writes:
+
+* ``tests/golden/corpus/<case>.npy`` -- the exact input bytes (numpy, ``allow_pickle=False``);
+* ``tests/golden/expected.json``      -- per case: shape, dtype, sha256 of the input, and for each
+  of ``jacobi``, ``tsqr``, ``polar_qdwh`` the recorded iteration counts (``jacobi_sweeps``,
+  ``tsqr_leaves``, ``polar_iterations``, ``eigen_sweeps``, ``polar_fixed_point_error``) and the
+  worst-case accuracy metrics, for every device the reference recorded (``cpu`` / ``cuda:0``).
+
+Run once in a container where the reference tree is mounted::
+
+    python tests/golden/extract_golden.py /root/reference
+"""
+import hashlib
+import json
+import shutil
+import sys
+from pathlib import Path
+
+import numpy as np
+
+HOT = ('jacobi', 'tsqr', 'polar_qdwh')
+FILES = ('standalone-cpu-hetzner.jsonl', 'standalone-hetzner.jsonl', 'standalone-scale-hetzner.jsonl')
+CASES = ('square32', 'square64', 'batch8x16', 'tall256x16', 'tall2048x32', 'condition1e8',
+         'condition1e12', 'rank_deficient', 'zero', 'clustered_complex', 'wide_complex')
+
+
+def main(ref):
+    ref = Path(ref)
+    here = Path(__file__).resolve().parent
+    (here / 'corpus').mkdir(exist_ok=True)
+    out = {}
+    for case in CASES:
+        src = ref / 'corpus' / 'standalone' / f'{case}.npy'
+        a = np.load(src, allow_pickle=False)
+        shutil.copyfile(src, here / 'corpus' / f'{case}.npy')
+        out[case] = {'shape': list(a.shape), 'dtype': str(a.dtype),
+                     'sha256': hashlib.sha256(a.tobytes(order='C')).hexdigest(), 'runs': []}
+    for name in FILES:
+        for line_no, line in enumerate((ref / 'results' / name).read_text().splitlines(), 1):
+            row = json.loads(line)
+            if row.get('type') != 'measurement' or row.get('status') != 'ok':
+                continue
+            method = row['algorithm'].get('method')
+            case = row['case']['name']
+            if method not in HOT or case not in out:
+                continue
+            if row['input_sha256'] != out[case]['sha256']:
+                raise SystemExit(f'{name}:{line_no}: input hash does not match the corpus file')
+            diag = {k: v for k, v in row['native_diagnostics'].items()
+                    if k in ('jacobi_sweeps', 'tsqr_leaves', 'polar_iterations', 'eigen_sweeps',
+                             'polar_fixed_point_error')}
+            out[case]['runs'].append({'source': f'results/{name}:{line_no}', 'method': method,
+                                      'device': row['device'], 'case': row['case'],
+                                      'diagnostics': diag, 'metrics': row['metrics']['worst']})
+    (here / 'expected.json').write_text(json.dumps(out, indent=1, sort_keys=True) + '\n')
+
+
+if __name__ == '__main__':
+    main(sys.argv[1] if len(sys.argv) > 1 else '/root/reference')
