@@ -1,0 +1,82 @@
+/*
+ * pavelsvd.h -- C ABI of the MI355X (gfx950) standalone SVD hot path.
+ *
+ * The reference (svdbench/standalone, pure Python + PyTorch) has no native interface; these
+ * entry points are the device-side halves of its Python primitives, with the contract of
+ * svdbench/standalone/solvers.py:264-311 and primitives.py: plain device pointers and sizes,
+ * caller-allocated outputs and workspace, a workspace-size query, an int status return and
+ * per-matrix device-side info.  No entry point allocates device memory or synchronises the host,
+ * so every call can be captured in a hipGraph (except where noted).
+ *
+ * Layouts: matrices are row-major (the PyTorch default of the reference), element offsets are
+ * counted in elements of the stated dtype.  `stream` is a hipStream_t (NULL = default stream).
+ */
+#ifndef PAVELSVD_H
+#define PAVELSVD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void *psvd_stream_t;
+
+/* return status */
+#define PSVD_OK 0
+#define PSVD_EINVAL 1     /* bad argument (shape, dtype, flags, workspace too small) */
+#define PSVD_ELAUNCH 2    /* the HIP runtime refused a launch */
+#define PSVD_ETOOBIG 3    /* the problem does not fit this kernel's on-chip budget */
+
+/* dtype codes */
+#define PSVD_F32 0
+#define PSVD_F64 1
+
+/* per-matrix device info: info[2*i] = sweeps (or iterations), info[2*i+1] = status */
+#define PSVD_INFO_OK 0
+#define PSVD_INFO_NOCONV 1      /* maps to RuntimeError('... did not converge ...') */
+#define PSVD_INFO_COMPLETION 2  /* maps to RuntimeError('null-space completion failed ...') */
+#define PSVD_INFO_NONFINITE 3   /* maps to ValueError('input contains NaN or infinity') */
+#define PSVD_INFO_BADSHAPE 4    /* a descriptor violates 1 <= n <= m, m <= max_m, n <= max_n */
+
+/* flags for the Jacobi entry points */
+#define PSVD_SCALE 1        /* divide by max|a| first and multiply sigma back (solvers.py:294-299) */
+#define PSVD_CUT_FLOOR 2    /* zero sigma only below eps*sigma_1 (the Jacobi rotation floor)
+                               instead of eps*max(m,n)*sigma_1 (primitives.py:362) */
+#define PSVD_NO_VECTORS 4   /* sigma only: no V accumulation, U and Vh are not written */
+
+/* Library identification: returns a static string ("pavelsvd <version> gfx950"). */
+const char *psvd_version(void);
+
+/* ---------------------------------------------------------------------------------------
+ * Batched one-sided Jacobi SVD, LDS resident: one workgroup per matrix drawn from a device
+ * work queue (persistent launch), W and V held in LDS, column dot products reduced with
+ * wavefront shuffles and accumulated in FP64, FP64 convergence test, FP64 sigma.
+ * Replaces primitives.jacobi_svd (svdbench/standalone/primitives.py:316-367) including the
+ * round-robin schedule (:309-313), the sort / cut / completion tail (:356-366,
+ * complete_null_columns :430-455) and the max-abs scaling of solvers.svd (:294-299).
+ *
+ * Matrices are tall (m >= n; the caller transposes wide inputs as solvers.py:226-228 does).
+ * Uniform mode (dims == NULL): batch matrices of m_uniform x n_uniform packed contiguously.
+ * Uneven mode: dims[2i] = m_i, dims[2i+1] = n_i; offs[3i] = element offset of A_i and U_i,
+ * offs[3i+1] = offset of s_i, offs[3i+2] = offset of Vh_i (n_i x n_i).  order (optional) is a
+ * permutation of [0, batch) giving the processing order (longest first balances the queue).
+ * max_m / max_n bound the shapes (they size the LDS).  tol <= 0 selects 10*eps(dtype).
+ * lanes = lanes per column pair: 1..64 (power of two) fixed for the batch, or -k (k in 1..64)
+ * for about k rows per lane chosen per matrix (0 means -4).  threads = 256 or 512.
+ * ------------------------------------------------------------------------------------- */
+size_t psvd_jacobi_lds_workspace_bytes(int64_t batch);
+int psvd_jacobi_lds_smem_bytes(int dtype, int max_m, int max_n, int flags, size_t *bytes);
+int psvd_jacobi_lds(int dtype, int64_t batch, int m_uniform, int n_uniform,
+                    const int32_t *dims, const int64_t *offs, const int32_t *order,
+                    int max_m, int max_n,
+                    const void *a, void *u, void *s, void *vh, int32_t *info,
+                    double tol, int max_sweeps, int flags, int lanes, int threads,
+                    void *workspace, size_t workspace_bytes, psvd_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PAVELSVD_H */

@@ -1,0 +1,58 @@
+// CPython binding of the C ABI in include/pavelsvd.h (module `_hip` of the package).
+//
+// Every function takes plain integers (device pointers, sizes, a hipStream_t handle) and returns
+// the C status code; no PyTorch types cross this boundary.  Built with g++ against
+// libpavelsvd.so, which holds the gfx950 kernels.
+#define PY_SSIZE_T_CLEAN
+#include <Python.h>
+
+#include "pavelsvd.h"
+
+#define PTR(x) reinterpret_cast<void *>(static_cast<uintptr_t>(x))
+
+static PyObject *py_version(PyObject *, PyObject *) { return PyUnicode_FromString(psvd_version()); }
+
+static PyObject *py_jacobi_lds_workspace_bytes(PyObject *, PyObject *args) {
+  long long batch;
+  if (!PyArg_ParseTuple(args, "L", &batch)) return nullptr;
+  return PyLong_FromSize_t(psvd_jacobi_lds_workspace_bytes(batch));
+}
+
+static PyObject *py_jacobi_lds_smem_bytes(PyObject *, PyObject *args) {
+  int dtype, m, n, flags;
+  if (!PyArg_ParseTuple(args, "iiii", &dtype, &m, &n, &flags)) return nullptr;
+  size_t bytes = 0;
+  int st = psvd_jacobi_lds_smem_bytes(dtype, m, n, flags, &bytes);
+  return Py_BuildValue("(in)", st, (Py_ssize_t)bytes);
+}
+
+static PyObject *py_jacobi_lds(PyObject *, PyObject *args) {
+  int dtype, mu, nu, max_m, max_n, max_sweeps, flags, lanes, threads;
+  long long batch;
+  unsigned long long dims, offs, order, a, u, s, vh, info, ws, ws_bytes, stream;
+  double tol;
+  if (!PyArg_ParseTuple(args, "iLiiKKKiiKKKKKdiiiiKKK", &dtype, &batch, &mu, &nu, &dims, &offs, &order,
+                        &max_m, &max_n, &a, &u, &s, &vh, &info, &tol, &max_sweeps, &flags, &lanes,
+                        &threads, &ws, &ws_bytes, &stream))
+    return nullptr;
+  int st;
+  Py_BEGIN_ALLOW_THREADS
+  st = psvd_jacobi_lds(dtype, batch, mu, nu, static_cast<const int32_t *>(PTR(dims)),
+                       static_cast<const int64_t *>(PTR(offs)), static_cast<const int32_t *>(PTR(order)),
+                       max_m, max_n, PTR(a), PTR(u), PTR(s), PTR(vh), static_cast<int32_t *>(PTR(info)),
+                       tol, max_sweeps, flags, lanes, threads, PTR(ws), (size_t)ws_bytes, PTR(stream));
+  Py_END_ALLOW_THREADS
+  return PyLong_FromLong(st);
+}
+
+static PyMethodDef kMethods[] = {
+    {"version", py_version, METH_NOARGS, "library version string"},
+    {"jacobi_lds_workspace_bytes", py_jacobi_lds_workspace_bytes, METH_VARARGS, nullptr},
+    {"jacobi_lds_smem_bytes", py_jacobi_lds_smem_bytes, METH_VARARGS, nullptr},
+    {"jacobi_lds", py_jacobi_lds, METH_VARARGS, nullptr},
+    {nullptr, nullptr, 0, nullptr}};
+
+static struct PyModuleDef kModule = {PyModuleDef_HEAD_INIT, "_hip", "gfx950 SVD kernels (C ABI binding)",
+                                     -1, kMethods};
+
+PyMODINIT_FUNC PyInit__hip(void) { return PyModule_Create(&kModule); }

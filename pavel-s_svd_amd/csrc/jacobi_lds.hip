@@ -1,0 +1,475 @@
+// K1: batched one-sided Jacobi SVD with each matrix resident in LDS.
+//
+// One workgroup solves one matrix at a time and pulls the next matrix index from a device work
+// queue (a persistent launch: grid = resident workgroups, not batch).  W (m x n, column-major,
+// odd leading dimension) and V (n x n, column-major) live in LDS for the whole solve.  A round of
+// the round-robin tournament has n/2 disjoint column pairs; each pair is owned by a group of
+// `lanes` lanes that strides over the rows, so the three column dot products of a pair are a
+// group butterfly reduction with wavefront shuffles.  Dot products are accumulated in FP64 (the
+// products of FP32 values are exact in FP64), the rotation is computed in FP64 and applied in
+// the storage type.  Sweep convergence is the reference test (primitives.py:332-352) evaluated
+// on those FP64 sums.
+//
+// Follows svdbench/standalone/primitives.py:316-367 (jacobi_svd), :309-313 (schedule),
+// :430-455 (complete_null_columns) and solvers.py:294-299 (max-abs scaling).
+#include <float.h>
+
+#include "common.hpp"
+
+namespace psvd {
+
+template <typename T>
+struct LdsParams {
+  const T *a;
+  T *u, *s, *vh;
+  int32_t *info;
+  const int32_t *dims;
+  const int64_t *offs;
+  const int32_t *order;
+  int32_t *queue;
+  int64_t batch;
+  int mu, nu, max_m, max_n;
+  double tol;
+  int max_sweeps, flags, lanes;
+};
+
+__host__ __device__ inline int pad_even(int n) { return n + (n & 1); }
+__host__ __device__ inline int lds_ld(int m) { return m | 1; }
+
+template <typename T>
+__host__ __device__ inline size_t lds_bytes(int m, int n, bool vectors) {
+  const int np = pad_even(n), ldw = lds_ld(m);
+  size_t b = sizeof(double) * (size_t)(3 * (np / 2) + 2 * np + m);
+  b += sizeof(T) * ((size_t)np * ldw + (vectors ? (size_t)np * np : 0));
+  b += sizeof(int) * (size_t)np;
+  return (b + 15) & ~size_t(15);
+}
+
+__global__ void zero_queue_kernel(int32_t *q) {
+  if (threadIdx.x == 0) q[0] = 0;
+}
+
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void jacobi_lds_kernel(LdsParams<T> P) {
+  extern __shared__ double smem_d[];
+  __shared__ int s_idx;
+  __shared__ int s_flag[2];
+  __shared__ int s_nk;
+  __shared__ double s_red[NT / kWave];
+  __shared__ int s_redi[NT / kWave];
+
+  constexpr int NW = NT / kWave;
+  const int tid = threadIdx.x;
+  const int lane = tid & (kWave - 1), wave = tid / kWave;
+  const bool vectors = !(P.flags & PSVD_NO_VECTORS);
+  const double eps = Eps<T>::value;
+  const double tol = P.tol > 0.0 ? P.tol : 10.0 * eps;
+
+  for (;;) {
+    if (tid == 0) s_idx = atomicAdd(P.queue, 1);
+    __syncthreads();
+    const int64_t idx = s_idx;
+    __syncthreads();
+    if (idx >= P.batch) return;
+    const int64_t mi = P.order ? (int64_t)P.order[idx] : idx;
+    int m, n;
+    int64_t aoff, soff, voff;
+    if (P.dims) {
+      m = P.dims[2 * mi];
+      n = P.dims[2 * mi + 1];
+      aoff = P.offs[3 * mi];
+      soff = P.offs[3 * mi + 1];
+      voff = P.offs[3 * mi + 2];
+    } else {
+      m = P.mu;
+      n = P.nu;
+      aoff = mi * (int64_t)m * n;
+      soff = mi * (int64_t)n;
+      voff = mi * (int64_t)n * n;
+    }
+    if (m < 1 || n < 1 || n > m || m > P.max_m || n > P.max_n) {
+      if (tid == 0) {
+        P.info[2 * mi] = 0;
+        P.info[2 * mi + 1] = PSVD_INFO_BADSHAPE;
+      }
+      continue;  // uniform across the block: every thread read the same descriptor
+    }
+    const int np = pad_even(n), ldw = lds_ld(m), npairs = np / 2;
+    // lanes per column pair: fixed (> 0) or about -P.lanes rows per lane (<= 0; 0 means 4)
+    int L = P.lanes;
+    if (L <= 0) {
+      const int rpl = L < 0 ? -L : 4;
+      const int target = (m + rpl - 1) / rpl;
+      L = 4;
+      while (L < target && L < kWave) L <<= 1;
+    }
+    const int grp = tid / L, lig = tid - grp * L, ngrp = NT / L;
+    double *dots = smem_d;
+    double *sig = dots + 3 * npairs;
+    double *ytmp = sig + np;
+    double *vt = ytmp + np;
+    T *W = reinterpret_cast<T *>(vt + m);
+    T *V = W + (size_t)np * ldw;
+    int *ord = reinterpret_cast<int *>(V + (vectors ? (size_t)np * np : 0));
+
+    // ---- load A (row-major) into W (column-major), check finiteness, optional max-abs scale
+    const T *A = P.a + aoff;
+    double amax = 0.0, bad = 0.0;
+    for (int e = tid; e < m * n; e += NT) {
+      const int i = e / n, j = e - i * n;
+      const T x = A[e];
+      W[j * ldw + i] = x;
+      const double ax = fabs((double)x);
+      if (ax <= DBL_MAX) amax = fmax(amax, ax); else bad = 1.0;
+    }
+    if (np > n)
+      for (int i = tid; i < m; i += NT) W[n * ldw + i] = T(0);
+    if (vectors)
+      for (int e = tid; e < np * np; e += NT) V[e] = (e / np == e % np) ? T(1) : T(0);
+    if (block_max<NT>(bad, s_red) > 0.0) {
+      if (tid == 0) {
+        P.info[2 * mi] = 0;
+        P.info[2 * mi + 1] = PSVD_INFO_NONFINITE;
+      }
+      __syncthreads();
+      continue;
+    }
+    double scale = 1.0;
+    if (P.flags & PSVD_SCALE) {
+      const double mx = block_max<NT>(amax, s_red);
+      if (mx > 0.0) {
+        scale = mx;
+        const T st = (T)mx;  // exact: the maximum is one of the stored values
+        for (int e = tid; e < m * n; e += NT) {
+          const int j = e / m, i = e - j * m;
+          W[j * ldw + i] = W[j * ldw + i] / st;
+        }
+      }
+    }
+    __syncthreads();
+
+    // ---- sweeps
+    int sweeps = 0;
+    bool converged = false;
+    for (int sw = 1; sw <= P.max_sweeps; ++sw) {
+      sweeps = sw;
+      if (tid == 0) s_flag[sw & 1] = 0;
+      for (int r = 0; r < np - 1; ++r) {
+        for (int k = grp; k < npairs; k += ngrp) {
+          int p, q;
+          rr_pair(r, k, np, p, q);
+          const T *wp = W + p * ldw, *wq = W + q * ldw;
+          double app = 0.0, aqq = 0.0, apq = 0.0;
+          for (int i = lig; i < m; i += L) {
+            const double x = (double)wp[i], y = (double)wq[i];
+            app = fma(x, x, app);
+            aqq = fma(y, y, aqq);
+            apq = fma(x, y, apq);
+          }
+          app = group_sum(app, L);
+          aqq = group_sum(aqq, L);
+          apq = group_sum(apq, L);
+          if (lig == 0) {
+            dots[3 * k] = app;
+            dots[3 * k + 1] = aqq;
+            dots[3 * k + 2] = apq;
+          }
+        }
+        __syncthreads();
+        // floor = eps^2 * max over the round's pairs of max(app, aqq) (primitives.py:337)
+        double rm = 0.0;
+        for (int k = lane; k < npairs; k += kWave) rm = fmax(rm, fmax(dots[3 * k], dots[3 * k + 1]));
+        const double floor_ = eps * eps * wave_max(rm);
+        for (int k = grp; k < npairs; k += ngrp) {
+          const double app = dots[3 * k], aqq = dots[3 * k + 1], apq = dots[3 * k + 2];
+          const double ratio = fabs(apq) / fmax(sqrt(app) * sqrt(aqq), DBL_MIN);
+          if (ratio > tol && fmin(app, aqq) > floor_) {
+            double c, s;
+            jacobi_rotation(app, aqq, apq, c, s);
+            if (lig == 0) s_flag[sw & 1] = 1;
+            int p, q;
+            rr_pair(r, k, np, p, q);
+            const T ct = (T)c, st = (T)s;
+            T *wp = W + p * ldw, *wq = W + q * ldw;
+            for (int i = lig; i < m; i += L) {
+              const T x = wp[i], y = wq[i];
+              wp[i] = ct * x - st * y;
+              wq[i] = st * x + ct * y;
+            }
+            if (vectors) {
+              T *vp = V + p * np, *vq = V + q * np;
+              for (int i = lig; i < np; i += L) {
+                const T x = vp[i], y = vq[i];
+                vp[i] = ct * x - st * y;
+                vq[i] = st * x + ct * y;
+              }
+            }
+          }
+        }
+        __syncthreads();
+      }
+      if (s_flag[sw & 1] == 0) {
+        converged = true;
+        break;
+      }
+    }
+    int status = converged ? PSVD_INFO_OK : PSVD_INFO_NOCONV;
+
+    // ---- sigma in FP64, descending order, cut
+    for (int c = wave; c < n; c += NW) {
+      const T *wc = W + c * ldw;
+      double acc = 0.0;
+      for (int i = lane; i < m; i += kWave) {
+        const double x = (double)wc[i];
+        acc = fma(x, x, acc);
+      }
+      acc = wave_sum(acc);
+      if (lane == 0) sig[c] = sqrt(acc);
+    }
+    __syncthreads();
+    for (int j = tid; j < n; j += NT) {
+      const double sj = sig[j];
+      int rank = 0;
+      for (int k = 0; k < n; ++k) {
+        const double sk = sig[k];
+        rank += (sk > sj) || (sk == sj && k < j);
+      }
+      ord[rank] = j;
+    }
+    __syncthreads();
+    const double s1 = sig[ord[0]];
+    const double cut = (P.flags & PSVD_CUT_FLOOR) ? eps * s1 : eps * (double)(m > np ? m : np) * s1;
+    if (tid == 0) {
+      int nk = 0;
+      while (nk < n && sig[ord[nk]] > cut) ++nk;
+      s_nk = nk;
+    }
+    __syncthreads();
+    const int nk = s_nk;
+
+    // ---- U = W / sigma for kept columns; coordinate-vector completion for the rest
+    if (vectors) {
+      for (int e = tid; e < n * m; e += NT) {
+        const int r = e / m, i = e - r * m;
+        const int c = ord[r];
+        T *wc = W + c * ldw;
+        wc[i] = r < nk ? (T)((double)wc[i] / sig[c]) : T(0);
+      }
+      __syncthreads();
+      for (int r = nk; r < n; ++r) {
+        double best = -DBL_MAX;
+        int bi = 0x7fffffff;
+        for (int i = tid; i < m; i += NT) {
+          double acc = 0.0;
+          for (int c = 0; c < n; ++c) {
+            const double x = (double)W[c * ldw + i];
+            acc = fma(x, x, acc);
+          }
+          const double lev = 1.0 - acc;
+          if (lev > best) {
+            best = lev;
+            bi = i;
+          }
+        }
+        for (int o = kWave >> 1; o > 0; o >>= 1) {
+          const double ob = __shfl_xor(best, o, kWave);
+          const int oi = __shfl_xor(bi, o, kWave);
+          if (ob > best || (ob == best && oi < bi)) {
+            best = ob;
+            bi = oi;
+          }
+        }
+        __syncthreads();
+        if (lane == 0) {
+          s_red[wave] = best;
+          s_redi[wave] = bi;
+        }
+        __syncthreads();
+        best = s_red[0];
+        bi = s_redi[0];
+        for (int w = 1; w < NW; ++w)
+          if (s_red[w] > best || (s_red[w] == best && s_redi[w] < bi)) {
+            best = s_red[w];
+            bi = s_redi[w];
+          }
+        const int ix = bi;
+        // v = e_ix - U (U^T e_ix)
+        for (int i = tid; i < m; i += NT) {
+          double acc = (i == ix) ? 1.0 : 0.0;
+          for (int c = 0; c < n; ++c) acc -= (double)W[c * ldw + i] * (double)W[c * ldw + ix];
+          vt[i] = acc;
+        }
+        __syncthreads();
+        // v = v - U (U^T v)
+        for (int c = wave; c < n; c += NW) {
+          double acc = 0.0;
+          for (int i = lane; i < m; i += kWave) acc = fma((double)W[c * ldw + i], vt[i], acc);
+          acc = wave_sum(acc);
+          if (lane == 0) ytmp[c] = acc;
+        }
+        __syncthreads();
+        double l2 = 0.0;
+        for (int i = tid; i < m; i += NT) {
+          double acc = vt[i];
+          for (int c = 0; c < n; ++c) acc -= (double)W[c * ldw + i] * ytmp[c];
+          vt[i] = acc;
+          l2 = fma(acc, acc, l2);
+        }
+        const double len = sqrt(block_sum<NT>(l2, s_red));
+        if (len < 10.0 * eps) status = PSVD_INFO_COMPLETION;
+        const double inv = 1.0 / fmax(len, DBL_MIN);
+        T *wc = W + ord[r] * ldw;
+        for (int i = tid; i < m; i += NT) wc[i] = (T)(vt[i] * inv);
+        __syncthreads();
+      }
+    }
+
+    // ---- outputs
+    T *S = P.s + soff;
+    for (int r = tid; r < n; r += NT) S[r] = r < nk ? (T)(sig[ord[r]] * scale) : T(0);
+    if (vectors) {
+      T *U = P.u + aoff;
+      for (int e = tid; e < m * n; e += NT) {
+        const int i = e / n, r = e - i * n;
+        U[e] = W[ord[r] * ldw + i];
+      }
+      T *VH = P.vh + voff;
+      for (int e = tid; e < n * n; e += NT) {
+        const int r = e / n, j = e - r * n;
+        VH[e] = V[ord[r] * np + j];
+      }
+    }
+    if (tid == 0) {
+      P.info[2 * mi] = sweeps;
+      P.info[2 * mi + 1] = status;
+    }
+    __syncthreads();
+  }
+}
+
+constexpr size_t kLdsCap = 160 * 1024;
+
+struct DeviceProps {
+  int cus = 0;
+};
+
+static DeviceProps &props() {
+  static DeviceProps p;
+  if (p.cus == 0) {
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&p.cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (p.cus <= 0) p.cus = 256;
+  }
+  return p;
+}
+
+template <typename T, int NT>
+static int launch_lds(const LdsParams<T> &P, size_t smem, hipStream_t stream) {
+  static bool attr = false;
+  auto kern = jacobi_lds_kernel<T, NT>;
+  if (!attr) {
+    if (hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)(kLdsCap - 4096)) != hipSuccess)
+      return PSVD_ELAUNCH;
+    attr = true;
+  }
+  // occupancy per dynamic-LDS size, queried once outside any stream capture and then reused
+  static size_t last_smem = ~size_t(0);
+  static int occ = 1;
+  if (smem != last_smem) {
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, NT, smem) != hipSuccess || occ < 1) occ = 1;
+    last_smem = smem;
+  }
+  int64_t grid = (int64_t)props().cus * occ;
+  if (grid > P.batch) grid = P.batch;
+  hipLaunchKernelGGL(zero_queue_kernel, dim3(1), dim3(64), 0, stream, P.queue);
+  hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(NT), smem, stream, P);
+  return hipGetLastError() == hipSuccess ? PSVD_OK : PSVD_ELAUNCH;
+}
+
+template <typename T>
+static int run_lds(int64_t batch, int mu, int nu, const int32_t *dims, const int64_t *offs,
+                   const int32_t *order, int max_m, int max_n, const void *a, void *u, void *s,
+                   void *vh, int32_t *info, double tol, int max_sweeps, int flags, int lanes,
+                   int threads, void *ws, size_t ws_bytes, hipStream_t stream) {
+  const bool vectors = !(flags & PSVD_NO_VECTORS);
+  const size_t smem = lds_bytes<T>(max_m, max_n, vectors);
+  if (smem > kLdsCap - 4096) return PSVD_ETOOBIG;
+  if (lanes > kWave || (lanes > 0 && (lanes & (lanes - 1))) || lanes < -kWave) return PSVD_EINVAL;
+  LdsParams<T> P;
+  P.a = static_cast<const T *>(a);
+  P.u = static_cast<T *>(u);
+  P.s = static_cast<T *>(s);
+  P.vh = static_cast<T *>(vh);
+  P.info = info;
+  P.dims = dims;
+  P.offs = offs;
+  P.order = order;
+  P.queue = static_cast<int32_t *>(ws);
+  P.batch = batch;
+  P.mu = mu;
+  P.nu = nu;
+  P.max_m = max_m;
+  P.max_n = max_n;
+  P.tol = tol;
+  P.max_sweeps = max_sweeps;
+  P.flags = flags;
+  P.lanes = lanes;
+  (void)ws_bytes;
+  if (threads == 512) return launch_lds<T, 512>(P, smem, stream);
+  if (threads == 256 || threads == 0) return launch_lds<T, 256>(P, smem, stream);
+  return PSVD_EINVAL;
+}
+
+}  // namespace psvd
+
+extern "C" {
+
+const char *psvd_version(void) { return "pavelsvd 0.1.0 gfx950"; }
+
+size_t psvd_jacobi_lds_workspace_bytes(int64_t batch) {
+  (void)batch;
+  return 256;
+}
+
+int psvd_jacobi_lds_smem_bytes(int dtype, int max_m, int max_n, int flags, size_t *bytes) {
+  if (max_m < 1 || max_n < 1 || max_n > max_m || !bytes) return PSVD_EINVAL;
+  const bool vectors = !(flags & PSVD_NO_VECTORS);
+  size_t b;
+  if (dtype == PSVD_F32) b = psvd::lds_bytes<float>(max_m, max_n, vectors);
+  else if (dtype == PSVD_F64) b = psvd::lds_bytes<double>(max_m, max_n, vectors);
+  else return PSVD_EINVAL;
+  *bytes = b;
+  return b > psvd::kLdsCap - 4096 ? PSVD_ETOOBIG : PSVD_OK;
+}
+
+int psvd_jacobi_lds(int dtype, int64_t batch, int m_uniform, int n_uniform, const int32_t *dims,
+                    const int64_t *offs, const int32_t *order, int max_m, int max_n, const void *a,
+                    void *u, void *s, void *vh, int32_t *info, double tol, int max_sweeps, int flags,
+                    int lanes, int threads, void *workspace, size_t workspace_bytes,
+                    psvd_stream_t stream) {
+  if (batch < 0 || max_sweeps < 1 || !a || !s || !info || !workspace) return PSVD_EINVAL;
+  if (workspace_bytes < psvd_jacobi_lds_workspace_bytes(batch)) return PSVD_EINVAL;
+  if (!(flags & PSVD_NO_VECTORS) && (!u || !vh)) return PSVD_EINVAL;
+  if (dims == nullptr) {
+    if (m_uniform < 1 || n_uniform < 1 || n_uniform > m_uniform) return PSVD_EINVAL;
+    max_m = m_uniform;
+    max_n = n_uniform;
+  } else if (!offs || max_m < 1 || max_n < 1) {
+    return PSVD_EINVAL;
+  }
+  if (batch == 0) return PSVD_OK;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (dtype == PSVD_F32)
+    return psvd::run_lds<float>(batch, m_uniform, n_uniform, dims, offs, order, max_m, max_n, a, u, s,
+                                vh, info, tol, max_sweeps, flags, lanes, threads, workspace,
+                                workspace_bytes, st);
+  if (dtype == PSVD_F64)
+    return psvd::run_lds<double>(batch, m_uniform, n_uniform, dims, offs, order, max_m, max_n, a, u,
+                                 s, vh, info, tol, max_sweeps, flags, lanes, threads, workspace,
+                                 workspace_bytes, st);
+  return PSVD_EINVAL;
+}
+
+}  // extern "C"

@@ -1,0 +1,69 @@
+"""Binding to the gfx950 kernel library (``libpavelsvd.so`` through the ``_hip`` module).
+
+There is no fallback: importing this module without the built library raises ImportError, and
+every entry point refuses tensors that are not on a ROCm device.
+"""
+import torch
+
+try:
+    from . import _hip
+except ImportError as exc:  # pragma: no cover - exercised only on an unbuilt tree
+    raise ImportError('the gfx950 kernel library is not built: run `python pavel-s_svd_amd/build.py` '
+                      '(or __graft_entry__.build()) first') from exc
+
+F32, F64 = 0, 1
+DTYPE_CODE = {torch.float32: F32, torch.float64: F64}
+
+OK, EINVAL, ELAUNCH, ETOOBIG = 0, 1, 2, 3
+INFO_OK, INFO_NOCONV, INFO_COMPLETION, INFO_NONFINITE, INFO_BADSHAPE = 0, 1, 2, 3, 4
+
+SCALE, CUT_FLOOR, NO_VECTORS = 1, 2, 4
+
+_STATUS = {EINVAL: 'invalid argument', ELAUNCH: 'kernel launch failed', ETOOBIG: 'problem exceeds the kernel budget'}
+
+
+class NativeError(RuntimeError):
+    """A C-ABI call returned a non-zero status."""
+
+    def __init__(self, what, status):
+        super().__init__(f'{what}: {_STATUS.get(status, status)}')
+        self.status = status
+
+
+def check(status, what):
+    if status != OK:
+        raise NativeError(what, status)
+
+
+def version():
+    return _hip.version()
+
+
+def stream_of(t):
+    """hipStream_t handle of torch's current stream on the tensor's device."""
+    return torch.cuda.current_stream(t.device).cuda_stream
+
+
+def ptr(t):
+    return 0 if t is None else t.data_ptr()
+
+
+def dtype_code(dtype):
+    try:
+        return DTYPE_CODE[dtype]
+    except KeyError:
+        raise TypeError(f'the gfx950 kernels take float32/float64, not {dtype}') from None
+
+
+def raise_for_info(info, algorithm_error):
+    """Map per-matrix device status codes to the reference's exceptions
+    (primitives.py:355, :451-452; solvers.py:291-292)."""
+    codes = info[..., 1]
+    if bool((codes == INFO_NONFINITE).any()):
+        raise ValueError('input contains NaN or infinity')
+    if bool((codes == INFO_BADSHAPE).any()):
+        raise ValueError('matrix descriptor outside 1 <= n <= m <= max_m, n <= max_n')
+    if bool((codes == INFO_NOCONV).any()):
+        raise RuntimeError(algorithm_error)
+    if bool((codes == INFO_COMPLETION).any()):
+        raise RuntimeError('null-space completion failed: supplied live columns are not independent')

@@ -1,0 +1,59 @@
+"""Device primitives of the standalone hot path, each one gfx950 kernel launch (or a fixed
+sequence of them) through the C ABI of ``include/pavelsvd.h``.
+
+Mirrors the names and return conventions of svdbench/standalone/primitives.py for the hot path:
+
+    jacobi_svd   primitives.py:316-367  -> K1 (LDS-resident batched one-sided Jacobi)
+
+Tensors must live on a ROCm device; there is no CPU path.
+"""
+import torch
+
+from .. import native as N
+
+
+def lds_fits(dtype, m, n, vectors=True):
+    """Whether an m x n (m >= n) matrix fits the LDS-resident Jacobi kernel."""
+    st, _ = N._hip.jacobi_lds_smem_bytes(N.dtype_code(dtype), m, n, 0 if vectors else N.NO_VECTORS)
+    return st == N.OK
+
+
+def jacobi_lds_uniform(x, *, tol=None, max_sweeps=60, scale=False, cut='reference', vectors=True,
+                       lanes=0, threads=256):
+    """One-sided Jacobi SVD of every (m, n) matrix of the contiguous tall batch ``x`` (b, m, n).
+
+    Returns U (b, m, n), s (b, n) descending, Vh (b, n, n) (None, s, None without vectors) and
+    the per-matrix info tensor (b, 2) = [sweeps, status] on the device; no host synchronisation.
+    """
+    b, m, n = x.shape
+    if m < n:
+        raise ValueError('jacobi_lds_uniform takes tall matrices (m >= n)')
+    x = x.contiguous()
+    code = N.dtype_code(x.dtype)
+    flags = (N.SCALE if scale else 0) | (N.CUT_FLOOR if cut == 'floor' else 0) | (0 if vectors else N.NO_VECTORS)
+    dev = x.device
+    u = torch.empty_like(x) if vectors else None
+    s = torch.empty(b, n, dtype=x.dtype, device=dev)
+    vh = torch.empty(b, n, n, dtype=x.dtype, device=dev) if vectors else None
+    info = torch.empty(b, 2, dtype=torch.int32, device=dev)
+    wsb = N._hip.jacobi_lds_workspace_bytes(b)
+    ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
+    st = N._hip.jacobi_lds(code, b, m, n, 0, 0, 0, m, n, N.ptr(x), N.ptr(u), N.ptr(s), N.ptr(vh),
+                           N.ptr(info), float(tol or 0.0), int(max_sweeps), flags, int(lanes), int(threads),
+                           N.ptr(ws), wsb, N.stream_of(x))
+    N.check(st, 'psvd_jacobi_lds')
+    return u, s, vh, info
+
+
+def jacobi_svd(A, tol=None, max_sweeps=30, *, cut='reference'):
+    """One-sided Jacobi, round-robin ordering, batched (primitives.py:316-367).
+
+    Returns U (b, m, n), s (b, n) descending, V (b, n, n), sweeps -- the reference's convention
+    (V, not V^H).  Raises RuntimeError on non-convergence or failed completion."""
+    b, m, n = A.shape
+    if not lds_fits(A.dtype, m, n):
+        raise N.NativeError(f'jacobi_svd {m}x{n}', N.ETOOBIG)
+    u, s, vh, info = jacobi_lds_uniform(A, tol=tol, max_sweeps=max_sweeps, cut=cut)
+    info = info.cpu()
+    N.raise_for_info(info, f'one-sided Jacobi did not converge in {max_sweeps} sweeps')
+    return u, s, vh.mT, int(info[:, 0].max()) if b else 0

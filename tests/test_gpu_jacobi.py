@@ -1,0 +1,149 @@
+"""GPU parity of the LDS-resident batched Jacobi kernel against the CPU oracle and FP64 LAPACK.
+
+Bars (stated per test): sigma error over sigma_1 and U/V orthogonality against the FP64 gesvd
+oracle, sweep counts within the reference's recorded counts (the FP64 convergence test may stop a
+sweep earlier or later than the reference's FP32 test), and the reference's exceptions.
+"""
+import numpy as np
+import pytest
+import torch
+
+from conftest import corpus, load_package
+from oracle import matrices as X
+from oracle import metrics as M
+from oracle import restatement as R
+
+pytestmark = pytest.mark.gpu
+
+DEV = 'cuda:0'
+
+
+@pytest.fixture(scope='module')
+def pkg():
+    return load_package()
+
+
+def _np(t):
+    return None if t is None else t.detach().cpu().numpy()
+
+
+def _metrics(a, out):
+    return M.validate(a, tuple(_np(t) for t in out), M.reference(a))['worst']
+
+
+REAL_CASES = ['square32', 'square64', 'batch8x16', 'tall256x16', 'condition1e8', 'condition1e12',
+              'rank_deficient', 'zero']
+
+
+@pytest.mark.parametrize('case', REAL_CASES)
+def test_corpus_against_oracle_and_lapack(pkg, expected, case):
+    a = corpus(case)
+    u, s, vh, info = pkg.svd(torch.from_numpy(a).to(DEV), 'jacobi', return_info=True)
+    ref_u, ref_s, ref_vh, ref_info = R.svd(a, 'jacobi', return_info=True)
+    met = _metrics(a, (u, s, vh))
+    eps = np.finfo(a.dtype).eps
+    # sigma: within the oracle's own FP64-LAPACK distance plus a few eps (tolerance 8 eps * sqrt n)
+    top = max(float(np.abs(ref_s).max()), np.finfo(float).tiny)
+    assert np.max(np.abs(_np(s) - ref_s)) / top <= max(8 * eps * np.sqrt(a.shape[-1]), 1e-300) + \
+        M.validate(a, (ref_u, ref_s, ref_vh), M.reference(a))['worst']['spectrum_error_over_sigma1']
+    bar = 1e-5 if a.dtype == np.float32 else 1e-13
+    for key in ('spectrum_error_over_sigma1', 'orthogonality_u_fro_over_sqrt_cols',
+                'orthogonality_v_fro_over_sqrt_rows', 'reconstruction_relative_fro'):
+        if met[key] is not None:
+            assert met[key] <= bar, (key, met[key])
+    golden = [r['diagnostics']['jacobi_sweeps'] for r in expected[case]['runs'] if r['method'] == 'jacobi']
+    assert abs(info['jacobi_sweeps'] - golden[0]) <= 2, (info['jacobi_sweeps'], golden)
+    assert info['jacobi_sweeps'] >= 1
+
+
+def test_wide_matrix_is_solved_transposed(pkg):
+    a = X.make_matrix(dict(m=9, n=23, family='gaussian', dtype='float64', seed=3))
+    u, s, vh = pkg.svd(torch.from_numpy(a).to(DEV), 'jacobi')
+    assert tuple(u.shape) == (9, 9) and tuple(vh.shape) == (9, 23)
+    met = _metrics(a, (u, s, vh))
+    assert met['reconstruction_relative_fro'] < 1e-14 and met['orthogonality_v_fro_over_sqrt_rows'] < 1e-14
+
+
+@pytest.mark.parametrize('m,n', [(1, 1), (8, 1), (7, 7), (33, 17), (96, 96), (120, 40)])
+def test_shapes_odd_and_degenerate(pkg, m, n):
+    a = X.make_matrix(dict(m=m, n=n, family='gaussian', dtype='float32', seed=m * 100 + n))
+    u, s, vh = pkg.svd(torch.from_numpy(a).to(DEV), 'jacobi')
+    met = _metrics(a, (u, s, vh))
+    assert met['spectrum_error_over_sigma1'] < 1e-5
+    assert met['orthogonality_u_fro_over_sqrt_cols'] < 1e-5
+    assert met['orthogonality_v_fro_over_sqrt_rows'] < 1e-5
+
+
+def test_rank_deficient_completion_and_zero(pkg):
+    for a in (np.zeros((12, 8), np.float32), np.diag([0., 2., 0., 3., 1.]),
+              X.make_matrix(dict(m=15, n=7, dtype='float64', family='low_rank', rank=3, tail=0))):
+        u, s, vh = pkg.svd(torch.from_numpy(np.ascontiguousarray(a)).to(DEV), 'jacobi')
+        _, rs, _ = R.svd(a, 'jacobi')
+        assert np.array_equal(_np(s) == 0, rs == 0)          # same cut decisions as the oracle
+        uu = _np(u).astype(np.float64)
+        assert np.allclose(uu.T @ uu, np.eye(uu.shape[1]), atol=1e-5)
+
+
+def test_values_only_matches_vectors_bitwise(pkg):
+    a = torch.from_numpy(X.make_matrix(dict(m=40, n=24, batch=3, dtype='float32', family='gaussian'))).to(DEV)
+    _, s1, _ = pkg.svd(a, 'jacobi')
+    none, s2, none2 = pkg.svd(a, 'jacobi', values_only=True)
+    assert none is None and none2 is None
+    assert torch.equal(s1, s2)
+
+
+def test_errors_map_to_reference_exceptions(pkg):
+    a = torch.from_numpy(np.random.default_rng(0).normal(size=(12, 9))).to(DEV)
+    with pytest.raises(RuntimeError):
+        pkg.svd(a, 'jacobi', max_sweeps=1)
+    with pytest.raises(ValueError):
+        pkg.svd(a * float('nan'), 'jacobi')
+    saved = a.clone()
+    pkg.svd(a, 'jacobi')
+    assert torch.equal(a, saved)                                  # input never overwritten
+
+
+def test_uneven_batch_against_lapack_and_oracle(pkg):
+    from importlib import import_module
+    B = import_module('pavel-s_svd_amd.standalone.batched')
+    flat, ms, ns, offs = X.workload_uneven_batch(count=300, seed=11)
+    batch = B.UnevenBatch(torch.from_numpy(flat).to(DEV), ms, ns)
+    batch.configure(cut='floor').solve()
+    stats = batch.check()
+    mats = X.unpack(flat, ms, ns, offs)
+    worst = {}
+    for i, ((u, s, vh), a) in enumerate(zip(batch.results(), mats)):
+        met = _metrics(a, (u, s, vh))
+        for k in ('spectrum_error_over_sigma1', 'orthogonality_u_fro_over_sqrt_cols',
+                  'orthogonality_v_fro_over_sqrt_rows', 'reconstruction_relative_fro'):
+            worst[k] = max(worst.get(k, 0.0), met[k])
+        if i < 20:                                                   # oracle (reference FP32 path)
+            _, rs, _ = R.svd(a, 'jacobi')
+            assert np.max(np.abs(_np(s) - rs)) / rs[0] < 2e-5
+    assert all(v <= 1e-5 for v in worst.values()), worst
+    assert 1 <= stats['jacobi_sweeps'] <= 20
+
+
+def test_uneven_graph_replay_matches_eager(pkg):
+    from importlib import import_module
+    B = import_module('pavel-s_svd_amd.standalone.batched')
+    flat, ms, ns, _ = X.workload_uneven_batch(count=64, seed=12)
+    batch = B.UnevenBatch(torch.from_numpy(flat).to(DEV), ms, ns)
+    batch.solve()
+    torch.cuda.synchronize()
+    eager = (batch.u.clone(), batch.s.clone(), batch.vh.clone())
+    batch.capture()
+    for _ in range(3):
+        batch.u.zero_(); batch.s.zero_(); batch.vh.zero_()
+        torch.cuda.synchronize()
+        batch.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(batch.u, eager[0]) and torch.equal(batch.s, eager[1]) and torch.equal(batch.vh, eager[2])
+        batch.check()
+
+
+def test_factory_plugin_hook(pkg):
+    a = torch.from_numpy(corpus('square32')).to(DEV)
+    run = pkg.factory(a, {'name': 'pavel-s_svd_amd:factory', 'method': 'jacobi'})
+    u, s, vh = run()
+    assert run.diagnostics['jacobi_sweeps'] >= 1 and run.metadata['vendor_factorizations'] is False
