@@ -75,6 +75,19 @@ int psvd_jacobi_lds(int dtype, int64_t batch, int m_uniform, int n_uniform,
                     double tol, int max_sweeps, int flags, int lanes, int threads,
                     void *workspace, size_t workspace_bytes, psvd_stream_t stream);
 
+/* ---------------------------------------------------------------------------------------
+ * One-sided Jacobi SVD of ONE tall matrix that does not fit LDS (e.g. 512x256, 2048x32, the
+ * n x n R of TSQR), as a single cooperative launch: W and V in the workspace (HBM), each
+ * workgroup owns a strided share of every round's column pairs, one grid barrier per round,
+ * FP64 dot products / convergence test / sigma, sort, cut and completion on the device.
+ * Same algorithm and flags as psvd_jacobi_lds (primitives.py:316-367); info = [sweeps, status].
+ * Not capturable in a hipGraph (cooperative launch).
+ * ------------------------------------------------------------------------------------- */
+size_t psvd_jacobi_grid_workspace_bytes(int dtype, int m, int n);
+int psvd_jacobi_grid(int dtype, int m, int n, const void *a, void *u, void *s, void *vh, int32_t *info,
+                     double tol, int max_sweeps, int flags, void *workspace, size_t workspace_bytes,
+                     psvd_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -7,11 +7,12 @@ size bucket, vectors) and, when ``PAVELSVD_TUNE_CACHE`` names a JSON file, acros
 """
 import json
 import os
+import sys
 
 import torch
 
 CANDIDATES = [dict(lanes=lanes, threads=threads) for threads in (256, 512)
-              for lanes in (-2, -4, -8, 8, 16, 32)]
+              for lanes in (-4, -8, -12, 4, 8, 16)]
 _CACHE = {}
 
 
@@ -46,9 +47,11 @@ def time_launch(batch, reps=3):
     return e0.elapsed_time(e1) / reps
 
 
-def tune_uneven(batch, candidates=None, reps=3, verbose=False):
+def tune_uneven(batch, candidates=None, reps=3, verbose=None):
     """Return the fastest launch parameters for ``batch`` (an UnevenBatch) and leave the batch
     configured with them."""
+    if verbose is None:
+        verbose = bool(os.environ.get('PAVELSVD_TUNE_VERBOSE'))
     if not _CACHE:
         _load()
     key = _key(batch)
@@ -63,7 +66,7 @@ def tune_uneven(batch, candidates=None, reps=3, verbose=False):
         ms = time_launch(batch, reps)
         timings.append((ms, cand))
         if verbose:
-            print(f'autotune {key} {cand}: {ms:.3f} ms', flush=True)
+            print(f'autotune {key} {cand}: {ms:.3f} ms', file=sys.stderr, flush=True)
     ms, best = min(timings, key=lambda t: t[0])
     batch.configure(**{k: v for k, v in saved.items() if k not in best}, **best)
     _CACHE[key] = {'params': best, 'ms': ms, 'timings': [[t, c] for t, c in timings]}

@@ -45,11 +45,34 @@ static PyObject *py_jacobi_lds(PyObject *, PyObject *args) {
   return PyLong_FromLong(st);
 }
 
+static PyObject *py_jacobi_grid_workspace_bytes(PyObject *, PyObject *args) {
+  int dtype, m, n;
+  if (!PyArg_ParseTuple(args, "iii", &dtype, &m, &n)) return nullptr;
+  return PyLong_FromSize_t(psvd_jacobi_grid_workspace_bytes(dtype, m, n));
+}
+
+static PyObject *py_jacobi_grid(PyObject *, PyObject *args) {
+  int dtype, m, n, max_sweeps, flags;
+  unsigned long long a, u, s, vh, info, ws, ws_bytes, stream;
+  double tol;
+  if (!PyArg_ParseTuple(args, "iiiKKKKKdiiKKK", &dtype, &m, &n, &a, &u, &s, &vh, &info, &tol, &max_sweeps,
+                        &flags, &ws, &ws_bytes, &stream))
+    return nullptr;
+  int st;
+  Py_BEGIN_ALLOW_THREADS
+  st = psvd_jacobi_grid(dtype, m, n, PTR(a), PTR(u), PTR(s), PTR(vh), static_cast<int32_t *>(PTR(info)), tol,
+                        max_sweeps, flags, PTR(ws), (size_t)ws_bytes, PTR(stream));
+  Py_END_ALLOW_THREADS
+  return PyLong_FromLong(st);
+}
+
 static PyMethodDef kMethods[] = {
     {"version", py_version, METH_NOARGS, "library version string"},
     {"jacobi_lds_workspace_bytes", py_jacobi_lds_workspace_bytes, METH_VARARGS, nullptr},
     {"jacobi_lds_smem_bytes", py_jacobi_lds_smem_bytes, METH_VARARGS, nullptr},
     {"jacobi_lds", py_jacobi_lds, METH_VARARGS, nullptr},
+    {"jacobi_grid_workspace_bytes", py_jacobi_grid_workspace_bytes, METH_VARARGS, nullptr},
+    {"jacobi_grid", py_jacobi_grid, METH_VARARGS, nullptr},
     {nullptr, nullptr, 0, nullptr}};
 
 static struct PyModuleDef kModule = {PyModuleDef_HEAD_INIT, "_hip", "gfx950 SVD kernels (C ABI binding)",

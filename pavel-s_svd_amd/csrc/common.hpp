@@ -22,6 +22,28 @@ __device__ __forceinline__ double group_sum(double v, int width) {
   return v;
 }
 
+// DPP lane permutation of a double (two 32-bit moves).  CTRL is a gfx9 DPP control word.
+template <int CTRL>
+__device__ __forceinline__ double dpp_double(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+// Sum over aligned groups of L lanes with DPP (quad_perm xor 1, xor 2, row_half_mirror,
+// row_mirror) and wavefront shuffles above 16 lanes.  Each level pairs every lane with a partner
+// in the other half of its block, so all lanes of a group end with the same bits.
+template <int L>
+__device__ __forceinline__ double group_sum_dpp(double v) {
+  if constexpr (L >= 2) v += dpp_double<0xB1>(v);   // quad_perm [1,0,3,2]
+  if constexpr (L >= 4) v += dpp_double<0x4E>(v);   // quad_perm [2,3,0,1]
+  if constexpr (L >= 8) v += dpp_double<0x141>(v);  // row_half_mirror
+  if constexpr (L >= 16) v += dpp_double<0x140>(v); // row_mirror
+  if constexpr (L >= 32) v += __shfl_xor(v, 16, kWave);
+  if constexpr (L >= 64) v += __shfl_xor(v, 32, kWave);
+  return v;
+}
+
 __device__ __forceinline__ double wave_max(double v) {
   for (int o = kWave >> 1; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, kWave));
   return v;
@@ -90,6 +112,47 @@ __device__ __forceinline__ void jacobi_rotation(double app, double aqq, double a
   }
   c = 1.0 / sqrt(1.0 + t * t);
   s = t * c * (apq / a);
+}
+
+// The same rotation with the Jacobi angle evaluated in the storage precision: tau from the FP64
+// sums (difference taken in FP64), t, c, s in T.  `tb` returns t*|apq| for the exact-arithmetic
+// norm update app' = app - t|apq|, aqq' = aqq + t|apq|.
+template <typename T>
+__device__ __forceinline__ void jacobi_rotation_t(double app, double aqq, double apq, T &c, T &s, double &tb);
+
+template <>
+__device__ __forceinline__ void jacobi_rotation_t<float>(double app, double aqq, double apq, float &c, float &s,
+                                                         double &tb) {
+  // t in FP32 (the angle needs no more); c = 1/sqrt(1 + t^2) and s = t c in FP64 (v_rsq_f64 plus
+  // one Newton step) and rounded once, so that c^2 + s^2 = 1 to an ulp and V stays orthogonal
+  // over thousands of rotations.
+  const float fa = (float)fabs(apq);
+  const float tau = (float)(aqq - app) / (2.0f * fa);
+  float t;
+  if (tau == 0.0f) {
+    t = 1.0f;
+  } else {
+    const float at = fabsf(tau);
+    t = at > 1e18f ? 0.5f / at : 1.0f / (at + sqrtf(fmaf(at, at, 1.0f)));
+    t = copysignf(t, tau);
+  }
+  const double td = (double)t;
+  const double x = fma(td, td, 1.0);
+  double y = __builtin_amdgcn_rsq(x);
+  y = y * fma(-0.5 * x * y, y, 1.5);
+  c = (float)y;
+  s = (float)(apq < 0.0 ? -td * y : td * y);
+  tb = td * fabs(apq);
+}
+
+template <>
+__device__ __forceinline__ void jacobi_rotation_t<double>(double app, double aqq, double apq, double &c, double &s,
+                                                          double &tb) {
+  jacobi_rotation(app, aqq, apq, c, s);
+  const double a = fabs(apq);
+  const double tau = (aqq - app) / (2.0 * a);
+  const double t = tau == 0.0 ? 1.0 : copysign(1.0, tau) / (fabs(tau) + hypot(tau, 1.0));
+  tb = t * a;
 }
 
 }  // namespace psvd

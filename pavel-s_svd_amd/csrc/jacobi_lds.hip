@@ -49,11 +49,117 @@ __global__ void zero_queue_kernel(int32_t *q) {
   if (threadIdx.x == 0) q[0] = 0;
 }
 
+// Sweeps of one matrix with L lanes per column pair.  One barrier per round: a group computes
+// the three FP64 dot products of its pair (rows kept in registers when m <= L*kRowsPerLane),
+// decides, rotates and writes back; the round's rotation floor eps^2 * max_j |w_j|^2
+// (primitives.py:337) is the maximum the previous round left behind, gathered with an LDS atomic
+// from the exact-arithmetic norm updates app - t|apq|, aqq + t|apq|.  rmax[3] and flag[3] are
+// triple-buffered so that no slot is cleared while another wave may still read it.
+template <typename T> struct RowsPerLane { static constexpr int value = sizeof(T) == 4 ? 12 : 6; };
+
+template <typename T, int NT, int L>
+__device__ void jacobi_sweeps(T *W, T *V, int m, int np, int ldw, bool vectors, double tol, double eps,
+                              int max_sweeps, unsigned long long *rmax, int *flag, int &sweeps_out,
+                              bool &conv_out) {
+  constexpr int NG = NT / L;
+  constexpr int kRowsPerLane = RowsPerLane<T>::value;
+  const int tid = threadIdx.x;
+  const int grp = tid / L, lig = tid - grp * L;
+  const int npairs = np / 2;
+  const bool inreg = m <= L * kRowsPerLane;
+  const double tol2 = tol * tol, eps2 = eps * eps;
+  int sweeps = 0;
+  bool converged = false;
+  int g = 0;
+  for (int sw = 1; sw <= max_sweeps; ++sw) {
+    sweeps = sw;
+    if (tid == 0) flag[(sw + 1) % 3] = 0;
+    for (int r = 0; r < np - 1; ++r, ++g) {
+      if (tid == 0) rmax[(g + 2) % 3] = 0ull;
+      const double floor_ = eps2 * __longlong_as_double((long long)rmax[g % 3]);
+      for (int k = grp; k < npairs; k += NG) {
+        int p, q;
+        rr_pair(r, k, np, p, q);
+        T *wp = W + p * ldw, *wq = W + q * ldw;
+        T xs[kRowsPerLane], ys[kRowsPerLane];
+        double app = 0.0, aqq = 0.0, apq = 0.0;
+        if (inreg) {
+#pragma unroll
+          for (int j = 0; j < kRowsPerLane; ++j) {
+            const int i = lig + j * L;
+            if (i < m) {
+              xs[j] = wp[i];
+              ys[j] = wq[i];
+              const double x = (double)xs[j], y = (double)ys[j];
+              app = fma(x, x, app);
+              aqq = fma(y, y, aqq);
+              apq = fma(x, y, apq);
+            }
+          }
+        } else {
+          for (int i = lig; i < m; i += L) {
+            const double x = (double)wp[i], y = (double)wq[i];
+            app = fma(x, x, app);
+            aqq = fma(y, y, aqq);
+            apq = fma(x, y, apq);
+          }
+        }
+        app = group_sum_dpp<L>(app);
+        aqq = group_sum_dpp<L>(aqq);
+        apq = group_sum_dpp<L>(apq);
+        double nap = app, naq = aqq;
+        // active: |apq| / (|w_p| |w_q|) > tol and min(app, aqq) > floor (primitives.py:335-338)
+        if (apq * apq > tol2 * app * aqq && fmin(app, aqq) > floor_) {
+          T c, s;
+          double tb;
+          jacobi_rotation_t<T>(app, aqq, apq, c, s, tb);
+          nap = app - tb;
+          naq = aqq + tb;
+          if (lig == 0) flag[sw % 3] = 1;
+          if (inreg) {
+#pragma unroll
+            for (int j = 0; j < kRowsPerLane; ++j) {
+              const int i = lig + j * L;
+              if (i < m) {
+                wp[i] = c * xs[j] - s * ys[j];
+                wq[i] = s * xs[j] + c * ys[j];
+              }
+            }
+          } else {
+            for (int i = lig; i < m; i += L) {
+              const T x = wp[i], y = wq[i];
+              wp[i] = c * x - s * y;
+              wq[i] = s * x + c * y;
+            }
+          }
+          if (vectors) {
+            T *vp = V + p * np, *vq = V + q * np;
+            for (int i = lig; i < np; i += L) {
+              const T x = vp[i], y = vq[i];
+              vp[i] = c * x - s * y;
+              vq[i] = s * x + c * y;
+            }
+          }
+        }
+        if (lig == 0) atomicMax(&rmax[(g + 1) % 3], (unsigned long long)__double_as_longlong(fmax(nap, naq)));
+      }
+      __syncthreads();
+    }
+    if (flag[sw % 3] == 0) {
+      converged = true;
+      break;
+    }
+  }
+  sweeps_out = sweeps;
+  conv_out = converged;
+}
+
 template <typename T, int NT>
-__global__ __launch_bounds__(NT) void jacobi_lds_kernel(LdsParams<T> P) {
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void jacobi_lds_kernel(LdsParams<T> P) {
   extern __shared__ double smem_d[];
   __shared__ int s_idx;
-  __shared__ int s_flag[2];
+  __shared__ int s_flag[3];
+  __shared__ unsigned long long s_rmax[3];
   __shared__ int s_nk;
   __shared__ double s_red[NT / kWave];
   __shared__ int s_redi[NT / kWave];
@@ -103,7 +209,6 @@ __global__ __launch_bounds__(NT) void jacobi_lds_kernel(LdsParams<T> P) {
       L = 4;
       while (L < target && L < kWave) L <<= 1;
     }
-    const int grp = tid / L, lig = tid - grp * L, ngrp = NT / L;
     double *dots = smem_d;
     double *sig = dots + 3 * npairs;
     double *ytmp = sig + np;
@@ -111,6 +216,10 @@ __global__ __launch_bounds__(NT) void jacobi_lds_kernel(LdsParams<T> P) {
     T *W = reinterpret_cast<T *>(vt + m);
     T *V = W + (size_t)np * ldw;
     int *ord = reinterpret_cast<int *>(V + (vectors ? (size_t)np * np : 0));
+    if (tid == 0) {
+      s_flag[0] = s_flag[1] = s_flag[2] = 0;
+      s_rmax[0] = s_rmax[1] = s_rmax[2] = 0ull;
+    }
 
     // ---- load A (row-major) into W (column-major), check finiteness, optional max-abs scale
     const T *A = P.a + aoff;
@@ -147,71 +256,30 @@ __global__ __launch_bounds__(NT) void jacobi_lds_kernel(LdsParams<T> P) {
       }
     }
     __syncthreads();
+    // initial rotation floor: max_j |w_j|^2
+    for (int c = wave; c < n; c += NW) {
+      const T *wc = W + c * ldw;
+      double acc = 0.0;
+      for (int i = lane; i < m; i += kWave) {
+        const double x = (double)wc[i];
+        acc = fma(x, x, acc);
+      }
+      acc = wave_sum(acc);
+      if (lane == 0) atomicMax(&s_rmax[0], (unsigned long long)__double_as_longlong(acc));
+    }
+    __syncthreads();
 
     // ---- sweeps
     int sweeps = 0;
     bool converged = false;
-    for (int sw = 1; sw <= P.max_sweeps; ++sw) {
-      sweeps = sw;
-      if (tid == 0) s_flag[sw & 1] = 0;
-      for (int r = 0; r < np - 1; ++r) {
-        for (int k = grp; k < npairs; k += ngrp) {
-          int p, q;
-          rr_pair(r, k, np, p, q);
-          const T *wp = W + p * ldw, *wq = W + q * ldw;
-          double app = 0.0, aqq = 0.0, apq = 0.0;
-          for (int i = lig; i < m; i += L) {
-            const double x = (double)wp[i], y = (double)wq[i];
-            app = fma(x, x, app);
-            aqq = fma(y, y, aqq);
-            apq = fma(x, y, apq);
-          }
-          app = group_sum(app, L);
-          aqq = group_sum(aqq, L);
-          apq = group_sum(apq, L);
-          if (lig == 0) {
-            dots[3 * k] = app;
-            dots[3 * k + 1] = aqq;
-            dots[3 * k + 2] = apq;
-          }
-        }
-        __syncthreads();
-        // floor = eps^2 * max over the round's pairs of max(app, aqq) (primitives.py:337)
-        double rm = 0.0;
-        for (int k = lane; k < npairs; k += kWave) rm = fmax(rm, fmax(dots[3 * k], dots[3 * k + 1]));
-        const double floor_ = eps * eps * wave_max(rm);
-        for (int k = grp; k < npairs; k += ngrp) {
-          const double app = dots[3 * k], aqq = dots[3 * k + 1], apq = dots[3 * k + 2];
-          const double ratio = fabs(apq) / fmax(sqrt(app) * sqrt(aqq), DBL_MIN);
-          if (ratio > tol && fmin(app, aqq) > floor_) {
-            double c, s;
-            jacobi_rotation(app, aqq, apq, c, s);
-            if (lig == 0) s_flag[sw & 1] = 1;
-            int p, q;
-            rr_pair(r, k, np, p, q);
-            const T ct = (T)c, st = (T)s;
-            T *wp = W + p * ldw, *wq = W + q * ldw;
-            for (int i = lig; i < m; i += L) {
-              const T x = wp[i], y = wq[i];
-              wp[i] = ct * x - st * y;
-              wq[i] = st * x + ct * y;
-            }
-            if (vectors) {
-              T *vp = V + p * np, *vq = V + q * np;
-              for (int i = lig; i < np; i += L) {
-                const T x = vp[i], y = vq[i];
-                vp[i] = ct * x - st * y;
-                vq[i] = st * x + ct * y;
-              }
-            }
-          }
-        }
-        __syncthreads();
-      }
-      if (s_flag[sw & 1] == 0) {
-        converged = true;
-        break;
-      }
+    switch (L) {
+      case 1: jacobi_sweeps<T, NT, 1>(W, V, m, np, ldw, vectors, tol, eps, P.max_sweeps, s_rmax, s_flag, sweeps, converged); break;
+      case 2: jacobi_sweeps<T, NT, 2>(W, V, m, np, ldw, vectors, tol, eps, P.max_sweeps, s_rmax, s_flag, sweeps, converged); break;
+      case 4: jacobi_sweeps<T, NT, 4>(W, V, m, np, ldw, vectors, tol, eps, P.max_sweeps, s_rmax, s_flag, sweeps, converged); break;
+      case 8: jacobi_sweeps<T, NT, 8>(W, V, m, np, ldw, vectors, tol, eps, P.max_sweeps, s_rmax, s_flag, sweeps, converged); break;
+      case 16: jacobi_sweeps<T, NT, 16>(W, V, m, np, ldw, vectors, tol, eps, P.max_sweeps, s_rmax, s_flag, sweeps, converged); break;
+      case 32: jacobi_sweeps<T, NT, 32>(W, V, m, np, ldw, vectors, tol, eps, P.max_sweeps, s_rmax, s_flag, sweeps, converged); break;
+      default: jacobi_sweeps<T, NT, 64>(W, V, m, np, ldw, vectors, tol, eps, P.max_sweeps, s_rmax, s_flag, sweeps, converged); break;
     }
     int status = converged ? PSVD_INFO_OK : PSVD_INFO_NOCONV;
 

@@ -3,7 +3,8 @@ sequence of them) through the C ABI of ``include/pavelsvd.h``.
 
 Mirrors the names and return conventions of svdbench/standalone/primitives.py for the hot path:
 
-    jacobi_svd   primitives.py:316-367  -> K1 (LDS-resident batched one-sided Jacobi)
+    jacobi_svd   primitives.py:316-367  -> K1 (LDS-resident batched one-sided Jacobi) or,
+                                           for matrices beyond LDS, K2 (cooperative launch)
 
 Tensors must live on a ROCm device; there is no CPU path.
 """
@@ -45,15 +46,48 @@ def jacobi_lds_uniform(x, *, tol=None, max_sweeps=60, scale=False, cut='referenc
     return u, s, vh, info
 
 
+def jacobi_grid(x, *, tol=None, max_sweeps=60, scale=False, cut='reference', vectors=True):
+    """One-sided Jacobi SVD of each tall matrix of ``x`` (b, m, n) too large for LDS: one
+    cooperative launch per matrix.  Same outputs as ``jacobi_lds_uniform``."""
+    b, m, n = x.shape
+    if m < n:
+        raise ValueError('jacobi_grid takes tall matrices (m >= n)')
+    x = x.contiguous()
+    code = N.dtype_code(x.dtype)
+    flags = (N.SCALE if scale else 0) | (N.CUT_FLOOR if cut == 'floor' else 0) | (0 if vectors else N.NO_VECTORS)
+    dev = x.device
+    u = torch.empty_like(x) if vectors else None
+    s = torch.empty(b, n, dtype=x.dtype, device=dev)
+    vh = torch.empty(b, n, n, dtype=x.dtype, device=dev) if vectors else None
+    info = torch.empty(b, 2, dtype=torch.int32, device=dev)
+    wsb = N._hip.jacobi_grid_workspace_bytes(code, m, n)
+    ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
+    stream = N.stream_of(x)
+    for i in range(b):
+        st = N._hip.jacobi_grid(code, m, n, N.ptr(x[i]), N.ptr(u[i]) if vectors else 0, N.ptr(s[i]),
+                                N.ptr(vh[i]) if vectors else 0, N.ptr(info[i]), float(tol or 0.0),
+                                int(max_sweeps), flags, N.ptr(ws), wsb, stream)
+        N.check(st, 'psvd_jacobi_grid')
+    return u, s, vh, info
+
+
+def jacobi_any(x, **kw):
+    """K1 when the shape fits LDS, else K2."""
+    b, m, n = x.shape
+    if lds_fits(x.dtype, m, n, kw.get('vectors', True)):
+        return jacobi_lds_uniform(x, **kw)
+    kw.pop('lanes', None)
+    kw.pop('threads', None)
+    return jacobi_grid(x, **kw)
+
+
 def jacobi_svd(A, tol=None, max_sweeps=30, *, cut='reference'):
     """One-sided Jacobi, round-robin ordering, batched (primitives.py:316-367).
 
     Returns U (b, m, n), s (b, n) descending, V (b, n, n), sweeps -- the reference's convention
     (V, not V^H).  Raises RuntimeError on non-convergence or failed completion."""
-    b, m, n = A.shape
-    if not lds_fits(A.dtype, m, n):
-        raise N.NativeError(f'jacobi_svd {m}x{n}', N.ETOOBIG)
-    u, s, vh, info = jacobi_lds_uniform(A, tol=tol, max_sweeps=max_sweeps, cut=cut)
+    b = A.shape[0]
+    u, s, vh, info = jacobi_any(A, tol=tol, max_sweeps=max_sweeps, cut=cut)
     info = info.cpu()
     N.raise_for_info(info, f'one-sided Jacobi did not converge in {max_sweeps} sweeps')
     return u, s, vh.mT, int(info[:, 0].max()) if b else 0

@@ -32,14 +32,11 @@ SUPPORTED = ('jacobi',)
 
 def _jacobi(x, stats, opts, scale):
     """Tall batch (b, m, n) -> U, s, Vh with sigma rescaled (solvers.py:24-27 plus :294-299)."""
-    b, m, n = x.shape
     vectors = not opts.get('_values_only', False)
-    if not P.lds_fits(x.dtype, m, n, vectors):
-        raise Unsupported(f'jacobi for {m}x{n} needs the multi-workgroup Jacobi kernel')
     max_sweeps = opts.get('max_sweeps', 60)
-    u, s, vh, info = P.jacobi_lds_uniform(x, tol=opts.get('tol'), max_sweeps=max_sweeps, scale=scale,
-                                          cut=opts.get('cut', 'reference'), vectors=vectors,
-                                          lanes=opts.get('lanes', 0), threads=opts.get('threads', 256))
+    u, s, vh, info = P.jacobi_any(x, tol=opts.get('tol'), max_sweeps=max_sweeps, scale=scale,
+                                  cut=opts.get('cut', 'reference'), vectors=vectors,
+                                  lanes=opts.get('lanes', 0), threads=opts.get('threads', 256))
     info = info.cpu()
     from .. import native as N
     N.raise_for_info(info, f'one-sided Jacobi did not converge in {max_sweeps} sweeps')

@@ -147,3 +147,50 @@ def test_factory_plugin_hook(pkg):
     run = pkg.factory(a, {'name': 'pavel-s_svd_amd:factory', 'method': 'jacobi'})
     u, s, vh = run()
     assert run.diagnostics['jacobi_sweeps'] >= 1 and run.metadata['vendor_factorizations'] is False
+
+
+# --- K2: cooperative one-sided Jacobi for matrices beyond LDS ---------------------------------
+def _prims():
+    from importlib import import_module
+    return import_module('pavel-s_svd_amd.standalone.primitives')
+
+
+def test_grid_kernel_matches_lds_kernel():
+    P = _prims()
+    a = torch.from_numpy(X.make_matrix(dict(m=80, n=37, batch=2, dtype='float64', family='gaussian', seed=4))).to(DEV)
+    u1, s1, vh1, i1 = P.jacobi_lds_uniform(a, scale=True)
+    u2, s2, vh2, i2 = P.jacobi_grid(a, scale=True)
+    torch.cuda.synchronize()
+    assert (i1[:, 1] == 0).all() and (i2[:, 1] == 0).all()
+    assert torch.allclose(s1, s2, rtol=0, atol=1e-13)
+    for u, s, vh in ((u2, s2, vh2),):
+        rec = (u * s[:, None, :]) @ vh
+        assert torch.allclose(rec, a, atol=1e-13)
+
+
+def test_tall2048x32_corpus_via_grid_kernel(pkg, expected):
+    a = corpus('tall2048x32')
+    u, s, vh, info = pkg.svd(torch.from_numpy(a).to(DEV), 'jacobi', return_info=True)
+    met = _metrics(a, (u, s, vh))
+    assert met['spectrum_error_over_sigma1'] < 1e-5 and met['orthogonality_u_fro_over_sqrt_cols'] < 1e-5
+    assert met['orthogonality_v_fro_over_sqrt_rows'] < 1e-5 and met['reconstruction_relative_fro'] < 1e-5
+    assert abs(info['jacobi_sweeps'] - 7) <= 2
+    _, rs, _ = R.svd(a, 'jacobi')
+    assert np.max(np.abs(_np(s) - rs)) / rs[0] < 1e-5
+
+
+def test_single_512x256_geometric_tol_1e7():
+    """Single-matrix workload: 512x256 float32, sigma geometric 1..1e-6, Haar U and V, tol 1e-7
+    with the FP64 convergence test; gates against FP64 LAPACK."""
+    pkg = load_package()
+    a = X.workload_single_jacobi()
+    u, s, vh, info = pkg.svd(torch.from_numpy(a).to(DEV), 'jacobi', tol=1e-7, cut='floor', return_info=True)
+    met = _metrics(a, (u, s, vh))
+    assert met['spectrum_error_over_sigma1'] <= 1e-5, met
+    assert met['orthogonality_u_fro_over_sqrt_cols'] <= 1e-5, met
+    assert met['orthogonality_v_fro_over_sqrt_rows'] <= 1e-5, met
+    assert info['jacobi_sweeps'] <= 30
+    # reference cut semantics: the same sigma are zeroed as by the oracle
+    u2, s2 = pkg.svd(torch.from_numpy(a).to(DEV), 'jacobi')[:2]
+    _, rs, _ = R.svd(a, 'jacobi', max_sweeps=60)
+    assert abs(int((_np(s2) == 0).sum()) - int((rs == 0).sum())) <= 2
