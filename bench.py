@@ -92,8 +92,7 @@ def main():
     ap.add_argument('--warmup', type=int, default=3)
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-check', action='store_true')
-    ap.add_argument('--lanes', type=int, default=None)
-    ap.add_argument('--threads', type=int, default=None)
+    ap.add_argument('--no-tune', action='store_true', help='use the default launch-shape policy')
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) != 1:
         print('bench.py measures one GPU: every workload here runs on a single MI355X '
@@ -105,11 +104,7 @@ def main():
     dev = torch.device('cuda:0')
     flat, ms, ns, offs = uneven_workload()
     batch = B.UnevenBatch(torch.from_numpy(flat).to(dev), ms, ns)
-    params = T.tune_uneven(batch) if args.lanes is None else {}
-    if args.lanes is not None:
-        params['lanes'] = args.lanes
-    if args.threads is not None:
-        params['threads'] = args.threads
+    params = {} if args.no_tune else T.tune_uneven(batch)
     batch.configure(cut='floor', **params)
     batch.capture()
     for _ in range(args.warmup):
@@ -138,6 +133,7 @@ def main():
         'config': {'workload': 'uneven batch: 4096 float32 matrices, m~U{16..96}, n~U{8..m}, one-sided Jacobi '
                                '(FP64 dot products and convergence test, tol 10*eps32), persistent LDS launch in a hipGraph',
                    'count': COUNT, 'seed': SEED, 'params': batch.params,
+                   'classes': [{k: c[k] for k in ('threads', 'lanes', 'count', 'max_m', 'max_n', 'stream')} for c in batch.classes],
                    'sweeps_max': int(sweeps.max()), 'sweeps_mean': float(sweeps.mean())},
         'roofline': {'bound': 'mfma', 'achieved': achieved, 'peak': PEAK_FP32_TFLOPS, 'unit': 'TFLOP/s',
                      'frac': achieved / PEAK_FP32_TFLOPS, 'traffic': load_traffic(),

@@ -63,8 +63,10 @@ const char *psvd_version(void);
  * offs[3i+1] = offset of s_i, offs[3i+2] = offset of Vh_i (n_i x n_i).  order (optional) is a
  * permutation of [0, batch) giving the processing order (longest first balances the queue).
  * max_m / max_n bound the shapes (they size the LDS).  tol <= 0 selects 10*eps(dtype).
- * lanes = lanes per column pair: 1..64 (power of two) fixed for the batch, or -k (k in 1..64)
- * for about k rows per lane chosen per matrix (0 means -4).  threads = 256 or 512.
+ * lanes = lanes per column pair: 2..64 (power of two; 1 is raised to 2), or -k (k in 1..64) for
+ * about k rows per lane of the largest matrix (0 means -4).  threads = 64, 128, 256, 512 or 1024.
+ * One launch uses one lane count; callers with very uneven batches split them into size
+ * classes, one call each (the LDS budget is also sized per call from max_m, max_n).
  * ------------------------------------------------------------------------------------- */
 size_t psvd_jacobi_lds_workspace_bytes(int64_t batch);
 int psvd_jacobi_lds_smem_bytes(int dtype, int max_m, int max_n, int flags, size_t *bytes);

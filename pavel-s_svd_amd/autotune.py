@@ -1,8 +1,9 @@
 """Autotuner for the batched Jacobi launch shape.
 
-Tunes, on the actual batch, the lanes-per-column-pair policy (a fixed lane count, or about k rows
-per lane chosen per matrix) and the workgroup size, by timing each candidate launch with HIP
-events on the current stream.  Results are cached in-process per (dtype, max_m, max_n, batch
+Tunes, on the actual batch, the launch-shape policy -- the waves per CU the resident matrices
+should supply (which sets each matrix's workgroup size from its LDS footprint) and the minimum
+rows per lane that bounds the lanes per column pair; together they fix the size classes the
+batch is split into -- by timing each candidate solve with HIP events on the current stream.  Results are cached in-process per (dtype, max_m, max_n, batch
 size bucket, vectors) and, when ``PAVELSVD_TUNE_CACHE`` names a JSON file, across processes.
 """
 import json
@@ -11,8 +12,8 @@ import sys
 
 import torch
 
-CANDIDATES = [dict(lanes=lanes, threads=threads) for threads in (256, 512)
-              for lanes in (-4, -8, -12, 4, 8, 16)]
+CANDIDATES = [dict(target_waves=w, min_rows=r, passes=p, streams=4) for w in (8, 16, 32) for r in (2, 4, 6)
+              for p in (1, 2)]
 _CACHE = {}
 
 

@@ -25,23 +25,44 @@ __device__ __forceinline__ double group_sum(double v, int width) {
 // DPP lane permutation of a double (two 32-bit moves).  CTRL is a gfx9 DPP control word.
 template <int CTRL>
 __device__ __forceinline__ double dpp_double(double v) {
-  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
-  return __hiloint2double(hi, lo);
+  const int lo = __double2loint(v), hi = __double2hiint(v);
+  return __hiloint2double(__builtin_amdgcn_update_dpp(hi, hi, CTRL, 0xF, 0xF, false),
+                          __builtin_amdgcn_update_dpp(lo, lo, CTRL, 0xF, 0xF, false));
 }
 
-// Sum over aligned groups of L lanes with DPP (quad_perm xor 1, xor 2, row_half_mirror,
-// row_mirror) and wavefront shuffles above 16 lanes.  Each level pairs every lane with a partner
-// in the other half of its block, so all lanes of a group end with the same bits.
+__device__ __forceinline__ double readlane_double(double v, int lane) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), lane),
+                          __builtin_amdgcn_readlane(__double2loint(v), lane));
+}
+
+// Sum over aligned groups of L lanes: DPP within rows of 16 (quad_perm xor 1, xor 2,
+// row_half_mirror, row_mirror), then v_readlane of the row sums for 32 and 64.  Each DPP level
+// pairs every lane with a partner in the other half of its block, and the readlane levels hand
+// one scalar to the whole group, so all lanes of a group end with the same bits.
 template <int L>
 __device__ __forceinline__ double group_sum_dpp(double v) {
   if constexpr (L >= 2) v += dpp_double<0xB1>(v);   // quad_perm [1,0,3,2]
   if constexpr (L >= 4) v += dpp_double<0x4E>(v);   // quad_perm [2,3,0,1]
   if constexpr (L >= 8) v += dpp_double<0x141>(v);  // row_half_mirror
   if constexpr (L >= 16) v += dpp_double<0x140>(v); // row_mirror
-  if constexpr (L >= 32) v += __shfl_xor(v, 16, kWave);
-  if constexpr (L >= 64) v += __shfl_xor(v, 32, kWave);
+  if constexpr (L == 32) {
+    const double r01 = readlane_double(v, 0) + readlane_double(v, 16);
+    const double r23 = readlane_double(v, 32) + readlane_double(v, 48);
+    v = (threadIdx.x & 32) ? r23 : r01;
+  } else if constexpr (L >= 64) {
+    v = (readlane_double(v, 0) + readlane_double(v, 16)) + (readlane_double(v, 32) + readlane_double(v, 48));
+  }
   return v;
+}
+
+// Maximum over the whole wave (DPP within rows, readlane across rows); wave-uniform result.
+__device__ __forceinline__ double wave_max_fast(double v) {
+  v = fmax(v, dpp_double<0xB1>(v));
+  v = fmax(v, dpp_double<0x4E>(v));
+  v = fmax(v, dpp_double<0x141>(v));
+  v = fmax(v, dpp_double<0x140>(v));
+  return fmax(fmax(readlane_double(v, 0), readlane_double(v, 16)),
+              fmax(readlane_double(v, 32), readlane_double(v, 48)));
 }
 
 __device__ __forceinline__ double wave_max(double v) {
@@ -87,9 +108,10 @@ __device__ __forceinline__ double block_max(double v, double *red) {
 // 1 + ((j - 1 - r) mod (np - 1)); pair k faces seats k and np-1-k, ordered (min, max).
 __device__ __forceinline__ void rr_pair(int r, int k, int np, int &p, int &q) {
   const int nm1 = np - 1;
+  // 0 <= j - 1 <= np - 2 and 0 <= r <= np - 2, so one conditional add reduces mod (np - 1)
   auto seat = [&](int j) -> int {
     if (j == 0) return 0;
-    int t = (j - 1 - r) % nm1;
+    int t = j - 1 - r;
     if (t < 0) t += nm1;
     return 1 + t;
   };
@@ -126,14 +148,16 @@ __device__ __forceinline__ void jacobi_rotation_t<float>(double app, double aqq,
   // t in FP32 (the angle needs no more); c = 1/sqrt(1 + t^2) and s = t c in FP64 (v_rsq_f64 plus
   // one Newton step) and rounded once, so that c^2 + s^2 = 1 to an ulp and V stays orthogonal
   // over thousands of rotations.
+  // native v_rcp_f32 / v_sqrt_f32 (about 1 ulp): t to a few ulps is all the angle needs
   const float fa = (float)fabs(apq);
-  const float tau = (float)(aqq - app) / (2.0f * fa);
+  const float tau = (float)(aqq - app) * __builtin_amdgcn_rcpf(2.0f * fa);
   float t;
   if (tau == 0.0f) {
     t = 1.0f;
   } else {
     const float at = fabsf(tau);
-    t = at > 1e18f ? 0.5f / at : 1.0f / (at + sqrtf(fmaf(at, at, 1.0f)));
+    t = at > 1e18f ? 0.5f * __builtin_amdgcn_rcpf(at)
+                   : __builtin_amdgcn_rcpf(at + __builtin_amdgcn_sqrtf(fmaf(at, at, 1.0f)));
     t = copysignf(t, tau);
   }
   const double td = (double)t;

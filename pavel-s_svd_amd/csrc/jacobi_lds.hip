@@ -49,60 +49,118 @@ __global__ void zero_queue_kernel(int32_t *q) {
   if (threadIdx.x == 0) q[0] = 0;
 }
 
-// Sweeps of one matrix with L lanes per column pair.  One barrier per round: a group computes
-// the three FP64 dot products of its pair (rows kept in registers when m <= L*kRowsPerLane),
-// decides, rotates and writes back; the round's rotation floor eps^2 * max_j |w_j|^2
-// (primitives.py:337) is the maximum the previous round left behind, gathered with an LDS atomic
-// from the exact-arithmetic norm updates app - t|apq|, aqq + t|apq|.  rmax[3] and flag[3] are
-// triple-buffered so that no slot is cleared while another wave may still read it.
-template <typename T> struct RowsPerLane { static constexpr int value = sizeof(T) == 4 ? 12 : 6; };
+// Sweeps of one matrix with L lanes per column pair; the caller picks NT and L so that the
+// round's pairs times L fill the workgroup.  One barrier per round: a group streams its pair's
+// rows in unconditional chunks of four (row index clamped into the column, contributions
+// masked), reduces the three FP64 dot products over its L lanes, decides, and applies the
+// rotation to its W and V rows.  The round's rotation floor eps^2 * max_j |w_j|^2
+// (primitives.py:337) is the maximum the previous round left behind: every wave publishes the
+// maximum of the exact-arithmetic norm updates app - t|apq|, aqq + t|apq| of its pairs in its
+// own slot of a double-buffered per-wave array (no atomics, no clearing) and every thread takes
+// the maximum over the slots.  Convergence (no active pair in a sweep) is gathered the same way
+// once per sweep.
+template <int NW>
+__device__ __forceinline__ double slots_max(const double *slot) {
+  double v = slot[0];
+#pragma unroll
+  for (int w = 1; w < NW; ++w) v = fmax(v, slot[w]);
+  return v;
+}
+
+constexpr int kChunk = 4;
+
+template <typename T, int L>
+__device__ __forceinline__ void pair_dots(const T *wp, const T *wq, int m, int lig, double &app, double &aqq,
+                                          double &apq) {
+  app = aqq = apq = 0.0;
+  for (int base = 0; base < m; base += kChunk * L) {
+    T xs[kChunk], ys[kChunk];
+#pragma unroll
+    for (int j = 0; j < kChunk; ++j) {
+      const int i = base + lig + j * L;
+      const int ic = i < m ? i : m - 1;
+      xs[j] = wp[ic];
+      ys[j] = wq[ic];
+    }
+#pragma unroll
+    for (int j = 0; j < kChunk; ++j) {
+      const bool live = base + lig + j * L < m;
+      const double x = live ? (double)xs[j] : 0.0, y = live ? (double)ys[j] : 0.0;
+      app = fma(x, x, app);
+      aqq = fma(y, y, aqq);
+      apq = fma(x, y, apq);
+    }
+  }
+}
+
+template <typename T, int L>
+__device__ __forceinline__ void pair_rotate(T *xp, T *xq, int rows, int lig, T c, T s) {
+  for (int base = 0; base < rows; base += kChunk * L) {
+    T xs[kChunk], ys[kChunk];
+#pragma unroll
+    for (int j = 0; j < kChunk; ++j) {
+      const int i = base + lig + j * L;
+      const int ic = i < rows ? i : rows - 1;
+      xs[j] = xp[ic];
+      ys[j] = xq[ic];
+    }
+#pragma unroll
+    for (int j = 0; j < kChunk; ++j) {
+      const int i = base + lig + j * L;
+      if (i < rows) {
+        xp[i] = c * xs[j] - s * ys[j];
+        xq[i] = s * xs[j] + c * ys[j];
+      }
+    }
+  }
+}
+
+template <typename T> struct CachedRows { static constexpr int value = sizeof(T) == 4 ? 8 : 4; };
 
 template <typename T, int NT, int L>
 __device__ void jacobi_sweeps(T *W, T *V, int m, int np, int ldw, bool vectors, double tol, double eps,
-                              int max_sweeps, unsigned long long *rmax, int *flag, int &sweeps_out,
-                              bool &conv_out) {
-  constexpr int NG = NT / L;
-  constexpr int kRowsPerLane = RowsPerLane<T>::value;
-  const int tid = threadIdx.x;
+                              int max_sweeps, double *rslot, int *fslot, int &sweeps_out, bool &conv_out) {
+  constexpr int NG = NT / L, NW = NT / kWave;
+  constexpr int RC = CachedRows<T>::value;       // rows per lane kept in registers between phases
+  const int tid = threadIdx.x, wave = tid / kWave, lane = tid & (kWave - 1);
   const int grp = tid / L, lig = tid - grp * L;
   const int npairs = np / 2;
-  const bool inreg = m <= L * kRowsPerLane;
+  const bool cached = m <= RC * L;
   const double tol2 = tol * tol, eps2 = eps * eps;
   int sweeps = 0;
   bool converged = false;
   int g = 0;
   for (int sw = 1; sw <= max_sweeps; ++sw) {
     sweeps = sw;
-    if (tid == 0) flag[(sw + 1) % 3] = 0;
+    bool wave_active = false;
     for (int r = 0; r < np - 1; ++r, ++g) {
-      if (tid == 0) rmax[(g + 2) % 3] = 0ull;
-      const double floor_ = eps2 * __longlong_as_double((long long)rmax[g % 3]);
+      const double floor_ = eps2 * slots_max<NW>(rslot + (g & 1) * NW);
+      double mymax = 0.0;
       for (int k = grp; k < npairs; k += NG) {
         int p, q;
         rr_pair(r, k, np, p, q);
         T *wp = W + p * ldw, *wq = W + q * ldw;
-        T xs[kRowsPerLane], ys[kRowsPerLane];
         double app = 0.0, aqq = 0.0, apq = 0.0;
-        if (inreg) {
+        T xs[RC], ys[RC];
+        if (cached) {
+          // all rows of this lane in registers: unconditional loads (index clamped), masked sums
 #pragma unroll
-          for (int j = 0; j < kRowsPerLane; ++j) {
+          for (int j = 0; j < RC; ++j) {
             const int i = lig + j * L;
-            if (i < m) {
-              xs[j] = wp[i];
-              ys[j] = wq[i];
-              const double x = (double)xs[j], y = (double)ys[j];
-              app = fma(x, x, app);
-              aqq = fma(y, y, aqq);
-              apq = fma(x, y, apq);
-            }
+            const int ic = i < m ? i : m - 1;
+            xs[j] = wp[ic];
+            ys[j] = wq[ic];
           }
-        } else {
-          for (int i = lig; i < m; i += L) {
-            const double x = (double)wp[i], y = (double)wq[i];
+#pragma unroll
+          for (int j = 0; j < RC; ++j) {
+            const bool live = lig + j * L < m;
+            const double x = live ? (double)xs[j] : 0.0, y = live ? (double)ys[j] : 0.0;
             app = fma(x, x, app);
             aqq = fma(y, y, aqq);
             apq = fma(x, y, apq);
           }
+        } else {
+          pair_dots<T, L>(wp, wq, m, lig, app, aqq, apq);
         }
         app = group_sum_dpp<L>(app);
         aqq = group_sum_dpp<L>(aqq);
@@ -115,10 +173,10 @@ __device__ void jacobi_sweeps(T *W, T *V, int m, int np, int ldw, bool vectors, 
           jacobi_rotation_t<T>(app, aqq, apq, c, s, tb);
           nap = app - tb;
           naq = aqq + tb;
-          if (lig == 0) flag[sw % 3] = 1;
-          if (inreg) {
+          wave_active = true;
+          if (cached) {
 #pragma unroll
-            for (int j = 0; j < kRowsPerLane; ++j) {
+            for (int j = 0; j < RC; ++j) {
               const int i = lig + j * L;
               if (i < m) {
                 wp[i] = c * xs[j] - s * ys[j];
@@ -126,26 +184,24 @@ __device__ void jacobi_sweeps(T *W, T *V, int m, int np, int ldw, bool vectors, 
               }
             }
           } else {
-            for (int i = lig; i < m; i += L) {
-              const T x = wp[i], y = wq[i];
-              wp[i] = c * x - s * y;
-              wq[i] = s * x + c * y;
-            }
+            pair_rotate<T, L>(wp, wq, m, lig, c, s);
           }
-          if (vectors) {
-            T *vp = V + p * np, *vq = V + q * np;
-            for (int i = lig; i < np; i += L) {
-              const T x = vp[i], y = vq[i];
-              vp[i] = c * x - s * y;
-              vq[i] = s * x + c * y;
-            }
-          }
+          if (vectors) pair_rotate<T, L>(V + p * np, V + q * np, np, lig, c, s);
         }
-        if (lig == 0) atomicMax(&rmax[(g + 1) % 3], (unsigned long long)__double_as_longlong(fmax(nap, naq)));
+        mymax = fmax(mymax, fmax(nap, naq));
       }
+      mymax = wave_max_fast(mymax);
+      if (lane == 0) rslot[((g + 1) & 1) * NW + wave] = mymax;
       __syncthreads();
     }
-    if (flag[sw % 3] == 0) {
+    // convergence: no wave had an active pair in this sweep
+    const int wave_any = __any(wave_active) ? 1 : 0;
+    if (lane == 0) fslot[(sw & 1) * NW + wave] = wave_any;
+    __syncthreads();
+    int any = 0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) any |= fslot[(sw & 1) * NW + w];
+    if (!any) {
       converged = true;
       break;
     }
@@ -154,12 +210,16 @@ __device__ void jacobi_sweeps(T *W, T *V, int m, int np, int ldw, bool vectors, 
   conv_out = converged;
 }
 
-template <typename T, int NT>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void jacobi_lds_kernel(LdsParams<T> P) {
+// Waves per SIMD the register budget is sized for (128 VGPRs: four waves per SIMD).
+template <int NT> struct WavesPerEU { static constexpr int value = 4; };
+
+template <typename T, int NT, int L>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WavesPerEU<NT>::value)))
+void jacobi_lds_kernel(LdsParams<T> P) {
   extern __shared__ double smem_d[];
   __shared__ int s_idx;
-  __shared__ int s_flag[3];
-  __shared__ unsigned long long s_rmax[3];
+  __shared__ int s_fslot[2 * (NT / kWave)];
+  __shared__ double s_rslot[2 * (NT / kWave)];
   __shared__ int s_nk;
   __shared__ double s_red[NT / kWave];
   __shared__ int s_redi[NT / kWave];
@@ -201,14 +261,6 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void ja
       continue;  // uniform across the block: every thread read the same descriptor
     }
     const int np = pad_even(n), ldw = lds_ld(m), npairs = np / 2;
-    // lanes per column pair: fixed (> 0) or about -P.lanes rows per lane (<= 0; 0 means 4)
-    int L = P.lanes;
-    if (L <= 0) {
-      const int rpl = L < 0 ? -L : 4;
-      const int target = (m + rpl - 1) / rpl;
-      L = 4;
-      while (L < target && L < kWave) L <<= 1;
-    }
     double *dots = smem_d;
     double *sig = dots + 3 * npairs;
     double *ytmp = sig + np;
@@ -216,10 +268,6 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void ja
     T *W = reinterpret_cast<T *>(vt + m);
     T *V = W + (size_t)np * ldw;
     int *ord = reinterpret_cast<int *>(V + (vectors ? (size_t)np * np : 0));
-    if (tid == 0) {
-      s_flag[0] = s_flag[1] = s_flag[2] = 0;
-      s_rmax[0] = s_rmax[1] = s_rmax[2] = 0ull;
-    }
 
     // ---- load A (row-major) into W (column-major), check finiteness, optional max-abs scale
     const T *A = P.a + aoff;
@@ -256,31 +304,26 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void ja
       }
     }
     __syncthreads();
-    // initial rotation floor: max_j |w_j|^2
-    for (int c = wave; c < n; c += NW) {
-      const T *wc = W + c * ldw;
-      double acc = 0.0;
-      for (int i = lane; i < m; i += kWave) {
-        const double x = (double)wc[i];
-        acc = fma(x, x, acc);
+    // initial rotation floor: max_j |w_j|^2, one slot per wave
+    {
+      double mx = 0.0;
+      for (int c = wave; c < n; c += NW) {
+        const T *wc = W + c * ldw;
+        double acc = 0.0;
+        for (int i = lane; i < m; i += kWave) {
+          const double x = (double)wc[i];
+          acc = fma(x, x, acc);
+        }
+        mx = fmax(mx, wave_sum(acc));
       }
-      acc = wave_sum(acc);
-      if (lane == 0) atomicMax(&s_rmax[0], (unsigned long long)__double_as_longlong(acc));
+      if (lane == 0) s_rslot[wave] = mx;
     }
     __syncthreads();
 
     // ---- sweeps
     int sweeps = 0;
     bool converged = false;
-    switch (L) {
-      case 1: jacobi_sweeps<T, NT, 1>(W, V, m, np, ldw, vectors, tol, eps, P.max_sweeps, s_rmax, s_flag, sweeps, converged); break;
-      case 2: jacobi_sweeps<T, NT, 2>(W, V, m, np, ldw, vectors, tol, eps, P.max_sweeps, s_rmax, s_flag, sweeps, converged); break;
-      case 4: jacobi_sweeps<T, NT, 4>(W, V, m, np, ldw, vectors, tol, eps, P.max_sweeps, s_rmax, s_flag, sweeps, converged); break;
-      case 8: jacobi_sweeps<T, NT, 8>(W, V, m, np, ldw, vectors, tol, eps, P.max_sweeps, s_rmax, s_flag, sweeps, converged); break;
-      case 16: jacobi_sweeps<T, NT, 16>(W, V, m, np, ldw, vectors, tol, eps, P.max_sweeps, s_rmax, s_flag, sweeps, converged); break;
-      case 32: jacobi_sweeps<T, NT, 32>(W, V, m, np, ldw, vectors, tol, eps, P.max_sweeps, s_rmax, s_flag, sweeps, converged); break;
-      default: jacobi_sweeps<T, NT, 64>(W, V, m, np, ldw, vectors, tol, eps, P.max_sweeps, s_rmax, s_flag, sweeps, converged); break;
-    }
+    jacobi_sweeps<T, NT, L>(W, V, m, np, ldw, vectors, tol, eps, P.max_sweeps, s_rslot, s_fslot, sweeps, converged);
     int status = converged ? PSVD_INFO_OK : PSVD_INFO_NOCONV;
 
     // ---- sigma in FP64, descending order, cut
@@ -432,10 +475,10 @@ static DeviceProps &props() {
   return p;
 }
 
-template <typename T, int NT>
+template <typename T, int NT, int L>
 static int launch_lds(const LdsParams<T> &P, size_t smem, hipStream_t stream) {
   static bool attr = false;
-  auto kern = jacobi_lds_kernel<T, NT>;
+  auto kern = jacobi_lds_kernel<T, NT, L>;
   if (!attr) {
     if (hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)(kLdsCap - 4096)) != hipSuccess)
@@ -465,6 +508,13 @@ static int run_lds(int64_t batch, int mu, int nu, const int32_t *dims, const int
   const size_t smem = lds_bytes<T>(max_m, max_n, vectors);
   if (smem > kLdsCap - 4096) return PSVD_ETOOBIG;
   if (lanes > kWave || (lanes > 0 && (lanes & (lanes - 1))) || lanes < -kWave) return PSVD_EINVAL;
+  if (lanes <= 0) {  // about -lanes rows per lane (4 when 0) for the largest matrix of the call
+    const int rpl = lanes < 0 ? -lanes : 4;
+    const int target = (max_m + rpl - 1) / rpl;
+    lanes = 2;
+    while (lanes < target && lanes < kWave) lanes <<= 1;
+  }
+  if (lanes == 1) lanes = 2;
   LdsParams<T> P;
   P.a = static_cast<const T *>(a);
   P.u = static_cast<T *>(u);
@@ -485,8 +535,15 @@ static int run_lds(int64_t batch, int mu, int nu, const int32_t *dims, const int
   P.flags = flags;
   P.lanes = lanes;
   (void)ws_bytes;
-  if (threads == 512) return launch_lds<T, 512>(P, smem, stream);
-  if (threads == 256 || threads == 0) return launch_lds<T, 256>(P, smem, stream);
+  if (threads == 0) threads = 256;
+#define PSVD_LDS_CASE(NTV, LV) \
+  if (threads == NTV && lanes == LV) return launch_lds<T, NTV, LV>(P, smem, stream);
+#define PSVD_LDS_NT(NTV) \
+  PSVD_LDS_CASE(NTV, 2) PSVD_LDS_CASE(NTV, 4) PSVD_LDS_CASE(NTV, 8) PSVD_LDS_CASE(NTV, 16) \
+  PSVD_LDS_CASE(NTV, 32) PSVD_LDS_CASE(NTV, 64)
+  PSVD_LDS_NT(64) PSVD_LDS_NT(128) PSVD_LDS_NT(256) PSVD_LDS_NT(512) PSVD_LDS_NT(1024)
+#undef PSVD_LDS_NT
+#undef PSVD_LDS_CASE
   return PSVD_EINVAL;
 }
 

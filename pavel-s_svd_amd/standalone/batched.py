@@ -2,15 +2,41 @@
 
 The reference API takes one (..., m, n) tensor (solvers.py:289-290), so a batch of mixed shapes
 is a Python loop of ``svd`` calls there.  ``UnevenBatch`` packs the matrices row-major into one
-device buffer, builds the descriptor arrays once (shapes, element offsets, and a longest-first
-processing order by the Jacobi cost n^2 (12 m + 6 n) that balances the device work queue), and
-solves the whole batch with one launch of the LDS-resident Jacobi kernel.  ``capture`` records
-that launch in a hipGraph (through torch.cuda.CUDAGraph), after which ``replay`` re-solves the
-current contents of ``a`` with a single graph launch.
+device buffer and builds the descriptor arrays once: shapes, element offsets, and a processing
+order.  The order groups the matrices into size classes by the lanes-per-column-pair count their
+row count calls for (about ``rows_per_lane`` rows per lane), and within a class puts the largest
+Jacobi cost n^2 (12 m + 6 n) first so the device work queue balances.  Each class is one
+persistent launch of the LDS-resident Jacobi kernel sized for that class (its own LDS budget,
+hence its own occupancy); ``capture`` records the launches of all classes in one hipGraph
+(through torch.cuda.CUDAGraph), after which ``replay`` re-solves the current contents of ``a``
+with a single graph launch.
 """
 import torch
 
 from .. import native as N
+
+# Launch-shape policy.  LDS decides how many matrices can be resident on a CU (160 KiB each);
+# the waves per matrix are chosen so that the resident matrices together supply about
+# `target_waves` waves per CU (one to sixteen waves per matrix), and the lanes per column pair
+# fill the workgroup with the matrix's n/2 pairs (at most `passes` pairs per group per round)
+# while keeping at least `min_rows` rows per lane.
+DEFAULT_POLICY = dict(target_waves=16, min_rows=4, passes=2)
+LDS_PER_CU = 160 * 1024
+
+
+def launch_shape(code, m, n, vectors, policy):
+    """(threads, lanes per column pair) for an m x n matrix under ``policy``."""
+    st, lds = N._hip.jacobi_lds_smem_bytes(code, m, n, 0 if vectors else N.NO_VECTORS)
+    resident = max(1, LDS_PER_CU // max(lds, 1))
+    waves = 1
+    while waves * resident < policy['target_waves'] and waves < 16:
+        waves *= 2
+    threads = 64 * waves
+    npairs = (n + 1) // 2
+    lanes = 2
+    while lanes * 2 * npairs <= threads * policy['passes'] and lanes * 2 * policy['min_rows'] <= m and lanes < 64:
+        lanes *= 2
+    return threads, lanes
 
 
 class UnevenBatch:
@@ -50,24 +76,55 @@ class UnevenBatch:
         if offsets is not None and any(o < 0 or o + m * n > flat.numel() for o, m, n in zip(a_off, ms, ns)):
             raise ValueError('an offset places a matrix outside flat')
         self.a_off, self.s_off, self.v_off = a_off, s_off, v_off
-        cost = [n * n * (12 * m + 6 * n) for m, n in zip(ms, ns)]
-        order = sorted(range(self.batch), key=lambda i: -cost[i])
         host_dims = torch.tensor([[m, n] for m, n in zip(ms, ns)], dtype=torch.int32).reshape(-1)
         host_offs = torch.tensor(list(zip(a_off, s_off, v_off)), dtype=torch.int64).reshape(-1)
         self.dims = host_dims.to(dev)
         self.offs = host_offs.to(dev)
-        self.order = torch.tensor(order, dtype=torch.int32).to(dev)
+        self.classes = []
+        self.order = None
         self.u = torch.empty(flat.numel(), dtype=flat.dtype, device=dev) if vectors else None
         self.s = torch.empty(ps, dtype=flat.dtype, device=dev)
         self.vh = torch.empty(pv, dtype=flat.dtype, device=dev) if vectors else None
         self.info = torch.empty(self.batch, 2, dtype=torch.int32, device=dev)
         self.ws_bytes = N._hip.jacobi_lds_workspace_bytes(self.batch)
-        self.ws = torch.empty(self.ws_bytes, dtype=torch.uint8, device=dev)
+        self.ws = None
         st, self.smem = N._hip.jacobi_lds_smem_bytes(self.code, self.max_m, self.max_n,
                                                       0 if vectors else N.NO_VECTORS)
         N.check(st, f'uneven batch with max shape {self.max_m}x{self.max_n}')
         self.graph = None
-        self.params = dict(tol=None, max_sweeps=60, scale=True, cut='reference', lanes=0, threads=256)
+        self.params = dict(tol=None, max_sweeps=60, scale=True, cut='reference', streams=4, **DEFAULT_POLICY)
+        self.side = []
+        self._plan()
+
+    def _plan(self):
+        """Size classes (one launch shape each) and the processing order (longest first inside
+        each class)."""
+        cost = [n * n * (12 * m + 6 * n) for m, n in zip(self.ms, self.ns)]
+        groups = {}
+        for i, (m, n) in enumerate(zip(self.ms, self.ns)):
+            groups.setdefault(launch_shape(self.code, m, n, self.vectors, self.params), []).append(i)
+        order, classes = [], []
+        for (threads, lanes) in sorted(groups, reverse=True):
+            idx = sorted(groups[(threads, lanes)], key=lambda i: -cost[i])
+            mm = max(self.ms[i] for i in idx)
+            nn = max(self.ns[i] for i in idx)
+            st, _ = N._hip.jacobi_lds_smem_bytes(self.code, mm, nn, 0 if self.vectors else N.NO_VECTORS)
+            N.check(st, f'size class with max shape {mm}x{nn}')
+            classes.append(dict(threads=threads, lanes=lanes, start=len(order), count=len(idx), max_m=mm, max_n=nn,
+                                cost=sum(cost[i] for i in idx)))
+            order.extend(idx)
+        # classes run concurrently on side streams, greedily balanced by total cost
+        nstreams = max(1, min(int(self.params['streams']), len(classes)))
+        load = [0] * nstreams
+        for cl in sorted(classes, key=lambda c: -c['cost']):
+            k = load.index(min(load))
+            cl['stream'] = k
+            load[k] += cl['cost']
+        self.classes = classes
+        self.order = torch.tensor(order, dtype=torch.int32).to(self.a.device)
+        dev = self.a.device
+        self.ws = torch.empty(self.ws_bytes * max(1, len(classes)), dtype=torch.uint8, device=dev)
+        self.side = [torch.cuda.Stream(device=dev) for _ in range(nstreams)] if len(classes) > 1 else []
 
     @property
     def flops_per_sweep(self):
@@ -79,7 +136,10 @@ class UnevenBatch:
         unknown = set(params) - set(self.params)
         if unknown:
             raise ValueError(f'unknown parameters {sorted(unknown)}')
+        replan = any((k in DEFAULT_POLICY or k == 'streams') and v != self.params[k] for k, v in params.items())
         self.params.update(params)
+        if replan:
+            self._plan()
         self.graph = None
         return self
 
@@ -88,12 +148,25 @@ class UnevenBatch:
         p = self.params
         flags = (N.SCALE if p['scale'] else 0) | (N.CUT_FLOOR if p['cut'] == 'floor' else 0) | \
                 (0 if self.vectors else N.NO_VECTORS)
-        st = N._hip.jacobi_lds(self.code, self.batch, 0, 0, N.ptr(self.dims), N.ptr(self.offs),
-                               N.ptr(self.order), self.max_m, self.max_n, N.ptr(self.a), N.ptr(self.u),
-                               N.ptr(self.s), N.ptr(self.vh), N.ptr(self.info), float(p['tol'] or 0.0),
-                               int(p['max_sweeps']), flags, int(p['lanes']), int(p['threads']),
-                               N.ptr(self.ws), self.ws_bytes, N.stream_of(self.a))
-        N.check(st, 'psvd_jacobi_lds (uneven)')
+        main = torch.cuda.current_stream(self.a.device)
+        if self.side:
+            fork = torch.cuda.Event()
+            fork.record(main)
+            for st in self.side:
+                st.wait_event(fork)
+        for ci, cl in enumerate(self.classes):
+            stream = self.side[cl['stream']].cuda_stream if self.side else main.cuda_stream
+            order_ptr = N.ptr(self.order) + 4 * cl['start']
+            st = N._hip.jacobi_lds(self.code, cl['count'], 0, 0, N.ptr(self.dims), N.ptr(self.offs), order_ptr,
+                                   cl['max_m'], cl['max_n'], N.ptr(self.a), N.ptr(self.u), N.ptr(self.s),
+                                   N.ptr(self.vh), N.ptr(self.info), float(p['tol'] or 0.0),
+                                   int(p['max_sweeps']), flags, cl['lanes'], cl['threads'],
+                                   N.ptr(self.ws) + ci * self.ws_bytes, self.ws_bytes, stream)
+            N.check(st, f"psvd_jacobi_lds (uneven, {cl['threads']}x{cl['lanes']} class)")
+        for st in self.side:
+            join = torch.cuda.Event()
+            join.record(st)
+            main.wait_event(join)
 
     def capture(self):
         """Record one solve in a hipGraph; ``replay`` then re-runs it with one graph launch."""

@@ -20,7 +20,7 @@ def lds_fits(dtype, m, n, vectors=True):
 
 
 def jacobi_lds_uniform(x, *, tol=None, max_sweeps=60, scale=False, cut='reference', vectors=True,
-                       lanes=0, threads=256):
+                       lanes=None, threads=None):
     """One-sided Jacobi SVD of every (m, n) matrix of the contiguous tall batch ``x`` (b, m, n).
 
     Returns U (b, m, n), s (b, n) descending, Vh (b, n, n) (None, s, None without vectors) and
@@ -39,6 +39,11 @@ def jacobi_lds_uniform(x, *, tol=None, max_sweeps=60, scale=False, cut='referenc
     info = torch.empty(b, 2, dtype=torch.int32, device=dev)
     wsb = N._hip.jacobi_lds_workspace_bytes(b)
     ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
+    if lanes is None or threads is None:
+        from .batched import DEFAULT_POLICY, launch_shape
+        t_auto, l_auto = launch_shape(code, m, n, vectors, DEFAULT_POLICY)
+        threads = t_auto if threads is None else threads
+        lanes = l_auto if lanes is None else lanes
     st = N._hip.jacobi_lds(code, b, m, n, 0, 0, 0, m, n, N.ptr(x), N.ptr(u), N.ptr(s), N.ptr(vh),
                            N.ptr(info), float(tol or 0.0), int(max_sweeps), flags, int(lanes), int(threads),
                            N.ptr(ws), wsb, N.stream_of(x))

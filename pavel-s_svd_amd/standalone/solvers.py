@@ -36,7 +36,7 @@ def _jacobi(x, stats, opts, scale):
     max_sweeps = opts.get('max_sweeps', 60)
     u, s, vh, info = P.jacobi_any(x, tol=opts.get('tol'), max_sweeps=max_sweeps, scale=scale,
                                   cut=opts.get('cut', 'reference'), vectors=vectors,
-                                  lanes=opts.get('lanes', 0), threads=opts.get('threads', 256))
+                                  lanes=opts.get('lanes'), threads=opts.get('threads'))
     info = info.cpu()
     from .. import native as N
     N.raise_for_info(info, f'one-sided Jacobi did not converge in {max_sweeps} sweeps')
