@@ -46,6 +46,11 @@ typedef void *psvd_stream_t;
                                instead of eps*max(m,n)*sigma_1 (primitives.py:362) */
 #define PSVD_NO_VECTORS 4   /* sigma only: no V accumulation, U and Vh are not written */
 
+/* GEMM arithmetic modes */
+#define PSVD_GEMM_F32 0     /* FP32 operands, FP32 MFMA, FP32 accumulation */
+#define PSVD_GEMM_BF16X3 1  /* FP32 operands split hi+lo in bf16, three bf16 MFMA products, FP32 acc */
+#define PSVD_GEMM_F64 2     /* FP64 operands, FP64 MFMA */
+
 /* Library identification: returns a static string ("pavelsvd <version> gfx950"). */
 const char *psvd_version(void);
 
@@ -76,6 +81,17 @@ int psvd_jacobi_lds(int dtype, int64_t batch, int m_uniform, int n_uniform,
                     const void *a, void *u, void *s, void *vh, int32_t *info,
                     double tol, int max_sweeps, int flags, int lanes, int threads,
                     void *workspace, size_t workspace_bytes, psvd_stream_t stream);
+
+/* ---------------------------------------------------------------------------------------
+ * Batched strided GEMM on MFMA: C_i = alpha * op(A_i) * op(B_i) + beta * C_i for i < batch,
+ * row-major: op(A) is m x k (A is m x k, or k x m when transa), op(B) is k x n (B is k x n, or
+ * n x k when transb); X_i = X + i * stride_x.  Replaces the `@` products of the reference's
+ * QDWH, TSQR and QR paths (solvers.py:134, 151-153, 159-160, 168, 173; :93-94, :248;
+ * primitives.py:64-65, 449) with one kernel whose arithmetic is chosen by `mode`.
+ * ------------------------------------------------------------------------------------- */
+int psvd_gemm(int mode, int transa, int transb, int64_t m, int64_t n, int64_t k, double alpha,
+              const void *a, int64_t lda, int64_t stride_a, const void *b, int64_t ldb, int64_t stride_b,
+              double beta, void *c, int64_t ldc, int64_t stride_c, int64_t batch, psvd_stream_t stream);
 
 /* ---------------------------------------------------------------------------------------
  * One-sided Jacobi SVD of ONE tall matrix that does not fit LDS (e.g. 512x256, 2048x32, the

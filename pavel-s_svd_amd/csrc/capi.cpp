@@ -66,6 +66,22 @@ static PyObject *py_jacobi_grid(PyObject *, PyObject *args) {
   return PyLong_FromLong(st);
 }
 
+static PyObject *py_gemm(PyObject *, PyObject *args) {
+  int mode, ta, tb;
+  long long m, n, k, lda, sa, ldb, sb, ldc, sc, batch;
+  unsigned long long a, b, c, stream;
+  double alpha, beta;
+  if (!PyArg_ParseTuple(args, "iiiLLLdKLLKLLdKLLLK", &mode, &ta, &tb, &m, &n, &k, &alpha, &a, &lda, &sa, &b, &ldb,
+                        &sb, &beta, &c, &ldc, &sc, &batch, &stream))
+    return nullptr;
+  int st;
+  Py_BEGIN_ALLOW_THREADS
+  st = psvd_gemm(mode, ta, tb, m, n, k, alpha, PTR(a), lda, sa, PTR(b), ldb, sb, beta, PTR(c), ldc, sc, batch,
+                 PTR(stream));
+  Py_END_ALLOW_THREADS
+  return PyLong_FromLong(st);
+}
+
 static PyMethodDef kMethods[] = {
     {"version", py_version, METH_NOARGS, "library version string"},
     {"jacobi_lds_workspace_bytes", py_jacobi_lds_workspace_bytes, METH_VARARGS, nullptr},
@@ -73,6 +89,7 @@ static PyMethodDef kMethods[] = {
     {"jacobi_lds", py_jacobi_lds, METH_VARARGS, nullptr},
     {"jacobi_grid_workspace_bytes", py_jacobi_grid_workspace_bytes, METH_VARARGS, nullptr},
     {"jacobi_grid", py_jacobi_grid, METH_VARARGS, nullptr},
+    {"gemm", py_gemm, METH_VARARGS, nullptr},
     {nullptr, nullptr, 0, nullptr}};
 
 static struct PyModuleDef kModule = {PyModuleDef_HEAD_INIT, "_hip", "gfx950 SVD kernels (C ABI binding)",
