@@ -94,6 +94,32 @@ int psvd_gemm(int mode, int transa, int transb, int64_t m, int64_t n, int64_t k,
               double beta, void *c, int64_t ldc, int64_t stride_c, int64_t batch, psvd_stream_t stream);
 
 /* ---------------------------------------------------------------------------------------
+ * Batched utilities between solver stages (graph-capturable; outputs cleared by a kernel).
+ *  psvd_reduce: op 0 out[b] = max|x_b|; op 1 out[b] = ||x_b||_F; op 2 out[b] = max|x_b - y_b| and
+ *               out[batch + b] = max|x_b| (out: device doubles).  The max-abs and Frobenius
+ *               normalisations of solvers.py:294-295 and :124-125, the QDWH test :161-163.
+ *  psvd_scale:  y_b = x_b / d_b (invert, T division as the reference's x / scale) or
+ *               y_b = (s * d_b) x_b; d_b <= 0 counts as 1; d may be NULL.
+ *  psvd_symmetrize: a_b = (a_b + a_b^T) / 2 in place (solvers.py:156, 169).
+ * ------------------------------------------------------------------------------------- */
+int psvd_reduce(int dtype, int op, int64_t batch, int rows, int cols, const void *x, int64_t ldx, int64_t sx,
+                const void *y, int64_t ldy, int64_t sy, double *out, psvd_stream_t stream);
+int psvd_scale(int dtype, int64_t batch, int rows, int cols, const void *x, int64_t ldx, int64_t sx, void *y,
+               int64_t ldy, int64_t sy, const double *d, int invert, double s, psvd_stream_t stream);
+int psvd_symmetrize(int dtype, int64_t batch, int n, void *a, int64_t lda, int64_t sa, psvd_stream_t stream);
+
+/* ---------------------------------------------------------------------------------------
+ * Blocked Householder QR of a batch of tall matrices: A_i = Q_i R_i, Q_i thin (m x n, row-major,
+ * contiguous), R_i (n x n, upper triangular).  Reflectors in the reference's xLARFG convention,
+ * so R matches primitives.qr (svdbench/standalone/primitives.py:41-89) up to rounding.  A is
+ * read only (row stride lda, batch stride stride_a).  q or r may be NULL to skip that output.
+ * gemm_mode: PSVD_GEMM_F32 or PSVD_GEMM_BF16X3 for dtype PSVD_F32 (ignored for PSVD_F64).
+ * ------------------------------------------------------------------------------------- */
+size_t psvd_qr_workspace_bytes(int dtype, int64_t batch, int m, int n);
+int psvd_qr(int dtype, int gemm_mode, int64_t batch, int m, int n, const void *a, int64_t lda, int64_t stride_a,
+            void *q, void *r, void *workspace, size_t workspace_bytes, psvd_stream_t stream);
+
+/* ---------------------------------------------------------------------------------------
  * One-sided Jacobi SVD of ONE tall matrix that does not fit LDS (e.g. 512x256, 2048x32, the
  * n x n R of TSQR), as a single cooperative launch: W and V in the workspace (HBM), each
  * workgroup owns a strided share of every round's column pairs, one grid barrier per round,

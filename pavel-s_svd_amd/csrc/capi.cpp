@@ -82,6 +82,58 @@ static PyObject *py_gemm(PyObject *, PyObject *args) {
   return PyLong_FromLong(st);
 }
 
+static PyObject *py_qr_workspace_bytes(PyObject *, PyObject *args) {
+  int dtype, m, n;
+  long long batch;
+  if (!PyArg_ParseTuple(args, "iLii", &dtype, &batch, &m, &n)) return nullptr;
+  return PyLong_FromSize_t(psvd_qr_workspace_bytes(dtype, batch, m, n));
+}
+
+static PyObject *py_qr(PyObject *, PyObject *args) {
+  int dtype, gmode, m, n;
+  long long batch, lda, sa;
+  unsigned long long a, q, r, ws, ws_bytes, stream;
+  if (!PyArg_ParseTuple(args, "iiLiiKLLKKKKK", &dtype, &gmode, &batch, &m, &n, &a, &lda, &sa, &q, &r, &ws,
+                        &ws_bytes, &stream))
+    return nullptr;
+  int st;
+  Py_BEGIN_ALLOW_THREADS
+  st = psvd_qr(dtype, gmode, batch, m, n, PTR(a), lda, sa, PTR(q), PTR(r), PTR(ws), (size_t)ws_bytes, PTR(stream));
+  Py_END_ALLOW_THREADS
+  return PyLong_FromLong(st);
+}
+
+static PyObject *py_reduce(PyObject *, PyObject *args) {
+  int dtype, op, rows, cols;
+  long long batch, ldx, sx, ldy, sy;
+  unsigned long long x, y, out, stream;
+  if (!PyArg_ParseTuple(args, "iiLiiKLLKLLKK", &dtype, &op, &batch, &rows, &cols, &x, &ldx, &sx, &y, &ldy, &sy, &out,
+                        &stream))
+    return nullptr;
+  return PyLong_FromLong(psvd_reduce(dtype, op, batch, rows, cols, PTR(x), ldx, sx, PTR(y), ldy, sy,
+                                     static_cast<double *>(PTR(out)), PTR(stream)));
+}
+
+static PyObject *py_scale(PyObject *, PyObject *args) {
+  int dtype, rows, cols, invert;
+  long long batch, ldx, sx, ldy, sy;
+  unsigned long long x, y, d, stream;
+  double s;
+  if (!PyArg_ParseTuple(args, "iLiiKLLKLLKidK", &dtype, &batch, &rows, &cols, &x, &ldx, &sx, &y, &ldy, &sy, &d,
+                        &invert, &s, &stream))
+    return nullptr;
+  return PyLong_FromLong(psvd_scale(dtype, batch, rows, cols, PTR(x), ldx, sx, PTR(y), ldy, sy,
+                                    static_cast<const double *>(PTR(d)), invert, s, PTR(stream)));
+}
+
+static PyObject *py_symmetrize(PyObject *, PyObject *args) {
+  int dtype, n;
+  long long batch, lda, sa;
+  unsigned long long a, stream;
+  if (!PyArg_ParseTuple(args, "iLiKLLK", &dtype, &batch, &n, &a, &lda, &sa, &stream)) return nullptr;
+  return PyLong_FromLong(psvd_symmetrize(dtype, batch, n, PTR(a), lda, sa, PTR(stream)));
+}
+
 static PyMethodDef kMethods[] = {
     {"version", py_version, METH_NOARGS, "library version string"},
     {"jacobi_lds_workspace_bytes", py_jacobi_lds_workspace_bytes, METH_VARARGS, nullptr},
@@ -90,6 +142,11 @@ static PyMethodDef kMethods[] = {
     {"jacobi_grid_workspace_bytes", py_jacobi_grid_workspace_bytes, METH_VARARGS, nullptr},
     {"jacobi_grid", py_jacobi_grid, METH_VARARGS, nullptr},
     {"gemm", py_gemm, METH_VARARGS, nullptr},
+    {"reduce", py_reduce, METH_VARARGS, nullptr},
+    {"scale", py_scale, METH_VARARGS, nullptr},
+    {"symmetrize", py_symmetrize, METH_VARARGS, nullptr},
+    {"qr_workspace_bytes", py_qr_workspace_bytes, METH_VARARGS, nullptr},
+    {"qr", py_qr, METH_VARARGS, nullptr},
     {nullptr, nullptr, 0, nullptr}};
 
 static struct PyModuleDef kModule = {PyModuleDef_HEAD_INIT, "_hip", "gfx950 SVD kernels (C ABI binding)",

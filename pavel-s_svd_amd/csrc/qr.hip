@@ -1,0 +1,289 @@
+// K4: blocked Householder QR of a batch of tall matrices, A = Q R with thin Q (m x n) and
+// R (n x n), in the reference's reflector convention (LAPACK xLARFG: beta = -sign(alpha) *
+// hypot(alpha, |x2|), tau = (beta - alpha) / beta, v = x / (alpha - beta), v0 = 1, and the
+// trivial reflector tau = 0 when x2 = 0), so R matches primitives.qr up to rounding.
+//
+// Per panel of kPanel columns: one workgroup per matrix factors the panel (rows owned by threads,
+// norms and the panel-internal w = v^T A products as FP64 workgroup reductions), a GEMM forms
+// G = V^T V, one workgroup per matrix builds the compact-WY factor T (H_1...H_k = I - V T V^T,
+// LAPACK larft, forward columnwise), and three GEMMs apply Q^T = I - V T^T V^T to the trailing
+// columns.  Q is accumulated backwards from [I; 0] with the same three GEMMs per panel (orgqr).
+// The host loop below only enqueues kernels: no allocation, no synchronisation.
+//
+// Follows svdbench/standalone/primitives.py:41-59 (householder_vec), :62-65 (_apply_left) and
+// :74-89 (qr).
+#include <float.h>
+
+#include "common.hpp"
+
+namespace psvd {
+
+constexpr int kPanel = 32;
+constexpr int kPanelThreads = 512;
+
+// Sum of K <= kPanel doubles per thread over the workgroup; results broadcast through `out`.
+template <int NT>
+__device__ __forceinline__ void block_sum_vec(double (&v)[kPanel], int k_lo, int k_hi, double *red, double *out) {
+  constexpr int NW = NT / kWave;
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+#pragma unroll
+  for (int k = 0; k < kPanel; ++k)
+    if (k >= k_lo && k < k_hi) {
+      const double s = group_sum_dpp<kWave>(v[k]);
+      if (lane == 0) red[wave * kPanel + k] = s;
+    }
+  __syncthreads();
+  if ((int)threadIdx.x >= k_lo && (int)threadIdx.x < k_hi) {
+    double s = 0.0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) s += red[w * kPanel + threadIdx.x];
+    out[threadIdx.x] = s;
+  }
+  __syncthreads();
+}
+
+template <int NT>
+__device__ __forceinline__ double block_sum_one(double v, double *red) {
+  constexpr int NW = NT / kWave;
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  v = group_sum_dpp<kWave>(v);
+  __syncthreads();
+  if (lane == 0) red[wave] = v;
+  __syncthreads();
+  double s = 0.0;
+#pragma unroll
+  for (int w = 0; w < NW; ++w) s += red[w];
+  return s;
+}
+
+// Factor columns j0 .. j0+nbc-1 of F (rows j0 .. m-1) in place: R entries stay in F (zeros
+// written below the diagonal), the explicit unit-lower reflectors go to V (same layout as F),
+// tau[j] per column.
+template <typename T>
+__global__ __launch_bounds__(kPanelThreads) void qr_panel_kernel(T *F, int64_t ldf, int64_t sf, T *V, int64_t sv,
+                                                                 T *tau, int64_t stau, int m, int j0, int nbc) {
+  constexpr int NT = kPanelThreads;
+  __shared__ double red[(NT / kWave) * kPanel];
+  __shared__ double wsum[kPanel];
+  const int tid = threadIdx.x;
+  T *Fb = F + (int64_t)blockIdx.x * sf;
+  T *Vb = V + (int64_t)blockIdx.x * sv;
+  T *tb = tau + (int64_t)blockIdx.x * stau;
+  for (int c = 0; c < nbc; ++c) {
+    const int j = j0 + c;
+    double part = 0.0;
+    for (int i = tid; i < m; i += NT)
+      if (i > j) {
+        const double x = (double)Fb[(int64_t)i * ldf + j];
+        part = fma(x, x, part);
+      }
+    const double xnorm = sqrt(block_sum_one<NT>(part, red));
+    const double alpha = (double)Fb[(int64_t)j * ldf + j];
+    double beta, tj, scal;
+    const bool trivial = xnorm == 0.0;
+    if (trivial) {
+      beta = alpha;
+      tj = 0.0;
+      scal = 0.0;
+    } else {
+      beta = -copysign(1.0, alpha == 0.0 ? 1.0 : alpha) * hypot(alpha, xnorm);
+      tj = (beta - alpha) / beta;
+      scal = 1.0 / (alpha - beta);
+    }
+    // v (explicit, unit at row j; zero for the trivial reflector), R diagonal, zeros below it
+    for (int i = tid; i < m; i += NT) {
+      if (i > j) {
+        const T vi = (T)((double)Fb[(int64_t)i * ldf + j] * scal);
+        Vb[(int64_t)i * ldf + j] = vi;
+        Fb[(int64_t)i * ldf + j] = T(0);
+      } else if (i == j) {
+        Vb[(int64_t)i * ldf + j] = trivial ? T(0) : T(1);
+        Fb[(int64_t)i * ldf + j] = (T)beta;
+      } else if (i >= j0) {
+        Vb[(int64_t)i * ldf + j] = T(0);
+      }
+    }
+    if (tid == 0) tb[j] = (T)tj;
+    if (!trivial && c + 1 < nbc) {
+      // w_k = v^T F(:, k) over rows >= j, then F(:, k) -= tau v w_k, for the panel's later columns
+      double w[kPanel];
+#pragma unroll
+      for (int k = 0; k < kPanel; ++k) w[k] = 0.0;
+      for (int i = tid; i < m; i += NT)
+        if (i >= j) {
+          const double vi = (double)Vb[(int64_t)i * ldf + j];
+          const T *row = Fb + (int64_t)i * ldf + j0;
+#pragma unroll
+          for (int k = 0; k < kPanel; ++k)
+            if (k > c && k < nbc) w[k] = fma(vi, (double)row[k], w[k]);
+        }
+      block_sum_vec<NT>(w, c + 1, nbc, red, wsum);
+      for (int i = tid; i < m; i += NT)
+        if (i >= j) {
+          const double tv = tj * (double)Vb[(int64_t)i * ldf + j];
+          T *row = Fb + (int64_t)i * ldf + j0;
+#pragma unroll
+          for (int k = 0; k < kPanel; ++k)
+            if (k > c && k < nbc) row[k] = (T)((double)row[k] - tv * wsum[k]);
+        }
+    }
+    __syncthreads();
+  }
+}
+
+// Compact-WY factor of one panel: T (nbc x nbc upper triangular, stored kPanel x kPanel) from
+// G = V^T V and tau (LAPACK larft, forward columnwise): T_ii = tau_i,
+// T(0:i, i) = -tau_i T(0:i, 0:i) G(0:i, i).
+template <typename T>
+__global__ __launch_bounds__(64) void qr_larft_kernel(const T *G, int64_t sg, const T *tau, int64_t stau, int j0,
+                                                      T *Tf, int64_t stf, int nbc) {
+  __shared__ double Ts[kPanel][kPanel + 1];
+  __shared__ double Gs[kPanel][kPanel + 1];
+  const int tid = threadIdx.x;
+  const T *Gb = G + (int64_t)blockIdx.x * sg;
+  const T *taub = tau + (int64_t)blockIdx.x * stau + j0;
+  T *Tb = Tf + (int64_t)blockIdx.x * stf;
+  for (int e = tid; e < kPanel * kPanel; e += 64) {
+    const int r = e / kPanel, c = e % kPanel;
+    Ts[r][c] = 0.0;
+    Gs[r][c] = (r < nbc && c < nbc) ? (double)Gb[r * kPanel + c] : 0.0;
+  }
+  __syncthreads();
+  for (int i = 0; i < nbc; ++i) {
+    const double ti = (double)taub[i];
+    if (tid < i) {
+      double s = 0.0;
+      for (int l = tid; l < i; ++l) s = fma(Ts[tid][l], Gs[l][i], s);
+      Ts[tid][i] = -ti * s;
+    }
+    if (tid == i) Ts[i][i] = ti;
+    __syncthreads();
+  }
+  for (int e = tid; e < kPanel * kPanel; e += 64) Tb[e] = (T)Ts[e / kPanel][e % kPanel];
+}
+
+// dst (rows x cols, ld_d) = src (ld_s) for each batch member; optional unit diagonal fill.
+template <typename T>
+__global__ void copy2d_kernel(const T *src, int64_t lds, int64_t ss, T *dst, int64_t ldd, int64_t sd, int rows,
+                              int cols, int eye) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)rows * cols) return;
+  const int r = (int)(e / cols), c = (int)(e - (int64_t)r * cols);
+  const int64_t b = blockIdx.y;
+  dst[b * sd + (int64_t)r * ldd + c] = eye ? (r == c ? T(1) : T(0)) : src[b * ss + (int64_t)r * lds + c];
+}
+
+template <typename T>
+static int copy2d(const T *src, int64_t lds, int64_t ss, T *dst, int64_t ldd, int64_t sd, int rows, int cols,
+                  int64_t batch, bool eye, hipStream_t st) {
+  const int64_t e = (int64_t)rows * cols;
+  if (e == 0 || batch == 0) return PSVD_OK;
+  hipLaunchKernelGGL(copy2d_kernel<T>, dim3((unsigned)((e + 255) / 256), (unsigned)batch), dim3(256), 0, st, src,
+                     lds, ss, dst, ldd, sd, rows, cols, eye ? 1 : 0);
+  return hipGetLastError() == hipSuccess ? PSVD_OK : PSVD_ELAUNCH;
+}
+
+template <typename T>
+size_t qr_ws_bytes(int64_t batch, int m, int n) {
+  const int64_t np = (n + kPanel - 1) / kPanel;
+  auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+  return al(sizeof(T) * batch * m * n) * 2 + al(sizeof(T) * batch * np * kPanel * kPanel) + al(sizeof(T) * batch * n) +
+         al(sizeof(T) * batch * kPanel * kPanel) + al(sizeof(T) * batch * kPanel * n) * 2;
+}
+
+#define PSVD_TRY(x)            \
+  do {                         \
+    const int _st = (x);       \
+    if (_st != PSVD_OK) return _st; \
+  } while (0)
+
+template <typename T>
+static int run_qr(int gmode, int64_t batch, int m, int n, const T *a, int64_t lda, int64_t sa, T *q, T *r,
+                  void *ws, size_t ws_bytes, hipStream_t st) {
+  if (ws_bytes < qr_ws_bytes<T>(batch, m, n)) return PSVD_EINVAL;
+  const int64_t np = (n + kPanel - 1) / kPanel, mn = (int64_t)m * n;
+  char *w = static_cast<char *>(ws);
+  auto take = [&](size_t bytes) {
+    char *p = w;
+    w += (bytes + 255) & ~size_t(255);
+    return reinterpret_cast<T *>(p);
+  };
+  T *F = take(sizeof(T) * batch * mn);
+  T *V = take(sizeof(T) * batch * mn);
+  T *Tf = take(sizeof(T) * batch * np * kPanel * kPanel);
+  T *tau = take(sizeof(T) * batch * n);
+  T *G = take(sizeof(T) * batch * kPanel * kPanel);
+  T *W1 = take(sizeof(T) * batch * kPanel * n);
+  T *W2 = take(sizeof(T) * batch * kPanel * n);
+  const int64_t stf = np * kPanel * kPanel;
+  PSVD_TRY(copy2d<T>(a, lda, sa, F, n, mn, m, n, batch, false, st));
+  for (int64_t p = 0; p < np; ++p) {
+    const int j0 = (int)(p * kPanel), nbc = n - j0 < kPanel ? n - j0 : kPanel, rows = m - j0, n2 = n - j0 - nbc;
+    hipLaunchKernelGGL(qr_panel_kernel<T>, dim3((unsigned)batch), dim3(kPanelThreads), 0, st, F, (int64_t)n, mn, V,
+                       mn, tau, (int64_t)n, m, j0, nbc);
+    if (hipGetLastError() != hipSuccess) return PSVD_ELAUNCH;
+    T *Vp = V + (int64_t)j0 * n + j0;  // (rows x nbc), ld n
+    PSVD_TRY(psvd_gemm(gmode, 1, 0, nbc, nbc, rows, 1.0, Vp, n, mn, Vp, n, mn, 0.0, G, kPanel, kPanel * kPanel,
+                       batch, st));
+    hipLaunchKernelGGL(qr_larft_kernel<T>, dim3((unsigned)batch), dim3(64), 0, st, G, (int64_t)kPanel * kPanel, tau,
+                       (int64_t)n, j0, Tf + p * kPanel * kPanel, stf, nbc);
+    if (hipGetLastError() != hipSuccess) return PSVD_ELAUNCH;
+    if (n2 > 0) {
+      T *F2 = F + (int64_t)j0 * n + j0 + nbc;  // trailing block (rows x n2), ld n
+      T *Tp = Tf + p * kPanel * kPanel;
+      PSVD_TRY(psvd_gemm(gmode, 1, 0, nbc, n2, rows, 1.0, Vp, n, mn, F2, n, mn, 0.0, W1, n, (int64_t)kPanel * n, batch,
+                         st));
+      PSVD_TRY(psvd_gemm(gmode, 1, 0, nbc, n2, nbc, 1.0, Tp, kPanel, stf, W1, n, (int64_t)kPanel * n, 0.0, W2, n,
+                         (int64_t)kPanel * n, batch, st));
+      PSVD_TRY(psvd_gemm(gmode, 0, 0, rows, n2, nbc, -1.0, Vp, n, mn, W2, n, (int64_t)kPanel * n, 1.0, F2, n, mn, batch,
+                         st));
+    }
+  }
+  if (r) PSVD_TRY(copy2d<T>(F, n, mn, r, n, (int64_t)n * n, n, n, batch, false, st));
+  if (q) {
+    PSVD_TRY(copy2d<T>(nullptr, 0, 0, q, n, mn, m, n, batch, true, st));
+    for (int64_t p = np - 1; p >= 0; --p) {
+      const int j0 = (int)(p * kPanel), nbc = n - j0 < kPanel ? n - j0 : kPanel, rows = m - j0, nq = n - j0;
+      const T *Vp = V + (int64_t)j0 * n + j0;
+      T *Qs = q + (int64_t)j0 * n + j0;
+      T *Tp = Tf + p * kPanel * kPanel;
+      PSVD_TRY(psvd_gemm(gmode, 1, 0, nbc, nq, rows, 1.0, Vp, n, mn, Qs, n, mn, 0.0, W1, n, (int64_t)kPanel * n, batch,
+                         st));
+      PSVD_TRY(psvd_gemm(gmode, 0, 0, nbc, nq, nbc, 1.0, Tp, kPanel, stf, W1, n, (int64_t)kPanel * n, 0.0, W2, n,
+                         (int64_t)kPanel * n, batch, st));
+      PSVD_TRY(psvd_gemm(gmode, 0, 0, rows, nq, nbc, -1.0, Vp, n, mn, W2, n, (int64_t)kPanel * n, 1.0, Qs, n, mn, batch,
+                         st));
+    }
+  }
+  return PSVD_OK;
+}
+
+}  // namespace psvd
+
+extern "C" {
+
+size_t psvd_qr_workspace_bytes(int dtype, int64_t batch, int m, int n) {
+  if (batch < 0 || m < 1 || n < 1 || n > m) return 0;
+  if (dtype == PSVD_F32) return psvd::qr_ws_bytes<float>(batch, m, n);
+  if (dtype == PSVD_F64) return psvd::qr_ws_bytes<double>(batch, m, n);
+  return 0;
+}
+
+int psvd_qr(int dtype, int gemm_mode, int64_t batch, int m, int n, const void *a, int64_t lda, int64_t stride_a,
+            void *q, void *r, void *workspace, size_t workspace_bytes, psvd_stream_t stream) {
+  if (batch < 0 || m < 1 || n < 1 || n > m || lda < n || !a || !workspace) return PSVD_EINVAL;
+  if (batch == 0) return PSVD_OK;
+  if (batch > 65535) return PSVD_EINVAL;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (dtype == PSVD_F32) {
+    if (gemm_mode != PSVD_GEMM_F32 && gemm_mode != PSVD_GEMM_BF16X3) return PSVD_EINVAL;
+    return psvd::run_qr<float>(gemm_mode, batch, m, n, static_cast<const float *>(a), lda, stride_a,
+                               static_cast<float *>(q), static_cast<float *>(r), workspace, workspace_bytes, st);
+  }
+  if (dtype == PSVD_F64)
+    return psvd::run_qr<double>(PSVD_GEMM_F64, batch, m, n, static_cast<const double *>(a), lda, stride_a,
+                                static_cast<double *>(q), static_cast<double *>(r), workspace, workspace_bytes, st);
+  return PSVD_EINVAL;
+}
+
+}  // extern "C"

@@ -5,6 +5,7 @@ Mirrors the names and return conventions of svdbench/standalone/primitives.py fo
 
     jacobi_svd   primitives.py:316-367  -> K1 (LDS-resident batched one-sided Jacobi) or,
                                            for matrices beyond LDS, K2 (cooperative launch)
+    qr           primitives.py:41-89    -> K4 (blocked Householder QR, xLARFG reflectors)
 
 Tensors must live on a ROCm device; there is no CPU path.
 """
@@ -96,3 +97,71 @@ def jacobi_svd(A, tol=None, max_sweeps=30, *, cut='reference'):
     info = info.cpu()
     N.raise_for_info(info, f'one-sided Jacobi did not converge in {max_sweeps} sweeps')
     return u, s, vh.mT, int(info[:, 0].max()) if b else 0
+
+
+def qr(A, *, gemm='f32', want_q=True):
+    """Householder QR of the tall batch A (b, m, n): Q (b, m, n), R (b, n, n) (primitives.py:74-89).
+
+    gemm: 'f32' or 'bf16x3' arithmetic for the FP32 panel updates (FP64 inputs use FP64).
+    want_q=False skips forming Q (returns None for it)."""
+    b, m, n = A.shape
+    if m < n:
+        raise ValueError('qr takes tall matrices (m >= n)')
+    code = N.dtype_code(A.dtype)
+    if A.stride(-1) != 1:
+        A = A.contiguous()
+    q = torch.empty(b, m, n, dtype=A.dtype, device=A.device) if want_q else None
+    r = torch.empty(b, n, n, dtype=A.dtype, device=A.device)
+    wsb = N._hip.qr_workspace_bytes(code, b, m, n)
+    ws = torch.empty(wsb, dtype=torch.uint8, device=A.device)
+    gmode = {'f32': 0, 'bf16x3': 1}[gemm]
+    st = N._hip.qr(code, gmode, b, m, n, N.ptr(A), A.stride(-2), A.stride(0) if b > 1 else 0, N.ptr(q), N.ptr(r),
+                   N.ptr(ws), wsb, N.stream_of(A))
+    N.check(st, 'psvd_qr')
+    return q, r
+
+
+# --- per-matrix reductions and scalings (batched utility kernels) -----------------------------
+def _mat3(x):
+    x3 = x if x.dim() == 3 else x.unsqueeze(0)
+    if x3.stride(-1) != 1:
+        x3 = x3.contiguous()
+    return x3, x3.stride(-2), (x3.stride(0) if x3.shape[0] > 1 else 0)
+
+
+def reduce(x, op, y=None):
+    """Per-matrix reduction of (b, r, c) device tensors into FP64 device scalars:
+    'maxabs' -> (b,), 'fro' -> (b,), 'maxdiff' -> (b, 2) = [max|x-y|, max|x|]."""
+    x3, ldx, sx = _mat3(x)
+    b, rows, cols = x3.shape
+    code = {'maxabs': 0, 'fro': 1, 'maxdiff': 2}[op]
+    out = torch.empty(2 * b if code == 2 else b, dtype=torch.float64, device=x.device)
+    if code == 2:
+        y3, ldy, sy = _mat3(y)
+    else:
+        y3, ldy, sy = None, 0, 0
+    st = N._hip.reduce(N.dtype_code(x.dtype), code, b, rows, cols, N.ptr(x3), ldx, sx, N.ptr(y3), ldy, sy,
+                       N.ptr(out), N.stream_of(x))
+    N.check(st, 'psvd_reduce')
+    return out.view(2, b).mT if code == 2 else out
+
+
+def scale(x, d=None, *, invert=False, s=1.0, out=None):
+    """out_b = x_b / d_b (invert; x / scale as the reference rounds it) or (s * d_b) * x_b."""
+    x3, ldx, sx = _mat3(x)
+    b, rows, cols = x3.shape
+    if out is None:
+        out = torch.empty_like(x3)
+    o3, ldo, so = _mat3(out)
+    st = N._hip.scale(N.dtype_code(x.dtype), b, rows, cols, N.ptr(x3), ldx, sx, N.ptr(o3), ldo, so,
+                      N.ptr(d), int(invert), float(s), N.stream_of(x))
+    N.check(st, 'psvd_scale')
+    return out if out.dim() == x.dim() else out.view_as(x)
+
+
+def symmetrize_(a):
+    """a <- (a + a^T) / 2 in place for a (b, n, n) or (n, n) device tensor."""
+    a3, lda, sa = _mat3(a)
+    st = N._hip.symmetrize(N.dtype_code(a.dtype), a3.shape[0], a3.shape[-1], N.ptr(a3), lda, sa, N.stream_of(a))
+    N.check(st, 'psvd_symmetrize')
+    return a
