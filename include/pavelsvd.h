@@ -101,12 +101,27 @@ int psvd_gemm(int mode, int transa, int transb, int64_t m, int64_t n, int64_t k,
  *  psvd_scale:  y_b = x_b / d_b (invert, T division as the reference's x / scale) or
  *               y_b = (s * d_b) x_b; d_b <= 0 counts as 1; d may be NULL.
  *  psvd_symmetrize: a_b = (a_b + a_b^T) / 2 in place (solvers.py:156, 169).
+ *  psvd_axpby: y_b = alpha x_b + beta y_b (the QDWH update, solvers.py:153, 160).
  * ------------------------------------------------------------------------------------- */
 int psvd_reduce(int dtype, int op, int64_t batch, int rows, int cols, const void *x, int64_t ldx, int64_t sx,
                 const void *y, int64_t ldy, int64_t sy, double *out, psvd_stream_t stream);
 int psvd_scale(int dtype, int64_t batch, int rows, int cols, const void *x, int64_t ldx, int64_t sx, void *y,
                int64_t ldy, int64_t sy, const double *d, int invert, double s, psvd_stream_t stream);
 int psvd_symmetrize(int dtype, int64_t batch, int n, void *a, int64_t lda, int64_t sa, psvd_stream_t stream);
+int psvd_axpby(int dtype, int64_t batch, int rows, int cols, double alpha, const void *x, int64_t ldx, int64_t sx,
+               double beta, void *y, int64_t ldy, int64_t sy, psvd_stream_t stream);
+
+/* ---------------------------------------------------------------------------------------
+ * Completion of missing basis columns (primitives.py:430-467): for each b, columns [nk[b], n) of
+ * U_b (m x n, row-major) are replaced in order by the projected coordinate vector at the row of
+ * largest remaining leverage (two Gram-Schmidt passes, FP64); status[b] = PSVD_INFO_COMPLETION
+ * when a residual falls below 10 eps.  Serves complete_null_columns (sorted sigma: the missing
+ * columns are the tail) and complete_basis (full_matrices).
+ * ------------------------------------------------------------------------------------- */
+size_t psvd_complete_workspace_bytes(int64_t batch, int m, int n);
+int psvd_complete_columns(int dtype, int64_t batch, int m, int n, void *u, int64_t ldu, int64_t stride_u,
+                          const int32_t *nk, int32_t *status, void *workspace, size_t workspace_bytes,
+                          psvd_stream_t stream);
 
 /* ---------------------------------------------------------------------------------------
  * Blocked Householder QR of a batch of tall matrices: A_i = Q_i R_i, Q_i thin (m x n, row-major,
@@ -118,6 +133,28 @@ int psvd_symmetrize(int dtype, int64_t batch, int n, void *a, int64_t lda, int64
 size_t psvd_qr_workspace_bytes(int dtype, int64_t batch, int m, int n);
 int psvd_qr(int dtype, int gemm_mode, int64_t batch, int m, int n, const void *a, int64_t lda, int64_t stride_a,
             void *q, void *r, void *workspace, size_t workspace_bytes, psvd_stream_t stream);
+
+/* ---------------------------------------------------------------------------------------
+ * Two-sided cyclic Jacobi eigensolver of a batch of symmetric n x n matrices (primitives.py:
+ * 370-427): w (batch, n) ascending, v (batch, n, n) row-major eigenvectors as columns,
+ * info = [sweeps, status].  Uses an LDS-resident kernel (one workgroup per matrix) when the
+ * matrix fits, otherwise one cooperative launch per matrix with the workspace of
+ * psvd_eigh_workspace_bytes.  tol <= 0 selects 10*eps(dtype).
+ * ------------------------------------------------------------------------------------- */
+size_t psvd_eigh_workspace_bytes(int dtype, int n);
+int psvd_eigh(int dtype, int64_t batch, int n, const void *h, void *w, void *v, int32_t *info, double tol,
+              int max_sweeps, void *workspace, size_t workspace_bytes, psvd_stream_t stream);
+
+/* ---------------------------------------------------------------------------------------
+ * Blocked Cholesky A = L L^T in place (lower; strict upper cleared) with the reference's failure
+ * mask fail[b] (a pivot <= 0 is clamped and flagged, primitives.py:145-159), and right-sided
+ * triangular solves B <- B L^-T (trans = 1) or B <- B L^-1 (trans = 0) for rows x n blocks B
+ * (primitives.py:162-179 in the transposed form QDWH uses, solvers.py:159).
+ * ------------------------------------------------------------------------------------- */
+int psvd_cholesky(int dtype, int gemm_mode, int64_t batch, int n, void *a, int64_t lda, int64_t stride_a,
+                  int32_t *fail, psvd_stream_t stream);
+int psvd_trsm_right(int dtype, int gemm_mode, int64_t batch, int trans, int n, int rows, const void *l, int64_t ldl,
+                    int64_t stride_l, void *b, int64_t ldb, int64_t stride_b, psvd_stream_t stream);
 
 /* ---------------------------------------------------------------------------------------
  * One-sided Jacobi SVD of ONE tall matrix that does not fit LDS (e.g. 512x256, 2048x32, the

@@ -134,6 +134,87 @@ static PyObject *py_symmetrize(PyObject *, PyObject *args) {
   return PyLong_FromLong(psvd_symmetrize(dtype, batch, n, PTR(a), lda, sa, PTR(stream)));
 }
 
+static PyObject *py_eigh_workspace_bytes(PyObject *, PyObject *args) {
+  int dtype, n;
+  if (!PyArg_ParseTuple(args, "ii", &dtype, &n)) return nullptr;
+  return PyLong_FromSize_t(psvd_eigh_workspace_bytes(dtype, n));
+}
+
+static PyObject *py_eigh(PyObject *, PyObject *args) {
+  int dtype, n, max_sweeps;
+  long long batch;
+  unsigned long long h, w, v, info, ws, ws_bytes, stream;
+  double tol;
+  if (!PyArg_ParseTuple(args, "iLiKKKKdiKKK", &dtype, &batch, &n, &h, &w, &v, &info, &tol, &max_sweeps, &ws, &ws_bytes,
+                        &stream))
+    return nullptr;
+  int st;
+  Py_BEGIN_ALLOW_THREADS
+  st = psvd_eigh(dtype, batch, n, PTR(h), PTR(w), PTR(v), static_cast<int32_t *>(PTR(info)), tol, max_sweeps, PTR(ws),
+                 (size_t)ws_bytes, PTR(stream));
+  Py_END_ALLOW_THREADS
+  return PyLong_FromLong(st);
+}
+
+static PyObject *py_cholesky(PyObject *, PyObject *args) {
+  int dtype, gmode, n;
+  long long batch, lda, sa;
+  unsigned long long a, fail, stream;
+  if (!PyArg_ParseTuple(args, "iiLiKLLKK", &dtype, &gmode, &batch, &n, &a, &lda, &sa, &fail, &stream)) return nullptr;
+  int st;
+  Py_BEGIN_ALLOW_THREADS
+  st = psvd_cholesky(dtype, gmode, batch, n, PTR(a), lda, sa, static_cast<int32_t *>(PTR(fail)), PTR(stream));
+  Py_END_ALLOW_THREADS
+  return PyLong_FromLong(st);
+}
+
+static PyObject *py_trsm_right(PyObject *, PyObject *args) {
+  int dtype, gmode, trans, n, rows;
+  long long batch, ldl, sl, ldb, sb;
+  unsigned long long l, b, stream;
+  if (!PyArg_ParseTuple(args, "iiLiiiKLLKLLK", &dtype, &gmode, &batch, &trans, &n, &rows, &l, &ldl, &sl, &b, &ldb, &sb,
+                        &stream))
+    return nullptr;
+  int st;
+  Py_BEGIN_ALLOW_THREADS
+  st = psvd_trsm_right(dtype, gmode, batch, trans, n, rows, PTR(l), ldl, sl, PTR(b), ldb, sb, PTR(stream));
+  Py_END_ALLOW_THREADS
+  return PyLong_FromLong(st);
+}
+
+static PyObject *py_axpby(PyObject *, PyObject *args) {
+  int dtype, rows, cols;
+  long long batch, ldx, sx, ldy, sy;
+  unsigned long long x, y, stream;
+  double alpha, beta;
+  if (!PyArg_ParseTuple(args, "iLiidKLLdKLLK", &dtype, &batch, &rows, &cols, &alpha, &x, &ldx, &sx, &beta, &y, &ldy,
+                        &sy, &stream))
+    return nullptr;
+  return PyLong_FromLong(psvd_axpby(dtype, batch, rows, cols, alpha, PTR(x), ldx, sx, beta, PTR(y), ldy, sy, PTR(stream)));
+}
+
+static PyObject *py_complete_workspace_bytes(PyObject *, PyObject *args) {
+  long long batch;
+  int m, n;
+  if (!PyArg_ParseTuple(args, "Lii", &batch, &m, &n)) return nullptr;
+  return PyLong_FromSize_t(psvd_complete_workspace_bytes(batch, m, n));
+}
+
+static PyObject *py_complete_columns(PyObject *, PyObject *args) {
+  int dtype, m, n;
+  long long batch, ldu, su;
+  unsigned long long u, nk, status, ws, ws_bytes, stream;
+  if (!PyArg_ParseTuple(args, "iLiiKLLKKKKK", &dtype, &batch, &m, &n, &u, &ldu, &su, &nk, &status, &ws, &ws_bytes,
+                        &stream))
+    return nullptr;
+  int st;
+  Py_BEGIN_ALLOW_THREADS
+  st = psvd_complete_columns(dtype, batch, m, n, PTR(u), ldu, su, static_cast<const int32_t *>(PTR(nk)),
+                             static_cast<int32_t *>(PTR(status)), PTR(ws), (size_t)ws_bytes, PTR(stream));
+  Py_END_ALLOW_THREADS
+  return PyLong_FromLong(st);
+}
+
 static PyMethodDef kMethods[] = {
     {"version", py_version, METH_NOARGS, "library version string"},
     {"jacobi_lds_workspace_bytes", py_jacobi_lds_workspace_bytes, METH_VARARGS, nullptr},
@@ -142,9 +223,16 @@ static PyMethodDef kMethods[] = {
     {"jacobi_grid_workspace_bytes", py_jacobi_grid_workspace_bytes, METH_VARARGS, nullptr},
     {"jacobi_grid", py_jacobi_grid, METH_VARARGS, nullptr},
     {"gemm", py_gemm, METH_VARARGS, nullptr},
+    {"eigh_workspace_bytes", py_eigh_workspace_bytes, METH_VARARGS, nullptr},
+    {"eigh", py_eigh, METH_VARARGS, nullptr},
+    {"cholesky", py_cholesky, METH_VARARGS, nullptr},
+    {"trsm_right", py_trsm_right, METH_VARARGS, nullptr},
     {"reduce", py_reduce, METH_VARARGS, nullptr},
     {"scale", py_scale, METH_VARARGS, nullptr},
     {"symmetrize", py_symmetrize, METH_VARARGS, nullptr},
+    {"axpby", py_axpby, METH_VARARGS, nullptr},
+    {"complete_workspace_bytes", py_complete_workspace_bytes, METH_VARARGS, nullptr},
+    {"complete_columns", py_complete_columns, METH_VARARGS, nullptr},
     {"qr_workspace_bytes", py_qr_workspace_bytes, METH_VARARGS, nullptr},
     {"qr", py_qr, METH_VARARGS, nullptr},
     {nullptr, nullptr, 0, nullptr}};

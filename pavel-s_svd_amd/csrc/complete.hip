@@ -1,0 +1,143 @@
+// Completion of missing columns of a batch of orthonormal bases (svdbench/standalone/
+// primitives.py:430-467): U (m x n, row-major) has live columns [0, nk_b) for matrix b and the
+// columns [nk_b, n) are replaced, one after the other, by the coordinate vector at the row of
+// largest remaining leverage 1 - |U(i, :)|^2 (first such row), projected twice against all
+// current columns (classical Gram-Schmidt, twice) and normalised; a residual below 10 eps(T)
+// sets the failure status (the reference raises).  Columns still to be completed are zero while
+// earlier ones are used, exactly as in the reference.  FP64 accumulation throughout.
+//
+// One workgroup per matrix: leverage and the row products run one thread per row (contiguous row
+// reads), U^T v runs one thread per column (coalesced across the workgroup).
+#include <float.h>
+
+#include "common.hpp"
+
+namespace psvd {
+
+constexpr int kCompleteThreads = 1024;
+
+template <typename T>
+__global__ __launch_bounds__(kCompleteThreads) void complete_kernel(T *U, int64_t ldu, int64_t su, int m, int n,
+                                                                    const int32_t *nk, int32_t *status, double *vbuf,
+                                                                    double *ybuf) {
+  constexpr int NT = kCompleteThreads, NW = NT / kWave;
+  __shared__ double red[NW];
+  __shared__ int redi[NW];
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+  const int64_t b = blockIdx.x;
+  T *Ub = U + b * su;
+  double *v = vbuf + b * (int64_t)m;
+  double *y = ybuf + b * (int64_t)n;
+  const int k0 = nk[b] < 0 ? 0 : (nk[b] > n ? n : nk[b]);
+  const double eps = Eps<T>::value;
+  int st = 0;
+  // missing columns start at zero
+  for (int64_t e = tid; e < (int64_t)m * (n - k0); e += NT) {
+    const int i = (int)(e / (n - k0)), c = k0 + (int)(e - (int64_t)i * (n - k0));
+    Ub[(int64_t)i * ldu + c] = T(0);
+  }
+  __syncthreads();
+  for (int r = k0; r < n; ++r) {
+    double best = -DBL_MAX;
+    int bi = 0x7fffffff;
+    for (int i = tid; i < m; i += NT) {
+      const T *row = Ub + (int64_t)i * ldu;
+      double acc = 0.0;
+      for (int c = 0; c < n; ++c) acc = fma((double)row[c], (double)row[c], acc);
+      const double lev = 1.0 - acc;
+      if (lev > best) {
+        best = lev;
+        bi = i;
+      }
+    }
+    for (int o = kWave >> 1; o > 0; o >>= 1) {
+      const double ob = __shfl_xor(best, o, kWave);
+      const int oi = __shfl_xor(bi, o, kWave);
+      if (ob > best || (ob == best && oi < bi)) {
+        best = ob;
+        bi = oi;
+      }
+    }
+    if (lane == 0) {
+      red[wave] = best;
+      redi[wave] = bi;
+    }
+    __syncthreads();
+    best = red[0];
+    bi = redi[0];
+    for (int w = 1; w < NW; ++w)
+      if (red[w] > best || (red[w] == best && redi[w] < bi)) {
+        best = red[w];
+        bi = redi[w];
+      }
+    const int ix = bi;
+    const T *rix = Ub + (int64_t)ix * ldu;
+    // v = e_ix - U (U^T e_ix) = e_ix - U U(ix, :)^T
+    for (int i = tid; i < m; i += NT) {
+      const T *row = Ub + (int64_t)i * ldu;
+      double acc = (i == ix) ? 1.0 : 0.0;
+      for (int c = 0; c < n; ++c) acc -= (double)row[c] * (double)rix[c];
+      v[i] = acc;
+    }
+    __syncthreads();
+    // y = U^T v (one thread per column), then v -= U y and |v|
+    for (int c = tid; c < n; c += NT) {
+      double acc = 0.0;
+      for (int i = 0; i < m; ++i) acc = fma((double)Ub[(int64_t)i * ldu + c], v[i], acc);
+      y[c] = acc;
+    }
+    __syncthreads();
+    double l2 = 0.0;
+    for (int i = tid; i < m; i += NT) {
+      const T *row = Ub + (int64_t)i * ldu;
+      double acc = v[i];
+      for (int c = 0; c < n; ++c) acc -= (double)row[c] * y[c];
+      v[i] = acc;
+      l2 = fma(acc, acc, l2);
+    }
+    l2 = wave_sum(l2);
+    __syncthreads();
+    if (lane == 0) red[wave] = l2;
+    __syncthreads();
+    double len2 = 0.0;
+    for (int w = 0; w < NW; ++w) len2 += red[w];
+    const double len = sqrt(len2);
+    if (len < 10.0 * eps) st = PSVD_INFO_COMPLETION;
+    const double inv = 1.0 / fmax(len, DBL_MIN);
+    for (int i = tid; i < m; i += NT) Ub[(int64_t)i * ldu + r] = (T)(v[i] * inv);
+    __syncthreads();
+  }
+  if (tid == 0) status[b] = st;
+}
+
+}  // namespace psvd
+
+extern "C" {
+
+size_t psvd_complete_workspace_bytes(int64_t batch, int m, int n) {
+  if (batch < 0 || m < 0 || n < 0) return 0;
+  return (size_t)8 * batch * ((size_t)m + n) + 256;
+}
+
+int psvd_complete_columns(int dtype, int64_t batch, int m, int n, void *u, int64_t ldu, int64_t stride_u,
+                          const int32_t *nk, int32_t *status, void *workspace, size_t workspace_bytes,
+                          psvd_stream_t stream) {
+  if (batch < 0 || m < 1 || n < 0 || n > m || ldu < n || !u || !nk || !status || !workspace || batch > 0x7fffffff)
+    return PSVD_EINVAL;
+  if (batch == 0 || n == 0) return PSVD_OK;
+  if (workspace_bytes < psvd_complete_workspace_bytes(batch, m, n)) return PSVD_EINVAL;
+  double *vb = static_cast<double *>(workspace);
+  double *yb = vb + batch * (int64_t)m;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (dtype == PSVD_F32)
+    hipLaunchKernelGGL(psvd::complete_kernel<float>, dim3((unsigned)batch), dim3(psvd::kCompleteThreads), 0, st,
+                       static_cast<float *>(u), ldu, stride_u, m, n, nk, status, vb, yb);
+  else if (dtype == PSVD_F64)
+    hipLaunchKernelGGL(psvd::complete_kernel<double>, dim3((unsigned)batch), dim3(psvd::kCompleteThreads), 0, st,
+                       static_cast<double *>(u), ldu, stride_u, m, n, nk, status, vb, yb);
+  else
+    return PSVD_EINVAL;
+  return hipGetLastError() == hipSuccess ? PSVD_OK : PSVD_ELAUNCH;
+}
+
+}  // extern "C"

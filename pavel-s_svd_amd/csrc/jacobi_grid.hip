@@ -375,11 +375,13 @@ static int grid_size(int n) {
   (void)hipGetDevice(&dev);
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   if (cus <= 0) cus = 1;
+  // at most one workgroup per CU, so that every workgroup of the cooperative grid is resident
+  // without relying on the occupancy estimate
+  (void)occ;
   const int np = n + (n & 1);
   int g = np / 2;
   if (g < 64) g = 64;
-  const int cap = cus * occ;
-  return g < cap ? g : cap;
+  return g < cus ? g : cus;
 }
 
 template <typename T>

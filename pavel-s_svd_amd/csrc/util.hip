@@ -1,7 +1,7 @@
 // Batched element-wise and reduction kernels used between the solver stages: per-matrix maximum
 // of |a| and Frobenius norm (the two input normalisations of solvers.py:294-295 and :124-125),
 // the QDWH convergence pair max|x' - x|, max|x'| (solvers.py:161-163), scaling by a per-matrix
-// device scalar, symmetrisation (H + H^T)/2 (solvers.py:156, 169) and column gathers.
+// device scalar, symmetrisation (H + H^T)/2 (solvers.py:156, 169) and a + b axpby updates.
 //
 // Reductions run over a 2-D grid (chunks of one matrix x batch) and combine chunk results with
 // device atomics on the bit patterns of non-negative doubles (monotone as integers) or double
@@ -87,6 +87,18 @@ __global__ __launch_bounds__(kUtilThreads) void scale_kernel(const T *x, int64_t
 }
 
 template <typename T>
+__global__ __launch_bounds__(kUtilThreads) void axpby_kernel(T alpha, const T *x, int64_t ldx, int64_t sx, T beta, T *y,
+                                                             int64_t ldy, int64_t sy, int rows, int cols) {
+  const int64_t b = blockIdx.y;
+  const int64_t total = (int64_t)rows * cols;
+  for (int64_t e = (int64_t)blockIdx.x * kUtilThreads + threadIdx.x; e < total; e += (int64_t)gridDim.x * kUtilThreads) {
+    const int r = (int)(e / cols), c = (int)(e - (int64_t)r * cols);
+    T *py = y + b * sy + (int64_t)r * ldy + c;
+    *py = alpha * x[b * sx + (int64_t)r * ldx + c] + beta * *py;
+  }
+}
+
+template <typename T>
 __global__ __launch_bounds__(kUtilThreads) void symmetrize_kernel(T *a, int64_t lda, int64_t sa, int n) {
   const int64_t b = blockIdx.y;
   const int64_t total = (int64_t)n * n;
@@ -151,6 +163,23 @@ int psvd_scale(int dtype, int64_t batch, int rows, int cols, const void *x, int6
     hipLaunchKernelGGL(psvd::scale_kernel<double>, grid, dim3(psvd::kUtilThreads), 0, st,
                        static_cast<const double *>(x), ldx, sx, static_cast<double *>(y), ldy, sy, rows, cols, d,
                        invert, s);
+  else
+    return PSVD_EINVAL;
+  return hipGetLastError() == hipSuccess ? PSVD_OK : PSVD_ELAUNCH;
+}
+
+int psvd_axpby(int dtype, int64_t batch, int rows, int cols, double alpha, const void *x, int64_t ldx, int64_t sx,
+               double beta, void *y, int64_t ldy, int64_t sy, psvd_stream_t stream) {
+  if (batch < 0 || rows < 0 || cols < 0 || !x || !y || batch > 65535) return PSVD_EINVAL;
+  if (batch == 0 || rows == 0 || cols == 0) return PSVD_OK;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const dim3 grid(psvd::grid_chunks((int64_t)rows * cols), (unsigned)batch);
+  if (dtype == PSVD_F32)
+    hipLaunchKernelGGL(psvd::axpby_kernel<float>, grid, dim3(psvd::kUtilThreads), 0, st, (float)alpha,
+                       static_cast<const float *>(x), ldx, sx, (float)beta, static_cast<float *>(y), ldy, sy, rows, cols);
+  else if (dtype == PSVD_F64)
+    hipLaunchKernelGGL(psvd::axpby_kernel<double>, grid, dim3(psvd::kUtilThreads), 0, st, alpha,
+                       static_cast<const double *>(x), ldx, sx, beta, static_cast<double *>(y), ldy, sy, rows, cols);
   else
     return PSVD_EINVAL;
   return hipGetLastError() == hipSuccess ? PSVD_OK : PSVD_ELAUNCH;

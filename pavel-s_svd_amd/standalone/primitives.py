@@ -6,6 +6,9 @@ Mirrors the names and return conventions of svdbench/standalone/primitives.py fo
     jacobi_svd   primitives.py:316-367  -> K1 (LDS-resident batched one-sided Jacobi) or,
                                            for matrices beyond LDS, K2 (cooperative launch)
     qr           primitives.py:41-89    -> K4 (blocked Householder QR, xLARFG reflectors)
+    jacobi_eigh  primitives.py:370-427  -> K6 (two-sided Jacobi, LDS-resident or cooperative)
+    cholesky     primitives.py:145-159  -> K5 (blocked Cholesky with the failure mask)
+    trsm_lower / trsm_upper  :162-179   -> K5 (right-sided blocked solves, transposed form)
 
 Tensors must live on a ROCm device; there is no CPU path.
 """
@@ -165,3 +168,107 @@ def symmetrize_(a):
     st = N._hip.symmetrize(N.dtype_code(a.dtype), a3.shape[0], a3.shape[-1], N.ptr(a3), lda, sa, N.stream_of(a))
     N.check(st, 'psvd_symmetrize')
     return a
+
+
+def jacobi_eigh_raw(H, tol=None, max_sweeps=30):
+    """Two-sided Jacobi of the symmetric batch H (b, n, n): w ascending, V, info (device)."""
+    b, n, _ = H.shape
+    H = H.contiguous()
+    code = N.dtype_code(H.dtype)
+    w = torch.empty(b, n, dtype=H.dtype, device=H.device)
+    v = torch.empty(b, n, n, dtype=H.dtype, device=H.device)
+    info = torch.empty(b, 2, dtype=torch.int32, device=H.device)
+    wsb = N._hip.eigh_workspace_bytes(code, n)
+    ws = torch.empty(wsb, dtype=torch.uint8, device=H.device)
+    st = N._hip.eigh(code, b, n, N.ptr(H), N.ptr(w), N.ptr(v), N.ptr(info), float(tol or 0.0), int(max_sweeps),
+                     N.ptr(ws), wsb, N.stream_of(H))
+    N.check(st, 'psvd_eigh')
+    return w, v, info
+
+
+def jacobi_eigh(H, tol=None, max_sweeps=30):
+    """Hermitian (real symmetric) H (b, n, n) -> w (b, n) ascending, V (b, n, n), sweeps
+    (primitives.py:370-427).  Raises RuntimeError on non-convergence."""
+    w, v, info = jacobi_eigh_raw(H, tol, max_sweeps)
+    info = info.cpu()
+    N.raise_for_info(info, f'Hermitian Jacobi did not converge in {max_sweeps} sweeps')
+    return w, v, int(info[:, 0].max()) if H.shape[0] else 0
+
+
+def cholesky(G, *, gemm='f32'):
+    """G (b, n, n) symmetric -> L lower (G = L L^T), fail (b,) bool (primitives.py:145-159)."""
+    b, n, _ = G.shape
+    L = G.contiguous().clone()
+    fail = torch.empty(b, dtype=torch.int32, device=G.device)
+    st = N._hip.cholesky(N.dtype_code(G.dtype), {'f32': 0, 'bf16x3': 1}.get(gemm, 0), b, n, N.ptr(L), n, n * n,
+                         N.ptr(fail), N.stream_of(G))
+    N.check(st, 'psvd_cholesky')
+    return L, fail != 0
+
+
+def trsm_right_(B, L, *, trans, gemm='f32'):
+    """In place: B <- B L^-T (trans) or B <- B L^-1 for B (b, rows, n), L (b, n, n) lower."""
+    b, rows, n = B.shape
+    if (B.stride(-1) != 1 and n > 1) or (L.stride(-1) != 1 and n > 1):
+        raise ValueError('trsm_right_ takes row-contiguous operands')
+    ldb = B.stride(-2) if rows > 1 else n
+    ldl = L.stride(-2) if n > 1 else 1
+    st = N._hip.trsm_right(N.dtype_code(B.dtype), {'f32': 0, 'bf16x3': 1}.get(gemm, 0), b, int(trans), n, rows,
+                           N.ptr(L), ldl, L.stride(0) if b > 1 else 0, N.ptr(B), max(ldb, n),
+                           B.stride(0) if b > 1 else 0, N.stream_of(B))
+    N.check(st, 'psvd_trsm_right')
+    return B
+
+
+def trsm_lower(L, B):
+    """Solve L X = B (primitives.py:162-169) as X^T = B^T L^-T."""
+    Xt = B.mT.contiguous()
+    trsm_right_(Xt, L.contiguous(), trans=True)
+    return Xt.mT
+
+
+def trsm_upper(U, B):
+    """Solve U X = B with U upper (primitives.py:172-179): U = L^T, X^T = B^T L^-1."""
+    Xt = B.mT.contiguous()
+    trsm_right_(Xt, U.mT.contiguous(), trans=False)
+    return Xt.mT
+
+
+def axpby_(alpha, x, beta, y):
+    """y <- alpha * x + beta * y in place (batched)."""
+    x3, ldx, sx = _mat3(x)
+    y3, ldy, sy = _mat3(y)
+    b, rows, cols = y3.shape
+    st = N._hip.axpby(N.dtype_code(y.dtype), b, rows, cols, float(alpha), N.ptr(x3), ldx, sx, float(beta), N.ptr(y3),
+                      ldy, sy, N.stream_of(y))
+    N.check(st, 'psvd_axpby')
+    return y
+
+
+def complete_columns_(u, nk):
+    """Complete columns [nk_b, n) of u (b, m, n) in place (primitives.py:430-455); nk is a (b,)
+    int32 device tensor.  Returns the (b,) status tensor (device, no synchronisation)."""
+    b, m, n = u.shape
+    status = torch.empty(b, dtype=torch.int32, device=u.device)
+    wsb = N._hip.complete_workspace_bytes(b, m, n)
+    ws = torch.empty(wsb, dtype=torch.uint8, device=u.device)
+    st = N._hip.complete_columns(N.dtype_code(u.dtype), b, m, n, N.ptr(u), u.stride(-2), u.stride(0) if b > 1 else 0,
+                                 N.ptr(nk), N.ptr(status), N.ptr(ws), wsb, N.stream_of(u))
+    N.check(st, 'psvd_complete_columns')
+    return status
+
+
+def complete_basis(q, size):
+    """Extend orthonormal columns q (b, m, k) to `size` columns (primitives.py:458-467)."""
+    b, m, k = q.shape
+    if size == k:
+        return q
+    if not k <= size <= m:
+        raise ValueError('invalid basis size')
+    out = torch.zeros(b, m, size, dtype=q.dtype, device=q.device)
+    out[:, :, :k] = q
+    nk = torch.full((b,), k, dtype=torch.int32, device=q.device)
+    status = complete_columns_(out, nk)
+    if bool((status != 0).any()):
+        raise RuntimeError('null-space completion failed: supplied live columns are not independent')
+    return out

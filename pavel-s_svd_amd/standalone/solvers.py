@@ -12,6 +12,8 @@ Extension options (no reference counterpart):
                                 under which the Jacobi test stops enforcing orthogonality
     lanes, threads              Jacobi launch shape (see ``include/pavelsvd.h``)
 """
+import math
+
 import torch
 
 from . import primitives as P
@@ -28,7 +30,8 @@ ALGORITHMS = (
     'gram_jacobi', 'gram_dc', 'polar_qdwh', 'polar_newton_schulz',
     'polar_halley', 'polar_polynomial', 'randomized', 'subspace', 'block_krylov', 'lanczos',
 )
-SUPPORTED = ('jacobi', 'qr', 'tsqr')
+SUPPORTED = ('jacobi', 'qr', 'tsqr', 'polar_qdwh')
+EPS = {torch.float32: 1.1920928955078125e-07, torch.float64: 2.220446049250313e-16}
 
 
 def _jacobi(x, stats, opts, scale):
@@ -105,6 +108,95 @@ def _tsqr(x, rows, stats, opts):
     return torch.stack(qs), torch.stack(rs)
 
 
+def qdwh_weights(lower):
+    """Dynamically weighted Halley coefficients (a, b, c) for the current lower bound l on
+    sigma_min / sigma_max and the bound after the step (solvers.py:143-149; Nakatsukasa, Bai and
+    Gygi, SIAM J. Matrix Anal. Appl. 31(5), 2010): d = (4 (1 - l^2) / l^4)^(1/3),
+    a = sqrt(1 + d) + sqrt(8 - 4 d + 8 (2 - l^2) / (l^2 sqrt(1 + d))) / 2, b = (a - 1)^2 / 4,
+    c = a + b - 1, l' = min(1, l (a + b l^2) / (1 + c l^2))."""
+    l2 = lower * lower
+    d = (4.0 * (1.0 - l2) / (l2 * l2)) ** (1.0 / 3.0)
+    sd = math.sqrt(1.0 + d)
+    a = sd + 0.5 * math.sqrt(max(8.0 - 4.0 * d + 8.0 * (2.0 - l2) / (l2 * sd), 0.0))
+    b = (a - 1.0) ** 2 / 4.0
+    c = a + b - 1.0
+    return a, b, c, min(1.0, lower * (a + b * l2) / (1.0 + c * l2))
+
+
+def _eye(b, n, dtype, device):
+    return torch.eye(n, dtype=dtype, device=device).expand(b, n, n).contiguous()
+
+
+def _polar(a, stats, opts):
+    """QDWH polar decomposition, then the Jacobi eigensolve of H = sym(X^T A)
+    (solvers.py:122-177) on the GPU kernels: the QR branch stacks [sqrt(c) X; I] for the blocked
+    Householder QR and folds X' = (b/c) X + (a - b/c)/sqrt(c) Q_top Q_bot^T into one GEMM; the
+    Cholesky branch forms Z = I + c X^T X with one GEMM, factors it and applies X Z^-1 as two
+    right-sided triangular solves.  One host synchronisation per iteration reads the convergence
+    test max|X' - X| <= tol * max(max|X'|, 1) (over the whole batch, as the reference)."""
+    b, m, n = a.shape
+    dt, dev = a.dtype, a.device
+    eps = EPS[dt]
+    mode = _gmode(a, opts)
+    qmode = mode if mode != 'f64' else 'f32'
+    x = P.scale(a, P.reduce(a, 'fro'), invert=True)
+    tol = opts.get('tol') or 10 * eps
+    lower = opts.get('lower_bound', eps)
+    if not 0 < lower <= 1:
+        raise ValueError('lower_bound must be in (0,1]')
+    max_iter = opts.get('max_iter', 40)
+    converged = False
+    it = 0
+    for it in range(max_iter):
+        aa, bb, cc, lower = qdwh_weights(lower)
+        if cc > 100:
+            stacked = torch.empty(b, m + n, n, dtype=dt, device=dev)
+            P.scale(x, None, s=math.sqrt(cc), out=stacked[:, :m])
+            stacked[:, m:] = torch.eye(n, dtype=dt, device=dev)
+            q, _ = P.qr(stacked, gemm=qmode)
+            xn = x.clone()
+            gemm(q[:, :m], q[:, m:].mT, alpha=(aa - bb / cc) / math.sqrt(cc), beta=bb / cc, out=xn, mode=mode)
+        else:
+            z = _eye(b, n, dt, dev)
+            gemm(x.mT, x, alpha=cc, beta=1.0, out=z, mode=mode)
+            P.symmetrize_(z)
+            chol, fail = P.cholesky(z, gemm=qmode)
+            if bool(fail.any()):
+                raise RuntimeError('polar Cholesky lost positive definiteness')
+            w = x.clone()
+            P.trsm_right_(w, chol, trans=True, gemm=qmode)
+            P.trsm_right_(w, chol, trans=False, gemm=qmode)
+            xn = P.axpby_(aa - bb / cc, w, bb / cc, x.clone())
+        test = P.reduce(xn, 'maxdiff', x).amax(0).cpu()   # [max|x' - x|, max|x'|] over the batch
+        x = xn
+        if float(test[0]) <= tol * max(float(test[1]), 1.0):
+            converged = True
+            break
+    if not converged:
+        raise RuntimeError(f'polar_qdwh did not converge in {max_iter} iterations')
+    h = gemm(x.mT, a, mode=mode)
+    P.symmetrize_(h)
+    w, v, info = P.jacobi_eigh_raw(h, max_sweeps=opts.get('max_sweeps', 60))
+    info_h = info.cpu()
+    from .. import native as N
+    N.raise_for_info(info_h, f"Hermitian Jacobi did not converge in {opts.get('max_sweeps', 60)} sweeps")
+    s = w.flip(-1).clamp_min(0)
+    v = v.flip(-1).contiguous()
+    cut_factor = eps if opts.get('cut', 'reference') == 'floor' else eps * max(m, n)
+    keep = s > cut_factor * s[:, :1]
+    u = gemm(x, v, mode=mode)
+    nk = keep.sum(-1).to(torch.int32)
+    if not bool(keep.all()):
+        status = P.complete_columns_(u, nk)
+        if bool((status != 0).any()):
+            raise RuntimeError('null-space completion failed: supplied live columns are not independent')
+    s = torch.where(keep, s, torch.zeros_like(s))
+    fp = gemm(x, gemm(x.mT, x, mode=mode), mode=mode)
+    stats.update(polar_iterations=it + 1, eigen_sweeps=int(info_h[:, 0].max()),
+                 polar_fixed_point_error=float(P.reduce(fp, 'maxdiff', x)[:, 0].max()))
+    return u, s, v
+
+
 def _raw(x, method, stats, opts):
     """Wide inputs are solved transposed (solvers.py:226-228); dispatch by name."""
     if x.shape[-2] < x.shape[-1]:
@@ -122,6 +214,9 @@ def _raw(x, method, stats, opts):
             q, r = _tsqr(x, opts.get('tile_rows', 4 * n), stats, opts)
         u, s, vh = _jacobi(r, stats, opts, scale=False)
         return (None if values else gemm(q, u, mode=mode)), s, vh
+    if method == 'polar_qdwh':
+        u, s, v = _polar(x, stats, opts)
+        return u, s, v.mT
     raise Unsupported(f'{method!r} is not built for MI355X yet')
 
 
@@ -159,8 +254,6 @@ def svd(a, algorithm='jacobi', *, full_matrices=False, values_only=False, return
         raise Unsupported('complex dtypes are not built for MI355X yet')
     if a.device.type != 'cuda':
         raise Unsupported('the MI355X solvers take ROCm device tensors; there is no CPU path')
-    if full_matrices:
-        raise Unsupported('full_matrices completion is not built for MI355X yet')
     stats = {'algorithm': algorithm, 'vendor_factorizations': 0, 'converged': True}
     m, n = x.shape[-2:]
     with torch.no_grad():
@@ -175,7 +268,10 @@ def svd(a, algorithm='jacobi', *, full_matrices=False, values_only=False, return
         u, s, vh = _raw(x, algorithm, stats, inner)
         if scale is not None:
             s = P.scale(s.unsqueeze(1), scale).squeeze(1)
-        if not values_only:
+        if full_matrices and not values_only:
+            u = P.complete_basis(u.contiguous(), a.shape[-2])
+            vh = P.complete_basis(vh.mT.contiguous(), a.shape[-1]).mT
+        elif not values_only:
             u, vh = u[:, :, :rank], vh[:, :rank, :]
         s = s[:, :rank]
         outs = (s,) if values_only else (u, s, vh)
