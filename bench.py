@@ -23,6 +23,7 @@ sys.path.insert(0, ROOT)
 PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 (vector and matrix peaks are equal on gfx950)
 PEAK_HBM_GBS = 8000.0
 COUNT = 4096
+COPY_BYTES = 256 << 20
 SEED = 20261011
 
 
@@ -93,11 +94,22 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-check', action='store_true')
     ap.add_argument('--no-tune', action='store_true', help='use the default launch-shape policy')
+    ap.add_argument('--copy-calibration', action='store_true',
+                    help='only run a plain 256 MiB device-to-device copy 5 times (the reference kernel for '
+                         'scaling FETCH_SIZE / WRITE_SIZE counter readings to bytes) and exit')
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) != 1:
         print('bench.py measures one GPU: every workload here runs on a single MI355X '
               '(multi-GPU sharding is a design note, not a measured path).', file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        x = torch.randn(COPY_BYTES // 4, device='cuda:0')
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({'copy_bytes_read': COPY_BYTES, 'copy_bytes_written': COPY_BYTES}), flush=True)
+        return
     pkg = importlib.import_module('pavel-s_svd_amd')
     B = importlib.import_module('pavel-s_svd_amd.standalone.batched')
     T = importlib.import_module('pavel-s_svd_amd.autotune')
