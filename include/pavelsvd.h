@@ -69,7 +69,7 @@ const char *psvd_version(void);
  * permutation of [0, batch) giving the processing order (longest first balances the queue).
  * max_m / max_n bound the shapes (they size the LDS).  tol <= 0 selects 10*eps(dtype).
  * lanes = lanes per column pair: 2..64 (power of two; 1 is raised to 2), or -k (k in 1..64) for
- * about k rows per lane of the largest matrix (0 means -4).  threads = 64, 128, 256, 512 or 1024.
+ * about k rows per lane of the largest matrix (0 means -4).  threads = 64, 128, 256, 512 or 1024 (0 selects 256).
  * One launch uses one lane count; callers with very uneven batches split them into size
  * classes, one call each (the LDS budget is also sized per call from max_m, max_n).
  * ------------------------------------------------------------------------------------- */
