@@ -110,6 +110,13 @@ int psvd_scale(int dtype, int64_t batch, int rows, int cols, const void *x, int6
 int psvd_symmetrize(int dtype, int64_t batch, int n, void *a, int64_t lda, int64_t sa, psvd_stream_t stream);
 int psvd_axpby(int dtype, int64_t batch, int rows, int cols, double alpha, const void *x, int64_t ldx, int64_t sx,
                double beta, void *y, int64_t ldy, int64_t sy, psvd_stream_t stream);
+/* psvd_sort_desc: perm_b = descending order of keys_b (n FP64 values, ties by index), then
+ * s_b[c] = max(keys_b[perm_b[c]], 0) and y_b[:, c] = x_b[:, perm_b[c]] (rows x n; rows may be 0;
+ * y must not alias x).  Orders the singular values of the FP64 polishing step and their vectors
+ * (no reference counterpart; the reference flips an ascending eigensolve, solvers.py:171);
+ * perm: batch * n int32 workspace. */
+int psvd_sort_desc(int dtype, int64_t batch, int rows, int n, const double *keys, void *s, const void *x, int64_t ldx,
+                   int64_t sx, void *y, int64_t ldy, int64_t sy, int32_t *perm, psvd_stream_t stream);
 
 /* ---------------------------------------------------------------------------------------
  * Completion of missing basis columns (primitives.py:430-467): for each b, columns [nk[b], n) of
@@ -138,12 +145,26 @@ int psvd_qr(int dtype, int gemm_mode, int64_t batch, int m, int n, const void *a
  * Two-sided cyclic Jacobi eigensolver of a batch of symmetric n x n matrices (primitives.py:
  * 370-427): w (batch, n) ascending, v (batch, n, n) row-major eigenvectors as columns,
  * info = [sweeps, status].  Uses an LDS-resident kernel (one workgroup per matrix) when the
- * matrix fits, otherwise one cooperative launch per matrix with the workspace of
- * psvd_eigh_workspace_bytes.  tol <= 0 selects 10*eps(dtype).
+ * matrix fits, otherwise two kernels per round on G and V^T in the workspace of
+ * psvd_eigh_workspace_bytes, with one host read of the activity flag per sweep (not capturable).
+ * The rotation order is the reference's, so sweep counts track it.  tol <= 0 selects 10*eps(dtype).
  * ------------------------------------------------------------------------------------- */
 size_t psvd_eigh_workspace_bytes(int dtype, int n);
 int psvd_eigh(int dtype, int64_t batch, int n, const void *h, void *w, void *v, int32_t *info, double tol,
               int max_sweeps, void *workspace, size_t workspace_bytes, psvd_stream_t stream);
+
+/* ---------------------------------------------------------------------------------------
+ * Blocked two-sided Jacobi eigensolver for large symmetric matrices; same outputs, tolerance
+ * test and stopping rule as psvd_eigh (primitives.py:370-427: |g_pq| > tol * max|G|, stop after
+ * a sweep without rotations), with the rotations grouped: n is split into 32-index blocks, each
+ * round of a tournament over the blocks sweeps the 64 x 64 subproblem of every block pair in LDS
+ * (`inner_sweeps` cyclic sweeps, >= 1) and applies the accumulated 64 x 64 rotations to the rest
+ * of G and to V with MFMA products.  info[0] counts block sweeps.  One host read per sweep.
+ * ------------------------------------------------------------------------------------- */
+size_t psvd_eigh_blocked_workspace_bytes(int dtype, int n);
+int psvd_eigh_blocked(int dtype, int64_t batch, int n, const void *h, void *w, void *v, int32_t *info, double tol,
+                      int max_sweeps, int inner_sweeps, void *workspace, size_t workspace_bytes,
+                      psvd_stream_t stream);
 
 /* ---------------------------------------------------------------------------------------
  * Blocked Cholesky A = L L^T in place (lower; strict upper cleared) with the reference's failure

@@ -156,6 +156,28 @@ static PyObject *py_eigh(PyObject *, PyObject *args) {
   return PyLong_FromLong(st);
 }
 
+static PyObject *py_eigh_blocked_workspace_bytes(PyObject *, PyObject *args) {
+  int dtype, n;
+  if (!PyArg_ParseTuple(args, "ii", &dtype, &n)) return nullptr;
+  return PyLong_FromSize_t(psvd_eigh_blocked_workspace_bytes(dtype, n));
+}
+
+static PyObject *py_eigh_blocked(PyObject *, PyObject *args) {
+  int dtype, n, max_sweeps, inner;
+  long long batch;
+  unsigned long long h, w, v, info, ws, ws_bytes, stream;
+  double tol;
+  if (!PyArg_ParseTuple(args, "iLiKKKKdiiKKK", &dtype, &batch, &n, &h, &w, &v, &info, &tol, &max_sweeps, &inner, &ws,
+                        &ws_bytes, &stream))
+    return nullptr;
+  int st;
+  Py_BEGIN_ALLOW_THREADS
+  st = psvd_eigh_blocked(dtype, batch, n, PTR(h), PTR(w), PTR(v), static_cast<int32_t *>(PTR(info)), tol, max_sweeps,
+                         inner, PTR(ws), (size_t)ws_bytes, PTR(stream));
+  Py_END_ALLOW_THREADS
+  return PyLong_FromLong(st);
+}
+
 static PyObject *py_cholesky(PyObject *, PyObject *args) {
   int dtype, gmode, n;
   long long batch, lda, sa;
@@ -193,6 +215,17 @@ static PyObject *py_axpby(PyObject *, PyObject *args) {
   return PyLong_FromLong(psvd_axpby(dtype, batch, rows, cols, alpha, PTR(x), ldx, sx, beta, PTR(y), ldy, sy, PTR(stream)));
 }
 
+static PyObject *py_sort_desc(PyObject *, PyObject *args) {
+  int dtype, rows, n;
+  long long batch, ldx, sx, ldy, sy;
+  unsigned long long keys, s, x, y, perm, stream;
+  if (!PyArg_ParseTuple(args, "iLiiKKKLLKLLKK", &dtype, &batch, &rows, &n, &keys, &s, &x, &ldx, &sx, &y, &ldy, &sy,
+                        &perm, &stream))
+    return nullptr;
+  return PyLong_FromLong(psvd_sort_desc(dtype, batch, rows, n, static_cast<const double *>(PTR(keys)), PTR(s), PTR(x),
+                                        ldx, sx, PTR(y), ldy, sy, static_cast<int32_t *>(PTR(perm)), PTR(stream)));
+}
+
 static PyObject *py_complete_workspace_bytes(PyObject *, PyObject *args) {
   long long batch;
   int m, n;
@@ -225,12 +258,15 @@ static PyMethodDef kMethods[] = {
     {"gemm", py_gemm, METH_VARARGS, nullptr},
     {"eigh_workspace_bytes", py_eigh_workspace_bytes, METH_VARARGS, nullptr},
     {"eigh", py_eigh, METH_VARARGS, nullptr},
+    {"eigh_blocked_workspace_bytes", py_eigh_blocked_workspace_bytes, METH_VARARGS, nullptr},
+    {"eigh_blocked", py_eigh_blocked, METH_VARARGS, nullptr},
     {"cholesky", py_cholesky, METH_VARARGS, nullptr},
     {"trsm_right", py_trsm_right, METH_VARARGS, nullptr},
     {"reduce", py_reduce, METH_VARARGS, nullptr},
     {"scale", py_scale, METH_VARARGS, nullptr},
     {"symmetrize", py_symmetrize, METH_VARARGS, nullptr},
     {"axpby", py_axpby, METH_VARARGS, nullptr},
+    {"sort_desc", py_sort_desc, METH_VARARGS, nullptr},
     {"complete_workspace_bytes", py_complete_workspace_bytes, METH_VARARGS, nullptr},
     {"complete_columns", py_complete_columns, METH_VARARGS, nullptr},
     {"qr_workspace_bytes", py_qr_workspace_bytes, METH_VARARGS, nullptr},

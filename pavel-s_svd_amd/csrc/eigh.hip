@@ -414,6 +414,390 @@ static int run_eigh(int64_t batch, int n, const T *h, T *w, T *v, int32_t *info,
   return PSVD_OK;
 }
 
+// ---- blocked two-sided Jacobi for matrices beyond LDS --------------------------------------
+// Indices are grouped in blocks of kEb = 32 (n padded to a multiple of 64 with decoupled zero
+// rows and columns that never rotate); round r of the tournament over the nb blocks pairs block
+// I with block J, and the 64 x 64 subproblem G[I u J, I u J] is swept in LDS by `inner_sweeps`
+// cyclic sweeps of the scalar path's plane rotations and test (|g_pq| > tol * max|G|, max|G| of
+// the whole matrix at the start of the round, primitives.py:397-399).  The subproblem's
+// accumulated rotation Q_p is then applied to the rest of G -- block (p, q) <- Q_p^T B Q_q and
+// its mirror -- and to the rows of V^T, as 64 x 64 x 64 MFMA products.  A sweep in which no
+// subproblem rotates leaves G unchanged, so every off-diagonal |g_ij| <= tol * max|G| at the end:
+// the reference's stopping rule (primitives.py:418).  max|G| is kept per 32 x 32 block so that
+// blocks no rotation touches still contribute to the next round's threshold without being read.
+
+constexpr int kEb = 32, kEs = 2 * kEb, kEsLd = kEs + 1;
+
+typedef float e4f __attribute__((ext_vector_type(4)));
+typedef double e4d __attribute__((ext_vector_type(4)));
+
+template <typename T> struct Mfma;
+template <> struct Mfma<float> {
+  using V = e4f;
+  static __device__ __forceinline__ V op(float a, float b, V c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+  // D register r of lane l holds row 4 * (l / 16) + r, column l % 16
+  static __device__ __forceinline__ int row(int r, int lk) { return 4 * lk + r; }
+};
+template <> struct Mfma<double> {
+  using V = e4d;
+  static __device__ __forceinline__ V op(double a, double b, V c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+  // D register r of lane l holds row 4 * r + l / 16, column l % 16
+  static __device__ __forceinline__ int row(int r, int lk) { return 4 * r + lk; }
+};
+
+template <typename T>
+struct BlkParams {
+  const T *h;
+  int n, N, nb, inner_sweeps;
+  double tol;
+  T *G, *Vt, *Q;               // G, V^T: N x N (ld N); Q: nb/2 matrices Q_p^T, 64 x 64 row-major
+  int *act;                    // nb/2: subproblem p rotated in this round
+  double *bm;                  // nb x nb: max |G| of each 32 x 32 block
+  unsigned long long *slot;    // 3 rotating max|G| slots (bit patterns of non-negative doubles)
+  int *flag;                   // any rotation in the current sweep
+};
+
+__device__ __forceinline__ int blk_row(int I, int J, int i) { return i < kEb ? I * kEb + i : J * kEb + (i - kEb); }
+
+__device__ __forceinline__ void slot_max(unsigned long long *s, double v) {
+  atomicMax(s, (unsigned long long)__double_as_longlong(v));
+}
+
+// upper-triangle index e of an np2 x np2 grid (k1 <= k2, row-major) -> (k1, k2)
+__device__ __forceinline__ void tri_decode(int64_t e, int np2, int &k1, int &k2) {
+  int a = (int)((2.0 * np2 + 1.0 - sqrt((2.0 * np2 + 1.0) * (2.0 * np2 + 1.0) - 8.0 * (double)e)) / 2.0);
+  if (a < 0) a = 0;
+  if (a > np2 - 1) a = np2 - 1;
+  while (a > 0 && (int64_t)a * np2 - (int64_t)a * (a - 1) / 2 > e) --a;
+  while (a < np2 - 1 && (int64_t)(a + 1) * np2 - (int64_t)(a + 1) * a / 2 <= e) ++a;
+  k1 = a;
+  k2 = a + (int)(e - ((int64_t)a * np2 - (int64_t)a * (a - 1) / 2));
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void blk_load_kernel(BlkParams<T> P) {
+  __shared__ double red[4];
+  const int I = blockIdx.y, K = blockIdx.x, tid = threadIdx.x;
+  double mx = 0.0;
+  for (int e = tid; e < kEb * kEb; e += 256) {
+    const int i = I * kEb + e / kEb, k = K * kEb + (e & (kEb - 1));
+    const T x = (i < P.n && k < P.n) ? P.h[(int64_t)i * P.n + k] : T(0);
+    P.G[(int64_t)i * P.N + k] = x;
+    P.Vt[(int64_t)i * P.N + k] = i == k ? T(1) : T(0);
+    mx = fmax(mx, fabs((double)x));
+  }
+  mx = block_max<256>(mx, red);
+  if (tid == 0) {
+    P.bm[I * P.nb + K] = mx;
+    slot_max(&P.slot[0], mx);
+  }
+}
+
+__global__ void blk_reset_kernel(unsigned long long *slot, int *flag, int what) {
+  if (threadIdx.x == 0) {
+    if (what & 1) slot[0] = slot[1] = slot[2] = 0ull;
+    if (what & 2) *flag = 0;
+  }
+}
+
+template <typename T>
+__host__ __device__ inline size_t blk_inner_lds() {
+  return sizeof(double) * 2 * kEb + sizeof(int) * 2 * kEb + sizeof(T) * 2 * (size_t)kEs * kEsLd;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void blk_inner_kernel(BlkParams<T> P, int r, int g) {
+  extern __shared__ double smem_bi[];
+  double *cc = smem_bi, *ss = cc + kEb;
+  int *pp = reinterpret_cast<int *>(ss + kEb), *qq = pp + kEb;
+  T *S = reinterpret_cast<T *>(qq + kEb);
+  T *Qt = S + kEs * kEsLd;
+  const int p = blockIdx.x, tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+  int I, J;
+  rr_pair(r, p, P.nb, I, J);
+  if (p == 0 && tid == 0) P.slot[(g + 2) % 3] = 0ull;
+  const double scale = __longlong_as_double((long long)P.slot[g % 3]);
+  const double tol = P.tol > 0.0 ? P.tol : 10.0 * Eps<T>::value;
+  for (int e = tid; e < kEs * kEs; e += 256) {
+    const int i = e / kEs, j = e - i * kEs;
+    S[i * kEsLd + j] = P.G[(int64_t)blk_row(I, J, i) * P.N + blk_row(I, J, j)];
+    Qt[i * kEsLd + j] = i == j ? T(1) : T(0);
+  }
+  __syncthreads();
+  int any = 0;
+  for (int sw = 0; sw < P.inner_sweeps; ++sw) {
+    int act_sw = 0;
+    for (int rr = 0; rr < kEs - 1; ++rr) {
+      if (tid < kEb) {
+        int a, b;
+        rr_pair(rr, tid, kEs, a, b);
+        double c, s, ratio;
+        bool act;
+        pair_params(S, kEsLd, a, b, tol, scale, c, s, act, ratio);
+        cc[tid] = act ? c : 1.0;
+        ss[tid] = act ? s : 0.0;
+        pp[tid] = a;
+        qq[tid] = b;
+        act_sw |= act;
+      }
+      __syncthreads();
+      for (int e = tid; e < kEb * (kEb + 1) / 2; e += 256) {
+        int k1, k2;
+        tri_decode(e, kEb, k1, k2);
+        const double s1 = ss[k1], s2 = ss[k2];
+        if (s1 != 0.0 || s2 != 0.0)
+          block_update(S, kEsLd, pp[k1], qq[k1], pp[k2], qq[k2], cc[k1], s1, cc[k2], s2, s1 != 0.0, s2 != 0.0);
+      }
+      for (int e = tid; e < kEb * kEs; e += 256) {
+        const int k = e / kEs, j = e - k * kEs;
+        const double s = ss[k];
+        if (s == 0.0) continue;
+        const double c = cc[k];
+        T *vp = Qt + pp[k] * kEsLd + j, *vq = Qt + qq[k] * kEsLd + j;
+        const double x = (double)*vp, y = (double)*vq;
+        *vp = (T)(c * x - s * y);
+        *vq = (T)(s * x + c * y);
+      }
+      __syncthreads();
+    }
+    act_sw = __syncthreads_or(act_sw);
+    any |= act_sw;
+    if (!act_sw) break;
+  }
+  if (any) {
+    for (int e = tid; e < kEs * kEs; e += 256) {
+      const int i = e / kEs, j = e - i * kEs;
+      P.G[(int64_t)blk_row(I, J, i) * P.N + blk_row(I, J, j)] = S[i * kEsLd + j];
+      P.Q[(int64_t)p * kEs * kEs + e] = Qt[i * kEsLd + j];
+    }
+  }
+  // maxima of the four 32 x 32 blocks, one per wave
+  const int bi = wave >> 1, bj = wave & 1;
+  double mx = 0.0;
+  for (int e = lane; e < kEb * kEb; e += kWave) {
+    const int i = e / kEb, j = e & (kEb - 1);
+    mx = fmax(mx, fabs((double)S[(bi * kEb + i) * kEsLd + bj * kEb + j]));
+  }
+  mx = wave_max_fast(mx);
+  if (lane == 0) {
+    P.bm[(bi ? J : I) * P.nb + (bj ? J : I)] = mx;
+    slot_max(&P.slot[(g + 1) % 3], mx);
+  }
+  if (tid == 0) {
+    P.act[p] = any;
+    if (any) atomicOr(P.flag, 1);
+  }
+}
+
+// acc = A B on a 64 x 64 x 64 tile from LDS (ld kEsLd); BT: B is given transposed (B[k][j] at
+// Bs[j][k]).  Wave w computes the 32 x 32 quadrant (w / 2, w % 2) as 2 x 2 MFMA tiles.
+template <typename T, bool BT>
+__device__ __forceinline__ void mm64(const T *As, const T *Bs, typename Mfma<T>::V (&acc)[2][2], int wm, int wn,
+                                     int lr, int lk) {
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[i][j][r] = 0;
+#pragma unroll 4
+  for (int k = 0; k < kEs; k += 4) {
+    T a[2], b[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) a[i] = As[(wm + 16 * i + lr) * kEsLd + k + lk];
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+      b[j] = BT ? Bs[(wn + 16 * j + lr) * kEsLd + k + lk] : Bs[(k + lk) * kEsLd + wn + 16 * j + lr];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) acc[i][j] = Mfma<T>::op(a[i], b[j], acc[i][j]);
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void store64(T *Ds, const typename Mfma<T>::V (&acc)[2][2], int wm, int wn, int lr, int lk) {
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) Ds[(wm + 16 * i + Mfma<T>::row(r, lk)) * kEsLd + wn + 16 * j + lr] = (T)acc[i][j][r];
+}
+
+template <typename T>
+__host__ __device__ inline size_t blk_update_lds() {
+  return sizeof(T) * 3 * (size_t)kEs * kEsLd;
+}
+
+// Blocks [0, nG): G block (p, q), p < q, and its mirror.  Blocks [nG, nG + npair * N / 64): the
+// 64 x 64 tile (rows of pair p, column chunk c) of V^T.
+template <typename T>
+__global__ __launch_bounds__(256) void blk_update_kernel(BlkParams<T> P, int r, int g) {
+  extern __shared__ double smem_bu[];
+  T *Bs = reinterpret_cast<T *>(smem_bu);
+  T *Qa = Bs + kEs * kEsLd, *Qb = Qa + kEs * kEsLd;
+  const int npair = P.nb / 2;
+  const int64_t nG = (int64_t)npair * (npair - 1) / 2;
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave, lr = lane & 15, lk = lane >> 4;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  const int64_t bid = blockIdx.x;
+  typename Mfma<T>::V acc[2][2];
+  if (bid < nG) {
+    // strictly upper triangle of the pair grid: e -> (p, q), p < q
+    int p, q;
+    tri_decode(bid, npair - 1, p, q);
+    q += 1;
+    int Ip, Jp, Iq, Jq;
+    rr_pair(r, p, P.nb, Ip, Jp);
+    rr_pair(r, q, P.nb, Iq, Jq);
+    const int ap = P.act[p], aq = P.act[q];
+    if (!ap && !aq) {
+      if (tid < 4) {
+        const int X = (tid >> 1) ? Jp : Ip, Y = (tid & 1) ? Jq : Iq;
+        slot_max(&P.slot[(g + 1) % 3], P.bm[X * P.nb + Y]);
+      }
+      return;
+    }
+    for (int e = tid; e < kEs * kEs; e += 256) {
+      const int i = e / kEs, j = e - i * kEs;
+      Bs[i * kEsLd + j] = P.G[(int64_t)blk_row(Ip, Jp, i) * P.N + blk_row(Iq, Jq, j)];
+      if (ap) Qa[i * kEsLd + j] = P.Q[(int64_t)p * kEs * kEs + e];
+      if (aq) Qb[i * kEsLd + j] = P.Q[(int64_t)q * kEs * kEs + e];
+    }
+    __syncthreads();
+    if (ap) {  // B <- Q_p^T B
+      mm64<T, false>(Qa, Bs, acc, wm, wn, lr, lk);
+      __syncthreads();
+      store64<T>(Bs, acc, wm, wn, lr, lk);
+      __syncthreads();
+    }
+    if (aq) {  // B <- B Q_q
+      mm64<T, true>(Bs, Qb, acc, wm, wn, lr, lk);
+      __syncthreads();
+      store64<T>(Bs, acc, wm, wn, lr, lk);
+      __syncthreads();
+    }
+    for (int e = tid; e < kEs * kEs; e += 256) {
+      const int i = e / kEs, j = e - i * kEs;
+      P.G[(int64_t)blk_row(Ip, Jp, i) * P.N + blk_row(Iq, Jq, j)] = Bs[i * kEsLd + j];
+      P.G[(int64_t)blk_row(Iq, Jq, i) * P.N + blk_row(Ip, Jp, j)] = Bs[j * kEsLd + i];
+    }
+    double mx = 0.0;
+    for (int e = lane; e < 32 * 32; e += kWave) mx = fmax(mx, fabs((double)Bs[(wm + (e >> 5)) * kEsLd + wn + (e & 31)]));
+    mx = wave_max_fast(mx);
+    if (lane == 0) {
+      const int X = wm ? Jp : Ip, Y = wn ? Jq : Iq;
+      P.bm[X * P.nb + Y] = mx;
+      P.bm[Y * P.nb + X] = mx;
+      slot_max(&P.slot[(g + 1) % 3], mx);
+    }
+  } else {
+    const int64_t t = bid - nG, chunks = P.N / kEs;
+    const int p = (int)(t / chunks), c0 = (int)(t - (int64_t)p * chunks) * kEs;
+    if (!P.act[p]) return;
+    int Ip, Jp;
+    rr_pair(r, p, P.nb, Ip, Jp);
+    for (int e = tid; e < kEs * kEs; e += 256) {
+      const int i = e / kEs, j = e - i * kEs;
+      Qa[i * kEsLd + j] = P.Q[(int64_t)p * kEs * kEs + e];
+      Bs[i * kEsLd + j] = P.Vt[(int64_t)blk_row(Ip, Jp, i) * P.N + c0 + j];
+    }
+    __syncthreads();
+    mm64<T, false>(Qa, Bs, acc, wm, wn, lr, lk);
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int rg = 0; rg < 4; ++rg)
+          P.Vt[(int64_t)blk_row(Ip, Jp, wm + 16 * i + Mfma<T>::row(rg, lk)) * P.N + c0 + wn + 16 * j + lr] = (T)acc[i][j][rg];
+  }
+}
+
+template <typename T>
+size_t blk_ws_bytes(int n) {
+  const int N = (n + kEs - 1) / kEs * kEs, nb = N / kEb, npair = nb / 2;
+  auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+  return 2 * al(sizeof(T) * (size_t)N * N) + al(sizeof(T) * (size_t)npair * kEs * kEs) + al(sizeof(int) * npair) +
+         al(sizeof(double) * (size_t)nb * nb) + al(8 * 3) + al(4 * 3) + al(sizeof(double) * N) + al(sizeof(int) * N);
+}
+
+template <typename T>
+static int run_eigh_blocked(int64_t batch, int n, const T *h, T *w, T *v, int32_t *info, double tol, int max_sweeps,
+                            int inner_sweeps, void *ws, size_t ws_bytes, hipStream_t st) {
+  if (ws_bytes < blk_ws_bytes<T>(n)) return PSVD_EINVAL;
+  static bool attr = false;
+  if (!attr) {
+    if (hipFuncSetAttribute((const void *)blk_inner_kernel<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)blk_inner_lds<T>()) != hipSuccess ||
+        hipFuncSetAttribute((const void *)blk_update_kernel<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)blk_update_lds<T>()) != hipSuccess)
+      return PSVD_ELAUNCH;
+    attr = true;
+  }
+  BlkParams<T> P{};
+  P.n = n;
+  P.N = (n + kEs - 1) / kEs * kEs;
+  P.nb = P.N / kEb;
+  P.inner_sweeps = inner_sweeps;
+  P.tol = tol;
+  const int npair = P.nb / 2;
+  char *cur = static_cast<char *>(ws);
+  auto take = [&](size_t bytes) {
+    char *q = cur;
+    cur += (bytes + 255) & ~size_t(255);
+    return q;
+  };
+  P.G = reinterpret_cast<T *>(take(sizeof(T) * (size_t)P.N * P.N));
+  P.Vt = reinterpret_cast<T *>(take(sizeof(T) * (size_t)P.N * P.N));
+  P.Q = reinterpret_cast<T *>(take(sizeof(T) * (size_t)npair * kEs * kEs));
+  P.act = reinterpret_cast<int *>(take(sizeof(int) * npair));
+  P.bm = reinterpret_cast<double *>(take(sizeof(double) * (size_t)P.nb * P.nb));
+  P.slot = reinterpret_cast<unsigned long long *>(take(8 * 3));
+  P.flag = reinterpret_cast<int *>(take(4 * 3));
+  double *dw = reinterpret_cast<double *>(take(sizeof(double) * P.N));
+  int *ord = reinterpret_cast<int *>(take(sizeof(int) * P.N));
+  const int64_t nG = (int64_t)npair * (npair - 1) / 2;
+  const unsigned gupd = (unsigned)(nG + (int64_t)npair * (P.N / kEs));
+  auto blocks = [](int64_t items) { return (unsigned)((items + kEighThreads - 1) / kEighThreads); };
+  for (int64_t b = 0; b < batch; ++b) {
+    P.h = h + b * (int64_t)n * n;
+    hipLaunchKernelGGL(blk_reset_kernel, dim3(1), dim3(64), 0, st, P.slot, P.flag, 3);
+    hipLaunchKernelGGL(blk_load_kernel<T>, dim3(P.nb, P.nb), dim3(256), 0, st, P);
+    int sweeps = 0, g = 0;
+    for (int sw = 1; sw <= max_sweeps; ++sw) {
+      sweeps = sw;
+      hipLaunchKernelGGL(blk_reset_kernel, dim3(1), dim3(64), 0, st, P.slot, P.flag, 2);
+      for (int r = 0; r < P.nb - 1; ++r, ++g) {
+        hipLaunchKernelGGL(blk_inner_kernel<T>, dim3(npair), dim3(256), blk_inner_lds<T>(), st, P, r, g);
+        hipLaunchKernelGGL(blk_update_kernel<T>, dim3(gupd), dim3(256), blk_update_lds<T>(), st, P, r, g);
+      }
+      if (hipGetLastError() != hipSuccess) return PSVD_ELAUNCH;
+      int active = 0;
+      if (hipMemcpyAsync(&active, P.flag, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess) return PSVD_ELAUNCH;
+      if (hipStreamSynchronize(st) != hipSuccess) return PSVD_ELAUNCH;
+      if (!active) break;
+    }
+    EighParams<T> E{};
+    E.n = n;
+    E.G = P.G;
+    E.Vt = P.Vt;
+    E.flag = P.flag;
+    E.dw = dw;
+    E.ord = ord;
+    E.w = w + b * (int64_t)n;
+    E.v = v + b * (int64_t)n * n;
+    E.info = info + 2 * b;
+    hipLaunchKernelGGL(eigh_diag_kernel<T>, dim3(blocks(n)), dim3(kEighThreads), 0, st, E);
+    hipLaunchKernelGGL(eigh_sort_kernel<T>, dim3(blocks(n)), dim3(kEighThreads), 0, st, E);
+    hipLaunchKernelGGL(eigh_out_kernel<T>, dim3(blocks((int64_t)n * n) < 4096 ? blocks((int64_t)n * n) : 4096),
+                       dim3(kEighThreads), 0, st, E, sweeps);
+    if (hipGetLastError() != hipSuccess) return PSVD_ELAUNCH;
+  }
+  return PSVD_OK;
+}
+
 }  // namespace psvd
 
 extern "C" {
@@ -436,6 +820,32 @@ int psvd_eigh(int dtype, int64_t batch, int n, const void *h, void *w, void *v, 
   if (dtype == PSVD_F64)
     return psvd::run_eigh<double>(batch, n, static_cast<const double *>(h), static_cast<double *>(w),
                                   static_cast<double *>(v), info, tol, max_sweeps, workspace, workspace_bytes, st);
+  return PSVD_EINVAL;
+}
+
+size_t psvd_eigh_blocked_workspace_bytes(int dtype, int n) {
+  if (n < 1) return 0;
+  if (dtype == PSVD_F32) return psvd::blk_ws_bytes<float>(n);
+  if (dtype == PSVD_F64) return psvd::blk_ws_bytes<double>(n);
+  return 0;
+}
+
+int psvd_eigh_blocked(int dtype, int64_t batch, int n, const void *h, void *w, void *v, int32_t *info, double tol,
+                      int max_sweeps, int inner_sweeps, void *workspace, size_t workspace_bytes,
+                      psvd_stream_t stream) {
+  if (batch < 0 || n < 1 || max_sweeps < 1 || inner_sweeps < 1 || !h || !w || !v || !info || !workspace ||
+      n > 65536)
+    return PSVD_EINVAL;
+  if (batch == 0) return PSVD_OK;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (dtype == PSVD_F32)
+    return psvd::run_eigh_blocked<float>(batch, n, static_cast<const float *>(h), static_cast<float *>(w),
+                                         static_cast<float *>(v), info, tol, max_sweeps, inner_sweeps, workspace,
+                                         workspace_bytes, st);
+  if (dtype == PSVD_F64)
+    return psvd::run_eigh_blocked<double>(batch, n, static_cast<const double *>(h), static_cast<double *>(w),
+                                          static_cast<double *>(v), info, tol, max_sweeps, inner_sweeps, workspace,
+                                          workspace_bytes, st);
   return PSVD_EINVAL;
 }
 

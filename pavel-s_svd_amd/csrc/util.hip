@@ -1,7 +1,8 @@
 // Batched element-wise and reduction kernels used between the solver stages: per-matrix maximum
 // of |a| and Frobenius norm (the two input normalisations of solvers.py:294-295 and :124-125),
 // the QDWH convergence pair max|x' - x|, max|x'| (solvers.py:161-163), scaling by a per-matrix
-// device scalar, symmetrisation (H + H^T)/2 (solvers.py:156, 169) and a + b axpby updates.
+// device scalar, symmetrisation (H + H^T)/2 (solvers.py:156, 169), axpby updates and a descending sort
+// of FP64 keys with the matching column permutation.
 //
 // Reductions run over a 2-D grid (chunks of one matrix x batch) and combine chunk results with
 // device atomics on the bit patterns of non-negative doubles (monotone as integers) or double
@@ -135,6 +136,37 @@ static int run_reduce(int op, int64_t batch, int rows, int cols, const void *x, 
   return hipGetLastError() == hipSuccess ? PSVD_OK : PSVD_ELAUNCH;
 }
 
+
+// Descending order of per-matrix FP64 keys (ties by index, the stable order of a flip of an
+// ascending sort) by rank counting, then y[:, c] = x[:, perm[c]] and s[c] = max(keys[perm[c]], 0).
+__global__ __launch_bounds__(kUtilThreads) void order_desc_kernel(const double *keys, int n, int32_t *perm) {
+  const int64_t b = blockIdx.y;
+  const int j = blockIdx.x * kUtilThreads + threadIdx.x;
+  if (j >= n) return;
+  const double *k = keys + b * n;
+  const double x = k[j];
+  int rank = 0;
+  for (int i = 0; i < n; ++i) {
+    const double y = k[i];
+    rank += (y > x) || (y == x && i < j);
+  }
+  perm[b * n + rank] = j;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kUtilThreads) void gather_cols_kernel(const double *keys, const int32_t *perm, int rows,
+                                                                   int n, const T *x, int64_t ldx, int64_t sx, T *y,
+                                                                   int64_t ldy, int64_t sy, T *s) {
+  const int64_t b = blockIdx.y;
+  const int64_t e = (int64_t)blockIdx.x * kUtilThreads + threadIdx.x;
+  if (e >= (int64_t)(rows > 0 ? rows : 1) * n) return;
+  const int r = (int)(e / n), c = (int)(e - (int64_t)r * n);
+  int src = perm[b * n + c];
+  src = src < 0 ? 0 : (src >= n ? n - 1 : src);   // NaN keys leave rank gaps; never index outside x
+  if (rows > 0) y[b * sy + (int64_t)r * ldy + c] = x[b * sx + (int64_t)r * ldx + src];
+  if (r == 0) s[b * n + c] = (T)fmax(keys[b * n + src], 0.0);
+}
+
 }  // namespace psvd
 
 extern "C" {
@@ -196,6 +228,31 @@ int psvd_symmetrize(int dtype, int64_t batch, int n, void *a, int64_t lda, int64
   else if (dtype == PSVD_F64)
     hipLaunchKernelGGL(psvd::symmetrize_kernel<double>, grid, dim3(psvd::kUtilThreads), 0, st,
                        static_cast<double *>(a), lda, sa, n);
+  else
+    return PSVD_EINVAL;
+  return hipGetLastError() == hipSuccess ? PSVD_OK : PSVD_ELAUNCH;
+}
+
+int psvd_sort_desc(int dtype, int64_t batch, int rows, int n, const double *keys, void *s, const void *x, int64_t ldx,
+                   int64_t sx, void *y, int64_t ldy, int64_t sy, int32_t *perm, psvd_stream_t stream) {
+  if (batch < 0 || rows < 0 || n < 0 || !keys || !s || !perm || ldx < n || ldy < n || batch > 65535 ||
+      (rows > 0 && (!x || !y)))
+    return PSVD_EINVAL;
+  if (batch == 0 || n == 0) return PSVD_OK;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(psvd::order_desc_kernel, dim3((unsigned)((n + psvd::kUtilThreads - 1) / psvd::kUtilThreads),
+                                                    (unsigned)batch),
+                     dim3(psvd::kUtilThreads), 0, st, keys, n, perm);
+  const int r = rows > 0 ? rows : 1;
+  const dim3 grid((unsigned)(((int64_t)r * n + psvd::kUtilThreads - 1) / psvd::kUtilThreads), (unsigned)batch);
+  if (dtype == PSVD_F32)
+    hipLaunchKernelGGL(psvd::gather_cols_kernel<float>, grid, dim3(psvd::kUtilThreads), 0, st, keys, perm, rows, n,
+                       static_cast<const float *>(x), ldx, sx, static_cast<float *>(y), ldy, sy,
+                       static_cast<float *>(s));
+  else if (dtype == PSVD_F64)
+    hipLaunchKernelGGL(psvd::gather_cols_kernel<double>, grid, dim3(psvd::kUtilThreads), 0, st, keys, perm, rows, n,
+                       static_cast<const double *>(x), ldx, sx, static_cast<double *>(y), ldy, sy,
+                       static_cast<double *>(s));
   else
     return PSVD_EINVAL;
   return hipGetLastError() == hipSuccess ? PSVD_OK : PSVD_ELAUNCH;

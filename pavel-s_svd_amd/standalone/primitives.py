@@ -170,26 +170,52 @@ def symmetrize_(a):
     return a
 
 
-def jacobi_eigh_raw(H, tol=None, max_sweeps=30):
-    """Two-sided Jacobi of the symmetric batch H (b, n, n): w ascending, V, info (device)."""
+EIGH_LDS_CAP = 160 * 1024 - 4096
+
+
+def eigh_lds_fits(dtype, n):
+    """Whether the LDS-resident eigensolver holds an n x n matrix (G, V^T and the round tables)."""
+    size = 4 if dtype == torch.float32 else 8
+    np_ = n + (n & 1)
+    b = size * 2 * np_ * (np_ + 1) + 16 * (np_ // 2) + 8 * (np_ // 2) + 12 * np_
+    return ((b + 15) & ~15) <= EIGH_LDS_CAP
+
+
+def jacobi_eigh_raw(H, tol=None, max_sweeps=30, *, method='auto', inner_sweeps=1):
+    """Two-sided Jacobi of the symmetric batch H (b, n, n): w ascending, V, info (device).
+
+    method 'cyclic' keeps the reference's rotation order (LDS kernel, or the per-round HBM path
+    for large n); 'blocked' groups rotations into 64 x 64 subproblems applied with MFMA; 'auto'
+    takes the LDS kernel when the matrix fits and the blocked path otherwise."""
     b, n, _ = H.shape
     H = H.contiguous()
     code = N.dtype_code(H.dtype)
+    if method == 'auto':
+        method = 'cyclic' if eigh_lds_fits(H.dtype, n) else 'blocked'
+    if method not in ('cyclic', 'blocked'):
+        raise ValueError(f'unknown eigensolver method {method!r}')
     w = torch.empty(b, n, dtype=H.dtype, device=H.device)
     v = torch.empty(b, n, n, dtype=H.dtype, device=H.device)
     info = torch.empty(b, 2, dtype=torch.int32, device=H.device)
-    wsb = N._hip.eigh_workspace_bytes(code, n)
-    ws = torch.empty(wsb, dtype=torch.uint8, device=H.device)
-    st = N._hip.eigh(code, b, n, N.ptr(H), N.ptr(w), N.ptr(v), N.ptr(info), float(tol or 0.0), int(max_sweeps),
-                     N.ptr(ws), wsb, N.stream_of(H))
-    N.check(st, 'psvd_eigh')
+    if method == 'cyclic':
+        wsb = N._hip.eigh_workspace_bytes(code, n)
+        ws = torch.empty(wsb, dtype=torch.uint8, device=H.device)
+        st = N._hip.eigh(code, b, n, N.ptr(H), N.ptr(w), N.ptr(v), N.ptr(info), float(tol or 0.0), int(max_sweeps),
+                         N.ptr(ws), wsb, N.stream_of(H))
+        N.check(st, 'psvd_eigh')
+    else:
+        wsb = N._hip.eigh_blocked_workspace_bytes(code, n)
+        ws = torch.empty(wsb, dtype=torch.uint8, device=H.device)
+        st = N._hip.eigh_blocked(code, b, n, N.ptr(H), N.ptr(w), N.ptr(v), N.ptr(info), float(tol or 0.0),
+                                 int(max_sweeps), int(inner_sweeps), N.ptr(ws), wsb, N.stream_of(H))
+        N.check(st, 'psvd_eigh_blocked')
     return w, v, info
 
 
-def jacobi_eigh(H, tol=None, max_sweeps=30):
+def jacobi_eigh(H, tol=None, max_sweeps=30, *, method='auto', inner_sweeps=1):
     """Hermitian (real symmetric) H (b, n, n) -> w (b, n) ascending, V (b, n, n), sweeps
     (primitives.py:370-427).  Raises RuntimeError on non-convergence."""
-    w, v, info = jacobi_eigh_raw(H, tol, max_sweeps)
+    w, v, info = jacobi_eigh_raw(H, tol, max_sweeps, method=method, inner_sweeps=inner_sweeps)
     info = info.cpu()
     N.raise_for_info(info, f'Hermitian Jacobi did not converge in {max_sweeps} sweeps')
     return w, v, int(info[:, 0].max()) if H.shape[0] else 0
@@ -243,6 +269,23 @@ def axpby_(alpha, x, beta, y):
                       ldy, sy, N.stream_of(y))
     N.check(st, 'psvd_axpby')
     return y
+
+
+def sort_desc(keys, x):
+    """keys (b, n) float64, x (b, r, n) -> (s (b, n) in x's dtype = keys sorted descending and
+    clamped at 0, y = x with its columns in the same order)."""
+    x3, ldx, sx = _mat3(x)
+    b, rows, n = x3.shape
+    keys = keys.contiguous()
+    if tuple(keys.shape) != (b, n) or keys.dtype != torch.float64:
+        raise ValueError('sort_desc keys must be (batch, n) float64')
+    s = torch.empty(b, n, dtype=x.dtype, device=x.device)
+    y = torch.empty(b, rows, n, dtype=x.dtype, device=x.device)
+    perm = torch.empty(b, n, dtype=torch.int32, device=x.device)
+    st = N._hip.sort_desc(N.dtype_code(x.dtype), b, rows, n, N.ptr(keys), N.ptr(s), N.ptr(x3), ldx, sx, N.ptr(y),
+                          y.stride(1), y.stride(0), N.ptr(perm), N.stream_of(x))
+    N.check(st, 'psvd_sort_desc')
+    return s, y
 
 
 def complete_columns_(u, nk):

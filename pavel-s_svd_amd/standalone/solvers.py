@@ -11,6 +11,15 @@ Extension options (no reference counterpart):
                                 reference contract, default) or only below eps*sigma_1, the floor
                                 under which the Jacobi test stops enforcing orthogonality
     lanes, threads              Jacobi launch shape (see ``include/pavelsvd.h``)
+    gemm='f32' | 'bf16x3'       arithmetic of the FP32 QR / TSQR products and of the QR-branch QDWH
+                                iterations on MFMA (QDWH's Cholesky-branch steps stay FP32)
+    polish=False                QDWH, FP32 data: FP64 Newton-Schulz step(s) on X (polish_steps,
+                                default 1) and on V, H = sym(X^T A) in FP64, G = V^T H V finished by
+                                FP64 Jacobi sweeps (stat polish_sweeps), sigma = eig(G), U = X V W
+                                in FP64
+    eigh_method='auto'          QDWH eigensolver: 'cyclic' (the reference's rotation order),
+                                'blocked' (64 x 64 subproblems + MFMA) or 'auto' (LDS kernel when
+                                H fits, blocked otherwise)
 """
 import math
 
@@ -123,6 +132,13 @@ def qdwh_weights(lower):
     return a, b, c, min(1.0, lower * (a + b * l2) / (1.0 + c * l2))
 
 
+def _newton_schulz(x):
+    """One Newton-Schulz step toward the orthogonal polar factor, x <- x (3 I - x^T x) / 2, as two
+    FP64 MFMA products (x: (b, m, n) float64, m >= n)."""
+    g = gemm(x.mT, x, mode='f64')
+    return gemm(x, g, alpha=-0.5, beta=1.5, out=x.clone(), mode='f64')
+
+
 def _eye(b, n, dtype, device):
     return torch.eye(n, dtype=dtype, device=device).expand(b, n, n).contiguous()
 
@@ -157,15 +173,19 @@ def _polar(a, stats, opts):
             xn = x.clone()
             gemm(q[:, :m], q[:, m:].mT, alpha=(aa - bb / cc) / math.sqrt(cc), beta=bb / cc, out=xn, mode=mode)
         else:
+            # the Cholesky-branch steps end the iteration: their products stay in full FP32 (FP64
+            # for FP64 data) so that the fixed-point test at tol = 10 eps can be met -- BF16x3
+            # carries about 2^-16 relative precision
+            cmode = 'f32' if mode == 'bf16x3' else mode
             z = _eye(b, n, dt, dev)
-            gemm(x.mT, x, alpha=cc, beta=1.0, out=z, mode=mode)
+            gemm(x.mT, x, alpha=cc, beta=1.0, out=z, mode=cmode)
             P.symmetrize_(z)
-            chol, fail = P.cholesky(z, gemm=qmode)
+            chol, fail = P.cholesky(z, gemm=cmode if cmode != 'f64' else 'f32')
             if bool(fail.any()):
                 raise RuntimeError('polar Cholesky lost positive definiteness')
             w = x.clone()
-            P.trsm_right_(w, chol, trans=True, gemm=qmode)
-            P.trsm_right_(w, chol, trans=False, gemm=qmode)
+            P.trsm_right_(w, chol, trans=True, gemm=cmode if cmode != 'f64' else 'f32')
+            P.trsm_right_(w, chol, trans=False, gemm=cmode if cmode != 'f64' else 'f32')
             xn = P.axpby_(aa - bb / cc, w, bb / cc, x.clone())
         test = P.reduce(xn, 'maxdiff', x).amax(0).cpu()   # [max|x' - x|, max|x'|] over the batch
         x = xn
@@ -174,17 +194,46 @@ def _polar(a, stats, opts):
             break
     if not converged:
         raise RuntimeError(f'polar_qdwh did not converge in {max_iter} iterations')
-    h = gemm(x.mT, a, mode=mode)
-    P.symmetrize_(h)
-    w, v, info = P.jacobi_eigh_raw(h, max_sweeps=opts.get('max_sweeps', 60))
+    polish = bool(opts.get('polish', False)) and dt == torch.float32
+    if polish:
+        # FP64 tail: Newton-Schulz steps make X orthogonal to FP64 rounding, H = sym(X^T A) is
+        # formed in FP64 and rounded once for the FP32 eigensolve
+        x64 = x.double()
+        for _ in range(int(opts.get('polish_steps', 1))):
+            x64 = _newton_schulz(x64)
+        h64 = gemm(x64.mT, a.double(), mode='f64')
+        P.symmetrize_(h64)
+        h = h64.float()
+    else:
+        h = gemm(x.mT, a, mode=mode)
+        P.symmetrize_(h)
+    w, v, info = P.jacobi_eigh_raw(h, max_sweeps=opts.get('max_sweeps', 60),
+                                   method=opts.get('eigh_method', 'auto'))
     info_h = info.cpu()
     from .. import native as N
     N.raise_for_info(info_h, f"Hermitian Jacobi did not converge in {opts.get('max_sweeps', 60)} sweeps")
-    s = w.flip(-1).clamp_min(0)
-    v = v.flip(-1).contiguous()
+    polish_sweeps = 0
+    if polish:
+        # V orthogonal to FP64 rounding, then G = V^T H V (nearly diagonal: its off-diagonal is the
+        # FP32 eigensolve's backward error) finished by FP64 Jacobi sweeps; sigma = eig(G)
+        # descending, V <- V W
+        v64 = _newton_schulz(v.double())
+        g64 = gemm(v64.mT, gemm(h64, v64, mode='f64'), mode='f64')
+        P.symmetrize_(g64)
+        w2, z, info2 = P.jacobi_eigh_raw(g64, max_sweeps=opts.get('max_sweeps', 60))
+        info2 = info2.cpu()
+        N.raise_for_info(info2, f"Hermitian Jacobi did not converge in {opts.get('max_sweeps', 60)} sweeps")
+        polish_sweeps = int(info2[:, 0].max())
+        s64, v64 = P.sort_desc(w2, gemm(v64, z, mode='f64'))
+        s = s64.to(dt)
+        u = gemm(x64, v64, mode='f64').to(dt)
+        v = v64.to(dt)
+    else:
+        s = w.flip(-1).clamp_min(0)
+        v = v.flip(-1).contiguous()
+        u = gemm(x, v, mode=mode)
     cut_factor = eps if opts.get('cut', 'reference') == 'floor' else eps * max(m, n)
     keep = s > cut_factor * s[:, :1]
-    u = gemm(x, v, mode=mode)
     nk = keep.sum(-1).to(torch.int32)
     if not bool(keep.all()):
         status = P.complete_columns_(u, nk)
@@ -192,6 +241,8 @@ def _polar(a, stats, opts):
             raise RuntimeError('null-space completion failed: supplied live columns are not independent')
     s = torch.where(keep, s, torch.zeros_like(s))
     fp = gemm(x, gemm(x.mT, x, mode=mode), mode=mode)
+    if polish:
+        stats['polish_sweeps'] = polish_sweeps
     stats.update(polar_iterations=it + 1, eigen_sweeps=int(info_h[:, 0].max()),
                  polar_fixed_point_error=float(P.reduce(fp, 'maxdiff', x)[:, 0].max()))
     return u, s, v

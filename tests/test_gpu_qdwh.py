@@ -55,6 +55,44 @@ def test_eigh_matches_lapack(prims, dtype, b, n):
 
 
 @pytest.mark.parametrize('dtype', [np.float64, np.float32])
+@pytest.mark.parametrize('b,n,inner', [(2, 1, 1), (2, 2, 1), (2, 33, 1), (1, 64, 1), (2, 65, 2), (1, 100, 1),
+                                       (1, 200, 1), (1, 200, 3), (1, 300, 1)])
+def test_eigh_blocked_matches_lapack(prims, dtype, b, n, inner):
+    """The blocked solver (64 x 64 subproblems, MFMA rotations of the rest) reaches the same
+    decomposition as LAPACK within the scalar path's bars, and the same stopping rule."""
+    rng = np.random.default_rng(100 + n)
+    h = rng.standard_normal((b, n, n))
+    h = ((h + np.swapaxes(h, 1, 2)) / 2).astype(dtype)
+    w, v, sweeps = prims.jacobi_eigh(torch.from_numpy(h).to(DEV), max_sweeps=60, method='blocked', inner_sweeps=inner)
+    w, v = w.cpu().numpy().astype(np.float64), v.cpu().numpy().astype(np.float64)
+    bar = 1e-12 if dtype == np.float64 else 2e-6
+    for i in range(b):
+        ref = scipy.linalg.eigh(h[i].astype(np.float64), eigvals_only=True)
+        scale = max(np.abs(ref).max(), 1e-300)
+        assert np.all(np.diff(w[i]) >= 0)
+        assert np.max(np.abs(w[i] - ref)) / scale < bar * np.sqrt(n)
+        assert np.max(np.abs(v[i].T @ v[i] - np.eye(n))) < bar * n
+        assert np.max(np.abs((v[i] * w[i]) @ v[i].T - h[i])) / scale < bar * n
+    assert 1 <= sweeps < 60
+
+
+def test_eigh_blocked_special(prims):
+    # zero matrix: no rotation, one sweep, V = I; rank-deficient PSD; diagonal input
+    z = torch.zeros(1, 90, 90, dtype=torch.float64, device=DEV)
+    w, v, sweeps = prims.jacobi_eigh(z, method='blocked')
+    assert sweeps == 1 and not w.abs().any() and torch.equal(v, torch.eye(90, dtype=torch.float64, device=DEV)[None])
+    rng = np.random.default_rng(7)
+    x = rng.standard_normal((150, 20))
+    g = x @ x.T
+    w, v, _ = prims.jacobi_eigh(torch.from_numpy(g[None]).to(DEV), method='blocked')
+    ref = scipy.linalg.eigh(g, eigvals_only=True)
+    assert np.max(np.abs(w.cpu().numpy()[0] - ref)) < 1e-12 * ref.max() * 150
+    d = np.diag(rng.permutation(np.arange(1.0, 131.0)))
+    w, v, sweeps = prims.jacobi_eigh(torch.from_numpy(d[None]).to(DEV), method='blocked')
+    assert sweeps == 1 and np.array_equal(w.cpu().numpy()[0], np.arange(1.0, 131.0))
+
+
+@pytest.mark.parametrize('dtype', [np.float64, np.float32])
 @pytest.mark.parametrize('n', [1, 5, 32, 70, 129])
 def test_cholesky_and_trsm(prims, dtype, n):
     rng = np.random.default_rng(n + 1)
@@ -144,3 +182,42 @@ def test_polar_errors(pkg):
         pkg.svd(a, 'polar_qdwh', max_iter=1)
     with pytest.raises(ValueError):
         pkg.svd(a, 'polar_qdwh', lower_bound=0.0)
+
+
+def test_sort_desc(prims):
+    rng = np.random.default_rng(11)
+    x = rng.standard_normal((3, 300, 70))
+    ref = rng.standard_normal((3, 60))
+    for dt in (np.float64, np.float32):
+        xt = torch.from_numpy(x.astype(dt)).to(DEV)
+        keys = torch.from_numpy(np.concatenate([ref, ref[:, :10]], axis=1)).to(DEV)   # with ties
+        s, g = prims.sort_desc(keys, xt)
+        kk = keys.cpu().numpy()
+        order = np.stack([np.lexsort((np.arange(70), -k)) for k in kk])
+        assert np.array_equal(s.cpu().numpy(), np.maximum(np.take_along_axis(kk, order, 1), 0).astype(dt))
+        assert np.array_equal(g.cpu().numpy(), np.take_along_axis(x.astype(dt), order[:, None, :], 2))
+
+
+@pytest.mark.parametrize('n,method,gmode', [(64, 'auto', 'f32'), (64, 'auto', 'bf16x3'), (200, 'auto', 'bf16x3'),
+                                           (200, 'cyclic', 'f32'), (300, 'blocked', 'bf16x3')])
+def test_polar_qdwh_polish(pkg, n, method, gmode):
+    """FP64 tail on FP32 data, against FP64 LAPACK on the rounded input.  Orthogonality (FP64
+    Newton-Schulz on X and V) reaches 1e-6 in every mode.  Sigma comes from the FP64 Jacobi finish of
+    V^T H V, insensitive to first order to a rotation error of the polar factor: 1e-6 with FP32
+    products; with BF16x3 QR-branch steps the 2^-16 backward error leaves the subspace of sigma below
+    about 1e-5 sigma_1 unresolved, which bounds sigma error (1e-5) and reconstruction (3e-5)."""
+    a, _ = X.workload_qdwh(n, seed=3)
+    at = torch.from_numpy(a).to(DEV)
+    ref = M.reference(a)
+    u, s, vh, info = pkg.svd(at, 'polar_qdwh', gemm=gmode, polish=True, eigh_method=method, cut='floor',
+                             return_info=True)
+    met = M.validate(a, (u.cpu().numpy(), s.cpu().numpy(), vh.cpu().numpy()), ref)['worst']
+    bars = {'spectrum_error_over_sigma1': 1e-6 if gmode == 'f32' else 1e-5,
+            'orthogonality_u_fro_over_sqrt_cols': 1e-6, 'orthogonality_v_fro_over_sqrt_rows': 1e-6,
+            'reconstruction_relative_fro': 1e-6 if gmode == 'f32' else 3e-5}
+    for k in KEYS:
+        assert met[k] <= bars[k], (k, met[k])
+    assert np.all(np.diff(s.cpu().numpy()) <= 0)
+    u0, s0, vh0 = pkg.svd(at, 'polar_qdwh', gemm=gmode, eigh_method=method, cut='floor')
+    met0 = M.validate(a, (u0.cpu().numpy(), s0.cpu().numpy(), vh0.cpu().numpy()), ref)['worst']
+    assert met['orthogonality_u_fro_over_sqrt_cols'] <= met0['orthogonality_u_fro_over_sqrt_cols']
