@@ -179,4 +179,10 @@ __device__ __forceinline__ void jacobi_rotation_t<double>(double app, double aqq
   tb = t * a;
 }
 
+// MFMA GEMM with the psvd_gemm contract plus an optional split-K workspace (gemm.hip).
+int gemm_ws(int mode, int transa, int transb, int64_t m, int64_t n, int64_t k, double alpha, const void *a, int64_t lda,
+            int64_t stride_a, const void *b, int64_t ldb, int64_t stride_b, double beta, void *c, int64_t ldc,
+            int64_t stride_c, int64_t batch, void *ws, size_t ws_bytes, hipStream_t st);
+size_t gemm_splitk_bytes(int mode, int64_t m, int64_t n, int64_t batch);
+
 }  // namespace psvd

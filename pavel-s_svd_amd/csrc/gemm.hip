@@ -28,6 +28,8 @@ struct GemmParams {
   int64_t lda, ldb, ldc, sa, sb, sc;
   int ta, tb;
   double alpha, beta;
+  int64_t kchunk;   // split-K: z-slice z covers k in [(z % ksplit) * kchunk, ... + kchunk)
+  int ksplit;       // 1: C = alpha op(A) op(B) + beta C; > 1: raw partial sums into C + z * M * N
 };
 
 constexpr int kGemmThreads = 256;
@@ -97,10 +99,12 @@ __global__ __launch_bounds__(kGemmThreads) void gemm_kernel(GemmParams P) {
   __shared__ __align__(16) S Blo[SPLIT ? BN * LD : 1];
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int64_t bz = blockIdx.z;
+  const int64_t bz = blockIdx.z / P.ksplit, kz = blockIdx.z - bz * P.ksplit;
   const T *A = static_cast<const T *>(P.A) + bz * P.sa;
   const T *B = static_cast<const T *>(P.B) + bz * P.sb;
-  T *C = static_cast<T *>(P.C) + bz * P.sc;
+  const bool partial = P.ksplit > 1;
+  T *C = static_cast<T *>(P.C) + (partial ? (int64_t)blockIdx.z * P.M * P.N : bz * P.sc);
+  const int64_t kbeg = kz * P.kchunk, kend = kbeg + P.kchunk < P.K ? kbeg + P.kchunk : P.K;
   const int64_t m0 = (int64_t)blockIdx.y * BM, n0 = (int64_t)blockIdx.x * BN;
   const int wm = (wave >> 1) * WM, wn = (wave & 1) * WN;
   const int lr = lane & 15, lk = lane >> 4;
@@ -114,10 +118,10 @@ __global__ __launch_bounds__(kGemmThreads) void gemm_kernel(GemmParams P) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) acc[i][j][r] = 0;
 
-  for (int64_t k0 = 0; k0 < P.K; k0 += BK) {
+  for (int64_t k0 = kbeg; k0 < kend; k0 += BK) {
     __syncthreads();
-    stage_tile<T, S, BM, BK, LD, SPLIT>(As, Alo, A, P.lda, P.ta, m0, P.M, k0, P.K);
-    stage_tile<T, S, BN, BK, LD, SPLIT>(Bs, Blo, B, P.ldb, !P.tb, n0, P.N, k0, P.K);
+    stage_tile<T, S, BM, BK, LD, SPLIT>(As, Alo, A, P.lda, P.ta, m0, P.M, k0, kend);
+    stage_tile<T, S, BN, BK, LD, SPLIT>(Bs, Blo, B, P.ldb, !P.tb, n0, P.N, k0, kend);
     __syncthreads();
     if constexpr (MODE == PSVD_GEMM_BF16X3) {
       v8bf ah[TI], al[TI], bh[TJ], bl[TJ];
@@ -171,9 +175,13 @@ __global__ __launch_bounds__(kGemmThreads) void gemm_kernel(GemmParams P) {
         const int drow = MODE == PSVD_GEMM_F64 ? 4 * r + lk : 4 * lk + r;
         const int64_t row = m0 + wm + 16 * i + drow, col = n0 + wn + 16 * j + lr;
         if (row < P.M && col < P.N) {
-          T *c = C + row * P.ldc + col;
-          const T v = (T)P.alpha * (T)acc[i][j][r];
-          *c = P.beta == 0.0 ? v : v + (T)P.beta * *c;
+          if (partial) {
+            C[row * P.N + col] = (T)acc[i][j][r];
+          } else {
+            T *c = C + row * P.ldc + col;
+            const T v = (T)P.alpha * (T)acc[i][j][r];
+            *c = P.beta == 0.0 ? v : v + (T)P.beta * *c;
+          }
         }
       }
 }
@@ -182,9 +190,56 @@ template <int MODE>
 static int launch_gemm(const GemmParams &P, hipStream_t stream) {
   using Cfg = GemmCfg<MODE>;
   const dim3 grid((unsigned)((P.N + Cfg::BN - 1) / Cfg::BN), (unsigned)((P.M + Cfg::BM - 1) / Cfg::BM),
-                  (unsigned)P.batch);
+                  (unsigned)(P.batch * P.ksplit));
   hipLaunchKernelGGL(gemm_kernel<MODE>, grid, dim3(kGemmThreads), 0, stream, P);
   return hipGetLastError() == hipSuccess ? PSVD_OK : PSVD_ELAUNCH;
+}
+
+// C_b = alpha * sum_z part[b, z] + beta * C_b, the z partials added in order (deterministic).
+template <typename T>
+__global__ void splitk_reduce_kernel(const T *part, int ksplit, int64_t M, int64_t N, T *C, int64_t ldc, int64_t sc,
+                                     double alpha, double beta) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= M * N) return;
+  const int64_t b = blockIdx.y;
+  const T *p = part + b * ksplit * M * N + e;
+  T acc = p[0];
+  for (int z = 1; z < ksplit; ++z) acc += p[(int64_t)z * M * N];
+  T *c = C + b * sc + (e / N) * ldc + (e % N);
+  const T v = (T)alpha * acc;
+  *c = beta == 0.0 ? v : v + (T)beta * *c;
+}
+
+template <int MODE>
+static int run_gemm(GemmParams &P, void *ws, size_t ws_bytes, hipStream_t st) {
+  using Cfg = GemmCfg<MODE>;
+  using T = typename Cfg::T;
+  P.ksplit = 1;
+  P.kchunk = P.K;
+  const int64_t tiles = ((P.N + Cfg::BN - 1) / Cfg::BN) * ((P.M + Cfg::BM - 1) / Cfg::BM) * P.batch;
+  int64_t want = tiles >= 256 ? 1 : (512 + tiles - 1) / tiles;
+  const int64_t kmax = P.K / (8 * Cfg::BK);                   // at least eight k-steps per slice
+  if (want > kmax) want = kmax;
+  if (want > 32) want = 32;
+  while (want > 1 && (size_t)want * P.batch * P.M * P.N * sizeof(T) > ws_bytes) --want;
+  if (!ws || want <= 1 || P.batch * want > 65535) return launch_gemm<MODE>(P, st);
+  P.ksplit = (int)want;
+  P.kchunk = ((P.K + want - 1) / want + Cfg::BK - 1) / Cfg::BK * Cfg::BK;
+  P.ksplit = (int)((P.K + P.kchunk - 1) / P.kchunk);
+  T *C = static_cast<T *>(P.C);
+  const int64_t ldc = P.ldc, sc = P.sc;
+  P.C = ws;
+  const int rc = launch_gemm<MODE>(P, st);
+  if (rc != PSVD_OK) return rc;
+  const int64_t e = P.M * P.N;
+  hipLaunchKernelGGL(splitk_reduce_kernel<T>, dim3((unsigned)((e + 255) / 256), (unsigned)P.batch), dim3(256), 0, st,
+                     static_cast<const T *>(ws), P.ksplit, P.M, P.N, C, ldc, sc, P.alpha, P.beta);
+  return hipGetLastError() == hipSuccess ? PSVD_OK : PSVD_ELAUNCH;
+}
+
+// Workspace that lets psvd::gemm_ws split K for an M x N output (batch of them): 32 slices.
+size_t gemm_splitk_bytes(int mode, int64_t m, int64_t n, int64_t batch) {
+  return (size_t)32 * batch * m * n * (mode == PSVD_GEMM_F64 ? 8 : 4);
 }
 
 }  // namespace psvd
@@ -193,12 +248,23 @@ extern "C" int psvd_gemm(int mode, int transa, int transb, int64_t m, int64_t n,
                          const void *a, int64_t lda, int64_t stride_a, const void *b, int64_t ldb,
                          int64_t stride_b, double beta, void *c, int64_t ldc, int64_t stride_c, int64_t batch,
                          psvd_stream_t stream) {
+  return psvd::gemm_ws(mode, transa, transb, m, n, k, alpha, a, lda, stride_a, b, ldb, stride_b, beta, c, ldc,
+                       stride_c, batch, nullptr, 0, static_cast<hipStream_t>(stream));
+}
+
+namespace psvd {
+
+// Same contract as psvd_gemm; with a workspace of gemm_splitk_bytes(), outputs too small to fill
+// the GPU are computed as K slices and reduced in a fixed order.
+int gemm_ws(int mode, int transa, int transb, int64_t m, int64_t n, int64_t k, double alpha, const void *a, int64_t lda,
+            int64_t stride_a, const void *b, int64_t ldb, int64_t stride_b, double beta, void *c, int64_t ldc,
+            int64_t stride_c, int64_t batch, void *ws, size_t ws_bytes, hipStream_t st) {
   if (m < 0 || n < 0 || k < 0 || batch < 0 || !c) return PSVD_EINVAL;
   if (m == 0 || n == 0 || batch == 0) return PSVD_OK;
   if (k > 0 && (!a || !b)) return PSVD_EINVAL;
   if (ldc < n || lda < (transa ? m : k) || ldb < (transb ? k : n)) return PSVD_EINVAL;
   if (batch > 65535 || (n + 127) / 128 > 0x7fffffff || (m + 63) / 64 > 65535) return PSVD_EINVAL;
-  psvd::GemmParams P;
+  GemmParams P;
   P.M = m;
   P.N = n;
   P.K = k;
@@ -216,9 +282,10 @@ extern "C" int psvd_gemm(int mode, int transa, int transb, int64_t m, int64_t n,
   P.tb = transb;
   P.alpha = alpha;
   P.beta = beta;
-  hipStream_t st = static_cast<hipStream_t>(stream);
-  if (mode == PSVD_GEMM_F32) return psvd::launch_gemm<PSVD_GEMM_F32>(P, st);
-  if (mode == PSVD_GEMM_BF16X3) return psvd::launch_gemm<PSVD_GEMM_BF16X3>(P, st);
-  if (mode == PSVD_GEMM_F64) return psvd::launch_gemm<PSVD_GEMM_F64>(P, st);
+  if (mode == PSVD_GEMM_F32) return run_gemm<PSVD_GEMM_F32>(P, ws, ws_bytes, st);
+  if (mode == PSVD_GEMM_BF16X3) return run_gemm<PSVD_GEMM_BF16X3>(P, ws, ws_bytes, st);
+  if (mode == PSVD_GEMM_F64) return run_gemm<PSVD_GEMM_F64>(P, ws, ws_bytes, st);
   return PSVD_EINVAL;
 }
+
+}  // namespace psvd

@@ -14,7 +14,11 @@
 // :74-89 (qr).
 #include <float.h>
 
+#include <hip/hip_cooperative_groups.h>
+
 #include "common.hpp"
+
+namespace cg = cooperative_groups;
 
 namespace psvd {
 
@@ -131,6 +135,195 @@ __global__ __launch_bounds__(kPanelThreads) void qr_panel_kernel(T *F, int64_t l
   }
 }
 
+// ---- multi-workgroup panel for tall panels ---------------------------------------------------
+// The same reflectors as qr_panel_kernel, with the panel's rows spread over nw workgroups per
+// matrix (KR rows per thread, held in registers for the whole panel: one read, one write).  Each
+// column needs two reductions over all rows -- |x2|^2, then w = v^T F(:, later columns) -- done as
+// per-workgroup partial sums in FP64 that every workgroup then adds in the same fixed order after
+// a grid barrier (deterministic, no atomics).  Scratch buffers alternate with the column parity so
+// that a fast workgroup never overwrites partials a slow one is still reading.
+struct CoopScr {
+  int nw;
+  __host__ __device__ static size_t doubles(int nw) { return 2 * ((size_t)nw + 1 + (size_t)nw * kPanel); }
+};
+
+constexpr int kCoopLd = kPanel + 1;
+
+template <typename T, int KR>
+__host__ __device__ constexpr size_t coop_lds_bytes() {
+  return sizeof(T) * (size_t)256 * KR * kCoopLd;
+}
+
+template <typename T, int KR>
+__global__ __launch_bounds__(256) void qr_panel_coop_kernel(T *F, int64_t ldf, int64_t sf, T *V, int64_t sv, T *tau,
+                                                            int64_t stau, int m, int j0, int nbc, int nw,
+                                                            double *scr) {
+  constexpr int NT = 256;
+  extern __shared__ double smem_coop[];
+  __shared__ double red[(NT / kWave) * kPanel];
+  __shared__ double wsum[kPanel];
+  __shared__ double seg[8][kPanel];
+  cg::grid_group grid = cg::this_grid();
+  const int tid = threadIdx.x;
+  const int b = blockIdx.x / nw, wi = blockIdx.x - b * nw;
+  T *Fb = F + (int64_t)b * sf;
+  T *Vb = V + (int64_t)b * sv;
+  T *tb = tau + (int64_t)b * stau;
+  double *sb = scr + (size_t)b * CoopScr::doubles(nw);
+  const size_t half = (size_t)nw + 1 + (size_t)nw * kPanel;
+  // the thread's KR panel rows, private to it (no barrier needed around them)
+  T *xs = reinterpret_cast<T *>(smem_coop);
+  auto X = [&](int q, int k) -> T & { return xs[(q * NT + tid) * kCoopLd + k]; };
+  int row[KR];
+#pragma unroll
+  for (int q = 0; q < KR; ++q) {
+    row[q] = j0 + (wi * KR + q) * NT + tid;
+#pragma unroll
+    for (int k = 0; k < kPanel; ++k) X(q, k) = (row[q] < m && k < nbc) ? Fb[(int64_t)row[q] * ldf + j0 + k] : T(0);
+  }
+  for (int c = 0; c < nbc; ++c) {
+    const int j = j0 + c;
+    double *np_ = sb + (c & 1) * half;        // nw norm partials, then alpha
+    double *al = np_ + nw;
+    double *wp = al + 1;                      // nw x kPanel w partials
+    double part = 0.0;
+#pragma unroll
+    for (int q = 0; q < KR; ++q)
+      if (row[q] > j && row[q] < m) part = fma((double)X(q, c), (double)X(q, c), part);
+    part = block_sum_one<NT>(part, red);
+    if (tid == 0) np_[wi] = part;
+#pragma unroll
+    for (int q = 0; q < KR; ++q)
+      if (row[q] == j) *al = (double)X(q, c);
+    grid.sync();
+    double tot = 0.0;
+    for (int t = tid; t < nw; t += NT) tot += np_[t];
+    const double xnorm = sqrt(block_sum_one<NT>(tot, red));
+    const double alpha = *al;
+    double beta, tj, scal;
+    const bool trivial = xnorm == 0.0;
+    if (trivial) {
+      beta = alpha;
+      tj = 0.0;
+      scal = 0.0;
+    } else {
+      beta = -copysign(1.0, alpha == 0.0 ? 1.0 : alpha) * hypot(alpha, xnorm);
+      tj = (beta - alpha) / beta;
+      scal = 1.0 / (alpha - beta);
+    }
+    double v[KR];
+#pragma unroll
+    for (int q = 0; q < KR; ++q) {
+      const int i = row[q];
+      v[q] = 0.0;
+      if (i >= m) continue;
+      if (i > j) {
+        const T vi = (T)((double)X(q, c) * scal);
+        v[q] = (double)vi;
+        Vb[(int64_t)i * ldf + j] = vi;
+        X(q, c) = T(0);
+      } else if (i == j) {
+        v[q] = trivial ? 0.0 : 1.0;
+        Vb[(int64_t)i * ldf + j] = trivial ? T(0) : T(1);
+        X(q, c) = (T)beta;
+      } else {
+        Vb[(int64_t)i * ldf + j] = T(0);
+      }
+    }
+    if (wi == 0 && tid == 0) tb[j] = (T)tj;
+    if (!trivial && c + 1 < nbc) {
+      double w[kPanel];
+#pragma unroll
+      for (int k = 0; k < kPanel; ++k) {
+        w[k] = 0.0;
+        if (k > c && k < nbc)
+#pragma unroll
+          for (int q = 0; q < KR; ++q) w[k] = fma(v[q], (double)X(q, k), w[k]);
+      }
+      block_sum_vec<NT>(w, c + 1, nbc, red, wsum);
+      if (tid > c && tid < nbc) wp[(size_t)wi * kPanel + tid] = wsum[tid];
+      grid.sync();
+      {  // thread (k = tid % 32, s = tid / 32) adds the partials of workgroups s, s + 8, ...
+        const int k = tid & (kPanel - 1), s = tid / kPanel;
+        double acc = 0.0;
+        if (k > c && k < nbc)
+          for (int t = s; t < nw; t += 8) acc += wp[(size_t)t * kPanel + k];
+        seg[s][k] = acc;
+      }
+      __syncthreads();
+      if (tid < kPanel) {
+        double acc = 0.0;
+#pragma unroll
+        for (int s = 0; s < 8; ++s) acc += seg[s][tid];
+        wsum[tid] = acc;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < KR; ++q) {
+        const double tv = tj * v[q];
+#pragma unroll
+        for (int k = 0; k < kPanel; ++k)
+          if (k > c && k < nbc) X(q, k) = (T)((double)X(q, k) - tv * wsum[k]);
+      }
+      __syncthreads();
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < KR; ++q)
+    if (row[q] < m)
+#pragma unroll
+      for (int k = 0; k < kPanel; ++k)
+        if (k < nbc) Fb[(int64_t)row[q] * ldf + j0 + k] = X(q, k);
+}
+
+static int device_cus() {
+  static int cus = -1;
+  if (cus < 0) {
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 1;
+  }
+  return cus;
+}
+
+// Rows per thread for the cooperative panel of a `batch` of matrices with `rows` panel rows, or 0
+// when even the largest setting would need more workgroups than there are CUs (one resident
+// workgroup per CU keeps the cooperative grid co-resident without relying on occupancy).
+template <typename T>
+static int coop_rows_per_thread(int64_t batch, int rows) {
+  const int cus = device_cus();
+  const int kmax = sizeof(T) == 4 ? 4 : 2;
+  for (int kr = 1; kr <= kmax; kr *= 2) {
+    const int64_t nw = (rows + 256LL * kr - 1) / (256LL * kr);
+    if (nw * batch <= cus) return kr;
+  }
+  return 0;
+}
+
+template <typename T>
+static size_t coop_scr_bytes(int64_t batch, int m) {
+  const int64_t nw = (m + 255) / 256;   // the largest nw any panel of this matrix can use
+  return sizeof(double) * (size_t)batch * CoopScr::doubles((int)nw);
+}
+
+template <typename T, int KR>
+static int launch_coop(T *F, int64_t ldf, int64_t sf, T *V, int64_t sv, T *tau, int64_t stau, int m, int j0, int nbc,
+                       int64_t batch, double *scr, hipStream_t st) {
+  static bool attr = false;
+  if (!attr) {
+    if (hipFuncSetAttribute(reinterpret_cast<const void *>(qr_panel_coop_kernel<T, KR>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)coop_lds_bytes<T, KR>()) != hipSuccess)
+      return PSVD_ELAUNCH;
+    attr = true;
+  }
+  int nw = (int)((m - j0 + 256LL * KR - 1) / (256LL * KR));
+  void *args[] = {&F, &ldf, &sf, &V, &sv, &tau, &stau, &m, &j0, &nbc, &nw, &scr};
+  const hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<const void *>(qr_panel_coop_kernel<T, KR>),
+                                                  dim3((unsigned)(nw * batch)), dim3(256), args,
+                                                  (unsigned)coop_lds_bytes<T, KR>(), st);
+  return e == hipSuccess ? PSVD_OK : PSVD_ELAUNCH;
+}
+
 // Compact-WY factor of one panel: T (nbc x nbc upper triangular, stored kPanel x kPanel) from
 // G = V^T V and tau (LAPACK larft, forward columnwise): T_ii = tau_i,
 // T(0:i, i) = -tau_i T(0:i, 0:i) G(0:i, i).
@@ -188,7 +381,8 @@ size_t qr_ws_bytes(int64_t batch, int m, int n) {
   const int64_t np = (n + kPanel - 1) / kPanel;
   auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
   return al(sizeof(T) * batch * m * n) * 2 + al(sizeof(T) * batch * np * kPanel * kPanel) + al(sizeof(T) * batch * n) +
-         al(sizeof(T) * batch * kPanel * kPanel) + al(sizeof(T) * batch * kPanel * n) * 2;
+         al(sizeof(T) * batch * kPanel * kPanel) + al(sizeof(T) * batch * kPanel * n) * 2 + al(coop_scr_bytes<T>(batch, m)) +
+         al(gemm_splitk_bytes(sizeof(T) == 8 ? PSVD_GEMM_F64 : PSVD_GEMM_F32, kPanel, n, batch));
 }
 
 #define PSVD_TRY(x)            \
@@ -215,24 +409,37 @@ static int run_qr(int gmode, int64_t batch, int m, int n, const T *a, int64_t ld
   T *G = take(sizeof(T) * batch * kPanel * kPanel);
   T *W1 = take(sizeof(T) * batch * kPanel * n);
   T *W2 = take(sizeof(T) * batch * kPanel * n);
+  double *scr = reinterpret_cast<double *>(take(coop_scr_bytes<T>(batch, m)));
+  const size_t skb = gemm_splitk_bytes(sizeof(T) == 8 ? PSVD_GEMM_F64 : PSVD_GEMM_F32, kPanel, n, batch);
+  void *sk = take(skb);
   const int64_t stf = np * kPanel * kPanel;
   PSVD_TRY(copy2d<T>(a, lda, sa, F, n, mn, m, n, batch, false, st));
   for (int64_t p = 0; p < np; ++p) {
     const int j0 = (int)(p * kPanel), nbc = n - j0 < kPanel ? n - j0 : kPanel, rows = m - j0, n2 = n - j0 - nbc;
-    hipLaunchKernelGGL(qr_panel_kernel<T>, dim3((unsigned)batch), dim3(kPanelThreads), 0, st, F, (int64_t)n, mn, V,
-                       mn, tau, (int64_t)n, m, j0, nbc);
-    if (hipGetLastError() != hipSuccess) return PSVD_ELAUNCH;
+    // tall panels: rows spread over several workgroups per matrix; short or very many: one each
+    const int kr = rows > kPanelThreads ? coop_rows_per_thread<T>(batch, rows) : 0;
+    if (kr == 1) {
+      PSVD_TRY((launch_coop<T, 1>(F, n, mn, V, mn, tau, n, m, j0, nbc, batch, scr, st)));
+    } else if (kr == 2) {
+      PSVD_TRY((launch_coop<T, 2>(F, n, mn, V, mn, tau, n, m, j0, nbc, batch, scr, st)));
+    } else if (kr == 4 && sizeof(T) == 4) {
+      PSVD_TRY((launch_coop<T, sizeof(T) == 4 ? 4 : 2>(F, n, mn, V, mn, tau, n, m, j0, nbc, batch, scr, st)));
+    } else {
+      hipLaunchKernelGGL(qr_panel_kernel<T>, dim3((unsigned)batch), dim3(kPanelThreads), 0, st, F, (int64_t)n, mn, V,
+                         mn, tau, (int64_t)n, m, j0, nbc);
+      if (hipGetLastError() != hipSuccess) return PSVD_ELAUNCH;
+    }
     T *Vp = V + (int64_t)j0 * n + j0;  // (rows x nbc), ld n
-    PSVD_TRY(psvd_gemm(gmode, 1, 0, nbc, nbc, rows, 1.0, Vp, n, mn, Vp, n, mn, 0.0, G, kPanel, kPanel * kPanel,
-                       batch, st));
+    PSVD_TRY(gemm_ws(gmode, 1, 0, nbc, nbc, rows, 1.0, Vp, n, mn, Vp, n, mn, 0.0, G, kPanel, kPanel * kPanel, batch,
+                     sk, skb, st));
     hipLaunchKernelGGL(qr_larft_kernel<T>, dim3((unsigned)batch), dim3(64), 0, st, G, (int64_t)kPanel * kPanel, tau,
                        (int64_t)n, j0, Tf + p * kPanel * kPanel, stf, nbc);
     if (hipGetLastError() != hipSuccess) return PSVD_ELAUNCH;
     if (n2 > 0) {
       T *F2 = F + (int64_t)j0 * n + j0 + nbc;  // trailing block (rows x n2), ld n
       T *Tp = Tf + p * kPanel * kPanel;
-      PSVD_TRY(psvd_gemm(gmode, 1, 0, nbc, n2, rows, 1.0, Vp, n, mn, F2, n, mn, 0.0, W1, n, (int64_t)kPanel * n, batch,
-                         st));
+      PSVD_TRY(gemm_ws(gmode, 1, 0, nbc, n2, rows, 1.0, Vp, n, mn, F2, n, mn, 0.0, W1, n, (int64_t)kPanel * n, batch,
+                       sk, skb, st));
       PSVD_TRY(psvd_gemm(gmode, 1, 0, nbc, n2, nbc, 1.0, Tp, kPanel, stf, W1, n, (int64_t)kPanel * n, 0.0, W2, n,
                          (int64_t)kPanel * n, batch, st));
       PSVD_TRY(psvd_gemm(gmode, 0, 0, rows, n2, nbc, -1.0, Vp, n, mn, W2, n, (int64_t)kPanel * n, 1.0, F2, n, mn, batch,
@@ -247,8 +454,8 @@ static int run_qr(int gmode, int64_t batch, int m, int n, const T *a, int64_t ld
       const T *Vp = V + (int64_t)j0 * n + j0;
       T *Qs = q + (int64_t)j0 * n + j0;
       T *Tp = Tf + p * kPanel * kPanel;
-      PSVD_TRY(psvd_gemm(gmode, 1, 0, nbc, nq, rows, 1.0, Vp, n, mn, Qs, n, mn, 0.0, W1, n, (int64_t)kPanel * n, batch,
-                         st));
+      PSVD_TRY(gemm_ws(gmode, 1, 0, nbc, nq, rows, 1.0, Vp, n, mn, Qs, n, mn, 0.0, W1, n, (int64_t)kPanel * n, batch,
+                       sk, skb, st));
       PSVD_TRY(psvd_gemm(gmode, 0, 0, nbc, nq, nbc, 1.0, Tp, kPanel, stf, W1, n, (int64_t)kPanel * n, 0.0, W2, n,
                          (int64_t)kPanel * n, batch, st));
       PSVD_TRY(psvd_gemm(gmode, 0, 0, rows, nq, nbc, -1.0, Vp, n, mn, W2, n, (int64_t)kPanel * n, 1.0, Qs, n, mn, batch,

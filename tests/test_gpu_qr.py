@@ -23,7 +23,8 @@ def prims():
 
 
 @pytest.mark.parametrize('dtype', [np.float64, np.float32])
-@pytest.mark.parametrize('b,m,n', [(1, 5, 3), (2, 40, 40), (3, 100, 33), (1, 300, 70), (1, 1000, 64), (2, 257, 129)])
+@pytest.mark.parametrize('b,m,n', [(1, 5, 3), (2, 40, 40), (3, 100, 33), (1, 300, 70), (1, 1000, 64), (2, 257, 129),
+                                   (2, 3000, 70), (1, 9000, 40), (4, 20000, 33), (8, 20000, 8)])
 def test_qr_matches_oracle(prims, dtype, b, m, n):
     rng = np.random.default_rng(m + n)
     a = rng.standard_normal((b, m, n)).astype(dtype)
@@ -50,6 +51,25 @@ def test_qr_rank_deficient_and_zero_columns(prims):
     qo, ro = R.qr(a[None])
     assert np.allclose(r.cpu().numpy(), ro, atol=1e-10)
     assert np.allclose(q.cpu().numpy() @ r.cpu().numpy(), a, atol=1e-10)
+
+
+def test_qr_tall_rank_deficient_multi_workgroup_panel(prims):
+    # panels taller than one workgroup: trivial reflectors (zero columns) and a dependent column
+    a = np.zeros((3000, 6))
+    a[:, 1] = np.sin(np.arange(3000))
+    a[:, 3] = a[:, 1] * 2
+    a[:, 4] = np.cos(np.arange(3000))
+    for dt in (np.float64, np.float32):
+        x = a.astype(dt)
+        q, r = prims.qr(torch.from_numpy(x[None]).to(DEV))
+        ro = scipy.linalg.qr(x.astype(np.float64), mode='economic')[1]
+        bar = 1e-12 if dt == np.float64 else 2e-5
+        q, r = q.cpu().numpy()[0].astype(np.float64), r.cpu().numpy()[0].astype(np.float64)
+        assert np.max(np.abs(q @ r - x)) < bar * 60
+        assert np.max(np.abs(q.T @ q - np.eye(6))) < bar * 60
+        # R is unique up to the dependent column 3; past it the reflector of a rounding-noise
+        # column (|x2| ~ 1e-14, not exactly zero) points in a noise direction in any implementation
+        assert np.max(np.abs(r[:3, :4] - ro[:3, :4])) < bar * 60 * np.abs(ro).max()
 
 
 def test_qr_bf16x3_panel_updates(prims):
