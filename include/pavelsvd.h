@@ -190,6 +190,28 @@ int psvd_jacobi_grid(int dtype, int m, int n, const void *a, void *u, void *s, v
                      double tol, int max_sweeps, int flags, void *workspace, size_t workspace_bytes,
                      psvd_stream_t stream);
 
+/* ---------------------------------------------------------------------------------------
+ * Learned QDWH lower bound (no reference counterpart: the reference starts QDWH from l0 = eps,
+ * solvers.py:128-130).  A 32 -> 64 -> 64 -> 1 tanh MLP, FP32, parameters in one flat vector of
+ * psvd_mlp_param_count() floats: W1 (64 x 32), b1, W2 (64 x 64), b2, w3 (64), b3.
+ *  psvd_mlp_forward: y[i] = MLP(x[i, :]) for x (batch x 32).
+ *  psvd_mlp_grad: y as above and grad = d/dparams of (1 / 2 batch) sum (y_i - target_i)^2, loss
+ *                 (device double, may be NULL); per-workgroup partial gradients are added in a
+ *                 fixed order (deterministic).  Workspace of psvd_mlp_workspace_bytes(batch).
+ *  psvd_adam_step: Adam update of n parameters with bias correction (step >= 1).
+ *  psvd_colstats: per matrix, {min, max, mean, std} of log10 of the column norms and log10 max|a|
+ *                 (out: batch x 5 floats), the column-norm part of the sketch features.
+ * ------------------------------------------------------------------------------------- */
+int psvd_mlp_param_count(void);
+size_t psvd_mlp_workspace_bytes(int64_t batch);
+int psvd_mlp_forward(const float *params, const float *x, int64_t batch, float *y, psvd_stream_t stream);
+int psvd_mlp_grad(const float *params, const float *x, const float *target, int64_t batch, float *y, float *grad,
+                  double *loss, void *workspace, size_t workspace_bytes, psvd_stream_t stream);
+int psvd_adam_step(float *params, const float *grad, float *m, float *v, int n, float lr, float beta1, float beta2,
+                   float eps, int step, psvd_stream_t stream);
+int psvd_colstats(int dtype, int64_t batch, int rows, int cols, const void *a, int64_t lda, int64_t stride_a,
+                  float *out, psvd_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

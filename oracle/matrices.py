@@ -136,3 +136,32 @@ def workload_tall_lowrank(m=8192, n=2048, rank=64, seed=SEED):
     s = np.geomspace(1, 1e-3, rank)
     a = (g1 * s) @ g2.T + rng.normal(0.0, 1e-4, size=(m, n))
     return a.astype(np.float32)
+
+
+def l0_spectrum(n, rng, log10_cond, family):
+    """Singular values for the learned-l0 workload: 'geometric' from 1 to 10^-log10_cond, or
+    'clustered' -- log-uniform on that range with a tenth of the values in a cluster of relative
+    width 1e-4 at a random position; sorted descending with the extremes pinned."""
+    lo = 10.0 ** -log10_cond
+    if family == 'geometric':
+        return np.geomspace(1.0, lo, n)
+    k = max(1, n // 10)
+    s = np.exp(rng.uniform(np.log(lo), 0.0, n - k))
+    c = np.exp(rng.uniform(np.log(lo), 0.0)) * (1.0 + 1e-4 * (rng.random(k) - 0.5))
+    s = np.sort(np.concatenate([s, c]))[::-1]
+    s[0], s[-1] = 1.0, lo
+    return s
+
+
+def workload_l0(count, n, seed=SEED, cond_lo=1.0, cond_hi=12.0):
+    """`count` float32 n x n matrices U diag(sigma) V^T (Haar U, V; FP64 product rounded once) with
+    log10 condition ~ U[cond_lo, cond_hi] and geometric or clustered spectra (alternating).
+    Returns (a (count, n, n) float32, log10 condition numbers, families)."""
+    rng = np.random.default_rng(seed)
+    conds = rng.uniform(cond_lo, cond_hi, count)
+    fams = ['geometric' if i % 2 == 0 else 'clustered' for i in range(count)]
+    out = np.empty((count, n, n), dtype=np.float32)
+    for i in range(count):
+        s = l0_spectrum(n, rng, conds[i], fams[i])
+        out[i] = ((haar(n, rng) * s) @ haar(n, rng).T).astype(np.float32)
+    return out, conds, fams

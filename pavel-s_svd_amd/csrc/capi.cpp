@@ -226,6 +226,51 @@ static PyObject *py_sort_desc(PyObject *, PyObject *args) {
                                         ldx, sx, PTR(y), ldy, sy, static_cast<int32_t *>(PTR(perm)), PTR(stream)));
 }
 
+static PyObject *py_mlp_param_count(PyObject *, PyObject *) { return PyLong_FromLong(psvd_mlp_param_count()); }
+
+static PyObject *py_mlp_workspace_bytes(PyObject *, PyObject *args) {
+  long long batch;
+  if (!PyArg_ParseTuple(args, "L", &batch)) return nullptr;
+  return PyLong_FromSize_t(psvd_mlp_workspace_bytes(batch));
+}
+
+static PyObject *py_mlp_forward(PyObject *, PyObject *args) {
+  unsigned long long prm, x, y, stream;
+  long long batch;
+  if (!PyArg_ParseTuple(args, "KKLKK", &prm, &x, &batch, &y, &stream)) return nullptr;
+  return PyLong_FromLong(psvd_mlp_forward(static_cast<const float *>(PTR(prm)), static_cast<const float *>(PTR(x)), batch,
+                                          static_cast<float *>(PTR(y)), PTR(stream)));
+}
+
+static PyObject *py_mlp_grad(PyObject *, PyObject *args) {
+  unsigned long long prm, x, t, y, g, loss, ws, ws_bytes, stream;
+  long long batch;
+  if (!PyArg_ParseTuple(args, "KKKLKKKKKK", &prm, &x, &t, &batch, &y, &g, &loss, &ws, &ws_bytes, &stream)) return nullptr;
+  return PyLong_FromLong(psvd_mlp_grad(static_cast<const float *>(PTR(prm)), static_cast<const float *>(PTR(x)),
+                                       static_cast<const float *>(PTR(t)), batch, static_cast<float *>(PTR(y)),
+                                       static_cast<float *>(PTR(g)), static_cast<double *>(PTR(loss)), PTR(ws),
+                                       (size_t)ws_bytes, PTR(stream)));
+}
+
+static PyObject *py_adam_step(PyObject *, PyObject *args) {
+  unsigned long long prm, g, m, v, stream;
+  int n, step;
+  float lr, b1, b2, eps;
+  if (!PyArg_ParseTuple(args, "KKKKiffffiK", &prm, &g, &m, &v, &n, &lr, &b1, &b2, &eps, &step, &stream)) return nullptr;
+  return PyLong_FromLong(psvd_adam_step(static_cast<float *>(PTR(prm)), static_cast<const float *>(PTR(g)),
+                                        static_cast<float *>(PTR(m)), static_cast<float *>(PTR(v)), n, lr, b1, b2, eps,
+                                        step, PTR(stream)));
+}
+
+static PyObject *py_colstats(PyObject *, PyObject *args) {
+  int dtype, rows, cols;
+  long long batch, lda, sa;
+  unsigned long long a, out, stream;
+  if (!PyArg_ParseTuple(args, "iLiiKLLKK", &dtype, &batch, &rows, &cols, &a, &lda, &sa, &out, &stream)) return nullptr;
+  return PyLong_FromLong(
+      psvd_colstats(dtype, batch, rows, cols, PTR(a), lda, sa, static_cast<float *>(PTR(out)), PTR(stream)));
+}
+
 static PyObject *py_complete_workspace_bytes(PyObject *, PyObject *args) {
   long long batch;
   int m, n;
@@ -267,6 +312,12 @@ static PyMethodDef kMethods[] = {
     {"symmetrize", py_symmetrize, METH_VARARGS, nullptr},
     {"axpby", py_axpby, METH_VARARGS, nullptr},
     {"sort_desc", py_sort_desc, METH_VARARGS, nullptr},
+    {"mlp_param_count", py_mlp_param_count, METH_NOARGS, nullptr},
+    {"mlp_workspace_bytes", py_mlp_workspace_bytes, METH_VARARGS, nullptr},
+    {"mlp_forward", py_mlp_forward, METH_VARARGS, nullptr},
+    {"mlp_grad", py_mlp_grad, METH_VARARGS, nullptr},
+    {"adam_step", py_adam_step, METH_VARARGS, nullptr},
+    {"colstats", py_colstats, METH_VARARGS, nullptr},
     {"complete_workspace_bytes", py_complete_workspace_bytes, METH_VARARGS, nullptr},
     {"complete_columns", py_complete_columns, METH_VARARGS, nullptr},
     {"qr_workspace_bytes", py_qr_workspace_bytes, METH_VARARGS, nullptr},

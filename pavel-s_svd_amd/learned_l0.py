@@ -1,0 +1,173 @@
+"""Learned starting bound l0 ~ sigma_min / sigma_max for QDWH.
+
+No reference counterpart: the reference starts QDWH from l0 = eps(dtype)
+(svdbench/standalone/solvers.py:128-130), which is safe for any input but spends the first
+iterations (the expensive QR-branch ones) lifting singular values the weights were not fitted to.
+Here a 32 -> 64 -> 64 -> 1 tanh MLP predicts log10(sigma_min / |A|_F) -- the smallest singular
+value of QDWH's starting iterate X0 = A / |A|_F, which is what the bound l0 must not exceed -- from
+cheap sketch features of A; the bound handed to QDWH is 10^(prediction - margin), the margin taken
+from held-out residuals so that the bound stays below the true value, clamped to [eps, 1].  QDWH's own convergence test and
+the polar fixed-point check judge the result; the reference's l0 = eps is the fallback.
+
+Features (all passes over A run on the package's kernels: Frobenius norm and column statistics
+kernels, MFMA GEMMs, Householder QR, LDS Jacobi eigensolver):
+    0       log10(|A|_F / (s1 sqrt(n)))         s1 = top Ritz value after four steps
+    1..28   log10(r_t[k] / s1)                  Ritz values k = 1..7 (0-based) after steps t = 0..3
+                                                of subspace iteration on A^T A from a fixed
+                                                orthonormal 8-column Gaussian start
+    29, 30  log10 max/min column norm, std of log10 column norms
+    31      log10(n)
+The network (inference and training) runs on the MLP kernels of ``libpavelsvd.so``.
+"""
+import json
+import math
+import os
+
+import numpy as np
+import torch
+
+from . import native as N
+from .standalone import primitives as P
+from .standalone.blas import gemm
+
+IN, HID = 32, 64
+SKETCH = 8
+OMEGA_SEED = 20261011
+DEFAULT_MODEL = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'l0_model.json')
+_OMEGA = {}
+
+
+def _omega(n, dtype, device):
+    key = (n, dtype, str(device))
+    if key not in _OMEGA:
+        g = np.random.default_rng(OMEGA_SEED).standard_normal((n, SKETCH))
+        q, _ = P.qr(torch.from_numpy(g).to(device=device, dtype=torch.float64)[None])
+        _OMEGA[key] = q[0].to(dtype).contiguous()
+    return _OMEGA[key]
+
+
+def features(a):
+    """(B, m, n) device tensor (m >= n, FP32 or FP64) -> (B, 32) float32 device features."""
+    if a.dim() == 2:
+        a = a.unsqueeze(0)
+    b, m, n = a.shape
+    if m < n:
+        a = a.mT.contiguous()
+        b, m, n = a.shape
+    a = a.contiguous()
+    fro = P.reduce(a, 'fro')
+    cs = torch.empty(b, 5, dtype=torch.float32, device=a.device)
+    N.check(N._hip.colstats(N.dtype_code(a.dtype), b, m, n, N.ptr(a), a.stride(1), a.stride(0) if b > 1 else 0,
+                            N.ptr(cs), N.stream_of(a)), 'psvd_colstats')
+    x = _omega(n, a.dtype, a.device)
+    ritz = []
+    for t in range(4):
+        y = gemm(a, x)                                   # (b, m, 8)
+        w, _, _ = P.jacobi_eigh(gemm(y.mT, y))           # 8 x 8 Gram, ascending
+        ritz.append(w.double().clamp_min(0.0).sqrt().flip(-1))
+        if t < 3:
+            x, _ = P.qr(gemm(a.mT, y))
+    r = torch.stack(ritz, 1)                             # (b, 4, 8)
+    s1 = r[:, 3, :1].clamp_min(1e-30)
+    lg = lambda v: torch.log10(v.clamp_min(1e-30))
+    f = torch.cat([lg(fro[:, None] / (s1 * math.sqrt(n))), lg(r[:, :, 1:] / s1[:, :, None]).reshape(b, 28),
+                   (cs[:, 1] - cs[:, 0]).double()[:, None], cs[:, 3].double()[:, None],
+                   torch.full((b, 1), math.log10(n), dtype=torch.float64, device=a.device)], 1)
+    return f.clamp(-30.0, 30.0).float().contiguous()
+
+
+def glorot(seed=0):
+    """Flat parameter vector: Glorot-uniform weights, zero biases."""
+    rng = np.random.default_rng(seed)
+    u = lambda fo, fi: rng.uniform(-1, 1, (fo, fi)) * math.sqrt(6.0 / (fo + fi))
+    return np.concatenate([u(HID, IN).ravel(), np.zeros(HID), u(HID, HID).ravel(), np.zeros(HID),
+                           u(1, HID).ravel(), [0.0]]).astype(np.float32)
+
+
+class L0Model:
+    """The MLP, its input standardisation and the conservative margin (decades)."""
+
+    def __init__(self, params, mean=None, std=None, margin=1.0, device='cuda:0'):
+        self.device = torch.device(device)
+        self.params = torch.as_tensor(np.asarray(params, np.float32)).to(self.device).contiguous()
+        if self.params.numel() != N._hip.mlp_param_count():
+            raise ValueError('l0 model has the wrong number of parameters')
+        self.mean = torch.as_tensor(np.zeros(IN) if mean is None else mean, dtype=torch.float32).to(self.device)
+        self.std = torch.as_tensor(np.ones(IN) if std is None else std, dtype=torch.float32).to(self.device)
+        self.margin = float(margin)
+
+    @classmethod
+    def load(cls, path=DEFAULT_MODEL, device='cuda:0'):
+        with open(path) as f:
+            d = json.load(f)
+        return cls(d['params'], d['mean'], d['std'], d['margin'], device)
+
+    def save(self, path=DEFAULT_MODEL, **meta):
+        with open(path, 'w') as f:
+            json.dump({'params': self.params.cpu().tolist(), 'mean': self.mean.cpu().tolist(),
+                       'std': self.std.cpu().tolist(), 'margin': self.margin, **meta}, f)
+
+    def _inputs(self, feats):
+        return ((feats.to(self.device) - self.mean) / self.std).contiguous()
+
+    def predict(self, feats):
+        """(B, 32) features -> (B,) predicted log10(sigma_min / |A|_F)."""
+        x = self._inputs(feats)
+        y = torch.empty(x.shape[0], dtype=torch.float32, device=self.device)
+        N.check(N._hip.mlp_forward(N.ptr(self.params), N.ptr(x), x.shape[0], N.ptr(y), N.stream_of(x)),
+                'psvd_mlp_forward')
+        return y
+
+    def lower_bound(self, a, eps):
+        """Conservative l0 for the batch a (the smallest over its matrices), or None when the
+        features are not finite."""
+        pred = self.predict(features(a))
+        lo = float(pred.min()) - self.margin
+        if not math.isfinite(lo):
+            return None
+        return min(1.0, max(eps, 10.0 ** lo))
+
+    def grad(self, feats, target):
+        """Loss and gradient of the standardised batch (for checks): (loss, grad, y)."""
+        x = self._inputs(feats)
+        t = target.to(self.device, torch.float32).contiguous()
+        b = x.shape[0]
+        y = torch.empty(b, dtype=torch.float32, device=self.device)
+        g = torch.empty_like(self.params)
+        loss = torch.empty(1, dtype=torch.float64, device=self.device)
+        wsb = N._hip.mlp_workspace_bytes(b)
+        ws = torch.empty(wsb, dtype=torch.uint8, device=self.device)
+        N.check(N._hip.mlp_grad(N.ptr(self.params), N.ptr(x), N.ptr(t), b, N.ptr(y), N.ptr(g), N.ptr(loss), N.ptr(ws),
+                                wsb, N.stream_of(x)), 'psvd_mlp_grad')
+        return loss, g, y
+
+    def fit(self, feats, target, *, steps=3000, lr=3e-3, val=None, log_every=500):
+        """Full-batch Adam on the GPU kernels; standardisation from the training features.  With
+        ``val = (feats, target)`` the margin becomes the 95th percentile of the held-out
+        over-prediction (at least 0.25 decades).  Returns the loss history."""
+        feats = feats.to(self.device)
+        self.mean = feats.mean(0)
+        self.std = feats.std(0).clamp_min(1e-3)
+        x = self._inputs(feats)
+        t = target.to(self.device, torch.float32).contiguous()
+        b = x.shape[0]
+        y = torch.empty(b, dtype=torch.float32, device=self.device)
+        g = torch.empty_like(self.params)
+        m = torch.zeros_like(self.params)
+        v = torch.zeros_like(self.params)
+        loss = torch.empty(1, dtype=torch.float64, device=self.device)
+        wsb = N._hip.mlp_workspace_bytes(b)
+        ws = torch.empty(wsb, dtype=torch.uint8, device=self.device)
+        st = N.stream_of(x)
+        hist = []
+        for step in range(1, steps + 1):
+            N.check(N._hip.mlp_grad(N.ptr(self.params), N.ptr(x), N.ptr(t), b, N.ptr(y), N.ptr(g), N.ptr(loss),
+                                    N.ptr(ws), wsb, st), 'psvd_mlp_grad')
+            N.check(N._hip.adam_step(N.ptr(self.params), N.ptr(g), N.ptr(m), N.ptr(v), self.params.numel(), lr, 0.9,
+                                     0.999, 1e-8, step, st), 'psvd_adam_step')
+            if step == 1 or step % log_every == 0 or step == steps:
+                hist.append((step, float(loss)))
+        if val is not None:
+            r = (self.predict(val[0]) - val[1].to(self.device, torch.float32)).cpu().numpy()
+            self.margin = max(0.25, float(np.quantile(r, 0.95)))
+        return hist

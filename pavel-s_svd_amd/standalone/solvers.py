@@ -17,6 +17,8 @@ Extension options (no reference counterpart):
                                 default 1) and on V, H = sym(X^T A) in FP64, G = V^T H V finished by
                                 FP64 Jacobi sweeps (stat polish_sweeps), sigma = eig(G), U = X V W
                                 in FP64
+    l0_model=None               QDWH: start from the learned lower bound (an L0Model, a JSON path,
+                                or True for the packaged model); stats l0, l0_source
     eigh_method='auto'          QDWH eigensolver: 'cyclic' (the reference's rotation order),
                                 'blocked' (64 x 64 subproblems + MFMA) or 'auto' (LDS kernel when
                                 H fits, blocked otherwise)
@@ -143,29 +145,16 @@ def _eye(b, n, dtype, device):
     return torch.eye(n, dtype=dtype, device=device).expand(b, n, n).contiguous()
 
 
-def _polar(a, stats, opts):
-    """QDWH polar decomposition, then the Jacobi eigensolve of H = sym(X^T A)
-    (solvers.py:122-177) on the GPU kernels: the QR branch stacks [sqrt(c) X; I] for the blocked
-    Householder QR and folds X' = (b/c) X + (a - b/c)/sqrt(c) Q_top Q_bot^T into one GEMM; the
-    Cholesky branch forms Z = I + c X^T X with one GEMM, factors it and applies X Z^-1 as two
-    right-sided triangular solves.  One host synchronisation per iteration reads the convergence
-    test max|X' - X| <= tol * max(max|X'|, 1) (over the whole batch, as the reference)."""
-    b, m, n = a.shape
-    dt, dev = a.dtype, a.device
-    eps = EPS[dt]
-    mode = _gmode(a, opts)
-    qmode = mode if mode != 'f64' else 'f32'
-    x = P.scale(a, P.reduce(a, 'fro'), invert=True)
-    tol = opts.get('tol') or 10 * eps
-    lower = opts.get('lower_bound', eps)
-    if not 0 < lower <= 1:
-        raise ValueError('lower_bound must be in (0,1]')
-    max_iter = opts.get('max_iter', 40)
-    converged = False
-    it = 0
+def _qdwh_iterate(x, lower, tol, max_iter, mode, qmode):
+    """QDWH iterations from x = A / |A|_F and the lower bound `lower` (solvers.py:131-167);
+    returns (X, iterations, QR-branch iterations).  x is not modified."""
+    b, m, n = x.shape
+    dt, dev = x.dtype, x.device
+    nqr = 0
     for it in range(max_iter):
         aa, bb, cc, lower = qdwh_weights(lower)
         if cc > 100:
+            nqr += 1
             stacked = torch.empty(b, m + n, n, dtype=dt, device=dev)
             P.scale(x, None, s=math.sqrt(cc), out=stacked[:, :m])
             stacked[:, m:] = torch.eye(n, dtype=dt, device=dev)
@@ -190,10 +179,62 @@ def _polar(a, stats, opts):
         test = P.reduce(xn, 'maxdiff', x).amax(0).cpu()   # [max|x' - x|, max|x'|] over the batch
         x = xn
         if float(test[0]) <= tol * max(float(test[1]), 1.0):
-            converged = True
-            break
-    if not converged:
-        raise RuntimeError(f'polar_qdwh did not converge in {max_iter} iterations')
+            return x, it + 1, nqr
+    raise RuntimeError(f'polar_qdwh did not converge in {max_iter} iterations')
+
+
+def _fixed_point_error(x, mode):
+    """max |X (X^T X) - X| over the batch (solvers.py:175-176)."""
+    fp = gemm(x, gemm(x.mT, x, mode=mode), mode=mode)
+    return float(P.reduce(fp, 'maxdiff', x)[:, 0].max())
+
+
+def _polar(a, stats, opts):
+    """QDWH polar decomposition, then the Jacobi eigensolve of H = sym(X^T A)
+    (solvers.py:122-177) on the GPU kernels: the QR branch stacks [sqrt(c) X; I] for the blocked
+    Householder QR and folds X' = (b/c) X + (a - b/c)/sqrt(c) Q_top Q_bot^T into one GEMM; the
+    Cholesky branch forms Z = I + c X^T X with one GEMM, factors it and applies X Z^-1 as two
+    right-sided triangular solves.  One host synchronisation per iteration reads the convergence
+    test max|X' - X| <= tol * max(max|X'|, 1) (over the whole batch, as the reference).
+
+    With ``l0_model`` (an L0Model, a path, or True for the packaged model) and no explicit
+    lower_bound, the iteration starts from the learned bound; a run that raises or ends with a
+    fixed-point error above 100 tol is discarded and repeated from the reference's l0 = eps."""
+    b, m, n = a.shape
+    dt, dev = a.dtype, a.device
+    eps = EPS[dt]
+    mode = _gmode(a, opts)
+    qmode = mode if mode != 'f64' else 'f32'
+    x0 = P.scale(a, P.reduce(a, 'fro'), invert=True)
+    tol = opts.get('tol') or 10 * eps
+    lower = opts.get('lower_bound', eps)
+    if not 0 < lower <= 1:
+        raise ValueError('lower_bound must be in (0,1]')
+    max_iter = opts.get('max_iter', 40)
+    x = None
+    l0_model = opts.get('l0_model')
+    if l0_model is not None and l0_model is not False and 'lower_bound' not in opts:
+        from ..learned_l0 import L0Model, DEFAULT_MODEL
+        model = l0_model if isinstance(l0_model, L0Model) else \
+            L0Model.load(DEFAULT_MODEL if l0_model is True else l0_model, dev)
+        learned = model.lower_bound(a, eps) if min(m, n) >= 16 else None
+        stats['l0_source'] = 'fallback'
+        if learned is not None:
+            try:
+                x, it, nqr = _qdwh_iterate(x0, learned, tol, max_iter, mode, qmode)
+                if _fixed_point_error(x, mode) > 100 * tol:
+                    x = None
+            except RuntimeError:
+                x = None
+            stats['l0_learned'] = learned
+            if x is not None:
+                stats['l0_source'] = 'learned'
+                lower = learned
+    if x is None:
+        x, it, nqr = _qdwh_iterate(x0, lower, tol, max_iter, mode, qmode)
+    stats['polar_qr_iterations'] = nqr
+    if l0_model is not None and l0_model is not False:
+        stats['l0'] = lower
     polish = bool(opts.get('polish', False)) and dt == torch.float32
     if polish:
         # FP64 tail: Newton-Schulz steps make X orthogonal to FP64 rounding, H = sym(X^T A) is
@@ -240,11 +281,10 @@ def _polar(a, stats, opts):
         if bool((status != 0).any()):
             raise RuntimeError('null-space completion failed: supplied live columns are not independent')
     s = torch.where(keep, s, torch.zeros_like(s))
-    fp = gemm(x, gemm(x.mT, x, mode=mode), mode=mode)
     if polish:
         stats['polish_sweeps'] = polish_sweeps
-    stats.update(polar_iterations=it + 1, eigen_sweeps=int(info_h[:, 0].max()),
-                 polar_fixed_point_error=float(P.reduce(fp, 'maxdiff', x)[:, 0].max()))
+    stats.update(polar_iterations=it, eigen_sweeps=int(info_h[:, 0].max()),
+                 polar_fixed_point_error=_fixed_point_error(x, mode))
     return u, s, v
 
 

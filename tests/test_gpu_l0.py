@@ -1,0 +1,127 @@
+"""GPU checks of the learned QDWH lower bound: the MLP kernels' forward pass, gradient and Adam
+step against the FP64 oracle (oracle/mlp.py, itself pinned by finite differences), the sketch
+features against the oracle's, and an ablation -- a network trained on the GPU kernels against the
+reference's l0 = eps start on held-out matrices: fewer QR-branch QDWH iterations (the expensive
+kind) and no more iterations in total, at the same accuracy.
+
+Bars: forward and gradient 1e-4 relative (FP32 kernels vs FP64 oracle); features 2e-3 absolute in
+log10 units (FP32 sketches vs FP64).
+"""
+from importlib import import_module
+
+import numpy as np
+import pytest
+import torch
+
+from conftest import load_package
+from oracle import matrices as X
+from oracle import metrics as M
+from oracle import mlp as O
+
+pytestmark = pytest.mark.gpu
+DEV = 'cuda:0'
+
+
+@pytest.fixture(scope='module')
+def L():
+    load_package()
+    return import_module('pavel-s_svd_amd.learned_l0')
+
+
+def _model(L, seed):
+    rng = np.random.default_rng(seed)
+    return L.L0Model(rng.normal(0, 0.2, O.NPARAM).astype(np.float32), device=DEV)
+
+
+@pytest.mark.parametrize('batch', [1, 15, 16, 17, 100])
+def test_forward_and_gradient_match_oracle(L, batch):
+    m = _model(L, batch)
+    rng = np.random.default_rng(batch + 7)
+    x = rng.normal(size=(batch, O.IN)).astype(np.float32)
+    t = rng.normal(size=batch).astype(np.float32)
+    y = m.predict(torch.from_numpy(x)).cpu().numpy()
+    p = m.params.cpu().numpy().astype(np.float64)
+    yo, _ = O.forward(p, x.astype(np.float64))
+    assert np.max(np.abs(y - yo)) <= 1e-4 * max(1.0, np.abs(yo).max())
+    loss, g, y2 = m.grad(torch.from_numpy(x), torch.from_numpy(t))
+    lo, go = O.loss_grad(p, x.astype(np.float64), t.astype(np.float64))
+    assert abs(float(loss) - lo) <= 1e-4 * max(lo, 1e-3)
+    g = g.cpu().numpy()
+    assert np.max(np.abs(g - go)) <= 1e-4 * np.abs(go).max()
+    assert np.array_equal(y2.cpu().numpy(), y)
+    _, g2, _ = m.grad(torch.from_numpy(x), torch.from_numpy(t))      # deterministic reduction
+    assert np.array_equal(g2.cpu().numpy(), g)
+
+
+def test_adam_step_matches_formula(L):
+    N = import_module('pavel-s_svd_amd.native')
+    rng = np.random.default_rng(3)
+    p, g = rng.normal(size=300).astype(np.float32), rng.normal(size=300).astype(np.float32)
+    m, v = rng.normal(size=300).astype(np.float32) * 0.1, rng.random(300).astype(np.float32) * 0.1
+    tp, tg, tm, tv = (torch.from_numpy(z.copy()).to(DEV) for z in (p, g, m, v))
+    N.check(N._hip.adam_step(N.ptr(tp), N.ptr(tg), N.ptr(tm), N.ptr(tv), 300, 1e-2, 0.9, 0.999, 1e-8, 3,
+                             N.stream_of(tp)), 'adam')
+    # the kernel's constants are FP32: 1 - float32(0.999) = 0.000999987..., not 0.001
+    b1, b2 = np.float32(0.9), np.float32(0.999)
+    m2 = b1 * m + (np.float32(1) - b1) * g
+    v2 = b2 * v + (np.float32(1) - b2) * g * g
+    p2 = p - 1e-2 * (m2 / (1 - b1 ** 3)) / (np.sqrt(v2 / (1 - b2 ** 3)) + 1e-8)
+    assert np.allclose(tm.cpu().numpy(), m2, rtol=1e-6, atol=1e-8)
+    assert np.allclose(tv.cpu().numpy(), v2, rtol=1e-6, atol=1e-10)
+    assert np.allclose(tp.cpu().numpy(), p2, rtol=1e-5, atol=1e-7)
+
+
+@pytest.mark.parametrize('dtype', [np.float32, np.float64])
+def test_features_match_oracle(L, dtype):
+    a, _, _ = X.workload_l0(4, 96, seed=11, cond_lo=1, cond_hi=6)
+    f = L.features(torch.from_numpy(a.astype(dtype)).to(DEV)).cpu().numpy()
+    fo = np.stack([O.features(ai.astype(dtype)) for ai in a])
+    assert f.shape == (4, 32)
+    assert np.max(np.abs(f - fo)) <= (2e-3 if dtype == np.float32 else 1e-6)
+    tall = np.random.default_rng(2).normal(size=(2, 150, 40)).astype(dtype)
+    ft = L.features(torch.from_numpy(tall).to(DEV)).cpu().numpy()
+    assert np.max(np.abs(ft - np.stack([O.features(t) for t in tall]))) <= (2e-3 if dtype == np.float32 else 1e-6)
+
+
+def test_learned_l0_ablation(L):
+    """Train on 448 matrices (64 x 64, log10 condition in [1, 6], geometric or clustered spectra,
+    targets log10(sigma_min / |A|_F) -- the smallest singular value of QDWH's start X0 = A / |A|_F
+    -- from the plain batched Jacobi solver), keep 64 for the margin, and solve 16 held-out matrices
+    with QDWH from the learned bound and from eps."""
+    pkg = load_package()
+    a, conds, _ = X.workload_l0(528, 64, seed=21, cond_lo=1, cond_hi=6)
+    at = torch.from_numpy(a).to(DEV)
+    # targets: the plain solver's sigma_min / sigma_max, kept down to the eps * sigma_1 floor (the
+    # reference cut eps * max(m, n) * sigma_1 would zero the smallest values of this workload)
+    _, s, _ = pkg.svd(at, 'jacobi', values_only=True, cut='floor')
+    s = s.double()
+    target = torch.log10((s[:, -1] / s.square().sum(1).sqrt()).clamp_min(1e-8)).float()
+    feats = L.features(at)
+    model = L.L0Model(L.glorot(0), device=DEV)
+    hist = model.fit(feats[:448], target[:448], steps=2000, lr=3e-3, val=(feats[448:512], target[448:512]))
+    assert hist[-1][1] < 0.05 * hist[0][1]
+    pred = model.predict(feats[512:]).cpu().numpy()
+    true = target[512:].cpu().numpy()
+    print('mean |error| (decades)', np.mean(np.abs(pred - true)), 'margin', model.margin)
+    assert np.mean(np.abs(pred - true)) < 0.5
+    it_plain, it_learned, qr_plain, qr_learned = [], [], [], []
+    for i in range(512, 528):
+        u0, s0, vh0, i0 = pkg.svd(at[i], 'polar_qdwh', return_info=True)
+        u1, s1, vh1, i1 = pkg.svd(at[i], 'polar_qdwh', return_info=True, l0_model=model)
+        it_plain.append(i0['polar_iterations'])
+        it_learned.append(i1['polar_iterations'])
+        qr_plain.append(i0['polar_qr_iterations'])
+        qr_learned.append(i1['polar_qr_iterations'])
+        assert i1['l0_source'] in ('learned', 'fallback')
+        ref = M.reference(a[i])
+        m0 = M.validate(a[i], (u0.cpu().numpy(), s0.cpu().numpy(), vh0.cpu().numpy()), ref)['worst']
+        m1 = M.validate(a[i], (u1.cpu().numpy(), s1.cpu().numpy(), vh1.cpu().numpy()), ref)['worst']
+        for k in ('spectrum_error_over_sigma1', 'orthogonality_u_fro_over_sqrt_cols', 'reconstruction_relative_fro'):
+            assert m1[k] <= max(2 * m0[k], 2e-5), (i, k, m0[k], m1[k])
+    print('iterations plain', it_plain, 'learned', it_learned)
+    print('QR-branch iterations plain', qr_plain, 'learned', qr_learned)
+    # QDWH converges cubically, so a tighter start saves at most one iteration in total; what it
+    # saves is the expensive QR-branch steps (c > 100), for sigma_min(X0) above about 1e-5
+    assert sum(l > p for l, p in zip(it_learned, it_plain)) <= 1       # the margin is a 95% bound
+    assert np.sum(qr_learned) <= np.sum(qr_plain) - 8
+    assert np.sum(it_learned) <= np.sum(it_plain)
