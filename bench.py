@@ -1,15 +1,17 @@
-"""Benchmark: batched uneven one-sided Jacobi SVD on one MI355X.
+"""Benchmark: batched uneven one-sided Jacobi SVD on one MI355X (the headline line), plus the
+single-matrix, QDWH, TSQR and learned-l0 workloads under "configs".
 
 Workload: 4096 float32 matrices, m ~ U{16..96}, n ~ U{8..m}, entries i.i.d. N(0,1) (seed
 20261011), packed row-major in HBM.  One step = one solve of the whole batch: a single
 hipGraph replay of the persistent LDS-resident Jacobi launch (device work queue, FP64 dot
 products and convergence test, FP64 sigma).  Prints one JSON line.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-check]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-check] [--only c2]
 """
 import argparse
 import importlib
 import json
+import math
 import os
 import sys
 import time
@@ -78,6 +80,135 @@ def cpu_baseline(flat, ms, ns, offs, budget_s=15.0):
                       f'{dt:.1f} s on 1 host core (numpy, FP32 arithmetic as the reference)'}
 
 
+def _haar64(n, k, gen, dev):
+    """n x k Haar columns: Householder QR (this package's FP64 kernel) of seeded N(0, 1), R-diagonal
+    sign fix."""
+    P = importlib.import_module('pavel-s_svd_amd.standalone.primitives')
+    g = torch.randn(1, n, k, generator=gen, device=dev, dtype=torch.float64)
+    q, r = P.qr(g)
+    return (q * torch.sign(torch.diagonal(r, dim1=1, dim2=2))[:, None, :])[0]
+
+
+def _product(u, s, v):
+    gemm = importlib.import_module('pavel-s_svd_amd.standalone.blas').gemm
+    return gemm((u * s[None, :]).contiguous(), v.mT, mode='f64')
+
+
+def _orth(u):
+    """|U^T U - I|_F / sqrt(k) in FP64 on the GPU."""
+    gemm = importlib.import_module('pavel-s_svd_amd.standalone.blas').gemm
+    u = u.double()
+    g = gemm(u.mT, u, mode='f64')
+    g.diagonal().sub_(1.0)
+    return float(g.norm() / math.sqrt(u.shape[1]))
+
+
+def _timed(fn, reps):
+    torch.cuda.synchronize()
+    out = fn()                                          # warm-up (also returns the result)
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        out = fn()
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts)), out
+
+
+def config_single(pkg, dev):
+    """512 x 256 float32, Haar U and V, sigma geometric 1 .. 1e-6; one-sided Jacobi at tol 1e-7."""
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(SEED)
+    s = torch.from_numpy(np.geomspace(1.0, 1e-6, 256)).to(dev)
+    a = _product(_haar64(512, 256, gen, dev), s, _haar64(256, 256, gen, dev)).float()
+    res = {}
+    for cut in ('reference', 'floor'):
+        t, (u, sv, vh, info) = _timed(lambda: pkg.svd(a, 'jacobi', tol=1e-7, cut=cut, return_info=True), 3)
+        res[cut] = {'seconds': t, 'jacobi_sweeps': info['jacobi_sweeps'],
+                    'spectrum_error_over_sigma1': float((sv.double() - s).abs().max()),
+                    'orthogonality_u': _orth(u), 'orthogonality_v': _orth(vh.mT)}
+    return {'workload': '512x256 float32, Haar U/V, sigma geomspace(1, 1e-6); jacobi tol 1e-7 (the cut '
+                        "'reference' zeroes sigma < eps*max(m,n)*sigma_1 = 6.1e-5 by the reference's contract)",
+            'unit': 's/solve', 'results': res}
+
+
+def config_qdwh(pkg, dev, n=4096):
+    """n x n float32, Haar U and V, sigma log-uniform on [1e-6, 1] with 10% in a width-1e-4 cluster
+    at 0.5: the MI355X form (BF16x3 QR-branch products, FP64 polish) against the plain FP32 form."""
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(SEED)
+    rng = np.random.default_rng(SEED)
+    k = int(round(0.1 * n))
+    sig = np.sort(np.concatenate([np.exp(rng.uniform(np.log(1e-6), 0.0, n - k)),
+                                  0.5 + 1e-4 * (rng.random(k) - 0.5)]))[::-1].copy()
+    sig[0] = 1.0
+    s = torch.from_numpy(sig).to(dev)
+    a = _product(_haar64(n, n, gen, dev), s, _haar64(n, n, gen, dev)).float()
+    forms = {'mi355x': dict(gemm='bf16x3', polish=True, cut='floor'), 'plain_fp32': dict(gemm='f32', cut='floor')}
+    res = {}
+    for name, opts in forms.items():
+        t, (u, sv, vh, info) = _timed(lambda: pkg.svd(a, 'polar_qdwh', return_info=True, **opts), 1)
+        res[name] = {'seconds': t, 'options': opts, 'polar_iterations': info['polar_iterations'],
+                     'polar_qr_iterations': info.get('polar_qr_iterations'), 'eigen_sweeps': info['eigen_sweeps'],
+                     'spectrum_error_over_sigma1': float((sv.double() - s).abs().max()),
+                     'orthogonality_u': _orth(u), 'orthogonality_v': _orth(vh.mT)}
+    return {'workload': f'QDWH polar SVD, {n}x{n} float32, Haar U/V, sigma log-uniform [1e-6, 1] with 10% in a '
+                        'width-1e-4 cluster at 0.5; sigma error against the generating spectrum',
+            'unit': 's/solve', 'results': res,
+            'speedup_vs_plain_fp32': res['plain_fp32']['seconds'] / res['mi355x']['seconds']}
+
+
+def config_tsqr(pkg, dev):
+    """8192 x 2048 float32: rank-64 G1 diag(sigma) G2^T (orthonormal factors, sigma geometric 1 ..
+    1e-3) plus N(0, 1e-8) noise; TSQR with 256-row panels, then Jacobi on R."""
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(SEED)
+    s = torch.from_numpy(np.geomspace(1.0, 1e-3, 64)).to(dev)
+    a = _product(_haar64(8192, 64, gen, dev), s, _haar64(2048, 64, gen, dev))
+    a = (a + 1e-4 * torch.randn(8192, 2048, generator=gen, device=dev, dtype=torch.float64)).float()
+    gemm = importlib.import_module('pavel-s_svd_amd.standalone.blas').gemm
+    res = {}
+    for name, opts in (('mi355x', dict(gemm='bf16x3', polish=True)), ('plain_fp32', dict(gemm='f32'))):
+        t, (u, sv, vh, info) = _timed(lambda: pkg.svd(a, 'tsqr', tile_rows=256, return_info=True, **opts), 1)
+        a64 = a.double()
+        resid = gemm((u.double() * sv.double()[None, :]).contiguous(), vh.double(), mode='f64') - a64
+        res[name] = {'seconds': t, 'options': opts, 'tsqr_leaves': info.get('tsqr_leaves'),
+                     'jacobi_sweeps': info['jacobi_sweeps'],
+                     'reconstruction_relative_fro': float(resid.norm() / a64.norm()),
+                     'orthogonality_u': _orth(u), 'orthogonality_v': _orth(vh.mT)}
+    return {'workload': 'TSQR, 8192x2048 float32, rank-64 (sigma 1 .. 1e-3) plus N(0, 1e-8) noise (its |E|_2 ~ '
+                        '0.014 sets the spectrum, so accuracy is reconstruction and orthogonality in FP64), 256-row '
+                        'panels (2048-row leaves: rows = max(tile_rows, n)), Jacobi on R', 'unit': 's/solve',
+            'results': res}
+
+
+def config_l0(pkg, dev, count=32):
+    """The learned QDWH lower bound on `count` matrices of its test split (1024 x 1024)."""
+    T = importlib.import_module('pavel-s_svd_amd.train_l0')
+    L = importlib.import_module('pavel-s_svd_amd.learned_l0')
+    if not os.path.exists(L.DEFAULT_MODEL):
+        return {'skipped': 'no trained model (pavel-s_svd_amd/l0_model.json)'}
+    with open(L.DEFAULT_MODEL) as f:
+        ev = json.load(f).get('evaluation', {})
+    model = L.L0Model.load()
+    n, start = ev.get('n', 1024), ev.get('train', 1280) + ev.get('holdout', 256)
+    rows = []
+    for i0 in range(start, start + count, T.CHUNK):
+        a = T.generate(i0, min(i0 + T.CHUNK, start + count), n)
+        for j in range(a.shape[0]):
+            T.polar_time(a[j:j + 1], None)                  # warm
+            rows.append((T.polar_time(a[j:j + 1], None), T.polar_time(a[j:j + 1], model)))
+    arr = np.array([[p[0], p[1], p[2], q[0], q[1], q[2]] for p, q in rows])
+    return {'workload': f'QDWH polar phase on {count} test matrices {n}x{n} of the learned-l0 workload (log10 '
+                        'cond ~ U[1, 12], geometric or clustered spectra); learned time includes features and '
+                        'inference', 'unit': 's/solve',
+            'plain_seconds': float(arr[:, 0].mean()), 'learned_seconds': float(arr[:, 3].mean()),
+            'plain_iterations': float(arr[:, 1].mean()), 'learned_iterations': float(arr[:, 4].mean()),
+            'plain_qr_iterations': float(arr[:, 2].mean()), 'learned_qr_iterations': float(arr[:, 5].mean()),
+            'model_evaluation': ev}
+
+
 def load_traffic():
     path = os.path.join(ROOT, 'profiles', 'jacobi_lds_uneven4096_traffic.json')
     if os.path.exists(path):
@@ -94,6 +225,7 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-check', action='store_true')
     ap.add_argument('--no-tune', action='store_true', help='use the default launch-shape policy')
+    ap.add_argument('--only', default='all', help="'c2' for the batched-Jacobi line alone")
     ap.add_argument('--copy-calibration', action='store_true',
                     help='only run a plain 256 MiB device-to-device copy 5 times (the reference kernel for '
                          'scaling FETCH_SIZE / WRITE_SIZE counter readings to bytes) and exit')
@@ -156,6 +288,15 @@ def main():
         out['accuracy_vs_fp64_lapack'] = accuracy(batch, flat, ms, ns, offs)
     if not args.no_cpu:
         out['cpu_baseline'] = cpu_baseline(flat, ms, ns, offs)
+    if args.only != 'c2':
+        del batch
+        torch.cuda.empty_cache()
+        out['configs'] = {}
+        for name, fn in (('single_512x256', config_single), ('qdwh_4096', config_qdwh),
+                         ('tsqr_8192x2048', config_tsqr), ('learned_l0_1024', config_l0)):
+            t0 = time.perf_counter()
+            out['configs'][name] = fn(pkg, dev)
+            print(f'{name} done in {time.perf_counter() - t0:.1f} s', file=sys.stderr, flush=True)
     print(json.dumps(out), flush=True)
 
 

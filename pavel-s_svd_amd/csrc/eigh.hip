@@ -518,14 +518,26 @@ __global__ __launch_bounds__(256) void blk_inner_kernel(BlkParams<T> P, int r, i
   if (p == 0 && tid == 0) P.slot[(g + 2) % 3] = 0ull;
   const double scale = __longlong_as_double((long long)P.slot[g % 3]);
   const double tol = P.tol > 0.0 ? P.tol : 10.0 * Eps<T>::value;
+  double offmax = 0.0;
   for (int e = tid; e < kEs * kEs; e += 256) {
     const int i = e / kEs, j = e - i * kEs;
-    S[i * kEsLd + j] = P.G[(int64_t)blk_row(I, J, i) * P.N + blk_row(I, J, j)];
+    const T x = P.G[(int64_t)blk_row(I, J, i) * P.N + blk_row(I, J, j)];
+    S[i * kEsLd + j] = x;
     Qt[i * kEsLd + j] = i == j ? T(1) : T(0);
+    if (i != j) offmax = fmax(offmax, fabs((double)x));
+  }
+  // a subproblem whose off-diagonal is already below the threshold rotates nowhere in its sweep
+  // (every pair test fails and nothing changes): skip it
+  const int live = __syncthreads_or(offmax > tol * scale);
+  int any = 0;
+  __shared__ unsigned short tri[kEb * (kEb + 1) / 2];     // upper-triangle item -> (k1 << 8) | k2
+  for (int e = tid; e < kEb * (kEb + 1) / 2; e += 256) {
+    int k1, k2;
+    tri_decode(e, kEb, k1, k2);
+    tri[e] = (unsigned short)((k1 << 8) | k2);
   }
   __syncthreads();
-  int any = 0;
-  for (int sw = 0; sw < P.inner_sweeps; ++sw) {
+  for (int sw = 0; live && sw < P.inner_sweeps; ++sw) {
     int act_sw = 0;
     for (int rr = 0; rr < kEs - 1; ++rr) {
       if (tid < kEb) {
@@ -542,8 +554,7 @@ __global__ __launch_bounds__(256) void blk_inner_kernel(BlkParams<T> P, int r, i
       }
       __syncthreads();
       for (int e = tid; e < kEb * (kEb + 1) / 2; e += 256) {
-        int k1, k2;
-        tri_decode(e, kEb, k1, k2);
+        const int k1 = tri[e] >> 8, k2 = tri[e] & 0xff;
         const double s1 = ss[k1], s2 = ss[k2];
         if (s1 != 0.0 || s2 != 0.0)
           block_update(S, kEsLd, pp[k1], qq[k1], pp[k2], qq[k2], cc[k1], s1, cc[k2], s2, s1 != 0.0, s2 != 0.0);

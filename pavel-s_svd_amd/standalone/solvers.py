@@ -13,7 +13,8 @@ Extension options (no reference counterpart):
     lanes, threads              Jacobi launch shape (see ``include/pavelsvd.h``)
     gemm='f32' | 'bf16x3'       arithmetic of the FP32 QR / TSQR products and of the QR-branch QDWH
                                 iterations on MFMA (QDWH's Cholesky-branch steps stay FP32)
-    polish=False                QDWH, FP32 data: FP64 Newton-Schulz step(s) on X (polish_steps,
+    polish=False                qr / tsqr, FP32 data: one FP64 Newton-Schulz step on U and on V.
+                                QDWH, FP32 data: FP64 Newton-Schulz step(s) on X (polish_steps,
                                 default 1) and on V, H = sym(X^T A) in FP64, G = V^T H V finished by
                                 FP64 Jacobi sweeps (stat polish_sweeps), sigma = eig(G), U = X V W
                                 in FP64
@@ -304,7 +305,15 @@ def _raw(x, method, stats, opts):
         else:
             q, r = _tsqr(x, opts.get('tile_rows', 4 * n), stats, opts)
         u, s, vh = _jacobi(r, stats, opts, scale=False)
-        return (None if values else gemm(q, u, mode=mode)), s, vh
+        if values:
+            return None, s, vh
+        u = gemm(q, u, mode=mode)
+        if opts.get('polish', False) and x.dtype == torch.float32:
+            # Q's orthogonality is that of its products (about 2^-16 with BF16x3) and V's that of
+            # the FP32 sweeps: one FP64 Newton-Schulz step on each restores it
+            u = _newton_schulz(u.double()).float()
+            vh = _newton_schulz(vh.mT.double()).mT.float().contiguous()
+        return u, s, vh
     if method == 'polar_qdwh':
         u, s, v = _polar(x, stats, opts)
         return u, s, v.mT
