@@ -191,6 +191,19 @@ int psvd_jacobi_grid(int dtype, int m, int n, const void *a, void *u, void *s, v
                      psvd_stream_t stream);
 
 /* ---------------------------------------------------------------------------------------
+ * Blocked one-sided Jacobi SVD of ONE large tall matrix (primitives.py:316-367 with the rotations
+ * grouped): columns in blocks of 32, a tournament over the blocks, for each block pair the 64 x 64
+ * Gram matrix in FP64 on MFMA swept by `inner_sweeps` (>= 1) cyclic sweeps of the reference's
+ * one-sided rotations and test, the accumulated rotation applied to W and V on MFMA; stops after a
+ * sweep without rotations.  Same flags, outputs and info as psvd_jacobi_grid (info[0] counts block
+ * sweeps).  One host read per sweep (not capturable).
+ * ------------------------------------------------------------------------------------- */
+size_t psvd_jacobi_blocked_workspace_bytes(int dtype, int m, int n);
+int psvd_jacobi_blocked(int dtype, int m, int n, const void *a, void *u, void *s, void *vh, int32_t *info, double tol,
+                        int max_sweeps, int flags, int inner_sweeps, void *workspace, size_t workspace_bytes,
+                        psvd_stream_t stream);
+
+/* ---------------------------------------------------------------------------------------
  * Learned QDWH lower bound (no reference counterpart: the reference starts QDWH from l0 = eps,
  * solvers.py:128-130).  A 32 -> 64 -> 64 -> 1 tanh MLP, FP32, parameters in one flat vector of
  * psvd_mlp_param_count() floats: W1 (64 x 32), b1, W2 (64 x 64), b2, w3 (64), b3.

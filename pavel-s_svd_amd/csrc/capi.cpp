@@ -66,6 +66,27 @@ static PyObject *py_jacobi_grid(PyObject *, PyObject *args) {
   return PyLong_FromLong(st);
 }
 
+static PyObject *py_jacobi_blocked_workspace_bytes(PyObject *, PyObject *args) {
+  int dtype, m, n;
+  if (!PyArg_ParseTuple(args, "iii", &dtype, &m, &n)) return nullptr;
+  return PyLong_FromSize_t(psvd_jacobi_blocked_workspace_bytes(dtype, m, n));
+}
+
+static PyObject *py_jacobi_blocked(PyObject *, PyObject *args) {
+  int dtype, m, n, max_sweeps, flags, inner;
+  unsigned long long a, u, s, vh, info, ws, ws_bytes, stream;
+  double tol;
+  if (!PyArg_ParseTuple(args, "iiiKKKKKdiiiKKK", &dtype, &m, &n, &a, &u, &s, &vh, &info, &tol, &max_sweeps, &flags,
+                        &inner, &ws, &ws_bytes, &stream))
+    return nullptr;
+  int st;
+  Py_BEGIN_ALLOW_THREADS
+  st = psvd_jacobi_blocked(dtype, m, n, PTR(a), PTR(u), PTR(s), PTR(vh), static_cast<int32_t *>(PTR(info)), tol,
+                           max_sweeps, flags, inner, PTR(ws), (size_t)ws_bytes, PTR(stream));
+  Py_END_ALLOW_THREADS
+  return PyLong_FromLong(st);
+}
+
 static PyObject *py_gemm(PyObject *, PyObject *args) {
   int mode, ta, tb;
   long long m, n, k, lda, sa, ldb, sb, ldc, sc, batch;
@@ -300,6 +321,8 @@ static PyMethodDef kMethods[] = {
     {"jacobi_lds", py_jacobi_lds, METH_VARARGS, nullptr},
     {"jacobi_grid_workspace_bytes", py_jacobi_grid_workspace_bytes, METH_VARARGS, nullptr},
     {"jacobi_grid", py_jacobi_grid, METH_VARARGS, nullptr},
+    {"jacobi_blocked_workspace_bytes", py_jacobi_blocked_workspace_bytes, METH_VARARGS, nullptr},
+    {"jacobi_blocked", py_jacobi_blocked, METH_VARARGS, nullptr},
     {"gemm", py_gemm, METH_VARARGS, nullptr},
     {"eigh_workspace_bytes", py_eigh_workspace_bytes, METH_VARARGS, nullptr},
     {"eigh", py_eigh, METH_VARARGS, nullptr},

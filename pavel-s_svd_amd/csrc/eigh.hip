@@ -49,30 +49,6 @@ __host__ __device__ inline size_t eigh_lds_bytes(int n) {
   return (b + 15) & ~size_t(15);
 }
 
-// Phase-2 work item: the 2x2 block (pair k1 rows) x (pair k2 cols), k1 <= k2, and its mirror.
-template <typename T>
-__device__ __forceinline__ double block_update(T *G, int ld, int p1, int q1, int p2, int q2, double c1, double s1,
-                                               double c2, double s2, bool act1, bool act2) {
-  T *a11 = G + p1 * ld + p2, *a12 = G + p1 * ld + q2, *a21 = G + q1 * ld + p2, *a22 = G + q1 * ld + q2;
-  double b11 = (double)*a11, b12 = (double)*a12, b21 = (double)*a21, b22 = (double)*a22;
-  if (act2) {  // columns (p2, q2): col_p = c col_p - s col_q, col_q = s col_p + c col_q
-    const double t11 = c2 * b11 - s2 * b12, t12 = s2 * b11 + c2 * b12;
-    const double t21 = c2 * b21 - s2 * b22, t22 = s2 * b21 + c2 * b22;
-    b11 = (double)(T)t11; b12 = (double)(T)t12; b21 = (double)(T)t21; b22 = (double)(T)t22;
-  }
-  if (act1) {  // rows (p1, q1): row_p = c row_p - s row_q, row_q = s row_p + c row_q
-    const double t11 = c1 * b11 - s1 * b21, t21 = s1 * b11 + c1 * b21;
-    const double t12 = c1 * b12 - s1 * b22, t22 = s1 * b12 + c1 * b22;
-    b11 = t11; b12 = t12; b21 = t21; b22 = t22;
-  }
-  const T n11 = (T)b11, n12 = (T)b12, n21 = (T)b21, n22 = (T)b22;
-  *a11 = n11; *a12 = n12; *a21 = n21; *a22 = n22;
-  if (p1 != p2) {  // mirror block (pair k2 rows) x (pair k1 cols) is the transpose
-    G[p2 * ld + p1] = n11; G[q2 * ld + p1] = n12; G[p2 * ld + q1] = n21; G[q2 * ld + q1] = n22;
-  }
-  return fmax(fmax(fabs((double)n11), fabs((double)n12)), fmax(fabs((double)n21), fabs((double)n22)));
-}
-
 template <typename T>
 __device__ __forceinline__ void pair_params(const T *G, int ld, int p, int q, double tol, double scale, double &c,
                                             double &s, bool &active, double &ratio) {

@@ -1,0 +1,511 @@
+// K2b: blocked one-sided Jacobi SVD of one large tall matrix (m x n, m >= n).
+//
+// The columns of W (m x N, N = n padded to a multiple of 64 with zero columns that never rotate)
+// are grouped in blocks of 32; round r of a tournament over the nb blocks pairs block I with block J.
+// For each pair p the 64 x 64 Gram matrix S_p = [W_I W_J]^T [W_I W_J] is formed in FP64 on MFMA
+// (row slices of W, per-slice partials added in a fixed order), and S_p is swept by `inner_sweeps`
+// cyclic sweeps of two-sided plane rotations in LDS -- the rotation of pair (a, b) of S is exactly
+// the one-sided rotation of columns a and b of W (app = s_aa, aqq = s_bb, apq = s_ab), with the
+// reference's test apq^2 > tol^2 app aqq and min(app, aqq) > eps^2 max_j |w_j|^2
+// (primitives.py:337-338; the column-norm maximum is taken at the start of each sweep).  The
+// accumulated 64 x 64 rotation Q_p then updates W[:, I u J] and V[:, I u J] on MFMA.  A sweep without
+// a rotation ends the iteration (primitives.py:352).  The tail -- FP64 sigma, descending order, cut
+// at eps max(m, n) sigma_1 (or eps sigma_1), U = W / sigma, coordinate-vector completion, row-major
+// outputs -- follows primitives.py:356-366 and :430-455.
+//
+// Work per round: 2 m 64^2 FP64 MFMA flops per pair for the Gram, 2 (m + N) 64^2 for the updates;
+// nb - 1 rounds per sweep.
+#include <float.h>
+
+#include "common.hpp"
+
+namespace psvd {
+
+constexpr int kJb = 32, kJs = 64, kJsLd = kJs + 1, kJbThreads = 256;
+
+typedef float j4f __attribute__((ext_vector_type(4)));
+typedef double j4d __attribute__((ext_vector_type(4)));
+
+template <typename T>
+struct BjParams {
+  const T *a;
+  T *u, *s, *vh;
+  int32_t *info;
+  int m, n, N, nb, splits, inner_sweeps, flags;
+  double tol;
+  T *W, *V, *Q;             // W: m x N, V: N x N (row-major, ld N); Q: npair x 64 x 64 (W_IJ <- W_IJ Q)
+  double *Sp;               // npair x splits x 64 x 64 Gram partials
+  int *act;                 // npair
+  unsigned long long *slot;  // [0] max|a| bits, [1] max column norm^2 bits (this sweep)
+  int *flag;                // [0] rotation in this sweep, [1] non-finite input, [2] completion status
+  double *sig;              // N
+  int *ord;                 // N
+  int32_t *nk;              // 1
+};
+
+__device__ __forceinline__ int bj_col(int I, int J, int i) { return i < kJb ? I * kJb + i : J * kJb + (i - kJb); }
+
+template <typename T>
+__global__ __launch_bounds__(kJbThreads) void bj_absmax_kernel(BjParams<T> P) {
+  double mx = 0.0;
+  int bad = 0;
+  for (int64_t e = (int64_t)blockIdx.x * kJbThreads + threadIdx.x; e < (int64_t)P.m * P.n;
+       e += (int64_t)gridDim.x * kJbThreads) {
+    const double x = (double)P.a[e];
+    bad |= !isfinite(x);
+    mx = fmax(mx, fabs(x));
+  }
+  mx = wave_max_fast(mx);
+  const bool wave_bad = __any(bad);
+  if ((threadIdx.x & (kWave - 1)) == 0) {
+    atomicMax(&P.slot[0], (unsigned long long)__double_as_longlong(mx));
+    if (wave_bad) atomicOr(&P.flag[1], 1);
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kJbThreads) void bj_load_kernel(BjParams<T> P) {
+  const double mx = __longlong_as_double((long long)P.slot[0]);
+  const double sc = (P.flags & PSVD_SCALE) && mx > 0.0 && isfinite(mx) ? mx : 1.0;
+  const int64_t total = (int64_t)P.m * P.N + (int64_t)P.N * P.N;
+  for (int64_t e = (int64_t)blockIdx.x * kJbThreads + threadIdx.x; e < total; e += (int64_t)gridDim.x * kJbThreads) {
+    if (e < (int64_t)P.m * P.N) {
+      const int i = (int)(e / P.N), j = (int)(e - (int64_t)i * P.N);
+      // T division, as the reference's x / scale
+      P.W[e] = j < P.n ? (T)(P.a[(int64_t)i * P.n + j] / (T)sc) : T(0);
+    } else {
+      const int64_t f = e - (int64_t)P.m * P.N;
+      const int i = (int)(f / P.N), j = (int)(f - (int64_t)i * P.N);
+      P.V[f] = i == j ? T(1) : T(0);
+    }
+  }
+}
+
+// max_j |w_j|^2 for the floor of this sweep (slot[1] cleared before the launch)
+template <typename T>
+__global__ __launch_bounds__(kJbThreads) void bj_colmax_kernel(BjParams<T> P) {
+  const int c = blockIdx.x * kJbThreads + threadIdx.x;
+  double acc = 0.0;
+  if (c < P.n)
+    for (int i = 0; i < P.m; ++i) {
+      const double x = (double)P.W[(int64_t)i * P.N + c];
+      acc = fma(x, x, acc);
+    }
+  acc = wave_max_fast(acc);
+  if ((threadIdx.x & (kWave - 1)) == 0) atomicMax(&P.slot[1], (unsigned long long)__double_as_longlong(acc));
+}
+
+// Gram partial of pair p over the rows of slice z: 64 x 64, FP64 MFMA (16x16x4), wave w owns the
+// 32 x 32 quadrant (w / 2, w % 2).
+template <typename T>
+__global__ __launch_bounds__(kJbThreads) void bj_gram_kernel(BjParams<T> P, int r) {
+  constexpr int KR = 64;                          // rows staged per step
+  __shared__ double Ws[KR][kJs + 1];
+  const int p = blockIdx.x, z = blockIdx.y, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lr = lane & 15, lk = lane >> 4, wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  int I, J;
+  rr_pair(r, p, P.nb, I, J);
+  const int rows_per = (P.m + P.splits - 1) / P.splits;
+  const int r0 = z * rows_per, r1 = r0 + rows_per < P.m ? r0 + rows_per : P.m;
+  j4d acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) acc[i][j][q] = 0.0;
+  for (int k0 = r0; k0 < r1; k0 += KR) {
+    __syncthreads();
+    for (int e = tid; e < KR * kJs; e += kJbThreads) {
+      const int rr = e / kJs, c = e - rr * kJs, row = k0 + rr;
+      Ws[rr][c] = row < r1 ? (double)P.W[(int64_t)row * P.N + bj_col(I, J, c)] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int k = 0; k < KR; k += 4) {
+      double a[2], b[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) a[i] = Ws[k + lk][wm + 16 * i + lr];   // A = W^T: A[col][row]
+#pragma unroll
+      for (int j = 0; j < 2; ++j) b[j] = Ws[k + lk][wn + 16 * j + lr];
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+  }
+  double *out = P.Sp + ((size_t)p * P.splits + z) * kJs * kJs;
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int q = 0; q < 4; ++q)   // FP64 MFMA D layout: register q of lane l is row 4 q + l / 16
+        out[(wm + 16 * i + 4 * q + lk) * kJs + wn + 16 * j + lr] = acc[i][j][q];
+}
+
+template <typename T>
+__host__ __device__ inline size_t bj_inner_lds() {
+  return sizeof(double) * (2 * kJb + 2 * (size_t)kJs * kJsLd) + sizeof(int) * 2 * kJb;
+}
+
+// Sweeps of S_p (FP64, in LDS) with the one-sided test; Q_p = accumulated rotation.
+template <typename T>
+__global__ __launch_bounds__(kJbThreads) void bj_inner_kernel(BjParams<T> P, int r) {
+  extern __shared__ double smem_bj[];
+  double *cc = smem_bj, *ss = cc + kJb;
+  double *S = ss + kJb, *Qt = S + kJs * kJsLd;
+  int *pp = reinterpret_cast<int *>(Qt + kJs * kJsLd), *qq = pp + kJb;
+  __shared__ unsigned short tri[kJb * (kJb + 1) / 2];
+  const int p = blockIdx.x, tid = threadIdx.x;
+  const double eps = Eps<T>::value, tol = P.tol > 0.0 ? P.tol : 10.0 * eps;
+  const double floor_ = eps * eps * __longlong_as_double((long long)P.slot[1]);
+  for (int e = tid; e < kJs * kJs; e += kJbThreads) {
+    const int i = e / kJs, j = e - i * kJs;
+    const double *sp = P.Sp + (size_t)p * P.splits * kJs * kJs + e;
+    double acc = sp[0];
+    for (int z = 1; z < P.splits; ++z) acc += sp[(size_t)z * kJs * kJs];
+    S[i * kJsLd + j] = acc;
+    Qt[i * kJsLd + j] = i == j ? 1.0 : 0.0;
+  }
+  for (int e = tid; e < kJb * (kJb + 1) / 2; e += kJbThreads) {
+    int a = (int)((2.0 * kJb + 1.0 - sqrt((2.0 * kJb + 1.0) * (2.0 * kJb + 1.0) - 8.0 * e)) / 2.0);
+    if (a < 0) a = 0;
+    if (a > kJb - 1) a = kJb - 1;
+    while (a > 0 && a * kJb - a * (a - 1) / 2 > e) --a;
+    while (a < kJb - 1 && (a + 1) * kJb - (a + 1) * a / 2 <= e) ++a;
+    tri[e] = (unsigned short)((a << 8) | (a + e - (a * kJb - a * (a - 1) / 2)));
+  }
+  __syncthreads();
+  int any = 0;
+  for (int sw = 0; sw < P.inner_sweeps; ++sw) {
+    int act_sw = 0;
+    for (int rr = 0; rr < kJs - 1; ++rr) {
+      if (tid < kJb) {
+        int a, b;
+        rr_pair(rr, tid, kJs, a, b);
+        const double app = S[a * kJsLd + a], aqq = S[b * kJsLd + b], apq = S[a * kJsLd + b];
+        const bool act = apq * apq > tol * tol * app * aqq && fmin(app, aqq) > floor_;
+        double c = 1.0, s = 0.0;
+        if (act) jacobi_rotation(app, aqq, apq, c, s);
+        cc[tid] = c;
+        ss[tid] = act ? s : 0.0;
+        pp[tid] = a;
+        qq[tid] = b;
+        act_sw |= act;
+      }
+      __syncthreads();
+      for (int e = tid; e < kJb * (kJb + 1) / 2; e += kJbThreads) {
+        const int k1 = tri[e] >> 8, k2 = tri[e] & 0xff;
+        const double s1 = ss[k1], s2 = ss[k2];
+        if (s1 != 0.0 || s2 != 0.0)
+          block_update(S, kJsLd, pp[k1], qq[k1], pp[k2], qq[k2], cc[k1], s1, cc[k2], s2, s1 != 0.0, s2 != 0.0);
+      }
+      for (int e = tid; e < kJb * kJs; e += kJbThreads) {
+        const int k = e / kJs, j = e - k * kJs;
+        const double s = ss[k];
+        if (s == 0.0) continue;
+        const double c = cc[k];
+        double *vp = Qt + pp[k] * kJsLd + j, *vq = Qt + qq[k] * kJsLd + j;
+        const double x = *vp, y = *vq;
+        *vp = c * x - s * y;
+        *vq = s * x + c * y;
+      }
+      __syncthreads();
+    }
+    act_sw = __syncthreads_or(act_sw);
+    any |= act_sw;
+    if (!act_sw) break;
+  }
+  if (any)
+    for (int e = tid; e < kJs * kJs; e += kJbThreads) {
+      const int i = e / kJs, j = e - i * kJs;
+      P.Q[(size_t)p * kJs * kJs + e] = (T)Qt[j * kJsLd + i];   // Q = Qt^T
+    }
+  if (tid == 0) {
+    P.act[p] = any;
+    if (any) atomicOr(&P.flag[0], 1);
+  }
+}
+
+template <typename T> struct BjMfma;
+template <> struct BjMfma<float> {
+  using V = j4f;
+  static __device__ __forceinline__ V op(float a, float b, V c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+  static __device__ __forceinline__ int row(int q, int lk) { return 4 * lk + q; }
+};
+template <> struct BjMfma<double> {
+  using V = j4d;
+  static __device__ __forceinline__ V op(double a, double b, V c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+  static __device__ __forceinline__ int row(int q, int lk) { return 4 * q + lk; }
+};
+
+// X[:, I u J] <- X[:, I u J] Q_p for a 64-row tile of W (tiles [0, mt)) or of V (tiles [mt, mt + nt)).
+template <typename T>
+__host__ __device__ inline size_t bj_update_lds() {
+  return sizeof(T) * 2 * (size_t)kJs * kJsLd;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kJbThreads) void bj_update_kernel(BjParams<T> P, int r) {
+  extern __shared__ double smem_bu2[];
+  T (*Xs)[kJsLd] = reinterpret_cast<T (*)[kJsLd]>(smem_bu2);
+  T (*Qs)[kJsLd] = Xs + kJs;
+  const int p = blockIdx.x, t = blockIdx.y, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (!P.act[p]) return;
+  const int lr = lane & 15, lk = lane >> 4, wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  const int mt = (P.m + kJs - 1) / kJs;
+  T *X = t < mt ? P.W : P.V;
+  const int rows = t < mt ? P.m : P.N, r0 = (t < mt ? t : t - mt) * kJs;
+  int I, J;
+  rr_pair(r, p, P.nb, I, J);
+  for (int e = tid; e < kJs * kJs; e += kJbThreads) {
+    const int i = e / kJs, j = e - i * kJs, row = r0 + i;
+    Xs[i][j] = row < rows ? X[(int64_t)row * P.N + bj_col(I, J, j)] : T(0);
+    Qs[i][j] = P.Q[(size_t)p * kJs * kJs + e];
+  }
+  __syncthreads();
+  typename BjMfma<T>::V acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) acc[i][j][q] = 0;
+#pragma unroll 4
+  for (int k = 0; k < kJs; k += 4) {
+    T a[2], b[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) a[i] = Xs[wm + 16 * i + lr][k + lk];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) b[j] = Qs[k + lk][wn + 16 * j + lr];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) acc[i][j] = BjMfma<T>::op(a[i], b[j], acc[i][j]);
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int row = r0 + wm + 16 * i + BjMfma<T>::row(q, lk);
+        if (row < rows) X[(int64_t)row * P.N + bj_col(I, J, wn + 16 * j + lr)] = (T)acc[i][j][q];
+      }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kJbThreads) void bj_sigma_kernel(BjParams<T> P) {
+  const int c = blockIdx.x * kJbThreads + threadIdx.x;
+  if (c >= P.n) return;
+  double acc = 0.0;
+  for (int i = 0; i < P.m; ++i) {
+    const double x = (double)P.W[(int64_t)i * P.N + c];
+    acc = fma(x, x, acc);
+  }
+  P.sig[c] = sqrt(acc);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kJbThreads) void bj_order_kernel(BjParams<T> P) {
+  const int j = blockIdx.x * kJbThreads + threadIdx.x;
+  if (j >= P.n) return;
+  const double sj = P.sig[j];
+  int rank = 0;
+  for (int k = 0; k < P.n; ++k) {
+    const double sk = P.sig[k];
+    rank += (sk > sj) || (sk == sj && k < j);
+  }
+  P.ord[rank] = j;
+}
+
+// s, U (kept columns W / sigma, others zero for the completion), Vh, nk, info
+template <typename T>
+__global__ __launch_bounds__(kJbThreads) void bj_out_kernel(BjParams<T> P, int sweeps, int converged) {
+  const int n = P.n, m = P.m, np = n + (n & 1);
+  const double eps = Eps<T>::value;
+  const double s1 = P.sig[P.ord[0]];
+  const double cut = (P.flags & PSVD_CUT_FLOOR) ? eps * s1 : eps * (double)(m > np ? m : np) * s1;
+  int lo = 0, hi = n;   // sorted descending: the first column at or below the cut
+  while (lo < hi) {
+    const int mid = (lo + hi) / 2;
+    if (P.sig[P.ord[mid]] > cut) lo = mid + 1; else hi = mid;
+  }
+  const int nk = lo;
+  const double mx = __longlong_as_double((long long)P.slot[0]);
+  const double sc = (P.flags & PSVD_SCALE) && mx > 0.0 && isfinite(mx) ? mx : 1.0;
+  const bool vectors = !(P.flags & PSVD_NO_VECTORS);
+  const int64_t gtid = (int64_t)blockIdx.x * kJbThreads + threadIdx.x, gsz = (int64_t)gridDim.x * kJbThreads;
+  for (int64_t r = gtid; r < n; r += gsz) P.s[r] = r < nk ? (T)(P.sig[P.ord[r]] * sc) : T(0);
+  if (vectors) {
+    for (int64_t e = gtid; e < (int64_t)m * n; e += gsz) {
+      const int i = (int)(e / n), r = (int)(e - (int64_t)i * n);
+      const int c = P.ord[r];
+      P.u[e] = r < nk ? (T)((double)P.W[(int64_t)i * P.N + c] / P.sig[c]) : T(0);
+    }
+    for (int64_t e = gtid; e < (int64_t)n * n; e += gsz) {
+      const int r = (int)(e / n), j = (int)(e - (int64_t)r * n);
+      P.vh[e] = P.V[(int64_t)j * P.N + P.ord[r]];
+    }
+  }
+  if (gtid == 0) {
+    *P.nk = vectors ? nk : n;
+    P.info[0] = sweeps;
+    P.info[1] = P.flag[1] ? PSVD_INFO_NONFINITE : (converged ? PSVD_INFO_OK : PSVD_INFO_NOCONV);
+  }
+}
+
+__global__ void bj_status_kernel(int32_t *info, const int *cstatus) {
+  if (threadIdx.x == 0 && info[1] == PSVD_INFO_OK && *cstatus != 0) info[1] = PSVD_INFO_COMPLETION;
+}
+
+__global__ void bj_reset_kernel(unsigned long long *slot, int *flag, int what) {
+  if (threadIdx.x == 0) {
+    if (what & 1) {
+      slot[0] = 0ull;
+      flag[1] = 0;
+      flag[2] = 0;
+    }
+    if (what & 2) {
+      slot[1] = 0ull;
+      flag[0] = 0;
+    }
+  }
+}
+
+static int bj_splits(int m, int npair) {
+  int s = (256 + npair - 1) / npair;
+  const int smax = (m + 127) / 128;   // at least 128 rows per slice
+  if (s > smax) s = smax;
+  if (s > 64) s = 64;
+  return s < 1 ? 1 : s;
+}
+
+template <typename T>
+size_t bj_ws_bytes(int m, int n) {
+  const int N = (n + kJs - 1) / kJs * kJs, npair = N / kJs;
+  auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+  return al(sizeof(T) * (size_t)m * N) + al(sizeof(T) * (size_t)N * N) + al(sizeof(T) * (size_t)npair * kJs * kJs) +
+         al(sizeof(double) * (size_t)npair * bj_splits(m, npair) * kJs * kJs) + al(sizeof(int) * npair) + al(8 * 2) +
+         al(4 * 3) + al(sizeof(double) * N) + al(sizeof(int) * N) + al(4) +
+         al(psvd_complete_workspace_bytes(1, m, n));
+}
+
+template <typename T>
+static int run_bj(int m, int n, const T *a, T *u, T *s, T *vh, int32_t *info, double tol, int max_sweeps, int flags,
+                  int inner_sweeps, void *ws, size_t ws_bytes, hipStream_t st) {
+  if (ws_bytes < bj_ws_bytes<T>(m, n)) return PSVD_EINVAL;
+  static bool attr = false;
+  if (!attr) {
+    if (hipFuncSetAttribute((const void *)bj_inner_kernel<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)bj_inner_lds<T>()) != hipSuccess ||
+        hipFuncSetAttribute((const void *)bj_update_kernel<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)bj_update_lds<T>()) != hipSuccess)
+      return PSVD_ELAUNCH;
+    attr = true;
+  }
+  BjParams<T> P{};
+  P.a = a;
+  P.u = u;
+  P.s = s;
+  P.vh = vh;
+  P.info = info;
+  P.m = m;
+  P.n = n;
+  P.N = (n + kJs - 1) / kJs * kJs;
+  P.nb = P.N / kJb;
+  const int npair = P.nb / 2;
+  P.splits = bj_splits(m, npair);
+  P.inner_sweeps = inner_sweeps;
+  P.flags = flags;
+  P.tol = tol;
+  char *cur = static_cast<char *>(ws);
+  auto take = [&](size_t bytes) {
+    char *q = cur;
+    cur += (bytes + 255) & ~size_t(255);
+    return q;
+  };
+  P.W = reinterpret_cast<T *>(take(sizeof(T) * (size_t)m * P.N));
+  P.V = reinterpret_cast<T *>(take(sizeof(T) * (size_t)P.N * P.N));
+  P.Q = reinterpret_cast<T *>(take(sizeof(T) * (size_t)npair * kJs * kJs));
+  P.Sp = reinterpret_cast<double *>(take(sizeof(double) * (size_t)npair * P.splits * kJs * kJs));
+  P.act = reinterpret_cast<int *>(take(sizeof(int) * npair));
+  P.slot = reinterpret_cast<unsigned long long *>(take(8 * 2));
+  P.flag = reinterpret_cast<int *>(take(4 * 3));
+  P.sig = reinterpret_cast<double *>(take(sizeof(double) * P.N));
+  P.ord = reinterpret_cast<int *>(take(sizeof(int) * P.N));
+  P.nk = reinterpret_cast<int32_t *>(take(4));
+  void *cws = take(psvd_complete_workspace_bytes(1, m, n));
+  const int mt = (m + kJs - 1) / kJs, nt = (flags & PSVD_NO_VECTORS) ? 0 : P.N / kJs;
+  auto blocks = [](int64_t items) {
+    const int64_t b = (items + kJbThreads - 1) / kJbThreads;
+    return (unsigned)(b < 4096 ? (b > 0 ? b : 1) : 4096);
+  };
+  hipLaunchKernelGGL(bj_reset_kernel, dim3(1), dim3(64), 0, st, P.slot, P.flag, 3);
+  hipLaunchKernelGGL(bj_absmax_kernel<T>, dim3(blocks((int64_t)m * n)), dim3(kJbThreads), 0, st, P);
+  hipLaunchKernelGGL(bj_load_kernel<T>, dim3(blocks((int64_t)m * P.N + (int64_t)P.N * P.N)), dim3(kJbThreads), 0, st,
+                     P);
+  int sweeps = 0, converged = 0;
+  for (int sw = 1; sw <= max_sweeps; ++sw) {
+    sweeps = sw;
+    hipLaunchKernelGGL(bj_reset_kernel, dim3(1), dim3(64), 0, st, P.slot, P.flag, 2);
+    hipLaunchKernelGGL(bj_colmax_kernel<T>, dim3((n + kJbThreads - 1) / kJbThreads), dim3(kJbThreads), 0, st, P);
+    for (int r = 0; r < P.nb - 1; ++r) {
+      hipLaunchKernelGGL(bj_gram_kernel<T>, dim3(npair, P.splits), dim3(kJbThreads), 0, st, P, r);
+      hipLaunchKernelGGL(bj_inner_kernel<T>, dim3(npair), dim3(kJbThreads), bj_inner_lds<T>(), st, P, r);
+      hipLaunchKernelGGL(bj_update_kernel<T>, dim3(npair, mt + nt), dim3(kJbThreads), bj_update_lds<T>(), st, P, r);
+    }
+    if (hipGetLastError() != hipSuccess) return PSVD_ELAUNCH;
+    int active = 0;
+    if (hipMemcpyAsync(&active, P.flag, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess) return PSVD_ELAUNCH;
+    if (hipStreamSynchronize(st) != hipSuccess) return PSVD_ELAUNCH;
+    if (!active) {
+      converged = 1;
+      break;
+    }
+  }
+  hipLaunchKernelGGL(bj_sigma_kernel<T>, dim3((n + kJbThreads - 1) / kJbThreads), dim3(kJbThreads), 0, st, P);
+  hipLaunchKernelGGL(bj_order_kernel<T>, dim3((n + kJbThreads - 1) / kJbThreads), dim3(kJbThreads), 0, st, P);
+  hipLaunchKernelGGL(bj_out_kernel<T>, dim3(blocks((int64_t)m * n)), dim3(kJbThreads), 0, st, P, sweeps, converged);
+  if (hipGetLastError() != hipSuccess) return PSVD_ELAUNCH;
+  if (!(flags & PSVD_NO_VECTORS)) {
+    const int rc = psvd_complete_columns(sizeof(T) == 4 ? PSVD_F32 : PSVD_F64, 1, m, n, u, n, (int64_t)m * n, P.nk,
+                                         &P.flag[2], cws, psvd_complete_workspace_bytes(1, m, n), st);
+    if (rc != PSVD_OK) return rc;
+    hipLaunchKernelGGL(bj_status_kernel, dim3(1), dim3(64), 0, st, info, &P.flag[2]);
+  }
+  return hipGetLastError() == hipSuccess ? PSVD_OK : PSVD_ELAUNCH;
+}
+
+}  // namespace psvd
+
+extern "C" {
+
+size_t psvd_jacobi_blocked_workspace_bytes(int dtype, int m, int n) {
+  if (m < 1 || n < 1 || n > m) return 0;
+  if (dtype == PSVD_F32) return psvd::bj_ws_bytes<float>(m, n);
+  if (dtype == PSVD_F64) return psvd::bj_ws_bytes<double>(m, n);
+  return 0;
+}
+
+int psvd_jacobi_blocked(int dtype, int m, int n, const void *a, void *u, void *s, void *vh, int32_t *info, double tol,
+                        int max_sweeps, int flags, int inner_sweeps, void *workspace, size_t workspace_bytes,
+                        psvd_stream_t stream) {
+  if (m < 1 || n < 1 || n > m || !a || !s || !info || !workspace || max_sweeps < 1 || inner_sweeps < 1 ||
+      (flags & ~(PSVD_SCALE | PSVD_CUT_FLOOR | PSVD_NO_VECTORS)) || n > 65536)
+    return PSVD_EINVAL;
+  if (!(flags & PSVD_NO_VECTORS) && (!u || !vh)) return PSVD_EINVAL;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (dtype == PSVD_F32)
+    return psvd::run_bj<float>(m, n, static_cast<const float *>(a), static_cast<float *>(u), static_cast<float *>(s),
+                               static_cast<float *>(vh), info, tol, max_sweeps, flags, inner_sweeps, workspace,
+                               workspace_bytes, st);
+  if (dtype == PSVD_F64)
+    return psvd::run_bj<double>(m, n, static_cast<const double *>(a), static_cast<double *>(u),
+                                static_cast<double *>(s), static_cast<double *>(vh), info, tol, max_sweeps, flags,
+                                inner_sweeps, workspace, workspace_bytes, st);
+  return PSVD_EINVAL;
+}
+
+}  // extern "C"

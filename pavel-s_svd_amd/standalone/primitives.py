@@ -80,13 +80,43 @@ def jacobi_grid(x, *, tol=None, max_sweeps=60, scale=False, cut='reference', vec
     return u, s, vh, info
 
 
-def jacobi_any(x, **kw):
-    """K1 when the shape fits LDS, else K2."""
+def jacobi_blocked(x, *, tol=None, max_sweeps=60, scale=False, cut='reference', vectors=True, inner_sweeps=1):
+    """Blocked one-sided Jacobi SVD of each tall matrix of ``x`` (b, m, n): 64 x 64 FP64 Gram
+    subproblems and MFMA column updates, one call per matrix.  Same outputs as ``jacobi_grid``."""
     b, m, n = x.shape
-    if lds_fits(x.dtype, m, n, kw.get('vectors', True)):
+    if m < n:
+        raise ValueError('jacobi_blocked takes tall matrices (m >= n)')
+    x = x.contiguous()
+    code = N.dtype_code(x.dtype)
+    flags = (N.SCALE if scale else 0) | (N.CUT_FLOOR if cut == 'floor' else 0) | (0 if vectors else N.NO_VECTORS)
+    dev = x.device
+    u = torch.empty_like(x) if vectors else None
+    s = torch.empty(b, n, dtype=x.dtype, device=dev)
+    vh = torch.empty(b, n, n, dtype=x.dtype, device=dev) if vectors else None
+    info = torch.empty(b, 2, dtype=torch.int32, device=dev)
+    wsb = N._hip.jacobi_blocked_workspace_bytes(code, m, n)
+    ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
+    stream = N.stream_of(x)
+    for i in range(b):
+        st = N._hip.jacobi_blocked(code, m, n, N.ptr(x[i]), N.ptr(u[i]) if vectors else 0, N.ptr(s[i]),
+                                   N.ptr(vh[i]) if vectors else 0, N.ptr(info[i]), float(tol or 0.0),
+                                   int(max_sweeps), flags, int(inner_sweeps), N.ptr(ws), wsb, stream)
+        N.check(st, 'psvd_jacobi_blocked')
+    return u, s, vh, info
+
+
+def jacobi_any(x, method='auto', **kw):
+    """K1 when the shape fits LDS; otherwise the blocked kernel (n >= 64) or the cooperative one.
+    method 'grid' or 'blocked' forces a large-matrix kernel."""
+    b, m, n = x.shape
+    if method == 'auto' and lds_fits(x.dtype, m, n, kw.get('vectors', True)):
         return jacobi_lds_uniform(x, **kw)
     kw.pop('lanes', None)
     kw.pop('threads', None)
+    if method == 'blocked' or (method == 'auto' and n >= 64):
+        return jacobi_blocked(x, **kw)
+    if method not in ('auto', 'grid'):
+        raise ValueError(f'unknown Jacobi method {method!r}')
     return jacobi_grid(x, **kw)
 
 

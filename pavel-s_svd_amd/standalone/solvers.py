@@ -11,13 +11,16 @@ Extension options (no reference counterpart):
                                 reference contract, default) or only below eps*sigma_1, the floor
                                 under which the Jacobi test stops enforcing orthogonality
     lanes, threads              Jacobi launch shape (see ``include/pavelsvd.h``)
+    jacobi_method='auto'        matrices beyond LDS: 'blocked' (64 x 64 Gram subproblems on MFMA,
+                                the default for n >= 64) or 'grid' (one rotation per pair and
+                                round, the reference's order)
     gemm='f32' | 'bf16x3'       arithmetic of the FP32 QR / TSQR products and of the QR-branch QDWH
                                 iterations on MFMA (QDWH's Cholesky-branch steps stay FP32)
     polish=False                qr / tsqr, FP32 data: one FP64 Newton-Schulz step on U and on V.
                                 QDWH, FP32 data: FP64 Newton-Schulz step(s) on X (polish_steps,
                                 default 1) and on V, H = sym(X^T A) in FP64, G = V^T H V finished by
                                 FP64 Jacobi sweeps (stat polish_sweeps), sigma = eig(G), U = X V W
-                                in FP64
+                                in FP64; eigh_tol loosens the FP32 eigensolve's threshold then
     l0_model=None               QDWH: start from the learned lower bound (an L0Model, a JSON path,
                                 or True for the packaged model); stats l0, l0_source
     eigh_method='auto'          QDWH eigensolver: 'cyclic' (the reference's rotation order),
@@ -50,9 +53,9 @@ def _jacobi(x, stats, opts, scale):
     """Tall batch (b, m, n) -> U, s, Vh with sigma rescaled (solvers.py:24-27 plus :294-299)."""
     vectors = not opts.get('_values_only', False)
     max_sweeps = opts.get('max_sweeps', 60)
-    u, s, vh, info = P.jacobi_any(x, tol=opts.get('tol'), max_sweeps=max_sweeps, scale=scale,
-                                  cut=opts.get('cut', 'reference'), vectors=vectors,
-                                  lanes=opts.get('lanes'), threads=opts.get('threads'))
+    u, s, vh, info = P.jacobi_any(x, method=opts.get('jacobi_method', 'auto'), tol=opts.get('tol'),
+                                  max_sweeps=max_sweeps, scale=scale, cut=opts.get('cut', 'reference'),
+                                  vectors=vectors, lanes=opts.get('lanes'), threads=opts.get('threads'))
     info = info.cpu()
     from .. import native as N
     N.raise_for_info(info, f'one-sided Jacobi did not converge in {max_sweeps} sweeps')
@@ -138,6 +141,7 @@ def qdwh_weights(lower):
 def _newton_schulz(x):
     """One Newton-Schulz step toward the orthogonal polar factor, x <- x (3 I - x^T x) / 2, as two
     FP64 MFMA products (x: (b, m, n) float64, m >= n)."""
+    x = x.contiguous()
     g = gemm(x.mT, x, mode='f64')
     return gemm(x, g, alpha=-0.5, beta=1.5, out=x.clone(), mode='f64')
 
@@ -249,8 +253,9 @@ def _polar(a, stats, opts):
     else:
         h = gemm(x.mT, a, mode=mode)
         P.symmetrize_(h)
-    w, v, info = P.jacobi_eigh_raw(h, max_sweeps=opts.get('max_sweeps', 60),
-                                   method=opts.get('eigh_method', 'auto'))
+    # with the FP64 finish the FP32 eigensolve only has to bring V^T H V close to diagonal
+    w, v, info = P.jacobi_eigh_raw(h, tol=opts.get('eigh_tol') if polish else None,
+                                   max_sweeps=opts.get('max_sweeps', 60), method=opts.get('eigh_method', 'auto'))
     info_h = info.cpu()
     from .. import native as N
     N.raise_for_info(info_h, f"Hermitian Jacobi did not converge in {opts.get('max_sweeps', 60)} sweeps")

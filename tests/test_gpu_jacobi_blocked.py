@@ -1,0 +1,89 @@
+"""GPU parity of the blocked one-sided Jacobi kernel (64 x 64 FP64 Gram subproblems, MFMA column
+updates) against FP64 LAPACK, the oracle and the cooperative kernel that keeps the reference's
+rotation order.
+
+Bars: sigma error over sigma_1, orthogonality of U and V, reconstruction: 1e-5 (FP32) / 1e-12 (FP64);
+sigma equal to the cooperative kernel's within the same bars; the reference cut zeroes the same
+singular values as the oracle (+-2 at the cut boundary).
+"""
+from importlib import import_module
+
+import numpy as np
+import pytest
+import torch
+
+from conftest import load_package
+from oracle import matrices as X
+from oracle import metrics as M
+from oracle import restatement as R
+
+pytestmark = pytest.mark.gpu
+DEV = 'cuda:0'
+KEYS = ('spectrum_error_over_sigma1', 'orthogonality_u_fro_over_sqrt_cols', 'orthogonality_v_fro_over_sqrt_rows',
+        'reconstruction_relative_fro')
+
+
+@pytest.fixture(scope='module')
+def P():
+    load_package()
+    return import_module('pavel-s_svd_amd.standalone.primitives')
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+@pytest.mark.parametrize('dtype', [np.float32, np.float64])
+@pytest.mark.parametrize('m,n', [(64, 64), (65, 64), (200, 100), (300, 256), (1000, 300), (700, 129)])
+def test_blocked_matches_lapack(P, dtype, m, n):
+    rng = np.random.default_rng(m * 7 + n)
+    a = rng.standard_normal((1, m, n)).astype(dtype)
+    u, s, vh, info = P.jacobi_blocked(torch.from_numpy(a).to(DEV), scale=True)
+    assert int(info[0, 1]) == 0 and 1 <= int(info[0, 0]) < 60
+    met = M.validate(a[0], (_np(u[0]), _np(s[0]), _np(vh[0])), M.reference(a[0]))['worst']
+    bar = 1e-5 if dtype == np.float32 else 1e-12
+    for k in KEYS:
+        assert met[k] <= bar, (k, met[k])
+    assert np.all(np.diff(_np(s[0])) <= 0)
+
+
+def test_blocked_equals_cooperative_kernel_and_cut(P):
+    a = X.workload_single_jacobi()
+    x = torch.from_numpy(a[None]).to(DEV)
+    _, s1, _, i1 = P.jacobi_blocked(x, tol=1e-7, scale=True, cut='floor')
+    _, s2, _, i2 = P.jacobi_grid(x, tol=1e-7, scale=True, cut='floor')
+    assert np.max(np.abs(_np(s1) - _np(s2))) <= 2e-6
+    # reference cut: the same sigma are zeroed as by the oracle
+    _, s3, _, _ = P.jacobi_blocked(x, scale=True)
+    _, rs, _ = R.svd(a, 'jacobi', max_sweeps=60)
+    assert abs(int((_np(s3) == 0).sum()) - int((rs == 0).sum())) <= 2
+
+
+def test_blocked_rank_deficient_zero_and_values_only(P):
+    rng = np.random.default_rng(3)
+    a = (rng.standard_normal((300, 20)) @ rng.standard_normal((20, 150))).astype(np.float64)
+    u, s, vh, info = P.jacobi_blocked(torch.from_numpy(a[None]).to(DEV), scale=True)
+    assert int(info[0, 1]) == 0
+    sv = _np(s[0])
+    assert np.count_nonzero(sv) == 20
+    uu = _np(u[0])
+    assert np.max(np.abs(uu.T @ uu - np.eye(150))) < 1e-12          # completed null columns
+    assert np.max(np.abs((uu * sv) @ _np(vh[0]) - a)) < 1e-11 * np.abs(a).max()
+    z = torch.zeros(1, 130, 70, dtype=torch.float32, device=DEV)
+    u, s, vh, info = P.jacobi_blocked(z, scale=True)
+    assert int(info[0, 0]) == 1 and not s.abs().any()
+    assert np.max(np.abs(_np(u[0]).T @ _np(u[0]) - np.eye(70))) < 1e-6
+    b = rng.standard_normal((1, 400, 200)).astype(np.float32)
+    _, s, _, _ = P.jacobi_blocked(torch.from_numpy(b).to(DEV), vectors=False, scale=True)
+    ref = np.linalg.svd(b[0].astype(np.float64), compute_uv=False)
+    assert np.max(np.abs(_np(s[0]) - ref)) / ref[0] < 1e-5
+
+
+def test_svd_dispatches_large_matrices_to_blocked(P):
+    pkg = load_package()
+    a = np.random.default_rng(5).standard_normal((600, 200)).astype(np.float32)
+    u, s, vh, info = pkg.svd(torch.from_numpy(a).to(DEV), 'jacobi', return_info=True)
+    u2, s2, vh2, info2 = pkg.svd(torch.from_numpy(a).to(DEV), 'jacobi', jacobi_method='grid', return_info=True)
+    assert np.max(np.abs(_np(s) - _np(s2))) / float(s[0]) < 2e-6
+    met = M.validate(a, (_np(u), _np(s), _np(vh)), M.reference(a))['worst']
+    assert all(met[k] <= 1e-5 for k in KEYS), met

@@ -77,3 +77,14 @@ def test_tall_lowrank_workload_small(pkg):
     assert info['tsqr_leaves'] == 8
     met = _metrics(a, (u, s, vh))
     assert all(met[k] <= 1e-5 for k in KEYS), met
+
+
+@pytest.mark.parametrize('gmode', ['bf16x3', 'f32'])
+def test_tsqr_polish_restores_orthogonality(pkg, gmode):
+    """polish=True: one FP64 Newton-Schulz step on U and on V brings both to FP32 rounding level."""
+    a = np.random.default_rng(9).standard_normal((3000, 300)).astype(np.float32)
+    u, s, vh = pkg.svd(torch.from_numpy(a).to(DEV), 'tsqr', tile_rows=600, gemm=gmode, polish=True)
+    met = _metrics(a, (u, s, vh))
+    assert met['orthogonality_u_fro_over_sqrt_cols'] <= 1e-6, met
+    assert met['orthogonality_v_fro_over_sqrt_rows'] <= 1e-6, met
+    assert met['spectrum_error_over_sigma1'] <= 1e-5 and met['reconstruction_relative_fro'] <= 3e-5, met
