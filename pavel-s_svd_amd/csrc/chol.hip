@@ -6,7 +6,8 @@
 // by row against the block (L21 = A21 L11^-T), and the trailing matrix is updated with one GEMM
 // (A22 -= L21 L21^T).  The strict upper triangle is cleared at the end.
 // psvd_trsm_right: B <- B L^-T (trans) or B <- B L^-1 with L lower triangular; block columns are
-// solved row-parallel against the diagonal block after a GEMM subtracts the solved part.
+// solved row-parallel against the diagonal block and then removed from the unsolved columns with
+// one GEMM (right-looking).
 // QDWH's x z^-1 = x L^-T L^-1 (solvers.py:159) is two calls, with no transposed copies.
 #include <float.h>
 
@@ -144,25 +145,28 @@ static int run_cholesky(int gmode, int64_t batch, int n, T *a, int64_t lda, int6
   return launched();
 }
 
+// Right-looking: each solved block column updates all the columns still to be solved with one
+// GEMM of K = kChol (a wide output that fills the GPU), instead of each new block column gathering
+// the solved ones (a skinny output with long K).
 template <typename T>
 static int run_trsm_right(int gmode, int64_t batch, int trans, int n, int rows, const T *L, int64_t ldl, int64_t sl,
                           T *B, int64_t ldb, int64_t sb, hipStream_t st) {
-  if (trans) {  // B L^-T: block columns left to right, B[:, J] -= B[:, :J] L[J, :J]^T
+  if (trans) {  // B L^-T, left to right: solve B[:, J], then B[:, J+1:] -= B[:, J] L[J+1:, J]^T
     for (int jb = 0; jb < n; jb += kChol) {
-      const int nb = n - jb < kChol ? n - jb : kChol;
-      if (jb > 0)
-        PSVD_TRY(psvd_gemm(gmode, 0, 1, rows, nb, jb, -1.0, B, ldb, sb, L + (int64_t)jb * ldl, ldl, sl, 1.0, B + jb, ldb,
-                           sb, batch, st));
+      const int nb = n - jb < kChol ? n - jb : kChol, rest = n - jb - nb;
       PSVD_TRY(run_rsolve<T>(L + (int64_t)jb * ldl + jb, ldl, sl, B + jb, ldb, sb, rows, nb, 1, batch, st));
+      if (rest > 0)
+        PSVD_TRY(psvd_gemm(gmode, 0, 1, rows, rest, nb, -1.0, B + jb, ldb, sb, L + (int64_t)(jb + nb) * ldl + jb, ldl,
+                           sl, 1.0, B + jb + nb, ldb, sb, batch, st));
     }
-  } else {  // B L^-1: block columns right to left, B[:, J] -= B[:, J+1:] L[J+1:, J]
+  } else {  // B L^-1, right to left: solve B[:, J], then B[:, :J] -= B[:, J] L[J, :J]
     const int nblk = (n + kChol - 1) / kChol;
     for (int bi = nblk - 1; bi >= 0; --bi) {
-      const int jb = bi * kChol, nb = n - jb < kChol ? n - jb : kChol, rest = n - jb - nb;
-      if (rest > 0)
-        PSVD_TRY(psvd_gemm(gmode, 0, 0, rows, nb, rest, -1.0, B + jb + nb, ldb, sb, L + (int64_t)(jb + nb) * ldl + jb,
-                           ldl, sl, 1.0, B + jb, ldb, sb, batch, st));
+      const int jb = bi * kChol, nb = n - jb < kChol ? n - jb : kChol;
       PSVD_TRY(run_rsolve<T>(L + (int64_t)jb * ldl + jb, ldl, sl, B + jb, ldb, sb, rows, nb, 0, batch, st));
+      if (jb > 0)
+        PSVD_TRY(psvd_gemm(gmode, 0, 0, rows, jb, nb, -1.0, B + jb, ldb, sb, L + (int64_t)jb * ldl, ldl, sl, 1.0, B, ldb,
+                           sb, batch, st));
     }
   }
   return PSVD_OK;
