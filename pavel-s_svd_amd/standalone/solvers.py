@@ -20,7 +20,9 @@ Extension options (no reference counterpart):
                                 QDWH, FP32 data: FP64 Newton-Schulz step(s) on X (polish_steps,
                                 default 1) and on V, H = sym(X^T A) in FP64, G = V^T H V finished by
                                 FP64 Jacobi sweeps (stat polish_sweeps), sigma = eig(G), U = X V W
-                                in FP64; eigh_tol loosens the FP32 eigensolve's threshold then
+                                in FP64 (finish threshold polish_tol, default 1e-10 max|G|);
+                                eigh_tol loosens the FP32 eigensolve's threshold then;
+                                eigh_inner_sweeps: sweeps per 64 x 64 subproblem (blocked solver)
     l0_model=None               QDWH: start from the learned lower bound (an L0Model, a JSON path,
                                 or True for the packaged model); stats l0, l0_source
     eigh_method='auto'          QDWH eigensolver: 'cyclic' (the reference's rotation order),
@@ -255,7 +257,8 @@ def _polar(a, stats, opts):
         P.symmetrize_(h)
     # with the FP64 finish the FP32 eigensolve only has to bring V^T H V close to diagonal
     w, v, info = P.jacobi_eigh_raw(h, tol=opts.get('eigh_tol') if polish else None,
-                                   max_sweeps=opts.get('max_sweeps', 60), method=opts.get('eigh_method', 'auto'))
+                                   max_sweeps=opts.get('max_sweeps', 60), method=opts.get('eigh_method', 'auto'),
+                                   inner_sweeps=opts.get('eigh_inner_sweeps', 1))
     info_h = info.cpu()
     from .. import native as N
     N.raise_for_info(info_h, f"Hermitian Jacobi did not converge in {opts.get('max_sweeps', 60)} sweeps")
@@ -267,7 +270,10 @@ def _polar(a, stats, opts):
         v64 = _newton_schulz(v.double())
         g64 = gemm(v64.mT, gemm(h64, v64, mode='f64'), mode='f64')
         P.symmetrize_(g64)
-        w2, z, info2 = P.jacobi_eigh_raw(g64, max_sweeps=opts.get('max_sweeps', 60))
+        # the finish stops at |g_pq| <= polish_tol * max|G| (default 1e-10): the eigenvalues then move
+        # by at most the remaining off-diagonal norm, far below FP32 resolution
+        w2, z, info2 = P.jacobi_eigh_raw(g64, tol=opts.get('polish_tol', 1e-10), max_sweeps=opts.get('max_sweeps', 60),
+                                         inner_sweeps=opts.get('eigh_inner_sweeps', 1))
         info2 = info2.cpu()
         N.raise_for_info(info2, f"Hermitian Jacobi did not converge in {opts.get('max_sweeps', 60)} sweeps")
         polish_sweeps = int(info2[:, 0].max())
