@@ -481,8 +481,10 @@ __host__ __device__ inline size_t blk_inner_lds() {
   return sizeof(double) * 2 * kEb + sizeof(int) * 2 * kEb + sizeof(T) * 2 * (size_t)kEs * kEsLd;
 }
 
+constexpr int kBlkInner = 512;
+
 template <typename T>
-__global__ __launch_bounds__(256) void blk_inner_kernel(BlkParams<T> P, int r, int g) {
+__global__ __launch_bounds__(kBlkInner) void blk_inner_kernel(BlkParams<T> P, int r, int g) {
   extern __shared__ double smem_bi[];
   double *cc = smem_bi, *ss = cc + kEb;
   int *pp = reinterpret_cast<int *>(ss + kEb), *qq = pp + kEb;
@@ -495,7 +497,7 @@ __global__ __launch_bounds__(256) void blk_inner_kernel(BlkParams<T> P, int r, i
   const double scale = __longlong_as_double((long long)P.slot[g % 3]);
   const double tol = P.tol > 0.0 ? P.tol : 10.0 * Eps<T>::value;
   double offmax = 0.0;
-  for (int e = tid; e < kEs * kEs; e += 256) {
+  for (int e = tid; e < kEs * kEs; e += kBlkInner) {
     const int i = e / kEs, j = e - i * kEs;
     const T x = P.G[(int64_t)blk_row(I, J, i) * P.N + blk_row(I, J, j)];
     S[i * kEsLd + j] = x;
@@ -507,7 +509,7 @@ __global__ __launch_bounds__(256) void blk_inner_kernel(BlkParams<T> P, int r, i
   const int live = __syncthreads_or(offmax > tol * scale);
   int any = 0;
   __shared__ unsigned short tri[kEb * (kEb + 1) / 2];     // upper-triangle item -> (k1 << 8) | k2
-  for (int e = tid; e < kEb * (kEb + 1) / 2; e += 256) {
+  for (int e = tid; e < kEb * (kEb + 1) / 2; e += kBlkInner) {
     int k1, k2;
     tri_decode(e, kEb, k1, k2);
     tri[e] = (unsigned short)((k1 << 8) | k2);
@@ -529,13 +531,13 @@ __global__ __launch_bounds__(256) void blk_inner_kernel(BlkParams<T> P, int r, i
         act_sw |= act;
       }
       __syncthreads();
-      for (int e = tid; e < kEb * (kEb + 1) / 2; e += 256) {
+      for (int e = tid; e < kEb * (kEb + 1) / 2; e += kBlkInner) {
         const int k1 = tri[e] >> 8, k2 = tri[e] & 0xff;
         const double s1 = ss[k1], s2 = ss[k2];
         if (s1 != 0.0 || s2 != 0.0)
           block_update(S, kEsLd, pp[k1], qq[k1], pp[k2], qq[k2], cc[k1], s1, cc[k2], s2, s1 != 0.0, s2 != 0.0);
       }
-      for (int e = tid; e < kEb * kEs; e += 256) {
+      for (int e = tid; e < kEb * kEs; e += kBlkInner) {
         const int k = e / kEs, j = e - k * kEs;
         const double s = ss[k];
         if (s == 0.0) continue;
@@ -552,23 +554,25 @@ __global__ __launch_bounds__(256) void blk_inner_kernel(BlkParams<T> P, int r, i
     if (!act_sw) break;
   }
   if (any) {
-    for (int e = tid; e < kEs * kEs; e += 256) {
+    for (int e = tid; e < kEs * kEs; e += kBlkInner) {
       const int i = e / kEs, j = e - i * kEs;
       P.G[(int64_t)blk_row(I, J, i) * P.N + blk_row(I, J, j)] = S[i * kEsLd + j];
       P.Q[(int64_t)p * kEs * kEs + e] = Qt[i * kEsLd + j];
     }
   }
-  // maxima of the four 32 x 32 blocks, one per wave
-  const int bi = wave >> 1, bj = wave & 1;
-  double mx = 0.0;
-  for (int e = lane; e < kEb * kEb; e += kWave) {
-    const int i = e / kEb, j = e & (kEb - 1);
-    mx = fmax(mx, fabs((double)S[(bi * kEb + i) * kEsLd + bj * kEb + j]));
-  }
-  mx = wave_max_fast(mx);
-  if (lane == 0) {
-    P.bm[(bi ? J : I) * P.nb + (bj ? J : I)] = mx;
-    slot_max(&P.slot[(g + 1) % 3], mx);
+  // maxima of the four 32 x 32 blocks, one per wave of the first four
+  if (wave < 4) {
+    const int bi = wave >> 1, bj = wave & 1;
+    double mx = 0.0;
+    for (int e = lane; e < kEb * kEb; e += kWave) {
+      const int i = e / kEb, j = e & (kEb - 1);
+      mx = fmax(mx, fabs((double)S[(bi * kEb + i) * kEsLd + bj * kEb + j]));
+    }
+    mx = wave_max_fast(mx);
+    if (lane == 0) {
+      P.bm[(bi ? J : I) * P.nb + (bj ? J : I)] = mx;
+      slot_max(&P.slot[(g + 1) % 3], mx);
+    }
   }
   if (tid == 0) {
     P.act[p] = any;
@@ -757,7 +761,7 @@ static int run_eigh_blocked(int64_t batch, int n, const T *h, T *w, T *v, int32_
       sweeps = sw;
       hipLaunchKernelGGL(blk_reset_kernel, dim3(1), dim3(64), 0, st, P.slot, P.flag, 2);
       for (int r = 0; r < P.nb - 1; ++r, ++g) {
-        hipLaunchKernelGGL(blk_inner_kernel<T>, dim3(npair), dim3(256), blk_inner_lds<T>(), st, P, r, g);
+        hipLaunchKernelGGL(blk_inner_kernel<T>, dim3(npair), dim3(kBlkInner), blk_inner_lds<T>(), st, P, r, g);
         hipLaunchKernelGGL(blk_update_kernel<T>, dim3(gupd), dim3(256), blk_update_lds<T>(), st, P, r, g);
       }
       if (hipGetLastError() != hipSuccess) return PSVD_ELAUNCH;

@@ -150,8 +150,10 @@ __host__ __device__ inline size_t bj_inner_lds() {
 }
 
 // Sweeps of S_p (FP64, in LDS) with the one-sided test; Q_p = accumulated rotation.
+constexpr int kJbInner = 512;
+
 template <typename T>
-__global__ __launch_bounds__(kJbThreads) void bj_inner_kernel(BjParams<T> P, int r) {
+__global__ __launch_bounds__(kJbInner) void bj_inner_kernel(BjParams<T> P, int r) {
   extern __shared__ double smem_bj[];
   double *cc = smem_bj, *ss = cc + kJb;
   double *S = ss + kJb, *Qt = S + kJs * kJsLd;
@@ -160,7 +162,7 @@ __global__ __launch_bounds__(kJbThreads) void bj_inner_kernel(BjParams<T> P, int
   const int p = blockIdx.x, tid = threadIdx.x;
   const double eps = Eps<T>::value, tol = P.tol > 0.0 ? P.tol : 10.0 * eps;
   const double floor_ = eps * eps * __longlong_as_double((long long)P.slot[1]);
-  for (int e = tid; e < kJs * kJs; e += kJbThreads) {
+  for (int e = tid; e < kJs * kJs; e += kJbInner) {
     const int i = e / kJs, j = e - i * kJs;
     const double *sp = P.Sp + (size_t)p * P.splits * kJs * kJs + e;
     double acc = sp[0];
@@ -168,7 +170,7 @@ __global__ __launch_bounds__(kJbThreads) void bj_inner_kernel(BjParams<T> P, int
     S[i * kJsLd + j] = acc;
     Qt[i * kJsLd + j] = i == j ? 1.0 : 0.0;
   }
-  for (int e = tid; e < kJb * (kJb + 1) / 2; e += kJbThreads) {
+  for (int e = tid; e < kJb * (kJb + 1) / 2; e += kJbInner) {
     int a = (int)((2.0 * kJb + 1.0 - sqrt((2.0 * kJb + 1.0) * (2.0 * kJb + 1.0) - 8.0 * e)) / 2.0);
     if (a < 0) a = 0;
     if (a > kJb - 1) a = kJb - 1;
@@ -177,8 +179,18 @@ __global__ __launch_bounds__(kJbThreads) void bj_inner_kernel(BjParams<T> P, int
     tri[e] = (unsigned short)((a << 8) | (a + e - (a * kJb - a * (a - 1) / 2)));
   }
   __syncthreads();
+  // a subproblem in which no pair passes the test rotates nowhere in its sweep: skip it
+  int live = 0;
+  for (int e = tid; e < kJs * kJs; e += kJbInner) {
+    const int i = e / kJs, j = e - i * kJs;
+    if (i < j) {
+      const double app = S[i * kJsLd + i], aqq = S[j * kJsLd + j], apq = S[i * kJsLd + j];
+      live |= apq * apq > tol * tol * app * aqq && fmin(app, aqq) > floor_;
+    }
+  }
+  live = __syncthreads_or(live);
   int any = 0;
-  for (int sw = 0; sw < P.inner_sweeps; ++sw) {
+  for (int sw = 0; live && sw < P.inner_sweeps; ++sw) {
     int act_sw = 0;
     for (int rr = 0; rr < kJs - 1; ++rr) {
       if (tid < kJb) {
@@ -195,13 +207,13 @@ __global__ __launch_bounds__(kJbThreads) void bj_inner_kernel(BjParams<T> P, int
         act_sw |= act;
       }
       __syncthreads();
-      for (int e = tid; e < kJb * (kJb + 1) / 2; e += kJbThreads) {
+      for (int e = tid; e < kJb * (kJb + 1) / 2; e += kJbInner) {
         const int k1 = tri[e] >> 8, k2 = tri[e] & 0xff;
         const double s1 = ss[k1], s2 = ss[k2];
         if (s1 != 0.0 || s2 != 0.0)
           block_update(S, kJsLd, pp[k1], qq[k1], pp[k2], qq[k2], cc[k1], s1, cc[k2], s2, s1 != 0.0, s2 != 0.0);
       }
-      for (int e = tid; e < kJb * kJs; e += kJbThreads) {
+      for (int e = tid; e < kJb * kJs; e += kJbInner) {
         const int k = e / kJs, j = e - k * kJs;
         const double s = ss[k];
         if (s == 0.0) continue;
@@ -218,7 +230,7 @@ __global__ __launch_bounds__(kJbThreads) void bj_inner_kernel(BjParams<T> P, int
     if (!act_sw) break;
   }
   if (any)
-    for (int e = tid; e < kJs * kJs; e += kJbThreads) {
+    for (int e = tid; e < kJs * kJs; e += kJbInner) {
       const int i = e / kJs, j = e - i * kJs;
       P.Q[(size_t)p * kJs * kJs + e] = (T)Qt[j * kJsLd + i];   // Q = Qt^T
     }
@@ -453,7 +465,7 @@ static int run_bj(int m, int n, const T *a, T *u, T *s, T *vh, int32_t *info, do
     hipLaunchKernelGGL(bj_colmax_kernel<T>, dim3((n + kJbThreads - 1) / kJbThreads), dim3(kJbThreads), 0, st, P);
     for (int r = 0; r < P.nb - 1; ++r) {
       hipLaunchKernelGGL(bj_gram_kernel<T>, dim3(npair, P.splits), dim3(kJbThreads), 0, st, P, r);
-      hipLaunchKernelGGL(bj_inner_kernel<T>, dim3(npair), dim3(kJbThreads), bj_inner_lds<T>(), st, P, r);
+      hipLaunchKernelGGL(bj_inner_kernel<T>, dim3(npair), dim3(kJbInner), bj_inner_lds<T>(), st, P, r);
       hipLaunchKernelGGL(bj_update_kernel<T>, dim3(npair, mt + nt), dim3(kJbThreads), bj_update_lds<T>(), st, P, r);
     }
     if (hipGetLastError() != hipSuccess) return PSVD_ELAUNCH;
