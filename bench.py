@@ -225,7 +225,9 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-check', action='store_true')
     ap.add_argument('--no-tune', action='store_true', help='use the default launch-shape policy')
-    ap.add_argument('--only', default='all', help="'c2' for the batched-Jacobi line alone")
+    ap.add_argument('--only', default='all', choices=['all', 'c2', 'single', 'qdwh', 'tsqr', 'l0'],
+                    help="'c2': the batched-Jacobi line alone; single / qdwh / tsqr / l0: that leg alone "
+                         "(printed as {\"configs\": ...}, for profiling)")
     ap.add_argument('--copy-calibration', action='store_true',
                     help='only run a plain 256 MiB device-to-device copy 5 times (the reference kernel for '
                          'scaling FETCH_SIZE / WRITE_SIZE counter readings to bytes) and exit')
@@ -243,6 +245,12 @@ def main():
         print(json.dumps({'copy_bytes_read': COPY_BYTES, 'copy_bytes_written': COPY_BYTES}), flush=True)
         return
     pkg = importlib.import_module('pavel-s_svd_amd')
+    legs = {'single': ('single_512x256', config_single), 'qdwh': ('qdwh_4096', config_qdwh),
+            'tsqr': ('tsqr_8192x2048', config_tsqr), 'l0': ('learned_l0_1024', config_l0)}
+    if args.only in legs:
+        name, fn = legs[args.only]
+        print(json.dumps({'configs': {name: fn(pkg, torch.device('cuda:0'))}}), flush=True)
+        return
     B = importlib.import_module('pavel-s_svd_amd.standalone.batched')
     T = importlib.import_module('pavel-s_svd_amd.autotune')
     dev = torch.device('cuda:0')
@@ -292,8 +300,7 @@ def main():
         del batch
         torch.cuda.empty_cache()
         out['configs'] = {}
-        for name, fn in (('single_512x256', config_single), ('qdwh_4096', config_qdwh),
-                         ('tsqr_8192x2048', config_tsqr), ('learned_l0_1024', config_l0)):
+        for name, fn in legs.values():
             t0 = time.perf_counter()
             out['configs'][name] = fn(pkg, dev)
             print(f'{name} done in {time.perf_counter() - t0:.1f} s', file=sys.stderr, flush=True)
