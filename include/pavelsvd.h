@@ -136,6 +136,8 @@ int psvd_complete_columns(int dtype, int64_t batch, int m, int n, void *u, int64
  * so R matches primitives.qr (svdbench/standalone/primitives.py:41-89) up to rounding.  A is
  * read only (row stride lda, batch stride stride_a).  q or r may be NULL to skip that output.
  * gemm_mode: PSVD_GEMM_F32 or PSVD_GEMM_BF16X3 for dtype PSVD_F32 (ignored for PSVD_F64).
+ * Panels taller than 512 rows are factored by several workgroups per matrix in one cooperative
+ * launch (at most one workgroup per CU); such calls are not capturable in a hipGraph.
  * ------------------------------------------------------------------------------------- */
 size_t psvd_qr_workspace_bytes(int dtype, int64_t batch, int m, int n);
 int psvd_qr(int dtype, int gemm_mode, int64_t batch, int m, int n, const void *a, int64_t lda, int64_t stride_a,
