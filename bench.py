@@ -198,15 +198,18 @@ def config_l0(pkg, dev, count=32):
         a = T.generate(i0, min(i0 + T.CHUNK, start + count), n)
         for j in range(a.shape[0]):
             T.polar_time(a[j:j + 1], None)                  # warm
-            rows.append((T.polar_time(a[j:j + 1], None), T.polar_time(a[j:j + 1], model)))
-    arr = np.array([[p[0], p[1], p[2], q[0], q[1], q[2]] for p, q in rows])
-    return {'workload': f'QDWH polar phase on {count} test matrices {n}x{n} of the learned-l0 workload (log10 '
-                        'cond ~ U[1, 12], geometric or clustered spectra); learned time includes features and '
-                        'inference', 'unit': 's/solve',
-            'plain_seconds': float(arr[:, 0].mean()), 'learned_seconds': float(arr[:, 3].mean()),
-            'plain_iterations': float(arr[:, 1].mean()), 'learned_iterations': float(arr[:, 4].mean()),
-            'plain_qr_iterations': float(arr[:, 2].mean()), 'learned_qr_iterations': float(arr[:, 5].mean()),
-            'model_evaluation': ev}
+            rows.append([T.polar_time(a[j:j + 1], mdl, stop) for stop in (None, 1e-2) for mdl in (None, model)])
+    arr = np.array([[v for run in r for v in run] for r in rows])
+    out = {'workload': f'QDWH polar phase on {count} test matrices {n}x{n} of the learned-l0 workload (log10 '
+                       'cond ~ U[1, 12], geometric or clustered spectra); learned time includes features and '
+                       'inference; orth_stop = the polished form (stop at |X^T X - I|_F <= 1e-2, three FP64 '
+                       'Newton-Schulz steps)', 'unit': 's/solve'}
+    for k, name in enumerate(('reference_stop_eps', 'reference_stop_learned', 'orth_stop_eps', 'orth_stop_learned')):
+        c = 4 * k
+        out[name] = {'seconds': float(arr[:, c].mean()), 'iterations': float(arr[:, c + 1].mean()),
+                     'qr_iterations': float(arr[:, c + 2].mean())}
+    out['model_evaluation'] = ev
+    return out
 
 
 def load_traffic():

@@ -6,7 +6,7 @@ iterations (the expensive QR-branch ones) lifting singular values the weights we
 Here a 32 -> 64 -> 64 -> 1 tanh MLP predicts log10(sigma_min / |A|_F) -- the smallest singular
 value of QDWH's starting iterate X0 = A / |A|_F, which is what the bound l0 must not exceed -- from
 cheap sketch features of A; the bound handed to QDWH is 10^(prediction - margin), the margin taken
-from held-out residuals so that the bound stays below the true value, clamped to [eps, 1].  QDWH's own convergence test and
+from held-out residuals so that the bound stays below the true value, clamped to [eps^2, 1].  QDWH's own convergence test and
 the polar fixed-point check judge the result; the reference's l0 = eps is the fallback.
 
 Features (all passes over A run on the package's kernels: Frobenius norm and column statistics
@@ -119,13 +119,15 @@ class L0Model:
         return y
 
     def lower_bound(self, a, eps):
-        """Conservative l0 for the batch a (the smallest over its matrices), or None when the
-        features are not finite."""
+        """Conservative l0 for the batch a (the smallest over its matrices), clamped to
+        [eps^2, 1], or None when the features are not finite.  The bound may lie below eps: for a
+        numerically singular matrix sigma_min(X0) does, and the reference's l0 = eps then
+        overestimates it and costs iterations."""
         pred = self.predict(features(a))
         lo = float(pred.min()) - self.margin
         if not math.isfinite(lo):
             return None
-        return min(1.0, max(eps, 10.0 ** lo))
+        return min(1.0, max(eps * eps, 10.0 ** lo))
 
     def grad(self, feats, target):
         """Loss and gradient of the standardised batch (for checks): (loss, grad, y)."""

@@ -17,8 +17,9 @@ Extension options (no reference counterpart):
     gemm='f32' | 'bf16x3'       arithmetic of the FP32 QR / TSQR products and of the QR-branch QDWH
                                 iterations on MFMA (QDWH's Cholesky-branch steps stay FP32)
     polish=False                qr / tsqr, FP32 data: one FP64 Newton-Schulz step on U and on V.
-                                QDWH, FP32 data: FP64 Newton-Schulz step(s) on X (polish_steps,
-                                default 1) and on V, H = sym(X^T A) in FP64, G = V^T H V finished by
+                                QDWH, FP32 data: the iteration stops once |X^T X - I|_F <= orth_stop
+                                (default 1e-2), then FP64 Newton-Schulz steps on X (polish_steps,
+                                default 3) and on V, H = sym(X^T A) in FP64, G = V^T H V finished by
                                 FP64 Jacobi sweeps (stat polish_sweeps), sigma = eig(G), U = X V W
                                 in FP64 (finish threshold polish_tol, default 1e-10 max|G|);
                                 eigh_tol loosens the FP32 eigensolve's threshold then;
@@ -152,9 +153,14 @@ def _eye(b, n, dtype, device):
     return torch.eye(n, dtype=dtype, device=device).expand(b, n, n).contiguous()
 
 
-def _qdwh_iterate(x, lower, tol, max_iter, mode, qmode):
+def _qdwh_iterate(x, lower, tol, max_iter, mode, qmode, orth_stop=None):
     """QDWH iterations from x = A / |A|_F and the lower bound `lower` (solvers.py:131-167);
-    returns (X, iterations, QR-branch iterations).  x is not modified."""
+    returns (X, iterations, QR-branch iterations).  x is not modified.
+
+    orth_stop (for the FP64-polished form only): a Cholesky-branch step first forms X^T X; when
+    |X^T X - I|_F <= orth_stop every singular value of X is within the quadratic basin of the FP64
+    Newton-Schulz steps that follow, and the iteration ends there instead of waiting for the
+    elementwise test, which FP32 rounding delays by several steps at large n."""
     b, m, n = x.shape
     dt, dev = x.dtype, x.device
     nqr = 0
@@ -173,8 +179,16 @@ def _qdwh_iterate(x, lower, tol, max_iter, mode, qmode):
             # for FP64 data) so that the fixed-point test at tol = 10 eps can be met -- BF16x3
             # carries about 2^-16 relative precision
             cmode = 'f32' if mode == 'bf16x3' else mode
-            z = _eye(b, n, dt, dev)
-            gemm(x.mT, x, alpha=cc, beta=1.0, out=z, mode=cmode)
+            if orth_stop is not None:
+                g = gemm(x.mT, x, mode=cmode)
+                eye = _eye(b, n, dt, dev)
+                dev_ = P.axpby_(1.0, g, -1.0, eye.clone())                 # X^T X - I
+                if float(P.reduce(dev_, 'fro').max()) <= orth_stop:
+                    return x, it, nqr
+                z = P.axpby_(cc, g, 1.0, eye)
+            else:
+                z = _eye(b, n, dt, dev)
+                gemm(x.mT, x, alpha=cc, beta=1.0, out=z, mode=cmode)
             P.symmetrize_(z)
             chol, fail = P.cholesky(z, gemm=cmode if cmode != 'f64' else 'f32')
             if bool(fail.any()):
@@ -218,6 +232,8 @@ def _polar(a, stats, opts):
     if not 0 < lower <= 1:
         raise ValueError('lower_bound must be in (0,1]')
     max_iter = opts.get('max_iter', 40)
+    polish = bool(opts.get('polish', False)) and dt == torch.float32
+    orth_stop = opts.get('orth_stop', 1e-2) if polish else None
     x = None
     l0_model = opts.get('l0_model')
     if l0_model is not None and l0_model is not False and 'lower_bound' not in opts:
@@ -228,8 +244,8 @@ def _polar(a, stats, opts):
         stats['l0_source'] = 'fallback'
         if learned is not None:
             try:
-                x, it, nqr = _qdwh_iterate(x0, learned, tol, max_iter, mode, qmode)
-                if _fixed_point_error(x, mode) > 100 * tol:
+                x, it, nqr = _qdwh_iterate(x0, learned, tol, max_iter, mode, qmode, orth_stop)
+                if _fixed_point_error(x, mode) > (2 * orth_stop if orth_stop is not None else 100 * tol):
                     x = None
             except RuntimeError:
                 x = None
@@ -238,16 +254,16 @@ def _polar(a, stats, opts):
                 stats['l0_source'] = 'learned'
                 lower = learned
     if x is None:
-        x, it, nqr = _qdwh_iterate(x0, lower, tol, max_iter, mode, qmode)
+        x, it, nqr = _qdwh_iterate(x0, lower, tol, max_iter, mode, qmode, orth_stop)
     stats['polar_qr_iterations'] = nqr
     if l0_model is not None and l0_model is not False:
         stats['l0'] = lower
-    polish = bool(opts.get('polish', False)) and dt == torch.float32
     if polish:
-        # FP64 tail: Newton-Schulz steps make X orthogonal to FP64 rounding, H = sym(X^T A) is
-        # formed in FP64 and rounded once for the FP32 eigensolve
+        # FP64 tail: Newton-Schulz steps make X orthogonal to FP64 rounding (three from the
+        # |X^T X - I|_F <= 1e-2 stop: 1e-2 -> 1e-4 -> 1e-8 -> 1e-16), H = sym(X^T A) is formed in
+        # FP64 and rounded once for the FP32 eigensolve
         x64 = x.double()
-        for _ in range(int(opts.get('polish_steps', 1))):
+        for _ in range(int(opts.get('polish_steps', 3))):
             x64 = _newton_schulz(x64)
         h64 = gemm(x64.mT, a.double(), mode='f64')
         P.symmetrize_(h64)
@@ -296,7 +312,7 @@ def _polar(a, stats, opts):
     if polish:
         stats['polish_sweeps'] = polish_sweeps
     stats.update(polar_iterations=it, eigen_sweeps=int(info_h[:, 0].max()),
-                 polar_fixed_point_error=_fixed_point_error(x, mode))
+                 polar_fixed_point_error=_fixed_point_error(x64, 'f64') if polish else _fixed_point_error(x, mode))
     return u, s, v
 
 

@@ -6,8 +6,10 @@ and rounded once; log10 condition ~ U[1, 12]; spectra alternate between geometri
 10^-cond) and clustered (log-uniform on that range with a tenth of the values in a cluster of
 relative width 1e-4).  Matrix i depends only on (seed, i), so any slice can be regenerated.
 
-Targets: log10(sigma_min / |A|_F) from the plain solver -- svd(A, 'polar_qdwh', values_only=True,
-cut='floor') with the reference's l0 = eps -- floored at 1e-8.
+Targets: log10(sigma_min / |A|_F) of the FP32 matrix, from the plain one-sided Jacobi solver run in
+FP64 on it -- svd(A.double(), 'jacobi', values_only=True, cut='floor') -- floored at 1e-16: an FP32
+solver resolves sigma_min only down to its own backward error (about 1e-7 |A|), while the bound QDWH
+needs for a numerically singular FP32 matrix lies below that.
 
     python -m pavel-s_svd_amd.train_l0 data --out DIR [--n 1024 --start 0 --count 2048]
     python -m pavel-s_svd_amd.train_l0 fit --data DIR [--train 1536 --holdout 256] [--model PATH]
@@ -15,9 +17,10 @@ cut='floor') with the reference's l0 = eps -- floored at 1e-8.
 `data` writes DIR/l0_data_<start>_<stop>.npz (features, targets, conditions of that slice);
 `fit` joins the slices in DIR, trains on the first
 train - holdout matrices with the GPU kernels, takes the margin from the holdout, evaluates the
-remaining (test) matrices -- QDWH from eps against QDWH from the learned bound: iterations, QR-branch
-iterations, polar-phase time including the feature and inference cost -- and saves the model with
-the evaluation in its metadata.
+remaining (test) matrices -- QDWH from eps against QDWH from the learned bound, with the reference's
+elementwise stop and with the polished form's orthogonality stop: iterations, QR-branch iterations,
+polar-phase time including the feature and inference cost -- and saves the model with the
+evaluation in its metadata.
 """
 import argparse
 import importlib
@@ -80,9 +83,8 @@ def generate(i0, i1, n, seed=SEED, device='cuda:0'):
 
 
 def targets_of(a):
-    _, s, _ = pkg.svd(a, 'polar_qdwh', values_only=True, cut='floor')
-    s = s.double()
-    return torch.log10((s[:, -1] / s.square().sum(1).sqrt()).clamp_min(1e-8)).float()
+    _, s, _ = pkg.svd(a.double(), 'jacobi', values_only=True, cut='floor')
+    return torch.log10((s[:, -1] / s.square().sum(1).sqrt()).clamp_min(1e-16)).float()
 
 
 def cmd_data(args):
@@ -117,18 +119,24 @@ def load_data(path):
     return {k: np.concatenate(v) for k, v in out.items()}, n
 
 
-def polar_time(a, model):
+def polar_time(a, model, orth_stop=None):
     """Polar phase of QDWH on one matrix (FP32 products): (seconds, iterations, QR-branch
-    iterations, fixed-point error); with a model the time includes features and inference."""
+    iterations, fixed-point error); with a model the time includes features and inference.
+    orth_stop: the polished form's stop (|X^T X - I|_F <= orth_stop, then FP64 Newton-Schulz,
+    whose three steps are included in the time)."""
     x0 = P.scale(a, P.reduce(a, 'fro'), invert=True)
     eps = S.EPS[torch.float32]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     lower = eps if model is None else model.lower_bound(a, eps)
-    x, it, nqr = S._qdwh_iterate(x0, lower, 10 * eps, 40, 'f32', 'f32')
+    x, it, nqr = S._qdwh_iterate(x0, lower, 10 * eps, 40, 'f32', 'f32', orth_stop)
+    if orth_stop is not None:
+        x = x.double()
+        for _ in range(3):
+            x = S._newton_schulz(x)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    return dt, it, nqr, S._fixed_point_error(x, 'f32')
+    return dt, it, nqr, S._fixed_point_error(x, 'f64' if orth_stop is not None else 'f32')
 
 
 def cmd_fit(args):
@@ -150,17 +158,16 @@ def cmd_fit(args):
         idx = test[i0:i0 + CHUNK]
         a = generate(idx[0], idx[-1] + 1, n)
         for j in range(len(idx)):
-            plain = polar_time(a[j:j + 1], None)
-            learned = polar_time(a[j:j + 1], model)
-            rows.append((plain, learned))
+            rows.append(tuple(polar_time(a[j:j + 1], mdl, stop) for stop in (None, 1e-2) for mdl in (None, model)))
         print(f'eval {len(rows)}/{len(test)}', flush=True)
-    arr = np.array([[p[0], p[1], p[2], p[3], q[0], q[1], q[2], q[3]] for p, q in rows])
-    res.update(eval_matrices=len(rows), plain_seconds=float(arr[:, 0].mean()), learned_seconds=float(arr[:, 4].mean()),
-               plain_iterations=float(arr[:, 1].mean()), learned_iterations=float(arr[:, 5].mean()),
-               plain_qr_iterations=float(arr[:, 2].mean()), learned_qr_iterations=float(arr[:, 6].mean()),
-               plain_fixed_point_error_max=float(arr[:, 3].max()),
-               learned_fixed_point_error_max=float(arr[:, 7].max()),
-               learned_more_iterations=int(np.sum(arr[:, 5] > arr[:, 1])))
+    arr = np.array([[v for run in r for v in run] for r in rows])      # 4 runs x (s, it, qr, fpe)
+    for k, name in enumerate(('reference_stop_eps', 'reference_stop_learned', 'orth_stop_eps', 'orth_stop_learned')):
+        c = 4 * k
+        res[name] = {'seconds': float(arr[:, c].mean()), 'iterations': float(arr[:, c + 1].mean()),
+                     'qr_iterations': float(arr[:, c + 2].mean()), 'fixed_point_error_max': float(arr[:, c + 3].max())}
+    res.update(eval_matrices=len(rows),
+               learned_more_iterations_reference_stop=int(np.sum(arr[:, 5] > arr[:, 1])),
+               learned_more_iterations_orth_stop=int(np.sum(arr[:, 13] > arr[:, 9])))
     print(json.dumps(res), flush=True)
     model.save(args.model, evaluation=res)
 
