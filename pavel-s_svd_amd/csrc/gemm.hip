@@ -53,8 +53,63 @@ __device__ __forceinline__ void split_bf16(float x, __bf16 &hi, __bf16 &lo) {
   lo = (__bf16)(x - (float)hi);
 }
 
-// Stage one (rows x BK) operand tile in LDS as S[row][k]; `trans` says the operand is stored
-// k-major (element (row, k) at k*ld + row) rather than row-major (row*ld + k).
+// One (rows x BK) operand tile held in registers between its global load and its LDS store, so
+// that the next tile's loads are in flight while the MFMAs consume the current one.  `trans` says
+// the operand is stored k-major (element (row, k) at k*ld + row) rather than row-major
+// (row*ld + k).  Thread t owns the elements t, t + 256, ...: a fixed column (row-major) or row
+// (k-major) and rows (columns) a constant stride apart, so the addresses advance by one multiply.
+template <typename T, int ROWS, int BK>
+struct TileRegs {
+  static_assert(kGemmThreads % BK == 0 && kGemmThreads % ROWS == 0, "tile shape");
+  static constexpr int E = ROWS * BK / kGemmThreads;
+  static constexpr int RSTEP = kGemmThreads / BK, KSTEP = kGemmThreads / ROWS;
+  T v[E];
+
+  __device__ __forceinline__ void load(const T *src, int64_t ld, bool trans, int64_t row0, int64_t rows, int64_t k0,
+                                       int64_t kend) {
+    const int tid = threadIdx.x;
+    if (!trans) {
+      const int r = tid / BK, kk = tid - (tid / BK) * BK;
+      const int64_t gk = k0 + kk;
+      const T *p = src + (row0 + r) * ld + gk;
+#pragma unroll
+      for (int i = 0; i < E; ++i)
+        v[i] = (row0 + r + i * RSTEP < rows && gk < kend) ? p[(int64_t)i * RSTEP * ld] : T(0);
+    } else {
+      const int kk = tid / ROWS, r = tid - (tid / ROWS) * ROWS;
+      const int64_t gr = row0 + r;
+      const T *p = src + (k0 + kk) * ld + gr;
+#pragma unroll
+      for (int i = 0; i < E; ++i)
+        v[i] = (gr < rows && k0 + kk + i * KSTEP < kend) ? p[(int64_t)i * KSTEP * ld] : T(0);
+    }
+  }
+
+  // LDS layout S[row][k] (ld LDS_LD); SPLIT writes hi = bf16(x) and lo = bf16(x - hi)
+  template <typename S, int LDS_LD, bool SPLIT>
+  __device__ __forceinline__ void store(S *dst, S *dst_lo, bool trans) const {
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      int r, kk;
+      if (!trans) {
+        r = tid / BK + i * RSTEP;
+        kk = tid - (tid / BK) * BK;
+      } else {
+        kk = tid / ROWS + i * KSTEP;
+        r = tid - (tid / ROWS) * ROWS;
+      }
+      if constexpr (SPLIT) {
+        split_bf16((float)v[i], dst[r * LDS_LD + kk], dst_lo[r * LDS_LD + kk]);
+      } else {
+        dst[r * LDS_LD + kk] = v[i];
+      }
+    }
+  }
+};
+
+// Stage one (rows x BK) operand tile straight from global memory into LDS (element loop, few
+// registers: the variant for products whose K spans one or two tiles).  `trans` as for TileRegs.
 template <typename T, typename S, int ROWS, int BK, int LDS_LD, bool SPLIT>
 __device__ __forceinline__ void stage_tile(S *dst, S *dst_lo, const T *src, int64_t ld, bool trans, int64_t row0,
                                            int64_t rows, int64_t k0, int64_t K) {
@@ -85,7 +140,10 @@ __device__ __forceinline__ void stage_tile(S *dst, S *dst_lo, const T *src, int6
   }
 }
 
-template <int MODE>
+// PF: the next k-tile's global loads are issued before the current tile's MFMAs (more registers,
+// one workgroup per CU); without it each tile is loaded and stored in turn (higher occupancy,
+// better when K spans one or two tiles, as in the K = 32 panel updates).
+template <int MODE, bool PF>
 __global__ __launch_bounds__(kGemmThreads) void gemm_kernel(GemmParams P) {
   using Cfg = GemmCfg<MODE>;
   using T = typename Cfg::T;
@@ -118,11 +176,26 @@ __global__ __launch_bounds__(kGemmThreads) void gemm_kernel(GemmParams P) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) acc[i][j][r] = 0;
 
+  TileRegs<T, PF ? BM : kGemmThreads / BK, BK> ra;   // (a one-element tile when unused)
+  TileRegs<T, PF ? BN : kGemmThreads / BK, BK> rb;
+  if (PF && kbeg < kend) {
+    ra.load(A, P.lda, P.ta, m0, P.M, kbeg, kend);
+    rb.load(B, P.ldb, !P.tb, n0, P.N, kbeg, kend);
+  }
   for (int64_t k0 = kbeg; k0 < kend; k0 += BK) {
     __syncthreads();
-    stage_tile<T, S, BM, BK, LD, SPLIT>(As, Alo, A, P.lda, P.ta, m0, P.M, k0, kend);
-    stage_tile<T, S, BN, BK, LD, SPLIT>(Bs, Blo, B, P.ldb, !P.tb, n0, P.N, k0, kend);
+    if constexpr (PF) {
+      ra.template store<S, LD, SPLIT>(As, Alo, P.ta);
+      rb.template store<S, LD, SPLIT>(Bs, Blo, !P.tb);
+    } else {
+      stage_tile<T, S, BM, BK, LD, SPLIT>(As, Alo, A, P.lda, P.ta, m0, P.M, k0, kend);
+      stage_tile<T, S, BN, BK, LD, SPLIT>(Bs, Blo, B, P.ldb, !P.tb, n0, P.N, k0, kend);
+    }
     __syncthreads();
+    if (PF && k0 + BK < kend) {   // next tile's loads overlap this tile's MFMAs
+      ra.load(A, P.lda, P.ta, m0, P.M, k0 + BK, kend);
+      rb.load(B, P.ldb, !P.tb, n0, P.N, k0 + BK, kend);
+    }
     if constexpr (MODE == PSVD_GEMM_BF16X3) {
       v8bf ah[TI], al[TI], bh[TJ], bl[TJ];
 #pragma unroll
@@ -191,7 +264,11 @@ static int launch_gemm(const GemmParams &P, hipStream_t stream) {
   using Cfg = GemmCfg<MODE>;
   const dim3 grid((unsigned)((P.N + Cfg::BN - 1) / Cfg::BN), (unsigned)((P.M + Cfg::BM - 1) / Cfg::BM),
                   (unsigned)(P.batch * P.ksplit));
-  hipLaunchKernelGGL(gemm_kernel<MODE>, grid, dim3(kGemmThreads), 0, stream, P);
+  const int64_t ktiles = (P.kchunk + Cfg::BK - 1) / Cfg::BK;
+  if (ktiles > 2)
+    hipLaunchKernelGGL((gemm_kernel<MODE, true>), grid, dim3(kGemmThreads), 0, stream, P);
+  else
+    hipLaunchKernelGGL((gemm_kernel<MODE, false>), grid, dim3(kGemmThreads), 0, stream, P);
   return hipGetLastError() == hipSuccess ? PSVD_OK : PSVD_ELAUNCH;
 }
 
