@@ -117,6 +117,11 @@ int psvd_axpby(int dtype, int64_t batch, int rows, int cols, double alpha, const
  * perm: batch * n int32 workspace. */
 int psvd_sort_desc(int dtype, int64_t batch, int rows, int n, const double *keys, void *s, const void *x, int64_t ldx,
                    int64_t sx, void *y, int64_t ldy, int64_t sy, int32_t *perm, psvd_stream_t stream);
+/* psvd_scale_columns: x_b[:, c] /= d_b[c] wherever d_b[c] > 0, in place (d in x's dtype, row stride
+ * sd per matrix): U = W / sigma of solvers.py:18-21 (_finish_columns) and U = A V / sigma of
+ * solvers.py:116 (_gram), before the completion of the columns whose sigma is cut. */
+int psvd_scale_columns(int dtype, int64_t batch, int rows, int cols, void *x, int64_t ldx, int64_t sx, const void *d,
+                       int64_t sd, psvd_stream_t stream);
 
 /* ---------------------------------------------------------------------------------------
  * Completion of missing basis columns (primitives.py:430-467): for each b, columns [nk[b], n) of
@@ -204,6 +209,27 @@ size_t psvd_jacobi_blocked_workspace_bytes(int dtype, int m, int n);
 int psvd_jacobi_blocked(int dtype, int m, int n, const void *a, void *u, void *s, void *vh, int32_t *info, double tol,
                         int max_sweeps, int flags, int inner_sweeps, void *workspace, size_t workspace_bytes,
                         psvd_stream_t stream);
+
+/* ---------------------------------------------------------------------------------------
+ * Block-pair one-sided Jacobi (solvers.py:30-68, `block_jacobi`): ONE sweep over the tournament of
+ * column groups of `block_size` columns (the last group may be short; an odd count gets an empty
+ * partner) on W (batch x m x n) and V (batch x n x n), both updated in place.  Each round runs one
+ * workgroup per group pair and matrix: the pair's k <= 2 block_size (<= 64, else PSVD_ETOOBIG)
+ * columns are orthogonalised by the reference's one-sided Jacobi (tolerance tol, 0 = 10 eps; at
+ * most max_sweeps sweeps; primitives.py:316-367) carried out on their FP64 Gram matrix, and
+ * W_pair, V_pair are multiplied by the accumulated rotation with its columns ordered by
+ * descending norm (solvers.py:48-55).  Then, for the sweep's convergence test (solvers.py:56-62):
+ * corr[b] = max over live columns i != j of |w_i^T w_j| / (|w_i| |w_j|) (live: |w_i|^2 >
+ * (eps max(m, n))^2 max_c |w_c|^2), sigma[b * n + c] = |w_c| (FP64 sums throughout).
+ * info[b] = {largest pair sweep count, 1 if a pair did not converge}.  The caller stops when
+ * corr <= its tolerance (40 eps by default).  A single group (n <= block_size) is PSVD_EINVAL: that
+ * case is the plain one-sided Jacobi SVD.  Workspace: psvd_block_jacobi_workspace_bytes(batch, n).
+ * ------------------------------------------------------------------------------------- */
+size_t psvd_block_jacobi_workspace_bytes(int64_t batch, int n);
+int psvd_block_jacobi_sweep(int dtype, int64_t batch, int m, int n, int block_size, void *w, int64_t ldw,
+                            int64_t stride_w, void *v, int64_t ldv, int64_t stride_v, double tol, int max_sweeps,
+                            int32_t *info, double *corr, double *sigma, void *workspace, size_t workspace_bytes,
+                            psvd_stream_t stream);
 
 /* ---------------------------------------------------------------------------------------
  * Learned QDWH lower bound (no reference counterpart: the reference starts QDWH from l0 = eps,

@@ -17,6 +17,10 @@ the leading batch dimension, so that iteration counts and rounding behaviour tra
     tsqr                   svdbench/standalone/solvers.py:71-96
     polar_qdwh             svdbench/standalone/solvers.py:122-177
     raw / svd              svdbench/standalone/solvers.py:224-311
+    finish_columns         svdbench/standalone/solvers.py:18-21
+    block_jacobi           svdbench/standalone/solvers.py:30-68
+    gram_jacobi            svdbench/standalone/solvers.py:99-119 (the Jacobi branch)
+    sketch / partial       svdbench/standalone/solvers.py:180-221 (randomized, subspace, block_krylov)
 
 Shapes follow the reference: A is (b, m, n).  Real and complex dtypes are supported.
 """
@@ -30,6 +34,8 @@ ALGORITHMS = (
     'polar_halley', 'polar_polynomial', 'randomized', 'subspace', 'block_krylov', 'lanczos',
 )
 HOT_PATH = ('jacobi', 'tsqr', 'polar_qdwh')
+CONSUMERS = ('qr', 'block_jacobi', 'gram_jacobi', 'randomized', 'subspace', 'block_krylov')
+PARTIAL = ('randomized', 'subspace', 'block_krylov')
 
 
 def real_dtype(dt):
@@ -437,20 +443,153 @@ def polar_qdwh(a, stats, opts):
     return u, s, v
 
 
+def finish_columns(w, s):
+    """U = W / sigma for sigma above eps * max(m, n) * sigma_1, completed elsewhere; sigma zeroed
+    there (solvers.py:18-21)."""
+    keep = s > eps(w.dtype) * max(w.shape[-2:]) * s[:, :1]
+    u = w / np.where(keep, s, np.ones_like(s))[:, None, :]
+    return complete_null_columns(u, keep), np.where(keep, s, np.zeros_like(s))
+
+
+def block_jacobi(a, stats, opts):
+    """Block-pair one-sided Jacobi (solvers.py:30-68): column groups of block_size, paired by the
+    round-robin tournament; each pair's columns are replaced by W_pair V_pair from a full one-sided
+    Jacobi SVD of W_pair (V_pair sorted by descending sigma); a sweep ends with the Gram test
+    max |g_ij| / sqrt(g_ii g_jj) over live columns <= tol (40 eps by default)."""
+    b, m, n = a.shape
+    width = opts.get('block_size', 4)
+    if width < 1:
+        raise ValueError('block_size must be positive')
+    groups = [list(range(j, min(j + width, n))) for j in range(0, n, width)]
+    if len(groups) == 1:
+        return _jacobi(a, stats, opts)
+    if len(groups) % 2:
+        groups.append([])
+    w, v = a.copy(), _eye(a, n)
+    tol = opts.get('tol') or 40 * eps(a.dtype)
+    rdt = real_dtype(a.dtype)
+    converged = False
+    sweep = 0
+    for sweep in range(opts.get('max_sweeps', 30)):
+        for pairs in round_robin(len(groups)):
+            for i, j in pairs:
+                cols = groups[i] + groups[j]
+                if len(cols) < 2:
+                    continue
+                _, _, vv = _jacobi(w[:, :, cols], stats, opts)
+                w[:, :, cols] = w[:, :, cols] @ vv
+                v[:, :, cols] = v[:, :, cols] @ vv
+        g = mh(w) @ w
+        d = np.maximum(np.diagonal(g, axis1=-2, axis2=-1).real, 0).astype(rdt)
+        denom = np.sqrt(d)[:, :, None] * np.sqrt(d)[:, None, :]
+        g[:, np.arange(n), np.arange(n)] = 0
+        live = d > (eps(a.dtype) * max(m, n)) ** 2 * d.max(-1, keepdims=True)
+        corr = np.where(live[:, :, None] & live[:, None, :], np.abs(g) / np.maximum(denom, tiny(rdt)), 0)
+        if corr.max() <= tol:
+            converged = True
+            break
+    if not converged:
+        raise RuntimeError('block Jacobi did not converge')
+    s = np.sqrt((np.abs(w) ** 2).sum(1))
+    order = np.argsort(-s, axis=-1, kind='stable')
+    s = np.take_along_axis(s, order, 1)
+    w = np.take_along_axis(w, order[:, None, :], 2)
+    v = np.take_along_axis(v, order[:, None, :], 2)
+    u, s = finish_columns(w, s)
+    stats['block_sweeps'] = sweep + 1
+    return u, s, v
+
+
+def gram_jacobi(a, stats, opts):
+    """Eigen-decomposition of the Gram matrix (solvers.py:99-119, Jacobi branch): G = sym(A^H A),
+    Hermitian Jacobi, eigenvalues descending; sigma = sqrt(w) above eps * n * w_1 (zero below,
+    counted in gram_discarded_values), U = A V / sigma completed."""
+    g = mh(a) @ a
+    g = (g + mh(g)) / 2
+    w, v, it = jacobi_eigh(g, max_sweeps=opts.get('max_sweeps', 60))
+    stats['eigen_sweeps'] = it
+    w, v = w[:, ::-1], v[:, :, ::-1]
+    cut = eps(a.dtype) * a.shape[-1] * np.maximum(w[:, :1], 0)
+    keep = w > cut
+    s = np.where(keep, np.sqrt(np.maximum(w, 0)), 0).astype(w.dtype)
+    u = (a @ v) / np.where(keep, s, np.ones_like(s))[:, None, :]
+    u = complete_null_columns(u, keep)
+    stats['gram_discarded_values'] = int((~keep).sum())
+    return u, s, v
+
+
+def sketch(a, columns, seed):
+    """The Gaussian test matrix (b, n, columns) of solvers.py:180-182, drawn by torch's CPU generator
+    (the reference's call on a CPU tensor) and returned as numpy."""
+    import torch
+    gen = torch.Generator().manual_seed(seed)
+    tdt = {np.dtype(np.float32): torch.float32, np.dtype(np.float64): torch.float64,
+           np.dtype(np.complex64): torch.complex64, np.dtype(np.complex128): torch.complex128}[np.dtype(a.dtype)]
+    return torch.randn(a.shape[0], a.shape[-1], columns, dtype=tdt, generator=gen).numpy()
+
+
+def partial(a, stats, opts, method):
+    """Randomized range finder with power (subspace) iterations or a block Krylov basis, then the
+    Jacobi SVD of Q^H A (solvers.py:185-221)."""
+    k = opts.get('rank', min(a.shape[-2:]))
+    ell = min(k + opts.get('oversample', 8), min(a.shape[-2:]))
+    omega = opts.get('initial_basis')
+    if omega is None:
+        omega = sketch(a, ell, opts.get('seed', 20261011))
+    else:
+        omega = np.asarray(omega)
+        if omega.ndim == 2:
+            omega = omega[None]
+        if omega.shape != (a.shape[0], a.shape[-1], ell):
+            raise ValueError('initial_basis shape must match batch, n, rank+oversample')
+        omega = omega.astype(a.dtype)
+        stats['warm_started'] = True
+    q, _ = qr(a @ omega)
+    bases = [q]
+    iterations = opts.get('iterations', 2)
+    for _ in range(iterations):
+        z, _ = qr(mh(a) @ q)
+        y = a @ z
+        if method == 'block_krylov':
+            joined = np.concatenate(bases, -1)
+            available = a.shape[-2] - joined.shape[-1]
+            if available <= 0:
+                break
+            for _ in range(2):
+                y = y - joined @ (mh(joined) @ y)
+            q, _ = qr(y[:, :, :min(ell, available)])
+            bases.append(q)
+        else:
+            q, _ = qr(y)
+    if method == 'block_krylov':
+        q, _ = qr(np.concatenate(bases, -1))
+    small = mh(q) @ a
+    u, s, v = raw(small, 'jacobi', stats, opts)
+    stats.update(sketch_columns=ell, range_columns=q.shape[-1], power_iterations=iterations)
+    return (q @ u)[:, :, :k], s[:, :k], v[:, :, :k]
+
+
 def raw(a, method, stats, opts):
-    """Wide inputs are solved transposed; dispatch by name (solvers.py:224-261)."""
-    if a.shape[-2] < a.shape[-1]:
+    """Wide inputs are solved transposed, except by the partial methods; dispatch by name
+    (solvers.py:224-261)."""
+    if a.shape[-2] < a.shape[-1] and method not in PARTIAL:
         u, s, v = raw(mh(a), method, stats, opts)
         return v, s, u
     if method == 'jacobi':
         return _jacobi(a, stats, opts)
+    if method == 'block_jacobi':
+        return block_jacobi(a, stats, opts)
     if method in ('qr', 'tsqr'):
         q, r = qr(a) if method == 'qr' else tsqr(a, opts.get('tile_rows', 4 * a.shape[-1]), stats)
         u, s, v = _jacobi(r, stats, opts)
         return q @ u, s, v
+    if method == 'gram_jacobi':
+        return gram_jacobi(a, stats, opts)
     if method == 'polar_qdwh':
         return polar_qdwh(a, stats, opts)
-    raise ValueError(f'the oracle restates only {HOT_PATH + ("qr",)}, not {method!r}')
+    if method in PARTIAL:
+        return partial(a, stats, opts, method)
+    raise ValueError(f'the oracle restates only {HOT_PATH + CONSUMERS}, not {method!r}')
 
 
 def svd(a, algorithm='jacobi', *, full_matrices=False, values_only=False, return_info=False, **opts):

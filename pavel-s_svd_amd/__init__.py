@@ -1,4 +1,5 @@
-"""MI355X-native (gfx950) standalone SVD hot path: one-sided Jacobi, QDWH polar and TSQR.
+"""MI355X-native (gfx950) standalone SVD hot path: one-sided Jacobi, QDWH polar and TSQR, and
+the solvers built on them (block-pair Jacobi, Gram + Jacobi, randomized range finders).
 
 Drop-in for ``svdbench.standalone``: ``svd(a, algorithm, ...)``, ``ALGORITHMS`` and
 ``factory(a, spec)`` keep the reference's names, options, stats and exceptions, and run on

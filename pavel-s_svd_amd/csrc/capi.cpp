@@ -247,6 +247,36 @@ static PyObject *py_sort_desc(PyObject *, PyObject *args) {
                                         ldx, sx, PTR(y), ldy, sy, static_cast<int32_t *>(PTR(perm)), PTR(stream)));
 }
 
+static PyObject *py_scale_columns(PyObject *, PyObject *args) {
+  int dtype, rows, cols;
+  long long batch, ldx, sx, sd;
+  unsigned long long x, d, stream;
+  if (!PyArg_ParseTuple(args, "iLiiKLLKLK", &dtype, &batch, &rows, &cols, &x, &ldx, &sx, &d, &sd, &stream))
+    return nullptr;
+  return PyLong_FromLong(psvd_scale_columns(dtype, batch, rows, cols, PTR(x), ldx, sx, PTR(d), sd, PTR(stream)));
+}
+
+static PyObject *py_block_jacobi_workspace_bytes(PyObject *, PyObject *args) {
+  long long batch;
+  int n;
+  if (!PyArg_ParseTuple(args, "Li", &batch, &n)) return nullptr;
+  return PyLong_FromSize_t(psvd_block_jacobi_workspace_bytes(batch, n));
+}
+
+static PyObject *py_block_jacobi_sweep(PyObject *, PyObject *args) {
+  int dtype, m, n, width, max_sweeps;
+  long long batch, ldw, sw, ldv, sv;
+  double tol;
+  unsigned long long w, v, info, corr, sigma, ws, wsb, stream;
+  if (!PyArg_ParseTuple(args, "iLiiiKLLKLLdiKKKKKK", &dtype, &batch, &m, &n, &width, &w, &ldw, &sw, &v, &ldv, &sv,
+                        &tol, &max_sweeps, &info, &corr, &sigma, &ws, &wsb, &stream))
+    return nullptr;
+  return PyLong_FromLong(psvd_block_jacobi_sweep(dtype, batch, m, n, width, PTR(w), ldw, sw, PTR(v), ldv, sv, tol,
+                                                 max_sweeps, static_cast<int32_t *>(PTR(info)),
+                                                 static_cast<double *>(PTR(corr)), static_cast<double *>(PTR(sigma)),
+                                                 PTR(ws), (size_t)wsb, PTR(stream)));
+}
+
 static PyObject *py_mlp_param_count(PyObject *, PyObject *) { return PyLong_FromLong(psvd_mlp_param_count()); }
 
 static PyObject *py_mlp_workspace_bytes(PyObject *, PyObject *args) {
@@ -335,6 +365,9 @@ static PyMethodDef kMethods[] = {
     {"symmetrize", py_symmetrize, METH_VARARGS, nullptr},
     {"axpby", py_axpby, METH_VARARGS, nullptr},
     {"sort_desc", py_sort_desc, METH_VARARGS, nullptr},
+    {"scale_columns", py_scale_columns, METH_VARARGS, nullptr},
+    {"block_jacobi_workspace_bytes", py_block_jacobi_workspace_bytes, METH_VARARGS, nullptr},
+    {"block_jacobi_sweep", py_block_jacobi_sweep, METH_VARARGS, nullptr},
     {"mlp_param_count", py_mlp_param_count, METH_NOARGS, nullptr},
     {"mlp_workspace_bytes", py_mlp_workspace_bytes, METH_VARARGS, nullptr},
     {"mlp_forward", py_mlp_forward, METH_VARARGS, nullptr},

@@ -167,6 +167,19 @@ __global__ __launch_bounds__(kUtilThreads) void gather_cols_kernel(const double 
   if (r == 0) s[b * n + c] = (T)fmax(keys[b * n + src], 0.0);
 }
 
+// x[:, c] /= d[c] for d[c] > 0 (others unchanged): U = W / sigma of solvers.py:18-21 and :116
+template <typename T>
+__global__ __launch_bounds__(kUtilThreads) void scale_cols_kernel(T *x, int64_t ldx, int64_t sx, int rows, int cols,
+                                                                  const T *d, int64_t sd) {
+  const int64_t b = blockIdx.y;
+  const int64_t total = (int64_t)rows * cols;
+  for (int64_t e = (int64_t)blockIdx.x * kUtilThreads + threadIdx.x; e < total; e += (int64_t)gridDim.x * kUtilThreads) {
+    const int r = (int)(e / cols), c = (int)(e - (int64_t)r * cols);
+    const T f = d[b * sd + c];
+    if (f > T(0)) x[b * sx + (int64_t)r * ldx + c] /= f;
+  }
+}
+
 }  // namespace psvd
 
 extern "C" {
@@ -253,6 +266,24 @@ int psvd_sort_desc(int dtype, int64_t batch, int rows, int n, const double *keys
     hipLaunchKernelGGL(psvd::gather_cols_kernel<double>, grid, dim3(psvd::kUtilThreads), 0, st, keys, perm, rows, n,
                        static_cast<const double *>(x), ldx, sx, static_cast<double *>(y), ldy, sy,
                        static_cast<double *>(s));
+  else
+    return PSVD_EINVAL;
+  return hipGetLastError() == hipSuccess ? PSVD_OK : PSVD_ELAUNCH;
+}
+
+int psvd_scale_columns(int dtype, int64_t batch, int rows, int cols, void *x, int64_t ldx, int64_t sx, const void *d,
+                       int64_t sd, psvd_stream_t stream) {
+  if (batch < 0 || rows < 0 || cols < 0 || batch > 65535 || ldx < cols || sd < cols) return PSVD_EINVAL;
+  if (batch == 0 || rows == 0 || cols == 0) return PSVD_OK;
+  if (!x || !d) return PSVD_EINVAL;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const dim3 grid(psvd::grid_chunks((int64_t)rows * cols), (unsigned)batch);
+  if (dtype == PSVD_F32)
+    hipLaunchKernelGGL(psvd::scale_cols_kernel<float>, grid, dim3(psvd::kUtilThreads), 0, st, static_cast<float *>(x),
+                       ldx, sx, rows, cols, static_cast<const float *>(d), sd);
+  else if (dtype == PSVD_F64)
+    hipLaunchKernelGGL(psvd::scale_cols_kernel<double>, grid, dim3(psvd::kUtilThreads), 0, st,
+                       static_cast<double *>(x), ldx, sx, rows, cols, static_cast<const double *>(d), sd);
   else
     return PSVD_EINVAL;
   return hipGetLastError() == hipSuccess ? PSVD_OK : PSVD_ELAUNCH;

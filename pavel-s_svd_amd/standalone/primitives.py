@@ -318,6 +318,47 @@ def sort_desc(keys, x):
     return s, y
 
 
+def scale_columns_(x, d):
+    """In place: x[b, :, c] /= d[b, c] wherever d[b, c] > 0 (d (b, cols) in x's dtype)."""
+    x3, ldx, sx = _mat3(x)
+    b, rows, cols = x3.shape
+    d = d.contiguous()
+    if tuple(d.shape) != (b, cols) or d.dtype != x.dtype:
+        raise ValueError('scale_columns_ takes d of shape (batch, cols) in the dtype of x')
+    st = N._hip.scale_columns(N.dtype_code(x.dtype), b, rows, cols, N.ptr(x3), ldx, sx, N.ptr(d), cols,
+                              N.stream_of(x))
+    N.check(st, 'psvd_scale_columns')
+    return x
+
+
+class BlockJacobiSweeper:
+    """Sweeps of the block-pair one-sided Jacobi (solvers.py:30-68) on W (b, m, n) and V (b, n, n),
+    both row-contiguous and updated in place; device outputs per sweep: corr (b,) FP64 (the Gram
+    test value), sigma (b, n) FP64 column norms, info (b, 2) int32 (largest pair sweep count,
+    non-convergence flag)."""
+
+    def __init__(self, w, v, block_size):
+        b, m, n = w.shape
+        if w.stride(-1) != 1 or v.stride(-1) != 1 or w.stride(-2) != n or v.stride(-2) != n:
+            raise ValueError('block Jacobi takes row-contiguous W and V')
+        self.w, self.v, self.block_size = w, v, int(block_size)
+        dev = w.device
+        self.info = torch.empty(b, 2, dtype=torch.int32, device=dev)
+        self.corr = torch.empty(b, dtype=torch.float64, device=dev)
+        self.sigma = torch.empty(b, n, dtype=torch.float64, device=dev)
+        wsb = N._hip.block_jacobi_workspace_bytes(b, n)
+        self.ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
+
+    def sweep(self, tol, max_sweeps):
+        w, v = self.w, self.v
+        b, m, n = w.shape
+        st = N._hip.block_jacobi_sweep(N.dtype_code(w.dtype), b, m, n, self.block_size, N.ptr(w), n, m * n,
+                                       N.ptr(v), n, n * n, float(tol or 0.0), int(max_sweeps), N.ptr(self.info),
+                                       N.ptr(self.corr), N.ptr(self.sigma), N.ptr(self.ws), self.ws.numel(),
+                                       N.stream_of(w))
+        N.check(st, 'psvd_block_jacobi_sweep')
+
+
 def complete_columns_(u, nk):
     """Complete columns [nk_b, n) of u (b, m, n) in place (primitives.py:430-455); nk is a (b,)
     int32 device tensor.  Returns the (b,) status tensor (device, no synchronisation)."""

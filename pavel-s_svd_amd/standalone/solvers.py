@@ -2,9 +2,12 @@
 
 Same names, signatures, options, stats keys and exceptions as svdbench/standalone/solvers.py:
 ``ALGORITHMS`` (:7-11), ``svd`` (:264-311), ``factory`` (:313-324).  The hot-path families
-``jacobi``, ``tsqr`` and ``polar_qdwh`` run on hand-written gfx950 kernels; the other names stay
-in ``ALGORITHMS`` (so argument validation is unchanged) and raise ``Unsupported`` -- a
-RuntimeError the harness records as an unsupported row -- instead of falling back to anything.
+``jacobi``, ``qr``, ``tsqr`` and ``polar_qdwh`` run on hand-written gfx950 kernels, and so do their
+consumers ``block_jacobi`` (its own fused pair kernel), ``gram_jacobi`` and the range finders
+``randomized``, ``subspace``, ``block_krylov`` (GEMM, QR and Jacobi kernels of the hot path); the
+other names (bidiagonal and tridiagonal D&C families, the other polar iterations, Lanczos) stay in
+``ALGORITHMS`` (so argument validation is unchanged) and raise ``Unsupported`` -- a RuntimeError
+the harness records as an unsupported row -- instead of falling back to anything.
 
 Extension options (no reference counterpart):
     cut='reference' | 'floor'   sigma below eps*max(m,n)*sigma_1 is zeroed and completed (the
@@ -26,7 +29,8 @@ Extension options (no reference counterpart):
                                 eigh_inner_sweeps: sweeps per 64 x 64 subproblem (blocked solver)
     l0_model=None               QDWH: start from the learned lower bound (an L0Model, a JSON path,
                                 or True for the packaged model); stats l0, l0_source
-    eigh_method='auto'          QDWH eigensolver: 'cyclic' (the reference's rotation order),
+    block_size=4                block_jacobi: columns per group (the reference's option; <= 32 here)
+    eigh_method='auto'          QDWH and gram_jacobi eigensolver: 'cyclic' (the reference's rotation order),
                                 'blocked' (64 x 64 subproblems + MFMA) or 'auto' (LDS kernel when
                                 H fits, blocked otherwise)
 """
@@ -48,7 +52,9 @@ ALGORITHMS = (
     'gram_jacobi', 'gram_dc', 'polar_qdwh', 'polar_newton_schulz',
     'polar_halley', 'polar_polynomial', 'randomized', 'subspace', 'block_krylov', 'lanczos',
 )
-SUPPORTED = ('jacobi', 'qr', 'tsqr', 'polar_qdwh')
+SUPPORTED = ('jacobi', 'qr', 'tsqr', 'polar_qdwh', 'block_jacobi', 'gram_jacobi', 'randomized', 'subspace',
+             'block_krylov')
+PARTIAL = ('randomized', 'subspace', 'block_krylov')
 EPS = {torch.float32: 1.1920928955078125e-07, torch.float64: 2.220446049250313e-16}
 
 
@@ -64,6 +70,142 @@ def _jacobi(x, stats, opts, scale):
     N.raise_for_info(info, f'one-sided Jacobi did not converge in {max_sweeps} sweeps')
     stats['jacobi_sweeps'] = max(stats.get('jacobi_sweeps', 0), int(info[:, 0].max()))
     return u, s, vh
+
+
+def _finish_columns(w, s):
+    """U = W / sigma for sigma above eps * max(m, n) * sigma_1, completed below the cut, where sigma is
+    zeroed (solvers.py:18-21).  W (b, m, n) is overwritten; s is sorted descending."""
+    b, m, n = w.shape
+    keep = s > EPS[w.dtype] * max(m, n) * s[:, :1]
+    P.scale_columns_(w, s)
+    if not bool(keep.all()):
+        status = P.complete_columns_(w, keep.sum(-1).to(torch.int32))
+        if bool((status != 0).any()):
+            raise RuntimeError('null-space completion failed: supplied live columns are not independent')
+    return w, torch.where(keep, s, torch.zeros_like(s))
+
+
+def _block_jacobi(x, stats, opts):
+    """Block-pair one-sided Jacobi (solvers.py:30-68): column groups of block_size (default 4) paired
+    by the round-robin tournament, each pair orthogonalised by a full one-sided Jacobi solve and
+    replaced by W_pair V_pair (sorted), a sweep ending with the Gram test max |g_ij| / sqrt(g_ii
+    g_jj) <= tol (40 eps by default) over live columns.  One fused kernel per round on the GPU
+    (block_jacobi.hip); one host read per sweep, as the reference's bool(corr.max() <= tol)."""
+    b, m, n = x.shape
+    width = opts.get('block_size', 4)
+    if width < 1:
+        raise ValueError('block_size must be positive')
+    if -(-n // width) == 1:
+        return _jacobi(x, stats, opts, scale=False)
+    if 2 * width > 64:
+        raise Unsupported(f'block_size {width} > 32: the pair Gram matrices of this build are held on chip')
+    from .. import native as N
+    w = x.clone()
+    v = _eye(b, n, x.dtype, x.device)
+    sweeper = P.BlockJacobiSweeper(w, v, width)
+    tol = opts.get('tol') or 40 * EPS[x.dtype]
+    inner_sweeps = opts.get('max_sweeps', 60)
+    jac, converged, sweep = 0, False, 0
+    for sweep in range(opts.get('max_sweeps', 30)):
+        sweeper.sweep(opts.get('tol'), inner_sweeps)
+        info = sweeper.info.cpu()
+        N.raise_for_info(info, f'one-sided Jacobi did not converge in {inner_sweeps} sweeps')
+        jac = max(jac, int(info[:, 0].max()))
+        if float(sweeper.corr.max()) <= tol:
+            converged = True
+            break
+    stats['jacobi_sweeps'] = max(stats.get('jacobi_sweeps', 0), jac)
+    if not converged:
+        raise RuntimeError('block Jacobi did not converge')
+    s, w = P.sort_desc(sweeper.sigma, w)
+    _, v = P.sort_desc(sweeper.sigma, v)
+    u, s = _finish_columns(w, s)
+    stats['block_sweeps'] = sweep + 1
+    return u, s, v.mT
+
+
+def _gram(x, stats, opts):
+    """Gram matrix and Hermitian Jacobi (solvers.py:99-119, the gram_jacobi branch): G = sym(A^T A)
+    on MFMA, eigenvalues descending, sigma = sqrt(w) above eps * n * w_1 (zero below, counted in
+    gram_discarded_values), U = A V / sigma on MFMA with the cut columns completed."""
+    b, m, n = x.shape
+    mode = _gmode(x, opts)
+    g = gemm(x.mT, x, mode=mode)
+    P.symmetrize_(g)
+    w, v, it = P.jacobi_eigh(g, max_sweeps=opts.get('max_sweeps', 60), method=opts.get('eigh_method', 'auto'))
+    stats['eigen_sweeps'] = it
+    w = w.flip(-1)
+    v = v.flip(-1).contiguous()
+    keep = w > EPS[x.dtype] * n * w[:, :1].clamp_min(0)
+    s = torch.where(keep, w.clamp_min(0).sqrt(), torch.zeros_like(w))
+    stats['gram_discarded_values'] = int((~keep).sum())
+    if opts.get('_values_only', False):
+        return None, s, None
+    u = gemm(x, v, mode=mode)
+    P.scale_columns_(u, s)
+    if not bool(keep.all()):
+        status = P.complete_columns_(u, keep.sum(-1).to(torch.int32))
+        if bool((status != 0).any()):
+            raise RuntimeError('null-space completion failed: supplied live columns are not independent')
+    return u, s, v.mT
+
+
+def _sketch(x, columns, seed):
+    """Gaussian test matrix (b, n, columns) from torch's generator on the input's device, seeded as
+    the reference's (solvers.py:180-182)."""
+    gen = torch.Generator(device=x.device).manual_seed(seed)
+    return torch.randn(x.shape[0], x.shape[-1], columns, dtype=x.dtype, device=x.device, generator=gen)
+
+
+def _partial(x, stats, opts, method):
+    """Randomized range finder (solvers.py:185-221): Q = orth(A Omega), `iterations` power steps
+    Q <- orth(A orth(A^T Q)) ('randomized', 'subspace'), or a block Krylov basis of the same steps
+    re-orthogonalised twice against the earlier blocks ('block_krylov'); then the Jacobi SVD of
+    Q^T A and U = Q U_small, truncated to `rank`.  Products on MFMA, QRs by the panel kernels."""
+    b, m, n = x.shape
+    k = opts.get('rank', min(m, n))
+    ell = min(k + opts.get('oversample', 8), min(m, n))
+    omega = opts.get('initial_basis')
+    if omega is None:
+        omega = _sketch(x, ell, opts.get('seed', 20261011))
+    else:
+        if omega.ndim == 2:
+            omega = omega[None]
+        if tuple(omega.shape) != (b, n, ell):
+            raise ValueError('initial_basis shape must match batch, n, rank+oversample')
+        omega = omega.to(device=x.device, dtype=x.dtype)
+        stats['warm_started'] = True
+    mode = _gmode(x, opts)
+    qmode = 'bf16x3' if mode == 'bf16x3' else 'f32'
+
+    def orth(y):
+        return P.qr(y.contiguous(), gemm=qmode)[0]
+
+    q = orth(gemm(x, omega.contiguous(), mode=mode))
+    bases = [q]
+    iterations = opts.get('iterations', 2)
+    for _ in range(iterations):
+        z = orth(gemm(x.mT, q, mode=mode))
+        y = gemm(x, z, mode=mode)
+        if method == 'block_krylov':
+            joined = torch.cat(bases, -1)
+            available = m - joined.shape[-1]
+            if available <= 0:
+                break
+            for _ in range(2):
+                y = gemm(joined, gemm(joined.mT, y, mode=mode), alpha=-1.0, beta=1.0, out=y, mode=mode)
+            q = orth(y[:, :, :min(ell, available)])
+            bases.append(q)
+        else:
+            q = orth(y)
+    if method == 'block_krylov':
+        q = orth(torch.cat(bases, -1))
+    small = gemm(q.mT, x, mode=mode)
+    u, s, vh = _raw(small, 'jacobi', stats, opts)
+    stats.update(sketch_columns=ell, range_columns=q.shape[-1], power_iterations=iterations)
+    if u is None:
+        return None, s[:, :k], None
+    return gemm(q, u, mode=mode)[:, :, :k], s[:, :k], vh[:, :k, :]
 
 
 def _gmode(x, opts):
@@ -317,12 +459,19 @@ def _polar(a, stats, opts):
 
 
 def _raw(x, method, stats, opts):
-    """Wide inputs are solved transposed (solvers.py:226-228); dispatch by name."""
-    if x.shape[-2] < x.shape[-1]:
+    """Wide inputs are solved transposed (solvers.py:226-228) except by the partial methods; dispatch
+    by name (solvers.py:229-261)."""
+    if x.shape[-2] < x.shape[-1] and method not in PARTIAL:
         u, s, vh = _raw(x.mT.contiguous(), method, stats, opts)
         return (None if vh is None else vh.mT), s, (None if u is None else u.mT)
     if method == 'jacobi':
         return _jacobi(x, stats, opts, scale=not opts.get('_prescaled', False))
+    if method == 'block_jacobi':
+        return _block_jacobi(x, stats, opts)
+    if method == 'gram_jacobi':
+        return _gram(x, stats, opts)
+    if method in PARTIAL:
+        return _partial(x, stats, opts, method)
     if method in ('qr', 'tsqr'):
         n = x.shape[-1]
         values = opts.get('_values_only', False)

@@ -6,9 +6,12 @@ writes:
 
 * ``tests/golden/corpus/<case>.npy`` -- the exact input bytes (numpy, ``allow_pickle=False``);
 * ``tests/golden/expected.json``      -- per case: shape, dtype, sha256 of the input, and for each
-  of ``jacobi``, ``tsqr``, ``polar_qdwh`` the recorded iteration counts (``jacobi_sweeps``,
-  ``tsqr_leaves``, ``polar_iterations``, ``eigen_sweeps``, ``polar_fixed_point_error``) and the
-  worst-case accuracy metrics, for every device the reference recorded (``cpu`` / ``cuda:0``).
+  of ``jacobi``, ``tsqr``, ``polar_qdwh`` (the hot path) and ``block_jacobi``, ``gram_jacobi``,
+  ``randomized``, ``subspace``, ``block_krylov`` (its consumers) the solver options of the row, the
+  recorded counts (``jacobi_sweeps``, ``tsqr_leaves``, ``polar_iterations``, ``eigen_sweeps``,
+  ``polar_fixed_point_error``, ``block_sweeps``, ``gram_discarded_values``, ``sketch_columns``,
+  ``range_columns``, ``power_iterations``) and the worst-case accuracy metrics, for every device the
+  reference recorded (``cpu`` / ``cuda:0``).
 
 Run once in a container where the reference tree is mounted::
 
@@ -22,10 +25,12 @@ from pathlib import Path
 
 import numpy as np
 
-HOT = ('jacobi', 'tsqr', 'polar_qdwh')
+HOT = ('jacobi', 'tsqr', 'polar_qdwh', 'block_jacobi', 'gram_jacobi', 'randomized', 'subspace', 'block_krylov')
+KEYS = ('jacobi_sweeps', 'tsqr_leaves', 'polar_iterations', 'eigen_sweeps', 'polar_fixed_point_error',
+        'block_sweeps', 'gram_discarded_values', 'sketch_columns', 'range_columns', 'power_iterations')
 FILES = ('standalone-cpu-hetzner.jsonl', 'standalone-hetzner.jsonl', 'standalone-scale-hetzner.jsonl')
 CASES = ('square32', 'square64', 'batch8x16', 'tall256x16', 'tall2048x32', 'condition1e8',
-         'condition1e12', 'rank_deficient', 'zero', 'clustered_complex', 'wide_complex')
+         'condition1e12', 'rank_deficient', 'zero', 'clustered_complex', 'wide_complex', 'lowrank512')
 
 
 def main(ref):
@@ -50,11 +55,10 @@ def main(ref):
                 continue
             if row['input_sha256'] != out[case]['sha256']:
                 raise SystemExit(f'{name}:{line_no}: input hash does not match the corpus file')
-            diag = {k: v for k, v in row['native_diagnostics'].items()
-                    if k in ('jacobi_sweeps', 'tsqr_leaves', 'polar_iterations', 'eigen_sweeps',
-                             'polar_fixed_point_error')}
+            diag = {k: v for k, v in row['native_diagnostics'].items() if k in KEYS}
+            options = {k: v for k, v in row['algorithm'].items() if k not in ('name', 'method', 'mode')}
             out[case]['runs'].append({'source': f'results/{name}:{line_no}', 'method': method,
-                                      'device': row['device'], 'case': row['case'],
+                                      'device': row['device'], 'case': row['case'], 'options': options,
                                       'diagnostics': diag, 'metrics': row['metrics']['worst']})
     (here / 'expected.json').write_text(json.dumps(out, indent=1, sort_keys=True) + '\n')
 
