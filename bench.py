@@ -1,5 +1,6 @@
 """Benchmark: batched uneven one-sided Jacobi SVD on one MI355X (the headline line), plus the
-single-matrix, QDWH, TSQR and learned-l0 workloads under "configs".
+single-matrix, QDWH, TSQR, learned-l0 and consumer-solver (block_jacobi, gram_jacobi, range
+finders) workloads under "configs".
 
 Workload: 4096 float32 matrices, m ~ U{16..96}, n ~ U{8..m}, entries i.i.d. N(0,1) (seed
 20261011), packed row-major in HBM.  One step = one solve of the whole batch: a single
@@ -183,6 +184,48 @@ def config_tsqr(pkg, dev):
             'results': res}
 
 
+def config_consumers(pkg, dev):
+    """Solvers built on the hot path's kernels: block_jacobi (fused block-pair rounds) and gram_jacobi
+    on the 512 x 256 single-matrix workload; randomized / subspace / block_krylov (rank 64,
+    oversample 8, two power steps) on the 8192 x 2048 rank-64-plus-noise TSQR workload."""
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(SEED)
+    s1 = torch.from_numpy(np.geomspace(1.0, 1e-6, 256)).to(dev)
+    a1 = _product(_haar64(512, 256, gen, dev), s1, _haar64(256, 256, gen, dev)).float()
+    eps32 = float(np.finfo(np.float32).eps)
+    res = {}
+    for name, opts in (('block_jacobi', {}), ('block_jacobi_16', {'block_size': 16}), ('gram_jacobi', {})):
+        method = name.split('_16')[0]
+        t, (u, sv, vh, info) = _timed(lambda: pkg.svd(a1, method, return_info=True, **opts), 3)
+        cut = eps32 * 512 * float(sv[0]) if method == 'block_jacobi' else math.sqrt(eps32 * 256) * float(sv[0])
+        kept = s1 > cut
+        res[name] = {'seconds': t, 'options': opts,
+                     **{k: info[k] for k in ('block_sweeps', 'jacobi_sweeps', 'eigen_sweeps', 'gram_discarded_values')
+                        if k in info},
+                     'kept_sigma': int((sv > 0).sum()),
+                     'kept_spectrum_error_over_sigma1': float((sv.double() - s1)[kept].abs().max()),
+                     'orthogonality_v': _orth(vh.mT)}
+    gen.manual_seed(SEED)
+    s4 = torch.from_numpy(np.geomspace(1.0, 1e-3, 64)).to(dev)
+    a4 = _product(_haar64(8192, 64, gen, dev), s4, _haar64(2048, 64, gen, dev))
+    a4 = (a4 + 1e-4 * torch.randn(8192, 2048, generator=gen, device=dev, dtype=torch.float64)).float()
+    ref = None
+    for method in ('randomized', 'subspace', 'block_krylov'):
+        t, (u, sv, vh, info) = _timed(lambda: pkg.svd(a4, method, rank=64, oversample=8, iterations=2, seed=31,
+                                                      return_info=True), 3)
+        if ref is None:     # FP64 Jacobi of the same matrix: the top-64 reference values
+            ref = pkg.svd(a4.double(), 'tsqr', tile_rows=256, values_only=True)[1][:64]
+        res[method] = {'seconds': t, **{k: info[k] for k in ('jacobi_sweeps', 'sketch_columns', 'range_columns')},
+                       'top64_sigma_error_over_sigma1': float((sv.double() - ref).abs().max() / ref[0]),
+                       'orthogonality_u': _orth(u), 'orthogonality_v': _orth(vh.mT)}
+    return {'workload': 'block_jacobi (block_size 4 and 16) and gram_jacobi on 512x256 float32, Haar U/V, sigma '
+                        'geomspace(1, 1e-6) (errors over the sigma each method keeps: block_jacobi cuts at '
+                        'eps*max(m,n)*sigma_1, gram_jacobi at sqrt(eps*n)*sigma_1); randomized / subspace / '
+                        'block_krylov, rank 64 + 8, two power steps, seed 31, on 8192x2048 float32 rank-64 '
+                        '(sigma 1 .. 1e-3) plus N(0, 1e-8) noise, top-64 sigma against FP64 TSQR + Jacobi',
+            'unit': 's/solve', 'results': res}
+
+
 def config_l0(pkg, dev, count=32):
     """The learned QDWH lower bound on `count` matrices of its test split (1024 x 1024)."""
     T = importlib.import_module('pavel-s_svd_amd.train_l0')
@@ -228,8 +271,8 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-check', action='store_true')
     ap.add_argument('--no-tune', action='store_true', help='use the default launch-shape policy')
-    ap.add_argument('--only', default='all', choices=['all', 'c2', 'single', 'qdwh', 'tsqr', 'l0'],
-                    help="'c2': the batched-Jacobi line alone; single / qdwh / tsqr / l0: that leg alone "
+    ap.add_argument('--only', default='all', choices=['all', 'c2', 'single', 'qdwh', 'tsqr', 'l0', 'consumers'],
+                    help="'c2': the batched-Jacobi line alone; single / qdwh / tsqr / l0 / consumers: that leg alone "
                          "(printed as {\"configs\": ...}, for profiling)")
     ap.add_argument('--copy-calibration', action='store_true',
                     help='only run a plain 256 MiB device-to-device copy 5 times (the reference kernel for '
@@ -249,7 +292,8 @@ def main():
         return
     pkg = importlib.import_module('pavel-s_svd_amd')
     legs = {'single': ('single_512x256', config_single), 'qdwh': ('qdwh_4096', config_qdwh),
-            'tsqr': ('tsqr_8192x2048', config_tsqr), 'l0': ('learned_l0_1024', config_l0)}
+            'tsqr': ('tsqr_8192x2048', config_tsqr), 'l0': ('learned_l0_1024', config_l0),
+            'consumers': ('consumers', config_consumers)}
     if args.only in legs:
         name, fn = legs[args.only]
         print(json.dumps({'configs': {name: fn(pkg, torch.device('cuda:0'))}}), flush=True)

@@ -32,9 +32,11 @@ namespace psvd {
 
 constexpr int kBjrThreads = 256;
 constexpr int kBjrMaxK = 64;      // columns of one pair: 2 * block_size
-constexpr int kBjrChunk = 1024;   // doubles of the row-chunk staging buffer
+constexpr int kBjrChunk = 4096;   // doubles of the row-chunk staging buffer
+constexpr int kBjrMaxEpt = (kBjrMaxK * (kBjrMaxK + 1) / 2 + kBjrThreads - 1) / kBjrThreads;   // Gram entries per thread
 constexpr int kCorrTile = 16;
 constexpr int kCorrRows = 64;
+constexpr int kNormCols = 16;     // columns per norms workgroup (16 row groups of 16 columns)
 
 __device__ __forceinline__ int bjr_group_cols(int g, int width, int n, int groups) {
   if (g >= groups) return 0;
@@ -52,10 +54,161 @@ __device__ __forceinline__ void bjr_unrank(int e, int k, int &p, int &q) {
   q = p + e;
 }
 
+// The rotation of primitives.py:340-347 (tau = (aqq - app) / (2 |apq|), t = sign(tau) / (|tau| +
+// hypot(tau, 1)), c = 1 / sqrt(1 + t^2), s = t c sign(apq)) with c and s in FP64 so that c^2 + s^2 =
+// 1 to FP64 rounding.  For FP32 data the angle itself is evaluated in FP32 (tau from an FP64
+// reciprocal with one Newton step): the data's own tolerance (10 eps32) needs no more, the
+// remaining off-diagonal is swept on; FP64 data takes the full FP64 formula.
+template <typename T>
+__device__ __forceinline__ void bjr_rotation(double app, double aqq, double apq, double &c, double &s);
+
+template <>
+__device__ __forceinline__ void bjr_rotation<double>(double app, double aqq, double apq, double &c, double &s) {
+  jacobi_rotation(app, aqq, apq, c, s);
+}
+
+template <>
+__device__ __forceinline__ void bjr_rotation<float>(double app, double aqq, double apq, double &c, double &s) {
+  const double a2 = 2.0 * fabs(apq);
+  double r = __builtin_amdgcn_rcp(a2);
+  r = r * fma(-a2, r, 2.0);
+  const double tau = (aqq - app) * r;
+  const double at = fabs(tau);
+  double t;
+  if (tau == 0.0) {
+    t = 1.0;
+  } else if (at < 1e18) {
+    const float af = (float)at;
+    t = (double)__builtin_amdgcn_rcpf(af + __builtin_amdgcn_sqrtf(fmaf(af, af, 1.0f)));
+  } else {
+    t = 0.5 / at;
+  }
+  t = copysign(t, tau);
+  const double x = fma(t, t, 1.0);
+  double y = __builtin_amdgcn_rsq(x);
+  y = y * fma(-0.5 * x * y, y, 1.5);
+  c = y;
+  s = apq < 0.0 ? -t * y : t * y;
+}
+
 static size_t bjr_lds_bytes(int kmax) {
   const size_t doubles = 2 * (size_t)kmax * (kmax + 1) + kBjrChunk + kBjrThreads + kBjrMaxK + 8;
   const size_t ints = 2 * (kBjrMaxK / 2) + kBjrMaxK + 2;
   return doubles * 8 + ints * 4;
+}
+
+// Stage rows [r0, r0 + nr) of the pair's k columns of M into X (row-major, k per row), as FP64.
+template <typename T, typename COL>
+__device__ __forceinline__ void bjr_stage(double *X, const T *M, int64_t ldm, int r0, int nr, int k, COL col) {
+  for (int e = threadIdx.x; e < nr * k; e += kBjrThreads) {
+    const int r = e / k, c = e - r * k;
+    X[e] = (double)M[(int64_t)(r0 + r) * ldm + col(c)];
+  }
+}
+
+// Ordering point for LDS traffic among the lanes of one wavefront (its instructions issue in
+// order; the fences keep the compiler from moving LDS accesses across).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One-sided Jacobi of a pair's k columns carried out on their Gram matrix G (k x k, row stride
+// ld, FP64, LDS), rotations accumulated into Q.  Round rr of the tournament over the k columns
+// (padded to even; the padding column is zero and never active) rotates pair (p, q) iff
+// |g_pq| > tol sqrt(g_pp) sqrt(g_qq) and min(g_pp, g_qq) > eps^2 max_c g_cc -- the reference's
+// |a_pq| / (sqrt a_pp sqrt a_qq) > tol test without the division (primitives.py:331-338) -- and
+// applies G <- J^T G J by disjoint 2 x 2 blocks (rows of pair i x columns of pair j) and Q <- Q J
+// in one pass.  A sweep with no active pair ends the solve (the reference's off <= tol, :352).
+// WAVE: run by wave 0 alone (np2^2 <= 64 blocks), else by the whole workgroup.  Leaves the sweep
+// count in flag[1] and convergence in flag[0].
+template <typename T, bool WAVE>
+__device__ __forceinline__ void bjr_inner(double *G, double *Q, int k, int ld, double tol, int max_sweeps,
+                                          double *rc, double *rs, int *rp, int *rq, double *act, int *flag) {
+  constexpr int NTH = WAVE ? kWave : kBjrThreads;
+  const int tid = threadIdx.x, wave = tid / kWave, lane = tid & (kWave - 1);
+  const int kp = k + (k & 1), np2 = kp / 2;
+  const double eps2 = Eps<T>::value * Eps<T>::value;
+  const double tol2 = tol * tol;
+  int sweeps = 0;
+  bool converged = false;
+  for (int sweep = 1; sweep <= max_sweeps; ++sweep) {
+    bool any = false;   // per lane of wave 0
+    for (int rr = 0; rr < kp - 1; ++rr) {
+      if (wave == 0) {
+        const double dmax = wave_max_fast(lane < k ? G[lane * ld + lane] : 0.0);
+        if (lane < np2) {
+          int p, q;
+          rr_pair(rr, lane, kp, p, q);
+          double c = 1.0, s = 0.0;
+          bool active = false;
+          if (q < k) {
+            const double app = G[p * ld + p], aqq = G[q * ld + q], apq = G[p * ld + q];
+            // squares for FP32 data (Gram entries far from FP64 overflow), square roots for FP64
+            const bool big = sizeof(T) == 4 ? apq * apq > tol2 * (app * aqq)
+                                            : fabs(apq) > tol * (sqrt(app) * sqrt(aqq));
+            active = big && fmin(app, aqq) > eps2 * dmax;
+            if (active) bjr_rotation<T>(app, aqq, apq, c, s);
+          }
+          any = any || active;
+          rp[lane] = p;
+          rq[lane] = q;
+          rc[lane] = c;
+          rs[lane] = s;
+          act[lane] = active ? 1.0 : 0.0;
+        }
+      }
+      if constexpr (WAVE) wave_sync(); else __syncthreads();
+      for (int it = tid; it < np2 * np2; it += NTH) {
+        const int i = it / np2, j = it - i * np2;
+        const bool ai = act[i] != 0.0, aj = act[j] != 0.0;
+        if (!ai && !aj) continue;
+        const int p1 = rp[i], q1 = rq[i], p2 = rp[j], q2 = rq[j];
+        const bool vq1 = q1 < k, vq2 = q2 < k;   // the padding index has no row or column
+        double b11 = G[p1 * ld + p2];
+        double b12 = vq2 ? G[p1 * ld + q2] : 0.0;
+        double b21 = vq1 ? G[q1 * ld + p2] : 0.0;
+        double b22 = vq1 && vq2 ? G[q1 * ld + q2] : 0.0;
+        if (aj) {
+          const double c = rc[j], s = rs[j];
+          const double t11 = c * b11 - s * b12, t12 = s * b11 + c * b12;
+          const double t21 = c * b21 - s * b22, t22 = s * b21 + c * b22;
+          b11 = t11; b12 = t12; b21 = t21; b22 = t22;
+        }
+        if (ai) {
+          const double c = rc[i], s = rs[i];
+          const double t11 = c * b11 - s * b21, t21 = s * b11 + c * b21;
+          const double t12 = c * b12 - s * b22, t22 = s * b12 + c * b22;
+          b11 = t11; b12 = t12; b21 = t21; b22 = t22;
+        }
+        G[p1 * ld + p2] = b11;
+        if (vq2) G[p1 * ld + q2] = b12;
+        if (vq1) G[q1 * ld + p2] = b21;
+        if (vq1 && vq2) G[q1 * ld + q2] = b22;
+      }
+      for (int it = tid; it < k * np2; it += NTH) {
+        const int r = it / np2, j = it - r * np2;
+        if (act[j] == 0.0) continue;
+        const double c = rc[j], s = rs[j];
+        const int p = rp[j], q = rq[j];
+        const double qp = Q[r * ld + p], qq = Q[r * ld + q];
+        Q[r * ld + p] = c * qp - s * qq;
+        Q[r * ld + q] = s * qp + c * qq;
+      }
+      if constexpr (WAVE) wave_sync(); else __syncthreads();
+    }
+    sweeps = sweep;
+    if (wave == 0) {
+      const bool none = !__any(any);
+      if (lane == 0) flag[0] = none;
+    }
+    if constexpr (WAVE) wave_sync(); else __syncthreads();
+    converged = flag[0] != 0;
+    if constexpr (WAVE) wave_sync(); else __syncthreads();
+    if (converged) break;
+  }
+  if (tid == 0) flag[1] = sweeps;
 }
 
 template <typename T>
@@ -74,7 +227,7 @@ __global__ __launch_bounds__(kBjrThreads) void bjr_round_kernel(T *W, int64_t ld
   const int k = ki + kj;
   if (k < 2) return;   // a lone single column: nothing to rotate (solvers.py:50-51)
   const int ci0 = gi * width, cj0 = gj * width;
-  auto col = [&](int c) { return c < ki ? ci0 + c : cj0 + (c - ki); };
+  auto col = [=](int c) { return c < ki ? ci0 + c : cj0 + (c - ki); };
   const int ld = k + 1;
   double *G = lds;
   double *Q = G + k * ld;
@@ -89,20 +242,41 @@ __global__ __launch_bounds__(kBjrThreads) void bjr_round_kernel(T *W, int64_t ld
   const int tid = threadIdx.x, wave = tid / kWave, lane = tid & (kWave - 1);
   T *Wb = W + b * sw;
   T *Vb = V + b * sv;
+  const int rows_per_chunk = kBjrChunk / k;
 
-  // 1. Gram matrix of the pair's columns
+  // 1. Gram matrix of the pair's columns from LDS-staged row chunks.  Fewer entries than threads:
+  // R replicas of each entry sum interleaved rows, combined in replica order; otherwise each
+  // thread owns entries tid, tid + NT, ...
   const int E = k * (k + 1) / 2;
-  if (E < NT) {
-    const int R = NT / E;
-    double acc = 0.0;
-    if (tid < R * E) {
-      const int e = tid % E, rep = tid / E;
-      int p, q;
-      bjr_unrank(e, k, p, q);
-      const T *cp = Wb + col(p), *cq = Wb + col(q);
-      for (int r = rep; r < m; r += R) acc = fma((double)cp[(int64_t)r * ldw], (double)cq[(int64_t)r * ldw], acc);
+  const int R = E < NT ? NT / E : 1;
+  const int e0 = E < NT ? tid % E : tid, rep = E < NT ? tid / E : 0;
+  const bool own = E < NT ? tid < R * E : true;
+  int ep[kBjrMaxEpt], eq[kBjrMaxEpt];
+  double acc[kBjrMaxEpt];
+#pragma unroll
+  for (int j = 0; j < kBjrMaxEpt; ++j) {
+    acc[j] = 0.0;
+    ep[j] = eq[j] = 0;
+    const int e = e0 + j * NT;
+    if (own && e < E && (j == 0 || E >= NT)) bjr_unrank(e, k, ep[j], eq[j]);
+  }
+  for (int r0 = 0; r0 < m; r0 += rows_per_chunk) {
+    const int nr = m - r0 < rows_per_chunk ? m - r0 : rows_per_chunk;
+    bjr_stage(X, Wb, ldw, r0, nr, k, col);
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kBjrMaxEpt; ++j) {
+      const int e = e0 + j * NT;
+      if (own && e < E && (j == 0 || E >= NT)) {
+        double a = acc[j];
+        for (int r = rep; r < nr; r += R) a = fma(X[r * k + ep[j]], X[r * k + eq[j]], a);
+        acc[j] = a;
+      }
     }
-    part[tid] = acc;
+    __syncthreads();
+  }
+  if (E < NT) {
+    part[tid] = own ? acc[0] : 0.0;
     __syncthreads();
     if (tid < E) {
       double s = 0.0;
@@ -113,14 +287,12 @@ __global__ __launch_bounds__(kBjrThreads) void bjr_round_kernel(T *W, int64_t ld
       G[q * ld + p] = s;
     }
   } else {
-    for (int e = tid; e < E; e += NT) {
-      int p, q;
-      bjr_unrank(e, k, p, q);
-      const T *cp = Wb + col(p), *cq = Wb + col(q);
-      double acc = 0.0;
-      for (int r = 0; r < m; ++r) acc = fma((double)cp[(int64_t)r * ldw], (double)cq[(int64_t)r * ldw], acc);
-      G[p * ld + q] = acc;
-      G[q * ld + p] = acc;
+#pragma unroll
+    for (int j = 0; j < kBjrMaxEpt; ++j) {
+      if (tid + j * NT < E) {
+        G[ep[j] * ld + eq[j]] = acc[j];
+        G[eq[j] * ld + ep[j]] = acc[j];
+      }
     }
   }
   for (int e = tid; e < k * k; e += NT) {
@@ -129,72 +301,19 @@ __global__ __launch_bounds__(kBjrThreads) void bjr_round_kernel(T *W, int64_t ld
   }
   __syncthreads();
 
-  // 2. one-sided Jacobi of the pair on its Gram matrix
-  const int kp = k + (k & 1), np2 = kp / 2;
-  const double eps2 = Eps<T>::value * Eps<T>::value;
-  int sweeps = 0;
-  bool converged = false;
-  for (int sweep = 1; sweep <= max_sweeps; ++sweep) {
-    double off = 0.0;   // kept by thread 0
-    for (int rr = 0; rr < kp - 1; ++rr) {
-      if (wave == 0) {
-        const double dmax = wave_max(lane < k ? G[lane * ld + lane] : 0.0);
-        if (tid < np2) {
-          int p, q;
-          rr_pair(rr, tid, kp, p, q);
-          double c = 1.0, s = 0.0, ratio = 0.0;
-          if (q < k) {   // q == k is the zero padding column: never active
-            const double app = G[p * ld + p], aqq = G[q * ld + q], apq = G[p * ld + q];
-            const double den = sqrt(app) * sqrt(aqq);
-            ratio = fabs(apq) / fmax(den, DBL_MIN);
-            if (ratio > tol && fmin(app, aqq) > eps2 * dmax)
-              jacobi_rotation(app, aqq, apq, c, s);
-            else
-              ratio = 0.0;
-          }
-          rp[tid] = p;
-          rq[tid] = q;
-          rc[tid] = c;
-          rs[tid] = ratio > 0.0 ? s : 0.0;
-          part[tid] = ratio;
-        }
-      }
-      __syncthreads();
-      if (tid == 0)
-        for (int i = 0; i < np2; ++i) off = fmax(off, part[i]);
-      // G <- G J and Q <- Q J: columns p, q of every active pair
-      for (int it = tid; it < np2 * k; it += NT) {
-        const int i = it / k, r = it - i * k;
-        if (part[i] == 0.0) continue;
-        const double c = rc[i], s = rs[i];
-        const int p = rp[i], q = rq[i];
-        const double gp = G[r * ld + p], gq = G[r * ld + q];
-        G[r * ld + p] = c * gp - s * gq;
-        G[r * ld + q] = s * gp + c * gq;
-        const double qp = Q[r * ld + p], qq = Q[r * ld + q];
-        Q[r * ld + p] = c * qp - s * qq;
-        Q[r * ld + q] = s * qp + c * qq;
-      }
-      __syncthreads();
-      // G <- J^T G: rows p, q of every active pair
-      for (int it = tid; it < np2 * k; it += NT) {
-        const int i = it / k, r = it - i * k;
-        if (part[i] == 0.0) continue;
-        const double c = rc[i], s = rs[i];
-        const int p = rp[i], q = rq[i];
-        const double gp = G[p * ld + r], gq = G[q * ld + r];
-        G[p * ld + r] = c * gp - s * gq;
-        G[q * ld + r] = s * gp + c * gq;
-      }
-      __syncthreads();
-    }
-    sweeps = sweep;
-    if (tid == 0) flag[0] = off <= tol;
+  // 2. one-sided Jacobi of the pair on its Gram matrix: pairs of up to 16 columns in wave 0 alone
+  // (no workgroup barriers), wider ones by the whole workgroup
+  int sweeps;
+  bool converged;
+  if (k <= 16) {
+    if (wave == 0) bjr_inner<T, true>(G, Q, k, ld, tol, max_sweeps, rc, rs, rp, rq, part, flag);
     __syncthreads();
-    converged = flag[0] != 0;
+  } else {
+    bjr_inner<T, false>(G, Q, k, ld, tol, max_sweeps, rc, rs, rp, rq, part, flag);
     __syncthreads();
-    if (converged) break;
   }
+  sweeps = flag[1];
+  converged = flag[0] != 0;
 
   // 3. columns of Q by descending sigma^2 = g_cc (ties by index), written into G's storage
   if (tid < k) {
@@ -214,24 +333,20 @@ __global__ __launch_bounds__(kBjrThreads) void bjr_round_kernel(T *W, int64_t ld
   __syncthreads();
 
   // 4. W_pair <- W_pair Q and V_pair <- V_pair Q, row chunks staged in LDS
-  const int rows_per_chunk = kBjrChunk / k;
   for (int pass = 0; pass < 2; ++pass) {
     T *M = pass == 0 ? Wb : Vb;
     const int64_t ldm = pass == 0 ? ldw : ldv;
     const int rows = pass == 0 ? m : n;
     for (int r0 = 0; r0 < rows; r0 += rows_per_chunk) {
       const int nr = rows - r0 < rows_per_chunk ? rows - r0 : rows_per_chunk;
-      for (int e = tid; e < nr * k; e += NT) {
-        const int r = e / k, c = e - r * k;
-        X[e] = (double)M[(int64_t)(r0 + r) * ldm + col(c)];
-      }
+      bjr_stage(X, M, ldm, r0, nr, k, col);
       __syncthreads();
       for (int e = tid; e < nr * k; e += NT) {
         const int r = e / k, j = e - r * k;
         const double *xr = X + r * k;
-        double acc = 0.0;
-        for (int c = 0; c < k; ++c) acc = fma(xr[c], G[c * ld + j], acc);
-        M[(int64_t)(r0 + r) * ldm + col(j)] = (T)acc;
+        double a = 0.0;
+        for (int c = 0; c < k; ++c) a = fma(xr[c], G[c * ld + j], a);
+        M[(int64_t)(r0 + r) * ldm + col(j)] = (T)a;
       }
       __syncthreads();
     }
@@ -252,24 +367,44 @@ __global__ void bjr_reset_kernel(int32_t *info, unsigned long long *corr, unsign
   }
 }
 
-// column norms^2 (FP64), the norms and the per-matrix maximum; grid (column chunks, batch)
+// column norms^2 (FP64), the norms and the per-matrix maximum; grid (column tiles of 16, batch).
+// Thread (g, c) sums rows g, g + 16, ... of column c (16 contiguous columns per row segment); the 16
+// row-group partials are added in group order.
 template <typename T>
 __global__ __launch_bounds__(kBjrThreads) void bjr_norms_kernel(const T *W, int64_t ldw, int64_t sw, int m, int n,
                                                                 double *d, double *sigma, unsigned long long *dmax) {
+  constexpr int NG = kBjrThreads / kNormCols;
+  __shared__ double part[NG][kNormCols];
   __shared__ double red[kBjrThreads / kWave];
   const int64_t b = blockIdx.y;
-  const int j = blockIdx.x * kBjrThreads + threadIdx.x;
+  const int g = threadIdx.x / kNormCols, c = threadIdx.x % kNormCols;
+  const int j = blockIdx.x * kNormCols + c;
   double acc = 0.0;
   if (j < n) {
-    const T *c = W + b * sw + j;
-    for (int r = 0; r < m; ++r) {
-      const double x = (double)c[(int64_t)r * ldw];
+    const T *cp = W + b * sw + j;
+    int r = g;
+    for (; r + 3 * NG < m; r += 4 * NG) {
+      const double x0 = (double)cp[(int64_t)r * ldw], x1 = (double)cp[(int64_t)(r + NG) * ldw];
+      const double x2 = (double)cp[(int64_t)(r + 2 * NG) * ldw], x3 = (double)cp[(int64_t)(r + 3 * NG) * ldw];
+      acc = fma(x0, x0, acc);
+      acc = fma(x1, x1, acc);
+      acc = fma(x2, x2, acc);
+      acc = fma(x3, x3, acc);
+    }
+    for (; r < m; r += NG) {
+      const double x = (double)cp[(int64_t)r * ldw];
       acc = fma(x, x, acc);
     }
-    d[b * n + j] = acc;
-    sigma[b * n + j] = sqrt(acc);
   }
-  const double mx = block_max<kBjrThreads>(acc, red);
+  part[g][c] = acc;
+  __syncthreads();
+  double tot = 0.0;
+  if (g == 0 && j < n) {
+    for (int x = 0; x < NG; ++x) tot += part[x][c];
+    d[b * n + j] = tot;
+    sigma[b * n + j] = sqrt(tot);
+  }
+  const double mx = block_max<kBjrThreads>(tot, red);
   if (threadIdx.x == 0) atomicMax(&dmax[b], (unsigned long long)__double_as_longlong(mx));
 }
 
@@ -328,7 +463,7 @@ static int run_block_jacobi_sweep(int64_t batch, int m, int n, int width, T *w, 
   for (int r = 0; r < gp - 1; ++r)
     hipLaunchKernelGGL(bjr_round_kernel<T>, dim3((unsigned)blocks), dim3(kBjrThreads), lds, st, w, ldw, sw, v, ldv, sv,
                        m, n, width, groups, gp, r, tol, max_sweeps, info);
-  hipLaunchKernelGGL(bjr_norms_kernel<T>, dim3((unsigned)((n + kBjrThreads - 1) / kBjrThreads), (unsigned)batch),
+  hipLaunchKernelGGL(bjr_norms_kernel<T>, dim3((unsigned)((n + kNormCols - 1) / kNormCols), (unsigned)batch),
                      dim3(kBjrThreads), 0, st, w, ldw, sw, m, n, d, sigma, dmax);
   const int tiles = (n + kCorrTile - 1) / kCorrTile;
   const double live = Eps<T>::value * (m > n ? m : n);

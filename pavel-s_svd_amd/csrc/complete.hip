@@ -4,10 +4,14 @@
 // largest remaining leverage 1 - |U(i, :)|^2 (first such row), projected twice against all
 // current columns (classical Gram-Schmidt, twice) and normalised; a residual below 10 eps(T)
 // sets the failure status (the reference raises).  Columns still to be completed are zero while
-// earlier ones are used, exactly as in the reference.  FP64 accumulation throughout.
+// earlier ones are used, exactly as in the reference, so the products run over the r columns
+// already filled.  FP64 accumulation throughout.
 //
-// One workgroup per matrix: leverage and the row products run one thread per row (contiguous row
-// reads), U^T v runs one thread per column (coalesced across the workgroup).
+// One workgroup per matrix.  The leverage vector is formed once and, after column r is written,
+// lowered by its squared entries (the reference recomputes 1 - |U(i, :)|^2 from scratch; the two
+// agree to rounding).  U(:, :r) u-products run one wave per row with the lanes across the row
+// (coalesced), U^T v runs row slices x columns with the slices added in a fixed order, so every
+// sum has a fixed order and the result is deterministic.
 #include <float.h>
 
 #include "common.hpp"
@@ -19,34 +23,45 @@ constexpr int kCompleteThreads = 1024;
 template <typename T>
 __global__ __launch_bounds__(kCompleteThreads) void complete_kernel(T *U, int64_t ldu, int64_t su, int m, int n,
                                                                     const int32_t *nk, int32_t *status, double *vbuf,
-                                                                    double *ybuf) {
+                                                                    double *ybuf, double *levbuf) {
   constexpr int NT = kCompleteThreads, NW = NT / kWave;
   __shared__ double red[NW];
   __shared__ int redi[NW];
+  __shared__ double ypart[NT];
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
   const int64_t b = blockIdx.x;
   T *Ub = U + b * su;
   double *v = vbuf + b * (int64_t)m;
   double *y = ybuf + b * (int64_t)n;
+  double *lev = levbuf + b * (int64_t)m;
   const int k0 = nk[b] < 0 ? 0 : (nk[b] > n ? n : nk[b]);
   const double eps = Eps<T>::value;
   int st = 0;
+  if (k0 == n) {
+    if (tid == 0) status[b] = 0;
+    return;
+  }
   // missing columns start at zero
   for (int64_t e = tid; e < (int64_t)m * (n - k0); e += NT) {
     const int i = (int)(e / (n - k0)), c = k0 + (int)(e - (int64_t)i * (n - k0));
     Ub[(int64_t)i * ldu + c] = T(0);
+  }
+  // leverage of the live columns
+  for (int i = wave; i < m; i += NW) {
+    const T *row = Ub + (int64_t)i * ldu;
+    double a = 0.0;
+    for (int c = lane; c < k0; c += kWave) a = fma((double)row[c], (double)row[c], a);
+    a = wave_sum(a);
+    if (lane == 0) lev[i] = 1.0 - a;
   }
   __syncthreads();
   for (int r = k0; r < n; ++r) {
     double best = -DBL_MAX;
     int bi = 0x7fffffff;
     for (int i = tid; i < m; i += NT) {
-      const T *row = Ub + (int64_t)i * ldu;
-      double acc = 0.0;
-      for (int c = 0; c < n; ++c) acc = fma((double)row[c], (double)row[c], acc);
-      const double lev = 1.0 - acc;
-      if (lev > best) {
-        best = lev;
+      const double l = lev[i];
+      if (l > best) {
+        best = l;
         bi = i;
       }
     }
@@ -73,30 +88,45 @@ __global__ __launch_bounds__(kCompleteThreads) void complete_kernel(T *U, int64_
     const int ix = bi;
     const T *rix = Ub + (int64_t)ix * ldu;
     // v = e_ix - U (U^T e_ix) = e_ix - U U(ix, :)^T
-    for (int i = tid; i < m; i += NT) {
+    for (int i = wave; i < m; i += NW) {
       const T *row = Ub + (int64_t)i * ldu;
-      double acc = (i == ix) ? 1.0 : 0.0;
-      for (int c = 0; c < n; ++c) acc -= (double)row[c] * (double)rix[c];
-      v[i] = acc;
+      double a = 0.0;
+      for (int c = lane; c < r; c += kWave) a = fma((double)row[c], (double)rix[c], a);
+      a = wave_sum(a);
+      if (lane == 0) v[i] = (i == ix ? 1.0 : 0.0) - a;
     }
     __syncthreads();
-    // y = U^T v (one thread per column), then v -= U y and |v|
-    for (int c = tid; c < n; c += NT) {
-      double acc = 0.0;
-      for (int i = 0; i < m; ++i) acc = fma((double)Ub[(int64_t)i * ldu + c], v[i], acc);
-      y[c] = acc;
+    // y = U(:, :r)^T v: S row slices per column, slices added in order
+    if (r > 0) {
+      const int S = r >= NT ? 1 : NT / r;
+      for (int c0 = 0; c0 < r; c0 += NT) {
+        const int c = c0 + tid % (r < NT ? r : NT), sl = r < NT ? tid / r : 0;
+        double a = 0.0;
+        if (sl < S && c < r)
+          for (int i = sl; i < m; i += S) a = fma((double)Ub[(int64_t)i * ldu + c], v[i], a);
+        ypart[tid] = a;
+        __syncthreads();
+        if (tid < (r - c0 < NT ? r - c0 : NT)) {
+          double t = 0.0;
+          for (int x = 0; x < S; ++x) t += ypart[x * (r < NT ? r : NT) + tid];
+          y[c0 + tid] = t;
+        }
+        __syncthreads();
+      }
     }
-    __syncthreads();
+    // v -= U(:, :r) y and |v|^2
     double l2 = 0.0;
-    for (int i = tid; i < m; i += NT) {
+    for (int i = wave; i < m; i += NW) {
       const T *row = Ub + (int64_t)i * ldu;
-      double acc = v[i];
-      for (int c = 0; c < n; ++c) acc -= (double)row[c] * y[c];
-      v[i] = acc;
-      l2 = fma(acc, acc, l2);
+      double a = 0.0;
+      for (int c = lane; c < r; c += kWave) a = fma((double)row[c], y[c], a);
+      a = wave_sum(a);
+      const double vi = v[i] - a;
+      if (lane == 0) {
+        v[i] = vi;
+        l2 = fma(vi, vi, l2);
+      }
     }
-    l2 = wave_sum(l2);
-    __syncthreads();
     if (lane == 0) red[wave] = l2;
     __syncthreads();
     double len2 = 0.0;
@@ -104,7 +134,11 @@ __global__ __launch_bounds__(kCompleteThreads) void complete_kernel(T *U, int64_
     const double len = sqrt(len2);
     if (len < 10.0 * eps) st = PSVD_INFO_COMPLETION;
     const double inv = 1.0 / fmax(len, DBL_MIN);
-    for (int i = tid; i < m; i += NT) Ub[(int64_t)i * ldu + r] = (T)(v[i] * inv);
+    for (int i = tid; i < m; i += NT) {
+      const T x = (T)(v[i] * inv);
+      Ub[(int64_t)i * ldu + r] = x;
+      lev[i] -= (double)x * (double)x;
+    }
     __syncthreads();
   }
   if (tid == 0) status[b] = st;
@@ -116,7 +150,7 @@ extern "C" {
 
 size_t psvd_complete_workspace_bytes(int64_t batch, int m, int n) {
   if (batch < 0 || m < 0 || n < 0) return 0;
-  return (size_t)8 * batch * ((size_t)m + n) + 256;
+  return (size_t)8 * batch * (2 * (size_t)m + n) + 256;
 }
 
 int psvd_complete_columns(int dtype, int64_t batch, int m, int n, void *u, int64_t ldu, int64_t stride_u,
@@ -128,13 +162,14 @@ int psvd_complete_columns(int dtype, int64_t batch, int m, int n, void *u, int64
   if (workspace_bytes < psvd_complete_workspace_bytes(batch, m, n)) return PSVD_EINVAL;
   double *vb = static_cast<double *>(workspace);
   double *yb = vb + batch * (int64_t)m;
+  double *lb = yb + batch * (int64_t)n;
   hipStream_t st = static_cast<hipStream_t>(stream);
   if (dtype == PSVD_F32)
     hipLaunchKernelGGL(psvd::complete_kernel<float>, dim3((unsigned)batch), dim3(psvd::kCompleteThreads), 0, st,
-                       static_cast<float *>(u), ldu, stride_u, m, n, nk, status, vb, yb);
+                       static_cast<float *>(u), ldu, stride_u, m, n, nk, status, vb, yb, lb);
   else if (dtype == PSVD_F64)
     hipLaunchKernelGGL(psvd::complete_kernel<double>, dim3((unsigned)batch), dim3(psvd::kCompleteThreads), 0, st,
-                       static_cast<double *>(u), ldu, stride_u, m, n, nk, status, vb, yb);
+                       static_cast<double *>(u), ldu, stride_u, m, n, nk, status, vb, yb, lb);
   else
     return PSVD_EINVAL;
   return hipGetLastError() == hipSuccess ? PSVD_OK : PSVD_ELAUNCH;
