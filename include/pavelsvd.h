@@ -218,7 +218,7 @@ int psvd_jacobi_blocked(int dtype, int m, int n, const void *a, void *u, void *s
  * columns are orthogonalised by the reference's one-sided Jacobi (tolerance tol, 0 = 10 eps; at
  * most max_sweeps sweeps; primitives.py:316-367) carried out on their FP64 Gram matrix, and
  * W_pair, V_pair are multiplied by the accumulated rotation with its columns ordered by
- * descending norm (solvers.py:48-55).  Then, for the sweep's convergence test (solvers.py:56-62):
+ * descending norm (solvers.py:43-50).  Then, for the sweep's convergence test (solvers.py:51-60):
  * corr[b] = max over live columns i != j of |w_i^T w_j| / (|w_i| |w_j|) (live: |w_i|^2 >
  * (eps max(m, n))^2 max_c |w_c|^2), sigma[b * n + c] = |w_c| (FP64 sums throughout).
  * info[b] = {largest pair sweep count, 1 if a pair did not converge}.  The caller stops when

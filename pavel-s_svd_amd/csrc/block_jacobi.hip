@@ -4,7 +4,7 @@
 // may be short; an odd group count gets an empty partner group), paired by the round-robin
 // tournament of primitives.py:309-313.  The reference solves each pair with a full one-sided
 // Jacobi SVD of its k <= 2 width columns and replaces them by W_pair V_pair, V_pair sorted by
-// descending sigma (solvers.py:48-55); V gets the same update.  Here one workgroup takes one
+// descending sigma (solvers.py:43-50); V gets the same update.  Here one workgroup takes one
 // pair of one matrix per round launch, and everything after the load stays on chip:
 //   1. G = W_pair^T W_pair in FP64 (FP32 products are exact in FP64), replicated row slices summed
 //      in a fixed order;
@@ -19,7 +19,7 @@
 // A rotation thus costs O(k) on the Gram matrix instead of O(m) on the columns, and W is read
 // twice and written once per pair and round.
 //
-// The convergence test of a sweep (solvers.py:56-62) follows in two kernels: FP64 column norms
+// The convergence test of a sweep (solvers.py:51-60) follows in two kernels: FP64 column norms
 // (with the per-matrix maximum) and the maximum of |w_i^T w_j| / (|w_i| |w_j|) over live column
 // pairs, live meaning |w_i|^2 > (eps max(m, n))^2 max_c |w_c|^2, from 16 x 16 column tiles with
 // FP64 sums (the reference forms W^H W in the data precision).  Per-matrix maxima are combined
@@ -225,7 +225,7 @@ __global__ __launch_bounds__(kBjrThreads) void bjr_round_kernel(T *W, int64_t ld
   rr_pair(round, pair, gp, gi, gj);
   const int ki = bjr_group_cols(gi, width, n, groups), kj = bjr_group_cols(gj, width, n, groups);
   const int k = ki + kj;
-  if (k < 2) return;   // a lone single column: nothing to rotate (solvers.py:50-51)
+  if (k < 2) return;   // a lone single column: nothing to rotate (solvers.py:46-47)
   const int ci0 = gi * width, cj0 = gj * width;
   auto col = [=](int c) { return c < ki ? ci0 + c : cj0 + (c - ki); };
   const int ld = k + 1;
