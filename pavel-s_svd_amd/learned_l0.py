@@ -138,7 +138,7 @@ class L0Model:
         g = torch.empty_like(self.params)
         loss = torch.empty(1, dtype=torch.float64, device=self.device)
         wsb = N._hip.mlp_workspace_bytes(b)
-        ws = torch.empty(wsb, dtype=torch.uint8, device=self.device)
+        ws = N.workspace(wsb, self.device)
         N.check(N._hip.mlp_grad(N.ptr(self.params), N.ptr(x), N.ptr(t), b, N.ptr(y), N.ptr(g), N.ptr(loss), N.ptr(ws),
                                 wsb, N.stream_of(x)), 'psvd_mlp_grad')
         return loss, g, y
@@ -159,7 +159,7 @@ class L0Model:
         v = torch.zeros_like(self.params)
         loss = torch.empty(1, dtype=torch.float64, device=self.device)
         wsb = N._hip.mlp_workspace_bytes(b)
-        ws = torch.empty(wsb, dtype=torch.uint8, device=self.device)
+        ws = N.workspace(wsb, self.device)
         st = N.stream_of(x)
         hist = []
         for step in range(1, steps + 1):

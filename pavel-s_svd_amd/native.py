@@ -67,3 +67,29 @@ def raise_for_info(info, algorithm_error):
         raise RuntimeError(algorithm_error)
     if bool((codes == INFO_COMPLETION).any()):
         raise RuntimeError('null-space completion failed: supplied live columns are not independent')
+
+
+# --- workspace accounting -----------------------------------------------------------------------
+# Every entry point takes caller-allocated workspace; this package allocates it from torch's caching
+# allocator (so the harness's allocator peaks include it) and the library itself allocates no
+# device memory.  The counters let an adapter report what one solve requested.
+_WS = {'largest': 0, 'total': 0, 'requests': 0}
+
+
+def workspace(nbytes, device):
+    """A uint8 device tensor of `nbytes` for one entry point's workspace, counted."""
+    nbytes = int(nbytes)
+    _WS['largest'] = max(_WS['largest'], nbytes)
+    _WS['total'] += nbytes
+    _WS['requests'] += 1
+    return torch.empty(nbytes, dtype=torch.uint8, device=device)
+
+
+def workspace_reset():
+    _WS.update(largest=0, total=0, requests=0)
+
+
+def workspace_stats():
+    """{'largest', 'total', 'requests'} since the last reset."""
+    return dict(_WS)
+

@@ -123,7 +123,7 @@ class UnevenBatch:
         self.classes = classes
         self.order = torch.tensor(order, dtype=torch.int32).to(self.a.device)
         dev = self.a.device
-        self.ws = torch.empty(self.ws_bytes * max(1, len(classes)), dtype=torch.uint8, device=dev)
+        self.ws = N.workspace(self.ws_bytes * max(1, len(classes)), dev)
         self.side = [torch.cuda.Stream(device=dev) for _ in range(nstreams)] if len(classes) > 1 else []
 
     @property

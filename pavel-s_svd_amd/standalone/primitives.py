@@ -42,7 +42,7 @@ def jacobi_lds_uniform(x, *, tol=None, max_sweeps=60, scale=False, cut='referenc
     vh = torch.empty(b, n, n, dtype=x.dtype, device=dev) if vectors else None
     info = torch.empty(b, 2, dtype=torch.int32, device=dev)
     wsb = N._hip.jacobi_lds_workspace_bytes(b)
-    ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
+    ws = N.workspace(wsb, dev)
     if lanes is None or threads is None:
         from .batched import DEFAULT_POLICY, launch_shape
         t_auto, l_auto = launch_shape(code, m, n, vectors, DEFAULT_POLICY)
@@ -70,7 +70,7 @@ def jacobi_grid(x, *, tol=None, max_sweeps=60, scale=False, cut='reference', vec
     vh = torch.empty(b, n, n, dtype=x.dtype, device=dev) if vectors else None
     info = torch.empty(b, 2, dtype=torch.int32, device=dev)
     wsb = N._hip.jacobi_grid_workspace_bytes(code, m, n)
-    ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
+    ws = N.workspace(wsb, dev)
     stream = N.stream_of(x)
     for i in range(b):
         st = N._hip.jacobi_grid(code, m, n, N.ptr(x[i]), N.ptr(u[i]) if vectors else 0, N.ptr(s[i]),
@@ -95,7 +95,7 @@ def jacobi_blocked(x, *, tol=None, max_sweeps=60, scale=False, cut='reference', 
     vh = torch.empty(b, n, n, dtype=x.dtype, device=dev) if vectors else None
     info = torch.empty(b, 2, dtype=torch.int32, device=dev)
     wsb = N._hip.jacobi_blocked_workspace_bytes(code, m, n)
-    ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
+    ws = N.workspace(wsb, dev)
     stream = N.stream_of(x)
     for i in range(b):
         st = N._hip.jacobi_blocked(code, m, n, N.ptr(x[i]), N.ptr(u[i]) if vectors else 0, N.ptr(s[i]),
@@ -146,7 +146,7 @@ def qr(A, *, gemm='f32', want_q=True):
     q = torch.empty(b, m, n, dtype=A.dtype, device=A.device) if want_q else None
     r = torch.empty(b, n, n, dtype=A.dtype, device=A.device)
     wsb = N._hip.qr_workspace_bytes(code, b, m, n)
-    ws = torch.empty(wsb, dtype=torch.uint8, device=A.device)
+    ws = N.workspace(wsb, A.device)
     gmode = {'f32': 0, 'bf16x3': 1}[gemm]
     st = N._hip.qr(code, gmode, b, m, n, N.ptr(A), A.stride(-2), A.stride(0) if b > 1 else 0, N.ptr(q), N.ptr(r),
                    N.ptr(ws), wsb, N.stream_of(A))
@@ -229,13 +229,13 @@ def jacobi_eigh_raw(H, tol=None, max_sweeps=30, *, method='auto', inner_sweeps=1
     info = torch.empty(b, 2, dtype=torch.int32, device=H.device)
     if method == 'cyclic':
         wsb = N._hip.eigh_workspace_bytes(code, n)
-        ws = torch.empty(wsb, dtype=torch.uint8, device=H.device)
+        ws = N.workspace(wsb, H.device)
         st = N._hip.eigh(code, b, n, N.ptr(H), N.ptr(w), N.ptr(v), N.ptr(info), float(tol or 0.0), int(max_sweeps),
                          N.ptr(ws), wsb, N.stream_of(H))
         N.check(st, 'psvd_eigh')
     else:
         wsb = N._hip.eigh_blocked_workspace_bytes(code, n)
-        ws = torch.empty(wsb, dtype=torch.uint8, device=H.device)
+        ws = N.workspace(wsb, H.device)
         st = N._hip.eigh_blocked(code, b, n, N.ptr(H), N.ptr(w), N.ptr(v), N.ptr(info), float(tol or 0.0),
                                  int(max_sweeps), int(inner_sweeps), N.ptr(ws), wsb, N.stream_of(H))
         N.check(st, 'psvd_eigh_blocked')
@@ -347,7 +347,7 @@ class BlockJacobiSweeper:
         self.corr = torch.empty(b, dtype=torch.float64, device=dev)
         self.sigma = torch.empty(b, n, dtype=torch.float64, device=dev)
         wsb = N._hip.block_jacobi_workspace_bytes(b, n)
-        self.ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
+        self.ws = N.workspace(wsb, dev)
 
     def sweep(self, tol, max_sweeps):
         w, v = self.w, self.v
@@ -365,7 +365,7 @@ def complete_columns_(u, nk):
     b, m, n = u.shape
     status = torch.empty(b, dtype=torch.int32, device=u.device)
     wsb = N._hip.complete_workspace_bytes(b, m, n)
-    ws = torch.empty(wsb, dtype=torch.uint8, device=u.device)
+    ws = N.workspace(wsb, u.device)
     st = N._hip.complete_columns(N.dtype_code(u.dtype), b, m, n, N.ptr(u), u.stride(-2), u.stride(0) if b > 1 else 0,
                                  N.ptr(nk), N.ptr(status), N.ptr(ws), wsb, N.stream_of(u))
     N.check(st, 'psvd_complete_columns')

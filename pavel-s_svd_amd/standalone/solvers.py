@@ -564,12 +564,27 @@ def factory(a, spec):
     (solvers.py:313-324)."""
     opts = {k: v for k, v in spec.items() if k not in ('name', 'method', 'mode', 'seed_offset')}
 
+    from .. import native as N
+    method = spec['method']
+
     def run():
-        u, s, vh, info = svd(a, spec['method'], full_matrices=spec.get('mode') == 'full',
+        N.workspace_reset()
+        u, s, vh, info = svd(a, method, full_matrices=spec.get('mode') == 'full',
                              values_only=spec.get('mode') == 'values', return_info=True, **opts)
         run.diagnostics = info
+        # the library allocates no device memory: all of its workspace comes from torch's allocator
+        # (inside the harness's allocator peaks); the last solve's requests are reported beside it
+        run.native_workspace_bytes = 0
+        run.library_workspace_bytes = N.workspace_stats()
         return u, s, vh
+    control = {'jacobi': 'device-side sweeps and convergence; one host read of the status per call',
+               'block_jacobi': 'one fused kernel per tournament round; one host read of the Gram test per sweep',
+               'polar_qdwh': 'one host read of the convergence pair per QDWH iteration, one per sweep of the '
+                             'eigensolver beyond LDS'}
     run.metadata = {'implementation': 'hand-written gfx950 HIP kernels', 'vendor_factorizations': False,
-                    'values_only_forms_vectors': False, 'rng_in_solve': False,
-                    'gpu_control': 'device-side sweeps and convergence; one host sync per call for status'}
+                    'values_only_forms_vectors': method in ('block_jacobi', 'polar_qdwh'),
+                    'rng_in_solve': method in PARTIAL and 'initial_basis' not in spec,
+                    'workspace': 'caller-allocated through the PyTorch allocator; no native allocations',
+                    'gpu_control': control.get(method, 'device-side sweeps and convergence; host reads of '
+                                                       'per-sweep or per-call status only')}
     return run

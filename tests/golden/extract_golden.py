@@ -5,6 +5,8 @@ rows of ``results/standalone-{cpu-hetzner,hetzner,scale-hetzner}.jsonl`` of the 
 writes:
 
 * ``tests/golden/corpus/<case>.npy`` -- the exact input bytes (numpy, ``allow_pickle=False``);
+* ``tests/golden/row_schema.json``   -- the field names every successful GPU measurement row of
+  ``results/standalone-hetzner.jsonl`` carries (row, ``memory``, timing summary);
 * ``tests/golden/expected.json``      -- per case: shape, dtype, sha256 of the input, and for each
   of ``jacobi``, ``tsqr``, ``polar_qdwh`` (the hot path) and ``block_jacobi``, ``gram_jacobi``,
   ``randomized``, ``subspace``, ``block_krylov`` (its consumers) the solver options of the row, the
@@ -61,6 +63,14 @@ def main(ref):
                                       'device': row['device'], 'case': row['case'], 'options': options,
                                       'diagnostics': diag, 'metrics': row['metrics']['worst']})
     (here / 'expected.json').write_text(json.dumps(out, indent=1, sort_keys=True) + '\n')
+    rows = [json.loads(line) for line in (ref / 'results' / FILES[1]).read_text().splitlines()]
+    ok = [r for r in rows if r.get('type') == 'measurement' and r.get('status') == 'ok']
+    schema = {'source': f'results/{FILES[1]}', 'rows': len(ok),
+              'row_keys': sorted(set.intersection(*(set(r) for r in ok))),
+              'memory_keys': sorted(set.intersection(*(set(r['memory']) for r in ok))),
+              'timing_keys': sorted(set.intersection(*(set(r['resident_solver']) for r in ok))),
+              'summary_keys': sorted(set.intersection(*(set(r['resident_solver']['wall']) for r in ok)))}
+    (here / 'row_schema.json').write_text(json.dumps(schema, indent=1) + '\n')
 
 
 if __name__ == '__main__':
