@@ -32,6 +32,8 @@ typedef void *psvd_stream_t;
 /* dtype codes */
 #define PSVD_F32 0
 #define PSVD_F64 1
+#define PSVD_C64 2        /* complex64: interleaved (re, im) float pairs; psvd_jacobi_lds only */
+#define PSVD_C128 3       /* complex128: interleaved (re, im) double pairs; psvd_jacobi_lds only */
 
 /* per-matrix device info: info[2*i] = sweeps (or iterations), info[2*i+1] = status */
 #define PSVD_INFO_OK 0
@@ -72,6 +74,10 @@ const char *psvd_version(void);
  * about k rows per lane of the largest matrix (0 means -4).  threads = 64, 128, 256, 512 or 1024 (0 selects 256).
  * One launch uses one lane count; callers with very uneven batches split them into size
  * classes, one call each (the LDS budget is also sized per call from max_m, max_n).
+ * Complex dtypes (PSVD_C64 / PSVD_C128, uniform mode only, threads fixed at 256): a, u and vh
+ * are interleaved (re, im) pairs, s is real (float / double); the complex rotation of
+ * primitives.py:340-351 (c real, s = t c a_pq / |a_pq|, w_p <- c w_p - conj(s) w_q) and
+ * vh = V^H.
  * ------------------------------------------------------------------------------------- */
 size_t psvd_jacobi_lds_workspace_bytes(int64_t batch);
 int psvd_jacobi_lds_smem_bytes(int dtype, int max_m, int max_n, int flags, size_t *bytes);

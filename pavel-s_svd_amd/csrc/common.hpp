@@ -103,6 +103,12 @@ __device__ __forceinline__ double block_max(double v, double *red) {
   return t;
 }
 
+// Complex LDS-resident one-sided Jacobi (csrc/jacobi_lds_complex.hip), reached through
+// psvd_jacobi_lds with dtype PSVD_C64 / PSVD_C128.
+size_t clds_smem_bytes(int dtype, int m, int n, bool vectors);
+int run_clds(int dtype, int64_t batch, int m, int n, const void *a, void *u, void *s, void *vh, int32_t *info,
+             double tol, int max_sweeps, int flags, int lanes, void *ws, hipStream_t stream);
+
 // Round-robin tournament (svdbench/standalone/primitives.py:309-313) in closed form: in round r
 // of an even-sized tournament on np players, seat 0 holds player 0 and seat j >= 1 holds player
 // 1 + ((j - 1 - r) mod (np - 1)); pair k faces seats k and np-1-k, ordered (min, max).

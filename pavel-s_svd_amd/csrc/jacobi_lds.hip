@@ -564,6 +564,7 @@ int psvd_jacobi_lds_smem_bytes(int dtype, int max_m, int max_n, int flags, size_
   size_t b;
   if (dtype == PSVD_F32) b = psvd::lds_bytes<float>(max_m, max_n, vectors);
   else if (dtype == PSVD_F64) b = psvd::lds_bytes<double>(max_m, max_n, vectors);
+  else if (dtype == PSVD_C64 || dtype == PSVD_C128) b = psvd::clds_smem_bytes(dtype, max_m, max_n, vectors);
   else return PSVD_EINVAL;
   *bytes = b;
   return b > psvd::kLdsCap - 4096 ? PSVD_ETOOBIG : PSVD_OK;
@@ -586,6 +587,11 @@ int psvd_jacobi_lds(int dtype, int64_t batch, int m_uniform, int n_uniform, cons
   }
   if (batch == 0) return PSVD_OK;
   hipStream_t st = static_cast<hipStream_t>(stream);
+  if (dtype == PSVD_C64 || dtype == PSVD_C128) {   // uniform batches only; threads fixed at 256
+    if (dims != nullptr || batch > 0x7fffffff) return PSVD_EINVAL;
+    return psvd::run_clds(dtype, batch, m_uniform, n_uniform, a, u, s, vh, info, tol, max_sweeps, flags, lanes,
+                          workspace, st);
+  }
   if (dtype == PSVD_F32)
     return psvd::run_lds<float>(batch, m_uniform, n_uniform, dims, offs, order, max_m, max_n, a, u, s,
                                 vh, info, tol, max_sweeps, flags, lanes, threads, workspace,

@@ -11,8 +11,8 @@ except ImportError as exc:  # pragma: no cover - exercised only on an unbuilt tr
     raise ImportError('the gfx950 kernel library is not built: run `python pavel-s_svd_amd/build.py` '
                       '(or __graft_entry__.build()) first') from exc
 
-F32, F64 = 0, 1
-DTYPE_CODE = {torch.float32: F32, torch.float64: F64}
+F32, F64, C64, C128 = 0, 1, 2, 3
+DTYPE_CODE = {torch.float32: F32, torch.float64: F64, torch.complex64: C64, torch.complex128: C128}
 
 OK, EINVAL, ELAUNCH, ETOOBIG = 0, 1, 2, 3
 INFO_OK, INFO_NOCONV, INFO_COMPLETION, INFO_NONFINITE, INFO_BADSHAPE = 0, 1, 2, 3, 4
@@ -52,7 +52,7 @@ def dtype_code(dtype):
     try:
         return DTYPE_CODE[dtype]
     except KeyError:
-        raise TypeError(f'the gfx950 kernels take float32/float64, not {dtype}') from None
+        raise TypeError(f'the gfx950 kernels take float32/float64 (complex64/128: LDS Jacobi), not {dtype}') from None
 
 
 def raise_for_info(info, algorithm_error):

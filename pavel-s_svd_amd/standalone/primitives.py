@@ -38,11 +38,13 @@ def jacobi_lds_uniform(x, *, tol=None, max_sweeps=60, scale=False, cut='referenc
     flags = (N.SCALE if scale else 0) | (N.CUT_FLOOR if cut == 'floor' else 0) | (0 if vectors else N.NO_VECTORS)
     dev = x.device
     u = torch.empty_like(x) if vectors else None
-    s = torch.empty(b, n, dtype=x.dtype, device=dev)
+    s = torch.empty(b, n, dtype=x.real.dtype if x.is_complex() else x.dtype, device=dev)
     vh = torch.empty(b, n, n, dtype=x.dtype, device=dev) if vectors else None
     info = torch.empty(b, 2, dtype=torch.int32, device=dev)
     wsb = N._hip.jacobi_lds_workspace_bytes(b)
     ws = N.workspace(wsb, dev)
+    if x.is_complex():   # the complex kernel runs 256 threads, lanes from the row count
+        threads, lanes = 256, 0 if lanes is None else lanes
     if lanes is None or threads is None:
         from .batched import DEFAULT_POLICY, launch_shape
         t_auto, l_auto = launch_shape(code, m, n, vectors, DEFAULT_POLICY)
@@ -111,6 +113,8 @@ def jacobi_any(x, method='auto', **kw):
     b, m, n = x.shape
     if method == 'auto' and lds_fits(x.dtype, m, n, kw.get('vectors', True)):
         return jacobi_lds_uniform(x, **kw)
+    if x.is_complex():
+        raise N.NativeError('complex one-sided Jacobi beyond LDS', N.ETOOBIG)
     kw.pop('lanes', None)
     kw.pop('threads', None)
     if method == 'blocked' or (method == 'auto' and n >= 64):
@@ -129,7 +133,7 @@ def jacobi_svd(A, tol=None, max_sweeps=30, *, cut='reference'):
     u, s, vh, info = jacobi_any(A, tol=tol, max_sweeps=max_sweeps, cut=cut)
     info = info.cpu()
     N.raise_for_info(info, f'one-sided Jacobi did not converge in {max_sweeps} sweeps')
-    return u, s, vh.mT, int(info[:, 0].max()) if b else 0
+    return u, s, vh.mH, int(info[:, 0].max()) if b else 0
 
 
 def qr(A, *, gemm='f32', want_q=True):
@@ -379,6 +383,8 @@ def complete_basis(q, size):
         return q
     if not k <= size <= m:
         raise ValueError('invalid basis size')
+    if q.is_complex():
+        raise N.NativeError('completion of complex bases', N.EINVAL)
     out = torch.zeros(b, m, size, dtype=q.dtype, device=q.device)
     out[:, :, :k] = q
     nk = torch.full((b,), k, dtype=torch.int32, device=q.device)

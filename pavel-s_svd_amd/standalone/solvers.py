@@ -526,8 +526,12 @@ def svd(a, algorithm='jacobi', *, full_matrices=False, values_only=False, return
         raise ValueError('input contains NaN or infinity')
     if algorithm not in SUPPORTED:
         raise Unsupported(f'{algorithm!r} is outside the MI355X hot path (built: {SUPPORTED})')
-    if a.is_complex():
-        raise Unsupported('complex dtypes are not built for MI355X yet')
+    if a.is_complex() and algorithm != 'jacobi':
+        raise Unsupported(f'complex {algorithm!r} is not built for MI355X (complex inputs: jacobi only)')
+    if a.is_complex() and not P.lds_fits(a.dtype, max(a.shape[-2:]), min(a.shape[-2:]), not values_only):
+        raise Unsupported('complex one-sided Jacobi is built for matrices that fit in LDS only')
+    if a.is_complex() and full_matrices and a.shape[-2] != a.shape[-1]:
+        raise Unsupported('full_matrices completion of complex bases is not built for MI355X')
     if a.device.type != 'cuda':
         raise Unsupported('the MI355X solvers take ROCm device tensors; there is no CPU path')
     stats = {'algorithm': algorithm, 'vendor_factorizations': 0, 'converged': True}

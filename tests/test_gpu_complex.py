@@ -1,0 +1,107 @@
+"""GPU parity of the complex one-sided Jacobi kernel (complex64 / complex128, LDS resident) against
+the reference's golden rows for its complex corpus inputs, the oracle (which restates the complex
+arithmetic of primitives.py:331-351) and complex LAPACK.
+
+Bars: sigma error over sigma_1, U/V orthogonality and reconstruction <= 1e-5 (complex64) / 1e-12
+(complex128); sweep counts within 1 of the reference's records and of the oracle's (FP64 sums);
+metrics within 3x of the reference's recorded ones on the corpus.
+"""
+import numpy as np
+import pytest
+import torch
+
+from conftest import corpus, load_package
+from oracle import metrics as M
+from oracle import restatement as R
+
+pytestmark = pytest.mark.gpu
+DEV = 'cuda:0'
+KEYS = ('spectrum_error_over_sigma1', 'orthogonality_u_fro_over_sqrt_cols', 'orthogonality_v_fro_over_sqrt_rows',
+        'reconstruction_relative_fro')
+
+
+@pytest.fixture(scope='module')
+def pkg():
+    return load_package()
+
+
+def _np(t):
+    return None if t is None else t.detach().cpu().numpy()
+
+
+def _metrics(a, out):
+    return M.validate(a, tuple(_np(t) for t in out), M.reference(a))['worst']
+
+
+@pytest.mark.parametrize('case', ['clustered_complex', 'wide_complex'])
+def test_complex_corpus_matches_reference_golden(pkg, expected, case):
+    a = corpus(case)
+    print(f'complex corpus {case} {a.shape} {a.dtype}', flush=True)
+    u, s, vh, info = pkg.svd(torch.from_numpy(a).to(DEV), 'jacobi', return_info=True)
+    assert u.dtype == torch.from_numpy(a).dtype and s.dtype == torch.from_numpy(a.real.copy()).dtype
+    met = _metrics(a, (u, s, vh))
+    runs = [r for r in expected[case]['runs'] if r['method'] == 'jacobi']
+    assert runs
+    for run in runs:
+        assert abs(info['jacobi_sweeps'] - run['diagnostics']['jacobi_sweeps']) <= 1, run['source']
+        floor = 64 * np.finfo(a.real.dtype).eps * max(a.shape[-2:]) ** 0.5
+        for key in KEYS:
+            assert met[key] <= max(3 * run['metrics'][key], floor), (key, met[key], run['metrics'][key])
+
+
+@pytest.mark.parametrize('dtype', [np.complex64, np.complex128])
+@pytest.mark.parametrize('shape', [(3, 8, 8), (2, 40, 17), (1, 64, 64), (1, 96, 96), (4, 33, 1), (2, 70, 50)])
+def test_complex_random_matches_lapack_and_oracle(pkg, dtype, shape):
+    rng = np.random.default_rng(shape[1] * 7 + shape[2])
+    a = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(dtype)
+    print(f'complex random {shape} {np.dtype(dtype).name}', flush=True)
+    from importlib import import_module
+    P = import_module('pavel-s_svd_amd.standalone.primitives')
+    if not P.lds_fits(torch.from_numpy(a).dtype, shape[1], shape[2]):   # 96 x 96 complex128: 300 KB
+        with pytest.raises(pkg.Unsupported):
+            pkg.svd(torch.from_numpy(a).to(DEV), 'jacobi')
+        return
+    u, s, vh, info = pkg.svd(torch.from_numpy(a).to(DEV), 'jacobi', return_info=True)
+    met = _metrics(a, (u, s, vh))
+    bar = 1e-5 if dtype == np.complex64 else 1e-12
+    for key in KEYS:
+        assert met[key] <= bar, (key, met[key])
+    if shape[1] <= 40:
+        _, rs, _, rinfo = R.svd(a, 'jacobi', return_info=True)
+        lap = np.stack([np.linalg.svd(x.astype(np.complex128), compute_uv=False) for x in a])
+        oracle_err = np.max(np.abs(rs - lap)) / lap.max()
+        assert np.max(np.abs(_np(s) - rs)) / rs.max() <= bar + oracle_err
+        assert abs(info['jacobi_sweeps'] - rinfo['jacobi_sweeps']) <= 1, (info, rinfo)
+
+
+def test_complex_rank_deficient_zero_wide_and_values_only(pkg):
+    rng = np.random.default_rng(21)
+    f1 = rng.standard_normal((30, 3)) + 1j * rng.standard_normal((30, 3))
+    f2 = rng.standard_normal((3, 12)) + 1j * rng.standard_normal((3, 12))
+    low = (f1 @ f2).astype(np.complex128)
+    u, s, vh = pkg.svd(torch.from_numpy(low).to(DEV), 'jacobi')
+    sv, uu = _np(s), _np(u)
+    assert np.count_nonzero(sv) == 3
+    assert np.max(np.abs(uu.conj().T @ uu - np.eye(12))) < 1e-12          # completed null columns
+    assert np.max(np.abs((uu * sv) @ _np(vh) - low)) < 1e-12 * np.abs(low).max()
+    z = torch.zeros(2, 10, 6, dtype=torch.complex64, device=DEV)
+    u, s, vh, info = pkg.svd(z, 'jacobi', return_info=True)
+    assert not s.any() and info['jacobi_sweeps'] == 1
+    uu = _np(u[0])
+    assert np.max(np.abs(uu.conj().T @ uu - np.eye(6))) < 1e-6
+    w = (rng.standard_normal((9, 25)) + 1j * rng.standard_normal((9, 25))).astype(np.complex64)
+    u, s, vh = pkg.svd(torch.from_numpy(w).to(DEV), 'jacobi')
+    assert np.max(np.abs((_np(u) * _np(s)) @ _np(vh) - w)) < 1e-5 * np.abs(w).max()
+    _, sv, _ = pkg.svd(torch.from_numpy(w).to(DEV), 'jacobi', values_only=True)
+    assert np.max(np.abs(_np(sv) - np.linalg.svd(w.astype(np.complex128), compute_uv=False))) < 1e-5 * float(sv[0])
+
+
+def test_complex_unsupported_paths(pkg):
+    x = torch.zeros(1, 20, 10, dtype=torch.complex64, device=DEV) + 1
+    for method in ('tsqr', 'polar_qdwh', 'block_jacobi', 'randomized'):
+        with pytest.raises(pkg.Unsupported):
+            pkg.svd(x, method)
+    with pytest.raises(pkg.Unsupported):
+        pkg.svd(torch.zeros(1, 600, 300, dtype=torch.complex128, device=DEV) + 1, 'jacobi')
+    with pytest.raises(pkg.Unsupported):
+        pkg.svd(x, 'jacobi', full_matrices=True)

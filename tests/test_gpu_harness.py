@@ -73,5 +73,9 @@ def test_unsupported_rows(H):
     assert row['status'] == 'unsupported'
     c = corpus('clustered_complex')
     row = H.measurement_row(c, {'name': 'clustered_complex'}, {'name': 'pavel-s_svd_amd:factory',
-                                                               'method': 'jacobi'}, 'cuda:0', warmup=0, repeat=1)
+                                                               'method': 'tsqr'}, 'cuda:0', warmup=0, repeat=1)
     assert row['status'] == 'unsupported' and 'complex' in row['error']
+    row = H.measurement_row(c, {'name': 'clustered_complex'}, {'name': 'pavel-s_svd_amd:factory',
+                                                               'method': 'jacobi'}, 'cuda:0', warmup=0, repeat=1)
+    assert row['status'] == 'ok'
+    json.dumps(row, allow_nan=False)
