@@ -3,8 +3,12 @@
 Tunes, on the actual batch, the launch-shape policy -- the waves per CU the resident matrices
 should supply (which sets each matrix's workgroup size from its LDS footprint) and the minimum
 rows per lane that bounds the lanes per column pair; together they fix the size classes the
-batch is split into -- by timing each candidate solve with HIP events on the current stream.  Results are cached in-process per (dtype, max_m, max_n, batch
-size bucket, vectors) and, when ``PAVELSVD_TUNE_CACHE`` names a JSON file, across processes.
+batch is split into.  Two stages, both on captured-graph replays (the path a caller replays)
+timed with HIP events on the current stream: every candidate over a few replays, then the
+fastest few again in interleaved rounds, ranked by their median replay time -- single short
+timings differ by a few percent from run to run, as much as the candidates differ.
+Results are cached in-process per (dtype, max_m, max_n, batch size bucket, vectors) and, when
+``PAVELSVD_TUNE_CACHE`` names a JSON file, across processes.
 """
 import json
 import os
@@ -48,7 +52,21 @@ def time_launch(batch, reps=3):
     return e0.elapsed_time(e1) / reps
 
 
-def tune_uneven(batch, candidates=None, reps=3, verbose=None):
+def time_replays(batch, reps=8):
+    """Per-replay times (ms) of the batch's captured graph, after one warm replay."""
+    stream = torch.cuda.current_stream(batch.a.device)
+    batch.capture()
+    batch.replay()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
+    for e0, e1 in ev:
+        e0.record(stream)
+        batch.replay()
+        e1.record(stream)
+    ev[-1][1].synchronize()
+    return [e0.elapsed_time(e1) for e0, e1 in ev]
+
+
+def tune_uneven(batch, candidates=None, reps=3, verbose=None, finalists=4, rounds=2):
     """Return the fastest launch parameters for ``batch`` (an UnevenBatch) and leave the batch
     configured with them."""
     if verbose is None:
@@ -64,12 +82,26 @@ def tune_uneven(batch, candidates=None, reps=3, verbose=None):
     timings = []
     for cand in candidates or CANDIDATES:
         batch.configure(**cand)
-        ms = time_launch(batch, reps)
+        t = sorted(time_replays(batch, reps + 1))
+        ms = t[len(t) // 2]
         timings.append((ms, cand))
         if verbose:
             print(f'autotune {key} {cand}: {ms:.3f} ms', file=sys.stderr, flush=True)
-    ms, best = min(timings, key=lambda t: t[0])
+    timings.sort(key=lambda t: t[0])
+    top = [c for _, c in timings[:max(1, finalists)]]
+    samples = [[] for _ in top]
+    for _ in range(rounds):
+        for i, cand in enumerate(top):
+            batch.configure(**cand)
+            samples[i] += time_replays(batch)
+    medians = [sorted(x)[len(x) // 2] for x in samples]
+    i_best = min(range(len(top)), key=lambda i: medians[i])
+    best, ms = top[i_best], medians[i_best]
+    if verbose:
+        for cand, med in zip(top, medians):
+            print(f'autotune {key} replay {cand}: median {med:.3f} ms', file=sys.stderr, flush=True)
     batch.configure(**{k: v for k, v in saved.items() if k not in best}, **best)
-    _CACHE[key] = {'params': best, 'ms': ms, 'timings': [[t, c] for t, c in timings]}
+    _CACHE[key] = {'params': best, 'ms': ms, 'timings': [[t, c] for t, c in timings],
+                   'replay_medians': [[med, c] for med, c in zip(medians, top)]}
     _store()
     return dict(best)
