@@ -376,6 +376,45 @@ def complete_columns_(u, nk):
     return status
 
 
+def complete_null_columns(u, keep):
+    """Replace the columns of u (b, m, n) where keep (b, n) is False by completed basis vectors
+    and keep the rest (primitives.py:430-455): the kept columns are moved to the front (stable),
+    the missing ones completed in their original order by the completion kernel -- each against
+    the kept and the already completed columns, as the reference's loop does -- and the columns
+    moved back.  Raises RuntimeError if a residual vanishes (:451-452)."""
+    if bool(keep.all()):
+        return u
+    b, m, n = u.shape
+    keep = keep.to(device=u.device, dtype=torch.bool).reshape(b, n)
+    idx = torch.arange(n, device=u.device).expand(b, n)
+    perm = torch.argsort((~keep).to(torch.int8) * n + idx, dim=1)        # kept first, each part in order
+    w = torch.gather(u, 2, perm[:, None, :].expand(b, m, n)).contiguous()
+    status = complete_columns_(w, keep.sum(1).to(torch.int32))
+    if bool((status != 0).any()):
+        raise RuntimeError('null-space completion failed: supplied live columns are not independent')
+    out = torch.empty_like(w)
+    out.scatter_(2, perm[:, None, :].expand(b, m, n), w)
+    return out
+
+
+def _eps(dtype):
+    """Machine epsilon of the (real part of the) dtype, as primitives.py's helper."""
+    return torch.finfo(dtype.to_real() if dtype.is_complex else dtype).eps
+
+
+def _real_dtype(dtype):
+    return dtype.to_real() if dtype.is_complex else dtype
+
+
+def _round_robin(n):
+    """Rounds of the round-robin tournament on n (even) players (primitives.py:309-313): n - 1
+    lists of n/2 disjoint pairs (min, max); seat 0 stays, the others rotate by one seat per round."""
+    seats = list(range(n))
+    for _ in range(n - 1):
+        yield [(min(seats[i], seats[n - 1 - i]), max(seats[i], seats[n - 1 - i])) for i in range(n // 2)]
+        seats = [seats[0], seats[-1]] + seats[1:-1]
+
+
 def complete_basis(q, size):
     """Extend orthonormal columns q (b, m, k) to `size` columns (primitives.py:458-467)."""
     b, m, k = q.shape
