@@ -81,6 +81,24 @@ def cpu_baseline(flat, ms, ns, offs, budget_s=15.0):
                       f'{dt:.1f} s on 1 host core (numpy, FP32 arithmetic as the reference)'}
 
 
+def plain_gpu_form(pkg, flat, ms, ns, offs, dev, count=512):
+    """The reference API on the GPU, one svd(a_i, 'jacobi') call per matrix (the API takes
+    uniform shapes only): the same kernels, tolerance, scaling and cut as the batched launch, without
+    batching, the persistent queue or graph replay; the first `count` matrices of the batch."""
+    mats = [torch.from_numpy(flat[o:o + m * n].reshape(m, n)).to(dev) for m, n, o in zip(ms[:count], ns[:count],
+                                                                                           offs[:count])]
+    for a in mats[:8]:
+        pkg.svd(a, 'jacobi', cut='floor')
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for a in mats:
+        pkg.svd(a, 'jacobi', cut='floor')
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    return {'value': len(mats) / dt, 'unit': 'SVDs/s', 'sample': f'first {len(mats)} of the {COUNT} matrices, one '
+            f'svd(..., "jacobi", cut="floor") call each on cuda:0, {dt:.2f} s'}
+
+
 def _haar64(n, k, gen, dev):
     """n x k Haar columns: Householder QR (this package's FP64 kernel) of seeded N(0, 1), R-diagonal
     sign fix."""
@@ -355,6 +373,9 @@ def main():
     }
     if not args.no_check:
         out['accuracy_vs_fp64_lapack'] = accuracy(batch, flat, ms, ns, offs)
+    plain = plain_gpu_form(pkg, flat, ms, ns, offs, dev)
+    plain['batched_speedup'] = out['value'] / plain['value']
+    out['plain_gpu_form'] = plain
     if not args.no_cpu:
         out['cpu_baseline'] = cpu_baseline(flat, ms, ns, offs)
     if args.only != 'c2':
