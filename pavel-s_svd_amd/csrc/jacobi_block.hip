@@ -84,15 +84,38 @@ __global__ __launch_bounds__(kJbThreads) void bj_load_kernel(BjParams<T> P) {
 // max_j |w_j|^2 for the floor of this sweep (slot[1] cleared before the launch)
 template <typename T>
 __global__ __launch_bounds__(kJbThreads) void bj_colmax_kernel(BjParams<T> P) {
-  const int c = blockIdx.x * kJbThreads + threadIdx.x;
+  // 16 columns per workgroup, 16 row groups; thread (g, c) sums rows g, g + 16, ... with four
+  // loads in flight, the row-group partials are added in group order, then the maximum
+  constexpr int NC = 16, NG = kJbThreads / NC;
+  __shared__ double part[NG][NC];
+  const int g = threadIdx.x / NC, cl = threadIdx.x % NC;
+  const int c = blockIdx.x * NC + cl;
   double acc = 0.0;
-  if (c < P.n)
-    for (int i = 0; i < P.m; ++i) {
-      const double x = (double)P.W[(int64_t)i * P.N + c];
+  if (c < P.n) {
+    const T *col = P.W + c;
+    int i = g;
+    for (; i + 3 * NG < P.m; i += 4 * NG) {
+      const double x0 = (double)col[(int64_t)i * P.N], x1 = (double)col[(int64_t)(i + NG) * P.N];
+      const double x2 = (double)col[(int64_t)(i + 2 * NG) * P.N], x3 = (double)col[(int64_t)(i + 3 * NG) * P.N];
+      acc = fma(x0, x0, acc);
+      acc = fma(x1, x1, acc);
+      acc = fma(x2, x2, acc);
+      acc = fma(x3, x3, acc);
+    }
+    for (; i < P.m; i += NG) {
+      const double x = (double)col[(int64_t)i * P.N];
       acc = fma(x, x, acc);
     }
-  acc = wave_max_fast(acc);
-  if ((threadIdx.x & (kWave - 1)) == 0) atomicMax(&P.slot[1], (unsigned long long)__double_as_longlong(acc));
+  }
+  part[g][cl] = acc;
+  __syncthreads();
+  if (threadIdx.x < kWave) {
+    double tot = 0.0;
+    if (threadIdx.x < NC)
+      for (int x = 0; x < NG; ++x) tot += part[x][threadIdx.x];
+    tot = wave_max_fast(tot);
+    if (threadIdx.x == 0) atomicMax(&P.slot[1], (unsigned long long)__double_as_longlong(tot));
+  }
 }
 
 // Gram partial of pair p over the rows of slice z: 64 x 64, FP64 MFMA (16x16x4), wave w owns the
@@ -309,14 +332,35 @@ __global__ __launch_bounds__(kJbThreads) void bj_update_kernel(BjParams<T> P, in
 
 template <typename T>
 __global__ __launch_bounds__(kJbThreads) void bj_sigma_kernel(BjParams<T> P) {
-  const int c = blockIdx.x * kJbThreads + threadIdx.x;
-  if (c >= P.n) return;
+  // column norms in FP64: 16 columns per workgroup, 16 row groups added in group order
+  constexpr int NC = 16, NG = kJbThreads / NC;
+  __shared__ double part[NG][NC];
+  const int g = threadIdx.x / NC, cl = threadIdx.x % NC;
+  const int c = blockIdx.x * NC + cl;
   double acc = 0.0;
-  for (int i = 0; i < P.m; ++i) {
-    const double x = (double)P.W[(int64_t)i * P.N + c];
-    acc = fma(x, x, acc);
+  if (c < P.n) {
+    const T *col = P.W + c;
+    int i = g;
+    for (; i + 3 * NG < P.m; i += 4 * NG) {
+      const double x0 = (double)col[(int64_t)i * P.N], x1 = (double)col[(int64_t)(i + NG) * P.N];
+      const double x2 = (double)col[(int64_t)(i + 2 * NG) * P.N], x3 = (double)col[(int64_t)(i + 3 * NG) * P.N];
+      acc = fma(x0, x0, acc);
+      acc = fma(x1, x1, acc);
+      acc = fma(x2, x2, acc);
+      acc = fma(x3, x3, acc);
+    }
+    for (; i < P.m; i += NG) {
+      const double x = (double)col[(int64_t)i * P.N];
+      acc = fma(x, x, acc);
+    }
   }
-  P.sig[c] = sqrt(acc);
+  part[g][cl] = acc;
+  __syncthreads();
+  if (threadIdx.x < NC && c < P.n) {
+    double tot = 0.0;
+    for (int x = 0; x < NG; ++x) tot += part[x][threadIdx.x];
+    P.sig[c] = sqrt(tot);
+  }
 }
 
 template <typename T>
@@ -462,7 +506,7 @@ static int run_bj(int m, int n, const T *a, T *u, T *s, T *vh, int32_t *info, do
   for (int sw = 1; sw <= max_sweeps; ++sw) {
     sweeps = sw;
     hipLaunchKernelGGL(bj_reset_kernel, dim3(1), dim3(64), 0, st, P.slot, P.flag, 2);
-    hipLaunchKernelGGL(bj_colmax_kernel<T>, dim3((n + kJbThreads - 1) / kJbThreads), dim3(kJbThreads), 0, st, P);
+    hipLaunchKernelGGL(bj_colmax_kernel<T>, dim3((n + 15) / 16), dim3(kJbThreads), 0, st, P);
     for (int r = 0; r < P.nb - 1; ++r) {
       hipLaunchKernelGGL(bj_gram_kernel<T>, dim3(npair, P.splits), dim3(kJbThreads), 0, st, P, r);
       hipLaunchKernelGGL(bj_inner_kernel<T>, dim3(npair), dim3(kJbInner), bj_inner_lds<T>(), st, P, r);
@@ -477,7 +521,7 @@ static int run_bj(int m, int n, const T *a, T *u, T *s, T *vh, int32_t *info, do
       break;
     }
   }
-  hipLaunchKernelGGL(bj_sigma_kernel<T>, dim3((n + kJbThreads - 1) / kJbThreads), dim3(kJbThreads), 0, st, P);
+  hipLaunchKernelGGL(bj_sigma_kernel<T>, dim3((n + 15) / 16), dim3(kJbThreads), 0, st, P);
   hipLaunchKernelGGL(bj_order_kernel<T>, dim3((n + kJbThreads - 1) / kJbThreads), dim3(kJbThreads), 0, st, P);
   hipLaunchKernelGGL(bj_out_kernel<T>, dim3(blocks((int64_t)m * n)), dim3(kJbThreads), 0, st, P, sweeps, converged);
   if (hipGetLastError() != hipSuccess) return PSVD_ELAUNCH;
