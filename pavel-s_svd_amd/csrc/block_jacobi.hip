@@ -54,43 +54,6 @@ __device__ __forceinline__ void bjr_unrank(int e, int k, int &p, int &q) {
   q = p + e;
 }
 
-// The rotation of primitives.py:340-347 (tau = (aqq - app) / (2 |apq|), t = sign(tau) / (|tau| +
-// hypot(tau, 1)), c = 1 / sqrt(1 + t^2), s = t c sign(apq)) with c and s in FP64 so that c^2 + s^2 =
-// 1 to FP64 rounding.  For FP32 data the angle itself is evaluated in FP32 (tau from an FP64
-// reciprocal with one Newton step): the data's own tolerance (10 eps32) needs no more, the
-// remaining off-diagonal is swept on; FP64 data takes the full FP64 formula.
-template <typename T>
-__device__ __forceinline__ void bjr_rotation(double app, double aqq, double apq, double &c, double &s);
-
-template <>
-__device__ __forceinline__ void bjr_rotation<double>(double app, double aqq, double apq, double &c, double &s) {
-  jacobi_rotation(app, aqq, apq, c, s);
-}
-
-template <>
-__device__ __forceinline__ void bjr_rotation<float>(double app, double aqq, double apq, double &c, double &s) {
-  const double a2 = 2.0 * fabs(apq);
-  double r = __builtin_amdgcn_rcp(a2);
-  r = r * fma(-a2, r, 2.0);
-  const double tau = (aqq - app) * r;
-  const double at = fabs(tau);
-  double t;
-  if (tau == 0.0) {
-    t = 1.0;
-  } else if (at < 1e18) {
-    const float af = (float)at;
-    t = (double)__builtin_amdgcn_rcpf(af + __builtin_amdgcn_sqrtf(fmaf(af, af, 1.0f)));
-  } else {
-    t = 0.5 / at;
-  }
-  t = copysign(t, tau);
-  const double x = fma(t, t, 1.0);
-  double y = __builtin_amdgcn_rsq(x);
-  y = y * fma(-0.5 * x * y, y, 1.5);
-  c = y;
-  s = apq < 0.0 ? -t * y : t * y;
-}
-
 static size_t bjr_lds_bytes(int kmax) {
   const size_t doubles = 2 * (size_t)kmax * (kmax + 1) + kBjrChunk + kBjrThreads + kBjrMaxK + 8;
   const size_t ints = 2 * (kBjrMaxK / 2) + kBjrMaxK + 2;
@@ -149,7 +112,7 @@ __device__ __forceinline__ void bjr_inner(double *G, double *Q, int k, int ld, d
             const bool big = sizeof(T) == 4 ? apq * apq > tol2 * (app * aqq)
                                             : fabs(apq) > tol * (sqrt(app) * sqrt(aqq));
             active = big && fmin(app, aqq) > eps2 * dmax;
-            if (active) bjr_rotation<T>(app, aqq, apq, c, s);
+            if (active) jacobi_rotation_fast<T>(app, aqq, apq, c, s);
           }
           any = any || active;
           rp[lane] = p;

@@ -142,6 +142,46 @@ __device__ __forceinline__ void jacobi_rotation(double app, double aqq, double a
   s = t * c * (apq / a);
 }
 
+// The rotation of jacobi_rotation with c and s in FP64 (c^2 + s^2 = 1 to FP64 rounding, so the
+// accumulated rotations stay orthogonal) and, for FP32 data, the angle t itself in FP32: tau from
+// an FP64 reciprocal with one Newton step, t = 1 / (|tau| + sqrt(tau^2 + 1)) with the native FP32
+// reciprocal and square root, c = 1 / sqrt(1 + t^2) by v_rsq_f64 and one Newton step.  An FP32
+// angle leaves an off-diagonal of about 1e-7 of the pair's spread, below the FP32 data's 10 eps
+// tolerance; FP64 data takes jacobi_rotation.
+template <typename T>
+__device__ __forceinline__ void jacobi_rotation_fast(double app, double aqq, double apq, double &c, double &s);
+
+template <>
+__device__ __forceinline__ void jacobi_rotation_fast<double>(double app, double aqq, double apq, double &c,
+                                                             double &s) {
+  jacobi_rotation(app, aqq, apq, c, s);
+}
+
+template <>
+__device__ __forceinline__ void jacobi_rotation_fast<float>(double app, double aqq, double apq, double &c,
+                                                            double &s) {
+  const double a2 = 2.0 * fabs(apq);
+  double r = __builtin_amdgcn_rcp(a2);
+  r = r * fma(-a2, r, 2.0);
+  const double tau = (aqq - app) * r;
+  const double at = fabs(tau);
+  double t;
+  if (tau == 0.0) {
+    t = 1.0;
+  } else if (at < 1e18) {
+    const float af = (float)at;
+    t = (double)__builtin_amdgcn_rcpf(af + __builtin_amdgcn_sqrtf(fmaf(af, af, 1.0f)));
+  } else {
+    t = 0.5 / at;
+  }
+  t = copysign(t, tau);
+  const double x = fma(t, t, 1.0);
+  double y = __builtin_amdgcn_rsq(x);
+  y = y * fma(-0.5 * x * y, y, 1.5);
+  c = y;
+  s = apq < 0.0 ? -t * y : t * y;
+}
+
 // The same rotation with the Jacobi angle evaluated in the storage precision: tau from the FP64
 // sums (difference taken in FP64), t, c, s in T.  `tb` returns t*|apq| for the exact-arithmetic
 // norm update app' = app - t|apq|, aqq' = aqq + t|apq|.

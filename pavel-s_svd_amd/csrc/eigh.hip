@@ -51,14 +51,21 @@ __host__ __device__ inline size_t eigh_lds_bytes(int n) {
 
 template <typename T>
 __device__ __forceinline__ void pair_params(const T *G, int ld, int p, int q, double tol, double scale, double &c,
-                                            double &s, bool &active, double &ratio) {
+                                            double &s, bool &active, double &ratio, bool fast = false) {
   const double gpp = (double)G[p * ld + p], gqq = (double)G[q * ld + q], gpq = (double)G[p * ld + q];
   const double a = fabs(gpq);
   ratio = a / fmax(scale, DBL_MIN);
   active = a > tol * scale;
   c = 1.0;
   s = 0.0;
-  if (active) jacobi_rotation(gpp, gqq, gpq, c, s);
+  // fast: the FP32 angle of jacobi_rotation_fast (blocked solver; the kernels that keep the
+  // reference's rotation order and sweep counts use the FP64 formula)
+  if (active) {
+    if (fast)
+      jacobi_rotation_fast<T>(gpp, gqq, gpq, c, s);
+    else
+      jacobi_rotation(gpp, gqq, gpq, c, s);
+  }
 }
 
 constexpr int kEighThreads = 256;
@@ -523,7 +530,7 @@ __global__ __launch_bounds__(kBlkInner) void blk_inner_kernel(BlkParams<T> P, in
         rr_pair(rr, tid, kEs, a, b);
         double c, s, ratio;
         bool act;
-        pair_params(S, kEsLd, a, b, tol, scale, c, s, act, ratio);
+        pair_params(S, kEsLd, a, b, tol, scale, c, s, act, ratio, true);
         cc[tid] = act ? c : 1.0;
         ss[tid] = act ? s : 0.0;
         pp[tid] = a;

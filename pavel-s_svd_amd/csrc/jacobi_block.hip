@@ -222,7 +222,7 @@ __global__ __launch_bounds__(kJbInner) void bj_inner_kernel(BjParams<T> P, int r
         const double app = S[a * kJsLd + a], aqq = S[b * kJsLd + b], apq = S[a * kJsLd + b];
         const bool act = apq * apq > tol * tol * app * aqq && fmin(app, aqq) > floor_;
         double c = 1.0, s = 0.0;
-        if (act) jacobi_rotation(app, aqq, apq, c, s);
+        if (act) jacobi_rotation_fast<T>(app, aqq, apq, c, s);
         cc[tid] = c;
         ss[tid] = act ? s : 0.0;
         pp[tid] = a;
