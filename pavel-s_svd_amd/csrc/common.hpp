@@ -109,6 +109,17 @@ size_t clds_smem_bytes(int dtype, int m, int n, bool vectors);
 int run_clds(int dtype, int64_t batch, int m, int n, const void *a, void *u, void *s, void *vh, int32_t *info,
              double tol, int max_sweeps, int flags, int lanes, void *ws, hipStream_t stream);
 
+// Item e of the upper triangle (k1 <= k2) of an np2 x np2 grid, row by row.
+__device__ __forceinline__ void tri_decode(int64_t e, int np2, int &k1, int &k2) {
+  int a = (int)((2.0 * np2 + 1.0 - sqrt((2.0 * np2 + 1.0) * (2.0 * np2 + 1.0) - 8.0 * (double)e)) / 2.0);
+  if (a < 0) a = 0;
+  if (a > np2 - 1) a = np2 - 1;
+  while (a > 0 && (int64_t)a * np2 - (int64_t)a * (a - 1) / 2 > e) --a;
+  while (a < np2 - 1 && (int64_t)(a + 1) * np2 - (int64_t)(a + 1) * a / 2 <= e) ++a;
+  k1 = a;
+  k2 = a + (int)(e - ((int64_t)a * np2 - (int64_t)a * (a - 1) / 2));
+}
+
 // Round-robin tournament (svdbench/standalone/primitives.py:309-313) in closed form: in round r
 // of an even-sized tournament on np players, seat 0 holds player 0 and seat j >= 1 holds player
 // 1 + ((j - 1 - r) mod (np - 1)); pair k faces seats k and np-1-k, ordered (min, max).

@@ -447,15 +447,6 @@ __device__ __forceinline__ void slot_max(unsigned long long *s, double v) {
 }
 
 // upper-triangle index e of an np2 x np2 grid (k1 <= k2, row-major) -> (k1, k2)
-__device__ __forceinline__ void tri_decode(int64_t e, int np2, int &k1, int &k2) {
-  int a = (int)((2.0 * np2 + 1.0 - sqrt((2.0 * np2 + 1.0) * (2.0 * np2 + 1.0) - 8.0 * (double)e)) / 2.0);
-  if (a < 0) a = 0;
-  if (a > np2 - 1) a = np2 - 1;
-  while (a > 0 && (int64_t)a * np2 - (int64_t)a * (a - 1) / 2 > e) --a;
-  while (a < np2 - 1 && (int64_t)(a + 1) * np2 - (int64_t)(a + 1) * a / 2 <= e) ++a;
-  k1 = a;
-  k2 = a + (int)(e - ((int64_t)a * np2 - (int64_t)a * (a - 1) / 2));
-}
 
 template <typename T>
 __global__ __launch_bounds__(256) void blk_load_kernel(BlkParams<T> P) {
@@ -492,10 +483,10 @@ constexpr int kBlkInner = 512;
 
 template <typename T>
 __global__ __launch_bounds__(kBlkInner) void blk_inner_kernel(BlkParams<T> P, int r, int g) {
-  extern __shared__ double smem_bi[];
-  double *cc = smem_bi, *ss = cc + kEb;
-  int *pp = reinterpret_cast<int *>(ss + kEb), *qq = pp + kEb;
-  T *S = reinterpret_cast<T *>(qq + kEb);
+  extern __shared__ __align__(16) double smem_bi[];
+  double2 *cs = reinterpret_cast<double2 *>(smem_bi);   // (c, s) of pair k of the round
+  int2 *pq = reinterpret_cast<int2 *>(cs + kEb);          // (p, q) of pair k
+  T *S = reinterpret_cast<T *>(pq + kEb);
   T *Qt = S + kEs * kEsLd;
   const int p = blockIdx.x, tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
   int I, J;
@@ -515,13 +506,18 @@ __global__ __launch_bounds__(kBlkInner) void blk_inner_kernel(BlkParams<T> P, in
   // (every pair test fails and nothing changes): skip it
   const int live = __syncthreads_or(offmax > tol * scale);
   int any = 0;
-  __shared__ unsigned short tri[kEb * (kEb + 1) / 2];     // upper-triangle item -> (k1 << 8) | k2
-  for (int e = tid; e < kEb * (kEb + 1) / 2; e += kBlkInner) {
-    int k1, k2;
-    tri_decode(e, kEb, k1, k2);
-    tri[e] = (unsigned short)((k1 << 8) | k2);
+  // this thread's items of the round, fixed for the launch: upper-triangle blocks (k1 <= k2) of
+  // the 32 x 32 pair grid (528: one or two per thread) and four Qt rows pairs at column tid % 64
+  constexpr int NTRI = kEb * (kEb + 1) / 2, NIT = (NTRI + kBlkInner - 1) / kBlkInner;
+  int it1[NIT], it2[NIT];
+#pragma unroll
+  for (int j = 0; j < NIT; ++j) {
+    const int e = tid + j * kBlkInner;
+    it1[j] = it2[j] = -1;
+    if (e < NTRI) tri_decode(e, kEb, it1[j], it2[j]);
   }
-  __syncthreads();
+  constexpr int QIT = kEb * kEs / kBlkInner;
+  const int qcol = tid % kEs, qk = tid / kEs;
   for (int sw = 0; live && sw < P.inner_sweeps; ++sw) {
     int act_sw = 0;
     for (int rr = 0; rr < kEs - 1; ++rr) {
@@ -531,28 +527,29 @@ __global__ __launch_bounds__(kBlkInner) void blk_inner_kernel(BlkParams<T> P, in
         double c, s, ratio;
         bool act;
         pair_params(S, kEsLd, a, b, tol, scale, c, s, act, ratio, true);
-        cc[tid] = act ? c : 1.0;
-        ss[tid] = act ? s : 0.0;
-        pp[tid] = a;
-        qq[tid] = b;
+        cs[tid] = make_double2(act ? c : 1.0, act ? s : 0.0);
+        pq[tid] = make_int2(a, b);
         act_sw |= act;
       }
       __syncthreads();
-      for (int e = tid; e < kEb * (kEb + 1) / 2; e += kBlkInner) {
-        const int k1 = tri[e] >> 8, k2 = tri[e] & 0xff;
-        const double s1 = ss[k1], s2 = ss[k2];
-        if (s1 != 0.0 || s2 != 0.0)
-          block_update(S, kEsLd, pp[k1], qq[k1], pp[k2], qq[k2], cc[k1], s1, cc[k2], s2, s1 != 0.0, s2 != 0.0);
+#pragma unroll
+      for (int j = 0; j < NIT; ++j) {
+        if (it1[j] < 0) continue;
+        const double2 r1 = cs[it1[j]], r2 = cs[it2[j]];
+        if (r1.y == 0.0 && r2.y == 0.0) continue;
+        const int2 a1 = pq[it1[j]], a2 = pq[it2[j]];
+        block_update(S, kEsLd, a1.x, a1.y, a2.x, a2.y, r1.x, r1.y, r2.x, r2.y, r1.y != 0.0, r2.y != 0.0);
       }
-      for (int e = tid; e < kEb * kEs; e += kBlkInner) {
-        const int k = e / kEs, j = e - k * kEs;
-        const double s = ss[k];
-        if (s == 0.0) continue;
-        const double c = cc[k];
-        T *vp = Qt + pp[k] * kEsLd + j, *vq = Qt + qq[k] * kEsLd + j;
+#pragma unroll
+      for (int j = 0; j < QIT; ++j) {
+        const int k = qk + j * (kBlkInner / kEs);
+        const double2 r = cs[k];
+        if (r.y == 0.0) continue;
+        const int2 a = pq[k];
+        T *vp = Qt + a.x * kEsLd + qcol, *vq = Qt + a.y * kEsLd + qcol;
         const double x = (double)*vp, y = (double)*vq;
-        *vp = (T)(c * x - s * y);
-        *vq = (T)(s * x + c * y);
+        *vp = (T)(r.x * x - r.y * y);
+        *vq = (T)(r.y * x + r.x * y);
       }
       __syncthreads();
     }

@@ -177,11 +177,10 @@ constexpr int kJbInner = 512;
 
 template <typename T>
 __global__ __launch_bounds__(kJbInner) void bj_inner_kernel(BjParams<T> P, int r) {
-  extern __shared__ double smem_bj[];
-  double *cc = smem_bj, *ss = cc + kJb;
-  double *S = ss + kJb, *Qt = S + kJs * kJsLd;
-  int *pp = reinterpret_cast<int *>(Qt + kJs * kJsLd), *qq = pp + kJb;
-  __shared__ unsigned short tri[kJb * (kJb + 1) / 2];
+  extern __shared__ __align__(16) double smem_bj[];
+  double2 *cs = reinterpret_cast<double2 *>(smem_bj);   // (c, s) of pair k of the round
+  double *S = reinterpret_cast<double *>(cs + kJb), *Qt = S + kJs * kJsLd;
+  int2 *pq = reinterpret_cast<int2 *>(Qt + kJs * kJsLd);   // (p, q) of pair k
   const int p = blockIdx.x, tid = threadIdx.x;
   const double eps = Eps<T>::value, tol = P.tol > 0.0 ? P.tol : 10.0 * eps;
   const double floor_ = eps * eps * __longlong_as_double((long long)P.slot[1]);
@@ -192,14 +191,6 @@ __global__ __launch_bounds__(kJbInner) void bj_inner_kernel(BjParams<T> P, int r
     for (int z = 1; z < P.splits; ++z) acc += sp[(size_t)z * kJs * kJs];
     S[i * kJsLd + j] = acc;
     Qt[i * kJsLd + j] = i == j ? 1.0 : 0.0;
-  }
-  for (int e = tid; e < kJb * (kJb + 1) / 2; e += kJbInner) {
-    int a = (int)((2.0 * kJb + 1.0 - sqrt((2.0 * kJb + 1.0) * (2.0 * kJb + 1.0) - 8.0 * e)) / 2.0);
-    if (a < 0) a = 0;
-    if (a > kJb - 1) a = kJb - 1;
-    while (a > 0 && a * kJb - a * (a - 1) / 2 > e) --a;
-    while (a < kJb - 1 && (a + 1) * kJb - (a + 1) * a / 2 <= e) ++a;
-    tri[e] = (unsigned short)((a << 8) | (a + e - (a * kJb - a * (a - 1) / 2)));
   }
   __syncthreads();
   // a subproblem in which no pair passes the test rotates nowhere in its sweep: skip it
@@ -213,6 +204,18 @@ __global__ __launch_bounds__(kJbInner) void bj_inner_kernel(BjParams<T> P, int r
   }
   live = __syncthreads_or(live);
   int any = 0;
+  // this thread's items of the round, fixed for the launch: upper-triangle blocks (k1 <= k2) of
+  // the 32 x 32 pair grid (528: one or two per thread) and four Qt row pairs at column tid % 64
+  constexpr int NTRI = kJb * (kJb + 1) / 2, NIT = (NTRI + kJbInner - 1) / kJbInner;
+  int it1[NIT], it2[NIT];
+#pragma unroll
+  for (int j = 0; j < NIT; ++j) {
+    const int e = tid + j * kJbInner;
+    it1[j] = it2[j] = -1;
+    if (e < NTRI) tri_decode(e, kJb, it1[j], it2[j]);
+  }
+  constexpr int QIT = kJb * kJs / kJbInner;
+  const int qcol = tid % kJs, qk = tid / kJs;
   for (int sw = 0; live && sw < P.inner_sweeps; ++sw) {
     int act_sw = 0;
     for (int rr = 0; rr < kJs - 1; ++rr) {
@@ -223,28 +226,29 @@ __global__ __launch_bounds__(kJbInner) void bj_inner_kernel(BjParams<T> P, int r
         const bool act = apq * apq > tol * tol * app * aqq && fmin(app, aqq) > floor_;
         double c = 1.0, s = 0.0;
         if (act) jacobi_rotation_fast<T>(app, aqq, apq, c, s);
-        cc[tid] = c;
-        ss[tid] = act ? s : 0.0;
-        pp[tid] = a;
-        qq[tid] = b;
+        cs[tid] = make_double2(c, act ? s : 0.0);
+        pq[tid] = make_int2(a, b);
         act_sw |= act;
       }
       __syncthreads();
-      for (int e = tid; e < kJb * (kJb + 1) / 2; e += kJbInner) {
-        const int k1 = tri[e] >> 8, k2 = tri[e] & 0xff;
-        const double s1 = ss[k1], s2 = ss[k2];
-        if (s1 != 0.0 || s2 != 0.0)
-          block_update(S, kJsLd, pp[k1], qq[k1], pp[k2], qq[k2], cc[k1], s1, cc[k2], s2, s1 != 0.0, s2 != 0.0);
+#pragma unroll
+      for (int j = 0; j < NIT; ++j) {
+        if (it1[j] < 0) continue;
+        const double2 r1 = cs[it1[j]], r2 = cs[it2[j]];
+        if (r1.y == 0.0 && r2.y == 0.0) continue;
+        const int2 a1 = pq[it1[j]], a2 = pq[it2[j]];
+        block_update(S, kJsLd, a1.x, a1.y, a2.x, a2.y, r1.x, r1.y, r2.x, r2.y, r1.y != 0.0, r2.y != 0.0);
       }
-      for (int e = tid; e < kJb * kJs; e += kJbInner) {
-        const int k = e / kJs, j = e - k * kJs;
-        const double s = ss[k];
-        if (s == 0.0) continue;
-        const double c = cc[k];
-        double *vp = Qt + pp[k] * kJsLd + j, *vq = Qt + qq[k] * kJsLd + j;
+#pragma unroll
+      for (int j = 0; j < QIT; ++j) {
+        const int k = qk + j * (kJbInner / kJs);
+        const double2 r = cs[k];
+        if (r.y == 0.0) continue;
+        const int2 a = pq[k];
+        double *vp = Qt + a.x * kJsLd + qcol, *vq = Qt + a.y * kJsLd + qcol;
         const double x = *vp, y = *vq;
-        *vp = c * x - s * y;
-        *vq = s * x + c * y;
+        *vp = r.x * x - r.y * y;
+        *vq = r.y * x + r.x * y;
       }
       __syncthreads();
     }
