@@ -126,6 +126,10 @@ __device__ void jacobi_sweeps(T *W, T *V, int m, int np, int ldw, bool vectors, 
   const int grp = tid / L, lig = tid - grp * L;
   const int npairs = np / 2;
   const bool cached = m <= RC * L;
+  // row slots a lane actually holds (uniform over the workgroup): slots past them are skipped, not
+  // loaded and masked
+  const int rc = (m + L - 1) / L, rcv = (np + L - 1) / L;
+  const bool vcached = np <= RC * L;
   const double tol2 = tol * tol, eps2 = eps * eps;
   int sweeps = 0;
   bool converged = false;
@@ -146,18 +150,22 @@ __device__ void jacobi_sweeps(T *W, T *V, int m, int np, int ldw, bool vectors, 
           // all rows of this lane in registers: unconditional loads (index clamped), masked sums
 #pragma unroll
           for (int j = 0; j < RC; ++j) {
-            const int i = lig + j * L;
-            const int ic = i < m ? i : m - 1;
-            xs[j] = wp[ic];
-            ys[j] = wq[ic];
+            if (j < rc) {
+              const int i = lig + j * L;
+              const int ic = i < m ? i : m - 1;
+              xs[j] = wp[ic];
+              ys[j] = wq[ic];
+            }
           }
 #pragma unroll
           for (int j = 0; j < RC; ++j) {
-            const bool live = lig + j * L < m;
-            const double x = live ? (double)xs[j] : 0.0, y = live ? (double)ys[j] : 0.0;
-            app = fma(x, x, app);
-            aqq = fma(y, y, aqq);
-            apq = fma(x, y, apq);
+            if (j < rc) {
+              const bool live = lig + j * L < m;
+              const double x = live ? (double)xs[j] : 0.0, y = live ? (double)ys[j] : 0.0;
+              app = fma(x, x, app);
+              aqq = fma(y, y, aqq);
+              apq = fma(x, y, apq);
+            }
           }
         } else {
           pair_dots<T, L>(wp, wq, m, lig, app, aqq, apq);
@@ -178,7 +186,7 @@ __device__ void jacobi_sweeps(T *W, T *V, int m, int np, int ldw, bool vectors, 
 #pragma unroll
             for (int j = 0; j < RC; ++j) {
               const int i = lig + j * L;
-              if (i < m) {
+              if (j < rc && i < m) {
                 wp[i] = c * xs[j] - s * ys[j];
                 wq[i] = s * xs[j] + c * ys[j];
               }
@@ -186,7 +194,21 @@ __device__ void jacobi_sweeps(T *W, T *V, int m, int np, int ldw, bool vectors, 
           } else {
             pair_rotate<T, L>(wp, wq, m, lig, c, s);
           }
-          if (vectors) pair_rotate<T, L>(V + p * np, V + q * np, np, lig, c, s);
+          if (vectors) {
+            T *vp = V + p * np, *vq = V + q * np;
+            if (vcached) {
+              for (int j = 0; j < rcv; ++j) {   // uniform trip count: the rows this lane holds
+                const int i = lig + j * L;
+                if (i < np) {
+                  const T x = vp[i], y = vq[i];
+                  vp[i] = c * x - s * y;
+                  vq[i] = s * x + c * y;
+                }
+              }
+            } else {
+              pair_rotate<T, L>(vp, vq, np, lig, c, s);
+            }
+          }
         }
         mymax = fmax(mymax, fmax(nap, naq));
       }
