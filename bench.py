@@ -7,7 +7,7 @@ Workload: 4096 float32 matrices, m ~ U{16..96}, n ~ U{8..m}, entries i.i.d. N(0,
 hipGraph replay of the persistent LDS-resident Jacobi launch (device work queue, FP64 dot
 products and convergence test, FP64 sigma).  Prints one JSON line.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-check] [--only c2]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-check] [--no-plain] [--only c2]
 """
 import argparse
 import importlib
@@ -303,6 +303,8 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-check', action='store_true')
     ap.add_argument('--no-tune', action='store_true', help='use the default launch-shape policy')
+    ap.add_argument('--no-plain', action='store_true', help='skip the plain GPU form (per-matrix calls), e.g. when '
+                    'profiling the batched launch alone')
     ap.add_argument('--only', default='all', choices=['all', 'c2', 'single', 'qdwh', 'tsqr', 'l0', 'consumers'],
                     help="'c2': the batched-Jacobi line alone; single / qdwh / tsqr / l0 / consumers: that leg alone "
                          "(printed as {\"configs\": ...}, for profiling)")
@@ -373,9 +375,10 @@ def main():
     }
     if not args.no_check:
         out['accuracy_vs_fp64_lapack'] = accuracy(batch, flat, ms, ns, offs)
-    plain = plain_gpu_form(pkg, flat, ms, ns, offs, dev)
-    plain['batched_speedup'] = out['value'] / plain['value']
-    out['plain_gpu_form'] = plain
+    if not args.no_plain:
+        plain = plain_gpu_form(pkg, flat, ms, ns, offs, dev)
+        plain['batched_speedup'] = out['value'] / plain['value']
+        out['plain_gpu_form'] = plain
     if not args.no_cpu:
         out['cpu_baseline'] = cpu_baseline(flat, ms, ns, offs)
     if args.only != 'c2':
