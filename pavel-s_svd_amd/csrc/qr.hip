@@ -142,9 +142,12 @@ __global__ __launch_bounds__(kPanelThreads) void qr_panel_kernel(T *F, int64_t l
 // per-workgroup partial sums in FP64 that every workgroup then adds in the same fixed order after
 // a grid barrier (deterministic, no atomics).  Scratch buffers alternate with the column parity so
 // that a fast workgroup never overwrites partials a slow one is still reading.
+// per workgroup and column: kPanel w partials and the five partial sums that carry the next
+// column's norm and alpha through the same grid barrier (kCoopW doubles)
+constexpr int kCoopW = kPanel + 8;
 struct CoopScr {
   int nw;
-  __host__ __device__ static size_t doubles(int nw) { return 2 * ((size_t)nw + 1 + (size_t)nw * kPanel); }
+  __host__ __device__ static size_t doubles(int nw) { return 2 * ((size_t)nw + 1 + (size_t)nw * kCoopW); }
 };
 
 constexpr int kCoopLd = kPanel + 1;
@@ -170,7 +173,9 @@ __global__ __launch_bounds__(256) void qr_panel_coop_kernel(T *F, int64_t ldf, i
   T *Vb = V + (int64_t)b * sv;
   T *tb = tau + (int64_t)b * stau;
   double *sb = scr + (size_t)b * CoopScr::doubles(nw);
-  const size_t half = (size_t)nw + 1 + (size_t)nw * kPanel;
+  const size_t half = (size_t)nw + 1 + (size_t)nw * kCoopW;
+  __shared__ double red5[(NT / kWave) * 8];
+  __shared__ double ext[8];
   // the thread's KR panel rows, private to it (no barrier needed around them)
   T *xs = reinterpret_cast<T *>(smem_coop);
   auto X = [&](int q, int k) -> T & { return xs[(q * NT + tid) * kCoopLd + k]; };
@@ -181,25 +186,37 @@ __global__ __launch_bounds__(256) void qr_panel_coop_kernel(T *F, int64_t ldf, i
 #pragma unroll
     for (int k = 0; k < kPanel; ++k) X(q, k) = (row[q] < m && k < nbc) ? Fb[(int64_t)row[q] * ldf + j0 + k] : T(0);
   }
+  // Column c's norm below the diagonal and alpha normally come through the reduction of column
+  // c - 1 (see below), saving one grid barrier per column; they are reduced directly for the first
+  // column, after a trivial reflector, and when the carried value would have cancelled.
+  bool carried = false;
+  double cx2 = 0.0, cal = 0.0;
   for (int c = 0; c < nbc; ++c) {
     const int j = j0 + c;
     double *np_ = sb + (c & 1) * half;        // nw norm partials, then alpha
     double *al = np_ + nw;
-    double *wp = al + 1;                      // nw x kPanel w partials
-    double part = 0.0;
+    double *wp = al + 1;                      // nw x kCoopW w partials and carried sums
+    double xnorm, alpha;
+    if (carried) {
+      xnorm = sqrt(cx2);
+      alpha = cal;
+    } else {
+      double part = 0.0;
 #pragma unroll
-    for (int q = 0; q < KR; ++q)
-      if (row[q] > j && row[q] < m) part = fma((double)X(q, c), (double)X(q, c), part);
-    part = block_sum_one<NT>(part, red);
-    if (tid == 0) np_[wi] = part;
+      for (int q = 0; q < KR; ++q)
+        if (row[q] > j && row[q] < m) part = fma((double)X(q, c), (double)X(q, c), part);
+      part = block_sum_one<NT>(part, red);
+      if (tid == 0) np_[wi] = part;
 #pragma unroll
-    for (int q = 0; q < KR; ++q)
-      if (row[q] == j) *al = (double)X(q, c);
-    grid.sync();
-    double tot = 0.0;
-    for (int t = tid; t < nw; t += NT) tot += np_[t];
-    const double xnorm = sqrt(block_sum_one<NT>(tot, red));
-    const double alpha = *al;
+      for (int q = 0; q < KR; ++q)
+        if (row[q] == j) *al = (double)X(q, c);
+      grid.sync();
+      double tot = 0.0;
+      for (int t = tid; t < nw; t += NT) tot += np_[t];
+      xnorm = sqrt(block_sum_one<NT>(tot, red));
+      alpha = *al;
+    }
+    carried = false;
     double beta, tj, scal;
     const bool trivial = xnorm == 0.0;
     if (trivial) {
@@ -240,15 +257,50 @@ __global__ __launch_bounds__(256) void qr_panel_coop_kernel(T *F, int64_t ldf, i
 #pragma unroll
           for (int q = 0; q < KR; ++q) w[k] = fma(v[q], (double)X(q, k), w[k]);
       }
-      block_sum_vec<NT>(w, c + 1, nbc, red, wsum);
-      if (tid > c && tid < nbc) wp[(size_t)wi * kPanel + tid] = wsum[tid];
+      // the next column x = X(:, c+1) over rows > j+1 after the update x - tau v w becomes
+      // |x|^2 - 2 tau w x.v + tau^2 w^2 |v|^2: its three sums, and x, v at row j+1 (its alpha)
+      double e[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int q = 0; q < KR; ++q) {
+        const int i = row[q];
+        if (i > j + 1 && i < m) {
+          const double x = (double)X(q, c + 1);
+          e[0] = fma(x, x, e[0]);
+          e[1] = fma(x, v[q], e[1]);
+          e[2] = fma(v[q], v[q], e[2]);
+        } else if (i == j + 1) {
+          e[3] = (double)X(q, c + 1);
+          e[4] = v[q];
+        }
+      }
+      {
+        const int lane = tid & (kWave - 1), wave = tid / kWave;
+#pragma unroll
+        for (int r = 0; r < 5; ++r) {
+          const double t = group_sum_dpp<kWave>(e[r]);
+          if (lane == 0) red5[wave * 8 + r] = t;
+        }
+      }
+      block_sum_vec<NT>(w, c + 1, nbc, red, wsum);   // its barriers also publish red5
+      if (tid > c && tid < nbc) wp[(size_t)wi * kCoopW + tid] = wsum[tid];
+      if (tid < 5) {
+        double t = 0.0;
+#pragma unroll
+        for (int x = 0; x < NT / kWave; ++x) t += red5[x * 8 + tid];
+        wp[(size_t)wi * kCoopW + kPanel + tid] = t;
+      }
       grid.sync();
       {  // thread (k = tid % 32, s = tid / 32) adds the partials of workgroups s, s + 8, ...
         const int k = tid & (kPanel - 1), s = tid / kPanel;
         double acc = 0.0;
         if (k > c && k < nbc)
-          for (int t = s; t < nw; t += 8) acc += wp[(size_t)t * kPanel + k];
+          for (int t = s; t < nw; t += 8) acc += wp[(size_t)t * kCoopW + k];
         seg[s][k] = acc;
+      }
+      if (tid < 5) {   // the carried sums, workgroups in order
+        double acc = 0.0;
+        for (int t = 0; t < nw; ++t) acc += wp[(size_t)t * kCoopW + kPanel + tid];
+        ext[tid] = acc;
       }
       __syncthreads();
       if (tid < kPanel) {
@@ -264,6 +316,13 @@ __global__ __launch_bounds__(256) void qr_panel_coop_kernel(T *F, int64_t ldf, i
 #pragma unroll
         for (int k = 0; k < kPanel; ++k)
           if (k > c && k < nbc) X(q, k) = (T)((double)X(q, k) - tv * wsum[k]);
+      }
+      {  // next column: norm^2 below the diagonal and alpha (rounded as the owner rounds X)
+        const double w1 = wsum[c + 1], s1 = ext[0];
+        const double n2 = s1 - 2.0 * tj * w1 * ext[1] + tj * tj * w1 * w1 * ext[2];
+        cal = (double)(T)(ext[3] - (tj * ext[4]) * w1);
+        cx2 = n2;
+        carried = n2 >= 1e-4 * s1 && n2 <= DBL_MAX;   // no cancellation: else reduce it directly
       }
       __syncthreads();
     }
