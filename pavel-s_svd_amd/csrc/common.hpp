@@ -22,12 +22,14 @@ __device__ __forceinline__ double group_sum(double v, int width) {
   return v;
 }
 
-// DPP lane permutation of a double (two 32-bit moves).  CTRL is a gfx9 DPP control word.
+// DPP lane permutation of a double (two 32-bit moves).  CTRL is a gfx9 DPP control word that
+// gives every lane a source inside its own row (the quad permutations and the row mirrors used
+// here), so no lane keeps an old value and the moves need no copy of their source.
 template <int CTRL>
 __device__ __forceinline__ double dpp_double(double v) {
   const int lo = __double2loint(v), hi = __double2hiint(v);
-  return __hiloint2double(__builtin_amdgcn_update_dpp(hi, hi, CTRL, 0xF, 0xF, false),
-                          __builtin_amdgcn_update_dpp(lo, lo, CTRL, 0xF, 0xF, false));
+  return __hiloint2double(__builtin_amdgcn_mov_dpp(hi, CTRL, 0xF, 0xF, false),
+                          __builtin_amdgcn_mov_dpp(lo, CTRL, 0xF, 0xF, false));
 }
 
 __device__ __forceinline__ double readlane_double(double v, int lane) {
@@ -54,6 +56,27 @@ __device__ __forceinline__ double group_sum_dpp(double v) {
   }
   return v;
 }
+
+// group_sum_dpp of three values with their levels interleaved, so that the DPP moves of one
+// value fill the other two's wait states; each result has the bits group_sum_dpp gives it.
+template <int L>
+__device__ __forceinline__ void group_sum3_dpp(double &a, double &b, double &c) {
+  if constexpr (L > 16) {
+    a = group_sum_dpp<L>(a);
+    b = group_sum_dpp<L>(b);
+    c = group_sum_dpp<L>(c);
+  } else {
+    if constexpr (L >= 2) { a += dpp_double<0xB1>(a); b += dpp_double<0xB1>(b); c += dpp_double<0xB1>(c); }
+    if constexpr (L >= 4) { a += dpp_double<0x4E>(a); b += dpp_double<0x4E>(b); c += dpp_double<0x4E>(c); }
+    if constexpr (L >= 8) { a += dpp_double<0x141>(a); b += dpp_double<0x141>(b); c += dpp_double<0x141>(c); }
+    if constexpr (L >= 16) { a += dpp_double<0x140>(a); b += dpp_double<0x140>(b); c += dpp_double<0x140>(c); }
+  }
+}
+
+// A value every lane of the wave holds (read back from one LDS word, say) moved to a scalar
+// register: the compiler then treats what is derived from it as uniform, so branches on it are
+// scalar branches and loop trip counts need no per-lane masks or selects.
+__device__ __forceinline__ int wave_uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
 // Maximum over the whole wave (DPP within rows, readlane across rows); wave-uniform result.
 __device__ __forceinline__ double wave_max_fast(double v) {

@@ -34,13 +34,17 @@ struct LdsParams {
 };
 
 __host__ __device__ inline int pad_even(int n) { return n + (n & 1); }
-__host__ __device__ inline int lds_ld(int m) { return m | 1; }
+// Column stride of W (m rows) and V (np rows) in LDS: the row count rounded up to a multiple of
+// 16, plus one so that the stride is odd (columns of one round start in different banks).  Rows
+// past the matrix's own are zero and stay zero under every rotation, so a group of L <= 16 lanes
+// can stride over round_up(rows, L) rows with no bounds test.
+__host__ __device__ inline int lds_ld(int rows) { return ((rows + 15) & ~15) | 1; }
 
 template <typename T>
 __host__ __device__ inline size_t lds_bytes(int m, int n, bool vectors) {
-  const int np = pad_even(n), ldw = lds_ld(m);
+  const int np = pad_even(n), ldw = lds_ld(m), ldv = lds_ld(np);
   size_t b = sizeof(double) * (size_t)(3 * (np / 2) + 2 * np + m);
-  b += sizeof(T) * ((size_t)np * ldw + (vectors ? (size_t)np * np : 0));
+  b += sizeof(T) * ((size_t)np * ldw + (vectors ? (size_t)np * ldv : 0));
   b += sizeof(int) * (size_t)np;
   return (b + 15) & ~size_t(15);
 }
@@ -118,7 +122,7 @@ __device__ __forceinline__ void pair_rotate(T *xp, T *xq, int rows, int lig, T c
 template <typename T> struct CachedRows { static constexpr int value = sizeof(T) == 4 ? 8 : 4; };
 
 template <typename T, int NT, int L>
-__device__ void jacobi_sweeps(T *W, T *V, int m, int np, int ldw, bool vectors, double tol, double eps,
+__device__ void jacobi_sweeps(T *W, T *V, int m, int np, int ldw, int ldv, bool vectors, double tol, double eps,
                               int max_sweeps, double *rslot, int *fslot, int &sweeps_out, bool &conv_out) {
   constexpr int NG = NT / L, NW = NT / kWave;
   constexpr int RC = CachedRows<T>::value;       // rows per lane kept in registers between phases
@@ -195,7 +199,7 @@ __device__ void jacobi_sweeps(T *W, T *V, int m, int np, int ldw, bool vectors, 
             pair_rotate<T, L>(wp, wq, m, lig, c, s);
           }
           if (vectors) {
-            T *vp = V + p * np, *vq = V + q * np;
+            T *vp = V + p * ldv, *vq = V + q * ldv;
             if (vcached) {
               for (int j = 0; j < rcv; ++j) {   // uniform trip count: the rows this lane holds
                 const int i = lig + j * L;
@@ -223,13 +227,121 @@ __device__ void jacobi_sweeps(T *W, T *V, int m, int np, int ldw, bool vectors, 
     int any = 0;
 #pragma unroll
     for (int w = 0; w < NW; ++w) any |= fslot[(sw & 1) * NW + w];
-    if (!any) {
+    if (!wave_uniform(any)) {
       converged = true;
       break;
     }
   }
   sweeps_out = sweeps;
   conv_out = converged;
+}
+
+// The same sweeps for L <= 16 lanes per pair and exactly S row slots per lane (S = ceil(m / L)):
+// with the zero-padded columns every slot is a real or a zero row, so the loads, the FP64 sums,
+// the rotation and the stores run unconditionally on registers (no clamped indices, no masks and
+// no selects), and the V rotation strides over round_up(np, L) rows.  The sums are formed in the
+// same order as in jacobi_sweeps, so both give the same bits.
+template <typename T, int NT, int L, int S>
+__device__ void jacobi_sweeps_fixed(T *W, T *V, int np, int ldw, int ldv, bool vectors, double tol, double eps,
+                                    int max_sweeps, double *rslot, int *fslot, int &sweeps_out, bool &conv_out) {
+  static_assert(L <= 16, "the zero padding covers groups of at most 16 lanes");
+  constexpr int NG = NT / L, NW = NT / kWave;
+  const int tid = threadIdx.x, wave = tid / kWave, lane = tid & (kWave - 1);
+  const int grp = tid / L, lig = tid - grp * L;
+  const int npairs = np / 2;
+  const int rcv = (np + L - 1) / L;
+  const double tol2 = tol * tol, eps2 = eps * eps;
+  int sweeps = 0;
+  bool converged = false;
+  int g = 0;
+  for (int sw = 1; sw <= max_sweeps; ++sw) {
+    sweeps = sw;
+    bool wave_active = false;
+    for (int r = 0; r < np - 1; ++r, ++g) {
+      const double floor_ = eps2 * slots_max<NW>(rslot + (g & 1) * NW);
+      double mymax = 0.0;
+      for (int k = grp; k < npairs; k += NG) {
+        int p, q;
+        rr_pair(r, k, np, p, q);
+        T *wp = W + p * ldw + lig, *wq = W + q * ldw + lig;
+        T xs[S], ys[S];
+#pragma unroll
+        for (int j = 0; j < S; ++j) {
+          xs[j] = wp[j * L];
+          ys[j] = wq[j * L];
+        }
+        double app = 0.0, aqq = 0.0, apq = 0.0;
+#pragma unroll
+        for (int j = 0; j < S; ++j) {
+          const double x = (double)xs[j], y = (double)ys[j];
+          app = fma(x, x, app);
+          aqq = fma(y, y, aqq);
+          apq = fma(x, y, apq);
+        }
+        group_sum3_dpp<L>(app, aqq, apq);
+        double nap = app, naq = aqq;
+        // active: |apq| / (|w_p| |w_q|) > tol and min(app, aqq) > floor (primitives.py:335-338)
+        if (apq * apq > tol2 * app * aqq && fmin(app, aqq) > floor_) {
+          T c, s;
+          double tb;
+          jacobi_rotation_t<T>(app, aqq, apq, c, s, tb);
+          nap = app - tb;
+          naq = aqq + tb;
+          wave_active = true;
+#pragma unroll
+          for (int j = 0; j < S; ++j) {
+            wp[j * L] = c * xs[j] - s * ys[j];
+            wq[j * L] = s * xs[j] + c * ys[j];
+          }
+          if (vectors) {
+            T *vp = V + p * ldv + lig, *vq = V + q * ldv + lig;
+            for (int j = 0; j < rcv; ++j) {
+              const T x = vp[j * L], y = vq[j * L];
+              vp[j * L] = c * x - s * y;
+              vq[j * L] = s * x + c * y;
+            }
+          }
+        }
+        mymax = fmax(mymax, fmax(nap, naq));
+      }
+      mymax = wave_max_fast(mymax);
+      if (lane == 0) rslot[((g + 1) & 1) * NW + wave] = mymax;
+      __syncthreads();
+    }
+    // convergence: no wave had an active pair in this sweep
+    const int wave_any = __any(wave_active) ? 1 : 0;
+    if (lane == 0) fslot[(sw & 1) * NW + wave] = wave_any;
+    __syncthreads();
+    int any = 0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) any |= fslot[(sw & 1) * NW + w];
+    if (!wave_uniform(any)) {
+      converged = true;
+      break;
+    }
+  }
+  sweeps_out = sweeps;
+  conv_out = converged;
+}
+
+// Picks the fixed-slot sweeps when the lane count and the matrix's rows allow it, else the
+// general ones.  m is uniform over the workgroup, so the switch is a scalar branch.
+template <typename T, int NT, int L>
+__device__ void run_sweeps(T *W, T *V, int m, int np, int ldw, int ldv, bool vectors, double tol, double eps,
+                           int max_sweeps, double *rslot, int *fslot, int &sweeps_out, bool &conv_out) {
+  if constexpr (L <= 16) {
+    constexpr int RC = CachedRows<T>::value;
+    const int rc = (m + L - 1) / L;
+#define PSVD_FIXED_CASE(SV)                                                                                  \
+  if constexpr (SV <= RC)                                                                                    \
+    if (rc == SV)                                                                                            \
+      return jacobi_sweeps_fixed<T, NT, L, SV>(W, V, np, ldw, ldv, vectors, tol, eps, max_sweeps, rslot, fslot, \
+                                               sweeps_out, conv_out);
+    PSVD_FIXED_CASE(1) PSVD_FIXED_CASE(2) PSVD_FIXED_CASE(3) PSVD_FIXED_CASE(4)
+    PSVD_FIXED_CASE(5) PSVD_FIXED_CASE(6) PSVD_FIXED_CASE(7) PSVD_FIXED_CASE(8)
+#undef PSVD_FIXED_CASE
+  }
+  jacobi_sweeps<T, NT, L>(W, V, m, np, ldw, ldv, vectors, tol, eps, max_sweeps, rslot, fslot, sweeps_out, conv_out);
 }
 
 // Waves per SIMD the register budget is sized for (128 VGPRs: four waves per SIMD).
@@ -256,7 +368,7 @@ void jacobi_lds_kernel(LdsParams<T> P) {
   for (;;) {
     if (tid == 0) s_idx = atomicAdd(P.queue, 1);
     __syncthreads();
-    const int64_t idx = s_idx;
+    const int64_t idx = wave_uniform(s_idx);
     __syncthreads();
     if (idx >= P.batch) return;
     const int64_t mi = P.order ? (int64_t)P.order[idx] : idx;
@@ -282,14 +394,14 @@ void jacobi_lds_kernel(LdsParams<T> P) {
       }
       continue;  // uniform across the block: every thread read the same descriptor
     }
-    const int np = pad_even(n), ldw = lds_ld(m), npairs = np / 2;
+    const int np = pad_even(n), ldw = lds_ld(m), ldv = lds_ld(np), npairs = np / 2;
     double *dots = smem_d;
     double *sig = dots + 3 * npairs;
     double *ytmp = sig + np;
     double *vt = ytmp + np;
     T *W = reinterpret_cast<T *>(vt + m);
     T *V = W + (size_t)np * ldw;
-    int *ord = reinterpret_cast<int *>(V + (vectors ? (size_t)np * np : 0));
+    int *ord = reinterpret_cast<int *>(V + (vectors ? (size_t)np * ldv : 0));
 
     // ---- load A (row-major) into W (column-major), check finiteness, optional max-abs scale
     const T *A = P.a + aoff;
@@ -301,10 +413,20 @@ void jacobi_lds_kernel(LdsParams<T> P) {
       const double ax = fabs((double)x);
       if (ax <= DBL_MAX) amax = fmax(amax, ax); else bad = 1.0;
     }
+    {  // zero rows past m in every column, and the padding column when n is odd
+      const int padr = ldw - m;
+      for (int e = tid; e < np * padr; e += NT) {
+        const int j = e / padr;
+        W[j * ldw + m + (e - j * padr)] = T(0);
+      }
+    }
     if (np > n)
       for (int i = tid; i < m; i += NT) W[n * ldw + i] = T(0);
     if (vectors)
-      for (int e = tid; e < np * np; e += NT) V[e] = (e / np == e % np) ? T(1) : T(0);
+      for (int e = tid; e < np * ldv; e += NT) {
+        const int j = e / ldv;
+        V[e] = (e - j * ldv == j) ? T(1) : T(0);
+      }
     if (block_max<NT>(bad, s_red) > 0.0) {
       if (tid == 0) {
         P.info[2 * mi] = 0;
@@ -345,7 +467,7 @@ void jacobi_lds_kernel(LdsParams<T> P) {
     // ---- sweeps
     int sweeps = 0;
     bool converged = false;
-    jacobi_sweeps<T, NT, L>(W, V, m, np, ldw, vectors, tol, eps, P.max_sweeps, s_rslot, s_fslot, sweeps, converged);
+    run_sweeps<T, NT, L>(W, V, m, np, ldw, ldv, vectors, tol, eps, P.max_sweeps, s_rslot, s_fslot, sweeps, converged);
     int status = converged ? PSVD_INFO_OK : PSVD_INFO_NOCONV;
 
     // ---- sigma in FP64, descending order, cut
@@ -378,7 +500,7 @@ void jacobi_lds_kernel(LdsParams<T> P) {
       s_nk = nk;
     }
     __syncthreads();
-    const int nk = s_nk;
+    const int nk = wave_uniform(s_nk);
 
     // ---- U = W / sigma for kept columns; coordinate-vector completion for the rest
     if (vectors) {
@@ -469,7 +591,7 @@ void jacobi_lds_kernel(LdsParams<T> P) {
       T *VH = P.vh + voff;
       for (int e = tid; e < n * n; e += NT) {
         const int r = e / n, j = e - r * n;
-        VH[e] = V[ord[r] * np + j];
+        VH[e] = V[ord[r] * ldv + j];
       }
     }
     if (tid == 0) {
