@@ -16,7 +16,7 @@ import sys
 
 import torch
 
-CANDIDATES = [dict(target_waves=w, min_rows=r, passes=p, streams=4) for w in (8, 16, 32) for r in (2, 4, 6)
+CANDIDATES = [dict(target_waves=w, min_rows=r, passes=p, streams=4) for w in (8, 16, 32) for r in (2, 4, 6, 8)
               for p in (1, 2)]
 _CACHE = {}
 

@@ -330,7 +330,7 @@ template <typename T, int NT, int L>
 __device__ void run_sweeps(T *W, T *V, int m, int np, int ldw, int ldv, bool vectors, double tol, double eps,
                            int max_sweeps, double *rslot, int *fslot, int &sweeps_out, bool &conv_out) {
   if constexpr (L <= 16) {
-    constexpr int RC = CachedRows<T>::value;
+    constexpr int RC = sizeof(T) == 4 ? 12 : 6;   // row slots a lane can keep in registers
     const int rc = (m + L - 1) / L;
 #define PSVD_FIXED_CASE(SV)                                                                                  \
   if constexpr (SV <= RC)                                                                                    \
@@ -339,6 +339,7 @@ __device__ void run_sweeps(T *W, T *V, int m, int np, int ldw, int ldv, bool vec
                                                sweeps_out, conv_out);
     PSVD_FIXED_CASE(1) PSVD_FIXED_CASE(2) PSVD_FIXED_CASE(3) PSVD_FIXED_CASE(4)
     PSVD_FIXED_CASE(5) PSVD_FIXED_CASE(6) PSVD_FIXED_CASE(7) PSVD_FIXED_CASE(8)
+    PSVD_FIXED_CASE(9) PSVD_FIXED_CASE(10) PSVD_FIXED_CASE(11) PSVD_FIXED_CASE(12)
 #undef PSVD_FIXED_CASE
   }
   jacobi_sweeps<T, NT, L>(W, V, m, np, ldw, ldv, vectors, tol, eps, max_sweeps, rslot, fslot, sweeps_out, conv_out);
