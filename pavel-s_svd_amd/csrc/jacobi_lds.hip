@@ -263,7 +263,8 @@ __device__ void jacobi_sweeps_fixed(T *W, T *V, int np, int ldw, int ldv, bool v
       for (int k = grp; k < npairs; k += NG) {
         int p, q;
         rr_pair(r, k, np, p, q);
-        T *wp = W + p * ldw + lig, *wq = W + q * ldw + lig;
+        // column offsets stay far below 2^24: 24-bit multiplies (full rate, unlike v_mul_lo_u32)
+        T *wp = W + __umul24(p, ldw) + lig, *wq = W + __umul24(q, ldw) + lig;
         T xs[S], ys[S];
 #pragma unroll
         for (int j = 0; j < S; ++j) {
@@ -294,7 +295,7 @@ __device__ void jacobi_sweeps_fixed(T *W, T *V, int np, int ldw, int ldv, bool v
             wq[j * L] = s * xs[j] + c * ys[j];
           }
           if (vectors) {
-            T *vp = V + p * ldv + lig, *vq = V + q * ldv + lig;
+            T *vp = V + __umul24(p, ldv) + lig, *vq = V + __umul24(q, ldv) + lig;
             for (int j = 0; j < rcv; ++j) {
               const T x = vp[j * L], y = vq[j * L];
               vp[j * L] = c * x - s * y;
@@ -330,7 +331,7 @@ template <typename T, int NT, int L>
 __device__ void run_sweeps(T *W, T *V, int m, int np, int ldw, int ldv, bool vectors, double tol, double eps,
                            int max_sweeps, double *rslot, int *fslot, int &sweeps_out, bool &conv_out) {
   if constexpr (L <= 16) {
-    constexpr int RC = sizeof(T) == 4 ? 12 : 6;   // row slots a lane can keep in registers
+    constexpr int RC = sizeof(T) == 4 ? 12 : 4;   // row slots a lane can keep in registers
     const int rc = (m + L - 1) / L;
 #define PSVD_FIXED_CASE(SV)                                                                                  \
   if constexpr (SV <= RC)                                                                                    \
