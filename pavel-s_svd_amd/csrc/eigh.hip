@@ -625,13 +625,25 @@ __host__ __device__ inline size_t blk_update_lds() {
   return sizeof(T) * 3 * (size_t)kEs * kEsLd;
 }
 
+// 16-byte vector of T for the tile loads and stores below: every tile row is made of 32-element
+// segments that start on a 128-byte boundary (N and the block offsets are multiples of 32
+// elements, the buffers 256-byte aligned), so a vector never straddles two segments.
+template <typename T>
+struct alignas(16) Vec16 {
+  static constexpr int n = 16 / (int)sizeof(T);
+  T v[16 / sizeof(T)];
+};
+
 // Blocks [0, nG): G block (p, q), p < q, and its mirror.  Blocks [nG, nG + npair * N / 64): the
-// 64 x 64 tile (rows of pair p, column chunk c) of V^T.
+// 64 x 64 tile (rows of pair p, column chunk c) of V^T.  Global loads and stores move 16 bytes per
+// lane; LDS tiles keep the odd stride kEsLd of the MFMA products.
 template <typename T>
 __global__ __launch_bounds__(256) void blk_update_kernel(BlkParams<T> P, int r, int g) {
   extern __shared__ double smem_bu[];
   T *Bs = reinterpret_cast<T *>(smem_bu);
   T *Qa = Bs + kEs * kEsLd, *Qb = Qa + kEs * kEsLd;
+  using V16 = Vec16<T>;
+  constexpr int VE = V16::n, RV = kEs / VE;   // elements per vector, vectors per tile row
   const int npair = P.nb / 2;
   const int64_t nG = (int64_t)npair * (npair - 1) / 2;
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave, lr = lane & 15, lk = lane >> 4;
@@ -654,11 +666,23 @@ __global__ __launch_bounds__(256) void blk_update_kernel(BlkParams<T> P, int r, 
       }
       return;
     }
-    for (int e = tid; e < kEs * kEs; e += 256) {
-      const int i = e / kEs, j = e - i * kEs;
-      Bs[i * kEsLd + j] = P.G[(int64_t)blk_row(Ip, Jp, i) * P.N + blk_row(Iq, Jq, j)];
-      if (ap) Qa[i * kEsLd + j] = P.Q[(int64_t)p * kEs * kEs + e];
-      if (aq) Qb[i * kEsLd + j] = P.Q[(int64_t)q * kEs * kEs + e];
+    const V16 *Qp = reinterpret_cast<const V16 *>(P.Q + (int64_t)p * kEs * kEs);
+    const V16 *Qq = reinterpret_cast<const V16 *>(P.Q + (int64_t)q * kEs * kEs);
+    for (int e = tid; e < kEs * RV; e += 256) {
+      const int i = e / RV, j = (e - i * RV) * VE;
+      const V16 x = *reinterpret_cast<const V16 *>(P.G + (int64_t)blk_row(Ip, Jp, i) * P.N + blk_row(Iq, Jq, j));
+#pragma unroll
+      for (int u = 0; u < VE; ++u) Bs[i * kEsLd + j + u] = x.v[u];
+      if (ap) {
+        const V16 y = Qp[e];
+#pragma unroll
+        for (int u = 0; u < VE; ++u) Qa[i * kEsLd + j + u] = y.v[u];
+      }
+      if (aq) {
+        const V16 y = Qq[e];
+#pragma unroll
+        for (int u = 0; u < VE; ++u) Qb[i * kEsLd + j + u] = y.v[u];
+      }
     }
     __syncthreads();
     if (ap) {  // B <- Q_p^T B
@@ -673,10 +697,16 @@ __global__ __launch_bounds__(256) void blk_update_kernel(BlkParams<T> P, int r, 
       store64<T>(Bs, acc, wm, wn, lr, lk);
       __syncthreads();
     }
-    for (int e = tid; e < kEs * kEs; e += 256) {
-      const int i = e / kEs, j = e - i * kEs;
-      P.G[(int64_t)blk_row(Ip, Jp, i) * P.N + blk_row(Iq, Jq, j)] = Bs[i * kEsLd + j];
-      P.G[(int64_t)blk_row(Iq, Jq, i) * P.N + blk_row(Ip, Jp, j)] = Bs[j * kEsLd + i];
+    for (int e = tid; e < kEs * RV; e += 256) {
+      const int i = e / RV, j = (e - i * RV) * VE;
+      V16 x, y;
+#pragma unroll
+      for (int u = 0; u < VE; ++u) {
+        x.v[u] = Bs[i * kEsLd + j + u];
+        y.v[u] = Bs[(j + u) * kEsLd + i];
+      }
+      *reinterpret_cast<V16 *>(P.G + (int64_t)blk_row(Ip, Jp, i) * P.N + blk_row(Iq, Jq, j)) = x;
+      *reinterpret_cast<V16 *>(P.G + (int64_t)blk_row(Iq, Jq, i) * P.N + blk_row(Ip, Jp, j)) = y;
     }
     double mx = 0.0;
     for (int e = lane; e < 32 * 32; e += kWave) mx = fmax(mx, fabs((double)Bs[(wm + (e >> 5)) * kEsLd + wn + (e & 31)]));
@@ -693,20 +723,29 @@ __global__ __launch_bounds__(256) void blk_update_kernel(BlkParams<T> P, int r, 
     if (!P.act[p]) return;
     int Ip, Jp;
     rr_pair(r, p, P.nb, Ip, Jp);
-    for (int e = tid; e < kEs * kEs; e += 256) {
-      const int i = e / kEs, j = e - i * kEs;
-      Qa[i * kEsLd + j] = P.Q[(int64_t)p * kEs * kEs + e];
-      Bs[i * kEsLd + j] = P.Vt[(int64_t)blk_row(Ip, Jp, i) * P.N + c0 + j];
+    const V16 *Qp = reinterpret_cast<const V16 *>(P.Q + (int64_t)p * kEs * kEs);
+    for (int e = tid; e < kEs * RV; e += 256) {
+      const int i = e / RV, j = (e - i * RV) * VE;
+      const V16 x = *reinterpret_cast<const V16 *>(P.Vt + (int64_t)blk_row(Ip, Jp, i) * P.N + c0 + j);
+      const V16 y = Qp[e];
+#pragma unroll
+      for (int u = 0; u < VE; ++u) {
+        Bs[i * kEsLd + j + u] = x.v[u];
+        Qa[i * kEsLd + j + u] = y.v[u];
+      }
     }
     __syncthreads();
     mm64<T, false>(Qa, Bs, acc, wm, wn, lr, lk);
+    __syncthreads();
+    store64<T>(Bs, acc, wm, wn, lr, lk);
+    __syncthreads();
+    for (int e = tid; e < kEs * RV; e += 256) {
+      const int i = e / RV, j = (e - i * RV) * VE;
+      V16 x;
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-      for (int j = 0; j < 2; ++j)
-#pragma unroll
-        for (int rg = 0; rg < 4; ++rg)
-          P.Vt[(int64_t)blk_row(Ip, Jp, wm + 16 * i + Mfma<T>::row(rg, lk)) * P.N + c0 + wn + 16 * j + lr] = (T)acc[i][j][rg];
+      for (int u = 0; u < VE; ++u) x.v[u] = Bs[i * kEsLd + j + u];
+      *reinterpret_cast<V16 *>(P.Vt + (int64_t)blk_row(Ip, Jp, i) * P.N + c0 + j) = x;
+    }
   }
 }
 
