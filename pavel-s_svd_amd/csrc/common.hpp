@@ -73,6 +73,14 @@ __device__ __forceinline__ void group_sum3_dpp(double &a, double &b, double &c) 
   }
 }
 
+// 16 bytes of T moved by one global load or store (global_load_dwordx4); the address must be
+// 16-byte aligned.
+template <typename T>
+struct alignas(16) Vec16 {
+  static constexpr int n = 16 / (int)sizeof(T);
+  T v[16 / sizeof(T)];
+};
+
 // A value every lane of the wave holds (read back from one LDS word, say) moved to a scalar
 // register: the compiler then treats what is derived from it as uniform, so branches on it are
 // scalar branches and loop trip counts need no per-lane masks or selects.

@@ -625,18 +625,11 @@ __host__ __device__ inline size_t blk_update_lds() {
   return sizeof(T) * 3 * (size_t)kEs * kEsLd;
 }
 
-// 16-byte vector of T for the tile loads and stores below: every tile row is made of 32-element
-// segments that start on a 128-byte boundary (N and the block offsets are multiples of 32
-// elements, the buffers 256-byte aligned), so a vector never straddles two segments.
-template <typename T>
-struct alignas(16) Vec16 {
-  static constexpr int n = 16 / (int)sizeof(T);
-  T v[16 / sizeof(T)];
-};
-
 // Blocks [0, nG): G block (p, q), p < q, and its mirror.  Blocks [nG, nG + npair * N / 64): the
 // 64 x 64 tile (rows of pair p, column chunk c) of V^T.  Global loads and stores move 16 bytes per
-// lane; LDS tiles keep the odd stride kEsLd of the MFMA products.
+// lane (every tile row is made of 32-element segments that start on a 128-byte boundary: N and
+// the block offsets are multiples of 32 elements, the buffers 256-byte aligned); LDS tiles keep
+// the odd stride kEsLd of the MFMA products.
 template <typename T>
 __global__ __launch_bounds__(256) void blk_update_kernel(BlkParams<T> P, int r, int g) {
   extern __shared__ double smem_bu[];

@@ -137,11 +137,20 @@ __global__ __launch_bounds__(kJbThreads) void bj_gram_kernel(BjParams<T> P, int 
     for (int j = 0; j < 2; ++j)
 #pragma unroll
       for (int q = 0; q < 4; ++q) acc[i][j][q] = 0.0;
+  using V16 = Vec16<T>;
+  constexpr int VE = V16::n, RV = kJs / VE;   // 16-byte vectors: a row's two 32-column blocks
   for (int k0 = r0; k0 < r1; k0 += KR) {
     __syncthreads();
-    for (int e = tid; e < KR * kJs; e += kJbThreads) {
-      const int rr = e / kJs, c = e - rr * kJs, row = k0 + rr;
-      Ws[rr][c] = row < r1 ? (double)P.W[(int64_t)row * P.N + bj_col(I, J, c)] : 0.0;
+    for (int e = tid; e < KR * RV; e += kJbThreads) {
+      const int rr = e / RV, c = (e - rr * RV) * VE, row = k0 + rr;
+      if (row < r1) {
+        const V16 x = *reinterpret_cast<const V16 *>(P.W + (int64_t)row * P.N + bj_col(I, J, c));
+#pragma unroll
+        for (int u = 0; u < VE; ++u) Ws[rr][c + u] = (double)x.v[u];
+      } else {
+#pragma unroll
+        for (int u = 0; u < VE; ++u) Ws[rr][c + u] = 0.0;
+      }
     }
     __syncthreads();
 #pragma unroll 4
@@ -298,10 +307,24 @@ __global__ __launch_bounds__(kJbThreads) void bj_update_kernel(BjParams<T> P, in
   const int rows = t < mt ? P.m : P.N, r0 = (t < mt ? t : t - mt) * kJs;
   int I, J;
   rr_pair(r, p, P.nb, I, J);
-  for (int e = tid; e < kJs * kJs; e += kJbThreads) {
-    const int i = e / kJs, j = e - i * kJs, row = r0 + i;
-    Xs[i][j] = row < rows ? X[(int64_t)row * P.N + bj_col(I, J, j)] : T(0);
-    Qs[i][j] = P.Q[(size_t)p * kJs * kJs + e];
+  using V16 = Vec16<T>;
+  constexpr int VE = V16::n, RV = kJs / VE;   // 16-byte vectors: a row's two 32-column blocks
+  const V16 *Qp = reinterpret_cast<const V16 *>(P.Q + (size_t)p * kJs * kJs);
+  for (int e = tid; e < kJs * RV; e += kJbThreads) {
+    const int i = e / RV, j = (e - i * RV) * VE, row = r0 + i;
+    V16 x;
+    if (row < rows) {
+      x = *reinterpret_cast<const V16 *>(X + (int64_t)row * P.N + bj_col(I, J, j));
+    } else {
+#pragma unroll
+      for (int u = 0; u < VE; ++u) x.v[u] = T(0);
+    }
+    const V16 y = Qp[e];
+#pragma unroll
+    for (int u = 0; u < VE; ++u) {
+      Xs[i][j + u] = x.v[u];
+      Qs[i][j + u] = y.v[u];
+    }
   }
   __syncthreads();
   typename BjMfma<T>::V acc[2][2];
@@ -323,15 +346,23 @@ __global__ __launch_bounds__(kJbThreads) void bj_update_kernel(BjParams<T> P, in
 #pragma unroll
       for (int j = 0; j < 2; ++j) acc[i][j] = BjMfma<T>::op(a[i], b[j], acc[i][j]);
   }
+  __syncthreads();   // every wave has read Xs
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j)
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int row = r0 + wm + 16 * i + BjMfma<T>::row(q, lk);
-        if (row < rows) X[(int64_t)row * P.N + bj_col(I, J, wn + 16 * j + lr)] = (T)acc[i][j][q];
-      }
+      for (int q = 0; q < 4; ++q) Xs[wm + 16 * i + BjMfma<T>::row(q, lk)][wn + 16 * j + lr] = (T)acc[i][j][q];
+  __syncthreads();
+  for (int e = tid; e < kJs * RV; e += kJbThreads) {
+    const int i = e / RV, j = (e - i * RV) * VE, row = r0 + i;
+    if (row < rows) {
+      V16 x;
+#pragma unroll
+      for (int u = 0; u < VE; ++u) x.v[u] = Xs[i][j + u];
+      *reinterpret_cast<V16 *>(X + (int64_t)row * P.N + bj_col(I, J, j)) = x;
+    }
+  }
 }
 
 template <typename T>
