@@ -295,11 +295,26 @@ __device__ void jacobi_sweeps_fixed(T *W, T *V, int np, int ldw, int ldv, bool v
             wq[j * L] = s * xs[j] + c * ys[j];
           }
           if (vectors) {
+            // four slots per step, loads before stores, so one LDS round trip serves four rows
             T *vp = V + __umul24(p, ldv) + lig, *vq = V + __umul24(q, ldv) + lig;
-            for (int j = 0; j < rcv; ++j) {
-              const T x = vp[j * L], y = vq[j * L];
-              vp[j * L] = c * x - s * y;
-              vq[j * L] = s * x + c * y;
+            int j = 0;
+            for (; j + 4 <= rcv; j += 4, vp += 4 * L, vq += 4 * L) {
+              T x[4], y[4];
+#pragma unroll
+              for (int u = 0; u < 4; ++u) {
+                x[u] = vp[u * L];
+                y[u] = vq[u * L];
+              }
+#pragma unroll
+              for (int u = 0; u < 4; ++u) {
+                vp[u * L] = c * x[u] - s * y[u];
+                vq[u * L] = s * x[u] + c * y[u];
+              }
+            }
+            for (; j < rcv; ++j, vp += L, vq += L) {
+              const T x = vp[0], y = vq[0];
+              vp[0] = c * x - s * y;
+              vq[0] = s * x + c * y;
             }
           }
         }
