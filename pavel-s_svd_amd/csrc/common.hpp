@@ -237,17 +237,14 @@ __device__ __forceinline__ void jacobi_rotation_t<float>(double app, double aqq,
   // one Newton step) and rounded once, so that c^2 + s^2 = 1 to an ulp and V stays orthogonal
   // over thousands of rotations.
   // native v_rcp_f32 / v_sqrt_f32 (about 1 ulp): t to a few ulps is all the angle needs
+  // Branch-free: at tau = 0 (never -0: app, aqq are sums of squares) the general formula gives
+  // rcp(0 + sqrt(1)) = 1 exactly, the reference's t = 1.
   const float fa = (float)fabs(apq);
   const float tau = (float)(aqq - app) * __builtin_amdgcn_rcpf(2.0f * fa);
-  float t;
-  if (tau == 0.0f) {
-    t = 1.0f;
-  } else {
-    const float at = fabsf(tau);
-    t = at > 1e18f ? 0.5f * __builtin_amdgcn_rcpf(at)
-                   : __builtin_amdgcn_rcpf(at + __builtin_amdgcn_sqrtf(fmaf(at, at, 1.0f)));
-    t = copysignf(t, tau);
-  }
+  const float at = fabsf(tau);
+  float t = at > 1e18f ? 0.5f * __builtin_amdgcn_rcpf(at)
+                       : __builtin_amdgcn_rcpf(at + __builtin_amdgcn_sqrtf(fmaf(at, at, 1.0f)));
+  t = copysignf(t, tau);
   const double td = (double)t;
   const double x = fma(td, td, 1.0);
   double y = __builtin_amdgcn_rsq(x);
