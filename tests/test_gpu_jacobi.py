@@ -74,6 +74,34 @@ def test_shapes_odd_and_degenerate(pkg, m, n):
     assert met['orthogonality_v_fro_over_sqrt_rows'] < 1e-5
 
 
+# Launch shapes across the fixed-slot loop (L <= 16 lanes and ceil(m / L) <= 12 row slots per
+# lane for FP32, <= 4 for FP64) and the general loop (L = 32, or more slots): row counts at, one
+# below and one above a multiple of 16 (the padded column stride), n odd and n = m odd (np > m).
+@pytest.mark.parametrize('dtype', ['float32', 'float64'])
+@pytest.mark.parametrize('m,n', [(16, 16), (17, 9), (31, 31), (48, 47), (63, 20), (95, 95), (96, 64), (97, 33)])
+def test_launch_shapes_fixed_and_general_loops(m, n, dtype):
+    import importlib
+    P = importlib.import_module('pavel-s_svd_amd.standalone.primitives')
+    a = np.stack([X.make_matrix(dict(m=m, n=n, family='gaussian', dtype=dtype, seed=7 * m + n + k))
+                  for k in range(3)])
+    x = torch.from_numpy(a).to(DEV)
+    tol = 1e-5 if dtype == 'float32' else 1e-12
+    ref = None
+    for threads, lanes in ((64, 2), (128, 4), (256, 8), (256, 16), (512, 16), (512, 32)):
+        u, s, vh, info = P.jacobi_lds_uniform(x, lanes=lanes, threads=threads)
+        assert (info[:, 1] == 0).all(), f'status {info[:, 1].tolist()} at {threads}x{lanes}'
+        for b in range(a.shape[0]):
+            met = _metrics(a[b], (u[b], s[b], vh[b]))
+            assert met['spectrum_error_over_sigma1'] < tol, (threads, lanes, met)
+            assert met['orthogonality_u_fro_over_sqrt_cols'] < tol, (threads, lanes, met)
+            assert met['orthogonality_v_fro_over_sqrt_rows'] < tol, (threads, lanes, met)
+        sv = _np(s)
+        if ref is None:
+            ref = sv
+        else:   # every launch shape gives the same spectrum to the accuracy of the data type
+            assert np.max(np.abs(sv - ref)) <= tol * np.max(ref), (threads, lanes)
+
+
 def test_rank_deficient_completion_and_zero(pkg):
     for a in (np.zeros((12, 8), np.float32), np.diag([0., 2., 0., 3., 1.]),
               X.make_matrix(dict(m=15, n=7, dtype='float64', family='low_rank', rank=3, tail=0))):
