@@ -165,7 +165,7 @@ __global__ __launch_bounds__(256) void qr_panel_coop_kernel(T *F, int64_t ldf, i
   extern __shared__ double smem_coop[];
   __shared__ double red[(NT / kWave) * kPanel];
   __shared__ double wsum[kPanel];
-  __shared__ double seg[8][kPanel];
+  __shared__ double seg[NT / kCoopW][kCoopW];
   cg::grid_group grid = cg::this_grid();
   const int tid = threadIdx.x;
   const int b = blockIdx.x / nw, wi = blockIdx.x - b * nw;
@@ -290,24 +290,34 @@ __global__ __launch_bounds__(256) void qr_panel_coop_kernel(T *F, int64_t ldf, i
         wp[(size_t)wi * kCoopW + kPanel + tid] = t;
       }
       grid.sync();
-      {  // thread (k = tid % 32, s = tid / 32) adds the partials of workgroups s, s + 8, ...
-        const int k = tid & (kPanel - 1), s = tid / kPanel;
+      {  // thread (k = tid % kCoopW, s = tid / kCoopW), s < NSEG, adds the partials of workgroups
+         // s, s + NSEG, ... in that order, four loads in flight at a time; then the NSEG segment
+         // sums in order.  Every workgroup forms the same sums in the same order.
+        constexpr int NSEG = NT / kCoopW;
+        const int k = tid % kCoopW, s = tid / kCoopW;
         double acc = 0.0;
-        if (k > c && k < nbc)
-          for (int t = s; t < nw; t += 8) acc += wp[(size_t)t * kCoopW + k];
-        seg[s][k] = acc;
-      }
-      if (tid < 5) {   // the carried sums, workgroups in order
-        double acc = 0.0;
-        for (int t = 0; t < nw; ++t) acc += wp[(size_t)t * kCoopW + kPanel + tid];
-        ext[tid] = acc;
+        if (s < NSEG && ((k > c && k < nbc) || (k >= kPanel && k < kPanel + 5))) {
+          const double *src = wp + k;
+          int t = s;
+          for (; t + 3 * NSEG < nw; t += 4 * NSEG) {
+            const double a0 = src[(size_t)t * kCoopW], a1 = src[(size_t)(t + NSEG) * kCoopW];
+            const double a2 = src[(size_t)(t + 2 * NSEG) * kCoopW], a3 = src[(size_t)(t + 3 * NSEG) * kCoopW];
+            acc += a0;
+            acc += a1;
+            acc += a2;
+            acc += a3;
+          }
+          for (; t < nw; t += NSEG) acc += src[(size_t)t * kCoopW];
+        }
+        if (s < NSEG) seg[s][k] = acc;
       }
       __syncthreads();
-      if (tid < kPanel) {
+      if (tid < kPanel + 5) {
         double acc = 0.0;
 #pragma unroll
-        for (int s = 0; s < 8; ++s) acc += seg[s][tid];
-        wsum[tid] = acc;
+        for (int s = 0; s < NT / kCoopW; ++s) acc += seg[s][tid];
+        if (tid < kPanel) wsum[tid] = acc;
+        else ext[tid - kPanel] = acc;
       }
       __syncthreads();
 #pragma unroll
