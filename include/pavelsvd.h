@@ -102,7 +102,9 @@ int psvd_gemm(int mode, int transa, int transb, int64_t m, int64_t n, int64_t k,
 /* ---------------------------------------------------------------------------------------
  * Batched utilities between solver stages (graph-capturable; outputs cleared by a kernel).
  *  psvd_reduce: op 0 out[b] = max|x_b|; op 1 out[b] = ||x_b||_F; op 2 out[b] = max|x_b - y_b| and
- *               out[batch + b] = max|x_b| (out: device doubles).  The max-abs and Frobenius
+ *               out[batch + b] = max|x_b| (out: device doubles); op 3 out[b] = the largest modulus
+ *               of the (re, im) pairs of each row (cols even: a complex matrix seen as real, the
+ *               reference's x.abs().amax() for complex x).  The max-abs and Frobenius
  *               normalisations of solvers.py:294-295 and :124-125, the QDWH test :161-163.
  *  psvd_scale:  y_b = x_b / d_b (invert, T division as the reference's x / scale) or
  *               y_b = (s * d_b) x_b; d_b <= 0 counts as 1; d may be NULL.
@@ -153,6 +155,23 @@ int psvd_complete_columns(int dtype, int64_t batch, int m, int n, void *u, int64
 size_t psvd_qr_workspace_bytes(int dtype, int64_t batch, int m, int n);
 int psvd_qr(int dtype, int gemm_mode, int64_t batch, int m, int n, const void *a, int64_t lda, int64_t stride_a,
             void *q, void *r, void *workspace, size_t workspace_bytes, psvd_stream_t stream);
+
+/* ---------------------------------------------------------------------------------------
+ * Householder QR of a batch of small complex matrices held in LDS (one workgroup per matrix):
+ * replaces primitives.qr for complex inputs (svdbench/standalone/primitives.py:41-89, the complex
+ * branch of householder_vec: beta real, tau = (beta - alpha) / beta, factorisation with conj(tau),
+ * Q accumulated backwards with tau).  dtype PSVD_C64 or PSVD_C128; a (batch, m, n), q (batch, m, n)
+ * and r (batch, n, n, upper triangular with a real diagonal) are contiguous, row-major, with
+ * interleaved (re, im) pairs; m >= n.  psvd_cqr_lds_smem_bytes gives the LDS the shape needs and
+ * returns PSVD_ETOOBIG when it does not fit (16 m n bytes for complex64, 32 m n for complex128).
+ * psvd_cmatmul: c = a b for each matrix of the batch (a: m x k, b: k x n, c: m x n, contiguous,
+ * interleaved complex, FP64 sums), the product q @ u of the reference's qr solver
+ * (svdbench/standalone/solvers.py:245-248); batch <= 65535.
+ * ------------------------------------------------------------------------------------- */
+int psvd_cqr_lds_smem_bytes(int dtype, int m, int n, size_t *bytes);
+int psvd_cqr_lds(int dtype, int64_t batch, int m, int n, const void *a, void *q, void *r, psvd_stream_t stream);
+int psvd_cmatmul(int dtype, int64_t batch, int m, int k, int n, const void *a, const void *b, void *c,
+                 psvd_stream_t stream);
 
 /* ---------------------------------------------------------------------------------------
  * Two-sided cyclic Jacobi eigensolver of a batch of symmetric n x n matrices (primitives.py:

@@ -256,6 +256,30 @@ static PyObject *py_scale_columns(PyObject *, PyObject *args) {
   return PyLong_FromLong(psvd_scale_columns(dtype, batch, rows, cols, PTR(x), ldx, sx, PTR(d), sd, PTR(stream)));
 }
 
+static PyObject *py_cqr_lds_smem_bytes(PyObject *, PyObject *args) {
+  int dtype, m, n;
+  if (!PyArg_ParseTuple(args, "iii", &dtype, &m, &n)) return nullptr;
+  size_t bytes = 0;
+  int st = psvd_cqr_lds_smem_bytes(dtype, m, n, &bytes);
+  return Py_BuildValue("(in)", st, (Py_ssize_t)bytes);
+}
+
+static PyObject *py_cqr_lds(PyObject *, PyObject *args) {
+  int dtype, m, n;
+  long long batch;
+  unsigned long long a, q, r, stream;
+  if (!PyArg_ParseTuple(args, "iLiiKKKK", &dtype, &batch, &m, &n, &a, &q, &r, &stream)) return nullptr;
+  return PyLong_FromLong(psvd_cqr_lds(dtype, batch, m, n, PTR(a), PTR(q), PTR(r), PTR(stream)));
+}
+
+static PyObject *py_cmatmul(PyObject *, PyObject *args) {
+  int dtype, m, k, n;
+  long long batch;
+  unsigned long long a, b, c, stream;
+  if (!PyArg_ParseTuple(args, "iLiiiKKKK", &dtype, &batch, &m, &k, &n, &a, &b, &c, &stream)) return nullptr;
+  return PyLong_FromLong(psvd_cmatmul(dtype, batch, m, k, n, PTR(a), PTR(b), PTR(c), PTR(stream)));
+}
+
 static PyObject *py_block_jacobi_workspace_bytes(PyObject *, PyObject *args) {
   long long batch;
   int n;
@@ -366,6 +390,9 @@ static PyMethodDef kMethods[] = {
     {"axpby", py_axpby, METH_VARARGS, nullptr},
     {"sort_desc", py_sort_desc, METH_VARARGS, nullptr},
     {"scale_columns", py_scale_columns, METH_VARARGS, nullptr},
+    {"cqr_lds_smem_bytes", py_cqr_lds_smem_bytes, METH_VARARGS, nullptr},
+    {"cqr_lds", py_cqr_lds, METH_VARARGS, nullptr},
+    {"cmatmul", py_cmatmul, METH_VARARGS, nullptr},
     {"block_jacobi_workspace_bytes", py_block_jacobi_workspace_bytes, METH_VARARGS, nullptr},
     {"block_jacobi_sweep", py_block_jacobi_sweep, METH_VARARGS, nullptr},
     {"mlp_param_count", py_mlp_param_count, METH_NOARGS, nullptr},

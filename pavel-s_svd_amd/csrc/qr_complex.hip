@@ -1,0 +1,228 @@
+// K4c: Householder QR of a batch of small complex matrices, each resident in LDS, and the small
+// batched complex product that forms U = Q U_R after the Jacobi solve of R.
+//
+// One workgroup per matrix.  A (m x n, m >= n; row-major with interleaved (re, im) pairs, as torch
+// stores complex tensors) is held column-major in LDS as a real and an imaginary plane.  Column j
+// takes the reference's complex xLARFG reflector (primitives.py:41-59):
+//   beta = -sign(Re alpha) hypot(|alpha|, |x2|) (real; sign(0) taken as +1),
+//   tau = (beta - alpha) / beta,  v = x / (alpha - beta),  v_0 = 1,
+// and the trivial reflector (tau = 0, v = 0, beta = Re alpha) when x2 = 0 and alpha is real.  The
+// trailing columns get A <- (I - conj(tau) v v^H) A (primitives.py:62-65, :79-83), one wave per
+// column, with the v^H A sums in FP64.  The reflectors stay below the diagonal.  Q (thin,
+// m x n) is then accumulated backwards from [I; 0] with A <- (I - tau v v^H) A
+// (primitives.py:84-88); R is the upper triangle with the real diagonal beta.
+#include <float.h>
+
+#include "common.hpp"
+
+namespace psvd {
+
+constexpr int kCqrThreads = 256;
+
+__host__ __device__ inline int cqr_ld(int m) { return m | 1; }
+
+template <typename R>
+__host__ __device__ inline size_t cqr_lds_bytes(int m, int n) {
+  return sizeof(double) * 2 * (size_t)n + sizeof(R) * 4 * (size_t)n * cqr_ld(m);
+}
+
+constexpr size_t kCqrLdsCap = 160 * 1024 - 4096;
+
+// x <- x - v z for the rows i >= j of one column (planes xr, xi), v from the reflector planes
+// (v_j = 1), z complex.
+template <typename R>
+__device__ __forceinline__ void cqr_axpy(R *xr, R *xi, const R *vr, const R *vi, int j, int m, int lane, double zr,
+                                         double zi) {
+  for (int i = j + lane; i < m; i += kWave) {
+    const double a = i == j ? 1.0 : (double)vr[i], b = i == j ? 0.0 : (double)vi[i];
+    xr[i] = (R)((double)xr[i] - (a * zr - b * zi));
+    xi[i] = (R)((double)xi[i] - (a * zi + b * zr));
+  }
+}
+
+// w = v^H x over the rows i >= j (v_j = 1), summed in FP64 over the wave; every lane gets it.
+template <typename R>
+__device__ __forceinline__ void cqr_dot(const R *xr, const R *xi, const R *vr, const R *vi, int j, int m, int lane,
+                                        double &wr, double &wi) {
+  double sr = 0.0, si = 0.0;
+  for (int i = j + lane; i < m; i += kWave) {
+    const double a = i == j ? 1.0 : (double)vr[i], b = i == j ? 0.0 : (double)vi[i];
+    const double x = xr[i], y = xi[i];
+    sr = fma(a, x, fma(b, y, sr));    // conj(v) x = (a - ib)(x + iy)
+    si = fma(a, y, fma(-b, x, si));
+  }
+  wr = wave_sum(sr);
+  wi = wave_sum(si);
+}
+
+template <typename R>
+__global__ __launch_bounds__(kCqrThreads) void cqr_lds_kernel(const R *a, R *q, R *r, int m, int n) {
+  extern __shared__ double smem_cq[];
+  __shared__ double red[kCqrThreads / kWave];
+  constexpr int NW = kCqrThreads / kWave;
+  const int ld = cqr_ld(m);
+  double *taur = smem_cq, *taui = taur + n;
+  R *Ar = reinterpret_cast<R *>(taui + n), *Ai = Ar + (size_t)n * ld;
+  R *Qr = Ai + (size_t)n * ld, *Qi = Qr + (size_t)n * ld;
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+  const int64_t b = blockIdx.x;
+  const R *A = a + b * 2 * (int64_t)m * n;
+  for (int e = tid; e < m * n; e += kCqrThreads) {
+    const int i = e / n, j = e - i * n;
+    Ar[j * ld + i] = A[2 * e];
+    Ai[j * ld + i] = A[2 * e + 1];
+  }
+  __syncthreads();
+  for (int j = 0; j < n; ++j) {
+    R *vr = Ar + j * ld, *vi = Ai + j * ld;
+    double part = 0.0;
+    for (int i = j + 1 + tid; i < m; i += kCqrThreads) {
+      const double x = vr[i], y = vi[i];
+      part = fma(x, x, fma(y, y, part));
+    }
+    const double xnorm = sqrt(block_sum<kCqrThreads>(part, red));
+    const double ar = vr[j], ai = vi[j];
+    const bool trivial = xnorm == 0.0 && ai == 0.0;
+    double beta = ar, tr = 0.0, ti = 0.0;
+    if (!trivial) {
+      beta = -(ar < 0.0 ? -1.0 : 1.0) * hypot(hypot(ar, ai), xnorm);
+      tr = (beta - ar) / beta;
+      ti = -ai / beta;
+      // v = x / (alpha - beta)
+      const double dr = ar - beta, di = ai, den = dr * dr + di * di;
+      const double ir = dr / den, ii = -di / den;
+      for (int i = j + 1 + tid; i < m; i += kCqrThreads) {
+        const double x = vr[i], y = vi[i];
+        vr[i] = (R)(x * ir - y * ii);
+        vi[i] = (R)(x * ii + y * ir);
+      }
+    }
+    __syncthreads();   // every thread has read alpha and written its part of v
+    if (tid == 0) {
+      vr[j] = (R)beta;
+      vi[j] = R(0);
+      taur[j] = tr;
+      taui[j] = ti;
+    }
+    if (!trivial) {   // A(j:, c) <- A(j:, c) - conj(tau) v (v^H A(j:, c)), c > j, one wave per column
+      for (int c = j + 1 + wave; c < n; c += NW) {
+        R *xr = Ar + c * ld, *xi = Ai + c * ld;
+        double wr, wi;
+        cqr_dot(xr, xi, vr, vi, j, m, lane, wr, wi);
+        cqr_axpy(xr, xi, vr, vi, j, m, lane, tr * wr + ti * wi, tr * wi - ti * wr);   // conj(tau) w
+      }
+    }
+    __syncthreads();
+  }
+  // Q = H_0 ... H_{n-1} [I; 0], accumulated backwards; rows >= j of columns < j are still zero
+  for (int e = tid; e < m * n; e += kCqrThreads) {
+    const int c = e / m, i = e - c * m;
+    Qr[c * ld + i] = i == c ? R(1) : R(0);
+    Qi[c * ld + i] = R(0);
+  }
+  __syncthreads();
+  for (int j = n - 1; j >= 0; --j) {
+    const double tr = taur[j], ti = taui[j];
+    if (tr != 0.0 || ti != 0.0) {
+      const R *vr = Ar + j * ld, *vi = Ai + j * ld;
+      for (int c = j + wave; c < n; c += NW) {
+        R *xr = Qr + c * ld, *xi = Qi + c * ld;
+        double wr, wi;
+        cqr_dot(xr, xi, vr, vi, j, m, lane, wr, wi);
+        cqr_axpy(xr, xi, vr, vi, j, m, lane, tr * wr - ti * wi, tr * wi + ti * wr);   // tau w
+      }
+    }
+    __syncthreads();
+  }
+  R *Qo = q + b * 2 * (int64_t)m * n, *Ro = r + b * 2 * (int64_t)n * n;
+  for (int e = tid; e < m * n; e += kCqrThreads) {
+    const int i = e / n, c = e - i * n;
+    Qo[2 * e] = Qr[c * ld + i];
+    Qo[2 * e + 1] = Qi[c * ld + i];
+  }
+  for (int e = tid; e < n * n; e += kCqrThreads) {
+    const int i = e / n, c = e - i * n;
+    Ro[2 * e] = i <= c ? Ar[c * ld + i] : R(0);
+    Ro[2 * e + 1] = i <= c ? Ai[c * ld + i] : R(0);
+  }
+}
+
+// C (m x n) = A (m x k) B (k x n) for each matrix of the batch, row-major interleaved complex,
+// one thread per entry of C with the sum in FP64.
+template <typename R>
+__global__ __launch_bounds__(256) void cmatmul_kernel(const R *a, const R *bm, R *c, int m, int k, int n) {
+  const int64_t bb = blockIdx.y;
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (e >= m * n) return;
+  const int i = e / n, j = e - i * n;
+  const R *A = a + bb * 2 * (int64_t)m * k + 2 * (int64_t)i * k;
+  const R *B = bm + bb * 2 * (int64_t)k * n + 2 * j;
+  double sr = 0.0, si = 0.0;
+  for (int t = 0; t < k; ++t) {
+    const double xr = A[2 * t], xi = A[2 * t + 1], yr = B[2 * (int64_t)t * n], yi = B[2 * (int64_t)t * n + 1];
+    sr = fma(xr, yr, fma(-xi, yi, sr));
+    si = fma(xr, yi, fma(xi, yr, si));
+  }
+  R *C = c + bb * 2 * (int64_t)m * n;
+  C[2 * e] = (R)sr;
+  C[2 * e + 1] = (R)si;
+}
+
+template <typename R>
+static int run_cqr(int64_t batch, int m, int n, const void *a, void *q, void *r, hipStream_t st) {
+  const size_t smem = cqr_lds_bytes<R>(m, n);
+  if (smem > kCqrLdsCap) return PSVD_ETOOBIG;
+  static bool attr = false;
+  if (!attr) {
+    if (hipFuncSetAttribute((const void *)cqr_lds_kernel<R>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)kCqrLdsCap) != hipSuccess)
+      return PSVD_ELAUNCH;
+    attr = true;
+  }
+  hipLaunchKernelGGL(cqr_lds_kernel<R>, dim3((unsigned)batch), dim3(kCqrThreads), smem, st,
+                     static_cast<const R *>(a), static_cast<R *>(q), static_cast<R *>(r), m, n);
+  return hipGetLastError() == hipSuccess ? PSVD_OK : PSVD_ELAUNCH;
+}
+
+}  // namespace psvd
+
+extern "C" {
+
+int psvd_cqr_lds_smem_bytes(int dtype, int m, int n, size_t *bytes) {
+  if (m < 1 || n < 1 || n > m || !bytes) return PSVD_EINVAL;
+  size_t b;
+  if (dtype == PSVD_C64) b = psvd::cqr_lds_bytes<float>(m, n);
+  else if (dtype == PSVD_C128) b = psvd::cqr_lds_bytes<double>(m, n);
+  else return PSVD_EINVAL;
+  *bytes = b;
+  return b > psvd::kCqrLdsCap ? PSVD_ETOOBIG : PSVD_OK;
+}
+
+int psvd_cqr_lds(int dtype, int64_t batch, int m, int n, const void *a, void *q, void *r, psvd_stream_t stream) {
+  if (batch < 0 || batch > 0x7fffffff || m < 1 || n < 1 || n > m || !a || !q || !r) return PSVD_EINVAL;
+  if (batch == 0) return PSVD_OK;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (dtype == PSVD_C64) return psvd::run_cqr<float>(batch, m, n, a, q, r, st);
+  if (dtype == PSVD_C128) return psvd::run_cqr<double>(batch, m, n, a, q, r, st);
+  return PSVD_EINVAL;
+}
+
+int psvd_cmatmul(int dtype, int64_t batch, int m, int k, int n, const void *a, const void *b, void *c,
+                 psvd_stream_t stream) {
+  if (batch < 0 || batch > 65535 || m < 1 || k < 1 || n < 1 || !a || !b || !c) return PSVD_EINVAL;
+  if ((int64_t)m * n > 0x7fffffff - 255) return PSVD_ETOOBIG;
+  if (batch == 0) return PSVD_OK;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const dim3 grid((unsigned)(((int64_t)m * n + 255) / 256), (unsigned)batch);
+  if (dtype == PSVD_C64)
+    hipLaunchKernelGGL(psvd::cmatmul_kernel<float>, grid, dim3(256), 0, st, static_cast<const float *>(a),
+                       static_cast<const float *>(b), static_cast<float *>(c), m, k, n);
+  else if (dtype == PSVD_C128)
+    hipLaunchKernelGGL(psvd::cmatmul_kernel<double>, grid, dim3(256), 0, st, static_cast<const double *>(a),
+                       static_cast<const double *>(b), static_cast<double *>(c), m, k, n);
+  else
+    return PSVD_EINVAL;
+  return hipGetLastError() == hipSuccess ? PSVD_OK : PSVD_ELAUNCH;
+}
+
+}  // extern "C"

@@ -40,6 +40,7 @@ __global__ __launch_bounds__(kUtilThreads) void reduce_kernel(const T *x, int64_
     const int r = (int)(e / cols), c = (int)(e - (int64_t)r * cols);
     const double a = (double)x[b * sx + (int64_t)r * ldx + c];
     if (OP == 0) v0 = fmax(v0, fabs(a));
+    if (OP == 3 && !(c & 1)) v0 = fmax(v0, hypot(a, (double)x[b * sx + (int64_t)r * ldx + c + 1]));
     if (OP == 1) v0 = fma(a, a, v0);
     if (OP == 2) {
       const double d = (double)y[b * sy + (int64_t)r * ldy + c];
@@ -131,7 +132,8 @@ static int run_reduce(int op, int64_t batch, int rows, int cols, const void *x, 
   const T *xp = static_cast<const T *>(x), *yp = static_cast<const T *>(y);
   if (op == 0) hipLaunchKernelGGL((reduce_kernel<T, 0>), grid, dim3(kUtilThreads), 0, st, xp, ldx, sx, yp, ldy, sy, rows, cols, o, batch);
   else if (op == 1) hipLaunchKernelGGL((reduce_kernel<T, 1>), grid, dim3(kUtilThreads), 0, st, xp, ldx, sx, yp, ldy, sy, rows, cols, o, batch);
-  else hipLaunchKernelGGL((reduce_kernel<T, 2>), grid, dim3(kUtilThreads), 0, st, xp, ldx, sx, yp, ldy, sy, rows, cols, o, batch);
+  else if (op == 2) hipLaunchKernelGGL((reduce_kernel<T, 2>), grid, dim3(kUtilThreads), 0, st, xp, ldx, sx, yp, ldy, sy, rows, cols, o, batch);
+  else hipLaunchKernelGGL((reduce_kernel<T, 3>), grid, dim3(kUtilThreads), 0, st, xp, ldx, sx, yp, ldy, sy, rows, cols, o, batch);
   if (op == 1) hipLaunchKernelGGL(sqrt_kernel, dim3((unsigned)((batch + 255) / 256)), dim3(256), 0, st, out, batch);
   return hipGetLastError() == hipSuccess ? PSVD_OK : PSVD_ELAUNCH;
 }
@@ -186,7 +188,8 @@ extern "C" {
 
 int psvd_reduce(int dtype, int op, int64_t batch, int rows, int cols, const void *x, int64_t ldx, int64_t sx,
                 const void *y, int64_t ldy, int64_t sy, double *out, psvd_stream_t stream) {
-  if (batch < 0 || rows < 0 || cols < 0 || op < 0 || op > 2 || !out || batch > 65535) return PSVD_EINVAL;
+  if (batch < 0 || rows < 0 || cols < 0 || op < 0 || op > 3 || !out || batch > 65535) return PSVD_EINVAL;
+  if (op == 3 && (cols & 1)) return PSVD_EINVAL;
   if (batch == 0) return PSVD_OK;
   if (!x || (op == 2 && !y)) return PSVD_EINVAL;
   hipStream_t st = static_cast<hipStream_t>(stream);

@@ -136,14 +136,51 @@ def jacobi_svd(A, tol=None, max_sweeps=30, *, cut='reference'):
     return u, s, vh.mH, int(info[:, 0].max()) if b else 0
 
 
+def cqr_fits(dtype, m, n):
+    """True when the complex LDS-resident QR takes an m x n matrix of this dtype."""
+    st, _ = N._hip.cqr_lds_smem_bytes(N.dtype_code(dtype), m, n)
+    return st == N.OK
+
+
+def cqr(A):
+    """Householder QR of the tall complex batch A (b, m, n) with each matrix in LDS: Q (b, m, n),
+    R (b, n, n) (primitives.py:41-89, complex branch)."""
+    b, m, n = A.shape
+    if m < n:
+        raise ValueError('qr takes tall matrices (m >= n)')
+    A = A.contiguous()
+    code = N.dtype_code(A.dtype)
+    q = torch.empty(b, m, n, dtype=A.dtype, device=A.device)
+    r = torch.empty(b, n, n, dtype=A.dtype, device=A.device)
+    N.check(N._hip.cqr_lds(code, b, m, n, N.ptr(A), N.ptr(q), N.ptr(r), N.stream_of(A)), 'psvd_cqr_lds')
+    return q, r
+
+
+def cmatmul(a, b):
+    """Batched complex product a (B, m, k) @ b (B, k, n) on the device (FP64 sums)."""
+    a, b = a.contiguous(), b.contiguous()
+    B, m, k = a.shape
+    c = torch.empty(B, m, b.shape[-1], dtype=a.dtype, device=a.device)
+    N.check(N._hip.cmatmul(N.dtype_code(a.dtype), B, m, k, b.shape[-1], N.ptr(a), N.ptr(b), N.ptr(c),
+                           N.stream_of(a)), 'psvd_cmatmul')
+    return c
+
+
 def qr(A, *, gemm='f32', want_q=True):
     """Householder QR of the tall batch A (b, m, n): Q (b, m, n), R (b, n, n) (primitives.py:74-89).
 
     gemm: 'f32' or 'bf16x3' arithmetic for the FP32 panel updates (FP64 inputs use FP64).
-    want_q=False skips forming Q (returns None for it)."""
+    want_q=False skips forming Q (returns None for it).  Complex inputs take the LDS-resident
+    complex kernel (``cqr``), for the shapes it holds."""
     b, m, n = A.shape
     if m < n:
         raise ValueError('qr takes tall matrices (m >= n)')
+    if A.is_complex():
+        if not cqr_fits(A.dtype, m, n):
+            from .solvers import Unsupported
+            raise Unsupported('complex QR is built for matrices that fit in LDS only')
+        q, r = cqr(A)
+        return (q if want_q else None), r
     code = N.dtype_code(A.dtype)
     if A.stride(-1) != 1:
         A = A.contiguous()
@@ -168,10 +205,16 @@ def _mat3(x):
 
 def reduce(x, op, y=None):
     """Per-matrix reduction of (b, r, c) device tensors into FP64 device scalars:
-    'maxabs' -> (b,), 'fro' -> (b,), 'maxdiff' -> (b, 2) = [max|x-y|, max|x|]."""
+    'maxabs' -> (b,), 'fro' -> (b,), 'maxdiff' -> (b, 2) = [max|x-y|, max|x|]; a complex x takes
+    'maxabs' as its largest modulus."""
+    if x.is_complex():
+        if op != 'maxabs':
+            raise ValueError('complex inputs take the maxabs reduction only')
+        xr = torch.view_as_real(x.contiguous())
+        x, op = xr.reshape(*xr.shape[:-2], xr.shape[-2] * 2), 'maxmod'
     x3, ldx, sx = _mat3(x)
     b, rows, cols = x3.shape
-    code = {'maxabs': 0, 'fro': 1, 'maxdiff': 2}[op]
+    code = {'maxabs': 0, 'fro': 1, 'maxdiff': 2, 'maxmod': 3}[op]
     out = torch.empty(2 * b if code == 2 else b, dtype=torch.float64, device=x.device)
     if code == 2:
         y3, ldy, sy = _mat3(y)
@@ -184,7 +227,15 @@ def reduce(x, op, y=None):
 
 
 def scale(x, d=None, *, invert=False, s=1.0, out=None):
-    """out_b = x_b / d_b (invert; x / scale as the reference rounds it) or (s * d_b) * x_b."""
+    """out_b = x_b / d_b (invert; x / scale as the reference rounds it) or (s * d_b) * x_b.  A complex
+    x is scaled through its real view (both parts by the real d_b)."""
+    if x.is_complex():
+        if out is not None:
+            raise ValueError('complex scaling returns a new tensor (out is not taken)')
+        xr = torch.view_as_real(x.contiguous())
+        flat = xr.reshape(*xr.shape[:-2], xr.shape[-2] * 2)
+        res = scale(flat, d, invert=invert, s=s)
+        return torch.view_as_complex(res.reshape(xr.shape))
     x3, ldx, sx = _mat3(x)
     b, rows, cols = x3.shape
     if out is None:

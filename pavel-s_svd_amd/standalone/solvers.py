@@ -483,6 +483,8 @@ def _raw(x, method, stats, opts):
         u, s, vh = _jacobi(r, stats, opts, scale=False)
         if values:
             return None, s, vh
+        if x.is_complex():
+            return P.cmatmul(q, u), s, vh
         u = gemm(q, u, mode=mode)
         if opts.get('polish', False) and x.dtype == torch.float32:
             # Q's orthogonality is that of its products (about 2^-16 with BF16x3) and V's that of
@@ -526,10 +528,14 @@ def svd(a, algorithm='jacobi', *, full_matrices=False, values_only=False, return
         raise ValueError('input contains NaN or infinity')
     if algorithm not in SUPPORTED:
         raise Unsupported(f'{algorithm!r} is outside the MI355X hot path (built: {SUPPORTED})')
-    if a.is_complex() and algorithm != 'jacobi':
-        raise Unsupported(f'complex {algorithm!r} is not built for MI355X (complex inputs: jacobi only)')
-    if a.is_complex() and not P.lds_fits(a.dtype, max(a.shape[-2:]), min(a.shape[-2:]), not values_only):
+    if a.is_complex() and algorithm not in ('jacobi', 'qr'):
+        raise Unsupported(f'complex {algorithm!r} is not built for MI355X (complex inputs: jacobi and qr)')
+    if a.is_complex() and algorithm == 'jacobi' and not P.lds_fits(a.dtype, max(a.shape[-2:]), min(a.shape[-2:]),
+                                                                    not values_only):
         raise Unsupported('complex one-sided Jacobi is built for matrices that fit in LDS only')
+    if a.is_complex() and algorithm == 'qr' and not (P.cqr_fits(a.dtype, max(a.shape[-2:]), min(a.shape[-2:])) and
+                                                     P.lds_fits(a.dtype, min(a.shape[-2:]), min(a.shape[-2:]), True)):
+        raise Unsupported('complex QR is built for matrices that fit in LDS only')
     if a.is_complex() and full_matrices and a.shape[-2] != a.shape[-1]:
         raise Unsupported('full_matrices completion of complex bases is not built for MI355X')
     if a.device.type != 'cuda':

@@ -1,6 +1,7 @@
-"""GPU parity of the complex one-sided Jacobi kernel (complex64 / complex128, LDS resident) against
-the reference's golden rows for its complex corpus inputs, the oracle (which restates the complex
-arithmetic of primitives.py:331-351) and complex LAPACK.
+"""GPU parity of the complex one-sided Jacobi kernel and the complex Householder QR (complex64 /
+complex128, LDS resident) against the reference's golden rows for its complex corpus inputs, the
+oracle (which restates the complex arithmetic of primitives.py:41-89 and :331-351) and complex
+LAPACK.
 
 Bars: sigma error over sigma_1, U/V orthogonality and reconstruction <= 1e-5 (complex64) / 1e-12
 (complex128); sweep counts within 1 of the reference's records and of the oracle's (FP64 sums);
@@ -96,8 +97,83 @@ def test_complex_rank_deficient_zero_wide_and_values_only(pkg):
     assert np.max(np.abs(_np(sv) - np.linalg.svd(w.astype(np.complex128), compute_uv=False))) < 1e-5 * float(sv[0])
 
 
+@pytest.mark.parametrize('dtype', [np.complex64, np.complex128])
+@pytest.mark.parametrize('shape', [(3, 8, 8), (2, 40, 17), (1, 64, 64), (4, 33, 1), (2, 70, 50), (1, 96, 48)])
+def test_complex_qr_factors_match_oracle_and_lapack(dtype, shape):
+    """Q and R of the LDS complex QR against the oracle's restatement of primitives.qr (LAPACK's
+    zgeqrf convention, so also scipy's economic QR): entrywise to 2e-5 (complex64) / 1e-12
+    (complex128) relative to max |A|, Q^H Q = I and QR = A to the same bars."""
+    import scipy.linalg
+    from importlib import import_module
+    P = import_module('pavel-s_svd_amd.standalone.primitives')
+    rng = np.random.default_rng(shape[1] * 11 + shape[2])
+    a = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(dtype)
+    print(f'complex qr {shape} {np.dtype(dtype).name}', flush=True)
+    q, r = P.qr(torch.from_numpy(a).to(DEV))
+    q, r = _np(q), _np(r)
+    tol = 2e-5 if dtype == np.complex64 else 1e-12
+    qo, ro = R.qr(a.astype(np.complex128))
+    scale = np.abs(a).max()
+    assert np.abs(r - ro).max() <= tol * scale
+    assert np.abs(q - qo).max() <= tol
+    for i in range(shape[0]):
+        qs, rs = scipy.linalg.qr(a[i].astype(np.complex128), mode='economic')
+        assert np.abs(r[i] - rs).max() <= tol * scale
+        assert np.abs(np.diag(r[i]).imag).max() == 0.0     # beta is real
+        assert np.abs(q[i].conj().T @ q[i] - np.eye(shape[2])).max() <= tol
+        assert np.abs(q[i] @ r[i] - a[i]).max() <= tol * scale
+
+
+def test_complex_qr_trivial_reflector():
+    """A column that is already reduced (real alpha, zeros below) takes the trivial reflector:
+    tau = 0, beta = alpha, so R keeps alpha's sign and Q keeps e_j (primitives.py:49-57)."""
+    from importlib import import_module
+    P = import_module('pavel-s_svd_amd.standalone.primitives')
+    a = np.zeros((1, 5, 3), dtype=np.complex128)
+    a[0, 0, 0] = -2.0
+    a[0, :, 1] = [1 + 1j, 3.0, 0.0, 1j, 2.0]
+    a[0, :, 2] = [0.5, -1j, 2.0, 0.0, 1.0]
+    q, r = P.qr(torch.from_numpy(a).to(DEV))
+    qo, ro = R.qr(a)
+    assert _np(r)[0, 0, 0] == -2.0 and np.allclose(_np(q)[0, :, 0], [1, 0, 0, 0, 0])
+    assert np.abs(_np(r) - ro).max() < 1e-13 and np.abs(_np(q) - qo).max() < 1e-13
+
+
+@pytest.mark.parametrize('case', ['clustered_complex', 'wide_complex'])
+def test_complex_qr_solver_matches_oracle_and_lapack(pkg, case):
+    """svd(a, 'qr') on the complex corpus inputs: QR of the (transposed, for wide) input, one-sided
+    Jacobi of R, U = Q U_R (solvers.py:245-248).  sigma against the oracle's qr solver and LAPACK,
+    metrics to the complex bars."""
+    a = corpus(case)
+    print(f'complex qr solver {case} {a.shape} {a.dtype}', flush=True)
+    u, s, vh, info = pkg.svd(torch.from_numpy(a).to(DEV), 'qr', return_info=True)
+    met = _metrics(a, (u, s, vh))
+    bar = 1e-5 if a.dtype == np.complex64 else 1e-12
+    for key in KEYS:
+        assert met[key] <= bar, (key, met[key])
+    _, so, _, oinfo = R.svd(a, 'qr', return_info=True)
+    assert np.abs(_np(s) - so).max() <= bar * so.max()
+    assert abs(info['jacobi_sweeps'] - oinfo['jacobi_sweeps']) <= 1
+
+
+@pytest.mark.parametrize('dtype', [np.complex64, np.complex128])
+def test_complex_qr_solver_random_batch(pkg, dtype):
+    rng = np.random.default_rng(5)
+    a = (rng.standard_normal((3, 60, 24)) + 1j * rng.standard_normal((3, 60, 24))).astype(dtype)
+    u, s, vh = pkg.svd(torch.from_numpy(a).to(DEV), 'qr')
+    uj, sj, vhj = pkg.svd(torch.from_numpy(a).to(DEV), 'jacobi')
+    bar = 1e-5 if dtype == np.complex64 else 1e-12
+    for i in range(3):
+        met = _metrics(a[i], (u[i], s[i], vh[i]))
+        for key in KEYS:
+            assert met[key] <= bar, (key, met[key])
+    assert np.abs(_np(s) - _np(sj)).max() <= bar * float(_np(sj).max())
+
+
 def test_complex_unsupported_paths(pkg):
     x = torch.zeros(1, 20, 10, dtype=torch.complex64, device=DEV) + 1
+    with pytest.raises(pkg.Unsupported):   # complex QR beyond LDS
+        pkg.svd(torch.zeros(1, 600, 300, dtype=torch.complex128, device=DEV) + 1, 'qr')
     for method in ('tsqr', 'polar_qdwh', 'block_jacobi', 'randomized'):
         with pytest.raises(pkg.Unsupported):
             pkg.svd(x, method)

@@ -75,7 +75,8 @@ def test_unsupported_rows(H):
     row = H.measurement_row(c, {'name': 'clustered_complex'}, {'name': 'pavel-s_svd_amd:factory',
                                                                'method': 'tsqr'}, 'cuda:0', warmup=0, repeat=1)
     assert row['status'] == 'unsupported' and 'complex' in row['error']
-    row = H.measurement_row(c, {'name': 'clustered_complex'}, {'name': 'pavel-s_svd_amd:factory',
-                                                               'method': 'jacobi'}, 'cuda:0', warmup=0, repeat=1)
-    assert row['status'] == 'ok'
+    for method in ('jacobi', 'qr'):
+        row = H.measurement_row(c, {'name': 'clustered_complex'}, {'name': 'pavel-s_svd_amd:factory',
+                                                                   'method': method}, 'cuda:0', warmup=0, repeat=1)
+        assert row['status'] == 'ok', (method, row.get('error'))
     json.dumps(row, allow_nan=False)
