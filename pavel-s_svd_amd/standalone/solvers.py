@@ -228,6 +228,14 @@ def tsqr_leaf_bounds(m, n, rows):
     return bounds
 
 
+def _complex_tsqr_fits(dtype, m, n, rows):
+    """Every QR of a complex TSQR (its leaves and the [R1; R2] merges) and the Jacobi of R fit in LDS."""
+    rows = 4 * n if rows is None else rows
+    bounds = tsqr_leaf_bounds(m, n, rows) or [(0, m)]
+    heights = {e - s for s, e in bounds} | ({2 * n} if len(bounds) > 1 else set())
+    return all(P.cqr_fits(dtype, h, n) for h in heights) and P.lds_fits(dtype, n, n, True)
+
+
 def _tsqr(x, rows, stats, opts):
     """TSQR: leaf QRs, then a binary tree of [R1; R2] QRs with the explicit Q back-transformation
     Q <- [Q1 q_top; Q2 q_bot] at every level (solvers.py:71-96).  Equal-height leaves of a matrix
@@ -256,6 +264,11 @@ def _tsqr(x, rows, stats, opts):
             qm, rm = P.qr(stacked, gemm=qmode)
             nxt = []
             for i, ((q1, _), (q2, _)) in enumerate(pairs):
+                if x.is_complex():
+                    out = torch.cat([P.cmatmul(q1[None], qm[i:i + 1, :n])[0],
+                                     P.cmatmul(q2[None], qm[i:i + 1, n:])[0]], 0)
+                    nxt.append((out, rm[i]))
+                    continue
                 out = torch.empty(q1.shape[0] + q2.shape[0], n, dtype=x.dtype, device=x.device)
                 gemm(q1, qm[i, :n], out=out[:q1.shape[0]], mode=mode)
                 gemm(q2, qm[i, n:], out=out[q1.shape[0]:], mode=mode)
@@ -528,8 +541,11 @@ def svd(a, algorithm='jacobi', *, full_matrices=False, values_only=False, return
         raise ValueError('input contains NaN or infinity')
     if algorithm not in SUPPORTED:
         raise Unsupported(f'{algorithm!r} is outside the MI355X hot path (built: {SUPPORTED})')
-    if a.is_complex() and algorithm not in ('jacobi', 'qr'):
-        raise Unsupported(f'complex {algorithm!r} is not built for MI355X (complex inputs: jacobi and qr)')
+    if a.is_complex() and algorithm not in ('jacobi', 'qr', 'tsqr'):
+        raise Unsupported(f'complex {algorithm!r} is not built for MI355X (complex inputs: jacobi, qr, tsqr)')
+    if a.is_complex() and algorithm == 'tsqr' and not _complex_tsqr_fits(a.dtype, max(a.shape[-2:]),
+                                                                          min(a.shape[-2:]), opts.get('tile_rows')):
+        raise Unsupported('complex TSQR is built for leaves that fit in LDS only')
     if a.is_complex() and algorithm == 'jacobi' and not P.lds_fits(a.dtype, max(a.shape[-2:]), min(a.shape[-2:]),
                                                                     not values_only):
         raise Unsupported('complex one-sided Jacobi is built for matrices that fit in LDS only')

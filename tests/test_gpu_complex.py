@@ -170,11 +170,34 @@ def test_complex_qr_solver_random_batch(pkg, dtype):
     assert np.abs(_np(s) - _np(sj)).max() <= bar * float(_np(sj).max())
 
 
+@pytest.mark.parametrize('dtype,shape,rows', [(np.complex64, (2, 200, 16), 48), (np.complex128, (1, 130, 24), 40),
+                                               (np.complex64, (1, 97, 8), 32)])
+def test_complex_tsqr_matches_oracle(pkg, dtype, shape, rows):
+    """svd(a, 'tsqr', tile_rows=rows) on complex inputs with several leaves (a short last block
+    joins the previous leaf) and a binary tree of [R1; R2] merges (solvers.py:71-96): leaf count,
+    sweeps and sigma against the oracle's TSQR, metrics to the complex bars."""
+    rng = np.random.default_rng(shape[1] + rows)
+    a = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(dtype)
+    print(f'complex tsqr {shape} rows {rows} {np.dtype(dtype).name}', flush=True)
+    u, s, vh, info = pkg.svd(torch.from_numpy(a).to(DEV), 'tsqr', tile_rows=rows, return_info=True)
+    _, so, _, oinfo = R.svd(a, 'tsqr', tile_rows=rows, return_info=True)
+    assert info['tsqr_leaves'] == oinfo['tsqr_leaves'] > 1
+    assert abs(info['jacobi_sweeps'] - oinfo['jacobi_sweeps']) <= 1
+    bar = 1e-5 if dtype == np.complex64 else 1e-12
+    assert np.abs(_np(s) - so).max() <= bar * so.max()
+    for i in range(shape[0]):
+        met = _metrics(a[i], (u[i], s[i], vh[i]))
+        for key in KEYS:
+            assert met[key] <= bar, (key, met[key])
+
+
 def test_complex_unsupported_paths(pkg):
     x = torch.zeros(1, 20, 10, dtype=torch.complex64, device=DEV) + 1
     with pytest.raises(pkg.Unsupported):   # complex QR beyond LDS
         pkg.svd(torch.zeros(1, 600, 300, dtype=torch.complex128, device=DEV) + 1, 'qr')
-    for method in ('tsqr', 'polar_qdwh', 'block_jacobi', 'randomized'):
+    with pytest.raises(pkg.Unsupported):   # complex TSQR leaves (256 x 64 complex128) beyond LDS
+        pkg.svd(torch.zeros(1, 600, 64, dtype=torch.complex128, device=DEV) + 1, 'tsqr')
+    for method in ('polar_qdwh', 'block_jacobi', 'randomized'):
         with pytest.raises(pkg.Unsupported):
             pkg.svd(x, method)
     with pytest.raises(pkg.Unsupported):
