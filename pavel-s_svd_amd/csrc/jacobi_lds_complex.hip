@@ -46,13 +46,16 @@ __global__ void zero_queue_kernel_c(int32_t *q) {
 }
 
 __host__ __device__ inline int c_pad_even(int n) { return n + (n & 1); }
-__host__ __device__ inline int c_ld(int m) { return m | 1; }
+// Column stride of the W (m rows) and V (np rows) planes: the row count rounded up to a multiple
+// of 16, plus one (odd); the extra rows are zero and stay zero under the rotations, so a group of
+// L <= 16 lanes strides over ceil(rows / L) row slots with no bounds test.
+__host__ __device__ inline int c_ld(int rows) { return ((rows + 15) & ~15) | 1; }
 
 template <typename R>
 __host__ __device__ inline size_t clds_bytes(int m, int n, bool vectors) {
-  const int np = c_pad_even(n), ldw = c_ld(m);
+  const int np = c_pad_even(n), ldw = c_ld(m), ldv = c_ld(np);
   size_t b = sizeof(double) * (size_t)(3 * np + 2 * m);   // sigma, y (re, im), v (re, im)
-  b += sizeof(R) * 2 * ((size_t)np * ldw + (vectors ? (size_t)np * np : 0));
+  b += sizeof(R) * 2 * ((size_t)np * ldw + (vectors ? (size_t)np * ldv : 0));
   b += sizeof(int) * (size_t)np;
   return (b + 15) & ~size_t(15);
 }
@@ -81,7 +84,9 @@ __device__ __forceinline__ void cplx_rotate(R *xr, R *xi, R *yr, R *yi, int rows
   }
 }
 
-template <typename R, int NT, int L>
+// S > 0: every lane holds exactly S = ceil(m / L) row slots of a pair (L <= 16), kept in registers
+// from the sums to the rotation with no bounds tests; S = 0: rows streamed with a bounds test.
+template <typename R, int NT, int L, int S>
 __global__ __launch_bounds__(NT) void jacobi_lds_cplx_kernel(CLdsParams<R> P) {
   extern __shared__ double smem_d[];
   __shared__ int s_idx;
@@ -99,15 +104,16 @@ __global__ __launch_bounds__(NT) void jacobi_lds_cplx_kernel(CLdsParams<R> P) {
   const double tol = P.tol > 0.0 ? P.tol : 10.0 * eps;
   const double tol2 = tol * tol, eps2 = eps * eps;
   const int m = P.m, n = P.n;
-  const int np = c_pad_even(n), ldw = c_ld(m), npairs = np / 2;
+  const int np = c_pad_even(n), ldw = c_ld(m), ldv = c_ld(np), npairs = np / 2;
+  const int rcv = (np + L - 1) / L;
   double *sig = smem_d;
   double *yr = sig + np, *yi = yr + np;
   double *vr = yi + np, *vi = vr + m;
   R *Wr = reinterpret_cast<R *>(vi + m);
   R *Wi = Wr + (size_t)np * ldw;
   R *Vr = Wi + (size_t)np * ldw;
-  R *Vi = Vr + (vectors ? (size_t)np * np : 0);
-  int *ord = reinterpret_cast<int *>(Vi + (vectors ? (size_t)np * np : 0));
+  R *Vi = Vr + (vectors ? (size_t)np * ldv : 0);
+  int *ord = reinterpret_cast<int *>(Vi + (vectors ? (size_t)np * ldv : 0));
 
   for (;;) {
     if (tid == 0) s_idx = atomicAdd(P.queue, 1);
@@ -128,14 +134,23 @@ __global__ __launch_bounds__(NT) void jacobi_lds_cplx_kernel(CLdsParams<R> P) {
       const double ax = hypot((double)xr, (double)xi);
       if (ax <= DBL_MAX) amax = fmax(amax, ax); else bad = 1.0;
     }
+    {  // zero rows past m in every column, and the padding column when n is odd
+      const int padr = ldw - m;
+      for (int e = tid; e < np * padr; e += NT) {
+        const int j = e / padr, i = m + (e - j * padr);
+        Wr[j * ldw + i] = R(0);
+        Wi[j * ldw + i] = R(0);
+      }
+    }
     if (np > n)
       for (int i = tid; i < m; i += NT) {
         Wr[n * ldw + i] = R(0);
         Wi[n * ldw + i] = R(0);
       }
     if (vectors)
-      for (int e = tid; e < np * np; e += NT) {
-        Vr[e] = (e / np == e % np) ? R(1) : R(0);
+      for (int e = tid; e < np * ldv; e += NT) {
+        const int j = e / ldv;
+        Vr[e] = (e - j * ldv == j) ? R(1) : R(0);
         Vi[e] = R(0);
       }
     if (block_max<NT>(bad, s_red) > 0.0) {
@@ -190,16 +205,33 @@ __global__ __launch_bounds__(NT) void jacobi_lds_cplx_kernel(CLdsParams<R> P) {
           rr_pair(r, k, np, p, q);
           R *pr = Wr + p * ldw, *pi = Wi + p * ldw, *qr = Wr + q * ldw, *qi = Wi + q * ldw;
           double app = 0.0, aqq = 0.0, ar = 0.0, ai = 0.0;
-          for (int i = lig; i < m; i += L) {
-            const double xr = pr[i], xi = pi[i], br = qr[i], bi = qi[i];
-            app = fma(xr, xr, fma(xi, xi, app));
-            aqq = fma(br, br, fma(bi, bi, aqq));
-            ar = fma(xr, br, fma(xi, bi, ar));
-            ai = fma(xr, bi, fma(-xi, br, ai));
+          R xr_[S > 0 ? S : 1], xi_[S > 0 ? S : 1], br_[S > 0 ? S : 1], bi_[S > 0 ? S : 1];
+          if constexpr (S > 0) {
+#pragma unroll
+            for (int j = 0; j < S; ++j) {
+              xr_[j] = pr[lig + j * L];
+              xi_[j] = pi[lig + j * L];
+              br_[j] = qr[lig + j * L];
+              bi_[j] = qi[lig + j * L];
+            }
+#pragma unroll
+            for (int j = 0; j < S; ++j) {
+              const double xr = xr_[j], xi = xi_[j], br = br_[j], bi = bi_[j];
+              app = fma(xr, xr, fma(xi, xi, app));
+              aqq = fma(br, br, fma(bi, bi, aqq));
+              ar = fma(xr, br, fma(xi, bi, ar));
+              ai = fma(xr, bi, fma(-xi, br, ai));
+            }
+          } else {
+            for (int i = lig; i < m; i += L) {
+              const double xr = pr[i], xi = pi[i], br = qr[i], bi = qi[i];
+              app = fma(xr, xr, fma(xi, xi, app));
+              aqq = fma(br, br, fma(bi, bi, aqq));
+              ar = fma(xr, br, fma(xi, bi, ar));
+              ai = fma(xr, bi, fma(-xi, br, ai));
+            }
           }
-          app = group_sum_dpp<L>(app);
-          aqq = group_sum_dpp<L>(aqq);
-          ar = group_sum_dpp<L>(ar);
+          group_sum3_dpp<L>(app, aqq, ar);
           ai = group_sum_dpp<L>(ai);
           const double apq2 = ar * ar + ai * ai;
           double nap = app, naq = aqq;
@@ -209,9 +241,32 @@ __global__ __launch_bounds__(NT) void jacobi_lds_cplx_kernel(CLdsParams<R> P) {
             nap = app - tb;
             naq = aqq + tb;
             wave_active = true;
-            cplx_rotate<R, L>(pr, pi, qr, qi, m, lig, (R)c, (R)sr, (R)si);
-            if (vectors) cplx_rotate<R, L>(Vr + p * np, Vi + p * np, Vr + q * np, Vi + q * np, np, lig, (R)c, (R)sr,
-                                           (R)si);
+            const R cc = (R)c, rs = (R)sr, is = (R)si;
+            if constexpr (S > 0) {
+#pragma unroll
+              for (int j = 0; j < S; ++j) {
+                const R ar_ = xr_[j], ai_ = xi_[j], br = br_[j], bi = bi_[j];
+                pr[lig + j * L] = cc * ar_ - (rs * br + is * bi);
+                pi[lig + j * L] = cc * ai_ - (rs * bi - is * br);
+                qr[lig + j * L] = (rs * ar_ - is * ai_) + cc * br;
+                qi[lig + j * L] = (rs * ai_ + is * ar_) + cc * bi;
+              }
+              if (vectors) {   // rows past np are zero: rcv slots with no bounds test
+                R *vpr = Vr + p * ldv + lig, *vpi = Vi + p * ldv + lig, *vqr = Vr + q * ldv + lig,
+                  *vqi = Vi + q * ldv + lig;
+                for (int j = 0; j < rcv; ++j) {
+                  const R ar_ = vpr[j * L], ai_ = vpi[j * L], br = vqr[j * L], bi = vqi[j * L];
+                  vpr[j * L] = cc * ar_ - (rs * br + is * bi);
+                  vpi[j * L] = cc * ai_ - (rs * bi - is * br);
+                  vqr[j * L] = (rs * ar_ - is * ai_) + cc * br;
+                  vqi[j * L] = (rs * ai_ + is * ar_) + cc * bi;
+                }
+              }
+            } else {
+              cplx_rotate<R, L>(pr, pi, qr, qi, m, lig, cc, rs, is);
+              if (vectors) cplx_rotate<R, L>(Vr + p * ldv, Vi + p * ldv, Vr + q * ldv, Vi + q * ldv, np, lig, cc, rs,
+                                             is);
+            }
           }
           mymax = fmax(mymax, fmax(nap, naq));
         }
@@ -368,8 +423,8 @@ __global__ __launch_bounds__(NT) void jacobi_lds_cplx_kernel(CLdsParams<R> P) {
     }
 
     // ---- outputs: U (m x n), sigma (real), Vh = V^H (n x n), interleaved complex
-    R *S = P.s + soff;
-    for (int r = tid; r < n; r += NT) S[r] = r < nk ? (R)(sig[ord[r]] * scale) : R(0);
+    R *So = P.s + soff;
+    for (int r = tid; r < n; r += NT) So[r] = r < nk ? (R)(sig[ord[r]] * scale) : R(0);
     if (vectors) {
       R *U = P.u + 2 * aoff;
       for (int e = tid; e < m * n; e += NT) {
@@ -380,8 +435,8 @@ __global__ __launch_bounds__(NT) void jacobi_lds_cplx_kernel(CLdsParams<R> P) {
       R *VH = P.vh + 2 * voff;
       for (int e = tid; e < n * n; e += NT) {
         const int r = e / n, j = e - r * n;
-        VH[2 * e] = Vr[ord[r] * np + j];
-        VH[2 * e + 1] = -Vi[ord[r] * np + j];
+        VH[2 * e] = Vr[ord[r] * ldv + j];
+        VH[2 * e + 1] = -Vi[ord[r] * ldv + j];
       }
     }
     if (tid == 0) {
@@ -394,10 +449,10 @@ __global__ __launch_bounds__(NT) void jacobi_lds_cplx_kernel(CLdsParams<R> P) {
 
 constexpr size_t kCLdsCap = 160 * 1024 - 4096;
 
-template <typename R, int L>
+template <typename R, int L, int S>
 static int launch_clds(const CLdsParams<R> &P, size_t smem, hipStream_t stream) {
   constexpr int NT = 256;
-  auto kern = jacobi_lds_cplx_kernel<R, NT, L>;
+  auto kern = jacobi_lds_cplx_kernel<R, NT, L, S>;
   static bool attr = false;
   if (!attr) {
     if (hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kCLdsCap) !=
@@ -444,14 +499,31 @@ static int run_clds_t(int64_t batch, int m, int n, const void *a, void *u, void 
   P.tol = tol;
   P.max_sweeps = max_sweeps;
   P.flags = flags;
-  switch (lanes) {
-    case 2: return launch_clds<R, 2>(P, smem, stream);
-    case 4: return launch_clds<R, 4>(P, smem, stream);
-    case 8: return launch_clds<R, 8>(P, smem, stream);
-    case 16: return launch_clds<R, 16>(P, smem, stream);
-    case 32: return launch_clds<R, 32>(P, smem, stream);
-    default: return launch_clds<R, 64>(P, smem, stream);
+  // row slots per lane kept in registers: up to 8 (complex64) or 4 (complex128), L <= 16
+  constexpr int SMAX = sizeof(R) == 4 ? 8 : 4;
+  const int slots = (m + lanes - 1) / lanes;
+  const int S = (lanes <= 16 && slots <= SMAX) ? slots : 0;
+#define PSVD_CLDS_L(LV)                                                                                   \
+  switch (S) {                                                                                            \
+    case 1: return launch_clds<R, LV, 1>(P, smem, stream);                                                \
+    case 2: return launch_clds<R, LV, 2>(P, smem, stream);                                                \
+    case 3: return launch_clds<R, LV, 3>(P, smem, stream);                                                \
+    case 4: return launch_clds<R, LV, 4>(P, smem, stream);                                                \
+    case 5: return launch_clds<R, LV, (SMAX >= 5 ? 5 : 0)>(P, smem, stream);                              \
+    case 6: return launch_clds<R, LV, (SMAX >= 6 ? 6 : 0)>(P, smem, stream);                              \
+    case 7: return launch_clds<R, LV, (SMAX >= 7 ? 7 : 0)>(P, smem, stream);                              \
+    case 8: return launch_clds<R, LV, (SMAX >= 8 ? 8 : 0)>(P, smem, stream);                              \
+    default: return launch_clds<R, LV, 0>(P, smem, stream);                                               \
   }
+  switch (lanes) {
+    case 2: PSVD_CLDS_L(2)
+    case 4: PSVD_CLDS_L(4)
+    case 8: PSVD_CLDS_L(8)
+    case 16: PSVD_CLDS_L(16)
+    case 32: return launch_clds<R, 32, 0>(P, smem, stream);
+    default: return launch_clds<R, 64, 0>(P, smem, stream);
+  }
+#undef PSVD_CLDS_L
 }
 
 size_t clds_smem_bytes(int dtype, int m, int n, bool vectors) {
