@@ -7,8 +7,8 @@
 //   beta = -sign(Re alpha) hypot(|alpha|, |x2|) (real; sign(0) taken as +1),
 //   tau = (beta - alpha) / beta,  v = x / (alpha - beta),  v_0 = 1,
 // and the trivial reflector (tau = 0, v = 0, beta = Re alpha) when x2 = 0 and alpha is real.  The
-// trailing columns get A <- (I - conj(tau) v v^H) A (primitives.py:62-65, :79-83), one wave per
-// column, with the v^H A sums in FP64.  The reflectors stay below the diagonal.  Q (thin,
+// trailing columns get A <- (I - conj(tau) v v^H) A (primitives.py:62-65, :79-83), one group of 16
+// lanes per column, with the v^H A sums in FP64.  The reflectors stay below the diagonal.  Q (thin,
 // m x n) is then accumulated backwards from [I; 0] with A <- (I - tau v v^H) A
 // (primitives.py:84-88); R is the upper triangle with the real diagonal beta.
 #include <float.h>
@@ -28,43 +28,48 @@ __host__ __device__ inline size_t cqr_lds_bytes(int m, int n) {
 
 constexpr size_t kCqrLdsCap = 160 * 1024 - 4096;
 
+// Columns are processed by groups of kCqrGroup lanes (one DPP row each), so a wave works on four
+// columns at once and a column's sums are reduced by DPP within its row.
+constexpr int kCqrGroup = 16;
+
 // x <- x - v z for the rows i >= j of one column (planes xr, xi), v from the reflector planes
-// (v_j = 1), z complex.
+// (v_j = 1), z complex; lig is the lane's index in its group.
 template <typename R>
-__device__ __forceinline__ void cqr_axpy(R *xr, R *xi, const R *vr, const R *vi, int j, int m, int lane, double zr,
+__device__ __forceinline__ void cqr_axpy(R *xr, R *xi, const R *vr, const R *vi, int j, int m, int lig, double zr,
                                          double zi) {
-  for (int i = j + lane; i < m; i += kWave) {
+  for (int i = j + lig; i < m; i += kCqrGroup) {
     const double a = i == j ? 1.0 : (double)vr[i], b = i == j ? 0.0 : (double)vi[i];
     xr[i] = (R)((double)xr[i] - (a * zr - b * zi));
     xi[i] = (R)((double)xi[i] - (a * zi + b * zr));
   }
 }
 
-// w = v^H x over the rows i >= j (v_j = 1), summed in FP64 over the wave; every lane gets it.
+// w = v^H x over the rows i >= j (v_j = 1), summed in FP64 over the group; every lane of the group
+// gets it.
 template <typename R>
-__device__ __forceinline__ void cqr_dot(const R *xr, const R *xi, const R *vr, const R *vi, int j, int m, int lane,
+__device__ __forceinline__ void cqr_dot(const R *xr, const R *xi, const R *vr, const R *vi, int j, int m, int lig,
                                         double &wr, double &wi) {
   double sr = 0.0, si = 0.0;
-  for (int i = j + lane; i < m; i += kWave) {
+  for (int i = j + lig; i < m; i += kCqrGroup) {
     const double a = i == j ? 1.0 : (double)vr[i], b = i == j ? 0.0 : (double)vi[i];
     const double x = xr[i], y = xi[i];
     sr = fma(a, x, fma(b, y, sr));    // conj(v) x = (a - ib)(x + iy)
     si = fma(a, y, fma(-b, x, si));
   }
-  wr = wave_sum(sr);
-  wi = wave_sum(si);
+  wr = group_sum_dpp<kCqrGroup>(sr);
+  wi = group_sum_dpp<kCqrGroup>(si);
 }
 
 template <typename R>
 __global__ __launch_bounds__(kCqrThreads) void cqr_lds_kernel(const R *a, R *q, R *r, int m, int n) {
   extern __shared__ double smem_cq[];
   __shared__ double red[kCqrThreads / kWave];
-  constexpr int NW = kCqrThreads / kWave;
   const int ld = cqr_ld(m);
   double *taur = smem_cq, *taui = taur + n;
   R *Ar = reinterpret_cast<R *>(taui + n), *Ai = Ar + (size_t)n * ld;
   R *Qr = Ai + (size_t)n * ld, *Qi = Qr + (size_t)n * ld;
-  const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+  constexpr int NG = kCqrThreads / kCqrGroup;   // column groups in the workgroup
+  const int tid = threadIdx.x, grp = tid / kCqrGroup, lig = tid - grp * kCqrGroup;
   const int64_t b = blockIdx.x;
   const R *A = a + b * 2 * (int64_t)m * n;
   for (int e = tid; e < m * n; e += kCqrThreads) {
@@ -104,12 +109,12 @@ __global__ __launch_bounds__(kCqrThreads) void cqr_lds_kernel(const R *a, R *q, 
       taur[j] = tr;
       taui[j] = ti;
     }
-    if (!trivial) {   // A(j:, c) <- A(j:, c) - conj(tau) v (v^H A(j:, c)), c > j, one wave per column
-      for (int c = j + 1 + wave; c < n; c += NW) {
+    if (!trivial) {   // A(j:, c) <- A(j:, c) - conj(tau) v (v^H A(j:, c)), c > j, one group per column
+      for (int c = j + 1 + grp; c < n; c += NG) {
         R *xr = Ar + c * ld, *xi = Ai + c * ld;
         double wr, wi;
-        cqr_dot(xr, xi, vr, vi, j, m, lane, wr, wi);
-        cqr_axpy(xr, xi, vr, vi, j, m, lane, tr * wr + ti * wi, tr * wi - ti * wr);   // conj(tau) w
+        cqr_dot(xr, xi, vr, vi, j, m, lig, wr, wi);
+        cqr_axpy(xr, xi, vr, vi, j, m, lig, tr * wr + ti * wi, tr * wi - ti * wr);   // conj(tau) w
       }
     }
     __syncthreads();
@@ -125,11 +130,11 @@ __global__ __launch_bounds__(kCqrThreads) void cqr_lds_kernel(const R *a, R *q, 
     const double tr = taur[j], ti = taui[j];
     if (tr != 0.0 || ti != 0.0) {
       const R *vr = Ar + j * ld, *vi = Ai + j * ld;
-      for (int c = j + wave; c < n; c += NW) {
+      for (int c = j + grp; c < n; c += NG) {
         R *xr = Qr + c * ld, *xi = Qi + c * ld;
         double wr, wi;
-        cqr_dot(xr, xi, vr, vi, j, m, lane, wr, wi);
-        cqr_axpy(xr, xi, vr, vi, j, m, lane, tr * wr - ti * wi, tr * wi + ti * wr);   // tau w
+        cqr_dot(xr, xi, vr, vi, j, m, lig, wr, wi);
+        cqr_axpy(xr, xi, vr, vi, j, m, lig, tr * wr - ti * wi, tr * wi + ti * wr);   // tau w
       }
     }
     __syncthreads();

@@ -156,6 +156,29 @@ def test_complex_qr_solver_matches_oracle_and_lapack(pkg, case):
     assert abs(info['jacobi_sweeps'] - oinfo['jacobi_sweeps']) <= 1
 
 
+def test_complex_qr_solver_zero_rank_deficient_and_scaled(pkg):
+    """Zero input (every reflector trivial, sigma = 0, completed basis), a rank-2 input (columns
+    past the rank cut and completed) and a 1e30-scaled input (max-modulus scaling before the sums)
+    through svd(a, 'qr'), against LAPACK."""
+    rng = np.random.default_rng(17)
+    z = np.zeros((1, 12, 5), dtype=np.complex128)
+    u, s, vh = pkg.svd(torch.from_numpy(z).to(DEV), 'qr')
+    assert float(_np(s).max()) == 0.0
+    uu = _np(u)[0]
+    assert np.abs(uu.conj().T @ uu - np.eye(5)).max() < 1e-12
+    f = rng.standard_normal((30, 2)) + 1j * rng.standard_normal((30, 2))
+    g = rng.standard_normal((2, 9)) + 1j * rng.standard_normal((2, 9))
+    lr = (f @ g)[None].astype(np.complex64)
+    u, s, vh = pkg.svd(torch.from_numpy(lr).to(DEV), 'qr')
+    ref = np.linalg.svd(lr[0].astype(np.complex128), compute_uv=False)
+    assert np.abs(_np(s)[0] - ref).max() <= 1e-5 * ref[0]
+    assert np.abs(_np(u)[0].conj().T @ _np(u)[0] - np.eye(9)).max() < 1e-5
+    big = (1e30 * (rng.standard_normal((1, 20, 7)) + 1j * rng.standard_normal((1, 20, 7)))).astype(np.complex128)
+    u, s, vh = pkg.svd(torch.from_numpy(big).to(DEV), 'qr')
+    ref = np.linalg.svd(big[0], compute_uv=False)
+    assert np.abs(_np(s)[0] - ref).max() <= 1e-12 * ref[0]
+
+
 @pytest.mark.parametrize('dtype', [np.complex64, np.complex128])
 def test_complex_qr_solver_random_batch(pkg, dtype):
     rng = np.random.default_rng(5)
