@@ -166,7 +166,7 @@ int psvd_qr(int dtype, int gemm_mode, int64_t batch, int m, int n, const void *a
  * returns PSVD_ETOOBIG when it does not fit (16 m n bytes for complex64, 32 m n for complex128).
  * psvd_cmatmul: c = a b for each matrix of the batch (a: m x k, b: k x n, c: m x n, contiguous,
  * interleaved complex, FP64 sums), the product q @ u of the reference's qr solver
- * (svdbench/standalone/solvers.py:245-248); batch <= 65535.
+ * (svdbench/standalone/solvers.py:245-248).
  * ------------------------------------------------------------------------------------- */
 int psvd_cqr_lds_smem_bytes(int dtype, int m, int n, size_t *bytes);
 int psvd_cqr_lds(int dtype, int64_t batch, int m, int n, const void *a, void *q, void *r, psvd_stream_t stream);

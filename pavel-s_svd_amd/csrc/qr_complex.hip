@@ -214,20 +214,27 @@ int psvd_cqr_lds(int dtype, int64_t batch, int m, int n, const void *a, void *q,
 
 int psvd_cmatmul(int dtype, int64_t batch, int m, int k, int n, const void *a, const void *b, void *c,
                  psvd_stream_t stream) {
-  if (batch < 0 || batch > 65535 || m < 1 || k < 1 || n < 1 || !a || !b || !c) return PSVD_EINVAL;
+  if (batch < 0 || m < 1 || k < 1 || n < 1 || !a || !b || !c) return PSVD_EINVAL;
+  if (dtype != PSVD_C64 && dtype != PSVD_C128) return PSVD_EINVAL;
   if ((int64_t)m * n > 0x7fffffff - 255) return PSVD_ETOOBIG;
-  if (batch == 0) return PSVD_OK;
   hipStream_t st = static_cast<hipStream_t>(stream);
-  const dim3 grid((unsigned)(((int64_t)m * n + 255) / 256), (unsigned)batch);
-  if (dtype == PSVD_C64)
-    hipLaunchKernelGGL(psvd::cmatmul_kernel<float>, grid, dim3(256), 0, st, static_cast<const float *>(a),
-                       static_cast<const float *>(b), static_cast<float *>(c), m, k, n);
-  else if (dtype == PSVD_C128)
-    hipLaunchKernelGGL(psvd::cmatmul_kernel<double>, grid, dim3(256), 0, st, static_cast<const double *>(a),
-                       static_cast<const double *>(b), static_cast<double *>(c), m, k, n);
-  else
-    return PSVD_EINVAL;
-  return hipGetLastError() == hipSuccess ? PSVD_OK : PSVD_ELAUNCH;
+  const size_t es = dtype == PSVD_C64 ? 2 * sizeof(float) : 2 * sizeof(double);
+  // the grid's y dimension carries the matrix index: launches of at most 65535 matrices
+  for (int64_t b0 = 0; b0 < batch; b0 += 65535) {
+    const int64_t nb = batch - b0 < 65535 ? batch - b0 : 65535;
+    const dim3 grid((unsigned)(((int64_t)m * n + 255) / 256), (unsigned)nb);
+    const char *ap = static_cast<const char *>(a) + (size_t)b0 * m * k * es;
+    const char *bp = static_cast<const char *>(b) + (size_t)b0 * k * n * es;
+    char *cp = static_cast<char *>(c) + (size_t)b0 * m * n * es;
+    if (dtype == PSVD_C64)
+      hipLaunchKernelGGL(psvd::cmatmul_kernel<float>, grid, dim3(256), 0, st, reinterpret_cast<const float *>(ap),
+                         reinterpret_cast<const float *>(bp), reinterpret_cast<float *>(cp), m, k, n);
+    else
+      hipLaunchKernelGGL(psvd::cmatmul_kernel<double>, grid, dim3(256), 0, st, reinterpret_cast<const double *>(ap),
+                         reinterpret_cast<const double *>(bp), reinterpret_cast<double *>(cp), m, k, n);
+    if (hipGetLastError() != hipSuccess) return PSVD_ELAUNCH;
+  }
+  return PSVD_OK;
 }
 
 }  // extern "C"

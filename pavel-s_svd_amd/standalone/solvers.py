@@ -52,6 +52,7 @@ ALGORITHMS = (
     'gram_jacobi', 'gram_dc', 'polar_qdwh', 'polar_newton_schulz',
     'polar_halley', 'polar_polynomial', 'randomized', 'subspace', 'block_krylov', 'lanczos',
 )
+MAX_LAUNCH_BATCH = 65535   # matrices per launch of the kernels indexed by the grid's y dimension
 SUPPORTED = ('jacobi', 'qr', 'tsqr', 'polar_qdwh', 'block_jacobi', 'gram_jacobi', 'randomized', 'subspace',
              'block_krylov')
 PARTIAL = ('randomized', 'subspace', 'block_krylov')
@@ -556,6 +557,21 @@ def svd(a, algorithm='jacobi', *, full_matrices=False, values_only=False, return
         raise Unsupported('full_matrices completion of complex bases is not built for MI355X')
     if a.device.type != 'cuda':
         raise Unsupported('the MI355X solvers take ROCm device tensors; there is no CPU path')
+    if algorithm != 'jacobi' and x.shape[0] > MAX_LAUNCH_BATCH:
+        # the batched kernels of the other solvers index matrices by the grid's y dimension
+        # (at most 65 535): solve in slices of that many matrices and join the results
+        outs = [svd(x[i:i + MAX_LAUNCH_BATCH], algorithm, full_matrices=full_matrices, values_only=values_only,
+                    return_info=True, **opts) for i in range(0, x.shape[0], MAX_LAUNCH_BATCH)]
+        s = torch.cat([o[1] for o in outs])
+        result = (None if values_only else torch.cat([o[0] for o in outs]).reshape(*shape, *outs[0][0].shape[-2:]),
+                  s.reshape(*shape, s.shape[-1]),
+                  None if values_only else torch.cat([o[2] for o in outs]).reshape(*shape, *outs[0][2].shape[-2:]))
+        stats = dict(outs[0][3])
+        for o in outs[1:]:
+            for k, v in o[3].items():
+                if isinstance(v, (int, float)) and not isinstance(v, bool) and isinstance(stats.get(k), (int, float)):
+                    stats[k] = max(stats[k], v)
+        return (*result, stats) if return_info else result
     stats = {'algorithm': algorithm, 'vendor_factorizations': 0, 'converged': True}
     m, n = x.shape[-2:]
     with torch.no_grad():

@@ -179,6 +179,18 @@ def test_complex_qr_solver_zero_rank_deficient_and_scaled(pkg):
     assert np.abs(_np(s)[0] - ref).max() <= 1e-12 * ref[0]
 
 
+def test_complex_qr_batch_beyond_one_launch_of_the_product(pkg):
+    """70 000 complex64 6 x 3 matrices through svd(a, 'qr'): the product U = Q U_R is launched in
+    chunks of at most 65 535 matrices; every matrix's sigma against LAPACK."""
+    rng = np.random.default_rng(23)
+    a = (rng.standard_normal((70000, 6, 3)) + 1j * rng.standard_normal((70000, 6, 3))).astype(np.complex64)
+    u, s, vh = pkg.svd(torch.from_numpy(a).to(DEV), 'qr')
+    ref = np.linalg.svd(a.astype(np.complex128), compute_uv=False)
+    assert np.abs(_np(s) - ref).max() <= 1e-5 * ref.max()
+    rec = _np(u) @ (_np(s)[:, :, None] * _np(vh))
+    assert np.abs(rec - a).max() <= 1e-5 * np.abs(a).max()
+
+
 @pytest.mark.parametrize('dtype', [np.complex64, np.complex128])
 def test_complex_qr_solver_random_batch(pkg, dtype):
     rng = np.random.default_rng(5)

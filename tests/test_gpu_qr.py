@@ -79,3 +79,18 @@ def test_qr_bf16x3_panel_updates(prims):
     q, r = q.cpu().numpy().astype(np.float64), r.cpu().numpy().astype(np.float64)
     assert np.max(np.abs(q @ r - a)) / np.abs(a).max() < 3e-4
     assert np.max(np.abs(q[0].T @ q[0] - np.eye(96))) < 3e-4
+
+
+@pytest.mark.parametrize('method', ['qr', 'polar_qdwh'])
+def test_solvers_take_batches_beyond_one_launch(method):
+    """70 000 float32 6 x 3 matrices: batches past the 65 535 matrices one launch of the batched
+    utility and QR kernels indexes are solved in slices; every matrix's sigma against LAPACK."""
+    from conftest import load_package
+    pkg = load_package()
+    rng = np.random.default_rng(29)
+    a = rng.standard_normal((70000, 6, 3)).astype(np.float32)
+    u, s, vh, info = pkg.svd(torch.from_numpy(a).to(DEV), method, return_info=True)
+    ref = np.linalg.svd(a.astype(np.float64), compute_uv=False)
+    assert tuple(s.shape) == (70000, 3) and tuple(u.shape) == (70000, 6, 3) and tuple(vh.shape) == (70000, 3, 3)
+    assert np.abs(s.cpu().numpy() - ref).max() <= 2e-5 * ref.max()
+    assert info['algorithm'] == method
