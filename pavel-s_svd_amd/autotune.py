@@ -3,10 +3,13 @@
 Tunes, on the actual batch, the launch-shape policy -- the waves per CU the resident matrices
 should supply (which sets each matrix's workgroup size from its LDS footprint) and the minimum
 rows per lane that bounds the lanes per column pair; together they fix the size classes the
-batch is split into.  Two stages, both on captured-graph replays (the path a caller replays)
-timed with HIP events on the current stream: every candidate over a few replays, then the
-fastest few again in interleaved rounds, ranked by their median replay time -- single short
-timings differ by a few percent from run to run, as much as the candidates differ.
+batch is split into -- and then, class by class, the workgroup size of each class.  Three
+stages, all on captured-graph replays (the path a caller replays) timed with HIP events on the
+current stream: every policy candidate over a few replays; the fastest few again in interleaved
+rounds, ranked by their median replay time (single short timings differ by a few percent from
+run to run, as much as the candidates differ); then each class, largest first, at twice and half
+its workgroup size, a change kept when it beats the current setting, timed in alternation with
+it, by more than 3 %.
 Results are cached in-process per (dtype, max_m, max_n, batch size bucket, vectors) and, when
 ``PAVELSVD_TUNE_CACHE`` names a JSON file, across processes.
 """
@@ -66,7 +69,42 @@ def time_replays(batch, reps=8):
     return [e0.elapsed_time(e1) for e0, e1 in ev]
 
 
-def tune_uneven(batch, candidates=None, reps=3, verbose=None, finalists=4, rounds=2):
+def _median_pair(batch, a, b, rounds):
+    """Replay medians of configurations a and b, timed in alternation."""
+    sa, sb = [], []
+    for _ in range(rounds):
+        batch.configure(**a)
+        sa += time_replays(batch)
+        batch.configure(**b)
+        sb += time_replays(batch)
+    return sorted(sa)[len(sa) // 2], sorted(sb)[len(sb) // 2]
+
+
+def _tune_classes(batch, policy, rounds, verbose, key, gain=0.97):
+    """Second stage: with the policy fixed, try each size class (largest cost first) at twice and
+    half its workgroup size; keep a change when its replay median beats the current one by more
+    than 1 - gain (the two timed in alternation).  Returns the class_threads map and its median."""
+    batch.configure(**policy, class_threads={})
+    cur = {}
+    base = None
+    for cl in sorted(batch.classes, key=lambda c: -c['cost']):
+        for nt in (cl['threads'] * 2, cl['threads'] // 2):
+            if nt < 64 or nt > 1024:
+                continue
+            trial = dict(cur, **{cl['name']: nt})
+            m_cur, m_trial = _median_pair(batch, dict(policy, class_threads=cur), dict(policy, class_threads=trial),
+                                          rounds)
+            if verbose:
+                print(f'autotune {key} class {cl["name"]} -> {nt}: {m_trial:.3f} vs {m_cur:.3f} ms', file=sys.stderr,
+                      flush=True)
+            base = m_cur if base is None else base
+            if m_trial < gain * m_cur:
+                cur, base = trial, m_trial
+                break
+    return cur, base
+
+
+def tune_uneven(batch, candidates=None, reps=3, verbose=None, finalists=4, rounds=2, class_stage=True):
     """Return the fastest launch parameters for ``batch`` (an UnevenBatch) and leave the batch
     configured with them."""
     if verbose is None:
@@ -96,10 +134,12 @@ def tune_uneven(batch, candidates=None, reps=3, verbose=None, finalists=4, round
             samples[i] += time_replays(batch)
     medians = [sorted(x)[len(x) // 2] for x in samples]
     i_best = min(range(len(top)), key=lambda i: medians[i])
-    best, ms = top[i_best], medians[i_best]
+    best, ms = dict(top[i_best]), medians[i_best]
     if verbose:
         for cand, med in zip(top, medians):
             print(f'autotune {key} replay {cand}: median {med:.3f} ms', file=sys.stderr, flush=True)
+    if class_stage:
+        best['class_threads'], ms = _tune_classes(batch, best, rounds, verbose, key)
     batch.configure(**{k: v for k, v in saved.items() if k not in best}, **best)
     _CACHE[key] = {'params': best, 'ms': ms, 'timings': [[t, c] for t, c in timings],
                    'replay_medians': [[med, c] for med, c in zip(medians, top)]}

@@ -92,7 +92,10 @@ class UnevenBatch:
                                                       0 if vectors else N.NO_VECTORS)
         N.check(st, f'uneven batch with max shape {self.max_m}x{self.max_n}')
         self.graph = None
-        self.params = dict(tol=None, max_sweeps=60, scale=True, cut='reference', streams=4, **DEFAULT_POLICY)
+        # class_threads: per size class ("threads,lanes" of the policy) a workgroup size replacing the
+        # policy's, as the autotuner's second stage picks it
+        self.params = dict(tol=None, max_sweeps=60, scale=True, cut='reference', streams=4, class_threads={},
+                           **DEFAULT_POLICY)
         self.side = []
         self._plan()
 
@@ -104,14 +107,19 @@ class UnevenBatch:
         for i, (m, n) in enumerate(zip(self.ms, self.ns)):
             groups.setdefault(launch_shape(self.code, m, n, self.vectors, self.params), []).append(i)
         order, classes = [], []
+        overrides = self.params.get('class_threads') or {}
         for (threads, lanes) in sorted(groups, reverse=True):
             idx = sorted(groups[(threads, lanes)], key=lambda i: -cost[i])
             mm = max(self.ms[i] for i in idx)
             nn = max(self.ns[i] for i in idx)
             st, _ = N._hip.jacobi_lds_smem_bytes(self.code, mm, nn, 0 if self.vectors else N.NO_VECTORS)
             N.check(st, f'size class with max shape {mm}x{nn}')
-            classes.append(dict(threads=threads, lanes=lanes, start=len(order), count=len(idx), max_m=mm, max_n=nn,
-                                cost=sum(cost[i] for i in idx)))
+            name = f'{threads},{lanes}'
+            nt = int(overrides.get(name, threads))
+            if nt not in (64, 128, 256, 512, 1024):
+                raise ValueError(f'class_threads[{name!r}] must be 64, 128, 256, 512 or 1024')
+            classes.append(dict(threads=nt, lanes=lanes, name=name, start=len(order), count=len(idx), max_m=mm,
+                                max_n=nn, cost=sum(cost[i] for i in idx)))
             order.extend(idx)
         # classes run concurrently on side streams, greedily balanced by total cost
         nstreams = max(1, min(int(self.params['streams']), len(classes)))
@@ -136,7 +144,8 @@ class UnevenBatch:
         unknown = set(params) - set(self.params)
         if unknown:
             raise ValueError(f'unknown parameters {sorted(unknown)}')
-        replan = any((k in DEFAULT_POLICY or k == 'streams') and v != self.params[k] for k, v in params.items())
+        replan = any((k in DEFAULT_POLICY or k in ('streams', 'class_threads')) and v != self.params[k]
+                     for k, v in params.items())
         self.params.update(params)
         if replan:
             self._plan()

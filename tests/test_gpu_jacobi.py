@@ -170,6 +170,40 @@ def test_uneven_graph_replay_matches_eager(pkg):
         batch.check()
 
 
+def test_autotuner_and_class_workgroup_overrides(pkg, monkeypatch):
+    """The autotuner (two policy candidates, then the per-class stage) leaves the batch configured
+    with valid launch shapes, and an explicit per-class workgroup size changes only the launch: sigma
+    against LAPACK, and against the default shape to 1e-5 of sigma_1."""
+    from importlib import import_module
+    B = import_module('pavel-s_svd_amd.standalone.batched')
+    T = import_module('pavel-s_svd_amd.autotune')
+    monkeypatch.delenv('PAVELSVD_TUNE_CACHE', raising=False)
+    monkeypatch.setattr(T, '_CACHE', {})
+    flat, ms, ns, _ = X.workload_uneven_batch(count=300, seed=31)
+    batch = B.UnevenBatch(torch.from_numpy(flat).to(DEV), ms, ns)
+    best = T.tune_uneven(batch, candidates=T.CANDIDATES[:2], finalists=2, rounds=1)
+    assert set(best['class_threads']) <= {c['name'] for c in batch.classes}
+    assert all(c['threads'] in (64, 128, 256, 512, 1024) for c in batch.classes)
+    batch.configure(class_threads={})
+    batch.solve()
+    s_default = batch.s.cpu().numpy().copy()
+    big = max(batch.classes, key=lambda c: c['cost'])
+    other = big['threads'] * 2 if big['threads'] < 1024 else big['threads'] // 2
+    batch.configure(class_threads={big['name']: other})
+    assert [c['threads'] for c in batch.classes if c['name'] == big['name']] == [other]
+    batch.solve()
+    batch.check()
+    s_over = batch.s.cpu().numpy()
+    for i, (m, n) in enumerate(zip(ms, ns)):
+        so, sd = s_over[batch.s_off[i]:batch.s_off[i] + n], s_default[batch.s_off[i]:batch.s_off[i] + n]
+        ai = flat[batch.a_off[i]:batch.a_off[i] + m * n].reshape(m, n).astype(np.float64)
+        ref = np.linalg.svd(ai, compute_uv=False)
+        assert np.abs(so - ref).max() <= 1e-5 * ref[0]
+        assert np.abs(so - sd).max() <= 1e-5 * ref[0]
+    with pytest.raises(ValueError):
+        batch.configure(class_threads={big['name']: 96})
+
+
 def test_factory_plugin_hook(pkg):
     a = torch.from_numpy(corpus('square32')).to(DEV)
     run = pkg.factory(a, {'name': 'pavel-s_svd_amd:factory', 'method': 'jacobi'})
