@@ -243,7 +243,8 @@ __device__ void jacobi_sweeps(T *W, T *V, int m, int np, int ldw, int ldv, bool 
 // same order as in jacobi_sweeps, so both give the same bits.
 template <typename T, int NT, int L, int S>
 __device__ void jacobi_sweeps_fixed(T *W, T *V, int np, int ldw, int ldv, bool vectors, double tol, double eps,
-                                    int max_sweeps, double *rslot, int *fslot, int &sweeps_out, bool &conv_out) {
+                                    int max_sweeps, double *rslot, int *fslot, int *ptab, int &sweeps_out,
+                                    bool &conv_out) {
   static_assert(L <= 16, "the zero padding covers groups of at most 16 lanes");
   constexpr int NG = NT / L, NW = NT / kWave;
   const int tid = threadIdx.x, wave = tid / kWave, lane = tid & (kWave - 1);
@@ -251,6 +252,15 @@ __device__ void jacobi_sweeps_fixed(T *W, T *V, int np, int ldw, int ldv, bool v
   const int npairs = np / 2;
   const int rcv = (np + L - 1) / L;
   const double tol2 = tol * tol, eps2 = eps * eps;
+  // the round's pairs come from a double-buffered table, ptab[slot * npairs + k] = p | q << 16:
+  // threads k < npairs write the next round's entries during a round and its closing barrier
+  // publishes them, so a pass reads one word instead of evaluating the tournament closed form
+  for (int k = tid; k < npairs; k += NT) {
+    int p, q;
+    rr_pair(0, k, np, p, q);
+    ptab[k] = p | (q << 16);
+  }
+  __syncthreads();
   int sweeps = 0;
   bool converged = false;
   int g = 0;
@@ -259,10 +269,11 @@ __device__ void jacobi_sweeps_fixed(T *W, T *V, int np, int ldw, int ldv, bool v
     bool wave_active = false;
     for (int r = 0; r < np - 1; ++r, ++g) {
       const double floor_ = eps2 * slots_max<NW>(rslot + (g & 1) * NW);
+      const int *tab = ptab + (g & 1) * npairs;
       double mymax = 0.0;
       for (int k = grp; k < npairs; k += NG) {
-        int p, q;
-        rr_pair(r, k, np, p, q);
+        const int pq = tab[k];
+        const int p = pq & 0xffff, q = pq >> 16;
         // column offsets stay far below 2^24: 24-bit multiplies (full rate, unlike v_mul_lo_u32)
         T *wp = W + __umul24(p, ldw) + lig, *wq = W + __umul24(q, ldw) + lig;
         T xs[S], ys[S];
@@ -320,6 +331,14 @@ __device__ void jacobi_sweeps_fixed(T *W, T *V, int np, int ldw, int ldv, bool v
         }
         mymax = fmax(mymax, fmax(nap, naq));
       }
+      {  // the next round's pairs (round 0 of the next sweep after the last)
+        const int rn = r + 1 < np - 1 ? r + 1 : 0;
+        for (int k = tid; k < npairs; k += NT) {
+          int p, q;
+          rr_pair(rn, k, np, p, q);
+          ptab[((g + 1) & 1) * npairs + k] = p | (q << 16);
+        }
+      }
       mymax = wave_max_fast(mymax);
       if (lane == 0) rslot[((g + 1) & 1) * NW + wave] = mymax;
       __syncthreads();
@@ -344,7 +363,7 @@ __device__ void jacobi_sweeps_fixed(T *W, T *V, int np, int ldw, int ldv, bool v
 // general ones.  m is uniform over the workgroup, so the switch is a scalar branch.
 template <typename T, int NT, int L>
 __device__ void run_sweeps(T *W, T *V, int m, int np, int ldw, int ldv, bool vectors, double tol, double eps,
-                           int max_sweeps, double *rslot, int *fslot, int &sweeps_out, bool &conv_out) {
+                           int max_sweeps, double *rslot, int *fslot, int *ptab, int &sweeps_out, bool &conv_out) {
   if constexpr (L <= 16) {
     constexpr int RC = sizeof(T) == 4 ? 12 : 4;   // row slots a lane can keep in registers
     const int rc = (m + L - 1) / L;
@@ -352,7 +371,7 @@ __device__ void run_sweeps(T *W, T *V, int m, int np, int ldw, int ldv, bool vec
   if constexpr (SV <= RC)                                                                                    \
     if (rc == SV)                                                                                            \
       return jacobi_sweeps_fixed<T, NT, L, SV>(W, V, np, ldw, ldv, vectors, tol, eps, max_sweeps, rslot, fslot, \
-                                               sweeps_out, conv_out);
+                                               ptab, sweeps_out, conv_out);
     PSVD_FIXED_CASE(1) PSVD_FIXED_CASE(2) PSVD_FIXED_CASE(3) PSVD_FIXED_CASE(4)
     PSVD_FIXED_CASE(5) PSVD_FIXED_CASE(6) PSVD_FIXED_CASE(7) PSVD_FIXED_CASE(8)
     PSVD_FIXED_CASE(9) PSVD_FIXED_CASE(10) PSVD_FIXED_CASE(11) PSVD_FIXED_CASE(12)
@@ -412,8 +431,8 @@ void jacobi_lds_kernel(LdsParams<T> P) {
       continue;  // uniform across the block: every thread read the same descriptor
     }
     const int np = pad_even(n), ldw = lds_ld(m), ldv = lds_ld(np), npairs = np / 2;
-    double *dots = smem_d;
-    double *sig = dots + 3 * npairs;
+    int *ptab = reinterpret_cast<int *>(smem_d);   // 2 npairs pair words (room for 6 npairs)
+    double *sig = smem_d + 3 * npairs;
     double *ytmp = sig + np;
     double *vt = ytmp + np;
     T *W = reinterpret_cast<T *>(vt + m);
@@ -484,7 +503,8 @@ void jacobi_lds_kernel(LdsParams<T> P) {
     // ---- sweeps
     int sweeps = 0;
     bool converged = false;
-    run_sweeps<T, NT, L>(W, V, m, np, ldw, ldv, vectors, tol, eps, P.max_sweeps, s_rslot, s_fslot, sweeps, converged);
+    run_sweeps<T, NT, L>(W, V, m, np, ldw, ldv, vectors, tol, eps, P.max_sweeps, s_rslot, s_fslot, ptab, sweeps,
+                         converged);
     int status = converged ? PSVD_INFO_OK : PSVD_INFO_NOCONV;
 
     // ---- sigma in FP64, descending order, cut
