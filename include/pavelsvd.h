@@ -104,7 +104,9 @@ int psvd_gemm(int mode, int transa, int transb, int64_t m, int64_t n, int64_t k,
  *  psvd_reduce: op 0 out[b] = max|x_b|; op 1 out[b] = ||x_b||_F; op 2 out[b] = max|x_b - y_b| and
  *               out[batch + b] = max|x_b| (out: device doubles); op 3 out[b] = the largest modulus
  *               of the (re, im) pairs of each row (cols even: a complex matrix seen as real, the
- *               reference's x.abs().amax() for complex x).  The max-abs and Frobenius
+ *               reference's x.abs().amax() for complex x); op 4 the same two outputs as op 2 with
+ *               moduli of (re, im) pairs (cols even: (x - y).abs().amax() and x.abs().amax() of
+ *               complex x, y).  The max-abs and Frobenius
  *               normalisations of solvers.py:294-295 and :124-125, the QDWH test :161-163.
  *  psvd_scale:  y_b = x_b / d_b (invert, T division as the reference's x / scale) or
  *               y_b = (s * d_b) x_b; d_b <= 0 counts as 1; d may be NULL.
@@ -164,14 +166,18 @@ int psvd_qr(int dtype, int gemm_mode, int64_t batch, int m, int n, const void *a
  * and r (batch, n, n, upper triangular with a real diagonal) are contiguous, row-major, with
  * interleaved (re, im) pairs; m >= n.  psvd_cqr_lds_smem_bytes gives the LDS the shape needs and
  * returns PSVD_ETOOBIG when it does not fit (16 m n bytes for complex64, 32 m n for complex128).
- * psvd_cmatmul: c = a b for each matrix of the batch (a: m x k, b: k x n, c: m x n, contiguous,
- * interleaved complex, FP64 sums), the product q @ u of the reference's qr solver
- * (svdbench/standalone/solvers.py:245-248).
+ * psvd_cmatmul: c = op(a) op(b) for each matrix of the batch (op(a): m x k, a stored m x k, or
+ * k x m with transa = 1 for a^H; op(b): k x n, b stored k x n, or n x k with transb = 1 for b^H;
+ * c: m x n; contiguous, interleaved complex, FP64 sums): the complex products of the reference's
+ * qr solver (q @ u, solvers.py:245-248) and QDWH iteration (top @ bot^H, x^H @ a, x @ v,
+ * solvers.py:151-153, 168-173).
+ * psvd_chermitize: a <- (a + a^H) / 2 for each n x n complex matrix (solvers.py:169).
  * ------------------------------------------------------------------------------------- */
 int psvd_cqr_lds_smem_bytes(int dtype, int m, int n, size_t *bytes);
 int psvd_cqr_lds(int dtype, int64_t batch, int m, int n, const void *a, void *q, void *r, psvd_stream_t stream);
-int psvd_cmatmul(int dtype, int64_t batch, int m, int k, int n, const void *a, const void *b, void *c,
-                 psvd_stream_t stream);
+int psvd_cmatmul(int dtype, int transa, int transb, int64_t batch, int m, int k, int n, const void *a, const void *b,
+                 void *c, psvd_stream_t stream);
+int psvd_chermitize(int dtype, int64_t batch, int n, void *a, psvd_stream_t stream);
 
 /* ---------------------------------------------------------------------------------------
  * Two-sided cyclic Jacobi eigensolver of a batch of symmetric n x n matrices (primitives.py:

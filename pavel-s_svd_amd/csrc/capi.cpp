@@ -273,11 +273,20 @@ static PyObject *py_cqr_lds(PyObject *, PyObject *args) {
 }
 
 static PyObject *py_cmatmul(PyObject *, PyObject *args) {
-  int dtype, m, k, n;
+  int dtype, ta, tb, m, k, n;
   long long batch;
   unsigned long long a, b, c, stream;
-  if (!PyArg_ParseTuple(args, "iLiiiKKKK", &dtype, &batch, &m, &k, &n, &a, &b, &c, &stream)) return nullptr;
-  return PyLong_FromLong(psvd_cmatmul(dtype, batch, m, k, n, PTR(a), PTR(b), PTR(c), PTR(stream)));
+  if (!PyArg_ParseTuple(args, "iiiLiiiKKKK", &dtype, &ta, &tb, &batch, &m, &k, &n, &a, &b, &c, &stream))
+    return nullptr;
+  return PyLong_FromLong(psvd_cmatmul(dtype, ta, tb, batch, m, k, n, PTR(a), PTR(b), PTR(c), PTR(stream)));
+}
+
+static PyObject *py_chermitize(PyObject *, PyObject *args) {
+  int dtype, n;
+  long long batch;
+  unsigned long long a, stream;
+  if (!PyArg_ParseTuple(args, "iLiKK", &dtype, &batch, &n, &a, &stream)) return nullptr;
+  return PyLong_FromLong(psvd_chermitize(dtype, batch, n, PTR(a), PTR(stream)));
 }
 
 static PyObject *py_block_jacobi_workspace_bytes(PyObject *, PyObject *args) {
@@ -393,6 +402,7 @@ static PyMethodDef kMethods[] = {
     {"cqr_lds_smem_bytes", py_cqr_lds_smem_bytes, METH_VARARGS, nullptr},
     {"cqr_lds", py_cqr_lds, METH_VARARGS, nullptr},
     {"cmatmul", py_cmatmul, METH_VARARGS, nullptr},
+    {"chermitize", py_chermitize, METH_VARARGS, nullptr},
     {"block_jacobi_workspace_bytes", py_block_jacobi_workspace_bytes, METH_VARARGS, nullptr},
     {"block_jacobi_sweep", py_block_jacobi_sweep, METH_VARARGS, nullptr},
     {"mlp_param_count", py_mlp_param_count, METH_NOARGS, nullptr},

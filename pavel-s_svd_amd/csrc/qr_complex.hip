@@ -152,25 +152,50 @@ __global__ __launch_bounds__(kCqrThreads) void cqr_lds_kernel(const R *a, R *q, 
   }
 }
 
-// C (m x n) = A (m x k) B (k x n) for each matrix of the batch, row-major interleaved complex,
-// one thread per entry of C with the sum in FP64.
+// C (m x n) = op(A) op(B) for each matrix of the batch, row-major interleaved complex, one thread
+// per entry of C with the sum in FP64.  op(A) is A (stored m x k) or, with ta, A^H (A stored
+// k x m); op(B) is B (stored k x n) or, with tb, B^H (B stored n x k).
 template <typename R>
-__global__ __launch_bounds__(256) void cmatmul_kernel(const R *a, const R *bm, R *c, int m, int k, int n) {
+__global__ __launch_bounds__(256) void cmatmul_kernel(const R *a, const R *bm, R *c, int m, int k, int n, int ta,
+                                                      int tb) {
   const int64_t bb = blockIdx.y;
   const int e = blockIdx.x * 256 + threadIdx.x;
   if (e >= m * n) return;
   const int i = e / n, j = e - i * n;
-  const R *A = a + bb * 2 * (int64_t)m * k + 2 * (int64_t)i * k;
-  const R *B = bm + bb * 2 * (int64_t)k * n + 2 * j;
+  const R *A = a + bb * 2 * (int64_t)m * k;
+  const R *B = bm + bb * 2 * (int64_t)k * n;
+  // element t of row i of op(A) and of column j of op(B): (offset, stride) in complex entries
+  const int64_t a0 = ta ? i : (int64_t)i * k, as = ta ? m : 1;
+  const int64_t b0 = tb ? (int64_t)j * k : j, bs = tb ? 1 : n;
+  const double ca = ta ? -1.0 : 1.0, cb = tb ? -1.0 : 1.0;   // conjugation signs of the imaginary parts
   double sr = 0.0, si = 0.0;
   for (int t = 0; t < k; ++t) {
-    const double xr = A[2 * t], xi = A[2 * t + 1], yr = B[2 * (int64_t)t * n], yi = B[2 * (int64_t)t * n + 1];
+    const double xr = A[2 * (a0 + t * as)], xi = ca * A[2 * (a0 + t * as) + 1];
+    const double yr = B[2 * (b0 + t * bs)], yi = cb * B[2 * (b0 + t * bs) + 1];
     sr = fma(xr, yr, fma(-xi, yi, sr));
     si = fma(xr, yi, fma(xi, yr, si));
   }
   R *C = c + bb * 2 * (int64_t)m * n;
   C[2 * e] = (R)sr;
   C[2 * e + 1] = (R)si;
+}
+
+// a <- (a + a^H) / 2 for each n x n complex matrix of the batch (contiguous, interleaved), one
+// thread per entry pair (i <= j); the diagonal becomes real.
+template <typename R>
+__global__ __launch_bounds__(256) void chermitize_kernel(R *a, int n) {
+  const int64_t bb = blockIdx.y;
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (e >= n * n) return;
+  const int i = e / n, j = e - i * n;
+  if (i > j) return;
+  R *A = a + bb * 2 * (int64_t)n * n;
+  R *x = A + 2 * ((int64_t)i * n + j), *y = A + 2 * ((int64_t)j * n + i);
+  const R re = (x[0] + y[0]) / R(2), im = (x[1] - y[1]) / R(2);
+  x[0] = re;
+  x[1] = im;
+  y[0] = re;
+  y[1] = -im;
 }
 
 template <typename R>
@@ -212,9 +237,10 @@ int psvd_cqr_lds(int dtype, int64_t batch, int m, int n, const void *a, void *q,
   return PSVD_EINVAL;
 }
 
-int psvd_cmatmul(int dtype, int64_t batch, int m, int k, int n, const void *a, const void *b, void *c,
-                 psvd_stream_t stream) {
+int psvd_cmatmul(int dtype, int transa, int transb, int64_t batch, int m, int k, int n, const void *a, const void *b,
+                 void *c, psvd_stream_t stream) {
   if (batch < 0 || m < 1 || k < 1 || n < 1 || !a || !b || !c) return PSVD_EINVAL;
+  if ((transa != 0 && transa != 1) || (transb != 0 && transb != 1)) return PSVD_EINVAL;
   if (dtype != PSVD_C64 && dtype != PSVD_C128) return PSVD_EINVAL;
   if ((int64_t)m * n > 0x7fffffff - 255) return PSVD_ETOOBIG;
   hipStream_t st = static_cast<hipStream_t>(stream);
@@ -228,10 +254,31 @@ int psvd_cmatmul(int dtype, int64_t batch, int m, int k, int n, const void *a, c
     char *cp = static_cast<char *>(c) + (size_t)b0 * m * n * es;
     if (dtype == PSVD_C64)
       hipLaunchKernelGGL(psvd::cmatmul_kernel<float>, grid, dim3(256), 0, st, reinterpret_cast<const float *>(ap),
-                         reinterpret_cast<const float *>(bp), reinterpret_cast<float *>(cp), m, k, n);
+                         reinterpret_cast<const float *>(bp), reinterpret_cast<float *>(cp), m, k, n, transa,
+                         transb);
     else
       hipLaunchKernelGGL(psvd::cmatmul_kernel<double>, grid, dim3(256), 0, st, reinterpret_cast<const double *>(ap),
-                         reinterpret_cast<const double *>(bp), reinterpret_cast<double *>(cp), m, k, n);
+                         reinterpret_cast<const double *>(bp), reinterpret_cast<double *>(cp), m, k, n, transa,
+                         transb);
+    if (hipGetLastError() != hipSuccess) return PSVD_ELAUNCH;
+  }
+  return PSVD_OK;
+}
+
+int psvd_chermitize(int dtype, int64_t batch, int n, void *a, psvd_stream_t stream) {
+  if (batch < 0 || n < 1 || !a) return PSVD_EINVAL;
+  if (dtype != PSVD_C64 && dtype != PSVD_C128) return PSVD_EINVAL;
+  if ((int64_t)n * n > 0x7fffffff - 255) return PSVD_ETOOBIG;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const size_t es = dtype == PSVD_C64 ? 2 * sizeof(float) : 2 * sizeof(double);
+  for (int64_t b0 = 0; b0 < batch; b0 += 65535) {
+    const int64_t nb = batch - b0 < 65535 ? batch - b0 : 65535;
+    const dim3 grid((unsigned)(((int64_t)n * n + 255) / 256), (unsigned)nb);
+    char *ap = static_cast<char *>(a) + (size_t)b0 * n * n * es;
+    if (dtype == PSVD_C64)
+      hipLaunchKernelGGL(psvd::chermitize_kernel<float>, grid, dim3(256), 0, st, reinterpret_cast<float *>(ap), n);
+    else
+      hipLaunchKernelGGL(psvd::chermitize_kernel<double>, grid, dim3(256), 0, st, reinterpret_cast<double *>(ap), n);
     if (hipGetLastError() != hipSuccess) return PSVD_ELAUNCH;
   }
   return PSVD_OK;

@@ -41,6 +41,12 @@ __global__ __launch_bounds__(kUtilThreads) void reduce_kernel(const T *x, int64_
     const double a = (double)x[b * sx + (int64_t)r * ldx + c];
     if (OP == 0) v0 = fmax(v0, fabs(a));
     if (OP == 3 && !(c & 1)) v0 = fmax(v0, hypot(a, (double)x[b * sx + (int64_t)r * ldx + c + 1]));
+    if (OP == 4 && !(c & 1)) {
+      const double ai = (double)x[b * sx + (int64_t)r * ldx + c + 1];
+      const double dr = (double)y[b * sy + (int64_t)r * ldy + c], di = (double)y[b * sy + (int64_t)r * ldy + c + 1];
+      v0 = fmax(v0, hypot(a - dr, ai - di));
+      v1 = fmax(v1, hypot(a, ai));
+    }
     if (OP == 1) v0 = fma(a, a, v0);
     if (OP == 2) {
       const double d = (double)y[b * sy + (int64_t)r * ldy + c];
@@ -52,14 +58,14 @@ __global__ __launch_bounds__(kUtilThreads) void reduce_kernel(const T *x, int64_
     v0 = group_sum_dpp<kWave>(v0);
   } else {
     v0 = wave_max_fast(v0);
-    if (OP == 2) v1 = wave_max_fast(v1);
+    if (OP == 2 || OP == 4) v1 = wave_max_fast(v1);
   }
   if ((threadIdx.x & (kWave - 1)) == 0) {
     if (OP == 1) {
       atomicAdd(reinterpret_cast<double *>(&out[b]), v0);
     } else {
       atomicMax(&out[b], (unsigned long long)__double_as_longlong(v0));
-      if (OP == 2) atomicMax(&out[batch + b], (unsigned long long)__double_as_longlong(v1));
+      if (OP == 2 || OP == 4) atomicMax(&out[batch + b], (unsigned long long)__double_as_longlong(v1));
     }
   }
 }
@@ -125,7 +131,7 @@ template <typename T>
 static int run_reduce(int op, int64_t batch, int rows, int cols, const void *x, int64_t ldx, int64_t sx,
                       const void *y, int64_t ldy, int64_t sy, double *out, hipStream_t st) {
   unsigned long long *o = reinterpret_cast<unsigned long long *>(out);
-  const int64_t nout = op == 2 ? 2 * batch : batch;
+  const int64_t nout = (op == 2 || op == 4) ? 2 * batch : batch;
   int e = fill_u64(o, nout, 0ull, st);  // the bit pattern of +0.0
   if (e != PSVD_OK) return e;
   const dim3 grid(grid_chunks((int64_t)rows * cols), (unsigned)batch);
@@ -133,7 +139,8 @@ static int run_reduce(int op, int64_t batch, int rows, int cols, const void *x, 
   if (op == 0) hipLaunchKernelGGL((reduce_kernel<T, 0>), grid, dim3(kUtilThreads), 0, st, xp, ldx, sx, yp, ldy, sy, rows, cols, o, batch);
   else if (op == 1) hipLaunchKernelGGL((reduce_kernel<T, 1>), grid, dim3(kUtilThreads), 0, st, xp, ldx, sx, yp, ldy, sy, rows, cols, o, batch);
   else if (op == 2) hipLaunchKernelGGL((reduce_kernel<T, 2>), grid, dim3(kUtilThreads), 0, st, xp, ldx, sx, yp, ldy, sy, rows, cols, o, batch);
-  else hipLaunchKernelGGL((reduce_kernel<T, 3>), grid, dim3(kUtilThreads), 0, st, xp, ldx, sx, yp, ldy, sy, rows, cols, o, batch);
+  else if (op == 3) hipLaunchKernelGGL((reduce_kernel<T, 3>), grid, dim3(kUtilThreads), 0, st, xp, ldx, sx, yp, ldy, sy, rows, cols, o, batch);
+  else hipLaunchKernelGGL((reduce_kernel<T, 4>), grid, dim3(kUtilThreads), 0, st, xp, ldx, sx, yp, ldy, sy, rows, cols, o, batch);
   if (op == 1) hipLaunchKernelGGL(sqrt_kernel, dim3((unsigned)((batch + 255) / 256)), dim3(256), 0, st, out, batch);
   return hipGetLastError() == hipSuccess ? PSVD_OK : PSVD_ELAUNCH;
 }
@@ -188,10 +195,10 @@ extern "C" {
 
 int psvd_reduce(int dtype, int op, int64_t batch, int rows, int cols, const void *x, int64_t ldx, int64_t sx,
                 const void *y, int64_t ldy, int64_t sy, double *out, psvd_stream_t stream) {
-  if (batch < 0 || rows < 0 || cols < 0 || op < 0 || op > 3 || !out || batch > 65535) return PSVD_EINVAL;
-  if (op == 3 && (cols & 1)) return PSVD_EINVAL;
+  if (batch < 0 || rows < 0 || cols < 0 || op < 0 || op > 4 || !out || batch > 65535) return PSVD_EINVAL;
+  if (op >= 3 && (cols & 1)) return PSVD_EINVAL;
   if (batch == 0) return PSVD_OK;
-  if (!x || (op == 2 && !y)) return PSVD_EINVAL;
+  if (!x || ((op == 2 || op == 4) && !y)) return PSVD_EINVAL;
   hipStream_t st = static_cast<hipStream_t>(stream);
   if (dtype == PSVD_F32) return psvd::run_reduce<float>(op, batch, rows, cols, x, ldx, sx, y, ldy, sy, out, st);
   if (dtype == PSVD_F64) return psvd::run_reduce<double>(op, batch, rows, cols, x, ldx, sx, y, ldy, sy, out, st);

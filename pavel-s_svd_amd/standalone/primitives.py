@@ -156,14 +156,24 @@ def cqr(A):
     return q, r
 
 
-def cmatmul(a, b):
-    """Batched complex product a (B, m, k) @ b (B, k, n) on the device (FP64 sums)."""
+def cmatmul(a, b, *, ha=False, hb=False):
+    """Batched complex product op(a) @ op(b) on the device (FP64 sums); op(x) is x, or x^H when ha /
+    hb is set (a is then stored (B, k, m), b (B, n, k))."""
     a, b = a.contiguous(), b.contiguous()
-    B, m, k = a.shape
-    c = torch.empty(B, m, b.shape[-1], dtype=a.dtype, device=a.device)
-    N.check(N._hip.cmatmul(N.dtype_code(a.dtype), B, m, k, b.shape[-1], N.ptr(a), N.ptr(b), N.ptr(c),
+    B = a.shape[0]
+    m, k = (a.shape[2], a.shape[1]) if ha else (a.shape[1], a.shape[2])
+    n = b.shape[1] if hb else b.shape[2]
+    c = torch.empty(B, m, n, dtype=a.dtype, device=a.device)
+    N.check(N._hip.cmatmul(N.dtype_code(a.dtype), int(ha), int(hb), B, m, k, n, N.ptr(a), N.ptr(b), N.ptr(c),
                            N.stream_of(a)), 'psvd_cmatmul')
     return c
+
+
+def hermitize_(a):
+    """a <- (a + a^H) / 2 in place for a contiguous complex (b, n, n) device tensor."""
+    N.check(N._hip.chermitize(N.dtype_code(a.dtype), a.shape[0], a.shape[-1], N.ptr(a), N.stream_of(a)),
+            'psvd_chermitize')
+    return a
 
 
 def qr(A, *, gemm='f32', want_q=True):
@@ -205,25 +215,28 @@ def _mat3(x):
 
 def reduce(x, op, y=None):
     """Per-matrix reduction of (b, r, c) device tensors into FP64 device scalars:
-    'maxabs' -> (b,), 'fro' -> (b,), 'maxdiff' -> (b, 2) = [max|x-y|, max|x|]; a complex x takes
-    'maxabs' as its largest modulus."""
+    'maxabs' -> (b,), 'fro' -> (b,), 'maxdiff' -> (b, 2) = [max|x-y|, max|x|]; complex inputs take
+    moduli for 'maxabs' and 'maxdiff' ('fro' is the same sum of squares over the real view)."""
+    def real_view(t):
+        tr = torch.view_as_real(t.contiguous())
+        return tr.reshape(*tr.shape[:-2], tr.shape[-2] * 2)
     if x.is_complex():
-        if op != 'maxabs':
-            raise ValueError('complex inputs take the maxabs reduction only')
-        xr = torch.view_as_real(x.contiguous())
-        x, op = xr.reshape(*xr.shape[:-2], xr.shape[-2] * 2), 'maxmod'
+        op = {'maxabs': 'maxmod', 'maxdiff': 'maxdiffmod', 'fro': 'fro'}[op]
+        x = real_view(x)
+        y = None if y is None else real_view(y)
     x3, ldx, sx = _mat3(x)
     b, rows, cols = x3.shape
-    code = {'maxabs': 0, 'fro': 1, 'maxdiff': 2, 'maxmod': 3}[op]
-    out = torch.empty(2 * b if code == 2 else b, dtype=torch.float64, device=x.device)
-    if code == 2:
+    code = {'maxabs': 0, 'fro': 1, 'maxdiff': 2, 'maxmod': 3, 'maxdiffmod': 4}[op]
+    pair = code in (2, 4)
+    out = torch.empty(2 * b if pair else b, dtype=torch.float64, device=x.device)
+    if pair:
         y3, ldy, sy = _mat3(y)
     else:
         y3, ldy, sy = None, 0, 0
     st = N._hip.reduce(N.dtype_code(x.dtype), code, b, rows, cols, N.ptr(x3), ldx, sx, N.ptr(y3), ldy, sy,
                        N.ptr(out), N.stream_of(x))
     N.check(st, 'psvd_reduce')
-    return out.view(2, b).mT if code == 2 else out
+    return out.view(2, b).mT if pair else out
 
 
 def scale(x, d=None, *, invert=False, s=1.0, out=None):

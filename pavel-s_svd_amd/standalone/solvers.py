@@ -472,6 +472,62 @@ def _polar(a, stats, opts):
     return u, s, v
 
 
+def _polar_complex(a, stats, opts):
+    """QDWH polar SVD of a complex tall batch held to LDS sizes (solvers.py:122-178 with the complex
+    arithmetic of primitives.py): X = A / ||A||_F, dynamically weighted Halley steps
+    X <- (b/c) X + (a - b/c) / sqrt(c) Q1 Q2^H from the QR of [sqrt(c) X; I] -- every step in this
+    QR form, which the reference takes while c > 100 and which equals its Cholesky form
+    X (I + c X^H X)^-1 in exact arithmetic -- stopping on max|X_new - X| <= tol max(max|X|, 1) over
+    the batch; then H = sym(X^H A), whose eigenpairs (H is positive semidefinite) are its singular
+    pairs, from the complex one-sided Jacobi kernel; U = X V."""
+    b, m, n = a.shape
+    real = torch.float32 if a.dtype == torch.complex64 else torch.float64
+    x = P.scale(a, P.reduce(a, 'fro'), invert=True)
+    tol = opts.get('tol') or 10 * EPS[real]
+    lower = opts.get('lower_bound', EPS[real])
+    if not 0 < lower <= 1:
+        raise ValueError('lower_bound must be in (0,1]')
+    max_iter = opts.get('max_iter', 40)
+    ident = torch.zeros(b, n, n, dtype=a.dtype, device=a.device)
+    ident.diagonal(dim1=-2, dim2=-1).fill_(1)
+    converged = False
+    for it in range(max_iter):
+        aa, bb, cc, lower = qdwh_weights(lower)
+        q, _ = P.cqr(torch.cat([P.scale(x, None, s=math.sqrt(cc)), ident], -2))
+        xn = P.cmatmul(q[:, :m], q[:, m:], hb=True)
+        _axpby_complex(bb / cc, x, (aa - bb / cc) / math.sqrt(cc), xn)
+        d = P.reduce(xn, 'maxdiff', x).max(0).values.cpu()
+        x = xn
+        if float(d[0]) <= tol * max(float(d[1]), 1.0):
+            converged = True
+            break
+    if not converged:
+        raise RuntimeError(f'polar_qdwh did not converge in {max_iter} iterations')
+    h = P.cmatmul(x, a, ha=True)
+    P.hermitize_(h)
+    _, w, vh, info = P.jacobi_lds_uniform(h, tol=None, max_sweeps=opts.get('max_sweeps', 60), cut='floor')
+    info = info.cpu()
+    from .. import native as N
+    N.raise_for_info(info, f"Hermitian Jacobi did not converge in {opts.get('max_sweeps', 60)} sweeps")
+    keep = w > EPS[real] * max(m, n) * w[:, :1]
+    if not bool(keep.all()):
+        raise Unsupported('complex polar_qdwh of a rank-deficient matrix needs complex basis completion, '
+                          'which is not built for MI355X')
+    u = P.cmatmul(x, vh, hb=True)
+    fixed = P.reduce(P.cmatmul(x, P.cmatmul(x, x, ha=True)), 'maxdiff', x)[:, 0].max()
+    stats.update(polar_iterations=it + 1, polar_qr_iterations=it + 1, eigen_sweeps=int(info[:, 0].max()),
+                 polar_fixed_point_error=float(fixed))
+    return u, w, vh
+
+
+def _axpby_complex(alpha, x, beta, y):
+    """y <- alpha x + beta y for complex x, y with real alpha, beta (through the real views)."""
+    xr = torch.view_as_real(x.contiguous())
+    yr = torch.view_as_real(y)
+    P.axpby_(alpha, xr.reshape(*xr.shape[:-2], -1), beta, yr.reshape(*yr.shape[:-2], -1))
+    return y
+
+
 def _raw(x, method, stats, opts):
     """Wide inputs are solved transposed (solvers.py:226-228) except by the partial methods; dispatch
     by name (solvers.py:229-261)."""
@@ -507,6 +563,8 @@ def _raw(x, method, stats, opts):
             vh = _newton_schulz(vh.mT.double()).mT.float().contiguous()
         return u, s, vh
     if method == 'polar_qdwh':
+        if x.is_complex():
+            return _polar_complex(x, stats, opts)
         u, s, v = _polar(x, stats, opts)
         return u, s, v.mT
     raise Unsupported(f'{method!r} is not built for MI355X yet')
@@ -542,8 +600,13 @@ def svd(a, algorithm='jacobi', *, full_matrices=False, values_only=False, return
         raise ValueError('input contains NaN or infinity')
     if algorithm not in SUPPORTED:
         raise Unsupported(f'{algorithm!r} is outside the MI355X hot path (built: {SUPPORTED})')
-    if a.is_complex() and algorithm not in ('jacobi', 'qr', 'tsqr'):
-        raise Unsupported(f'complex {algorithm!r} is not built for MI355X (complex inputs: jacobi, qr, tsqr)')
+    if a.is_complex() and algorithm not in ('jacobi', 'qr', 'tsqr', 'polar_qdwh'):
+        raise Unsupported(f'complex {algorithm!r} is not built for MI355X (complex inputs: jacobi, qr, tsqr, '
+                          'polar_qdwh)')
+    if a.is_complex() and algorithm == 'polar_qdwh' and not (
+            P.cqr_fits(a.dtype, sum(a.shape[-2:]), min(a.shape[-2:])) and
+            P.lds_fits(a.dtype, min(a.shape[-2:]), min(a.shape[-2:]), True)):
+        raise Unsupported('complex polar_qdwh is built for matrices whose [sqrt(c) X; I] fits in LDS only')
     if a.is_complex() and algorithm == 'tsqr' and not _complex_tsqr_fits(a.dtype, max(a.shape[-2:]),
                                                                           min(a.shape[-2:]), opts.get('tile_rows')):
         raise Unsupported('complex TSQR is built for leaves that fit in LDS only')

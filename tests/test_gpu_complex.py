@@ -226,13 +226,54 @@ def test_complex_tsqr_matches_oracle(pkg, dtype, shape, rows):
             assert met[key] <= bar, (key, met[key])
 
 
+@pytest.mark.parametrize('case', ['clustered_complex', 'wide_complex'])
+def test_complex_qdwh_matches_reference_golden(pkg, expected, case):
+    """svd(a, 'polar_qdwh') on the complex corpus: QDWH iterations within one of the reference's
+    recorded count (its late steps take the Cholesky form, every step here the QR form, the same
+    iterate in exact arithmetic), sigma against the oracle's solver, metrics within 3x of the
+    reference's recorded ones (or the data type's floor)."""
+    a = corpus(case)
+    print(f'complex qdwh {case} {a.shape} {a.dtype}', flush=True)
+    u, s, vh, info = pkg.svd(torch.from_numpy(a).to(DEV), 'polar_qdwh', return_info=True)
+    met = _metrics(a, (u, s, vh))
+    runs = [r for r in expected[case]['runs'] if r['method'] == 'polar_qdwh']
+    assert runs
+    for run in runs:
+        assert abs(info['polar_iterations'] - run['diagnostics']['polar_iterations']) <= 1, run['source']
+        floor = 64 * np.finfo(a.real.dtype).eps * max(a.shape[-2:]) ** 0.5
+        for key in KEYS:
+            assert met[key] <= max(3 * run['metrics'][key], floor), (key, met[key], run['metrics'][key])
+    _, so, _, oinfo = R.svd(a, 'polar_qdwh', return_info=True)
+    bar = 1e-5 if a.dtype == np.complex64 else 1e-12
+    assert np.abs(_np(s) - so).max() <= bar * so.max()
+    assert abs(info['polar_iterations'] - oinfo['polar_iterations']) <= 1
+
+
+@pytest.mark.parametrize('dtype,shape', [(np.complex64, (3, 40, 17)), (np.complex128, (2, 48, 48)),
+                                         (np.complex64, (1, 30, 62)), (np.complex128, (1, 64, 8))])
+def test_complex_qdwh_random_matches_lapack(pkg, dtype, shape):
+    rng = np.random.default_rng(shape[1] * 3 + shape[2])
+    a = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(dtype)
+    print(f'complex qdwh random {shape} {np.dtype(dtype).name}', flush=True)
+    u, s, vh, info = pkg.svd(torch.from_numpy(a).to(DEV), 'polar_qdwh', return_info=True)
+    bar = 1e-5 if dtype == np.complex64 else 1e-12
+    for i in range(shape[0]):
+        met = _metrics(a[i], (u[i], s[i], vh[i]))
+        for key in KEYS:
+            assert met[key] <= bar, (key, met[key])
+    real = np.float32 if dtype == np.complex64 else np.float64
+    assert info['polar_fixed_point_error'] <= 100 * np.finfo(real).eps
+
+
 def test_complex_unsupported_paths(pkg):
     x = torch.zeros(1, 20, 10, dtype=torch.complex64, device=DEV) + 1
     with pytest.raises(pkg.Unsupported):   # complex QR beyond LDS
         pkg.svd(torch.zeros(1, 600, 300, dtype=torch.complex128, device=DEV) + 1, 'qr')
     with pytest.raises(pkg.Unsupported):   # complex TSQR leaves (256 x 64 complex128) beyond LDS
         pkg.svd(torch.zeros(1, 600, 64, dtype=torch.complex128, device=DEV) + 1, 'tsqr')
-    for method in ('polar_qdwh', 'block_jacobi', 'randomized'):
+    with pytest.raises(pkg.Unsupported):   # complex QDWH of a rank-deficient matrix (no complex completion)
+        pkg.svd(torch.zeros(1, 20, 10, dtype=torch.complex128, device=DEV) + 1, 'polar_qdwh')
+    for method in ('block_jacobi', 'randomized'):
         with pytest.raises(pkg.Unsupported):
             pkg.svd(x, method)
     with pytest.raises(pkg.Unsupported):
