@@ -622,7 +622,7 @@ __device__ __forceinline__ void store64(T *Ds, const typename Mfma<T>::V (&acc)[
 
 template <typename T>
 __host__ __device__ inline size_t blk_update_lds() {
-  return sizeof(T) * 3 * (size_t)kEs * kEsLd;
+  return sizeof(T) * 2 * (size_t)kEs * kEsLd;   // the tile and one rotation at a time
 }
 
 // Blocks [0, nG): G block (p, q), p < q, and its mirror.  Blocks [nG, nG + npair * N / 64): the
@@ -634,7 +634,7 @@ template <typename T>
 __global__ __launch_bounds__(256) void blk_update_kernel(BlkParams<T> P, int r, int g) {
   extern __shared__ double smem_bu[];
   T *Bs = reinterpret_cast<T *>(smem_bu);
-  T *Qa = Bs + kEs * kEsLd, *Qb = Qa + kEs * kEsLd;
+  T *Qa = Bs + kEs * kEsLd;   // Q_p, then Q_q: two tiles of LDS (twice the workgroups per CU in FP64)
   using V16 = Vec16<T>;
   constexpr int VE = V16::n, RV = kEs / VE;   // elements per vector, vectors per tile row
   const int npair = P.nb / 2;
@@ -661,31 +661,34 @@ __global__ __launch_bounds__(256) void blk_update_kernel(BlkParams<T> P, int r, 
     }
     const V16 *Qp = reinterpret_cast<const V16 *>(P.Q + (int64_t)p * kEs * kEs);
     const V16 *Qq = reinterpret_cast<const V16 *>(P.Q + (int64_t)q * kEs * kEs);
+    // the tile with Q_p (or, when only q rotated, Q_q) beside it
+    const V16 *Q1 = ap ? Qp : Qq;
     for (int e = tid; e < kEs * RV; e += 256) {
       const int i = e / RV, j = (e - i * RV) * VE;
       const V16 x = *reinterpret_cast<const V16 *>(P.G + (int64_t)blk_row(Ip, Jp, i) * P.N + blk_row(Iq, Jq, j));
+      const V16 y = Q1[e];
 #pragma unroll
-      for (int u = 0; u < VE; ++u) Bs[i * kEsLd + j + u] = x.v[u];
-      if (ap) {
-        const V16 y = Qp[e];
-#pragma unroll
-        for (int u = 0; u < VE; ++u) Qa[i * kEsLd + j + u] = y.v[u];
-      }
-      if (aq) {
-        const V16 y = Qq[e];
-#pragma unroll
-        for (int u = 0; u < VE; ++u) Qb[i * kEsLd + j + u] = y.v[u];
+      for (int u = 0; u < VE; ++u) {
+        Bs[i * kEsLd + j + u] = x.v[u];
+        Qa[i * kEsLd + j + u] = y.v[u];
       }
     }
     __syncthreads();
     if (ap) {  // B <- Q_p^T B
       mm64<T, false>(Qa, Bs, acc, wm, wn, lr, lk);
-      __syncthreads();
+      __syncthreads();   // every wave has read Q_p and B
       store64<T>(Bs, acc, wm, wn, lr, lk);
+      if (aq)
+        for (int e = tid; e < kEs * RV; e += 256) {   // Q_q into Q_p's place
+          const int i = e / RV, j = (e - i * RV) * VE;
+          const V16 y = Qq[e];
+#pragma unroll
+          for (int u = 0; u < VE; ++u) Qa[i * kEsLd + j + u] = y.v[u];
+        }
       __syncthreads();
     }
     if (aq) {  // B <- B Q_q
-      mm64<T, true>(Bs, Qb, acc, wm, wn, lr, lk);
+      mm64<T, true>(Bs, Qa, acc, wm, wn, lr, lk);
       __syncthreads();
       store64<T>(Bs, acc, wm, wn, lr, lk);
       __syncthreads();
