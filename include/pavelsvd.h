@@ -172,12 +172,18 @@ int psvd_qr(int dtype, int gemm_mode, int64_t batch, int m, int n, const void *a
  * qr solver (q @ u, solvers.py:245-248) and QDWH iteration (top @ bot^H, x^H @ a, x @ v,
  * solvers.py:151-153, 168-173).
  * psvd_chermitize: a <- (a + a^H) / 2 for each n x n complex matrix (solvers.py:169).
+ * psvd_ccomplete_columns: complex counterpart of psvd_complete_columns (primitives.py:430-467):
+ * for each matrix of u (batch, m, n, contiguous interleaved), columns [nk[b], n) are replaced in
+ * order by the projected coordinate vector at the row of largest remaining leverage (two
+ * Gram-Schmidt passes, FP64); status[b] = PSVD_INFO_COMPLETION when a residual falls below 10 eps.
  * ------------------------------------------------------------------------------------- */
 int psvd_cqr_lds_smem_bytes(int dtype, int m, int n, size_t *bytes);
 int psvd_cqr_lds(int dtype, int64_t batch, int m, int n, const void *a, void *q, void *r, psvd_stream_t stream);
 int psvd_cmatmul(int dtype, int transa, int transb, int64_t batch, int m, int k, int n, const void *a, const void *b,
                  void *c, psvd_stream_t stream);
 int psvd_chermitize(int dtype, int64_t batch, int n, void *a, psvd_stream_t stream);
+int psvd_ccomplete_columns(int dtype, int64_t batch, int m, int n, void *u, const int32_t *nk, int32_t *status,
+                           psvd_stream_t stream);
 
 /* ---------------------------------------------------------------------------------------
  * Two-sided cyclic Jacobi eigensolver of a batch of symmetric n x n matrices (primitives.py:

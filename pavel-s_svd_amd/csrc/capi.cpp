@@ -281,6 +281,15 @@ static PyObject *py_cmatmul(PyObject *, PyObject *args) {
   return PyLong_FromLong(psvd_cmatmul(dtype, ta, tb, batch, m, k, n, PTR(a), PTR(b), PTR(c), PTR(stream)));
 }
 
+static PyObject *py_ccomplete_columns(PyObject *, PyObject *args) {
+  int dtype, m, n;
+  long long batch;
+  unsigned long long u, nk, status, stream;
+  if (!PyArg_ParseTuple(args, "iLiiKKKK", &dtype, &batch, &m, &n, &u, &nk, &status, &stream)) return nullptr;
+  return PyLong_FromLong(psvd_ccomplete_columns(dtype, batch, m, n, PTR(u), static_cast<const int32_t *>(PTR(nk)),
+                                                static_cast<int32_t *>(PTR(status)), PTR(stream)));
+}
+
 static PyObject *py_chermitize(PyObject *, PyObject *args) {
   int dtype, n;
   long long batch;
@@ -403,6 +412,7 @@ static PyMethodDef kMethods[] = {
     {"cqr_lds", py_cqr_lds, METH_VARARGS, nullptr},
     {"cmatmul", py_cmatmul, METH_VARARGS, nullptr},
     {"chermitize", py_chermitize, METH_VARARGS, nullptr},
+    {"ccomplete_columns", py_ccomplete_columns, METH_VARARGS, nullptr},
     {"block_jacobi_workspace_bytes", py_block_jacobi_workspace_bytes, METH_VARARGS, nullptr},
     {"block_jacobi_sweep", py_block_jacobi_sweep, METH_VARARGS, nullptr},
     {"mlp_param_count", py_mlp_param_count, METH_NOARGS, nullptr},

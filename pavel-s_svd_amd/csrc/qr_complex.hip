@@ -198,6 +198,115 @@ __global__ __launch_bounds__(256) void chermitize_kernel(R *a, int n) {
   y[1] = -im;
 }
 
+// Completion of missing complex basis columns (primitives.py:430-455): for each matrix, columns
+// [nk, n) of U (m x n, row-major interleaved) are replaced in order by the coordinate vector e_ix
+// at the row of largest remaining leverage 1 - sum_{c<r} |u_ic|^2 (first such row), projected
+// twice as v <- v - U (U^H v) against the columns before it and normalised; a residual below
+// 10 eps sets status PSVD_INFO_COMPLETION.  One workgroup per matrix, v in LDS, FP64.
+constexpr int kCcThreads = 256;
+
+template <typename R>
+__global__ __launch_bounds__(kCcThreads) void ccomplete_kernel(R *u, int m, int n, const int32_t *nk,
+                                                               int32_t *status, double eps) {
+  extern __shared__ double smem_cc[];
+  double *vr = smem_cc, *vi = vr + m, *yr = vi + m, *yi = yr + n;
+  __shared__ double s_best[kCcThreads / kWave];
+  __shared__ int s_bi[kCcThreads / kWave];
+  __shared__ double s_red[kCcThreads / kWave];
+  constexpr int NW = kCcThreads / kWave;
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+  const int64_t b = blockIdx.x;
+  R *U = u + b * 2 * (int64_t)m * n;
+  auto re = [&](int i, int c) -> R & { return U[2 * ((int64_t)i * n + c)]; };
+  auto im = [&](int i, int c) -> R & { return U[2 * ((int64_t)i * n + c) + 1]; };
+  int st = PSVD_INFO_OK;
+  const int k0 = nk[b] < 0 ? 0 : (nk[b] > n ? n : nk[b]);
+  for (int r = k0; r < n; ++r) {
+    // row of largest leverage (ties: the first)
+    double best = -DBL_MAX;
+    int bi = 0x7fffffff;
+    for (int i = tid; i < m; i += kCcThreads) {
+      double acc = 0.0;
+      for (int c = 0; c < r; ++c) {
+        const double x = re(i, c), y = im(i, c);
+        acc = fma(x, x, fma(y, y, acc));
+      }
+      const double lev = 1.0 - acc;
+      if (lev > best || (lev == best && i < bi)) {
+        best = lev;
+        bi = i;
+      }
+    }
+    for (int o = kWave >> 1; o > 0; o >>= 1) {
+      const double ob = __shfl_xor(best, o, kWave);
+      const int oi = __shfl_xor(bi, o, kWave);
+      if (ob > best || (ob == best && oi < bi)) {
+        best = ob;
+        bi = oi;
+      }
+    }
+    if (lane == 0) {
+      s_best[wave] = best;
+      s_bi[wave] = bi;
+    }
+    __syncthreads();
+    best = s_best[0];
+    bi = s_bi[0];
+    for (int w = 1; w < NW; ++w)
+      if (s_best[w] > best || (s_best[w] == best && s_bi[w] < bi)) {
+        best = s_best[w];
+        bi = s_bi[w];
+      }
+    const int ix = wave_uniform(bi);
+    for (int i = tid; i < m; i += kCcThreads) {
+      vr[i] = i == ix ? 1.0 : 0.0;
+      vi[i] = 0.0;
+    }
+    __syncthreads();
+    for (int pass = 0; pass < 2; ++pass) {
+      // y = U^H v over the columns before r, one wave per column
+      for (int c = wave; c < r; c += NW) {
+        double ar = 0.0, ai = 0.0;
+        for (int i = lane; i < m; i += kWave) {
+          const double ur = re(i, c), ui = im(i, c);
+          ar = fma(ur, vr[i], fma(ui, vi[i], ar));
+          ai = fma(ur, vi[i], fma(-ui, vr[i], ai));
+        }
+        ar = wave_sum(ar);
+        ai = wave_sum(ai);
+        if (lane == 0) {
+          yr[c] = ar;
+          yi[c] = ai;
+        }
+      }
+      __syncthreads();
+      // v = v - U y
+      for (int i = tid; i < m; i += kCcThreads) {
+        double ar = vr[i], ai = vi[i];
+        for (int c = 0; c < r; ++c) {
+          const double ur = re(i, c), ui = im(i, c);
+          ar -= ur * yr[c] - ui * yi[c];
+          ai -= ur * yi[c] + ui * yr[c];
+        }
+        vr[i] = ar;
+        vi[i] = ai;
+      }
+      __syncthreads();
+    }
+    double l2 = 0.0;
+    for (int i = tid; i < m; i += kCcThreads) l2 = fma(vr[i], vr[i], fma(vi[i], vi[i], l2));
+    const double len = sqrt(block_sum<kCcThreads>(l2, s_red));
+    if (len < 10.0 * eps) st = PSVD_INFO_COMPLETION;
+    const double inv = 1.0 / fmax(len, DBL_MIN);
+    for (int i = tid; i < m; i += kCcThreads) {
+      re(i, r) = (R)(vr[i] * inv);
+      im(i, r) = (R)(vi[i] * inv);
+    }
+    __syncthreads();
+  }
+  if (tid == 0) status[b] = st;
+}
+
 template <typename R>
 static int run_cqr(int64_t batch, int m, int n, const void *a, void *q, void *r, hipStream_t st) {
   const size_t smem = cqr_lds_bytes<R>(m, n);
@@ -263,6 +372,38 @@ int psvd_cmatmul(int dtype, int transa, int transb, int64_t batch, int m, int k,
     if (hipGetLastError() != hipSuccess) return PSVD_ELAUNCH;
   }
   return PSVD_OK;
+}
+
+int psvd_ccomplete_columns(int dtype, int64_t batch, int m, int n, void *u, const int32_t *nk, int32_t *status,
+                           psvd_stream_t stream) {
+  if (batch < 0 || batch > 0x7fffffff || m < 1 || n < 1 || n > m || !u || !nk || !status) return PSVD_EINVAL;
+  if (dtype != PSVD_C64 && dtype != PSVD_C128) return PSVD_EINVAL;
+  const size_t smem = sizeof(double) * 2 * ((size_t)m + n);
+  if (smem > psvd::kCqrLdsCap) return PSVD_ETOOBIG;
+  if (batch == 0) return PSVD_OK;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (dtype == PSVD_C64) {
+    static bool attr = false;
+    if (!attr) {
+      if (hipFuncSetAttribute((const void *)psvd::ccomplete_kernel<float>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)psvd::kCqrLdsCap) != hipSuccess)
+        return PSVD_ELAUNCH;
+      attr = true;
+    }
+    hipLaunchKernelGGL(psvd::ccomplete_kernel<float>, dim3((unsigned)batch), dim3(psvd::kCcThreads), smem, st,
+                       static_cast<float *>(u), m, n, nk, status, 1.1920928955078125e-07);
+  } else {
+    static bool attr = false;
+    if (!attr) {
+      if (hipFuncSetAttribute((const void *)psvd::ccomplete_kernel<double>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)psvd::kCqrLdsCap) != hipSuccess)
+        return PSVD_ELAUNCH;
+      attr = true;
+    }
+    hipLaunchKernelGGL(psvd::ccomplete_kernel<double>, dim3((unsigned)batch), dim3(psvd::kCcThreads), smem, st,
+                       static_cast<double *>(u), m, n, nk, status, 2.220446049250313e-16);
+  }
+  return hipGetLastError() == hipSuccess ? PSVD_OK : PSVD_ELAUNCH;
 }
 
 int psvd_chermitize(int dtype, int64_t batch, int n, void *a, psvd_stream_t stream) {

@@ -432,6 +432,12 @@ def complete_columns_(u, nk):
     int32 device tensor.  Returns the (b,) status tensor (device, no synchronisation)."""
     b, m, n = u.shape
     status = torch.empty(b, dtype=torch.int32, device=u.device)
+    if u.is_complex():
+        if not u.is_contiguous():
+            raise ValueError('complex completion takes a contiguous (b, m, n) tensor')
+        N.check(N._hip.ccomplete_columns(N.dtype_code(u.dtype), b, m, n, N.ptr(u), N.ptr(nk), N.ptr(status),
+                                         N.stream_of(u)), 'psvd_ccomplete_columns')
+        return status
     wsb = N._hip.complete_workspace_bytes(b, m, n)
     ws = N.workspace(wsb, u.device)
     st = N._hip.complete_columns(N.dtype_code(u.dtype), b, m, n, N.ptr(u), u.stride(-2), u.stride(0) if b > 1 else 0,
@@ -486,8 +492,6 @@ def complete_basis(q, size):
         return q
     if not k <= size <= m:
         raise ValueError('invalid basis size')
-    if q.is_complex():
-        raise N.NativeError('completion of complex bases', N.EINVAL)
     out = torch.zeros(b, m, size, dtype=q.dtype, device=q.device)
     out[:, :, :k] = q
     nk = torch.full((b,), k, dtype=torch.int32, device=q.device)

@@ -510,10 +510,8 @@ def _polar_complex(a, stats, opts):
     from .. import native as N
     N.raise_for_info(info, f"Hermitian Jacobi did not converge in {opts.get('max_sweeps', 60)} sweeps")
     keep = w > EPS[real] * max(m, n) * w[:, :1]
-    if not bool(keep.all()):
-        raise Unsupported('complex polar_qdwh of a rank-deficient matrix needs complex basis completion, '
-                          'which is not built for MI355X')
-    u = P.cmatmul(x, vh, hb=True)
+    u = P.complete_null_columns(P.cmatmul(x, vh, hb=True), keep)
+    w = torch.where(keep, w, torch.zeros_like(w))
     fixed = P.reduce(P.cmatmul(x, P.cmatmul(x, x, ha=True)), 'maxdiff', x)[:, 0].max()
     stats.update(polar_iterations=it + 1, polar_qr_iterations=it + 1, eigen_sweeps=int(info[:, 0].max()),
                  polar_fixed_point_error=float(fixed))
@@ -616,8 +614,6 @@ def svd(a, algorithm='jacobi', *, full_matrices=False, values_only=False, return
     if a.is_complex() and algorithm == 'qr' and not (P.cqr_fits(a.dtype, max(a.shape[-2:]), min(a.shape[-2:])) and
                                                      P.lds_fits(a.dtype, min(a.shape[-2:]), min(a.shape[-2:]), True)):
         raise Unsupported('complex QR is built for matrices that fit in LDS only')
-    if a.is_complex() and full_matrices and a.shape[-2] != a.shape[-1]:
-        raise Unsupported('full_matrices completion of complex bases is not built for MI355X')
     if a.device.type != 'cuda':
         raise Unsupported('the MI355X solvers take ROCm device tensors; there is no CPU path')
     if algorithm != 'jacobi' and x.shape[0] > MAX_LAUNCH_BATCH:

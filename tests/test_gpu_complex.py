@@ -271,12 +271,29 @@ def test_complex_unsupported_paths(pkg):
         pkg.svd(torch.zeros(1, 600, 300, dtype=torch.complex128, device=DEV) + 1, 'qr')
     with pytest.raises(pkg.Unsupported):   # complex TSQR leaves (256 x 64 complex128) beyond LDS
         pkg.svd(torch.zeros(1, 600, 64, dtype=torch.complex128, device=DEV) + 1, 'tsqr')
-    with pytest.raises(pkg.Unsupported):   # complex QDWH of a rank-deficient matrix (no complex completion)
-        pkg.svd(torch.zeros(1, 20, 10, dtype=torch.complex128, device=DEV) + 1, 'polar_qdwh')
     for method in ('block_jacobi', 'randomized'):
         with pytest.raises(pkg.Unsupported):
             pkg.svd(x, method)
     with pytest.raises(pkg.Unsupported):
         pkg.svd(torch.zeros(1, 600, 300, dtype=torch.complex128, device=DEV) + 1, 'jacobi')
-    with pytest.raises(pkg.Unsupported):
-        pkg.svd(x, 'jacobi', full_matrices=True)
+
+
+@pytest.mark.parametrize('method', ['jacobi', 'qr', 'tsqr', 'polar_qdwh'])
+@pytest.mark.parametrize('dtype', [np.complex64, np.complex128])
+def test_complex_full_matrices_and_rank_deficient(pkg, method, dtype):
+    """Rectangular full_matrices (complete_basis on complex bases) and a rank-2 input (columns past
+    the cut completed, primitives.py:430-467) through every complex solver: U and V unitary, sigma
+    against LAPACK, U diag(s) Vh = A."""
+    rng = np.random.default_rng(41)
+    f = rng.standard_normal((1, 24, 2)) + 1j * rng.standard_normal((1, 24, 2))
+    g = rng.standard_normal((1, 2, 9)) + 1j * rng.standard_normal((1, 2, 9))
+    a = (f @ g).astype(dtype)
+    bar = 1e-5 if dtype == np.complex64 else 1e-12
+    u, s, vh = pkg.svd(torch.from_numpy(a).to(DEV), method, full_matrices=True)
+    u, s, vh = _np(u)[0], _np(s)[0], _np(vh)[0]
+    assert u.shape == (24, 24) and vh.shape == (9, 9)
+    ref = np.linalg.svd(a[0].astype(np.complex128), compute_uv=False)
+    assert np.abs(s - ref).max() <= bar * ref[0] * 10
+    assert np.abs(u.conj().T @ u - np.eye(24)).max() <= 10 * bar
+    assert np.abs(vh @ vh.conj().T - np.eye(9)).max() <= 10 * bar
+    assert np.abs((u[:, :9] * s) @ vh - a[0]).max() <= 10 * bar * np.abs(a).max()
