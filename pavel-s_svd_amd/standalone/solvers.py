@@ -130,6 +130,8 @@ def _gram(x, stats, opts):
     on MFMA, eigenvalues descending, sigma = sqrt(w) above eps * n * w_1 (zero below, counted in
     gram_discarded_values), U = A V / sigma on MFMA with the cut columns completed."""
     b, m, n = x.shape
+    if x.is_complex():
+        return _gram_complex(x, stats, opts)
     mode = _gmode(x, opts)
     g = gemm(x.mT, x, mode=mode)
     P.symmetrize_(g)
@@ -149,6 +151,35 @@ def _gram(x, stats, opts):
         if bool((status != 0).any()):
             raise RuntimeError('null-space completion failed: supplied live columns are not independent')
     return u, s, v.mT
+
+
+def _gram_complex(x, stats, opts):
+    """Complex gram_jacobi (solvers.py:99-119) for n x n Gram matrices that fit in LDS: G = sym(A^H A),
+    its eigenpairs from the complex one-sided Jacobi (G is positive semidefinite, so its singular
+    pairs are its eigenpairs, descending; eigen_sweeps counts one-sided sweeps), sigma = sqrt(w) above
+    eps * n * w_1, U = A V / sigma with the cut columns completed."""
+    b, m, n = x.shape
+    real = torch.float32 if x.dtype == torch.complex64 else torch.float64
+    g = P.cmatmul(x, x, ha=True)
+    P.hermitize_(g)
+    _, w, vh, info = P.jacobi_lds_uniform(g, tol=None, max_sweeps=opts.get('max_sweeps', 60), cut='floor')
+    info = info.cpu()
+    from .. import native as N
+    N.raise_for_info(info, f"Hermitian Jacobi did not converge in {opts.get('max_sweeps', 60)} sweeps")
+    stats['eigen_sweeps'] = int(info[:, 0].max())
+    keep = w > EPS[real] * n * w[:, :1]
+    s = torch.where(keep, w.clamp_min(0).sqrt(), torch.zeros_like(w))
+    stats['gram_discarded_values'] = int((~keep).sum())
+    if opts.get('_values_only', False):
+        return None, s, None
+    u = P.cmatmul(x, vh, hb=True)
+    ur = torch.view_as_real(u)
+    P.scale_columns_(ur.reshape(b, m, 2 * n), s.repeat_interleave(2, dim=1).contiguous())
+    if not bool(keep.all()):
+        status = P.complete_columns_(u, keep.sum(-1).to(torch.int32))
+        if bool((status != 0).any()):
+            raise RuntimeError('null-space completion failed: supplied live columns are not independent')
+    return u, s, vh
 
 
 def _sketch(x, columns, seed):
@@ -598,9 +629,12 @@ def svd(a, algorithm='jacobi', *, full_matrices=False, values_only=False, return
         raise ValueError('input contains NaN or infinity')
     if algorithm not in SUPPORTED:
         raise Unsupported(f'{algorithm!r} is outside the MI355X hot path (built: {SUPPORTED})')
-    if a.is_complex() and algorithm not in ('jacobi', 'qr', 'tsqr', 'polar_qdwh'):
+    if a.is_complex() and algorithm not in ('jacobi', 'qr', 'tsqr', 'polar_qdwh', 'gram_jacobi'):
         raise Unsupported(f'complex {algorithm!r} is not built for MI355X (complex inputs: jacobi, qr, tsqr, '
-                          'polar_qdwh)')
+                          'polar_qdwh, gram_jacobi)')
+    if a.is_complex() and algorithm == 'gram_jacobi' and not P.lds_fits(a.dtype, min(a.shape[-2:]),
+                                                                        min(a.shape[-2:]), True):
+        raise Unsupported('complex gram_jacobi is built for Gram matrices that fit in LDS only')
     if a.is_complex() and algorithm == 'polar_qdwh' and not (
             P.cqr_fits(a.dtype, sum(a.shape[-2:]), min(a.shape[-2:])) and
             P.lds_fits(a.dtype, min(a.shape[-2:]), min(a.shape[-2:]), True)):

@@ -265,6 +265,29 @@ def test_complex_qdwh_random_matches_lapack(pkg, dtype, shape):
     assert info['polar_fixed_point_error'] <= 100 * np.finfo(real).eps
 
 
+@pytest.mark.parametrize('case', ['clustered_complex', 'wide_complex'])
+def test_complex_gram_matches_reference_golden(pkg, expected, case):
+    """svd(a, 'gram_jacobi') on the complex corpus: the number of discarded values equals the
+    reference's record; the kept sigma against the oracle's Gram solver; metrics within 3x of the
+    reference's recorded ones (or the data type's floor)."""
+    a = corpus(case)
+    print(f'complex gram {case} {a.shape} {a.dtype}', flush=True)
+    u, s, vh, info = pkg.svd(torch.from_numpy(a).to(DEV), 'gram_jacobi', return_info=True)
+    met = _metrics(a, (u, s, vh))
+    runs = [r for r in expected[case]['runs'] if r['method'] == 'gram_jacobi']
+    assert runs
+    for run in runs:
+        assert info['gram_discarded_values'] == run['diagnostics']['gram_discarded_values'], run['source']
+        floor = 64 * np.finfo(a.real.dtype).eps * max(a.shape[-2:]) ** 0.5
+        for key in ('orthogonality_u_fro_over_sqrt_cols', 'orthogonality_v_fro_over_sqrt_rows',
+                    'reconstruction_relative_fro', 'spectrum_error_over_sigma1'):
+            assert met[key] <= max(3 * run['metrics'][key], floor), (key, met[key], run['metrics'][key])
+    _, so, _, oinfo = R.svd(a, 'gram_jacobi', return_info=True)
+    kept = so > 0
+    bar = 1e-5 if a.dtype == np.complex64 else 1e-12
+    assert np.abs(_np(s)[..., :so.shape[-1]][kept] - so[kept]).max() <= 10 * bar * so.max()
+
+
 def test_complex_unsupported_paths(pkg):
     x = torch.zeros(1, 20, 10, dtype=torch.complex64, device=DEV) + 1
     with pytest.raises(pkg.Unsupported):   # complex QR beyond LDS
