@@ -209,35 +209,55 @@ def _partial(x, stats, opts, method):
         stats['warm_started'] = True
     mode = _gmode(x, opts)
     qmode = 'bf16x3' if mode == 'bf16x3' else 'f32'
+    cplx = x.is_complex()
 
     def orth(y):
         return P.qr(y.contiguous(), gemm=qmode)[0]
 
-    q = orth(gemm(x, omega.contiguous(), mode=mode))
+    def mm(a_, b_, ha=False):
+        """op(a_) @ b_ with op(a_) = a_^H (a_^T for real data) when ha."""
+        if cplx:
+            return P.cmatmul(a_, b_, ha=ha)
+        return gemm(a_.mT if ha else a_, b_, mode=mode)
+
+    q = orth(mm(x, omega.contiguous()))
     bases = [q]
     iterations = opts.get('iterations', 2)
     for _ in range(iterations):
-        z = orth(gemm(x.mT, q, mode=mode))
-        y = gemm(x, z, mode=mode)
+        z = orth(mm(x, q, ha=True))
+        y = mm(x, z)
         if method == 'block_krylov':
             joined = torch.cat(bases, -1)
             available = m - joined.shape[-1]
             if available <= 0:
                 break
             for _ in range(2):
-                y = gemm(joined, gemm(joined.mT, y, mode=mode), alpha=-1.0, beta=1.0, out=y, mode=mode)
+                if cplx:
+                    y = _axpby_complex(-1.0, P.cmatmul(joined, P.cmatmul(joined, y, ha=True)), 1.0, y.contiguous())
+                else:
+                    y = gemm(joined, gemm(joined.mT, y, mode=mode), alpha=-1.0, beta=1.0, out=y, mode=mode)
             q = orth(y[:, :, :min(ell, available)])
             bases.append(q)
         else:
             q = orth(y)
     if method == 'block_krylov':
         q = orth(torch.cat(bases, -1))
-    small = gemm(q.mT, x, mode=mode)
+    small = mm(q, x, ha=True)
     u, s, vh = _raw(small, 'jacobi', stats, opts)
     stats.update(sketch_columns=ell, range_columns=q.shape[-1], power_iterations=iterations)
     if u is None:
         return None, s[:, :k], None
-    return gemm(q, u, mode=mode)[:, :, :k], s[:, :k], vh[:, :k, :]
+    return mm(q, u)[:, :, :k], s[:, :k], vh[:, :k, :]
+
+
+def _complex_partial_fits(dtype, m, n, opts, method):
+    """The QRs of a complex range finder (m x l sketches and ranges, n x l power steps, the joined
+    block Krylov basis) and the Jacobi of Q^H A fit in LDS."""
+    k = opts.get('rank', min(m, n))
+    ell = min(k + opts.get('oversample', 8), min(m, n))
+    cols = min(m, ell * (opts.get('iterations', 2) + 1)) if method == 'block_krylov' else ell
+    return (P.cqr_fits(dtype, m, ell) and P.cqr_fits(dtype, n, ell) and P.cqr_fits(dtype, m, cols) and
+            P.lds_fits(dtype, max(cols, n), min(cols, n), True))
 
 
 def _gmode(x, opts):
@@ -629,9 +649,12 @@ def svd(a, algorithm='jacobi', *, full_matrices=False, values_only=False, return
         raise ValueError('input contains NaN or infinity')
     if algorithm not in SUPPORTED:
         raise Unsupported(f'{algorithm!r} is outside the MI355X hot path (built: {SUPPORTED})')
-    if a.is_complex() and algorithm not in ('jacobi', 'qr', 'tsqr', 'polar_qdwh', 'gram_jacobi'):
+    if a.is_complex() and algorithm not in ('jacobi', 'qr', 'tsqr', 'polar_qdwh', 'gram_jacobi') + PARTIAL:
         raise Unsupported(f'complex {algorithm!r} is not built for MI355X (complex inputs: jacobi, qr, tsqr, '
-                          'polar_qdwh, gram_jacobi)')
+                          'polar_qdwh, gram_jacobi, randomized, subspace, block_krylov)')
+    if a.is_complex() and algorithm in PARTIAL and not _complex_partial_fits(a.dtype, a.shape[-2], a.shape[-1],
+                                                                              opts, algorithm):
+        raise Unsupported(f'complex {algorithm} is built for sketches and ranges that fit in LDS only')
     if a.is_complex() and algorithm == 'gram_jacobi' and not P.lds_fits(a.dtype, min(a.shape[-2:]),
                                                                         min(a.shape[-2:]), True):
         raise Unsupported('complex gram_jacobi is built for Gram matrices that fit in LDS only')

@@ -288,15 +288,47 @@ def test_complex_gram_matches_reference_golden(pkg, expected, case):
     assert np.abs(_np(s)[..., :so.shape[-1]][kept] - so[kept]).max() <= 10 * bar * so.max()
 
 
+@pytest.mark.parametrize('method', ['randomized', 'subspace', 'block_krylov'])
+@pytest.mark.parametrize('case,opts', [('clustered_complex', {}), ('wide_complex', {}),
+                                       ('clustered_complex', {'rank': 6, 'oversample': 4, 'iterations': 2, 'seed': 7})])
+def test_complex_range_finders_match_oracle_on_the_same_sketch(pkg, expected, method, case, opts):
+    """Complex randomized / subspace / block_krylov (solvers.py:185-221) with the oracle's complex
+    sketch passed as initial_basis: column counts equal the oracle's and the reference's CPU rows,
+    sigma against the oracle's, metrics within 3x of the oracle's."""
+    a = corpus(case)
+    print(f'complex {method} {case} {opts}', flush=True)
+    a3 = a[None] if a.ndim == 2 else a
+    k = opts.get('rank', min(a.shape[-2:]))
+    ell = min(k + opts.get('oversample', 8), min(a.shape[-2:]))
+    omega = R.sketch(a3, ell, opts.get('seed', 20261011))
+    ru, rs, rvh, rinfo = R.svd(a, method, return_info=True, **opts)
+    u, s, vh, info = pkg.svd(torch.from_numpy(a).to(DEV), method, return_info=True,
+                             initial_basis=torch.from_numpy(omega[0] if a.ndim == 2 else omega), **opts)
+    for key in ('sketch_columns', 'range_columns', 'power_iterations'):
+        assert info[key] == rinfo[key], key
+    assert abs(info['jacobi_sweeps'] - rinfo['jacobi_sweeps']) <= 1
+    bar = 1e-5 if a.dtype == np.complex64 else 1e-12
+    top = max(float(np.max(rs)), 1e-30)
+    assert np.max(np.abs(_np(s) - rs)) / top <= bar
+    met = _metrics(a, (u, s, vh))
+    ref = _metrics(a, (torch.from_numpy(ru), torch.from_numpy(rs), torch.from_numpy(rvh)))
+    for key in KEYS:
+        assert met[key] <= max(3 * ref[key], 10 * bar), (key, met[key], ref[key])
+    runs = [r for r in expected[case]['runs'] if r['method'] == method and r['device'] == 'cpu'
+            and r['options'] == opts]
+    for run in runs:
+        for key in ('sketch_columns', 'range_columns', 'power_iterations'):
+            assert info[key] == run['diagnostics'][key]
+
+
 def test_complex_unsupported_paths(pkg):
     x = torch.zeros(1, 20, 10, dtype=torch.complex64, device=DEV) + 1
     with pytest.raises(pkg.Unsupported):   # complex QR beyond LDS
         pkg.svd(torch.zeros(1, 600, 300, dtype=torch.complex128, device=DEV) + 1, 'qr')
     with pytest.raises(pkg.Unsupported):   # complex TSQR leaves (256 x 64 complex128) beyond LDS
         pkg.svd(torch.zeros(1, 600, 64, dtype=torch.complex128, device=DEV) + 1, 'tsqr')
-    for method in ('block_jacobi', 'randomized'):
-        with pytest.raises(pkg.Unsupported):
-            pkg.svd(x, method)
+    with pytest.raises(pkg.Unsupported):
+        pkg.svd(x, 'block_jacobi')
     with pytest.raises(pkg.Unsupported):
         pkg.svd(torch.zeros(1, 600, 300, dtype=torch.complex128, device=DEV) + 1, 'jacobi')
 
