@@ -98,6 +98,8 @@ def _block_jacobi(x, stats, opts):
         raise ValueError('block_size must be positive')
     if -(-n // width) == 1:
         return _jacobi(x, stats, opts, scale=False)
+    if x.is_complex():
+        return _block_jacobi_complex(x, stats, opts, width)
     if 2 * width > 64:
         raise Unsupported(f'block_size {width} > 32: the pair Gram matrices of this build are held on chip')
     from .. import native as N
@@ -123,6 +125,75 @@ def _block_jacobi(x, stats, opts):
     u, s = _finish_columns(w, s)
     stats['block_sweeps'] = sweep + 1
     return u, s, v.mT
+
+
+def _block_jacobi_complex(x, stats, opts, width):
+    """Complex block_jacobi (solvers.py:30-68) for group pairs whose m x 2 block_size columns fit in
+    LDS: each tournament round gathers its group pairs into one batch, solves it with the complex
+    one-sided Jacobi kernel (the reference's _jacobi per pair, V's columns by descending sigma) and
+    replaces W_pair, V_pair by W_pair V, V_pair V through the complex product; a sweep ends with the
+    reference's Gram test max |g_ij| / sqrt(g_ii g_jj) over live columns (one host read per sweep)."""
+    from .. import native as N
+    b, m, n = x.shape
+    real = P._real_dtype(x.dtype)
+    groups = [list(range(j, min(j + width, n))) for j in range(0, n, width)]
+    if len(groups) % 2:
+        groups.append([])
+    # the rounds' column sets, grouped by width (full pairs together, the short group apart)
+    rounds = []
+    for pairs in P._round_robin(len(groups)):
+        by_width = {}
+        for i, j in pairs:
+            cols = groups[i] + groups[j]
+            if len(cols) >= 2:
+                by_width.setdefault(len(cols), []).append(cols)
+        rounds.append([(k, torch.tensor(c, device=x.device)) for k, c in by_width.items()])
+    w = x.clone()
+    v = _eye(b, n, x.dtype, x.device)
+    tol = opts.get('tol') or 40 * EPS[real]
+    inner_sweeps = opts.get('max_sweeps', 60)
+    jac, converged, sweep = 0, False, 0
+    for sweep in range(opts.get('max_sweeps', 30)):
+        infos = []
+        for sets in rounds:
+            for k, idx in sets:
+                p = idx.shape[0]
+                wp = w[:, :, idx].permute(0, 2, 1, 3).reshape(b * p, m, k).contiguous()
+                vp = v[:, :, idx].permute(0, 2, 1, 3).reshape(b * p, n, k).contiguous()
+                _, _, vh, info = P.jacobi_lds_uniform(wp, tol=opts.get('tol'), max_sweeps=inner_sweeps)
+                infos.append(info)
+                w[:, :, idx] = P.cmatmul(wp, vh, hb=True).reshape(b, p, m, k).permute(0, 2, 1, 3)
+                v[:, :, idx] = P.cmatmul(vp, vh, hb=True).reshape(b, p, n, k).permute(0, 2, 1, 3)
+        info = torch.cat(infos).cpu()
+        N.raise_for_info(info, f'one-sided Jacobi did not converge in {inner_sweeps} sweeps')
+        jac = max(jac, int(info[:, 0].max()))
+        g = P.cmatmul(w, w, ha=True)
+        d = g.diagonal(dim1=-2, dim2=-1).real.clamp_min(0)
+        sq = d.sqrt()
+        live = d > (EPS[real] * max(m, n)) ** 2 * d.amax(-1, keepdim=True)
+        ga = g.abs()
+        ga.diagonal(dim1=-2, dim2=-1).zero_()
+        corr = torch.where(live[:, :, None] & live[:, None, :],
+                           ga / (sq[:, :, None] * sq[:, None, :]).clamp_min(torch.finfo(real).tiny), 0)
+        if float(corr.max()) <= tol:
+            converged = True
+            break
+    stats['jacobi_sweeps'] = max(stats.get('jacobi_sweeps', 0), jac)
+    if not converged:
+        raise RuntimeError('block Jacobi did not converge')
+    s = w.abs().square().sum(1).sqrt()
+    order = s.argsort(-1, descending=True)
+    s = s.gather(1, order)
+    w = w.gather(2, order[:, None].expand_as(w)).contiguous()
+    v = v.gather(2, order[:, None].expand_as(v))
+    keep = s > EPS[real] * max(m, n) * s[:, :1]
+    P.scale_columns_(torch.view_as_real(w).reshape(b, m, 2 * n), s.repeat_interleave(2, dim=1).contiguous())
+    if not bool(keep.all()):
+        status = P.complete_columns_(w, keep.sum(-1).to(torch.int32))
+        if bool((status != 0).any()):
+            raise RuntimeError('null-space completion failed: supplied live columns are not independent')
+    stats['block_sweeps'] = sweep + 1
+    return w, torch.where(keep, s, torch.zeros_like(s)), v.mH.contiguous()
 
 
 def _gram(x, stats, opts):
@@ -649,9 +720,9 @@ def svd(a, algorithm='jacobi', *, full_matrices=False, values_only=False, return
         raise ValueError('input contains NaN or infinity')
     if algorithm not in SUPPORTED:
         raise Unsupported(f'{algorithm!r} is outside the MI355X hot path (built: {SUPPORTED})')
-    if a.is_complex() and algorithm not in ('jacobi', 'qr', 'tsqr', 'polar_qdwh', 'gram_jacobi') + PARTIAL:
-        raise Unsupported(f'complex {algorithm!r} is not built for MI355X (complex inputs: jacobi, qr, tsqr, '
-                          'polar_qdwh, gram_jacobi, randomized, subspace, block_krylov)')
+    if a.is_complex() and algorithm == 'block_jacobi' and not P.lds_fits(
+            a.dtype, max(a.shape[-2:]), min(min(a.shape[-2:]), 2 * max(1, opts.get('block_size', 4))), True):
+        raise Unsupported('complex block_jacobi is built for group pairs whose columns fit in LDS only')
     if a.is_complex() and algorithm in PARTIAL and not _complex_partial_fits(a.dtype, a.shape[-2], a.shape[-1],
                                                                               opts, algorithm):
         raise Unsupported(f'complex {algorithm} is built for sketches and ranges that fit in LDS only')

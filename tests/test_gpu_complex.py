@@ -288,6 +288,50 @@ def test_complex_gram_matches_reference_golden(pkg, expected, case):
     assert np.abs(_np(s)[..., :so.shape[-1]][kept] - so[kept]).max() <= 10 * bar * so.max()
 
 
+@pytest.mark.parametrize('case', ['clustered_complex', 'wide_complex'])
+def test_complex_block_jacobi_matches_reference_golden(pkg, expected, case):
+    """svd(a, 'block_jacobi') on the complex corpus: block sweeps within one of the reference's
+    record and of the oracle's; sigma against the oracle's; metrics within 3x of the reference's
+    recorded ones (or the data type's floor)."""
+    a = corpus(case)
+    print(f'complex block_jacobi {case} {a.shape} {a.dtype}', flush=True)
+    u, s, vh, info = pkg.svd(torch.from_numpy(a).to(DEV), 'block_jacobi', return_info=True)
+    met = _metrics(a, (u, s, vh))
+    runs = [r for r in expected[case]['runs'] if r['method'] == 'block_jacobi']
+    assert runs
+    for run in runs:
+        assert abs(info['block_sweeps'] - run['diagnostics']['block_sweeps']) <= 1, run['source']
+        floor = 64 * np.finfo(a.real.dtype).eps * max(a.shape[-2:]) ** 0.5
+        for key in KEYS:
+            assert met[key] <= max(3 * run['metrics'][key], floor), (key, met[key], run['metrics'][key])
+    _, so, _, oinfo = R.svd(a, 'block_jacobi', return_info=True)
+    assert abs(info['block_sweeps'] - oinfo['block_sweeps']) <= 1
+    bar = 1e-5 if a.dtype == np.complex64 else 1e-12
+    assert np.abs(_np(s) - so).max() <= 10 * bar * so.max()
+
+
+@pytest.mark.parametrize('dtype', [np.complex64, np.complex128])
+@pytest.mark.parametrize('shape,width', [((3, 40, 24), 4), ((2, 33, 21), 4), ((1, 64, 48), 8), ((2, 20, 20), 3),
+                                         ((5, 16, 9), 2)])
+def test_complex_block_jacobi_random_matches_lapack(pkg, dtype, shape, width):
+    """Complex block_jacobi over even and odd group counts (the empty group of solvers.py:37-38),
+    short last groups and several matrices per launch: sigma against LAPACK, U and V unitary,
+    U diag(s) Vh = A."""
+    rng = np.random.default_rng(sum(shape) + width)
+    a = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(dtype)
+    print(f'complex block_jacobi {shape} {width} {np.dtype(dtype).name}', flush=True)
+    u, s, vh, info = pkg.svd(torch.from_numpy(a).to(DEV), 'block_jacobi', return_info=True, block_size=width)
+    u, s, vh = _np(u), _np(s), _np(vh)
+    bar = 1e-5 if dtype == np.complex64 else 1e-12
+    ref = np.linalg.svd(a.astype(np.complex128), compute_uv=False)
+    assert np.abs(s - ref).max() <= 10 * bar * ref.max()
+    n = shape[-1]
+    assert np.abs(np.conj(np.swapaxes(u, -1, -2)) @ u - np.eye(n)).max() <= 10 * bar
+    assert np.abs(vh @ np.conj(np.swapaxes(vh, -1, -2)) - np.eye(n)).max() <= 10 * bar
+    assert np.abs((u * s[:, None, :]) @ vh - a).max() <= 10 * bar * np.abs(a).max()
+    assert 1 <= info['block_sweeps'] <= 30
+
+
 @pytest.mark.parametrize('method', ['randomized', 'subspace', 'block_krylov'])
 @pytest.mark.parametrize('case,opts', [('clustered_complex', {}), ('wide_complex', {}),
                                        ('clustered_complex', {'rank': 6, 'oversample': 4, 'iterations': 2, 'seed': 7})])
@@ -327,13 +371,15 @@ def test_complex_unsupported_paths(pkg):
         pkg.svd(torch.zeros(1, 600, 300, dtype=torch.complex128, device=DEV) + 1, 'qr')
     with pytest.raises(pkg.Unsupported):   # complex TSQR leaves (256 x 64 complex128) beyond LDS
         pkg.svd(torch.zeros(1, 600, 64, dtype=torch.complex128, device=DEV) + 1, 'tsqr')
+    with pytest.raises(pkg.Unsupported):   # complex128 group pairs of 600 x 64 beyond LDS
+        pkg.svd(torch.zeros(1, 600, 80, dtype=torch.complex128, device=DEV) + 1, 'block_jacobi', block_size=32)
     with pytest.raises(pkg.Unsupported):
-        pkg.svd(x, 'block_jacobi')
+        pkg.svd(x, 'lanczos')
     with pytest.raises(pkg.Unsupported):
         pkg.svd(torch.zeros(1, 600, 300, dtype=torch.complex128, device=DEV) + 1, 'jacobi')
 
 
-@pytest.mark.parametrize('method', ['jacobi', 'qr', 'tsqr', 'polar_qdwh'])
+@pytest.mark.parametrize('method', ['jacobi', 'qr', 'tsqr', 'polar_qdwh', 'block_jacobi'])
 @pytest.mark.parametrize('dtype', [np.complex64, np.complex128])
 def test_complex_full_matrices_and_rank_deficient(pkg, method, dtype):
     """Rectangular full_matrices (complete_basis on complex bases) and a rank-2 input (columns past

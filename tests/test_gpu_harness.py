@@ -73,9 +73,9 @@ def test_unsupported_rows(H):
     assert row['status'] == 'unsupported'
     c = corpus('clustered_complex')
     row = H.measurement_row(c, {'name': 'clustered_complex'}, {'name': 'pavel-s_svd_amd:factory',
-                                                               'method': 'block_jacobi'}, 'cuda:0', warmup=0, repeat=1)
-    assert row['status'] == 'unsupported' and 'complex' in row['error']
-    for method in ('jacobi', 'qr', 'tsqr', 'polar_qdwh'):
+                                                               'method': 'bidiag_dc'}, 'cuda:0', warmup=0, repeat=1)
+    assert row['status'] == 'unsupported'
+    for method in ('jacobi', 'qr', 'tsqr', 'polar_qdwh', 'block_jacobi'):
         row = H.measurement_row(c, {'name': 'clustered_complex'}, {'name': 'pavel-s_svd_amd:factory',
                                                                    'method': method}, 'cuda:0', warmup=0, repeat=1)
         assert row['status'] == 'ok', (method, row.get('error'))
