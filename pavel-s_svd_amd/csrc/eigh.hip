@@ -436,7 +436,7 @@ struct BlkParams {
   T *G, *Vt, *Q;               // G, V^T: N x N (ld N); Q: nb/2 matrices Q_p^T, 64 x 64 row-major
   int *act;                    // nb/2: subproblem p rotated in this round
   double *bm;                  // nb x nb: max |G| of each 32 x 32 block
-  unsigned long long *slot;    // 3 rotating max|G| slots (bit patterns of non-negative doubles)
+  unsigned long long *slot;    // 3 rotating max|G| slots of kSub words (bit patterns of non-negative doubles)
   int *flag;                   // any rotation in the current sweep
 };
 
@@ -444,6 +444,22 @@ __device__ __forceinline__ int blk_row(int I, int J, int i) { return i < kEb ? I
 
 __device__ __forceinline__ void slot_max(unsigned long long *s, double v) {
   atomicMax(s, (unsigned long long)__double_as_longlong(v));
+}
+
+// Each of the 3 rotating max|G| slots is spread over kSub words 256 bytes apart: the thousands of
+// tiles of a round add their maxima to kSub different L2 lines instead of queueing on one, and a
+// slot's value is the largest of its words.
+constexpr int kSub = 16, kSubLd = 32;
+
+__device__ __forceinline__ unsigned long long *sub_slot(unsigned long long *s, int which, int k) {
+  return s + ((size_t)which * kSub + k) * kSubLd;
+}
+
+__device__ __forceinline__ double read_slot(const unsigned long long *s, int which) {
+  unsigned long long v = 0ull;
+#pragma unroll
+  for (int k = 0; k < kSub; ++k) v = v > s[((size_t)which * kSub + k) * kSubLd] ? v : s[((size_t)which * kSub + k) * kSubLd];
+  return __longlong_as_double((long long)v);
 }
 
 // upper-triangle index e of an np2 x np2 grid (k1 <= k2, row-major) -> (k1, k2)
@@ -463,15 +479,14 @@ __global__ __launch_bounds__(256) void blk_load_kernel(BlkParams<T> P) {
   mx = block_max<256>(mx, red);
   if (tid == 0) {
     P.bm[I * P.nb + K] = mx;
-    slot_max(&P.slot[0], mx);
+    slot_max(sub_slot(P.slot, 0, (int)((blockIdx.y * gridDim.x + blockIdx.x) % kSub)), mx);
   }
 }
 
 __global__ void blk_reset_kernel(unsigned long long *slot, int *flag, int what) {
-  if (threadIdx.x == 0) {
-    if (what & 1) slot[0] = slot[1] = slot[2] = 0ull;
-    if (what & 2) *flag = 0;
-  }
+  const int t = threadIdx.x;   // 64 threads: one per word of the 3 x kSub slot words
+  if ((what & 1) && t < 3 * kSub) slot[(size_t)t * kSubLd] = 0ull;
+  if ((what & 2) && t == 0) *flag = 0;
 }
 
 template <typename T>
@@ -491,8 +506,8 @@ __global__ __launch_bounds__(kBlkInner) void blk_inner_kernel(BlkParams<T> P, in
   const int p = blockIdx.x, tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
   int I, J;
   rr_pair(r, p, P.nb, I, J);
-  if (p == 0 && tid == 0) P.slot[(g + 2) % 3] = 0ull;
-  const double scale = __longlong_as_double((long long)P.slot[g % 3]);
+  if (p == 0 && tid < kSub) *sub_slot(P.slot, (g + 2) % 3, tid) = 0ull;
+  const double scale = read_slot(P.slot, g % 3);
   const double tol = P.tol > 0.0 ? P.tol : 10.0 * Eps<T>::value;
   double offmax = 0.0;
   for (int e = tid; e < kEs * kEs; e += kBlkInner) {
@@ -575,7 +590,7 @@ __global__ __launch_bounds__(kBlkInner) void blk_inner_kernel(BlkParams<T> P, in
     mx = wave_max_fast(mx);
     if (lane == 0) {
       P.bm[(bi ? J : I) * P.nb + (bj ? J : I)] = mx;
-      slot_max(&P.slot[(g + 1) % 3], mx);
+      slot_max(sub_slot(P.slot, (g + 1) % 3, (p * 4 + wave) % kSub), mx);
     }
   }
   if (tid == 0) {
@@ -655,7 +670,7 @@ __global__ __launch_bounds__(256) void blk_update_kernel(BlkParams<T> P, int r, 
     if (!ap && !aq) {
       if (tid < 4) {
         const int X = (tid >> 1) ? Jp : Ip, Y = (tid & 1) ? Jq : Iq;
-        slot_max(&P.slot[(g + 1) % 3], P.bm[X * P.nb + Y]);
+        slot_max(sub_slot(P.slot, (g + 1) % 3, (int)((bid * 4 + tid) % kSub)), P.bm[X * P.nb + Y]);
       }
       return;
     }
@@ -711,7 +726,7 @@ __global__ __launch_bounds__(256) void blk_update_kernel(BlkParams<T> P, int r, 
       const int X = wm ? Jp : Ip, Y = wn ? Jq : Iq;
       P.bm[X * P.nb + Y] = mx;
       P.bm[Y * P.nb + X] = mx;
-      slot_max(&P.slot[(g + 1) % 3], mx);
+      slot_max(sub_slot(P.slot, (g + 1) % 3, (int)((bid * 4 + wave) % kSub)), mx);
     }
   } else {
     const int64_t t = bid - nG, chunks = P.N / kEs;
@@ -750,7 +765,7 @@ size_t blk_ws_bytes(int n) {
   const int N = (n + kEs - 1) / kEs * kEs, nb = N / kEb, npair = nb / 2;
   auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
   return 2 * al(sizeof(T) * (size_t)N * N) + al(sizeof(T) * (size_t)npair * kEs * kEs) + al(sizeof(int) * npair) +
-         al(sizeof(double) * (size_t)nb * nb) + al(8 * 3) + al(4 * 3) + al(sizeof(double) * N) + al(sizeof(int) * N);
+         al(sizeof(double) * (size_t)nb * nb) + al(8 * 3 * kSub * kSubLd) + al(4 * 3) + al(sizeof(double) * N) + al(sizeof(int) * N);
 }
 
 template <typename T>
@@ -784,7 +799,7 @@ static int run_eigh_blocked(int64_t batch, int n, const T *h, T *w, T *v, int32_
   P.Q = reinterpret_cast<T *>(take(sizeof(T) * (size_t)npair * kEs * kEs));
   P.act = reinterpret_cast<int *>(take(sizeof(int) * npair));
   P.bm = reinterpret_cast<double *>(take(sizeof(double) * (size_t)P.nb * P.nb));
-  P.slot = reinterpret_cast<unsigned long long *>(take(8 * 3));
+  P.slot = reinterpret_cast<unsigned long long *>(take(8 * 3 * kSub * kSubLd));
   P.flag = reinterpret_cast<int *>(take(4 * 3));
   double *dw = reinterpret_cast<double *>(take(sizeof(double) * P.N));
   int *ord = reinterpret_cast<int *>(take(sizeof(int) * P.N));
