@@ -404,7 +404,8 @@ static int run_eigh(int64_t batch, int n, const T *h, T *w, T *v, int32_t *info,
 // cyclic sweeps of the scalar path's plane rotations and test (|g_pq| > tol * max|G|, max|G| of
 // the whole matrix at the start of the round, primitives.py:397-399).  The subproblem's
 // accumulated rotation Q_p is then applied to the rest of G -- block (p, q) <- Q_p^T B Q_q and
-// its mirror -- and to the rows of V^T, as 64 x 64 x 64 MFMA products.  A sweep in which no
+// its mirror -- and to the rows of V^T, as 64 x 64 x 64 MFMA products (the V^T products of a
+// round run in the next round's first launch, beside its subproblem sweeps).  A sweep in which no
 // subproblem rotates leaves G unchanged, so every off-diagonal |g_ij| <= tol * max|G| at the end:
 // the reference's stopping rule (primitives.py:418).  max|G| is kept per 32 x 32 block so that
 // blocks no rotation touches still contribute to the next round's threshold without being read.
@@ -496,14 +497,25 @@ __host__ __device__ inline size_t blk_inner_lds() {
 
 constexpr int kBlkInner = 512;
 
+// Rotations Q_p^T and activity flags of round g live in buffer g & 1: the launch that sweeps the
+// subproblems of round g also applies round g - 1's rotations to V^T (see blk_round_kernel).
 template <typename T>
-__global__ __launch_bounds__(kBlkInner) void blk_inner_kernel(BlkParams<T> P, int r, int g) {
-  extern __shared__ __align__(16) double smem_bi[];
+__device__ __forceinline__ T *blk_qbuf(const BlkParams<T> &P, int g) {
+  return P.Q + (size_t)(g & 1) * (P.nb / 2) * kEs * kEs;
+}
+
+template <typename T>
+__device__ __forceinline__ int *blk_actbuf(const BlkParams<T> &P, int g) {
+  return P.act + (g & 1) * (P.nb / 2);
+}
+
+template <typename T>
+__device__ __forceinline__ void blk_inner_body(const BlkParams<T> &P, int p, int r, int g, double *smem_bi) {
   double2 *cs = reinterpret_cast<double2 *>(smem_bi);   // (c, s) of pair k of the round
   int2 *pq = reinterpret_cast<int2 *>(cs + kEb);          // (p, q) of pair k
   T *S = reinterpret_cast<T *>(pq + kEb);
   T *Qt = S + kEs * kEsLd;
-  const int p = blockIdx.x, tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
   int I, J;
   rr_pair(r, p, P.nb, I, J);
   if (p == 0 && tid < kSub) *sub_slot(P.slot, (g + 2) % 3, tid) = 0ull;
@@ -573,10 +585,11 @@ __global__ __launch_bounds__(kBlkInner) void blk_inner_kernel(BlkParams<T> P, in
     if (!act_sw) break;
   }
   if (any) {
+    T *Qg = blk_qbuf(P, g) + (int64_t)p * kEs * kEs;
     for (int e = tid; e < kEs * kEs; e += kBlkInner) {
       const int i = e / kEs, j = e - i * kEs;
       P.G[(int64_t)blk_row(I, J, i) * P.N + blk_row(I, J, j)] = S[i * kEsLd + j];
-      P.Q[(int64_t)p * kEs * kEs + e] = Qt[i * kEsLd + j];
+      Qg[e] = Qt[i * kEsLd + j];
     }
   }
   // maxima of the four 32 x 32 blocks, one per wave of the first four
@@ -594,7 +607,7 @@ __global__ __launch_bounds__(kBlkInner) void blk_inner_kernel(BlkParams<T> P, in
     }
   }
   if (tid == 0) {
-    P.act[p] = any;
+    blk_actbuf(P, g)[p] = any;
     if (any) atomicOr(P.flag, 1);
   }
 }
@@ -640,9 +653,8 @@ __host__ __device__ inline size_t blk_update_lds() {
   return sizeof(T) * 2 * (size_t)kEs * kEsLd;   // the tile and one rotation at a time
 }
 
-// Blocks [0, nG): G block (p, q), p < q, and its mirror.  Blocks [nG, nG + npair * N / 64): the
-// 64 x 64 tile (rows of pair p, column chunk c) of V^T.  Global loads and stores move 16 bytes per
-// lane (every tile row is made of 32-element segments that start on a 128-byte boundary: N and
+// Block e: G block (p, q), p < q (the e-th of the strictly upper pair grid), and its mirror.
+// Global loads and stores move 16 bytes per lane (every tile row is made of 32-element segments that start on a 128-byte boundary: N and
 // the block offsets are multiples of 32 elements, the buffers 256-byte aligned); LDS tiles keep
 // the odd stride kEsLd of the MFMA products.
 template <typename T>
@@ -666,7 +678,8 @@ __global__ __launch_bounds__(256) void blk_update_kernel(BlkParams<T> P, int r, 
     int Ip, Jp, Iq, Jq;
     rr_pair(r, p, P.nb, Ip, Jp);
     rr_pair(r, q, P.nb, Iq, Jq);
-    const int ap = P.act[p], aq = P.act[q];
+    const int *act = blk_actbuf(P, g);
+    const int ap = act[p], aq = act[q];
     if (!ap && !aq) {
       if (tid < 4) {
         const int X = (tid >> 1) ? Jp : Ip, Y = (tid & 1) ? Jq : Iq;
@@ -674,8 +687,8 @@ __global__ __launch_bounds__(256) void blk_update_kernel(BlkParams<T> P, int r, 
       }
       return;
     }
-    const V16 *Qp = reinterpret_cast<const V16 *>(P.Q + (int64_t)p * kEs * kEs);
-    const V16 *Qq = reinterpret_cast<const V16 *>(P.Q + (int64_t)q * kEs * kEs);
+    const V16 *Qp = reinterpret_cast<const V16 *>(blk_qbuf(P, g) + (int64_t)p * kEs * kEs);
+    const V16 *Qq = reinterpret_cast<const V16 *>(blk_qbuf(P, g) + (int64_t)q * kEs * kEs);
     // the tile with Q_p (or, when only q rotated, Q_q) beside it
     const V16 *Q1 = ap ? Qp : Qq;
     for (int e = tid; e < kEs * RV; e += 256) {
@@ -728,43 +741,70 @@ __global__ __launch_bounds__(256) void blk_update_kernel(BlkParams<T> P, int r, 
       P.bm[Y * P.nb + X] = mx;
       slot_max(sub_slot(P.slot, (g + 1) % 3, (int)((bid * 4 + wave) % kSub)), mx);
     }
-  } else {
-    const int64_t t = bid - nG, chunks = P.N / kEs;
-    const int p = (int)(t / chunks), c0 = (int)(t - (int64_t)p * chunks) * kEs;
-    if (!P.act[p]) return;
-    int Ip, Jp;
-    rr_pair(r, p, P.nb, Ip, Jp);
-    const V16 *Qp = reinterpret_cast<const V16 *>(P.Q + (int64_t)p * kEs * kEs);
-    for (int e = tid; e < kEs * RV; e += 256) {
-      const int i = e / RV, j = (e - i * RV) * VE;
-      const V16 x = *reinterpret_cast<const V16 *>(P.Vt + (int64_t)blk_row(Ip, Jp, i) * P.N + c0 + j);
-      const V16 y = Qp[e];
+  }
+}
+
+// V^T tile t (rows of pair p of round r, 64-column chunk c) <- Q_p^T tile, for the rotations of
+// round g (buffer g & 1), on a workgroup of kBlkInner threads: all threads move the tile, the
+// first four waves form the product.
+template <typename T>
+__device__ __forceinline__ void blk_vt_body(const BlkParams<T> &P, int64_t t, int r, int g, double *smem) {
+  T *Bs = reinterpret_cast<T *>(smem);
+  T *Qa = Bs + kEs * kEsLd;
+  using V16 = Vec16<T>;
+  constexpr int VE = V16::n, RV = kEs / VE;
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave, lr = lane & 15, lk = lane >> 4;
+  const int wm = ((wave & 3) >> 1) * 32, wn = (wave & 1) * 32;
+  const int64_t chunks = P.N / kEs;
+  const int p = (int)(t / chunks), c0 = (int)(t - (int64_t)p * chunks) * kEs;
+  if (!blk_actbuf(P, g)[p]) return;
+  int Ip, Jp;
+  rr_pair(r, p, P.nb, Ip, Jp);
+  const V16 *Qp = reinterpret_cast<const V16 *>(blk_qbuf(P, g) + (int64_t)p * kEs * kEs);
+  for (int e = tid; e < kEs * RV; e += kBlkInner) {
+    const int i = e / RV, j = (e - i * RV) * VE;
+    const V16 x = *reinterpret_cast<const V16 *>(P.Vt + (int64_t)blk_row(Ip, Jp, i) * P.N + c0 + j);
+    const V16 y = Qp[e];
 #pragma unroll
-      for (int u = 0; u < VE; ++u) {
-        Bs[i * kEsLd + j + u] = x.v[u];
-        Qa[i * kEsLd + j + u] = y.v[u];
-      }
-    }
-    __syncthreads();
-    mm64<T, false>(Qa, Bs, acc, wm, wn, lr, lk);
-    __syncthreads();
-    store64<T>(Bs, acc, wm, wn, lr, lk);
-    __syncthreads();
-    for (int e = tid; e < kEs * RV; e += 256) {
-      const int i = e / RV, j = (e - i * RV) * VE;
-      V16 x;
-#pragma unroll
-      for (int u = 0; u < VE; ++u) x.v[u] = Bs[i * kEsLd + j + u];
-      *reinterpret_cast<V16 *>(P.Vt + (int64_t)blk_row(Ip, Jp, i) * P.N + c0 + j) = x;
+    for (int u = 0; u < VE; ++u) {
+      Bs[i * kEsLd + j + u] = x.v[u];
+      Qa[i * kEsLd + j + u] = y.v[u];
     }
   }
+  __syncthreads();
+  typename Mfma<T>::V acc[2][2];
+  if (wave < 4) mm64<T, false>(Qa, Bs, acc, wm, wn, lr, lk);
+  __syncthreads();
+  if (wave < 4) store64<T>(Bs, acc, wm, wn, lr, lk);
+  __syncthreads();
+  for (int e = tid; e < kEs * RV; e += kBlkInner) {
+    const int i = e / RV, j = (e - i * RV) * VE;
+    V16 x;
+#pragma unroll
+    for (int u = 0; u < VE; ++u) x.v[u] = Bs[i * kEsLd + j + u];
+    *reinterpret_cast<V16 *>(P.Vt + (int64_t)blk_row(Ip, Jp, i) * P.N + c0 + j) = x;
+  }
+}
+
+// One tournament round's first launch: workgroups [0, ninner) sweep the 64 x 64 subproblems of
+// round (r, g); the next npair * N / 64 workgroups (when vg >= 0) apply round (vr, vg)'s rotations
+// to V^T.  V^T is read by no other kernel of the sweep, so its update for round g - 1 runs beside
+// the subproblem sweeps of round g, which occupy only nb / 2 workgroups.
+template <typename T>
+__global__ __launch_bounds__(kBlkInner) void blk_round_kernel(BlkParams<T> P, int r, int g, int ninner, int vr,
+                                                              int vg) {
+  extern __shared__ __align__(16) double smem_br[];
+  if ((int)blockIdx.x < ninner)
+    blk_inner_body<T>(P, (int)blockIdx.x, r, g, smem_br);
+  else
+    blk_vt_body<T>(P, (int64_t)blockIdx.x - ninner, vr, vg, smem_br);
 }
 
 template <typename T>
 size_t blk_ws_bytes(int n) {
   const int N = (n + kEs - 1) / kEs * kEs, nb = N / kEb, npair = nb / 2;
   auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
-  return 2 * al(sizeof(T) * (size_t)N * N) + al(sizeof(T) * (size_t)npair * kEs * kEs) + al(sizeof(int) * npair) +
+  return 2 * al(sizeof(T) * (size_t)N * N) + al(2 * sizeof(T) * (size_t)npair * kEs * kEs) + al(2 * sizeof(int) * npair) +
          al(sizeof(double) * (size_t)nb * nb) + al(8 * 3 * kSub * kSubLd) + al(4 * 3) + al(sizeof(double) * N) + al(sizeof(int) * N);
 }
 
@@ -774,7 +814,7 @@ static int run_eigh_blocked(int64_t batch, int n, const T *h, T *w, T *v, int32_
   if (ws_bytes < blk_ws_bytes<T>(n)) return PSVD_EINVAL;
   static bool attr = false;
   if (!attr) {
-    if (hipFuncSetAttribute((const void *)blk_inner_kernel<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+    if (hipFuncSetAttribute((const void *)blk_round_kernel<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)blk_inner_lds<T>()) != hipSuccess ||
         hipFuncSetAttribute((const void *)blk_update_kernel<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)blk_update_lds<T>()) != hipSuccess)
@@ -796,27 +836,31 @@ static int run_eigh_blocked(int64_t batch, int n, const T *h, T *w, T *v, int32_
   };
   P.G = reinterpret_cast<T *>(take(sizeof(T) * (size_t)P.N * P.N));
   P.Vt = reinterpret_cast<T *>(take(sizeof(T) * (size_t)P.N * P.N));
-  P.Q = reinterpret_cast<T *>(take(sizeof(T) * (size_t)npair * kEs * kEs));
-  P.act = reinterpret_cast<int *>(take(sizeof(int) * npair));
+  P.Q = reinterpret_cast<T *>(take(2 * sizeof(T) * (size_t)npair * kEs * kEs));
+  P.act = reinterpret_cast<int *>(take(2 * sizeof(int) * npair));
   P.bm = reinterpret_cast<double *>(take(sizeof(double) * (size_t)P.nb * P.nb));
   P.slot = reinterpret_cast<unsigned long long *>(take(8 * 3 * kSub * kSubLd));
   P.flag = reinterpret_cast<int *>(take(4 * 3));
   double *dw = reinterpret_cast<double *>(take(sizeof(double) * P.N));
   int *ord = reinterpret_cast<int *>(take(sizeof(int) * P.N));
   const int64_t nG = (int64_t)npair * (npair - 1) / 2;
-  const unsigned gupd = (unsigned)(nG + (int64_t)npair * (P.N / kEs));
+  const unsigned nvt = (unsigned)((int64_t)npair * (P.N / kEs));
   auto blocks = [](int64_t items) { return (unsigned)((items + kEighThreads - 1) / kEighThreads); };
   for (int64_t b = 0; b < batch; ++b) {
     P.h = h + b * (int64_t)n * n;
     hipLaunchKernelGGL(blk_reset_kernel, dim3(1), dim3(64), 0, st, P.slot, P.flag, 3);
     hipLaunchKernelGGL(blk_load_kernel<T>, dim3(P.nb, P.nb), dim3(256), 0, st, P);
-    int sweeps = 0, g = 0;
+    int sweeps = 0, g = 0, vr = -1;
     for (int sw = 1; sw <= max_sweeps; ++sw) {
       sweeps = sw;
       hipLaunchKernelGGL(blk_reset_kernel, dim3(1), dim3(64), 0, st, P.slot, P.flag, 2);
       for (int r = 0; r < P.nb - 1; ++r, ++g) {
-        hipLaunchKernelGGL(blk_inner_kernel<T>, dim3(npair), dim3(kBlkInner), blk_inner_lds<T>(), st, P, r, g);
-        hipLaunchKernelGGL(blk_update_kernel<T>, dim3(gupd), dim3(256), blk_update_lds<T>(), st, P, r, g);
+        // subproblems of round g beside the V^T update of round g - 1 (none before the first)
+        hipLaunchKernelGGL(blk_round_kernel<T>, dim3(npair + (g > 0 ? nvt : 0u)), dim3(kBlkInner), blk_inner_lds<T>(),
+                           st, P, r, g, npair, vr, g - 1);
+        if (nG > 0)
+          hipLaunchKernelGGL(blk_update_kernel<T>, dim3((unsigned)nG), dim3(256), blk_update_lds<T>(), st, P, r, g);
+        vr = r;
       }
       if (hipGetLastError() != hipSuccess) return PSVD_ELAUNCH;
       int active = 0;
@@ -824,6 +868,9 @@ static int run_eigh_blocked(int64_t batch, int n, const T *h, T *w, T *v, int32_
       if (hipStreamSynchronize(st) != hipSuccess) return PSVD_ELAUNCH;
       if (!active) break;
     }
+    // the last round's rotations on V^T
+    if (g > 0)
+      hipLaunchKernelGGL(blk_round_kernel<T>, dim3(nvt), dim3(kBlkInner), blk_inner_lds<T>(), st, P, 0, g, 0, vr, g - 1);
     EighParams<T> E{};
     E.n = n;
     E.G = P.G;
