@@ -31,7 +31,7 @@ struct EighParams {
   T *G, *Vt;
   T *cs;                      // 2 * npairs
   int *pq;                    // 2 * npairs
-  unsigned long long *slot;   // 3 max slots
+  unsigned long long *slot;   // max|G| slots of kSub words
   int *flag;                  // 3 sweep flags
   double *dw;                 // np (eigenvalues, FP64) for sorting
   int *ord;                   // np
@@ -187,6 +187,22 @@ __global__ __launch_bounds__(kEighThreads) void eigh_lds_kernel(EighParams<T> P)
   }
 }
 
+// Each rotating max|G| slot of the HBM solvers is spread over kSub words 256 bytes apart: the
+// thousands of workgroups of a round add their maxima to kSub different L2 lines instead of
+// queueing on one, and a slot's value is the largest of its words.
+constexpr int kSub = 16, kSubLd = 32;
+
+__device__ __forceinline__ unsigned long long *sub_slot(unsigned long long *s, int which, int k) {
+  return s + ((size_t)which * kSub + k) * kSubLd;
+}
+
+__device__ __forceinline__ double read_slot(const unsigned long long *s, int which) {
+  unsigned long long v = 0ull;
+#pragma unroll
+  for (int k = 0; k < kSub; ++k) v = v > s[((size_t)which * kSub + k) * kSubLd] ? v : s[((size_t)which * kSub + k) * kSubLd];
+  return __longlong_as_double((long long)v);
+}
+
 // ---- multi-launch version for one matrix beyond LDS -------------------------------------
 // Each round is two kernels (the kernel boundary is the barrier): eigh_params_kernel forms the
 // pair rotations from max|G| left in slot[g & 1] by the previous round and clears the other slot;
@@ -206,7 +222,9 @@ __global__ __launch_bounds__(kEighThreads) void eigh_load_kernel(EighParams<T> P
     mx = fmax(mx, fabs((double)x));
   }
   mx = wave_max_fast(mx);
-  if ((threadIdx.x & (kWave - 1)) == 0) atomicMax(&P.slot[0], (unsigned long long)__double_as_longlong(mx));
+  if ((threadIdx.x & (kWave - 1)) == 0)
+    atomicMax(sub_slot(P.slot, 0, (int)((blockIdx.x * (kEighThreads / kWave) + threadIdx.x / kWave) % kSub)),
+              (unsigned long long)__double_as_longlong(mx));
 }
 
 template <typename T>
@@ -215,19 +233,21 @@ __global__ __launch_bounds__(kEighThreads) void eigh_params_kernel(EighParams<T>
   const int64_t k = (int64_t)blockIdx.x * kEighThreads + threadIdx.x;
   const double eps = Eps<T>::value;
   const double tol = P.tol > 0.0 ? P.tol : 10.0 * eps;
-  if (k == 0) P.slot[(g + 1) & 1] = 0ull;
-  if (k >= npairs) return;
-  const double scale = __longlong_as_double((long long)P.slot[g & 1]);
-  int p, q;
-  rr_pair(r, (int)k, np, p, q);
-  double c, s, ratio;
-  bool act;
-  pair_params(P.G, ld, p, q, tol, scale, c, s, act, ratio);
-  P.cs[k] = act ? (T)c : T(1);
-  P.cs[npairs + k] = act ? (T)s : T(0);
-  P.pq[k] = p;
-  P.pq[npairs + k] = q;
-  if (act) atomicOr(P.flag, 1);
+  if (k < kSub) *sub_slot(P.slot, (g + 1) & 1, (int)k) = 0ull;
+  bool act = false;
+  if (k < npairs) {
+    const double scale = read_slot(P.slot, g & 1);
+    int p, q;
+    rr_pair(r, (int)k, np, p, q);
+    double c, s, ratio;
+    pair_params(P.G, ld, p, q, tol, scale, c, s, act, ratio);
+    P.cs[k] = act ? (T)c : T(1);
+    P.cs[npairs + k] = act ? (T)s : T(0);
+    P.pq[k] = p;
+    P.pq[npairs + k] = q;
+  }
+  // one flag update per wave with a rotation
+  if (__any(act) && (threadIdx.x & (kWave - 1)) == 0) atomicOr(P.flag, 1);
 }
 
 template <typename T>
@@ -267,7 +287,9 @@ __global__ __launch_bounds__(kEighThreads) void eigh_update_kernel(EighParams<T>
     *vq = (T)(s * x + c * y);
   }
   mymax = wave_max_fast(mymax);
-  if ((threadIdx.x & (kWave - 1)) == 0) atomicMax(&P.slot[(g + 1) & 1], (unsigned long long)__double_as_longlong(mymax));
+  if ((threadIdx.x & (kWave - 1)) == 0)
+    atomicMax(sub_slot(P.slot, (g + 1) & 1, (int)((blockIdx.x * (kEighThreads / kWave) + threadIdx.x / kWave) % kSub)),
+              (unsigned long long)__double_as_longlong(mymax));
 }
 
 template <typename T>
@@ -304,13 +326,9 @@ __global__ __launch_bounds__(kEighThreads) void eigh_out_kernel(EighParams<T> P,
 }
 
 __global__ void eigh_reset_kernel(unsigned long long *slot, int *flag, int what) {
-  if (threadIdx.x == 0) {
-    if (what & 1) {
-      slot[0] = 0ull;
-      slot[1] = 0ull;
-    }
-    if (what & 2) *flag = 0;
-  }
+  const int t = threadIdx.x;   // 64 threads: one per word of the 2 x kSub slot words
+  if ((what & 1) && t < 2 * kSub) slot[(size_t)t * kSubLd] = 0ull;
+  if ((what & 2) && t == 0) *flag = 0;
 }
 
 constexpr size_t kEighLdsCap = 160 * 1024 - 4096;
@@ -320,7 +338,7 @@ size_t eigh_grid_ws_bytes(int n) {
   const int np = n + (n & 1), ld = (np + 63) & ~63, npairs = np / 2;
   auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
   return al(sizeof(T) * (size_t)np * ld) * 2 + al(sizeof(T) * 2 * npairs) + al(sizeof(int) * 2 * npairs) +
-         al(8 * 3) + al(4 * 3) + al(sizeof(double) * np) + al(sizeof(int) * np);
+         al(8 * 3 * kSub * kSubLd) + al(4 * 3) + al(sizeof(double) * np) + al(sizeof(int) * np);
 }
 
 template <typename T>
@@ -358,7 +376,7 @@ static int run_eigh(int64_t batch, int n, const T *h, T *w, T *v, int32_t *info,
   P.Vt = reinterpret_cast<T *>(take(sizeof(T) * (size_t)np * ld));
   P.cs = reinterpret_cast<T *>(take(sizeof(T) * 2 * npairs));
   P.pq = reinterpret_cast<int *>(take(sizeof(int) * 2 * npairs));
-  P.slot = reinterpret_cast<unsigned long long *>(take(8 * 3));
+  P.slot = reinterpret_cast<unsigned long long *>(take(8 * 3 * kSub * kSubLd));
   P.flag = reinterpret_cast<int *>(take(4 * 3));
   P.dw = reinterpret_cast<double *>(take(sizeof(double) * np));
   P.ord = reinterpret_cast<int *>(take(sizeof(int) * np));
@@ -447,21 +465,6 @@ __device__ __forceinline__ void slot_max(unsigned long long *s, double v) {
   atomicMax(s, (unsigned long long)__double_as_longlong(v));
 }
 
-// Each of the 3 rotating max|G| slots is spread over kSub words 256 bytes apart: the thousands of
-// tiles of a round add their maxima to kSub different L2 lines instead of queueing on one, and a
-// slot's value is the largest of its words.
-constexpr int kSub = 16, kSubLd = 32;
-
-__device__ __forceinline__ unsigned long long *sub_slot(unsigned long long *s, int which, int k) {
-  return s + ((size_t)which * kSub + k) * kSubLd;
-}
-
-__device__ __forceinline__ double read_slot(const unsigned long long *s, int which) {
-  unsigned long long v = 0ull;
-#pragma unroll
-  for (int k = 0; k < kSub; ++k) v = v > s[((size_t)which * kSub + k) * kSubLd] ? v : s[((size_t)which * kSub + k) * kSubLd];
-  return __longlong_as_double((long long)v);
-}
 
 // upper-triangle index e of an np2 x np2 grid (k1 <= k2, row-major) -> (k1, k2)
 
