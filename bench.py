@@ -205,7 +205,8 @@ def config_tsqr(pkg, dev):
 def config_consumers(pkg, dev):
     """Solvers built on the hot path's kernels: block_jacobi (fused block-pair rounds) and gram_jacobi
     on the 512 x 256 single-matrix workload; complex jacobi on 1024 complex64 48 x 48 matrices,
-    complex qr on 1024 complex64 96 x 48 and complex polar_qdwh on 256 complex64 48 x 48;
+    complex qr on 1024 complex64 96 x 48, complex polar_qdwh and complex block_jacobi on 256 complex64
+    48 x 48;
     randomized / subspace / block_krylov (rank 64,
     oversample 8, two power steps) on the 8192 x 2048 rank-64-plus-noise TSQR workload."""
     gen = torch.Generator(device=dev)
@@ -270,6 +271,18 @@ def config_consumers(pkg, dev):
                                              'eigen_sweeps': info['eigen_sweeps'],
                                              'spectrum_error_over_sigma1_first64': err,
                                              'orthogonality_u_max_first64': orth}
+    ab = (rng.standard_normal((256, 48, 48)) + 1j * rng.standard_normal((256, 48, 48))).astype(np.complex64)
+    xb = torch.from_numpy(ab).to(dev)
+    t, (u, sv, vh, info) = _timed(lambda: pkg.svd(xb, 'block_jacobi', return_info=True), 3)
+    ref = np.linalg.svd(ab[:64].astype(np.complex128), compute_uv=False)
+    err = float(np.max(np.abs(sv[:64].double().cpu().numpy() - ref) / ref[:, :1]))
+    uu = u[:64].to(torch.complex128)
+    orth = float((uu.mH @ uu - torch.eye(48, dtype=torch.complex128, device=dev)).abs().max())
+    res['block_jacobi_complex64_256x48x48'] = {'seconds': t, 'svds_per_second': 256 / t,
+                                               'block_sweeps': info['block_sweeps'],
+                                               'jacobi_sweeps': info['jacobi_sweeps'],
+                                               'spectrum_error_over_sigma1_first64': err,
+                                               'orthogonality_u_max_first64': orth}
     return {'workload': 'block_jacobi (block_size 4 and 16) and gram_jacobi on 512x256 float32, Haar U/V, sigma '
                         'geomspace(1, 1e-6) (errors over the sigma each method keeps: block_jacobi cuts at '
                         'eps*max(m,n)*sigma_1, gram_jacobi at sqrt(eps*n)*sigma_1); randomized / subspace / '
@@ -277,7 +290,9 @@ def config_consumers(pkg, dev):
                         '(sigma 1 .. 1e-3) plus N(0, 1e-8) noise, top-64 sigma against FP64 TSQR + Jacobi; '
                         'complex jacobi on 1024 complex64 48x48 Gaussian matrices (one batched call, LDS kernel) '
                         'and complex qr on 1024 complex64 96x48 (LDS QR, Jacobi of R, Q U_R) and complex polar_qdwh '
-                        'on 256 complex64 48x48 (QR-form steps), sigma of the first 64 against complex128 LAPACK',
+                        'on 256 complex64 48x48 (QR-form steps) and complex block_jacobi (block_size 4) on 256 '
+                        'complex64 48x48 (each round one batched Jacobi launch over its group pairs), sigma of the '
+                        'first 64 against complex128 LAPACK',
             'unit': 's/solve', 'results': res}
 
 
