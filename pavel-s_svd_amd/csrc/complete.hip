@@ -10,8 +10,10 @@
 // One workgroup per matrix.  The leverage vector is formed once and, after column r is written,
 // lowered by its squared entries (the reference recomputes 1 - |U(i, :)|^2 from scratch; the two
 // agree to rounding).  U(:, :r) u-products run one wave per row with the lanes across the row
-// (coalesced), U^T v runs row slices x columns with the slices added in a fixed order, so every
-// sum has a fixed order and the result is deterministic.
+// (coalesced) and the lanes' partials added by DPP moves and four readlanes (VALU work: a
+// shuffle-based wave sum queues 12 LDS permutes per row on the CU's one LDS pipe); U^T v runs
+// row slices x columns with the slices added in a fixed order, so every sum has a fixed order
+// and the result is deterministic.
 #include <float.h>
 
 #include "common.hpp"
@@ -51,7 +53,7 @@ __global__ __launch_bounds__(kCompleteThreads) void complete_kernel(T *U, int64_
     const T *row = Ub + (int64_t)i * ldu;
     double a = 0.0;
     for (int c = lane; c < k0; c += kWave) a = fma((double)row[c], (double)row[c], a);
-    a = wave_sum(a);
+    a = group_sum_dpp<kWave>(a);
     if (lane == 0) lev[i] = 1.0 - a;
   }
   __syncthreads();
@@ -92,7 +94,7 @@ __global__ __launch_bounds__(kCompleteThreads) void complete_kernel(T *U, int64_
       const T *row = Ub + (int64_t)i * ldu;
       double a = 0.0;
       for (int c = lane; c < r; c += kWave) a = fma((double)row[c], (double)rix[c], a);
-      a = wave_sum(a);
+      a = group_sum_dpp<kWave>(a);
       if (lane == 0) v[i] = (i == ix ? 1.0 : 0.0) - a;
     }
     __syncthreads();
@@ -120,7 +122,7 @@ __global__ __launch_bounds__(kCompleteThreads) void complete_kernel(T *U, int64_
       const T *row = Ub + (int64_t)i * ldu;
       double a = 0.0;
       for (int c = lane; c < r; c += kWave) a = fma((double)row[c], y[c], a);
-      a = wave_sum(a);
+      a = group_sum_dpp<kWave>(a);
       const double vi = v[i] - a;
       if (lane == 0) {
         v[i] = vi;

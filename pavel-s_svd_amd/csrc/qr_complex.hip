@@ -272,8 +272,8 @@ __global__ __launch_bounds__(kCcThreads) void ccomplete_kernel(R *u, int m, int 
           ar = fma(ur, vr[i], fma(ui, vi[i], ar));
           ai = fma(ur, vi[i], fma(-ui, vr[i], ai));
         }
-        ar = wave_sum(ar);
-        ai = wave_sum(ai);
+        ar = group_sum_dpp<kWave>(ar);
+        ai = group_sum_dpp<kWave>(ai);
         if (lane == 0) {
           yr[c] = ar;
           yi[c] = ai;
