@@ -22,7 +22,9 @@ namespace psvd {
 
 constexpr int kCompleteThreads = 1024;
 
-template <typename T>
+// LDSV: v, the leverage vector and y live in LDS (8 (2m + n) bytes of dynamic LDS) instead of the
+// workspace, so no per-row global store sits in front of the next row's loads.
+template <typename T, bool LDSV>
 __global__ __launch_bounds__(kCompleteThreads) void complete_kernel(T *U, int64_t ldu, int64_t su, int m, int n,
                                                                     const int32_t *nk, int32_t *status, double *vbuf,
                                                                     double *ybuf, double *levbuf) {
@@ -33,9 +35,10 @@ __global__ __launch_bounds__(kCompleteThreads) void complete_kernel(T *U, int64_
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
   const int64_t b = blockIdx.x;
   T *Ub = U + b * su;
-  double *v = vbuf + b * (int64_t)m;
-  double *y = ybuf + b * (int64_t)n;
-  double *lev = levbuf + b * (int64_t)m;
+  extern __shared__ double smv[];
+  double *v = LDSV ? smv : vbuf + b * (int64_t)m;
+  double *lev = LDSV ? smv + m : levbuf + b * (int64_t)m;
+  double *y = LDSV ? smv + 2 * m : ybuf + b * (int64_t)n;
   const int k0 = nk[b] < 0 ? 0 : (nk[b] > n ? n : nk[b]);
   const double eps = Eps<T>::value;
   int st = 0;
@@ -166,14 +169,29 @@ int psvd_complete_columns(int dtype, int64_t batch, int m, int n, void *u, int64
   double *yb = vb + batch * (int64_t)m;
   double *lb = yb + batch * (int64_t)n;
   hipStream_t st = static_cast<hipStream_t>(stream);
-  if (dtype == PSVD_F32)
-    hipLaunchKernelGGL(psvd::complete_kernel<float>, dim3((unsigned)batch), dim3(psvd::kCompleteThreads), 0, st,
-                       static_cast<float *>(u), ldu, stride_u, m, n, nk, status, vb, yb, lb);
-  else if (dtype == PSVD_F64)
-    hipLaunchKernelGGL(psvd::complete_kernel<double>, dim3((unsigned)batch), dim3(psvd::kCompleteThreads), 0, st,
-                       static_cast<double *>(u), ldu, stride_u, m, n, nk, status, vb, yb, lb);
-  else
+  // v, leverage and y in LDS when they fit the default 64 KB of dynamic LDS
+  const size_t lds = sizeof(double) * (2 * (size_t)m + n);
+  const bool in_lds = lds <= 65536;
+  const dim3 grid((unsigned)batch), block(psvd::kCompleteThreads);
+  if (dtype == PSVD_F32) {
+    float *uf = static_cast<float *>(u);
+    if (in_lds)
+      hipLaunchKernelGGL((psvd::complete_kernel<float, true>), grid, block, lds, st, uf, ldu, stride_u, m, n, nk, status,
+                         vb, yb, lb);
+    else
+      hipLaunchKernelGGL((psvd::complete_kernel<float, false>), grid, block, 0, st, uf, ldu, stride_u, m, n, nk, status,
+                         vb, yb, lb);
+  } else if (dtype == PSVD_F64) {
+    double *ud = static_cast<double *>(u);
+    if (in_lds)
+      hipLaunchKernelGGL((psvd::complete_kernel<double, true>), grid, block, lds, st, ud, ldu, stride_u, m, n, nk,
+                         status, vb, yb, lb);
+    else
+      hipLaunchKernelGGL((psvd::complete_kernel<double, false>), grid, block, 0, st, ud, ldu, stride_u, m, n, nk,
+                         status, vb, yb, lb);
+  } else {
     return PSVD_EINVAL;
+  }
   return hipGetLastError() == hipSuccess ? PSVD_OK : PSVD_ELAUNCH;
 }
 
