@@ -9,11 +9,10 @@
 //
 // One workgroup per matrix.  The leverage vector is formed once and, after column r is written,
 // lowered by its squared entries (the reference recomputes 1 - |U(i, :)|^2 from scratch; the two
-// agree to rounding).  U(:, :r) u-products run one wave per row with the lanes across the row
-// (coalesced) and the lanes' partials added by DPP moves and four readlanes (VALU work: a
-// shuffle-based wave sum queues 12 LDS permutes per row on the CU's one LDS pipe); U^T v runs
-// row slices x columns with the slices added in a fixed order, so every sum has a fixed order
-// and the result is deterministic.
+// agree to rounding).  U(:, :r) u-products run four rows per wave with 16 lanes across each row
+// (64-byte segments) and the lanes' partials added by DPP moves within the 16-lane group; U^T v
+// runs row slices x columns with the slices added in a fixed order, so every sum has a fixed
+// order and the result is deterministic.
 #include <float.h>
 
 #include "common.hpp"
@@ -21,6 +20,10 @@
 namespace psvd {
 
 constexpr int kCompleteThreads = 1024;
+// the row products U(:, :r) x: each wave takes kRowGroups rows at a time, kRowLanes lanes per row,
+// so one 4-level DPP sum serves four rows (a full-wave sum per row made these loops issue-bound)
+constexpr int kRowLanes = 16, kRowGroups = kWave / kRowLanes;
+static_assert(kRowLanes == 16, "the leaders' sum below reads lanes 0, 16, 32, 48");
 
 // LDSV: v, the leverage vector and y live in LDS (8 (2m + n) bytes of dynamic LDS) instead of the
 // workspace, so no per-row global store sits in front of the next row's loads.
@@ -33,6 +36,7 @@ __global__ __launch_bounds__(kCompleteThreads) void complete_kernel(T *U, int64_
   __shared__ int redi[NW];
   __shared__ double ypart[NT];
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+  const int grp = lane / kRowLanes, gl = lane % kRowLanes;   // row group of the wave, lane in it
   const int64_t b = blockIdx.x;
   T *Ub = U + b * su;
   extern __shared__ double smv[];
@@ -93,12 +97,15 @@ __global__ __launch_bounds__(kCompleteThreads) void complete_kernel(T *U, int64_
     const int ix = bi;
     const T *rix = Ub + (int64_t)ix * ldu;
     // v = e_ix - U (U^T e_ix) = e_ix - U U(ix, :)^T
-    for (int i = wave; i < m; i += NW) {
-      const T *row = Ub + (int64_t)i * ldu;
+    for (int i0 = wave * kRowGroups; i0 < m; i0 += NW * kRowGroups) {
+      const int i = i0 + grp;
       double a = 0.0;
-      for (int c = lane; c < r; c += kWave) a = fma((double)row[c], (double)rix[c], a);
-      a = group_sum_dpp<kWave>(a);
-      if (lane == 0) v[i] = (i == ix ? 1.0 : 0.0) - a;
+      if (i < m) {
+        const T *row = Ub + (int64_t)i * ldu;
+        for (int c = gl; c < r; c += kRowLanes) a = fma((double)row[c], (double)rix[c], a);
+      }
+      a = group_sum_dpp<kRowLanes>(a);
+      if (gl == 0 && i < m) v[i] = (i == ix ? 1.0 : 0.0) - a;
     }
     __syncthreads();
     // y = U(:, :r)^T v: S row slices per column, slices added in order
@@ -121,17 +128,22 @@ __global__ __launch_bounds__(kCompleteThreads) void complete_kernel(T *U, int64_
     }
     // v -= U(:, :r) y and |v|^2
     double l2 = 0.0;
-    for (int i = wave; i < m; i += NW) {
-      const T *row = Ub + (int64_t)i * ldu;
+    for (int i0 = wave * kRowGroups; i0 < m; i0 += NW * kRowGroups) {
+      const int i = i0 + grp;
       double a = 0.0;
-      for (int c = lane; c < r; c += kWave) a = fma((double)row[c], y[c], a);
-      a = group_sum_dpp<kWave>(a);
-      const double vi = v[i] - a;
-      if (lane == 0) {
+      if (i < m) {
+        const T *row = Ub + (int64_t)i * ldu;
+        for (int c = gl; c < r; c += kRowLanes) a = fma((double)row[c], y[c], a);
+      }
+      a = group_sum_dpp<kRowLanes>(a);
+      if (gl == 0 && i < m) {
+        const double vi = v[i] - a;
         v[i] = vi;
         l2 = fma(vi, vi, l2);
       }
     }
+    // the four group leaders' sums, in lane order
+    l2 = (readlane_double(l2, 0) + readlane_double(l2, 16)) + (readlane_double(l2, 32) + readlane_double(l2, 48));
     if (lane == 0) red[wave] = l2;
     __syncthreads();
     double len2 = 0.0;
