@@ -1,6 +1,6 @@
 """CPU oracle for the standalone SVD hot path -- TEST INFRASTRUCTURE, NOT PRODUCT CODE.
 
-This package is the checker.  Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline``
+It checks the GPU path and is never shipped as part of it.  Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline``
 leg of ``bench.py`` may import it, and only to check or to time the CPU restatement beside the
 GPU path.  The product package never imports it and never falls back to it.
 
