@@ -120,11 +120,12 @@ __global__ __launch_bounds__(kJbThreads) void bj_colmax_kernel(BjParams<T> P) {
 
 // Gram partial of pair p over the rows of slice z: 64 x 64, FP64 MFMA (16x16x4), wave w owns the
 // 32 x 32 quadrant (w / 2, w % 2).
+constexpr int kGramRows = 64;   // rows of W staged per step of the Gram partial
+
 template <typename T>
-__global__ __launch_bounds__(kJbThreads) void bj_gram_kernel(BjParams<T> P, int r) {
-  constexpr int KR = 64;                          // rows staged per step
-  __shared__ double Ws[KR][kJs + 1];
-  const int p = blockIdx.x, z = blockIdx.y, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+__device__ __forceinline__ void bj_gram_body(const BjParams<T> &P, int p, int z, int r, double (*Ws)[kJs + 1]) {
+  constexpr int KR = kGramRows;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int lr = lane & 15, lk = lane >> 4, wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
   int I, J;
   rr_pair(r, p, P.nb, I, J);
@@ -295,11 +296,10 @@ __host__ __device__ inline size_t bj_update_lds() {
 }
 
 template <typename T>
-__global__ __launch_bounds__(kJbThreads) void bj_update_kernel(BjParams<T> P, int r) {
-  extern __shared__ double smem_bu2[];
-  T (*Xs)[kJsLd] = reinterpret_cast<T (*)[kJsLd]>(smem_bu2);
+__device__ __forceinline__ void bj_update_body(const BjParams<T> &P, int p, int t, int r, double *smem) {
+  T (*Xs)[kJsLd] = reinterpret_cast<T (*)[kJsLd]>(smem);
   T (*Qs)[kJsLd] = Xs + kJs;
-  const int p = blockIdx.x, t = blockIdx.y, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   if (!P.act[p]) return;
   const int lr = lane & 15, lk = lane >> 4, wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
   const int mt = (P.m + kJs - 1) / kJs;
@@ -362,6 +362,37 @@ __global__ __launch_bounds__(kJbThreads) void bj_update_kernel(BjParams<T> P, in
       for (int u = 0; u < VE; ++u) x.v[u] = Xs[i][j + u];
       *reinterpret_cast<V16 *>(X + (int64_t)row * P.N + bj_col(I, J, j)) = x;
     }
+  }
+}
+
+// W tiles of round r: blockIdx.x = pair, blockIdx.y = 64-row tile of W.
+template <typename T>
+__global__ __launch_bounds__(kJbThreads) void bj_update_kernel(BjParams<T> P, int r) {
+  extern __shared__ double smem_bu2[];
+  bj_update_body<T>(P, (int)blockIdx.x, (int)blockIdx.y, r, smem_bu2);
+}
+
+template <typename T>
+__host__ __device__ inline size_t bj_round_lds() {
+  const size_t gram = sizeof(double) * kGramRows * (kJs + 1), upd = bj_update_lds<T>();
+  return gram > upd ? gram : upd;
+}
+
+// First launch of round r: workgroups [0, ngram) form the Gram partials of round r (pair
+// b % npair, row slice b / npair); the rest, when vr >= 0, apply round vr's rotations to the V
+// tiles (pair, tile) -- V is read by no kernel of the sweep, and Q and the activity flags still
+// hold round vr until round r's subproblem sweeps run, so the V products of a round run beside
+// the next round's Gram partials.
+template <typename T>
+__global__ __launch_bounds__(kJbThreads) void bj_round_kernel(BjParams<T> P, int r, int ngram, int vr, int mt,
+                                                             int nt) {
+  extern __shared__ double smem_br2[];
+  const int b = (int)blockIdx.x, npair = P.nb / 2;
+  if (b < ngram) {
+    bj_gram_body<T>(P, b % npair, b / npair, r, reinterpret_cast<double (*)[kJs + 1]>(smem_br2));
+  } else {
+    const int t = b - ngram;
+    bj_update_body<T>(P, t / nt, mt + t % nt, vr, smem_br2);
   }
 }
 
@@ -492,7 +523,9 @@ static int run_bj(int m, int n, const T *a, T *u, T *s, T *vh, int32_t *info, do
     if (hipFuncSetAttribute((const void *)bj_inner_kernel<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)bj_inner_lds<T>()) != hipSuccess ||
         hipFuncSetAttribute((const void *)bj_update_kernel<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)bj_update_lds<T>()) != hipSuccess)
+                            (int)bj_update_lds<T>()) != hipSuccess ||
+        hipFuncSetAttribute((const void *)bj_round_kernel<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)bj_round_lds<T>()) != hipSuccess)
       return PSVD_ELAUNCH;
     attr = true;
   }
@@ -537,15 +570,19 @@ static int run_bj(int m, int n, const T *a, T *u, T *s, T *vh, int32_t *info, do
   hipLaunchKernelGGL(bj_absmax_kernel<T>, dim3(blocks((int64_t)m * n)), dim3(kJbThreads), 0, st, P);
   hipLaunchKernelGGL(bj_load_kernel<T>, dim3(blocks((int64_t)m * P.N + (int64_t)P.N * P.N)), dim3(kJbThreads), 0, st,
                      P);
-  int sweeps = 0, converged = 0;
+  int sweeps = 0, converged = 0, vr = -1;
+  const int ngram = npair * P.splits, nvt = npair * nt;
   for (int sw = 1; sw <= max_sweeps; ++sw) {
     sweeps = sw;
     hipLaunchKernelGGL(bj_reset_kernel, dim3(1), dim3(64), 0, st, P.slot, P.flag, 2);
     hipLaunchKernelGGL(bj_colmax_kernel<T>, dim3((n + 15) / 16), dim3(kJbThreads), 0, st, P);
     for (int r = 0; r < P.nb - 1; ++r) {
-      hipLaunchKernelGGL(bj_gram_kernel<T>, dim3(npair, P.splits), dim3(kJbThreads), 0, st, P, r);
+      // Gram partials of round r beside the V tiles of the previous round (none before the first)
+      hipLaunchKernelGGL(bj_round_kernel<T>, dim3((unsigned)(ngram + (vr >= 0 ? nvt : 0))), dim3(kJbThreads),
+                         bj_round_lds<T>(), st, P, r, ngram, vr, mt, nt);
       hipLaunchKernelGGL(bj_inner_kernel<T>, dim3(npair), dim3(kJbInner), bj_inner_lds<T>(), st, P, r);
-      hipLaunchKernelGGL(bj_update_kernel<T>, dim3(npair, mt + nt), dim3(kJbThreads), bj_update_lds<T>(), st, P, r);
+      hipLaunchKernelGGL(bj_update_kernel<T>, dim3(npair, mt), dim3(kJbThreads), bj_update_lds<T>(), st, P, r);
+      if (nt > 0) vr = r;
     }
     if (hipGetLastError() != hipSuccess) return PSVD_ELAUNCH;
     int active = 0;
@@ -556,6 +593,10 @@ static int run_bj(int m, int n, const T *a, T *u, T *s, T *vh, int32_t *info, do
       break;
     }
   }
+  // the last round's rotations on V
+  if (vr >= 0 && nvt > 0)
+    hipLaunchKernelGGL(bj_round_kernel<T>, dim3((unsigned)nvt), dim3(kJbThreads), bj_round_lds<T>(), st, P, 0, 0, vr,
+                       mt, nt);
   hipLaunchKernelGGL(bj_sigma_kernel<T>, dim3((n + 15) / 16), dim3(kJbThreads), 0, st, P);
   hipLaunchKernelGGL(bj_order_kernel<T>, dim3((n + kJbThreads - 1) / kJbThreads), dim3(kJbThreads), 0, st, P);
   hipLaunchKernelGGL(bj_out_kernel<T>, dim3(blocks((int64_t)m * n)), dim3(kJbThreads), 0, st, P, sweeps, converged);
