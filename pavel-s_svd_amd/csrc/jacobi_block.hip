@@ -183,7 +183,9 @@ __host__ __device__ inline size_t bj_inner_lds() {
 }
 
 // Sweeps of S_p (FP64, in LDS) with the one-sided test; Q_p = accumulated rotation.
-constexpr int kJbInner = 512;
+// 576 threads: the 528 upper-triangle 2 x 2 blocks of a round are one item per thread, so no
+// thread carries a second block on the round's critical path (512 threads left 16 with two)
+constexpr int kJbInner = 576;
 
 template <typename T>
 __global__ __launch_bounds__(kJbInner) void bj_inner_kernel(BjParams<T> P, int r) {
@@ -224,7 +226,7 @@ __global__ __launch_bounds__(kJbInner) void bj_inner_kernel(BjParams<T> P, int r
     it1[j] = it2[j] = -1;
     if (e < NTRI) tri_decode(e, kJb, it1[j], it2[j]);
   }
-  constexpr int QIT = kJb * kJs / kJbInner;
+  constexpr int QSTEP = kJbInner / kJs, QIT = (kJb + QSTEP - 1) / QSTEP;   // Qt row pairs per column
   const int qcol = tid % kJs, qk = tid / kJs;
   for (int sw = 0; live && sw < P.inner_sweeps; ++sw) {
     int act_sw = 0;
@@ -251,7 +253,8 @@ __global__ __launch_bounds__(kJbInner) void bj_inner_kernel(BjParams<T> P, int r
       }
 #pragma unroll
       for (int j = 0; j < QIT; ++j) {
-        const int k = qk + j * (kJbInner / kJs);
+        const int k = qk + j * QSTEP;
+        if (k >= kJb) continue;
         const double2 r = cs[k];
         if (r.y == 0.0) continue;
         const int2 a = pq[k];
