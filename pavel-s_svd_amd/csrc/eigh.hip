@@ -498,7 +498,9 @@ __host__ __device__ inline size_t blk_inner_lds() {
   return sizeof(double) * 2 * kEb + sizeof(int) * 2 * kEb + sizeof(T) * 2 * (size_t)kEs * kEsLd;
 }
 
-constexpr int kBlkInner = 512;
+// 576 threads: the 528 upper-triangle 2 x 2 blocks of a round are one item per thread, so no
+// thread carries a second block on the round's critical path (512 threads left 16 with two)
+constexpr int kBlkInner = 576;
 
 // Rotations Q_p^T and activity flags of round g live in buffer g & 1: the launch that sweeps the
 // subproblems of round g also applies round g - 1's rotations to V^T (see blk_round_kernel).
@@ -546,7 +548,7 @@ __device__ __forceinline__ void blk_inner_body(const BlkParams<T> &P, int p, int
     it1[j] = it2[j] = -1;
     if (e < NTRI) tri_decode(e, kEb, it1[j], it2[j]);
   }
-  constexpr int QIT = kEb * kEs / kBlkInner;
+  constexpr int QSTEP = kBlkInner / kEs, QIT = (kEb + QSTEP - 1) / QSTEP;   // Qt row pairs per column
   const int qcol = tid % kEs, qk = tid / kEs;
   for (int sw = 0; live && sw < P.inner_sweeps; ++sw) {
     int act_sw = 0;
@@ -572,7 +574,8 @@ __device__ __forceinline__ void blk_inner_body(const BlkParams<T> &P, int p, int
       }
 #pragma unroll
       for (int j = 0; j < QIT; ++j) {
-        const int k = qk + j * (kBlkInner / kEs);
+        const int k = qk + j * QSTEP;
+        if (k >= kEb) continue;
         const double2 r = cs[k];
         if (r.y == 0.0) continue;
         const int2 a = pq[k];
