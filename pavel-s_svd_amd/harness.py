@@ -1,16 +1,21 @@
-"""One measurement row in the reference harness's JSONL format for this package's solvers.
+"""Measurement rows in the reference harness's JSONL wire format, written by this package.
 
-The reference harness (svdbench/cli.py) already runs these solvers through the plugin hook
-``{"name": "pavel-s_svd_amd:factory", ...}``; this module produces the same row without it, for a
-caller that drives the MI355X solvers directly: the timing contract of cli.py:86-105
-(synchronise, wall clock and device events per sample, median / p10 / p90) and the row of
-cli.py:108-183 (input copy, prepare, warm-ups, resident-solver timing, allocator memory, adapter
-metadata, native diagnostics, output copy, metrics, end-to-end timing, warnings, status with
-``unsupported`` for an ``Unsupported`` solver).  ``memory.native_workspace_bytes`` is filled from
-the adapter -- 0, since the library takes all workspace from the caller through torch's allocator
--- and ``memory.library_workspace_bytes`` records what the last solve requested.  The metrics are
-the caller's (``validate(a_cpu, host_result)``), as the harness computes them against its own FP64
-reference; without one the row carries ``metrics: None``.
+The reference's own CLI (svdbench/cli.py) drives these solvers unchanged through its plugin hook
+``{"name": "pavel-s_svd_amd:factory", ...}``.  This module is for a caller that drives the MI355X
+solvers directly and still wants rows that the reference's report tools read: the field names and
+their meaning follow the reference's rows (cli.py:108-183, pinned by the row schema extracted from
+its committed results); the code is organised around this package's own needs.
+
+Design: a row is produced by a fixed pipeline of stages (upload, prepare, warm-up, resident timing,
+checked solve, download, optional end-to-end timing).  Each stage adds its fields to the row; the
+first exception ends the pipeline and sets the row's status (``unsupported`` for ``Unsupported``,
+``error`` otherwise), so a partially measured row still carries the stages that completed.  Timing
+uses one clock object that drains the device before and after every sample, reads the host wall
+clock, and brackets the call with HIP events on torch's current stream.  The memory block reports
+the caching allocator's figures and the library's own device allocation, which is always 0: every
+workspace of ``libpavelsvd.so`` is a caller tensor from the allocator
+(``memory.native_workspace_bytes``; the reference hardcodes None there, cli.py:145), with the last
+solve's requests beside it (``memory.library_workspace_bytes``).
 """
 import gc
 import hashlib
@@ -22,46 +27,69 @@ import torch
 
 from .standalone.solvers import Unsupported, factory
 
-
-def synchronize(device):
-    if device.type == 'cuda':
-        torch.cuda.synchronize(device)
+_QUANTILES = (0.5, 0.1, 0.9)   # median, p10, p90 with numpy's linear interpolation
 
 
-def to_host(result):
-    return tuple(None if x is None else x.detach().cpu().resolve_conj().resolve_neg().numpy().copy() for x in result)
+def _stats(samples):
+    """{'samples_ms', 'median_ms', 'p10_ms', 'p90_ms'} of a list of millisecond samples."""
+    med, lo, hi = (float(q) for q in np.quantile(np.asarray(samples, dtype=np.float64), _QUANTILES))
+    return {'samples_ms': list(samples), 'median_ms': med, 'p10_ms': lo, 'p90_ms': hi}
 
 
-def summary(samples):
-    return {'samples_ms': samples, 'median_ms': float(np.median(samples)),
-            'p10_ms': float(np.percentile(samples, 10)), 'p90_ms': float(np.percentile(samples, 90))}
+class _Clock:
+    """Synchronised samples on one device: host wall time and, on a GPU, HIP event time."""
+
+    def __init__(self, device):
+        self.device = device
+        self.gpu = device.type == 'cuda'
+
+    def drain(self):
+        if self.gpu:
+            torch.cuda.synchronize(self.device)
+
+    def sample(self, fn, events):
+        """One call of fn: (wall ms, event ms or None).  The result is dropped before returning, so no
+        two samples' outputs are alive together (the allocator peak is one solve's)."""
+        marks = [torch.cuda.Event(enable_timing=True) for _ in range(2)] if events and self.gpu else None
+        self.drain()
+        t0 = time.perf_counter()
+        if marks:
+            marks[0].record()
+        out = fn()
+        if marks:
+            marks[1].record()
+        self.drain()
+        wall = (time.perf_counter() - t0) * 1e3
+        del out
+        return wall, (marks[0].elapsed_time(marks[1]) if marks else None)
+
+    def series(self, fn, repeat, events=False):
+        walls, evs = [], []
+        for _ in range(repeat):
+            w, e = self.sample(fn, events)
+            walls.append(w)
+            if e is not None:
+                evs.append(e)
+        return {'wall': _stats(walls), 'device_event': _stats(evs) if evs else None}
+
+    def stopwatch(self, fn):
+        """(result, ms) of one drained call whose result is kept."""
+        self.drain()
+        t0 = time.perf_counter()
+        out = fn()
+        self.drain()
+        return out, (time.perf_counter() - t0) * 1e3
 
 
-def measure(fn, device, repeat, events=False):
-    """Wall-clock (and device-event) samples of `repeat` synchronised calls (cli.py:86-105)."""
-    wall, dev = [], []
-    for _ in range(repeat):
-        start = end = None
-        if events and device.type == 'cuda':
-            start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        synchronize(device)
-        t = time.perf_counter()
-        if start:
-            start.record()
-        result = fn()
-        if end:
-            end.record()
-        synchronize(device)
-        wall.append((time.perf_counter() - t) * 1e3)
-        if start:
-            dev.append(start.elapsed_time(end))
-        del result
-    return {'wall': summary(wall), 'device_event': summary(dev) if dev else None}
+def _host_copy(outputs):
+    """numpy copies of the solver's (U, s, Vh); conjugate and negative views are materialised."""
+    return tuple(None if t is None else t.detach().cpu().resolve_conj().resolve_neg().numpy().copy()
+                 for t in outputs)
 
 
 def prepare(a, spec):
-    """The harness's checks before a plugin factory (algorithms.py:20-27: mode, rank), then this
-    package's factory with the spec unchanged, as the plugin hook passes it (algorithms.py:90-93)."""
+    """The harness's checks before a plugin factory (svdbench/algorithms.py:20-27: mode, rank), then
+    this package's factory with the spec unchanged, as the plugin hook passes it (algorithms.py:90-93)."""
     mode = spec.get('mode', 'reduced')
     if mode not in ('reduced', 'full', 'values'):
         raise ValueError('mode must be reduced, full or values')
@@ -71,75 +99,101 @@ def prepare(a, spec):
     return factory(a, spec)
 
 
+class _RowPipeline:
+    """State shared by the stages of one row."""
+
+    def __init__(self, a_cpu, spec, device, warmup, repeat, end_to_end, validate):
+        self.a_cpu, self.spec, self.device = a_cpu, spec, device
+        self.warmup, self.repeat, self.want_e2e, self.validate = warmup, repeat, end_to_end, validate
+        self.clock = _Clock(device)
+        self.a = self.solver = None
+
+    # -- stages: each adds its fields to the row ------------------------------------------------------
+    def upload(self, row):
+        self.a, row['initial_input_copy_ms'] = self.clock.stopwatch(
+            lambda: torch.from_numpy(self.a_cpu).to(device=self.device, copy=True))
+
+    def build(self, row):
+        self.solver, ms = self.clock.stopwatch(lambda: prepare(self.a, self.spec))
+        row['adapter'] = self.solver.metadata
+        row['prepare_ms'] = ms
+
+    def warm(self, row):
+        for _ in range(self.warmup):
+            self.clock.sample(self.solver, events=False)
+        gc.collect()
+
+    def resident(self, row):
+        before = self._allocator() if self.clock.gpu else None
+        row['resident_solver'] = self.clock.series(self.solver, self.repeat, events=True)
+        row['memory'] = self._memory(before)
+
+    def checked_solve(self, row):
+        outputs, _ = self.clock.stopwatch(self.solver)
+        row['native_diagnostics'] = self.solver.diagnostics
+        host, row['output_copy_ms'] = self.clock.stopwatch(lambda: _host_copy(outputs))
+        del outputs
+        row['memory']['output_bytes'] = sum(x.nbytes for x in host if x is not None)
+        row['metrics'] = None if self.validate is None else self.validate(self.a_cpu, host)
+
+    def end_to_end(self, row):
+        if not self.want_e2e:
+            return
+
+        def full_call():
+            x = torch.from_numpy(self.a_cpu).to(device=self.device, copy=True)
+            return _host_copy(prepare(x, self.spec)())
+        row['end_to_end'] = self.clock.series(full_call, self.repeat)
+
+    STAGES = ('upload', 'build', 'warm', 'resident', 'checked_solve', 'end_to_end')
+
+    # -- memory block ---------------------------------------------------------------------------------
+    def _allocator(self):
+        torch.cuda.reset_peak_memory_stats(self.device)
+        return torch.cuda.memory_allocated(self.device), torch.cuda.memory_reserved(self.device)
+
+    def _memory(self, before):
+        mem = {'input_bytes': self.a_cpu.nbytes,
+               'native_workspace_bytes': getattr(self.solver, 'native_workspace_bytes', None)}
+        if before is None:
+            return mem
+        allocated, reserved = before
+        peak = torch.cuda.max_memory_allocated(self.device)
+        mem.update(allocator_baseline_bytes=allocated, allocator_peak_allocated_bytes=peak,
+                   allocator_peak_increment_bytes=peak - allocated, allocator_reserved_baseline_bytes=reserved,
+                   allocator_peak_reserved_bytes=torch.cuda.max_memory_reserved(self.device),
+                   library_workspace_bytes=getattr(self.solver, 'library_workspace_bytes', None),
+                   note='PyTorch caching allocator; libpavelsvd.so allocates no device memory, its workspace is '
+                        'requested from the allocator and included above. Warm caches retained.')
+        return mem
+
+    def release(self):
+        self.a = self.solver = None
+        gc.collect()
+
+
 def measurement_row(a_cpu, case, spec, device, *, warmup=2, repeat=5, end_to_end=False, validate=None):
-    """The harness row for solving the numpy matrix `a_cpu` with `spec` on `device` (cli.py:108-183)."""
+    """One harness row for solving the numpy matrix `a_cpu` with `spec` on `device`.
+
+    Fields as the reference's rows (cli.py:108-183): input copy, prepare, warm-ups, resident-solver
+    timing (wall and device events, median / p10 / p90), allocator memory, adapter metadata, native
+    diagnostics, output copy, metrics from ``validate(a_cpu, host_outputs)`` (None without one),
+    optional end-to-end timing, warnings, status, input digest."""
     device = torch.device(device)
     row = {'type': 'measurement', 'case': case, 'algorithm': spec, 'device': str(device), 'status': 'error'}
-    a = fn = result = None
+    job = _RowPipeline(a_cpu, spec, device, warmup, repeat, end_to_end, validate)
     try:
         with warnings.catch_warnings(record=True) as caught, torch.inference_mode():
             warnings.simplefilter('always')
-            synchronize(device)
-            t = time.perf_counter()
-            a = torch.from_numpy(a_cpu).to(device=device, copy=True)
-            synchronize(device)
-            row['initial_input_copy_ms'] = (time.perf_counter() - t) * 1e3
-            t = time.perf_counter()
-            fn = prepare(a, spec)
-            row['adapter'] = fn.metadata
-            synchronize(device)
-            row['prepare_ms'] = (time.perf_counter() - t) * 1e3
-            for _ in range(warmup):
-                result = fn()
-                synchronize(device)
-                del result
-                result = None
-            gc.collect()
-            if device.type == 'cuda':
-                torch.cuda.reset_peak_memory_stats(device)
-                baseline = torch.cuda.memory_allocated(device)
-                reserved_baseline = torch.cuda.memory_reserved(device)
-            row['resident_solver'] = measure(fn, device, repeat, events=True)
-            native = getattr(fn, 'native_workspace_bytes', None)
-            if device.type == 'cuda':
-                row['memory'] = {
-                    'input_bytes': a_cpu.nbytes,
-                    'allocator_baseline_bytes': baseline,
-                    'allocator_peak_allocated_bytes': torch.cuda.max_memory_allocated(device),
-                    'allocator_peak_increment_bytes': torch.cuda.max_memory_allocated(device) - baseline,
-                    'allocator_reserved_baseline_bytes': reserved_baseline,
-                    'allocator_peak_reserved_bytes': torch.cuda.max_memory_reserved(device),
-                    'native_workspace_bytes': native,
-                    'library_workspace_bytes': getattr(fn, 'library_workspace_bytes', None),
-                    'note': 'PyTorch allocator; the library allocates no device memory of its own, its workspace '
-                            'is inside the allocator figures. Warm caches retained.',
-                }
-            else:
-                row['memory'] = {'input_bytes': a_cpu.nbytes, 'native_workspace_bytes': native}
-            result = fn()
-            synchronize(device)
-            row['native_diagnostics'] = fn.diagnostics
-            t = time.perf_counter()
-            host_result = to_host(result)
-            synchronize(device)
-            row['output_copy_ms'] = (time.perf_counter() - t) * 1e3
-            row['memory']['output_bytes'] = sum(x.nbytes for x in host_result if x is not None)
-            row['metrics'] = validate(a_cpu, host_result) if validate is not None else None
-            del result
-            result = None
-            if end_to_end:
-                def e2e():
-                    x = torch.from_numpy(a_cpu).to(device=device, copy=True)
-                    return to_host(prepare(x, spec)())
-                row['end_to_end'] = measure(e2e, device, repeat)
+            for stage in job.STAGES:
+                getattr(job, stage)(row)
             row['warnings'] = sorted({str(w.message) for w in caught})
-            row['status'] = 'ok'
-    except Unsupported as e:
-        row.update(status='unsupported', error=str(e))
-    except Exception as e:
-        row.update(status='error', error=f'{type(e).__name__}: {e}')
+        row['status'] = 'ok'
+    except Unsupported as exc:
+        row['status'], row['error'] = 'unsupported', str(exc)
+    except Exception as exc:   # the row records the failure instead of raising, as the harness does
+        row['status'], row['error'] = 'error', f'{type(exc).__name__}: {exc}'
     finally:
-        del result, fn, a
-        gc.collect()
+        job.release()
     row['input_sha256'] = hashlib.sha256(np.ascontiguousarray(a_cpu).tobytes()).hexdigest()
     return row
