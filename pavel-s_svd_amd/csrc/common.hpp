@@ -239,8 +239,15 @@ __device__ __forceinline__ void jacobi_rotation_t<float>(double app, double aqq,
   // native v_rcp_f32 / v_sqrt_f32 (about 1 ulp): t to a few ulps is all the angle needs
   // Branch-free: at tau = 0 (never -0: app, aqq are sums of squares) the general formula gives
   // rcp(0 + sqrt(1)) = 1 exactly, the reference's t = 1.
-  const float fa = (float)fabs(apq);
-  const float tau = (float)(aqq - app) * __builtin_amdgcn_rcpf(2.0f * fa);
+  // tau in FP32 when its FP64 operands are normal FP32 numbers; otherwise (unscaled data whose sums
+  // leave the FP32 range, e.g. entries near 1e-25 or 1e20) the quotient is formed in FP64 and rounded,
+  // so |apq| never becomes 0 or inf in FP32 and tau stays finite (or +-inf, which gives t = 0)
+  const double a2 = 2.0 * fabs(apq), dd = aqq - app;
+  float tau;
+  if (a2 > 1e-30 && a2 < 1e30 && fabs(dd) < 1e37)
+    tau = (float)dd * __builtin_amdgcn_rcpf((float)a2);
+  else
+    tau = (float)(dd / a2);
   const float at = fabsf(tau);
   float t = at > 1e18f ? 0.5f * __builtin_amdgcn_rcpf(at)
                        : __builtin_amdgcn_rcpf(at + __builtin_amdgcn_sqrtf(fmaf(at, at, 1.0f)));
