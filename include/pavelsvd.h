@@ -88,6 +88,21 @@ int psvd_jacobi_lds(int dtype, int64_t batch, int m_uniform, int n_uniform,
                     double tol, int max_sweeps, int flags, int lanes, int threads,
                     void *workspace, size_t workspace_bytes, psvd_stream_t stream);
 
+/* psvd_jacobi_lds_noalloc: psvd_jacobi_lds without a workspace argument and without any device
+ * allocation.  No counterpart in the reference: it is for callers that cannot allocate (code under
+ * hipGraph capture, tight serving loops, harnesses that forbid hipMalloc).  The launch's only scratch,
+ * the device work-queue counter, lives in a library-owned 256-byte device array per GPU, held until
+ * the process ends; it is cleared by a fill kernel at the start of every call.  Every call on a GPU
+ * shares that GPU's array, so calls that overlap on one GPU (two streams, two host threads) race and
+ * must be serialised by the caller.  Maximum: batch <= 2^31 - 1 (one int32 counter), else
+ * PSVD_ETOOBIG.  Same arguments, outputs and statuses as psvd_jacobi_lds otherwise. */
+int psvd_jacobi_lds_noalloc(int dtype, int64_t batch, int m_uniform, int n_uniform,
+                            const int32_t *dims, const int64_t *offs, const int32_t *order,
+                            int max_m, int max_n,
+                            const void *a, void *u, void *s, void *vh, int32_t *info,
+                            double tol, int max_sweeps, int flags, int lanes, int threads,
+                            psvd_stream_t stream);
+
 /* ---------------------------------------------------------------------------------------
  * Batched strided GEMM on MFMA: C_i = alpha * op(A_i) * op(B_i) + beta * C_i for i < batch,
  * row-major: op(A) is m x k (A is m x k, or k x m when transa), op(B) is k x n (B is k x n, or

@@ -45,6 +45,24 @@ static PyObject *py_jacobi_lds(PyObject *, PyObject *args) {
   return PyLong_FromLong(st);
 }
 
+static PyObject *py_jacobi_lds_noalloc(PyObject *, PyObject *args) {
+  int dtype, mu, nu, max_m, max_n, max_sweeps, flags, lanes, threads;
+  long long batch;
+  unsigned long long dims, offs, order, a, u, s, vh, info, stream;
+  double tol;
+  if (!PyArg_ParseTuple(args, "iLiiKKKiiKKKKKdiiiiK", &dtype, &batch, &mu, &nu, &dims, &offs, &order, &max_m,
+                        &max_n, &a, &u, &s, &vh, &info, &tol, &max_sweeps, &flags, &lanes, &threads, &stream))
+    return nullptr;
+  int st;
+  Py_BEGIN_ALLOW_THREADS
+  st = psvd_jacobi_lds_noalloc(dtype, batch, mu, nu, static_cast<const int32_t *>(PTR(dims)),
+                               static_cast<const int64_t *>(PTR(offs)), static_cast<const int32_t *>(PTR(order)),
+                               max_m, max_n, PTR(a), PTR(u), PTR(s), PTR(vh), static_cast<int32_t *>(PTR(info)), tol,
+                               max_sweeps, flags, lanes, threads, PTR(stream));
+  Py_END_ALLOW_THREADS
+  return PyLong_FromLong(st);
+}
+
 static PyObject *py_jacobi_grid_workspace_bytes(PyObject *, PyObject *args) {
   int dtype, m, n;
   if (!PyArg_ParseTuple(args, "iii", &dtype, &m, &n)) return nullptr;
@@ -391,6 +409,7 @@ static PyMethodDef kMethods[] = {
     {"jacobi_lds_workspace_bytes", py_jacobi_lds_workspace_bytes, METH_VARARGS, nullptr},
     {"jacobi_lds_smem_bytes", py_jacobi_lds_smem_bytes, METH_VARARGS, nullptr},
     {"jacobi_lds", py_jacobi_lds, METH_VARARGS, nullptr},
+    {"jacobi_lds_noalloc", py_jacobi_lds_noalloc, METH_VARARGS, nullptr},
     {"jacobi_grid_workspace_bytes", py_jacobi_grid_workspace_bytes, METH_VARARGS, nullptr},
     {"jacobi_grid", py_jacobi_grid, METH_VARARGS, nullptr},
     {"jacobi_blocked_workspace_bytes", py_jacobi_blocked_workspace_bytes, METH_VARARGS, nullptr},
