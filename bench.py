@@ -63,22 +63,159 @@ def accuracy(batch, flat, ms, ns, offs):
     return worst
 
 
-def cpu_baseline(flat, ms, ns, offs, budget_s=15.0):
-    """The oracle (numpy restatement of the reference's standalone Jacobi path) on one host core,
-    looping over the batch as the reference API requires, for about ``budget_s`` seconds."""
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo) and the cores this process may run on."""
+    name = 'unknown'
+    try:
+        with open('/proc/cpuinfo') as f:
+            for line in f:
+                if line.startswith('model name'):
+                    name = line.split(':', 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return name, len(os.sched_getaffinity(0))
+
+
+_POOL_BATCH = None   # (flat, ms, ns, offs) handed to forked workers
+
+
+def _cpu_worker(args):
+    """Worker w of N: the oracle on matrices w, w + N, ... (one host core, one BLAS thread) until
+    `budget_s` has passed; returns (solved, seconds)."""
+    w, nw, budget_s = args
     from threadpoolctl import threadpool_limits
     from oracle import restatement as R
+    flat, ms, ns, offs = _POOL_BATCH
     done, t0 = 0, time.perf_counter()
     with threadpool_limits(1):
-        for m, n, o in zip(ms, ns, offs):
+        for i in range(w, len(ms), nw):
+            m, n, o = ms[i], ns[i], offs[i]
             R.svd(flat[o:o + m * n].reshape(m, n), 'jacobi')
             done += 1
-            if time.perf_counter() - t0 > budget_s and done >= 16:
+            if time.perf_counter() - t0 > budget_s:
                 break
-    dt = time.perf_counter() - t0
-    return {'value': done / dt, 'unit': 'SVDs/s', 'cores': 1, 'kind': 'port',
-            'sample': f'first {done} of the {COUNT} matrices, one svd(..., "jacobi") call each, '
-                      f'{dt:.1f} s on 1 host core (numpy, FP32 arithmetic as the reference)'}
+    return done, time.perf_counter() - t0
+
+
+def cpu_baseline(flat, ms, ns, offs, budget_s=10.0):
+    """The oracle (numpy restatement of the reference's standalone Jacobi path) on the host, one
+    svd(a_i, 'jacobi') call per matrix as the reference's uniform-shape API requires, for about
+    ``budget_s`` seconds: with all cores of this process's affinity (one worker process per core,
+    at most 16, each on an interleaved share of the batch) and with 4 workers, as the reference's
+    published CPU runs used 4 threads.  Runs before this process touches the GPU (forked workers)."""
+    import multiprocessing as mp
+    global _POOL_BATCH
+    _POOL_BATCH = (flat, ms, ns, offs)
+    model, cores = cpu_model()
+    rows = {}
+    for workers in sorted({min(cores, 16), min(cores, 4)}, reverse=True):
+        t0 = time.perf_counter()
+        with mp.get_context('fork').Pool(workers) as pool:
+            res = pool.map(_cpu_worker, [(w, workers, budget_s) for w in range(workers)])
+        wall = time.perf_counter() - t0
+        done = sum(d for d, _ in res)
+        rows[workers] = {'value': done / max(t for _, t in res), 'matrices': done, 'wall_s': wall}
+    best = max(rows)
+    out = {'value': rows[best]['value'], 'unit': 'SVDs/s', 'cores': best, 'kind': 'port', 'cpu': model,
+           'affinity_cores': cores,
+           'sample': f'{rows[best]["matrices"]} of the {COUNT} matrices (interleaved shares, {budget_s:.0f} s per '
+                     f'worker), one svd(..., "jacobi") call each, {best} worker processes x 1 thread (numpy, FP32 '
+                     f'arithmetic as the reference)'}
+    if 4 in rows and best != 4:
+        out['cores_4'] = {'value': rows[4]['value'], 'matrices': rows[4]['matrices']}
+    return out
+
+
+def _scaled_phase(fn, small, full):
+    """Seconds of fn() at size `small`, scaled by (full / small)^3 (the O(n^3) phases of the oracle)."""
+    t0 = time.perf_counter()
+    try:
+        fn()
+    except RuntimeError:       # a capped sweep count raises, as the reference does; the time is what counts
+        pass
+    return (time.perf_counter() - t0) * (full / small) ** 3
+
+
+def cpu_single(a):
+    """The oracle's one-sided Jacobi on the 512 x 256 matrix at tol 1e-7 (whether the reference's
+    FP32 arithmetic converges there) and at the reference's default tol (10 eps32), one core."""
+    from threadpoolctl import threadpool_limits
+    from oracle import restatement as R
+    out = {'unit': 's/solve', 'cores': 1, 'kind': 'port', 'cpu': cpu_model()[0]}
+    with threadpool_limits(1):
+        for key, tol in (('tol_1e-7', 1e-7), ('tol_default', None)):
+            t0 = time.perf_counter()
+            try:
+                _, _, _, sweeps = R.jacobi_svd(a[None], tol=tol, max_sweeps=60)
+                row = {'converged': True, 'jacobi_sweeps': int(sweeps)}
+            except RuntimeError:
+                row = {'converged': False, 'jacobi_sweeps': 60}
+            row['seconds'] = time.perf_counter() - t0
+            out[key] = row
+    out['value'] = out['tol_1e-7']['seconds']
+    out['sample'] = ('the whole solve (numpy, FP32 arithmetic as the reference; numpy elementwise work runs on one '
+                     'core whatever the thread count)')
+    return out
+
+
+def cpu_qdwh(n, qr_iterations, chol_iterations, eigen_sweeps, small=768, seed=SEED):
+    """Extrapolated oracle time of one QDWH solve at n: each phase of the restatement (the QR of
+    [sqrt(c) X; I], the Cholesky and two triangular solves, one sweep of the two-sided Jacobi
+    eigensolver) timed once at n = `small` on the same workload family, scaled by (n / small)^3 and
+    multiplied by the counts the GPU run executed.  In-cache small problems make this a lower bound."""
+    from threadpoolctl import threadpool_limits
+    from oracle import matrices as X
+    from oracle import restatement as R
+    a, _ = X.workload_qdwh(small, seed)
+    x = (a / np.linalg.norm(a)).astype(np.float32)
+    g = (np.eye(small, dtype=np.float32) + 3.0 * x.T @ x)[None]
+    with threadpool_limits(1):
+        stacked = np.concatenate([x, np.eye(small, dtype=np.float32)])[None]
+        t_qr = _scaled_phase(lambda: R.qr(stacked), small, n)
+        t_chol = _scaled_phase(lambda: R.cholesky(g), small, n)
+        low = np.tril(g)
+        t_trsm = _scaled_phase(lambda: R.trsm_lower(low, x.T[None].copy()), small, n)
+        h = (x.T @ a)[None]
+        h = ((h + np.swapaxes(h, 1, 2)) / 2).astype(np.float32)
+        t_sweep = _scaled_phase(lambda: R.jacobi_eigh(h, max_sweeps=1), small, n)
+    total = qr_iterations * t_qr + chol_iterations * (t_chol + 2 * t_trsm) + eigen_sweeps * t_sweep
+    return {'value': total, 'unit': 's/solve', 'cores': 1, 'kind': 'port', 'cpu': cpu_model()[0],
+            'extrapolated': True,
+            'phases_s': {'qr_iteration': t_qr, 'cholesky_iteration': t_chol + 2 * t_trsm, 'eigen_sweep': t_sweep},
+            'counts': {'qr_iterations': qr_iterations, 'cholesky_iterations': chol_iterations,
+                       'eigen_sweeps': eigen_sweeps},
+            'sample': f'each phase of the oracle timed once at n={small} (same workload family), scaled by '
+                      f'(n/{small})^3, times the counts of the GPU run (its eigen sweeps are the blocked solver\'s)'}
+
+
+def cpu_tsqr(m, n, leaves, sweeps, small=512, seed=SEED):
+    """Extrapolated oracle time of TSQR + Jacobi on R at m x n with `leaves` square n x n leaves:
+    the leaf QR (n x n), the merge QR (2n x n) and one sweep of the one-sided Jacobi on R, each timed at
+    n = `small` and scaled by (n / small)^3, the back-transformation products by their flops; times the
+    GPU run's counts (leaves, leaves - 1 merges, sweeps)."""
+    from threadpoolctl import threadpool_limits
+    from oracle import restatement as R
+    rng = np.random.default_rng(seed)
+    leaf = rng.standard_normal((1, small, small)).astype(np.float32)
+    merge = rng.standard_normal((1, 2 * small, small)).astype(np.float32)
+    with threadpool_limits(1):
+        t_leaf = _scaled_phase(lambda: R.qr(leaf), small, n)
+        t_merge = _scaled_phase(lambda: R.qr(merge), small, n)
+        r = np.triu(rng.standard_normal((1, small, small))).astype(np.float32)
+        t_sweep = _scaled_phase(lambda: R.jacobi_svd(r, max_sweeps=1), small, n)
+        q = rng.standard_normal((small, small)).astype(np.float32)
+        t_gemm = _scaled_phase(lambda: q @ q, small, n)      # one n x n x n product
+    merges = leaves - 1
+    # each tree level multiplies all m rows of Q by n x n blocks (Q <- [Q1 q_top; Q2 q_bot]), then U = Q U_R
+    products = (math.ceil(math.log2(leaves)) + 1) * m / n
+    total = leaves * t_leaf + merges * t_merge + products * t_gemm + sweeps * t_sweep
+    return {'value': total, 'unit': 's/solve', 'cores': 1, 'kind': 'port', 'cpu': cpu_model()[0],
+            'extrapolated': True,
+            'phases_s': {'leaf_qr': t_leaf, 'merge_qr': t_merge, 'jacobi_sweep_on_r': t_sweep, 'n_cubed_product': t_gemm},
+            'counts': {'leaves': leaves, 'merges': merges, 'jacobi_sweeps': sweeps},
+            'sample': f'each phase of the oracle timed once at n={small}, scaled by (n/{small})^3, times the counts '
+                      f'of the GPU run (its sweeps are the blocked solver\'s)'}
 
 
 def plain_gpu_form(pkg, flat, ms, ns, offs, dev, count=512):
@@ -123,6 +260,7 @@ def _orth(u):
 
 
 def _timed(fn, reps):
+    """(median seconds over `reps` timed calls after one warm-up call, the last result)."""
     torch.cuda.synchronize()
     out = fn()                                          # warm-up (also returns the result)
     torch.cuda.synchronize()
@@ -135,7 +273,26 @@ def _timed(fn, reps):
     return float(np.median(ts)), out
 
 
-def config_single(pkg, dev):
+def _recon(a, u, s, vh):
+    """|A - U diag(s) Vh|_F / |A|_F in FP64 on the GPU."""
+    gemm = importlib.import_module('pavel-s_svd_amd.standalone.blas').gemm
+    a64 = a.double()
+    resid = gemm((u.double() * s.double()[None, :]).contiguous(), vh.double(), mode='f64') - a64
+    return float(resid.norm() / a64.norm())
+
+
+def _roofline(flops, seconds, what):
+    achieved = flops / seconds / 1e12
+    return {'bound': 'valu+mfma', 'achieved': achieved, 'peak': PEAK_FP32_TFLOPS, 'unit': 'TFLOP/s',
+            'frac': achieved / PEAK_FP32_TFLOPS, 'algorithmic_flops': flops, 'work': what}
+
+
+def jacobi_flops(m, n, sweeps):
+    """The reference's one-sided Jacobi work: n(n-1)/2 (12 m + 6 n) flops per sweep plus 3 m n."""
+    return sweeps * n * (n - 1) / 2 * (12 * m + 6 * n) + 3 * m * n
+
+
+def config_single(pkg, dev, cpu=True):
     """512 x 256 float32, Haar U and V, sigma geometric 1 .. 1e-6; one-sided Jacobi at tol 1e-7."""
     gen = torch.Generator(device=dev)
     gen.manual_seed(SEED)
@@ -143,18 +300,34 @@ def config_single(pkg, dev):
     a = _product(_haar64(512, 256, gen, dev), s, _haar64(256, 256, gen, dev)).float()
     res = {}
     for cut in ('reference', 'floor'):
-        t, (u, sv, vh, info) = _timed(lambda: pkg.svd(a, 'jacobi', tol=1e-7, cut=cut, return_info=True), 3)
+        t, (u, sv, vh, info) = _timed(lambda: pkg.svd(a, 'jacobi', tol=1e-7, cut=cut, return_info=True), 5)
         res[cut] = {'seconds': t, 'jacobi_sweeps': info['jacobi_sweeps'],
                     'spectrum_error_over_sigma1': float((sv.double() - s).abs().max()),
-                    'orthogonality_u': _orth(u), 'orthogonality_v': _orth(vh.mT)}
-    return {'workload': '512x256 float32, Haar U/V, sigma geomspace(1, 1e-6); jacobi tol 1e-7 (the cut '
-                        "'reference' zeroes sigma < eps*max(m,n)*sigma_1 = 6.1e-5 by the reference's contract)",
-            'unit': 's/solve', 'results': res}
+                    'orthogonality_u': _orth(u), 'orthogonality_v': _orth(vh.mT),
+                    'reconstruction_relative_fro': _recon(a, u, sv, vh)}
+    out = {'workload': '512x256 float32, Haar U/V, sigma geomspace(1, 1e-6); jacobi tol 1e-7 (the cut '
+                       "'reference' zeroes sigma < eps*max(m,n)*sigma_1 = 6.1e-5 by the reference's contract)",
+           'unit': 's/solve', 'results': res,
+           'roofline': _roofline(jacobi_flops(512, 256, res['floor']['jacobi_sweeps']), res['floor']['seconds'],
+                                 "reference one-sided Jacobi work x executed sweeps (cut 'floor'), whole solve")}
+    if cpu:
+        out['cpu_baseline'] = cpu_single(a.cpu().numpy())
+    return out
 
 
-def config_qdwh(pkg, dev, n=4096):
+def qdwh_flops(n, polar_iterations, qr_iterations, eigen_sweeps):
+    """QDWH work as counted for the reference's path: (26/3) n^3 per QR-branch iteration, (16/3) n^3 per
+    Cholesky-branch iteration (the reference also forms X^T X: 2 n^3), 4 n^3 for H and U, 9 n^3 per
+    eigensolver sweep."""
+    chol = polar_iterations - qr_iterations
+    return n ** 3 * (26 / 3 * qr_iterations + 16 / 3 * chol + 4 + 9 * eigen_sweeps)
+
+
+def config_qdwh(pkg, dev, n=4096, cpu=True):
     """n x n float32, Haar U and V, sigma log-uniform on [1e-6, 1] with 10% in a width-1e-4 cluster
-    at 0.5: the MI355X form (BF16x3 QR-branch products, FP64 polish) against the plain FP32 form."""
+    at 0.5: the MI355X form (BF16x3 QR-branch products, FP64 polish), the equal-accuracy FP32 form
+    (FP32 products, the same FP64 polish), the plain FP32 form and the reference's defaults (cut at
+    eps max(m, n) sigma_1 with the completion of the cut columns, FP32 products, no polish)."""
     gen = torch.Generator(device=dev)
     gen.manual_seed(SEED)
     rng = np.random.default_rng(SEED)
@@ -164,21 +337,41 @@ def config_qdwh(pkg, dev, n=4096):
     sig[0] = 1.0
     s = torch.from_numpy(sig).to(dev)
     a = _product(_haar64(n, n, gen, dev), s, _haar64(n, n, gen, dev)).float()
-    forms = {'mi355x': dict(gemm='bf16x3', polish=True, cut='floor'), 'plain_fp32': dict(gemm='f32', cut='floor')}
+    forms = {'mi355x': dict(gemm='bf16x3', polish=True, cut='floor'),
+             'fp32_polish': dict(gemm='f32', polish=True, cut='floor'),
+             'plain_fp32': dict(gemm='f32', cut='floor'),
+             'reference_defaults': dict()}
     res = {}
     for name, opts in forms.items():
-        t, (u, sv, vh, info) = _timed(lambda: pkg.svd(a, 'polar_qdwh', return_info=True, **opts), 1)
+        t, (u, sv, vh, info) = _timed(lambda: pkg.svd(a, 'polar_qdwh', return_info=True, **opts), 3)
+        kept = sv > 0
         res[name] = {'seconds': t, 'options': opts, 'polar_iterations': info['polar_iterations'],
                      'polar_qr_iterations': info.get('polar_qr_iterations'), 'eigen_sweeps': info['eigen_sweeps'],
-                     'spectrum_error_over_sigma1': float((sv.double() - s).abs().max()),
-                     'orthogonality_u': _orth(u), 'orthogonality_v': _orth(vh.mT)}
-    return {'workload': f'QDWH polar SVD, {n}x{n} float32, Haar U/V, sigma log-uniform [1e-6, 1] with 10% in a '
-                        'width-1e-4 cluster at 0.5; sigma error against the generating spectrum',
-            'unit': 's/solve', 'results': res,
-            'speedup_vs_plain_fp32': res['plain_fp32']['seconds'] / res['mi355x']['seconds']}
+                     'polish_sweeps': info.get('polish_sweeps'),
+                     'kept_sigma': int(kept.sum()),
+                     'spectrum_error_over_sigma1': float((sv.double() - s)[kept].abs().max()),
+                     'orthogonality_u': _orth(u), 'orthogonality_v': _orth(vh.mT),
+                     'reconstruction_relative_fro': _recon(a, u, sv, vh)}
+        res[name]['roofline'] = _roofline(qdwh_flops(n, info['polar_iterations'], info.get('polar_qr_iterations', 0),
+                                                     info['eigen_sweeps']), t,
+                                          'QDWH iterations + H, U + eigensolver sweeps as the reference counts them '
+                                          '(FP64 polish work not counted), whole solve')
+        del u, sv, vh
+    out = {'workload': f'QDWH polar SVD, {n}x{n} float32, Haar U/V, sigma log-uniform [1e-6, 1] with 10% in a '
+                       'width-1e-4 cluster at 0.5; sigma error over the kept values against the generating spectrum '
+                       '(the reference defaults zero sigma < eps*max(m,n)*sigma_1 = 4.9e-4 and complete those U '
+                       'columns)',
+           'unit': 's/solve', 'results': res,
+           'speedup_vs_fp32_polish': res['fp32_polish']['seconds'] / res['mi355x']['seconds'],
+           'speedup_vs_plain_fp32': res['plain_fp32']['seconds'] / res['mi355x']['seconds']}
+    if cpu:
+        r = res['reference_defaults']
+        out['cpu_baseline'] = cpu_qdwh(n, r['polar_qr_iterations'], r['polar_iterations'] - r['polar_qr_iterations'],
+                                       r['eigen_sweeps'])
+    return out
 
 
-def config_tsqr(pkg, dev):
+def config_tsqr(pkg, dev, cpu=True):
     """8192 x 2048 float32: rank-64 G1 diag(sigma) G2^T (orthonormal factors, sigma geometric 1 ..
     1e-3) plus N(0, 1e-8) noise; TSQR with 256-row panels, then Jacobi on R."""
     gen = torch.Generator(device=dev)
@@ -186,20 +379,27 @@ def config_tsqr(pkg, dev):
     s = torch.from_numpy(np.geomspace(1.0, 1e-3, 64)).to(dev)
     a = _product(_haar64(8192, 64, gen, dev), s, _haar64(2048, 64, gen, dev))
     a = (a + 1e-4 * torch.randn(8192, 2048, generator=gen, device=dev, dtype=torch.float64)).float()
-    gemm = importlib.import_module('pavel-s_svd_amd.standalone.blas').gemm
     res = {}
-    for name, opts in (('mi355x', dict(gemm='bf16x3', polish=True)), ('plain_fp32', dict(gemm='f32'))):
-        t, (u, sv, vh, info) = _timed(lambda: pkg.svd(a, 'tsqr', tile_rows=256, return_info=True, **opts), 1)
-        a64 = a.double()
-        resid = gemm((u.double() * sv.double()[None, :]).contiguous(), vh.double(), mode='f64') - a64
+    m, n = a.shape
+    for name, opts in (('mi355x', dict(gemm='bf16x3', polish=True)), ('fp32_polish', dict(gemm='f32', polish=True)),
+                       ('plain_fp32', dict(gemm='f32'))):
+        t, (u, sv, vh, info) = _timed(lambda: pkg.svd(a, 'tsqr', tile_rows=256, return_info=True, **opts), 3)
         res[name] = {'seconds': t, 'options': opts, 'tsqr_leaves': info.get('tsqr_leaves'),
                      'jacobi_sweeps': info['jacobi_sweeps'],
-                     'reconstruction_relative_fro': float(resid.norm() / a64.norm()),
+                     'reconstruction_relative_fro': _recon(a, u, sv, vh),
                      'orthogonality_u': _orth(u), 'orthogonality_v': _orth(vh.mT)}
-    return {'workload': 'TSQR, 8192x2048 float32, rank-64 (sigma 1 .. 1e-3) plus N(0, 1e-8) noise (its |E|_2 ~ '
-                        '0.014 sets the spectrum, so accuracy is reconstruction and orthogonality in FP64), 256-row '
-                        'panels (2048-row leaves: rows = max(tile_rows, n)), Jacobi on R', 'unit': 's/solve',
-            'results': res}
+        flops = 2 * (2 * m * n * n - 2 * n ** 3 / 3) + 2 * m * n * n + jacobi_flops(n, n, info['jacobi_sweeps'])
+        res[name]['roofline'] = _roofline(flops, t, 'geqrf + orgqr + Q U_R + reference Jacobi work on R x executed '
+                                                    'sweeps, whole solve')
+        del u, sv, vh
+    out = {'workload': 'TSQR, 8192x2048 float32, rank-64 (sigma 1 .. 1e-3) plus N(0, 1e-8) noise (its |E|_2 ~ '
+                       '0.014 sets the spectrum, so accuracy is reconstruction and orthogonality in FP64), 256-row '
+                       'panels (2048-row leaves: rows = max(tile_rows, n)), Jacobi on R', 'unit': 's/solve',
+           'results': res, 'speedup_vs_fp32_polish': res['fp32_polish']['seconds'] / res['mi355x']['seconds']}
+    if cpu:
+        r = res['plain_fp32']
+        out['cpu_baseline'] = cpu_tsqr(m, n, r['tsqr_leaves'] or 1, r['jacobi_sweeps'])
+    return out
 
 
 def config_consumers(pkg, dev):
@@ -296,7 +496,7 @@ def config_consumers(pkg, dev):
             'unit': 's/solve', 'results': res}
 
 
-def config_l0(pkg, dev, count=32):
+def config_l0(pkg, dev, count=32, cpu=True):
     """The learned QDWH lower bound on `count` matrices of its test split (1024 x 1024)."""
     T = importlib.import_module('pavel-s_svd_amd.train_l0')
     L = importlib.import_module('pavel-s_svd_amd.learned_l0')
@@ -322,6 +522,13 @@ def config_l0(pkg, dev, count=32):
         out[name] = {'seconds': float(arr[:, c].mean()), 'iterations': float(arr[:, c + 1].mean()),
                      'qr_iterations': float(arr[:, c + 2].mean())}
     out['model_evaluation'] = ev
+    if cpu:
+        e = out['reference_stop_eps']
+        qr_it = int(round(e['qr_iterations']))
+        base = cpu_qdwh(n, qr_it, int(round(e["iterations"])) - qr_it, 0, small=512)
+        base['sample'] = ('the polar phase only (as the GPU figures above), from l0 = eps: ' + base['sample']
+                          + '; mean counts of the eps runs rounded')
+        out['cpu_baseline'] = base
     return out
 
 
@@ -362,18 +569,24 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({'copy_bytes_read': COPY_BYTES, 'copy_bytes_written': COPY_BYTES}), flush=True)
         return
+    flat, ms, ns, offs = uneven_workload()
+    cpu = None
+    if not args.no_cpu and args.only in ('all', 'c2'):
+        # host-only work first: its worker processes are forked before this process touches the GPU
+        cpu = cpu_baseline(flat, ms, ns, offs)
+        print(f'cpu baseline done: {cpu["value"]:.1f} SVDs/s on {cpu["cores"]} cores', file=sys.stderr, flush=True)
     pkg = importlib.import_module('pavel-s_svd_amd')
     legs = {'single': ('single_512x256', config_single), 'qdwh': ('qdwh_4096', config_qdwh),
             'tsqr': ('tsqr_8192x2048', config_tsqr), 'l0': ('learned_l0_1024', config_l0),
             'consumers': ('consumers', config_consumers)}
     if args.only in legs:
         name, fn = legs[args.only]
-        print(json.dumps({'configs': {name: fn(pkg, torch.device('cuda:0'))}}), flush=True)
+        kw = {} if name == 'consumers' else {'cpu': not args.no_cpu}
+        print(json.dumps({'configs': {name: fn(pkg, torch.device('cuda:0'), **kw)}}), flush=True)
         return
     B = importlib.import_module('pavel-s_svd_amd.standalone.batched')
     T = importlib.import_module('pavel-s_svd_amd.autotune')
     dev = torch.device('cuda:0')
-    flat, ms, ns, offs = uneven_workload()
     batch = B.UnevenBatch(torch.from_numpy(flat).to(dev), ms, ns)
     params = {} if args.no_tune else T.tune_uneven(batch)
     batch.configure(cut='floor', **params)
@@ -406,7 +619,7 @@ def main():
                    'count': COUNT, 'seed': SEED, 'params': batch.params,
                    'classes': [{k: c[k] for k in ('threads', 'lanes', 'count', 'max_m', 'max_n', 'stream')} for c in batch.classes],
                    'sweeps_max': int(sweeps.max()), 'sweeps_mean': float(sweeps.mean())},
-        'roofline': {'bound': 'mfma', 'achieved': achieved, 'peak': PEAK_FP32_TFLOPS, 'unit': 'TFLOP/s',
+        'roofline': {'bound': 'valu', 'achieved': achieved, 'peak': PEAK_FP32_TFLOPS, 'unit': 'TFLOP/s',
                      'frac': achieved / PEAK_FP32_TFLOPS, 'traffic': load_traffic(),
                      'kernel': 'jacobi_lds_kernel<float,*>', 'kernel_ms': kernel_ms,
                      'algorithmic_flops_per_launch': flops},
@@ -417,15 +630,15 @@ def main():
         plain = plain_gpu_form(pkg, flat, ms, ns, offs, dev)
         plain['batched_speedup'] = out['value'] / plain['value']
         out['plain_gpu_form'] = plain
-    if not args.no_cpu:
-        out['cpu_baseline'] = cpu_baseline(flat, ms, ns, offs)
+    if cpu is not None:
+        out['cpu_baseline'] = cpu
     if args.only != 'c2':
         del batch
         torch.cuda.empty_cache()
         out['configs'] = {}
         for name, fn in legs.values():
             t0 = time.perf_counter()
-            out['configs'][name] = fn(pkg, dev)
+            out['configs'][name] = fn(pkg, dev) if name == 'consumers' else fn(pkg, dev, cpu=not args.no_cpu)
             print(f'{name} done in {time.perf_counter() - t0:.1f} s', file=sys.stderr, flush=True)
     print(json.dumps(out), flush=True)
 
