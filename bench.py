@@ -618,7 +618,8 @@ def main():
                                '(FP64 dot products and convergence test, tol 10*eps32), persistent LDS launch in a hipGraph',
                    'count': COUNT, 'seed': SEED, 'params': batch.params,
                    'classes': [{k: c[k] for k in ('threads', 'lanes', 'count', 'max_m', 'max_n', 'stream')} for c in batch.classes],
-                   'sweeps_max': int(sweeps.max()), 'sweeps_mean': float(sweeps.mean())},
+                   'sweeps_max': int(sweeps.max()), 'sweeps_mean': float(sweeps.mean()),
+                   'schedule_tuning': (T.report(batch) or {}).get('schedules')},
         'roofline': {'bound': 'valu', 'achieved': achieved, 'peak': PEAK_FP32_TFLOPS, 'unit': 'TFLOP/s',
                      'frac': achieved / PEAK_FP32_TFLOPS, 'traffic': load_traffic(),
                      'kernel': 'jacobi_lds_kernel<float,*>', 'kernel_ms': kernel_ms,

@@ -47,6 +47,11 @@ typedef void *psvd_stream_t;
 #define PSVD_CUT_FLOOR 2    /* zero sigma only below eps*sigma_1 (the Jacobi rotation floor)
                                instead of eps*max(m,n)*sigma_1 (primitives.py:362) */
 #define PSVD_NO_VECTORS 4   /* sigma only: no V accumulation, U and Vh are not written */
+#define PSVD_SLOT_SPREAD 8  /* psvd_jacobi_lds: a round's pairs served in bank-spread slot order (a
+                               launch-shape choice; the results are bit-identical) */
+#define PSVD_ORDER_SORTED 16 /* psvd_jacobi_lds: pair ordering -- at the start of every sweep the
+                               tournament's labels are the columns in descending norm order (de Rijk)
+                               instead of the reference's identity labels (primitives.py:309-313) */
 
 /* GEMM arithmetic modes */
 #define PSVD_GEMM_F32 0     /* FP32 operands, FP32 MFMA, FP32 accumulation */

@@ -9,7 +9,11 @@ current stream: every policy candidate over a few replays; the fastest few again
 rounds, ranked by their median replay time (single short timings differ by a few percent from
 run to run, as much as the candidates differ); then each class, largest first, at twice and half
 its workgroup size, a change kept when it beats the current setting, timed in alternation with
-it, by more than 3 %.
+it, by more than 3 %; last, the pair schedule -- the pair ordering (the reference's tournament, or
+the tournament over columns relabelled in descending norm order every sweep) and the order in which a
+round's pairs are served (identity, or spread over the LDS banks) -- four combinations timed in
+alternation with their sweep counts recorded, one kept when it beats the reference's schedule by
+more than 2 %.
 Results are cached in-process per (dtype, max_m, max_n, batch size bucket, vectors) and, when
 ``PAVELSVD_TUNE_CACHE`` names a JSON file, across processes.
 """
@@ -26,7 +30,7 @@ _CACHE = {}
 
 def _key(batch):
     bucket = 1 << max(batch.batch - 1, 0).bit_length()
-    return f'{batch.code}:{batch.max_m}:{batch.max_n}:{bucket}:{int(batch.vectors)}'
+    return f'{batch.code}:{batch.max_m}:{batch.max_n}:{bucket}:{int(batch.vectors)}:v2'
 
 
 def _load():
@@ -104,7 +108,38 @@ def _tune_classes(batch, policy, rounds, verbose, key, gain=0.97):
     return cur, base
 
 
-def tune_uneven(batch, candidates=None, reps=3, verbose=None, finalists=4, rounds=2, class_stage=True):
+SCHEDULES = [dict(ordering=o, slots=sl) for o in ('tournament', 'sorted') for sl in ('identity', 'spread')]
+
+
+def _tune_schedule(batch, params, rounds, verbose, key, gain=0.98):
+    """Third stage: with the launch shape fixed, the pair ordering (the reference's tournament or
+    the norm-sorted one) and the slot order of a round's pairs, each timed in alternation with the
+    current best over `rounds` rounds, with the batch's sweep counts recorded.  A schedule replaces
+    the reference's (tournament, identity) only when its median beats it by more than 1 - gain."""
+    rows, medians = [], {}
+    samples = {i: [] for i in range(len(SCHEDULES))}
+    for _ in range(rounds):
+        for i, sched in enumerate(SCHEDULES):
+            batch.configure(**params, **sched)
+            samples[i] += time_replays(batch)
+    for i, sched in enumerate(SCHEDULES):
+        batch.configure(**params, **sched)
+        batch.solve()
+        sweeps = batch.info[:, 0].float()
+        medians[i] = sorted(samples[i])[len(samples[i]) // 2]
+        rows.append(dict(sched, median_ms=medians[i], sweeps_mean=float(sweeps.mean()), sweeps_max=int(sweeps.max())))
+        if verbose:
+            print(f'autotune {key} schedule {sched}: {medians[i]:.3f} ms, sweeps {rows[-1]["sweeps_mean"]:.2f}',
+                  file=sys.stderr, flush=True)
+    best = 0
+    for i in range(1, len(SCHEDULES)):
+        if medians[i] < gain * medians[best]:
+            best = i
+    return dict(SCHEDULES[best]), medians[best], rows
+
+
+def tune_uneven(batch, candidates=None, reps=3, verbose=None, finalists=4, rounds=2, class_stage=True,
+                schedule_stage=True):
     """Return the fastest launch parameters for ``batch`` (an UnevenBatch) and leave the batch
     configured with them."""
     if verbose is None:
@@ -140,8 +175,17 @@ def tune_uneven(batch, candidates=None, reps=3, verbose=None, finalists=4, round
             print(f'autotune {key} replay {cand}: median {med:.3f} ms', file=sys.stderr, flush=True)
     if class_stage:
         best['class_threads'], ms = _tune_classes(batch, best, rounds, verbose, key)
+    schedules = None
+    if schedule_stage:
+        sched, ms, schedules = _tune_schedule(batch, best, rounds, verbose, key)
+        best.update(sched)
     batch.configure(**{k: v for k, v in saved.items() if k not in best}, **best)
     _CACHE[key] = {'params': best, 'ms': ms, 'timings': [[t, c] for t, c in timings],
-                   'replay_medians': [[med, c] for med, c in zip(medians, top)]}
+                   'replay_medians': [[med, c] for med, c in zip(medians, top)], 'schedules': schedules}
     _store()
     return dict(best)
+
+
+def report(batch):
+    """The cached tuning record of ``batch``'s key (timings of every stage), or None."""
+    return _CACHE.get(_key(batch))

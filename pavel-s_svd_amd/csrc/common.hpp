@@ -297,6 +297,39 @@ __device__ __forceinline__ double block_update(T *G, int ld, int p1, int q1, int
   return fmax(fmax(fabs((double)n11), fabs((double)n12)), fmax(fabs((double)n21), fabs((double)n22)));
 }
 
+// Rows (p_k, q_k) of the accumulated rotation Qt (ld `ld`) rotated by (c_k, s_k) for the pairs
+// k = qk + j QSTEP (j < QIT, k < NP) of a round, at column qcol: one round trip to LDS for all of the
+// thread's pairs -- every (c, s), (p, q) and Qt load is issued first, then the FP64 rotations, then
+// the stores.  The pairs of a round are disjoint, so no row is both read and written by two items.
+template <typename T, int NP, int QSTEP, int QIT>
+__device__ __forceinline__ void rotate_rows_batched(T *Qt, int ld, const double2 *cs, const int2 *pq, int qk,
+                                                    int qcol) {
+  double2 r[QIT];
+  int2 a[QIT];
+  bool on[QIT];
+#pragma unroll
+  for (int j = 0; j < QIT; ++j) {
+    const int k = qk + j * QSTEP;
+    on[j] = k < NP;
+    r[j] = on[j] ? cs[k] : make_double2(1.0, 0.0);
+    a[j] = on[j] ? pq[k] : make_int2(0, 0);
+    on[j] = on[j] && r[j].y != 0.0;
+  }
+  double x[QIT], y[QIT];
+#pragma unroll
+  for (int j = 0; j < QIT; ++j)
+    if (on[j]) {
+      x[j] = (double)Qt[a[j].x * ld + qcol];
+      y[j] = (double)Qt[a[j].y * ld + qcol];
+    }
+#pragma unroll
+  for (int j = 0; j < QIT; ++j)
+    if (on[j]) {
+      Qt[a[j].x * ld + qcol] = (T)(r[j].x * x[j] - r[j].y * y[j]);
+      Qt[a[j].y * ld + qcol] = (T)(r[j].y * x[j] + r[j].x * y[j]);
+    }
+}
+
 // MFMA GEMM with the psvd_gemm contract plus an optional split-K workspace (gemm.hip).
 int gemm_ws(int mode, int transa, int transb, int64_t m, int64_t n, int64_t k, double alpha, const void *a, int64_t lda,
             int64_t stride_a, const void *b, int64_t ldb, int64_t stride_b, double beta, void *c, int64_t ldc,

@@ -572,18 +572,7 @@ __device__ __forceinline__ void blk_inner_body(const BlkParams<T> &P, int p, int
         const int2 a1 = pq[it1[j]], a2 = pq[it2[j]];
         block_update(S, kEsLd, a1.x, a1.y, a2.x, a2.y, r1.x, r1.y, r2.x, r2.y, r1.y != 0.0, r2.y != 0.0);
       }
-#pragma unroll
-      for (int j = 0; j < QIT; ++j) {
-        const int k = qk + j * QSTEP;
-        if (k >= kEb) continue;
-        const double2 r = cs[k];
-        if (r.y == 0.0) continue;
-        const int2 a = pq[k];
-        T *vp = Qt + a.x * kEsLd + qcol, *vq = Qt + a.y * kEsLd + qcol;
-        const double x = (double)*vp, y = (double)*vq;
-        *vp = (T)(r.x * x - r.y * y);
-        *vq = (T)(r.y * x + r.x * y);
-      }
+      rotate_rows_batched<T, kEb, QSTEP, QIT>(Qt, kEsLd, cs, pq, qk, qcol);
       __syncthreads();
     }
     act_sw = __syncthreads_or(act_sw);

@@ -251,18 +251,7 @@ __global__ __launch_bounds__(kJbInner) void bj_inner_kernel(BjParams<T> P, int r
         const int2 a1 = pq[it1[j]], a2 = pq[it2[j]];
         block_update(S, kJsLd, a1.x, a1.y, a2.x, a2.y, r1.x, r1.y, r2.x, r2.y, r1.y != 0.0, r2.y != 0.0);
       }
-#pragma unroll
-      for (int j = 0; j < QIT; ++j) {
-        const int k = qk + j * QSTEP;
-        if (k >= kJb) continue;
-        const double2 r = cs[k];
-        if (r.y == 0.0) continue;
-        const int2 a = pq[k];
-        double *vp = Qt + a.x * kJsLd + qcol, *vq = Qt + a.y * kJsLd + qcol;
-        const double x = *vp, y = *vq;
-        *vp = r.x * x - r.y * y;
-        *vq = r.y * x + r.x * y;
-      }
+      rotate_rows_batched<double, kJb, QSTEP, QIT>(Qt, kJsLd, cs, pq, qk, qcol);
       __syncthreads();
     }
     act_sw = __syncthreads_or(act_sw);

@@ -121,11 +121,94 @@ __device__ __forceinline__ void pair_rotate(T *xp, T *xq, int rows, int lig, T c
   }
 }
 
+// Bank-spread order of a round's pairs.  A half-wave's LDS access is served by 32 banks at once;
+// with L lanes per pair, its G = 32 / L groups read L consecutive words of their columns each.  The
+// tournament gives consecutive pair indices columns p, p +- 1, ..., so with a column stride of 1 or 17
+// (mod 32) consecutive groups would start one bank apart and their ranges overlap.  Each chunk of c <= 32
+// pairs is therefore laid out column-major over h = ceil(c / G) rows: pair k = b h + a goes to slot
+// a q + min(a, r) + b (q = c / h, r = c mod h), so the groups of one half-wave take pairs h apart, whose
+// columns start about h banks apart (L apart in a full chunk).  Pairs of a round are disjoint, so the
+// order in which the groups serve them is free: the results are bit-identical to the identity order.
+template <int L>
+__device__ __forceinline__ int pair_slot(int k, int npairs) {
+  if constexpr (L > 16) {
+    return k;
+  } else {
+    constexpr int G = 32 / L;
+    const int c0 = k & ~31, kk = k - c0, c = npairs - c0 < 32 ? npairs - c0 : 32;
+    const int h = (c + G - 1) / G, q = c / h, r = c - q * h;
+    const int a = kk % h, b = kk / h;
+    return c0 + a * q + (a < r ? a : r) + b;
+  }
+}
+
+// inverse of pair_slot: the pair served by slot t
+template <int L>
+__device__ __forceinline__ int slot_pair(int t, int npairs) {
+  if constexpr (L > 16) {
+    return t;
+  } else {
+    constexpr int G = 32 / L;
+    const int c0 = t & ~31, tt = t - c0, c = npairs - c0 < 32 ? npairs - c0 : 32;
+    const int h = (c + G - 1) / G, q = c / h, r = c - q * h;
+    int a, b;
+    if (tt < r * (q + 1)) {
+      a = tt / (q + 1);
+      b = tt - a * (q + 1);
+    } else {
+      const int u = tt - r * (q + 1);
+      a = r + u / q;
+      b = u - (a - r) * q;
+    }
+    return c0 + b * h + a;
+  }
+}
+
+// The pairs of round r: the reference's tournament (primitives.py:309-313) over column labels that
+// are the columns themselves (the reference's ordering) or, with PSVD_ORDER_SORTED, the columns in
+// descending order of their norms at the start of the sweep (perm[label] = column; de Rijk's
+// ordering).  A pair is returned as (smaller column, larger column), as the tournament orders it.
+__device__ __forceinline__ void round_pair(int r, int k, int np, const int *perm, int &p, int &q) {
+  rr_pair(r, k, np, p, q);
+  if (perm) {
+    const int a = perm[p], b = perm[q];
+    p = a < b ? a : b;
+    q = a < b ? b : a;
+  }
+}
+
+// perm[label] = column, labels in descending order of the columns' FP64 norms (ties by column), from
+// the current W; cn: np doubles of scratch.  Called by the whole workgroup between rounds.
+template <typename T, int NT>
+__device__ void sort_labels(const T *W, int m, int np, int ldw, double *cn, int *perm) {
+  constexpr int NW = NT / kWave;
+  const int tid = threadIdx.x, wave = tid / kWave, lane = tid & (kWave - 1);
+  for (int c = wave; c < np; c += NW) {
+    const T *wc = W + c * ldw;
+    double acc = 0.0;
+    for (int i = lane; i < m; i += kWave) {
+      const double x = (double)wc[i];
+      acc = fma(x, x, acc);
+    }
+    acc = wave_sum(acc);
+    if (lane == 0) cn[c] = acc;
+  }
+  __syncthreads();
+  for (int j = tid; j < np; j += NT) {
+    const double v = cn[j];
+    int rank = 0;
+    for (int i = 0; i < np; ++i) rank += cn[i] > v || (cn[i] == v && i < j);
+    perm[rank] = j;
+  }
+  __syncthreads();
+}
+
 template <typename T> struct CachedRows { static constexpr int value = sizeof(T) == 4 ? 8 : 4; };
 
 template <typename T, int NT, int L>
 __device__ void jacobi_sweeps(T *W, T *V, int m, int np, int ldw, int ldv, bool vectors, double tol, double eps,
-                              int max_sweeps, double *rslot, int *fslot, int &sweeps_out, bool &conv_out) {
+                              int max_sweeps, double *rslot, int *fslot, int flags, double *cn, int *perm,
+                              int &sweeps_out, bool &conv_out) {
   constexpr int NG = NT / L, NW = NT / kWave;
   constexpr int RC = CachedRows<T>::value;       // rows per lane kept in registers between phases
   const int tid = threadIdx.x, wave = tid / kWave, lane = tid & (kWave - 1);
@@ -140,15 +223,17 @@ __device__ void jacobi_sweeps(T *W, T *V, int m, int np, int ldw, int ldv, bool 
   int sweeps = 0;
   bool converged = false;
   int g = 0;
+  const bool spread = flags & PSVD_SLOT_SPREAD, sorted = flags & PSVD_ORDER_SORTED;
   for (int sw = 1; sw <= max_sweeps; ++sw) {
     sweeps = sw;
     bool wave_active = false;
+    if (sorted) sort_labels<T, NT>(W, m, np, ldw, cn, perm);
     for (int r = 0; r < np - 1; ++r, ++g) {
       const double floor_ = eps2 * slots_max<NW>(rslot + (g & 1) * NW);
       double mymax = 0.0;
       for (int k = grp; k < npairs; k += NG) {
         int p, q;
-        rr_pair(r, k, np, p, q);
+        round_pair(r, spread ? slot_pair<L>(k, npairs) : k, np, sorted ? perm : nullptr, p, q);
         T *wp = W + p * ldw, *wq = W + q * ldw;
         double app = 0.0, aqq = 0.0, apq = 0.0;
         T xs[RC], ys[RC];
@@ -244,9 +329,9 @@ __device__ void jacobi_sweeps(T *W, T *V, int m, int np, int ldw, int ldv, bool 
 // no selects), and the V rotation strides over round_up(np, L) rows.  The sums are formed in the
 // same order as in jacobi_sweeps, so both give the same bits.
 template <typename T, int NT, int L, int S>
-__device__ void jacobi_sweeps_fixed(T *W, T *V, int np, int ldw, int ldv, bool vectors, double tol, double eps,
-                                    int max_sweeps, double *rslot, int *fslot, int *ptab, int &sweeps_out,
-                                    bool &conv_out) {
+__device__ void jacobi_sweeps_fixed(T *W, T *V, int m, int np, int ldw, int ldv, bool vectors, double tol,
+                                    double eps, int max_sweeps, double *rslot, int *fslot, int *ptab, int flags,
+                                    double *cn, int *perm, int &sweeps_out, bool &conv_out) {
   static_assert(L <= 16, "the zero padding covers groups of at most 16 lanes");
   constexpr int NG = NT / L, NW = NT / kWave;
   const int tid = threadIdx.x, wave = tid / kWave, lane = tid & (kWave - 1);
@@ -257,18 +342,31 @@ __device__ void jacobi_sweeps_fixed(T *W, T *V, int np, int ldw, int ldv, bool v
   // the round's pairs come from a double-buffered table, ptab[slot * npairs + k] = p | q << 16:
   // threads k < npairs write the next round's entries during a round and its closing barrier
   // publishes them, so a pass reads one word instead of evaluating the tournament closed form
-  for (int k = tid; k < npairs; k += NT) {
-    int p, q;
-    rr_pair(0, k, np, p, q);
-    ptab[k] = p | (q << 16);
+  const bool spread = flags & PSVD_SLOT_SPREAD, sorted = flags & PSVD_ORDER_SORTED;
+  const int slot0 = spread && tid < npairs ? pair_slot<L>(tid, npairs) : tid;   // table slot of k = tid
+  if (!sorted) {
+    for (int k = tid; k < npairs; k += NT) {
+      int p, q;
+      rr_pair(0, k, np, p, q);
+      ptab[k == tid ? slot0 : (spread ? pair_slot<L>(k, npairs) : k)] = p | (q << 16);
+    }
+    __syncthreads();
   }
-  __syncthreads();
   int sweeps = 0;
   bool converged = false;
   int g = 0;
   for (int sw = 1; sw <= max_sweeps; ++sw) {
     sweeps = sw;
     bool wave_active = false;
+    if (sorted) {   // the sweep's labels, then its first round's table
+      sort_labels<T, NT>(W, m, np, ldw, cn, perm);
+      for (int k = tid; k < npairs; k += NT) {
+        int p, q;
+        round_pair(0, k, np, perm, p, q);
+        ptab[(g & 1) * npairs + (k == tid ? slot0 : (spread ? pair_slot<L>(k, npairs) : k))] = p | (q << 16);
+      }
+      __syncthreads();
+    }
     for (int r = 0; r < np - 1; ++r, ++g) {
       const double floor_ = eps2 * slots_max<NW>(rslot + (g & 1) * NW);
       const int *tab = ptab + (g & 1) * npairs;
@@ -333,12 +431,14 @@ __device__ void jacobi_sweeps_fixed(T *W, T *V, int np, int ldw, int ldv, bool v
         }
         mymax = fmax(mymax, fmax(nap, naq));
       }
-      {  // the next round's pairs (round 0 of the next sweep after the last)
+      if (r + 1 < np - 1 || !sorted) {  // the next round's pairs (round 0 of the next sweep after the
+                                         // last, unless the next sweep relabels the columns first)
         const int rn = r + 1 < np - 1 ? r + 1 : 0;
         for (int k = tid; k < npairs; k += NT) {
           int p, q;
-          rr_pair(rn, k, np, p, q);
-          ptab[((g + 1) & 1) * npairs + k] = p | (q << 16);
+          round_pair(rn, k, np, sorted ? perm : nullptr, p, q);
+          ptab[((g + 1) & 1) * npairs + (k == tid ? slot0 : (spread ? pair_slot<L>(k, npairs) : k))] =
+              p | (q << 16);
         }
       }
       mymax = wave_max_fast(mymax);
@@ -365,21 +465,23 @@ __device__ void jacobi_sweeps_fixed(T *W, T *V, int np, int ldw, int ldv, bool v
 // general ones.  m is uniform over the workgroup, so the switch is a scalar branch.
 template <typename T, int NT, int L>
 __device__ void run_sweeps(T *W, T *V, int m, int np, int ldw, int ldv, bool vectors, double tol, double eps,
-                           int max_sweeps, double *rslot, int *fslot, int *ptab, int &sweeps_out, bool &conv_out) {
+                           int max_sweeps, double *rslot, int *fslot, int *ptab, int flags, double *cn, int *perm,
+                           int &sweeps_out, bool &conv_out) {
   if constexpr (L <= 16) {
     constexpr int RC = sizeof(T) == 4 ? 12 : 4;   // row slots a lane can keep in registers
     const int rc = (m + L - 1) / L;
 #define PSVD_FIXED_CASE(SV)                                                                                  \
   if constexpr (SV <= RC)                                                                                    \
     if (rc == SV)                                                                                            \
-      return jacobi_sweeps_fixed<T, NT, L, SV>(W, V, np, ldw, ldv, vectors, tol, eps, max_sweeps, rslot, fslot, \
-                                               ptab, sweeps_out, conv_out);
+      return jacobi_sweeps_fixed<T, NT, L, SV>(W, V, m, np, ldw, ldv, vectors, tol, eps, max_sweeps, rslot,    \
+                                               fslot, ptab, flags, cn, perm, sweeps_out, conv_out);
     PSVD_FIXED_CASE(1) PSVD_FIXED_CASE(2) PSVD_FIXED_CASE(3) PSVD_FIXED_CASE(4)
     PSVD_FIXED_CASE(5) PSVD_FIXED_CASE(6) PSVD_FIXED_CASE(7) PSVD_FIXED_CASE(8)
     PSVD_FIXED_CASE(9) PSVD_FIXED_CASE(10) PSVD_FIXED_CASE(11) PSVD_FIXED_CASE(12)
 #undef PSVD_FIXED_CASE
   }
-  jacobi_sweeps<T, NT, L>(W, V, m, np, ldw, ldv, vectors, tol, eps, max_sweeps, rslot, fslot, sweeps_out, conv_out);
+  jacobi_sweeps<T, NT, L>(W, V, m, np, ldw, ldv, vectors, tol, eps, max_sweeps, rslot, fslot, flags, cn, perm,
+                          sweeps_out, conv_out);
 }
 
 // Waves per SIMD the register budget is sized for (128 VGPRs: four waves per SIMD).
@@ -505,7 +607,9 @@ void jacobi_lds_kernel(LdsParams<T> P) {
     // ---- sweeps
     int sweeps = 0;
     bool converged = false;
-    run_sweeps<T, NT, L>(W, V, m, np, ldw, ldv, vectors, tol, eps, P.max_sweeps, s_rslot, s_fslot, ptab, sweeps,
+    // sig and ord are free until the sweeps end: the sorted ordering's norms and labels live there
+    run_sweeps<T, NT, L>(W, V, m, np, ldw, ldv, vectors, tol, eps, P.max_sweeps, s_rslot, s_fslot, ptab, P.flags,
+                         sig, ord, sweeps,
                          converged);
     int status = converged ? PSVD_INFO_OK : PSVD_INFO_NOCONV;
 

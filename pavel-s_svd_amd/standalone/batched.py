@@ -94,8 +94,11 @@ class UnevenBatch:
         self.graph = None
         # class_threads: per size class ("threads,lanes" of the policy) a workgroup size replacing the
         # policy's, as the autotuner's second stage picks it
+        # ordering: 'tournament' (the reference's round-robin labels) or 'sorted' (labels in descending
+        # column-norm order each sweep); slots: 'identity' or 'spread' (the order in which a round's
+        # pairs are served -- a launch-shape choice, bit-identical results)
         self.params = dict(tol=None, max_sweeps=60, scale=True, cut='reference', streams=4, class_threads={},
-                           **DEFAULT_POLICY)
+                           ordering='tournament', slots='identity', **DEFAULT_POLICY)
         self.side = []
         self._plan()
 
@@ -144,6 +147,10 @@ class UnevenBatch:
         unknown = set(params) - set(self.params)
         if unknown:
             raise ValueError(f'unknown parameters {sorted(unknown)}')
+        if params.get('ordering', 'tournament') not in ('tournament', 'sorted'):
+            raise ValueError("ordering must be 'tournament' or 'sorted'")
+        if params.get('slots', 'identity') not in ('identity', 'spread'):
+            raise ValueError("slots must be 'identity' or 'spread'")
         replan = any((k in DEFAULT_POLICY or k in ('streams', 'class_threads')) and v != self.params[k]
                      for k, v in params.items())
         self.params.update(params)
@@ -156,7 +163,8 @@ class UnevenBatch:
         """Launch the solve on the current stream; no host synchronisation."""
         p = self.params
         flags = (N.SCALE if p['scale'] else 0) | (N.CUT_FLOOR if p['cut'] == 'floor' else 0) | \
-                (0 if self.vectors else N.NO_VECTORS)
+                (0 if self.vectors else N.NO_VECTORS) | (N.ORDER_SORTED if p['ordering'] == 'sorted' else 0) | \
+                (N.SLOT_SPREAD if p['slots'] == 'spread' else 0)
         main = torch.cuda.current_stream(self.a.device)
         if self.side:
             fork = torch.cuda.Event()

@@ -256,3 +256,61 @@ def test_single_512x256_geometric_tol_1e7():
     u2, s2 = pkg.svd(torch.from_numpy(a).to(DEV), 'jacobi')[:2]
     _, rs, _ = R.svd(a, 'jacobi', max_sweeps=60)
     assert abs(int((_np(s2) == 0).sum()) - int((rs == 0).sum())) <= 2
+
+
+@pytest.mark.parametrize('dtype', [np.float32, np.float64])
+def test_pair_schedules(pkg, dtype):
+    """The tuner's schedule axis on an uneven batch (every size class, fixed-slot and general loops):
+    slots='spread' serves the same pairs per round in another order, so U, s, Vh and the sweep counts
+    are bit-identical to the identity order; ordering='sorted' (the tournament over columns relabelled
+    by descending norm each sweep) is a different Jacobi ordering whose results meet the same bars
+    against LAPACK and the oracle's sigma (the reference's FP32 path)."""
+    from importlib import import_module
+    B = import_module('pavel-s_svd_amd.standalone.batched')
+    flat, ms, ns, offs = X.workload_uneven_batch(count=240, seed=41)
+    flat = flat.astype(dtype)
+    batch = B.UnevenBatch(torch.from_numpy(flat).to(DEV), ms, ns)
+    outs = {}
+    for ordering in ('tournament', 'sorted'):
+        for slots in ('identity', 'spread'):
+            batch.configure(ordering=ordering, slots=slots)
+            batch.solve()
+            st = batch.check()
+            outs[ordering, slots] = (batch.u.clone(), batch.s.clone(), batch.vh.clone(), st['sweeps_per_matrix'])
+    for ordering in ('tournament', 'sorted'):
+        a, b = outs[ordering, 'identity'], outs[ordering, 'spread']
+        assert all(torch.equal(x, y) for x, y in zip(a[:3], b[:3])) and a[3] == b[3]
+    mats = X.unpack(flat, ms, ns, offs)
+    bar = 1e-5 if dtype == np.float32 else 1e-12
+    for key, (u, s, vh, sweeps) in outs.items():
+        worst = {}
+        for i, a in enumerate(mats):
+            so, ao = batch.s_off[i], batch.a_off[i]
+            m, n = a.shape
+            met = _metrics(a, (u[ao:ao + m * n].view(m, n), s[so:so + n],
+                               vh[batch.v_off[i]:batch.v_off[i] + n * n].view(n, n)))
+            for k in ('spectrum_error_over_sigma1', 'orthogonality_u_fro_over_sqrt_cols',
+                      'orthogonality_v_fro_over_sqrt_rows', 'reconstruction_relative_fro'):
+                worst[k] = max(worst.get(k, 0.0), met[k])
+            if i < 12 and dtype == np.float32:
+                _, rs, _ = R.svd(a, 'jacobi')
+                assert np.max(np.abs(_np(s[so:so + n]) - rs)) / rs[0] < 2e-5
+        assert all(v <= bar for v in worst.values()), (key, worst)
+        assert 1 <= max(sweeps) <= 20
+
+
+@pytest.mark.parametrize('dtype', [np.float32, np.float64])
+def test_slot_spread_is_bit_identical(pkg, dtype):
+    """Same launch shape, identity vs spread slot order: bit-identical outputs and sweep counts."""
+    from importlib import import_module
+    B = import_module('pavel-s_svd_amd.standalone.batched')
+    flat, ms, ns, _ = X.workload_uneven_batch(count=300, seed=43)
+    batch = B.UnevenBatch(torch.from_numpy(flat.astype(dtype)).to(DEV), ms, ns)
+    res = []
+    for slots in ('identity', 'spread'):
+        batch.configure(slots=slots)
+        batch.solve()
+        st = batch.check()
+        res.append((batch.u.clone(), batch.s.clone(), batch.vh.clone(), st['sweeps_per_matrix']))
+    assert torch.equal(res[0][0], res[1][0]) and torch.equal(res[0][1], res[1][1])
+    assert torch.equal(res[0][2], res[1][2]) and res[0][3] == res[1][3]
