@@ -206,6 +206,28 @@ int psvd_ccomplete_columns(int dtype, int64_t batch, int m, int n, void *u, cons
                            psvd_stream_t stream);
 
 /* ---------------------------------------------------------------------------------------
+ * Complex Hermitian kernels for matrices held in LDS (one workgroup per matrix; dtype PSVD_C64 or
+ * PSVD_C128; interleaved (re, im) pairs, contiguous (batch, rows, cols) row-major; FP64 sums):
+ *  psvd_ceigh_lds: two-sided cyclic Jacobi of Hermitian h -- the complex branch of jacobi_eigh
+ *      (primitives.py:370-427): c real, s = t c g_pq / |g_pq|, G <- G J then J^H G, V <- V J, the
+ *      test |g_pq| > tol max|G| per round, stop after a sweep without rotations (tol <= 0 selects
+ *      10 eps); w (batch, n) real ascending, v (batch, n, n) eigenvectors as columns, info = [sweeps,
+ *      status].  psvd_ceigh_lds_smem_bytes gives the LDS the size needs (PSVD_ETOOBIG beyond).
+ *  psvd_ccholesky_lds: L (lower, strict upper zero) with G = L L^H from the lower triangle of the
+ *      Hermitian g, the reference's failure mask (a pivot <= 0 is clamped to tiny and flagged, fail[b] = 1;
+ *      primitives.py:145-159).
+ *  psvd_csolve_chol_lds: x <- x (L L^H)^-1 for x (batch, m, n), the combined solve
+ *      trsm_upper(L^H, trsm_lower(L, x^H))^H of the complex QDWH Cholesky branch (solvers.py:159-160,
+ *      primitives.py:162-179).  psvd_csolve_lds_smem_bytes gives its LDS.
+ * ------------------------------------------------------------------------------------- */
+int psvd_ceigh_lds_smem_bytes(int dtype, int n, size_t *bytes);
+int psvd_ceigh_lds(int dtype, int64_t batch, int n, const void *h, void *w, void *v, int32_t *info, double tol,
+                   int max_sweeps, psvd_stream_t stream);
+int psvd_ccholesky_lds(int dtype, int64_t batch, int n, const void *g, void *l, int32_t *fail, psvd_stream_t stream);
+int psvd_csolve_lds_smem_bytes(int dtype, int m, int n, size_t *bytes);
+int psvd_csolve_chol_lds(int dtype, int64_t batch, int m, int n, const void *l, void *x, psvd_stream_t stream);
+
+/* ---------------------------------------------------------------------------------------
  * Two-sided cyclic Jacobi eigensolver of a batch of symmetric n x n matrices (primitives.py:
  * 370-427): w (batch, n) ascending, v (batch, n, n) row-major eigenvectors as columns,
  * info = [sweeps, status].  Uses an LDS-resident kernel (one workgroup per matrix) when the

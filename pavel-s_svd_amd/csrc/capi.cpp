@@ -404,6 +404,61 @@ static PyObject *py_complete_columns(PyObject *, PyObject *args) {
   return PyLong_FromLong(st);
 }
 
+static PyObject *py_ceigh_lds_smem_bytes(PyObject *, PyObject *args) {
+  int dtype, n;
+  if (!PyArg_ParseTuple(args, "ii", &dtype, &n)) return nullptr;
+  size_t bytes = 0;
+  int st = psvd_ceigh_lds_smem_bytes(dtype, n, &bytes);
+  return Py_BuildValue("(in)", st, (Py_ssize_t)bytes);
+}
+
+static PyObject *py_ceigh_lds(PyObject *, PyObject *args) {
+  int dtype, n, max_sweeps;
+  long long batch;
+  unsigned long long h, w, v, info, stream;
+  double tol;
+  if (!PyArg_ParseTuple(args, "iLiKKKKdiK", &dtype, &batch, &n, &h, &w, &v, &info, &tol, &max_sweeps, &stream))
+    return nullptr;
+  int st;
+  Py_BEGIN_ALLOW_THREADS
+  st = psvd_ceigh_lds(dtype, batch, n, PTR(h), PTR(w), PTR(v), static_cast<int32_t *>(PTR(info)), tol, max_sweeps,
+                      PTR(stream));
+  Py_END_ALLOW_THREADS
+  return PyLong_FromLong(st);
+}
+
+static PyObject *py_ccholesky_lds(PyObject *, PyObject *args) {
+  int dtype, n;
+  long long batch;
+  unsigned long long g, l, fail, stream;
+  if (!PyArg_ParseTuple(args, "iLiKKKK", &dtype, &batch, &n, &g, &l, &fail, &stream)) return nullptr;
+  int st;
+  Py_BEGIN_ALLOW_THREADS
+  st = psvd_ccholesky_lds(dtype, batch, n, PTR(g), PTR(l), static_cast<int32_t *>(PTR(fail)), PTR(stream));
+  Py_END_ALLOW_THREADS
+  return PyLong_FromLong(st);
+}
+
+static PyObject *py_csolve_lds_smem_bytes(PyObject *, PyObject *args) {
+  int dtype, m, n;
+  if (!PyArg_ParseTuple(args, "iii", &dtype, &m, &n)) return nullptr;
+  size_t bytes = 0;
+  int st = psvd_csolve_lds_smem_bytes(dtype, m, n, &bytes);
+  return Py_BuildValue("(in)", st, (Py_ssize_t)bytes);
+}
+
+static PyObject *py_csolve_chol_lds(PyObject *, PyObject *args) {
+  int dtype, m, n;
+  long long batch;
+  unsigned long long l, x, stream;
+  if (!PyArg_ParseTuple(args, "iLiiKKK", &dtype, &batch, &m, &n, &l, &x, &stream)) return nullptr;
+  int st;
+  Py_BEGIN_ALLOW_THREADS
+  st = psvd_csolve_chol_lds(dtype, batch, m, n, PTR(l), PTR(x), PTR(stream));
+  Py_END_ALLOW_THREADS
+  return PyLong_FromLong(st);
+}
+
 static PyMethodDef kMethods[] = {
     {"version", py_version, METH_NOARGS, "library version string"},
     {"jacobi_lds_workspace_bytes", py_jacobi_lds_workspace_bytes, METH_VARARGS, nullptr},
@@ -444,6 +499,11 @@ static PyMethodDef kMethods[] = {
     {"complete_columns", py_complete_columns, METH_VARARGS, nullptr},
     {"qr_workspace_bytes", py_qr_workspace_bytes, METH_VARARGS, nullptr},
     {"qr", py_qr, METH_VARARGS, nullptr},
+    {"ceigh_lds_smem_bytes", py_ceigh_lds_smem_bytes, METH_VARARGS, nullptr},
+    {"ceigh_lds", py_ceigh_lds, METH_VARARGS, nullptr},
+    {"ccholesky_lds", py_ccholesky_lds, METH_VARARGS, nullptr},
+    {"csolve_lds_smem_bytes", py_csolve_lds_smem_bytes, METH_VARARGS, nullptr},
+    {"csolve_chol_lds", py_csolve_chol_lds, METH_VARARGS, nullptr},
     {nullptr, nullptr, 0, nullptr}};
 
 static struct PyModuleDef kModule = {PyModuleDef_HEAD_INIT, "_hip", "gfx950 SVD kernels (C ABI binding)",

@@ -279,15 +279,31 @@ def eigh_lds_fits(dtype, n):
     return ((b + 15) & ~15) <= EIGH_LDS_CAP
 
 
+def ceigh_fits(dtype, n):
+    """Whether the complex two-sided Jacobi kernel holds an n x n Hermitian matrix in LDS."""
+    st, _ = N._hip.ceigh_lds_smem_bytes(N.dtype_code(dtype), n)
+    return st == N.OK
+
+
 def jacobi_eigh_raw(H, tol=None, max_sweeps=30, *, method='auto', inner_sweeps=1):
-    """Two-sided Jacobi of the symmetric batch H (b, n, n): w ascending, V, info (device).
+    """Two-sided Jacobi of the symmetric (or, complex, Hermitian) batch H (b, n, n): w ascending,
+    V, info (device).
 
     method 'cyclic' keeps the reference's rotation order (LDS kernel, or the per-round HBM path
     for large n); 'blocked' groups rotations into 64 x 64 subproblems applied with MFMA; 'auto'
-    takes the LDS kernel when the matrix fits and the blocked path otherwise."""
+    takes the LDS kernel when the matrix fits and the blocked path otherwise.  Complex H takes the
+    complex LDS kernel (the reference's complex rotation, primitives.py:398-417) and must fit it."""
     b, n, _ = H.shape
     H = H.contiguous()
     code = N.dtype_code(H.dtype)
+    if H.is_complex():
+        w = torch.empty(b, n, dtype=H.real.dtype, device=H.device)
+        v = torch.empty(b, n, n, dtype=H.dtype, device=H.device)
+        info = torch.empty(b, 2, dtype=torch.int32, device=H.device)
+        st = N._hip.ceigh_lds(code, b, n, N.ptr(H), N.ptr(w), N.ptr(v), N.ptr(info), float(tol or 0.0),
+                              int(max_sweeps), N.stream_of(H))
+        N.check(st, 'psvd_ceigh_lds')
+        return w, v, info
     if method == 'auto':
         method = 'cyclic' if eigh_lds_fits(H.dtype, n) else 'blocked'
     if method not in ('cyclic', 'blocked'):
@@ -320,8 +336,16 @@ def jacobi_eigh(H, tol=None, max_sweeps=30, *, method='auto', inner_sweeps=1):
 
 
 def cholesky(G, *, gemm='f32'):
-    """G (b, n, n) symmetric -> L lower (G = L L^T), fail (b,) bool (primitives.py:145-159)."""
+    """G (b, n, n) symmetric (complex: Hermitian) -> L lower (G = L L^H), fail (b,) bool
+    (primitives.py:145-159).  Complex G is factored in LDS (one workgroup per matrix)."""
     b, n, _ = G.shape
+    if G.is_complex():
+        G = G.contiguous()
+        L = torch.empty_like(G)
+        fail = torch.empty(b, dtype=torch.int32, device=G.device)
+        N.check(N._hip.ccholesky_lds(N.dtype_code(G.dtype), b, n, N.ptr(G), N.ptr(L), N.ptr(fail),
+                                     N.stream_of(G)), 'psvd_ccholesky_lds')
+        return L, fail != 0
     L = G.contiguous().clone()
     fail = torch.empty(b, dtype=torch.int32, device=G.device)
     st = N._hip.cholesky(N.dtype_code(G.dtype), {'f32': 0, 'bf16x3': 1}.get(gemm, 0), b, n, N.ptr(L), n, n * n,
@@ -342,6 +366,23 @@ def trsm_right_(B, L, *, trans, gemm='f32'):
                            B.stride(0) if b > 1 else 0, N.stream_of(B))
     N.check(st, 'psvd_trsm_right')
     return B
+
+
+def csolve_fits(dtype, m, n):
+    """Whether psvd_csolve_chol_lds holds an m x n block and its n x n factor in LDS."""
+    st, _ = N._hip.csolve_lds_smem_bytes(N.dtype_code(dtype), m, n)
+    return st == N.OK
+
+
+def csolve_chol_(X, L):
+    """In place for complex X (b, m, n) and lower L (b, n, n): X <- X (L L^H)^-1, the reference's
+    trsm_upper(L^H, trsm_lower(L, X^H))^H (solvers.py:159-160)."""
+    b, m, n = X.shape
+    if not (X.is_contiguous() and L.is_contiguous()):
+        raise ValueError('csolve_chol_ takes contiguous operands')
+    N.check(N._hip.csolve_chol_lds(N.dtype_code(X.dtype), b, m, n, N.ptr(L), N.ptr(X), N.stream_of(X)),
+            'psvd_csolve_chol_lds')
+    return X
 
 
 def trsm_lower(L, B):

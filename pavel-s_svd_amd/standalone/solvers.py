@@ -226,17 +226,23 @@ def _gram(x, stats, opts):
 
 def _gram_complex(x, stats, opts):
     """Complex gram_jacobi (solvers.py:99-119) for n x n Gram matrices that fit in LDS: G = sym(A^H A),
-    its eigenpairs from the complex one-sided Jacobi (G is positive semidefinite, so its singular
-    pairs are its eigenpairs, descending; eigen_sweeps counts one-sided sweeps), sigma = sqrt(w) above
-    eps * n * w_1, U = A V / sigma with the cut columns completed."""
+    its eigenpairs from the complex two-sided Jacobi (the reference's jacobi_eigh, flipped to
+    descending; beyond that kernel's LDS budget the one-sided Jacobi of G, whose singular pairs are its
+    eigenpairs), sigma = sqrt(w) above eps * n * w_1, U = A V / sigma with the cut columns completed."""
     b, m, n = x.shape
     real = torch.float32 if x.dtype == torch.complex64 else torch.float64
     g = P.cmatmul(x, x, ha=True)
     P.hermitize_(g)
-    _, w, vh, info = P.jacobi_lds_uniform(g, tol=None, max_sweeps=opts.get('max_sweeps', 60), cut='floor')
-    info = info.cpu()
     from .. import native as N
-    N.raise_for_info(info, f"Hermitian Jacobi did not converge in {opts.get('max_sweeps', 60)} sweeps")
+    max_sweeps = opts.get('max_sweeps', 60)
+    if P.ceigh_fits(x.dtype, n):   # the reference's two-sided Jacobi (solvers.py:107), eigenvalues flipped
+        w, v, info = P.jacobi_eigh_raw(g, max_sweeps=max_sweeps)
+        w = w.flip(-1)
+        vh = v.flip(-1).mH.contiguous()
+    else:
+        _, w, vh, info = P.jacobi_lds_uniform(g, tol=None, max_sweeps=max_sweeps, cut='floor')
+    info = info.cpu()
+    N.raise_for_info(info, f"Hermitian Jacobi did not converge in {max_sweeps} sweeps")
     stats['eigen_sweeps'] = int(info[:, 0].max())
     keep = w > EPS[real] * n * w[:, :1]
     s = torch.where(keep, w.clamp_min(0).sqrt(), torch.zeros_like(w))
@@ -596,12 +602,15 @@ def _polar(a, stats, opts):
 
 def _polar_complex(a, stats, opts):
     """QDWH polar SVD of a complex tall batch held to LDS sizes (solvers.py:122-178 with the complex
-    arithmetic of primitives.py): X = A / ||A||_F, dynamically weighted Halley steps
-    X <- (b/c) X + (a - b/c) / sqrt(c) Q1 Q2^H from the QR of [sqrt(c) X; I] -- every step in this
-    QR form, which the reference takes while c > 100 and which equals its Cholesky form
-    X (I + c X^H X)^-1 in exact arithmetic -- stopping on max|X_new - X| <= tol max(max|X|, 1) over
-    the batch; then H = sym(X^H A), whose eigenpairs (H is positive semidefinite) are its singular
-    pairs, from the complex one-sided Jacobi kernel; U = X V."""
+    arithmetic of primitives.py): X = A / ||A||_F and dynamically weighted Halley steps with the
+    reference's branch rule -- while c > 100, X <- (b/c) X + (a - b/c) / sqrt(c) Q1 Q2^H from the QR
+    of [sqrt(c) X; I]; afterwards Z = I + c X^H X, L = chol(sym Z) (RuntimeError when it loses positive
+    definiteness) and X <- (b/c) X + (a - b/c) X Z^-1 by the two triangular solves -- stopping on
+    max|X_new - X| <= tol max(max|X|, 1) over the batch; then H = sym(X^H A) and its eigenpairs from
+    the complex two-sided Jacobi (primitives.py:370-427), sigma the flipped eigenvalues clamped at 0,
+    U = X V with the cut columns completed.  Shapes whose Cholesky step or eigensolve do not fit in
+    LDS take the QR form for every step and the one-sided Jacobi of H (H is positive semidefinite, so
+    its singular pairs are its eigenpairs)."""
     b, m, n = a.shape
     real = torch.float32 if a.dtype == torch.complex64 else torch.float64
     x = P.scale(a, P.reduce(a, 'fro'), invert=True)
@@ -612,12 +621,23 @@ def _polar_complex(a, stats, opts):
     max_iter = opts.get('max_iter', 40)
     ident = torch.zeros(b, n, n, dtype=a.dtype, device=a.device)
     ident.diagonal(dim1=-2, dim2=-1).fill_(1)
-    converged = False
+    chol_form = P.csolve_fits(a.dtype, m, n)
+    converged, nqr = False, 0
     for it in range(max_iter):
         aa, bb, cc, lower = qdwh_weights(lower)
-        q, _ = P.cqr(torch.cat([P.scale(x, None, s=math.sqrt(cc)), ident], -2))
-        xn = P.cmatmul(q[:, :m], q[:, m:], hb=True)
-        _axpby_complex(bb / cc, x, (aa - bb / cc) / math.sqrt(cc), xn)
+        if cc > 100 or not chol_form:
+            nqr += 1
+            q, _ = P.cqr(torch.cat([P.scale(x, None, s=math.sqrt(cc)), ident], -2))
+            xn = P.cmatmul(q[:, :m], q[:, m:], hb=True)
+            _axpby_complex(bb / cc, x, (aa - bb / cc) / math.sqrt(cc), xn)
+        else:
+            z = _axpby_complex(cc, P.cmatmul(x, x, ha=True), 1.0, ident.clone())    # I + c X^H X
+            P.hermitize_(z)
+            chol, fail = P.cholesky(z)
+            if bool(fail.any()):
+                raise RuntimeError('polar Cholesky lost positive definiteness')
+            xn = P.csolve_chol_(x.clone(), chol)                                     # X Z^-1
+            _axpby_complex(bb / cc, x, aa - bb / cc, xn)
         d = P.reduce(xn, 'maxdiff', x).max(0).values.cpu()
         x = xn
         if float(d[0]) <= tol * max(float(d[1]), 1.0):
@@ -627,15 +647,23 @@ def _polar_complex(a, stats, opts):
         raise RuntimeError(f'polar_qdwh did not converge in {max_iter} iterations')
     h = P.cmatmul(x, a, ha=True)
     P.hermitize_(h)
-    _, w, vh, info = P.jacobi_lds_uniform(h, tol=None, max_sweeps=opts.get('max_sweeps', 60), cut='floor')
-    info = info.cpu()
     from .. import native as N
-    N.raise_for_info(info, f"Hermitian Jacobi did not converge in {opts.get('max_sweeps', 60)} sweeps")
+    max_sweeps = opts.get('max_sweeps', 60)
+    if P.ceigh_fits(a.dtype, n):
+        w, v, info = P.jacobi_eigh_raw(h, max_sweeps=max_sweeps)
+        info = info.cpu()
+        N.raise_for_info(info, f'Hermitian Jacobi did not converge in {max_sweeps} sweeps')
+        w = w.flip(-1).clamp_min(0)
+        vh = v.flip(-1).mH.contiguous()
+    else:
+        _, w, vh, info = P.jacobi_lds_uniform(h, tol=None, max_sweeps=max_sweeps, cut='floor')
+        info = info.cpu()
+        N.raise_for_info(info, f'Hermitian Jacobi did not converge in {max_sweeps} sweeps')
     keep = w > EPS[real] * max(m, n) * w[:, :1]
     u = P.complete_null_columns(P.cmatmul(x, vh, hb=True), keep)
     w = torch.where(keep, w, torch.zeros_like(w))
     fixed = P.reduce(P.cmatmul(x, P.cmatmul(x, x, ha=True)), 'maxdiff', x)[:, 0].max()
-    stats.update(polar_iterations=it + 1, polar_qr_iterations=it + 1, eigen_sweeps=int(info[:, 0].max()),
+    stats.update(polar_iterations=it + 1, polar_qr_iterations=nqr, eigen_sweeps=int(info[:, 0].max()),
                  polar_fixed_point_error=float(fixed))
     return u, w, vh
 

@@ -9,6 +9,7 @@ metrics within 3x of the reference's recorded ones on the corpus.
 """
 import numpy as np
 import pytest
+import scipy.linalg
 import torch
 
 from conftest import corpus, load_package
@@ -228,10 +229,10 @@ def test_complex_tsqr_matches_oracle(pkg, dtype, shape, rows):
 
 @pytest.mark.parametrize('case', ['clustered_complex', 'wide_complex'])
 def test_complex_qdwh_matches_reference_golden(pkg, expected, case):
-    """svd(a, 'polar_qdwh') on the complex corpus: QDWH iterations within one of the reference's
-    recorded count (its late steps take the Cholesky form, every step here the QR form, the same
-    iterate in exact arithmetic), sigma against the oracle's solver, metrics within 3x of the
-    reference's recorded ones (or the data type's floor)."""
+    """svd(a, 'polar_qdwh') on the complex corpus: the reference's branch rule (QR form while c > 100,
+    then the Cholesky form) gives its recorded QDWH iteration count exactly and the two-sided
+    eigensolver of H its eigen sweeps within one; sigma against the oracle's solver, metrics within 3x
+    of the reference's recorded ones (or the data type's floor)."""
     a = corpus(case)
     print(f'complex qdwh {case} {a.shape} {a.dtype}', flush=True)
     u, s, vh, info = pkg.svd(torch.from_numpy(a).to(DEV), 'polar_qdwh', return_info=True)
@@ -239,14 +240,17 @@ def test_complex_qdwh_matches_reference_golden(pkg, expected, case):
     runs = [r for r in expected[case]['runs'] if r['method'] == 'polar_qdwh']
     assert runs
     for run in runs:
-        assert abs(info['polar_iterations'] - run['diagnostics']['polar_iterations']) <= 1, run['source']
+        assert info['polar_iterations'] == run['diagnostics']['polar_iterations'], run['source']
+        assert abs(info['eigen_sweeps'] - run['diagnostics']['eigen_sweeps']) <= 1, run['source']
         floor = 64 * np.finfo(a.real.dtype).eps * max(a.shape[-2:]) ** 0.5
         for key in KEYS:
             assert met[key] <= max(3 * run['metrics'][key], floor), (key, met[key], run['metrics'][key])
     _, so, _, oinfo = R.svd(a, 'polar_qdwh', return_info=True)
     bar = 1e-5 if a.dtype == np.complex64 else 1e-12
     assert np.abs(_np(s) - so).max() <= bar * so.max()
-    assert abs(info['polar_iterations'] - oinfo['polar_iterations']) <= 1
+    assert info['polar_iterations'] == oinfo['polar_iterations']
+    assert abs(info['eigen_sweeps'] - oinfo['eigen_sweeps']) <= 1
+    assert 1 <= info['polar_qr_iterations'] < info['polar_iterations']
 
 
 @pytest.mark.parametrize('dtype,shape', [(np.complex64, (3, 40, 17)), (np.complex128, (2, 48, 48)),
@@ -278,6 +282,7 @@ def test_complex_gram_matches_reference_golden(pkg, expected, case):
     assert runs
     for run in runs:
         assert info['gram_discarded_values'] == run['diagnostics']['gram_discarded_values'], run['source']
+        assert abs(info['eigen_sweeps'] - run['diagnostics']['eigen_sweeps']) <= 1, run['source']
         floor = 64 * np.finfo(a.real.dtype).eps * max(a.shape[-2:]) ** 0.5
         for key in ('orthogonality_u_fro_over_sqrt_cols', 'orthogonality_v_fro_over_sqrt_rows',
                     'reconstruction_relative_fro', 'spectrum_error_over_sigma1'):
@@ -398,3 +403,61 @@ def test_complex_full_matrices_and_rank_deficient(pkg, method, dtype):
     assert np.abs(u.conj().T @ u - np.eye(24)).max() <= 10 * bar
     assert np.abs(vh @ vh.conj().T - np.eye(9)).max() <= 10 * bar
     assert np.abs((u[:, :9] * s) @ vh - a[0]).max() <= 10 * bar * np.abs(a).max()
+
+
+@pytest.mark.parametrize('dtype', [np.complex64, np.complex128])
+@pytest.mark.parametrize('b,n', [(3, 1), (2, 2), (4, 7), (2, 16), (1, 33), (2, 64)])
+def test_complex_eigh_matches_lapack_and_oracle(dtype, b, n):
+    """The complex two-sided Jacobi (primitives.py:370-427, complex branch): eigenvalues against
+    LAPACK's zheevd, V unitary and V diag(w) V^H = H, sweep counts within one of the oracle's (the
+    reference's rotation order; FP64 sums change at most the last sweep)."""
+    from importlib import import_module
+    P = import_module('pavel-s_svd_amd.standalone.primitives')
+    rng = np.random.default_rng(7 * n + b)
+    h = rng.standard_normal((b, n, n)) + 1j * rng.standard_normal((b, n, n))
+    h = ((h + np.conj(np.swapaxes(h, 1, 2))) / 2).astype(dtype)
+    print(f'complex eigh {b}x{n} {np.dtype(dtype).name}', flush=True)
+    w, v, sweeps = P.jacobi_eigh(torch.from_numpy(h).to(DEV), max_sweeps=60)
+    w, v = w.cpu().numpy().astype(np.float64), v.cpu().numpy().astype(np.complex128)
+    bar = 2e-6 if dtype == np.complex64 else 1e-12
+    for i in range(b):
+        ref = scipy.linalg.eigh(h[i].astype(np.complex128), eigvals_only=True)
+        scale = max(np.abs(ref).max(), 1e-300)
+        assert np.all(np.diff(w[i]) >= 0)
+        assert np.max(np.abs(w[i] - ref)) / scale < bar * np.sqrt(n)
+        assert np.max(np.abs(np.conj(v[i].T) @ v[i] - np.eye(n))) < bar * n
+        assert np.max(np.abs((v[i] * w[i]) @ np.conj(v[i].T) - h[i])) / scale < bar * n
+    _, _, rsw = R.jacobi_eigh(h)
+    assert abs(sweeps - rsw) <= 1
+
+
+@pytest.mark.parametrize('dtype', [np.complex64, np.complex128])
+@pytest.mark.parametrize('b,m,n', [(3, 1, 1), (2, 20, 9), (2, 64, 48), (1, 96, 64)])
+def test_complex_cholesky_and_solve(dtype, b, m, n):
+    """Complex Cholesky against LAPACK's zpotrf (lower), the reference's failure mask on an
+    indefinite matrix, and X (L L^H)^-1 against a direct solve (solvers.py:159-160)."""
+    from importlib import import_module
+    P = import_module('pavel-s_svd_amd.standalone.primitives')
+    if not P.csolve_fits(torch.complex64 if dtype == np.complex64 else torch.complex128, m, n):
+        pytest.skip('beyond the LDS budget of the complex solve')
+    rng = np.random.default_rng(m + n)
+    x = (rng.standard_normal((b, n, n)) + 1j * rng.standard_normal((b, n, n)))
+    z = (np.eye(n) + 0.3 * np.conj(np.swapaxes(x, 1, 2)) @ x).astype(dtype)
+    L, fail = P.cholesky(torch.from_numpy(z).to(DEV))
+    assert not bool(fail.any())
+    L = L.cpu().numpy().astype(np.complex128)
+    bar = 3e-5 if dtype == np.complex64 else 1e-12
+    for i in range(b):
+        ref = scipy.linalg.cholesky(z[i].astype(np.complex128), lower=True)
+        assert np.max(np.abs(L[i] - ref)) <= bar * np.abs(ref).max()
+    rhs = (rng.standard_normal((b, m, n)) + 1j * rng.standard_normal((b, m, n))).astype(dtype)
+    xt = torch.from_numpy(rhs).to(DEV)
+    P.csolve_chol_(xt, torch.from_numpy(L.astype(dtype)).to(DEV))
+    got = xt.cpu().numpy().astype(np.complex128)
+    for i in range(b):
+        ref = np.linalg.solve(z[i].astype(np.complex128).T, rhs[i].astype(np.complex128).T).T   # X Z^-1
+        assert np.max(np.abs(got[i] - ref)) <= 50 * bar * np.abs(ref).max()
+    bad = z.copy()
+    bad[0, n - 1, n - 1] = -1.0
+    _, fail = P.cholesky(torch.from_numpy(bad).to(DEV))
+    assert bool(fail[0]) and not bool(fail[1:].any())
