@@ -269,6 +269,9 @@ int psvd_trsm_right(int dtype, int gemm_mode, int64_t batch, int trans, int n, i
  * workgroup owns a strided share of every round's column pairs, one grid barrier per round,
  * FP64 dot products / convergence test / sigma, sort, cut and completion on the device.
  * Same algorithm and flags as psvd_jacobi_lds (primitives.py:316-367); info = [sweeps, status].
+ * Complex dtypes (PSVD_C64 / PSVD_C128): interleaved a, u, vh, real s, the complex rotation of
+ * primitives.py:340-351, vh = V^H; the cut columns are completed by psvd_ccomplete_columns in the
+ * same stream (m + n up to about 9700).
  * Not capturable in a hipGraph (cooperative launch).
  * ------------------------------------------------------------------------------------- */
 size_t psvd_jacobi_grid_workspace_bytes(int dtype, int m, int n);

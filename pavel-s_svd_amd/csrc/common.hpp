@@ -140,6 +140,12 @@ size_t clds_smem_bytes(int dtype, int m, int n, bool vectors);
 int run_clds(int dtype, int64_t batch, int m, int n, const void *a, void *u, void *s, void *vh, int32_t *info,
              double tol, int max_sweeps, int flags, int lanes, void *ws, hipStream_t stream);
 
+// Complex cooperative one-sided Jacobi (csrc/jacobi_grid_complex.hip), reached through
+// psvd_jacobi_grid with dtype PSVD_C64 / PSVD_C128.
+size_t cgrid_workspace_bytes(int dtype, int m, int n);
+int cjacobi_grid(int dtype, int m, int n, const void *a, void *u, void *s, void *vh, int32_t *info, double tol,
+                 int max_sweeps, int flags, void *workspace, size_t workspace_bytes, hipStream_t stream);
+
 // Item e of the upper triangle (k1 <= k2) of an np2 x np2 grid, row by row.
 __device__ __forceinline__ void tri_decode(int64_t e, int np2, int &k1, int &k2) {
   int a = (int)((2.0 * np2 + 1.0 - sqrt((2.0 * np2 + 1.0) * (2.0 * np2 + 1.0) - 8.0 * (double)e)) / 2.0);

@@ -433,6 +433,7 @@ size_t psvd_jacobi_grid_workspace_bytes(int dtype, int m, int n) {
   if (m < 1 || n < 1 || n > m) return 0;
   if (dtype == PSVD_F32) return psvd::grid_ws_bytes<float>(m, n, psvd::grid_size<float>(n));
   if (dtype == PSVD_F64) return psvd::grid_ws_bytes<double>(m, n, psvd::grid_size<double>(n));
+  if (dtype == PSVD_C64 || dtype == PSVD_C128) return psvd::cgrid_workspace_bytes(dtype, m, n);
   return 0;
 }
 
@@ -446,6 +447,8 @@ int psvd_jacobi_grid(int dtype, int m, int n, const void *a, void *u, void *s, v
     return psvd::run_grid<float>(m, n, a, u, s, vh, info, tol, max_sweeps, flags, workspace, workspace_bytes, st);
   if (dtype == PSVD_F64)
     return psvd::run_grid<double>(m, n, a, u, s, vh, info, tol, max_sweeps, flags, workspace, workspace_bytes, st);
+  if (dtype == PSVD_C64 || dtype == PSVD_C128)
+    return psvd::cjacobi_grid(dtype, m, n, a, u, s, vh, info, tol, max_sweeps, flags, workspace, workspace_bytes, st);
   return PSVD_EINVAL;
 }
 

@@ -68,7 +68,7 @@ def jacobi_grid(x, *, tol=None, max_sweeps=60, scale=False, cut='reference', vec
     flags = (N.SCALE if scale else 0) | (N.CUT_FLOOR if cut == 'floor' else 0) | (0 if vectors else N.NO_VECTORS)
     dev = x.device
     u = torch.empty_like(x) if vectors else None
-    s = torch.empty(b, n, dtype=x.dtype, device=dev)
+    s = torch.empty(b, n, dtype=x.real.dtype if x.is_complex() else x.dtype, device=dev)
     vh = torch.empty(b, n, n, dtype=x.dtype, device=dev) if vectors else None
     info = torch.empty(b, 2, dtype=torch.int32, device=dev)
     wsb = N._hip.jacobi_grid_workspace_bytes(code, m, n)
@@ -108,15 +108,18 @@ def jacobi_blocked(x, *, tol=None, max_sweeps=60, scale=False, cut='reference', 
 
 
 def jacobi_any(x, method='auto', **kw):
-    """K1 when the shape fits LDS; otherwise the blocked kernel (n >= 64) or the cooperative one.
-    method 'grid' or 'blocked' forces a large-matrix kernel."""
+    """K1 when the shape fits LDS; otherwise the blocked kernel (n >= 64) or the cooperative one
+    (complex matrices beyond LDS: the complex cooperative kernel).  method 'grid' or 'blocked' forces
+    a large-matrix kernel."""
     b, m, n = x.shape
     if method == 'auto' and lds_fits(x.dtype, m, n, kw.get('vectors', True)):
         return jacobi_lds_uniform(x, **kw)
-    if x.is_complex():
-        raise N.NativeError('complex one-sided Jacobi beyond LDS', N.ETOOBIG)
     kw.pop('lanes', None)
     kw.pop('threads', None)
+    if x.is_complex():   # the complex cooperative kernel (the reference's rotation order) at any size
+        if method not in ('auto', 'grid'):
+            raise ValueError(f'complex matrices beyond LDS take the grid kernel, not {method!r}')
+        return jacobi_grid(x, **kw)
     if method == 'blocked' or (method == 'auto' and n >= 64):
         return jacobi_blocked(x, **kw)
     if method not in ('auto', 'grid'):

@@ -764,9 +764,6 @@ def svd(a, algorithm='jacobi', *, full_matrices=False, values_only=False, return
     if a.is_complex() and algorithm == 'tsqr' and not _complex_tsqr_fits(a.dtype, max(a.shape[-2:]),
                                                                           min(a.shape[-2:]), opts.get('tile_rows')):
         raise Unsupported('complex TSQR is built for leaves that fit in LDS only')
-    if a.is_complex() and algorithm == 'jacobi' and not P.lds_fits(a.dtype, max(a.shape[-2:]), min(a.shape[-2:]),
-                                                                    not values_only):
-        raise Unsupported('complex one-sided Jacobi is built for matrices that fit in LDS only')
     if a.is_complex() and algorithm == 'qr' and not (P.cqr_fits(a.dtype, max(a.shape[-2:]), min(a.shape[-2:])) and
                                                      P.lds_fits(a.dtype, min(a.shape[-2:]), min(a.shape[-2:]), True)):
         raise Unsupported('complex QR is built for matrices that fit in LDS only')
