@@ -532,6 +532,40 @@ def config_l0(pkg, dev, count=32, cpu=True):
     return out
 
 
+def config_gemm(pkg, dev):
+    """K3 (psvd_gemm) per arithmetic mode at the shapes the QDWH and TSQR solves issue: the square
+    products of QDWH 4096 (X^T X, Q_top Q_bot^T, H = X^T A) and TSQR's 2048^3 back-transformation are
+    compute-bound; the two products of a QR panel's trailing update on the stacked [sqrt(c) X; I]
+    (W = V^T A: 32 x 4064 x 8192, then A -= V W': 8192 x 4064 x 32) read or rewrite the whole trailing
+    matrix for 2 x 32 flops per element and are HBM-bound.  TF/s against the mode's MFMA peak and GB/s of the minimum traffic (C read and written
+    once, A and B read once) against 8 TB/s."""
+    gemm = importlib.import_module('pavel-s_svd_amd.standalone.blas').gemm
+    shapes = {'square_4096': (4096, 4096, 4096, False, False), 'qdwh_qtop_qbotT_4096': (4096, 4096, 4096, False, True),
+              'tsqr_backtransform_2048': (2048, 2048, 2048, False, False),
+              'qr_panel_vtA_32x4064x8192': (32, 4064, 8192, True, False),
+              'qr_trailing_update_8192x4064x32': (8192, 4064, 32, False, False)}
+    peaks = {'f32': PEAK_FP32_TFLOPS, 'bf16x3': 2500.0 / 3, 'f64': 78.6}
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(SEED)
+    res = {}
+    for name, (m, n, k, ta, tb) in shapes.items():
+        for mode in ('f32', 'bf16x3', 'f64'):
+            dt = torch.float64 if mode == 'f64' else torch.float32
+            a = torch.randn(*((k, m) if ta else (m, k)), generator=gen, device=dev, dtype=dt)
+            b = torch.randn(*((n, k) if tb else (k, n)), generator=gen, device=dev, dtype=dt)
+            c = torch.zeros(m, n, device=dev, dtype=dt)
+            aa, bb = (a.mT if ta else a), (b.mT if tb else b)
+            t, _ = _timed(lambda: gemm(aa, bb, alpha=-1.0, beta=1.0, out=c, mode=mode), 5)
+            size = 8 if mode == 'f64' else 4
+            tf = 2.0 * m * n * k / t / 1e12
+            gbs = size * (2 * m * n + m * k + k * n) / t / 1e9
+            res[f'{name}:{mode}'] = {'ms': t * 1e3, 'tflops': tf, 'frac_of_mode_peak': tf / peaks[mode],
+                                     'min_traffic_gbs': gbs, 'frac_of_hbm': gbs / PEAK_HBM_GBS}
+            del a, b, c
+    return {'workload': 'psvd_gemm C = A B - C at the QDWH / TSQR shapes, three arithmetic modes, median of 5 calls',
+            'unit': 'ms', 'peaks_tflops': peaks, 'results': res}
+
+
 def load_traffic():
     path = os.path.join(ROOT, 'profiles', 'jacobi_lds_uneven4096_traffic.json')
     if os.path.exists(path):
@@ -550,7 +584,7 @@ def main():
     ap.add_argument('--no-tune', action='store_true', help='use the default launch-shape policy')
     ap.add_argument('--no-plain', action='store_true', help='skip the plain GPU form (per-matrix calls), e.g. when '
                     'profiling the batched launch alone')
-    ap.add_argument('--only', default='all', choices=['all', 'c2', 'single', 'qdwh', 'tsqr', 'l0', 'consumers'],
+    ap.add_argument('--only', default='all', choices=['all', 'c2', 'single', 'qdwh', 'tsqr', 'l0', 'consumers', 'gemm'],
                     help="'c2': the batched-Jacobi line alone; single / qdwh / tsqr / l0 / consumers: that leg alone "
                          "(printed as {\"configs\": ...}, for profiling)")
     ap.add_argument('--copy-calibration', action='store_true',
@@ -578,10 +612,10 @@ def main():
     pkg = importlib.import_module('pavel-s_svd_amd')
     legs = {'single': ('single_512x256', config_single), 'qdwh': ('qdwh_4096', config_qdwh),
             'tsqr': ('tsqr_8192x2048', config_tsqr), 'l0': ('learned_l0_1024', config_l0),
-            'consumers': ('consumers', config_consumers)}
+            'consumers': ('consumers', config_consumers), 'gemm': ('gemm', config_gemm)}
     if args.only in legs:
         name, fn = legs[args.only]
-        kw = {} if name == 'consumers' else {'cpu': not args.no_cpu}
+        kw = {} if name in ('consumers', 'gemm') else {'cpu': not args.no_cpu}
         print(json.dumps({'configs': {name: fn(pkg, torch.device('cuda:0'), **kw)}}), flush=True)
         return
     B = importlib.import_module('pavel-s_svd_amd.standalone.batched')
@@ -639,7 +673,7 @@ def main():
         out['configs'] = {}
         for name, fn in legs.values():
             t0 = time.perf_counter()
-            out['configs'][name] = fn(pkg, dev) if name == 'consumers' else fn(pkg, dev, cpu=not args.no_cpu)
+            out['configs'][name] = fn(pkg, dev) if name in ('consumers', 'gemm') else fn(pkg, dev, cpu=not args.no_cpu)
             print(f'{name} done in {time.perf_counter() - t0:.1f} s', file=sys.stderr, flush=True)
     print(json.dumps(out), flush=True)
 
