@@ -325,9 +325,9 @@ def qdwh_flops(n, polar_iterations, qr_iterations, eigen_sweeps):
 
 def config_qdwh(pkg, dev, n=4096, cpu=True):
     """n x n float32, Haar U and V, sigma log-uniform on [1e-6, 1] with 10% in a width-1e-4 cluster
-    at 0.5: the MI355X form (BF16x3 QR-branch products, FP64 polish), the equal-accuracy FP32 form
-    (FP32 products, the same FP64 polish), the plain FP32 form and the reference's defaults (cut at
-    eps max(m, n) sigma_1 with the completion of the cut columns, FP32 products, no polish)."""
+    at 0.5: the MI355X form (FP32 products, FP64 polish), the same with BF16x3 QR-branch products,
+    the plain FP32 form and the reference's defaults (cut at eps max(m, n) sigma_1 with the
+    completion of the cut columns, FP32 products, no polish)."""
     gen = torch.Generator(device=dev)
     gen.manual_seed(SEED)
     rng = np.random.default_rng(SEED)
@@ -337,8 +337,8 @@ def config_qdwh(pkg, dev, n=4096, cpu=True):
     sig[0] = 1.0
     s = torch.from_numpy(sig).to(dev)
     a = _product(_haar64(n, n, gen, dev), s, _haar64(n, n, gen, dev)).float()
-    forms = {'mi355x': dict(gemm='bf16x3', polish=True, cut='floor'),
-             'fp32_polish': dict(gemm='f32', polish=True, cut='floor'),
+    forms = {'mi355x': dict(gemm='f32', polish=True, cut='floor'),
+             'bf16x3_polish': dict(gemm='bf16x3', polish=True, cut='floor'),
              'plain_fp32': dict(gemm='f32', cut='floor'),
              'reference_defaults': dict()}
     res = {}
@@ -362,7 +362,7 @@ def config_qdwh(pkg, dev, n=4096, cpu=True):
                        '(the reference defaults zero sigma < eps*max(m,n)*sigma_1 = 4.9e-4 and complete those U '
                        'columns)',
            'unit': 's/solve', 'results': res,
-           'speedup_vs_fp32_polish': res['fp32_polish']['seconds'] / res['mi355x']['seconds'],
+           'bf16x3_speedup': res['mi355x']['seconds'] / res['bf16x3_polish']['seconds'],
            'speedup_vs_plain_fp32': res['plain_fp32']['seconds'] / res['mi355x']['seconds']}
     if cpu:
         r = res['reference_defaults']
@@ -373,7 +373,8 @@ def config_qdwh(pkg, dev, n=4096, cpu=True):
 
 def config_tsqr(pkg, dev, cpu=True):
     """8192 x 2048 float32: rank-64 G1 diag(sigma) G2^T (orthonormal factors, sigma geometric 1 ..
-    1e-3) plus N(0, 1e-8) noise; TSQR with 256-row panels, then Jacobi on R."""
+    1e-3) plus N(0, 1e-8) noise; TSQR with 256-row panels, then Jacobi on R: the MI355X form (FP32
+    products, FP64 polish of U and V), the same with BF16x3 products, and the plain FP32 form."""
     gen = torch.Generator(device=dev)
     gen.manual_seed(SEED)
     s = torch.from_numpy(np.geomspace(1.0, 1e-3, 64)).to(dev)
@@ -381,7 +382,7 @@ def config_tsqr(pkg, dev, cpu=True):
     a = (a + 1e-4 * torch.randn(8192, 2048, generator=gen, device=dev, dtype=torch.float64)).float()
     res = {}
     m, n = a.shape
-    for name, opts in (('mi355x', dict(gemm='bf16x3', polish=True)), ('fp32_polish', dict(gemm='f32', polish=True)),
+    for name, opts in (('mi355x', dict(gemm='f32', polish=True)), ('bf16x3_polish', dict(gemm='bf16x3', polish=True)),
                        ('plain_fp32', dict(gemm='f32'))):
         t, (u, sv, vh, info) = _timed(lambda: pkg.svd(a, 'tsqr', tile_rows=256, return_info=True, **opts), 3)
         res[name] = {'seconds': t, 'options': opts, 'tsqr_leaves': info.get('tsqr_leaves'),
@@ -395,7 +396,7 @@ def config_tsqr(pkg, dev, cpu=True):
     out = {'workload': 'TSQR, 8192x2048 float32, rank-64 (sigma 1 .. 1e-3) plus N(0, 1e-8) noise (its |E|_2 ~ '
                        '0.014 sets the spectrum, so accuracy is reconstruction and orthogonality in FP64), 256-row '
                        'panels (2048-row leaves: rows = max(tile_rows, n)), Jacobi on R', 'unit': 's/solve',
-           'results': res, 'speedup_vs_fp32_polish': res['fp32_polish']['seconds'] / res['mi355x']['seconds']}
+           'results': res, 'bf16x3_speedup': res['mi355x']['seconds'] / res['bf16x3_polish']['seconds']}
     if cpu:
         r = res['plain_fp32']
         out['cpu_baseline'] = cpu_tsqr(m, n, r['tsqr_leaves'] or 1, r['jacobi_sweeps'])
@@ -537,8 +538,10 @@ def config_gemm(pkg, dev):
     products of QDWH 4096 (X^T X, Q_top Q_bot^T, H = X^T A) and TSQR's 2048^3 back-transformation are
     compute-bound; the two products of a QR panel's trailing update on the stacked [sqrt(c) X; I]
     (W = V^T A: 32 x 4064 x 8192, then A -= V W': 8192 x 4064 x 32) read or rewrite the whole trailing
-    matrix for 2 x 32 flops per element and are HBM-bound.  TF/s against the mode's MFMA peak and GB/s of the minimum traffic (C read and written
-    once, A and B read once) against 8 TB/s."""
+    matrix for 2 x 32 flops per element and are HBM-bound.  TF/s against the mode's MFMA peak and GB/s
+    of the minimum traffic (C read and written once, A and B read once) against 8 TB/s.  The public
+    psvd_gemm has no workspace, so the 32 x 4064 output runs unsplit on 32 workgroups; the QR solver
+    issues that product through the split-K path (32 K-slices and an ordered reduction)."""
     gemm = importlib.import_module('pavel-s_svd_amd.standalone.blas').gemm
     shapes = {'square_4096': (4096, 4096, 4096, False, False), 'qdwh_qtop_qbotT_4096': (4096, 4096, 4096, False, True),
               'tsqr_backtransform_2048': (2048, 2048, 2048, False, False),
@@ -562,7 +565,8 @@ def config_gemm(pkg, dev):
             res[f'{name}:{mode}'] = {'ms': t * 1e3, 'tflops': tf, 'frac_of_mode_peak': tf / peaks[mode],
                                      'min_traffic_gbs': gbs, 'frac_of_hbm': gbs / PEAK_HBM_GBS}
             del a, b, c
-    return {'workload': 'psvd_gemm C = A B - C at the QDWH / TSQR shapes, three arithmetic modes, median of 5 calls',
+    return {'workload': 'psvd_gemm C = A B - C at the QDWH / TSQR shapes, three arithmetic modes, median of 5 calls'
+                            ' (public entry point, no split-K workspace)',
             'unit': 'ms', 'peaks_tflops': peaks, 'results': res}
 
 

@@ -3,17 +3,18 @@
 * QDWH polar SVD of the 4096 x 4096 float32 matrix U diag(sigma) V^T (Haar U, V from numpy's QR with
   the R-diagonal sign fix, sigma log-uniform on [1e-6, 1] with 10 % in a width-1e-4 cluster at 0.5,
   FP64 product rounded once), generated on the host by the oracle's workload generator:
-  - the MI355X form (gemm='bf16x3', polish=True, cut='floor'): sigma against the generating spectrum
-    within 1e-5 sigma_1 (rounding A to FP32 moves sigma by at most |A - fl(A)|_2 <= 2^-24 |A|_F,
-    about 1e-6 here), |U^T U - I|_F / sqrt(n) and the same for V below 1e-5, reconstruction
-    |A - U S V^T|_F / |A|_F below 1e-4 (the BF16x3 products' 2^-16 backward error);
+  - the MI355X form (gemm='f32', polish=True, cut='floor') and the same with BF16x3 products:
+    sigma against the generating spectrum within 1e-5 sigma_1 (rounding A to FP32 moves sigma by at
+    most |A - fl(A)|_2 <= 2^-24 |A|_F, about 1e-6 here), |U^T U - I|_F / sqrt(n) and the same for V
+    below 1e-5, reconstruction |A - U S V^T|_F / |A|_F below 1e-4 (the bar the BF16x3 products'
+    2^-16 backward error allows);
   - the default options (the reference's cut at eps max(m, n) sigma_1, FP32 products, no polish):
     the kept sigma within the reference's FP32 envelope 3e-4 sigma_1, the cut ones exactly 0 and
     below the cut in the generating spectrum, U (with its completed columns) and V orthonormal to
     3e-4, reconstruction within the dropped part's norm plus 3e-4.
 * TSQR of the 8192 x 2048 float32 rank-64 + N(0, 1e-8) matrix with tile_rows=256 (2048-row leaves):
-  sigma against FP64 LAPACK within 1e-5 sigma_1, orthogonality below 1e-5 (MI355X form) / 1e-4 (the
-  default FP32 form), reconstruction below 1e-4.
+  sigma against FP64 LAPACK within 1e-5 sigma_1, orthogonality below 1e-5 (the polished forms, FP32
+  or BF16x3 products) / 1e-4 (the default FP32 form), reconstruction below 1e-4.
 * Learned l0 at 1024 x 1024: the packaged model on 16 matrices of its test split; QDWH from the
   learned bound takes fewer iterations on average than from the reference's eps, at equal accuracy
   against FP64 LAPACK (each gate within twice the eps run's, or 2e-5).
@@ -57,14 +58,15 @@ def qdwh_case():
     return a, sig
 
 
-def test_qdwh_4096_mi355x_form(pkg, qdwh_case):
+@pytest.mark.parametrize('form,gemm', [('mi355x', 'f32'), ('bf16x3_polish', 'bf16x3')])
+def test_qdwh_4096_polished_forms(pkg, qdwh_case, form, gemm):
     a, sig = qdwh_case
-    u, s, vh, info = pkg.svd(torch.from_numpy(a).to(DEV), 'polar_qdwh', gemm='bf16x3', polish=True, cut='floor',
+    u, s, vh, info = pkg.svd(torch.from_numpy(a).to(DEV), 'polar_qdwh', gemm=gemm, polish=True, cut='floor',
                              return_info=True)
     u, s, vh = u.cpu().numpy(), s.cpu().numpy().astype(np.float64), vh.cpu().numpy()
     err = float(np.max(np.abs(s - sig)) / sig[0])
     ou, ov, rec = _orth(u), _orth(vh.T), _recon(a.astype(np.float64), u, s, vh)
-    print(f"QDWH 4096 MI355X form: sigma {err:.2e} orth U {ou:.2e} V {ov:.2e} recon {rec:.2e} "
+    print(f"QDWH 4096 {form}: sigma {err:.2e} orth U {ou:.2e} V {ov:.2e} recon {rec:.2e} "
           f"iterations {info['polar_iterations']} eigen sweeps {info['eigen_sweeps']}")
     assert err <= 1e-5 and ou <= 1e-5 and ov <= 1e-5 and rec <= 1e-4
 
@@ -95,7 +97,8 @@ def tsqr_case():
     return a, scipy.linalg.svdvals(a.astype(np.float64))
 
 
-@pytest.mark.parametrize('form,opts,orth_bar', [('mi355x', dict(gemm='bf16x3', polish=True), 1e-5),
+@pytest.mark.parametrize('form,opts,orth_bar', [('mi355x', dict(gemm='f32', polish=True), 1e-5),
+                                                ('bf16x3_polish', dict(gemm='bf16x3', polish=True), 1e-5),
                                                 ('default', dict(), 1e-4)])
 def test_tsqr_8192x2048(pkg, tsqr_case, form, opts, orth_bar):
     a, ref = tsqr_case

@@ -80,3 +80,22 @@ def test_complete_null_columns_matches_oracle(P, dtype):
     full = torch.eye(3, 4, dtype=torch.float64, device=DEV)[None]
     with pytest.raises(RuntimeError):
         P.complete_null_columns(full, torch.tensor([[True, True, True, False]], device=DEV))
+
+
+@pytest.mark.parametrize('m', [3556, 3557])
+def test_complete_columns_lds_and_global_variants(P, m):
+    """The completion kernel keeps v, the leverages and y in LDS while 8 (2m + n) bytes plus its
+    8384 static bytes fit in 64 KiB: at n = 32 that is m <= 3556, so the two sizes run the LDS and the
+    global-memory variant of the same kernel (9 matrices: more than the whole-GPU path takes)."""
+    rng = np.random.default_rng(m)
+    b, n, k = 9, 32, 8
+    u = np.zeros((b, m, n))
+    keep = np.zeros((b, n), dtype=bool)
+    keep[:, :k] = True
+    for i in range(b):
+        u[i][:, :k] = scipy.linalg.qr(rng.standard_normal((m, k)), mode='economic')[0]
+    ref = R.complete_null_columns(u.copy(), keep)
+    out = P.complete_null_columns(torch.from_numpy(u).to(DEV), torch.from_numpy(keep).to(DEV)).cpu().numpy()
+    assert np.max(np.abs(out - ref)) < 1e-12
+    for i in range(b):
+        assert np.allclose(out[i].T @ out[i], np.eye(n), atol=1e-13)
