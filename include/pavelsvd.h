@@ -52,6 +52,9 @@ typedef void *psvd_stream_t;
 #define PSVD_ORDER_SORTED 16 /* psvd_jacobi_lds: pair ordering -- at the start of every sweep the
                                tournament's labels are the columns in descending norm order (de Rijk)
                                instead of the reference's identity labels (primitives.py:309-313) */
+#define PSVD_BLOCK_NARROW 32 /* psvd_jacobi_blocked: column blocks of 16 (32 x 32 subproblems; the
+                               default when neither block flag is given) */
+#define PSVD_BLOCK_WIDE 64   /* psvd_jacobi_blocked: column blocks of 32 (64 x 64 subproblems) */
 
 /* GEMM arithmetic modes */
 #define PSVD_GEMM_F32 0     /* FP32 operands, FP32 MFMA, FP32 accumulation */
@@ -281,11 +284,12 @@ int psvd_jacobi_grid(int dtype, int m, int n, const void *a, void *u, void *s, v
 
 /* ---------------------------------------------------------------------------------------
  * Blocked one-sided Jacobi SVD of ONE large tall matrix (primitives.py:316-367 with the rotations
- * grouped): columns in blocks of 32, a tournament over the blocks, for each block pair the 64 x 64
- * Gram matrix in FP64 on MFMA swept by `inner_sweeps` (>= 1) cyclic sweeps of the reference's
- * one-sided rotations and test, the accumulated rotation applied to W and V on MFMA; stops after a
- * sweep without rotations.  Same flags, outputs and info as psvd_jacobi_grid (info[0] counts block
- * sweeps).  One host read per sweep (not capturable).
+ * grouped): columns in blocks of 16 (or 32 with PSVD_BLOCK_WIDE), a tournament over the blocks, for
+ * each block pair the 32 x 32 (64 x 64) Gram matrix in FP64 on MFMA swept by `inner_sweeps` (>= 1)
+ * cyclic sweeps of the reference's one-sided rotations and test, the accumulated rotation applied
+ * to W and V on MFMA; stops after a sweep without rotations.  Same flags as psvd_jacobi_grid plus
+ * the block-width flags, same outputs and info (info[0] counts block sweeps).  One host read per
+ * sweep (not capturable).
  * ------------------------------------------------------------------------------------- */
 size_t psvd_jacobi_blocked_workspace_bytes(int dtype, int m, int n);
 int psvd_jacobi_blocked(int dtype, int m, int n, const void *a, void *u, void *s, void *vh, int32_t *info, double tol,

@@ -1,27 +1,37 @@
 // K2b: blocked one-sided Jacobi SVD of one large tall matrix (m x n, m >= n).
 //
-// The columns of W (m x N, N = n padded to a multiple of 64 with zero columns that never rotate)
-// are grouped in blocks of 32; round r of a tournament over the nb blocks pairs block I with block J.
-// For each pair p the 64 x 64 Gram matrix S_p = [W_I W_J]^T [W_I W_J] is formed in FP64 on MFMA
+// The columns of W (m x N, N = n padded to a multiple of 2 KB with zero columns that never rotate)
+// are grouped in blocks of KB = 16 or 32; round r of a tournament over the nb blocks pairs block I
+// with block J.  For each pair p the KS x KS (KS = 2 KB) Gram matrix S_p = [W_I W_J]^T [W_I W_J] is
+// formed in FP64 on MFMA
 // (row slices of W, per-slice partials added in a fixed order), and S_p is swept by `inner_sweeps`
 // cyclic sweeps of two-sided plane rotations in LDS -- the rotation of pair (a, b) of S is exactly
 // the one-sided rotation of columns a and b of W (app = s_aa, aqq = s_bb, apq = s_ab), with the
 // reference's test apq^2 > tol^2 app aqq and min(app, aqq) > eps^2 max_j |w_j|^2
 // (primitives.py:337-338; the column-norm maximum is taken at the start of each sweep).  The
-// accumulated 64 x 64 rotation Q_p then updates W[:, I u J] and V[:, I u J] on MFMA.  A sweep without
+// accumulated KS x KS rotation Q_p then updates W[:, I u J] and V[:, I u J] on MFMA.  A sweep without
 // a rotation ends the iteration (primitives.py:352).  The tail -- FP64 sigma, descending order, cut
 // at eps max(m, n) sigma_1 (or eps sigma_1), U = W / sigma, coordinate-vector completion, row-major
 // outputs -- follows primitives.py:356-366 and :430-455.
 //
-// Work per round: 2 m 64^2 FP64 MFMA flops per pair for the Gram, 2 (m + N) 64^2 for the updates;
-// nb - 1 rounds per sweep.
+// Work per round: 2 m KS^2 FP64 MFMA flops per pair for the Gram, 2 (m + N) KS^2 for the updates;
+// nb - 1 rounds per sweep, so the work per sweep does not depend on KB.  What KB sets is the
+// parallelism and the serial chain: a round runs nb / 2 pair sweeps side by side, each KS - 1
+// rotation rounds of O(KS^2) LDS traffic.  Blocks of 16 give twice the pairs per round and a pair
+// sweep of 31 short rounds instead of 63 rounds four times as heavy.
 #include <float.h>
 
 #include "common.hpp"
 
 namespace psvd {
 
-constexpr int kJb = 32, kJs = 64, kJsLd = kJs + 1, kJbThreads = 256;
+constexpr int kJbThreads = 256;
+
+// Block width KB: subproblem size KS, LDS row stride, threads of the pair-sweep kernel (one
+// upper-triangle 2 x 2 block of a rotation round per thread: 528 for KB = 32, 136 for KB = 16).
+template <int KB> struct BjShape {
+  static constexpr int KS = 2 * KB, LD = KS + 1, INNER = KB == 32 ? 576 : 256;
+};
 
 typedef float j4f __attribute__((ext_vector_type(4)));
 typedef double j4d __attribute__((ext_vector_type(4)));
@@ -43,7 +53,8 @@ struct BjParams {
   int32_t *nk;              // 1
 };
 
-__device__ __forceinline__ int bj_col(int I, int J, int i) { return i < kJb ? I * kJb + i : J * kJb + (i - kJb); }
+template <int KB>
+__device__ __forceinline__ int bj_col(int I, int J, int i) { return i < KB ? I * KB + i : J * KB + (i - KB); }
 
 template <typename T>
 __global__ __launch_bounds__(kJbThreads) void bj_absmax_kernel(BjParams<T> P) {
@@ -118,34 +129,35 @@ __global__ __launch_bounds__(kJbThreads) void bj_colmax_kernel(BjParams<T> P) {
   }
 }
 
-// Gram partial of pair p over the rows of slice z: 64 x 64, FP64 MFMA (16x16x4), wave w owns the
-// 32 x 32 quadrant (w / 2, w % 2).
+// Gram partial of pair p over the rows of slice z: KS x KS, FP64 MFMA (16x16x4), wave w owns the
+// KB x KB quadrant (w / 2, w % 2), TI x TI MFMA tiles.
 constexpr int kGramRows = 64;   // rows of W staged per step of the Gram partial
 
-template <typename T>
-__device__ __forceinline__ void bj_gram_body(const BjParams<T> &P, int p, int z, int r, double (*Ws)[kJs + 1]) {
-  constexpr int KR = kGramRows;
+template <typename T, int KB>
+__device__ __forceinline__ void bj_gram_body(const BjParams<T> &P, int p, int z, int r,
+                                             double (*Ws)[BjShape<KB>::LD]) {
+  constexpr int KR = kGramRows, KS = BjShape<KB>::KS, TI = KB / 16;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int lr = lane & 15, lk = lane >> 4, wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  const int lr = lane & 15, lk = lane >> 4, wm = (wave >> 1) * KB, wn = (wave & 1) * KB;
   int I, J;
   rr_pair(r, p, P.nb, I, J);
   const int rows_per = (P.m + P.splits - 1) / P.splits;
   const int r0 = z * rows_per, r1 = r0 + rows_per < P.m ? r0 + rows_per : P.m;
-  j4d acc[2][2];
+  j4d acc[TI][TI];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < TI; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int j = 0; j < TI; ++j)
 #pragma unroll
       for (int q = 0; q < 4; ++q) acc[i][j][q] = 0.0;
   using V16 = Vec16<T>;
-  constexpr int VE = V16::n, RV = kJs / VE;   // 16-byte vectors: a row's two 32-column blocks
+  constexpr int VE = V16::n, RV = KS / VE;   // 16-byte vectors: a row's two KB-column blocks
   for (int k0 = r0; k0 < r1; k0 += KR) {
     __syncthreads();
     for (int e = tid; e < KR * RV; e += kJbThreads) {
       const int rr = e / RV, c = (e - rr * RV) * VE, row = k0 + rr;
       if (row < r1) {
-        const V16 x = *reinterpret_cast<const V16 *>(P.W + (int64_t)row * P.N + bj_col(I, J, c));
+        const V16 x = *reinterpret_cast<const V16 *>(P.W + (int64_t)row * P.N + bj_col<KB>(I, J, c));
 #pragma unroll
         for (int u = 0; u < VE; ++u) Ws[rr][c + u] = (double)x.v[u];
       } else {
@@ -156,85 +168,86 @@ __device__ __forceinline__ void bj_gram_body(const BjParams<T> &P, int p, int z,
     __syncthreads();
 #pragma unroll 4
     for (int k = 0; k < KR; k += 4) {
-      double a[2], b[2];
+      double a[TI], b[TI];
 #pragma unroll
-      for (int i = 0; i < 2; ++i) a[i] = Ws[k + lk][wm + 16 * i + lr];   // A = W^T: A[col][row]
+      for (int i = 0; i < TI; ++i) a[i] = Ws[k + lk][wm + 16 * i + lr];   // A = W^T: A[col][row]
 #pragma unroll
-      for (int j = 0; j < 2; ++j) b[j] = Ws[k + lk][wn + 16 * j + lr];
+      for (int j = 0; j < TI; ++j) b[j] = Ws[k + lk][wn + 16 * j + lr];
 #pragma unroll
-      for (int i = 0; i < 2; ++i)
+      for (int i = 0; i < TI; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+        for (int j = 0; j < TI; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
     }
   }
-  double *out = P.Sp + ((size_t)p * P.splits + z) * kJs * kJs;
+  double *out = P.Sp + ((size_t)p * P.splits + z) * KS * KS;
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < TI; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int j = 0; j < TI; ++j)
 #pragma unroll
       for (int q = 0; q < 4; ++q)   // FP64 MFMA D layout: register q of lane l is row 4 q + l / 16
-        out[(wm + 16 * i + 4 * q + lk) * kJs + wn + 16 * j + lr] = acc[i][j][q];
+        out[(wm + 16 * i + 4 * q + lk) * KS + wn + 16 * j + lr] = acc[i][j][q];
 }
 
-template <typename T>
+template <typename T, int KB>
 __host__ __device__ inline size_t bj_inner_lds() {
-  return sizeof(double) * (2 * kJb + 2 * (size_t)kJs * kJsLd) + sizeof(int) * 2 * kJb;
+  constexpr int KS = BjShape<KB>::KS, LD = BjShape<KB>::LD;
+  return sizeof(double) * (2 * KB + 2 * (size_t)KS * LD) + sizeof(int) * 2 * KB;
 }
 
-// Sweeps of S_p (FP64, in LDS) with the one-sided test; Q_p = accumulated rotation.
-// 576 threads: the 528 upper-triangle 2 x 2 blocks of a round are one item per thread, so no
-// thread carries a second block on the round's critical path (512 threads left 16 with two)
-constexpr int kJbInner = 576;
-
-template <typename T>
-__global__ __launch_bounds__(kJbInner) void bj_inner_kernel(BjParams<T> P, int r) {
+// Sweeps of S_p (FP64, in LDS) with the one-sided test; Q_p = accumulated rotation.  One thread
+// per upper-triangle 2 x 2 block of a rotation round (KB (KB + 1) / 2 of them), so no thread
+// carries a second block on the round's critical path.
+template <typename T, int KB>
+__global__ __launch_bounds__(BjShape<KB>::INNER) void bj_inner_kernel(BjParams<T> P, int r) {
+  constexpr int KS = BjShape<KB>::KS, LD = BjShape<KB>::LD, NTI = BjShape<KB>::INNER;
   extern __shared__ __align__(16) double smem_bj[];
   double2 *cs = reinterpret_cast<double2 *>(smem_bj);   // (c, s) of pair k of the round
-  double *S = reinterpret_cast<double *>(cs + kJb), *Qt = S + kJs * kJsLd;
-  int2 *pq = reinterpret_cast<int2 *>(Qt + kJs * kJsLd);   // (p, q) of pair k
+  double *S = reinterpret_cast<double *>(cs + KB), *Qt = S + KS * LD;
+  int2 *pq = reinterpret_cast<int2 *>(Qt + KS * LD);   // (p, q) of pair k
   const int p = blockIdx.x, tid = threadIdx.x;
   const double eps = Eps<T>::value, tol = P.tol > 0.0 ? P.tol : 10.0 * eps;
   const double floor_ = eps * eps * __longlong_as_double((long long)P.slot[1]);
-  for (int e = tid; e < kJs * kJs; e += kJbInner) {
-    const int i = e / kJs, j = e - i * kJs;
-    const double *sp = P.Sp + (size_t)p * P.splits * kJs * kJs + e;
+  for (int e = tid; e < KS * KS; e += NTI) {
+    const int i = e / KS, j = e - i * KS;
+    const double *sp = P.Sp + (size_t)p * P.splits * KS * KS + e;
     double acc = sp[0];
-    for (int z = 1; z < P.splits; ++z) acc += sp[(size_t)z * kJs * kJs];
-    S[i * kJsLd + j] = acc;
-    Qt[i * kJsLd + j] = i == j ? 1.0 : 0.0;
+    for (int z = 1; z < P.splits; ++z) acc += sp[(size_t)z * KS * KS];
+    S[i * LD + j] = acc;
+    Qt[i * LD + j] = i == j ? 1.0 : 0.0;
   }
   __syncthreads();
   // a subproblem in which no pair passes the test rotates nowhere in its sweep: skip it
   int live = 0;
-  for (int e = tid; e < kJs * kJs; e += kJbInner) {
-    const int i = e / kJs, j = e - i * kJs;
+  for (int e = tid; e < KS * KS; e += NTI) {
+    const int i = e / KS, j = e - i * KS;
     if (i < j) {
-      const double app = S[i * kJsLd + i], aqq = S[j * kJsLd + j], apq = S[i * kJsLd + j];
+      const double app = S[i * LD + i], aqq = S[j * LD + j], apq = S[i * LD + j];
       live |= apq * apq > tol * tol * app * aqq && fmin(app, aqq) > floor_;
     }
   }
   live = __syncthreads_or(live);
   int any = 0;
   // this thread's items of the round, fixed for the launch: upper-triangle blocks (k1 <= k2) of
-  // the 32 x 32 pair grid (528: one or two per thread) and four Qt row pairs at column tid % 64
-  constexpr int NTRI = kJb * (kJb + 1) / 2, NIT = (NTRI + kJbInner - 1) / kJbInner;
+  // the KB x KB pair grid and the Qt row pairs k = tid / KS + j QSTEP at column tid % KS
+  constexpr int NTRI = KB * (KB + 1) / 2, NIT = (NTRI + NTI - 1) / NTI;
   int it1[NIT], it2[NIT];
 #pragma unroll
   for (int j = 0; j < NIT; ++j) {
-    const int e = tid + j * kJbInner;
+    const int e = tid + j * NTI;
     it1[j] = it2[j] = -1;
-    if (e < NTRI) tri_decode(e, kJb, it1[j], it2[j]);
+    if (e < NTRI) tri_decode(e, KB, it1[j], it2[j]);
   }
-  constexpr int QSTEP = kJbInner / kJs, QIT = (kJb + QSTEP - 1) / QSTEP;   // Qt row pairs per column
-  const int qcol = tid % kJs, qk = tid / kJs;
+  constexpr int QSTEP = NTI / KS, QIT = (KB + QSTEP - 1) / QSTEP;   // Qt row pairs per column
+  static_assert(NTI % KS == 0, "the Qt items tile the columns exactly");
+  const int qcol = tid % KS, qk = tid / KS;
   for (int sw = 0; live && sw < P.inner_sweeps; ++sw) {
     int act_sw = 0;
-    for (int rr = 0; rr < kJs - 1; ++rr) {
-      if (tid < kJb) {
+    for (int rr = 0; rr < KS - 1; ++rr) {
+      if (tid < KB) {
         int a, b;
-        rr_pair(rr, tid, kJs, a, b);
-        const double app = S[a * kJsLd + a], aqq = S[b * kJsLd + b], apq = S[a * kJsLd + b];
+        rr_pair(rr, tid, KS, a, b);
+        const double app = S[a * LD + a], aqq = S[b * LD + b], apq = S[a * LD + b];
         const bool act = apq * apq > tol * tol * app * aqq && fmin(app, aqq) > floor_;
         double c = 1.0, s = 0.0;
         if (act) jacobi_rotation_fast<T>(app, aqq, apq, c, s);
@@ -249,9 +262,9 @@ __global__ __launch_bounds__(kJbInner) void bj_inner_kernel(BjParams<T> P, int r
         const double2 r1 = cs[it1[j]], r2 = cs[it2[j]];
         if (r1.y == 0.0 && r2.y == 0.0) continue;
         const int2 a1 = pq[it1[j]], a2 = pq[it2[j]];
-        block_update(S, kJsLd, a1.x, a1.y, a2.x, a2.y, r1.x, r1.y, r2.x, r2.y, r1.y != 0.0, r2.y != 0.0);
+        block_update(S, LD, a1.x, a1.y, a2.x, a2.y, r1.x, r1.y, r2.x, r2.y, r1.y != 0.0, r2.y != 0.0);
       }
-      rotate_rows_batched<double, kJb, QSTEP, QIT>(Qt, kJsLd, cs, pq, qk, qcol);
+      rotate_rows_batched<double, KB, QSTEP, QIT>(Qt, LD, cs, pq, qk, qcol);
       __syncthreads();
     }
     act_sw = __syncthreads_or(act_sw);
@@ -259,9 +272,9 @@ __global__ __launch_bounds__(kJbInner) void bj_inner_kernel(BjParams<T> P, int r
     if (!act_sw) break;
   }
   if (any)
-    for (int e = tid; e < kJs * kJs; e += kJbInner) {
-      const int i = e / kJs, j = e - i * kJs;
-      P.Q[(size_t)p * kJs * kJs + e] = (T)Qt[j * kJsLd + i];   // Q = Qt^T
+    for (int e = tid; e < KS * KS; e += NTI) {
+      const int i = e / KS, j = e - i * KS;
+      P.Q[(size_t)p * KS * KS + e] = (T)Qt[j * LD + i];   // Q = Qt^T
     }
   if (tid == 0) {
     P.act[p] = any;
@@ -282,91 +295,97 @@ template <> struct BjMfma<double> {
 };
 
 // X[:, I u J] <- X[:, I u J] Q_p for a 64-row tile of W (tiles [0, mt)) or of V (tiles [mt, mt + nt)).
-template <typename T>
+constexpr int kTileRows = 64;
+
+template <typename T, int KB>
 __host__ __device__ inline size_t bj_update_lds() {
-  return sizeof(T) * 2 * (size_t)kJs * kJsLd;
+  return sizeof(T) * (size_t)(kTileRows + BjShape<KB>::KS) * BjShape<KB>::LD;
 }
 
-template <typename T>
+// The 64 x KS product on four waves: wave w owns rows 32 (w / 2) .. + 32 and columns KB (w % 2) .. + KB.
+template <typename T, int KB>
 __device__ __forceinline__ void bj_update_body(const BjParams<T> &P, int p, int t, int r, double *smem) {
-  T (*Xs)[kJsLd] = reinterpret_cast<T (*)[kJsLd]>(smem);
-  T (*Qs)[kJsLd] = Xs + kJs;
+  constexpr int KS = BjShape<KB>::KS, LD = BjShape<KB>::LD, TJ = KB / 16;
+  T (*Xs)[LD] = reinterpret_cast<T (*)[LD]>(smem);
+  T (*Qs)[LD] = Xs + kTileRows;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   if (!P.act[p]) return;
-  const int lr = lane & 15, lk = lane >> 4, wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
-  const int mt = (P.m + kJs - 1) / kJs;
+  const int lr = lane & 15, lk = lane >> 4, wm = (wave >> 1) * 32, wn = (wave & 1) * KB;
+  const int mt = (P.m + kTileRows - 1) / kTileRows;
   T *X = t < mt ? P.W : P.V;
-  const int rows = t < mt ? P.m : P.N, r0 = (t < mt ? t : t - mt) * kJs;
+  const int rows = t < mt ? P.m : P.N, r0 = (t < mt ? t : t - mt) * kTileRows;
   int I, J;
   rr_pair(r, p, P.nb, I, J);
   using V16 = Vec16<T>;
-  constexpr int VE = V16::n, RV = kJs / VE;   // 16-byte vectors: a row's two 32-column blocks
-  const V16 *Qp = reinterpret_cast<const V16 *>(P.Q + (size_t)p * kJs * kJs);
-  for (int e = tid; e < kJs * RV; e += kJbThreads) {
+  constexpr int VE = V16::n, RV = KS / VE;   // 16-byte vectors: a row's two KB-column blocks
+  const V16 *Qp = reinterpret_cast<const V16 *>(P.Q + (size_t)p * KS * KS);
+  for (int e = tid; e < kTileRows * RV; e += kJbThreads) {
     const int i = e / RV, j = (e - i * RV) * VE, row = r0 + i;
     V16 x;
     if (row < rows) {
-      x = *reinterpret_cast<const V16 *>(X + (int64_t)row * P.N + bj_col(I, J, j));
+      x = *reinterpret_cast<const V16 *>(X + (int64_t)row * P.N + bj_col<KB>(I, J, j));
     } else {
 #pragma unroll
       for (int u = 0; u < VE; ++u) x.v[u] = T(0);
     }
+#pragma unroll
+    for (int u = 0; u < VE; ++u) Xs[i][j + u] = x.v[u];
+  }
+  for (int e = tid; e < KS * RV; e += kJbThreads) {
+    const int i = e / RV, j = (e - i * RV) * VE;
     const V16 y = Qp[e];
 #pragma unroll
-    for (int u = 0; u < VE; ++u) {
-      Xs[i][j + u] = x.v[u];
-      Qs[i][j + u] = y.v[u];
-    }
+    for (int u = 0; u < VE; ++u) Qs[i][j + u] = y.v[u];
   }
   __syncthreads();
-  typename BjMfma<T>::V acc[2][2];
+  typename BjMfma<T>::V acc[2][TJ];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int j = 0; j < TJ; ++j)
 #pragma unroll
       for (int q = 0; q < 4; ++q) acc[i][j][q] = 0;
 #pragma unroll 4
-  for (int k = 0; k < kJs; k += 4) {
-    T a[2], b[2];
+  for (int k = 0; k < KS; k += 4) {
+    T a[2], b[TJ];
 #pragma unroll
     for (int i = 0; i < 2; ++i) a[i] = Xs[wm + 16 * i + lr][k + lk];
 #pragma unroll
-    for (int j = 0; j < 2; ++j) b[j] = Qs[k + lk][wn + 16 * j + lr];
+    for (int j = 0; j < TJ; ++j) b[j] = Qs[k + lk][wn + 16 * j + lr];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
-      for (int j = 0; j < 2; ++j) acc[i][j] = BjMfma<T>::op(a[i], b[j], acc[i][j]);
+      for (int j = 0; j < TJ; ++j) acc[i][j] = BjMfma<T>::op(a[i], b[j], acc[i][j]);
   }
   __syncthreads();   // every wave has read Xs
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int j = 0; j < TJ; ++j)
 #pragma unroll
       for (int q = 0; q < 4; ++q) Xs[wm + 16 * i + BjMfma<T>::row(q, lk)][wn + 16 * j + lr] = (T)acc[i][j][q];
   __syncthreads();
-  for (int e = tid; e < kJs * RV; e += kJbThreads) {
+  for (int e = tid; e < kTileRows * RV; e += kJbThreads) {
     const int i = e / RV, j = (e - i * RV) * VE, row = r0 + i;
     if (row < rows) {
       V16 x;
 #pragma unroll
       for (int u = 0; u < VE; ++u) x.v[u] = Xs[i][j + u];
-      *reinterpret_cast<V16 *>(X + (int64_t)row * P.N + bj_col(I, J, j)) = x;
+      *reinterpret_cast<V16 *>(X + (int64_t)row * P.N + bj_col<KB>(I, J, j)) = x;
     }
   }
 }
 
 // W tiles of round r: blockIdx.x = pair, blockIdx.y = 64-row tile of W.
-template <typename T>
+template <typename T, int KB>
 __global__ __launch_bounds__(kJbThreads) void bj_update_kernel(BjParams<T> P, int r) {
   extern __shared__ double smem_bu2[];
-  bj_update_body<T>(P, (int)blockIdx.x, (int)blockIdx.y, r, smem_bu2);
+  bj_update_body<T, KB>(P, (int)blockIdx.x, (int)blockIdx.y, r, smem_bu2);
 }
 
-template <typename T>
+template <typename T, int KB>
 __host__ __device__ inline size_t bj_round_lds() {
-  const size_t gram = sizeof(double) * kGramRows * (kJs + 1), upd = bj_update_lds<T>();
+  const size_t gram = sizeof(double) * kGramRows * BjShape<KB>::LD, upd = bj_update_lds<T, KB>();
   return gram > upd ? gram : upd;
 }
 
@@ -375,16 +394,16 @@ __host__ __device__ inline size_t bj_round_lds() {
 // tiles (pair, tile) -- V is read by no kernel of the sweep, and Q and the activity flags still
 // hold round vr until round r's subproblem sweeps run, so the V products of a round run beside
 // the next round's Gram partials.
-template <typename T>
+template <typename T, int KB>
 __global__ __launch_bounds__(kJbThreads) void bj_round_kernel(BjParams<T> P, int r, int ngram, int vr, int mt,
                                                              int nt) {
   extern __shared__ double smem_br2[];
   const int b = (int)blockIdx.x, npair = P.nb / 2;
   if (b < ngram) {
-    bj_gram_body<T>(P, b % npair, b / npair, r, reinterpret_cast<double (*)[kJs + 1]>(smem_br2));
+    bj_gram_body<T, KB>(P, b % npair, b / npair, r, reinterpret_cast<double (*)[BjShape<KB>::LD]>(smem_br2));
   } else {
     const int t = b - ngram;
-    bj_update_body<T>(P, t / nt, mt + t % nt, vr, smem_br2);
+    bj_update_body<T, KB>(P, t / nt, mt + t % nt, vr, smem_br2);
   }
 }
 
@@ -496,28 +515,37 @@ static int bj_splits(int m, int npair) {
   return s < 1 ? 1 : s;
 }
 
-template <typename T>
-size_t bj_ws_bytes(int m, int n) {
-  const int N = (n + kJs - 1) / kJs * kJs, npair = N / kJs;
+template <typename T, int KB>
+size_t bj_ws_bytes_kb(int m, int n) {
+  constexpr int KS = BjShape<KB>::KS;
+  const int N = (n + KS - 1) / KS * KS, npair = N / KS;
   auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
-  return al(sizeof(T) * (size_t)m * N) + al(sizeof(T) * (size_t)N * N) + al(sizeof(T) * (size_t)npair * kJs * kJs) +
-         al(sizeof(double) * (size_t)npair * bj_splits(m, npair) * kJs * kJs) + al(sizeof(int) * npair) + al(8 * 2) +
+  return al(sizeof(T) * (size_t)m * N) + al(sizeof(T) * (size_t)N * N) + al(sizeof(T) * (size_t)npair * KS * KS) +
+         al(sizeof(double) * (size_t)npair * bj_splits(m, npair) * KS * KS) + al(sizeof(int) * npair) + al(8 * 2) +
          al(4 * 3) + al(sizeof(double) * N) + al(sizeof(int) * N) + al(4) +
          al(psvd_complete_workspace_bytes(1, m, n));
 }
 
+// Either block width fits in the same workspace.
 template <typename T>
+size_t bj_ws_bytes(int m, int n) {
+  const size_t a = bj_ws_bytes_kb<T, 16>(m, n), b = bj_ws_bytes_kb<T, 32>(m, n);
+  return a > b ? a : b;
+}
+
+template <typename T, int KB>
 static int run_bj(int m, int n, const T *a, T *u, T *s, T *vh, int32_t *info, double tol, int max_sweeps, int flags,
                   int inner_sweeps, void *ws, size_t ws_bytes, hipStream_t st) {
-  if (ws_bytes < bj_ws_bytes<T>(m, n)) return PSVD_EINVAL;
+  constexpr int KS = BjShape<KB>::KS;
+  if (ws_bytes < bj_ws_bytes_kb<T, KB>(m, n)) return PSVD_EINVAL;
   static bool attr = false;
   if (!attr) {
-    if (hipFuncSetAttribute((const void *)bj_inner_kernel<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)bj_inner_lds<T>()) != hipSuccess ||
-        hipFuncSetAttribute((const void *)bj_update_kernel<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)bj_update_lds<T>()) != hipSuccess ||
-        hipFuncSetAttribute((const void *)bj_round_kernel<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)bj_round_lds<T>()) != hipSuccess)
+    if (hipFuncSetAttribute((const void *)bj_inner_kernel<T, KB>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)bj_inner_lds<T, KB>()) != hipSuccess ||
+        hipFuncSetAttribute((const void *)bj_update_kernel<T, KB>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)bj_update_lds<T, KB>()) != hipSuccess ||
+        hipFuncSetAttribute((const void *)bj_round_kernel<T, KB>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)bj_round_lds<T, KB>()) != hipSuccess)
       return PSVD_ELAUNCH;
     attr = true;
   }
@@ -529,8 +557,8 @@ static int run_bj(int m, int n, const T *a, T *u, T *s, T *vh, int32_t *info, do
   P.info = info;
   P.m = m;
   P.n = n;
-  P.N = (n + kJs - 1) / kJs * kJs;
-  P.nb = P.N / kJb;
+  P.N = (n + KS - 1) / KS * KS;
+  P.nb = P.N / KB;
   const int npair = P.nb / 2;
   P.splits = bj_splits(m, npair);
   P.inner_sweeps = inner_sweeps;
@@ -544,8 +572,8 @@ static int run_bj(int m, int n, const T *a, T *u, T *s, T *vh, int32_t *info, do
   };
   P.W = reinterpret_cast<T *>(take(sizeof(T) * (size_t)m * P.N));
   P.V = reinterpret_cast<T *>(take(sizeof(T) * (size_t)P.N * P.N));
-  P.Q = reinterpret_cast<T *>(take(sizeof(T) * (size_t)npair * kJs * kJs));
-  P.Sp = reinterpret_cast<double *>(take(sizeof(double) * (size_t)npair * P.splits * kJs * kJs));
+  P.Q = reinterpret_cast<T *>(take(sizeof(T) * (size_t)npair * KS * KS));
+  P.Sp = reinterpret_cast<double *>(take(sizeof(double) * (size_t)npair * P.splits * KS * KS));
   P.act = reinterpret_cast<int *>(take(sizeof(int) * npair));
   P.slot = reinterpret_cast<unsigned long long *>(take(8 * 2));
   P.flag = reinterpret_cast<int *>(take(4 * 3));
@@ -553,7 +581,7 @@ static int run_bj(int m, int n, const T *a, T *u, T *s, T *vh, int32_t *info, do
   P.ord = reinterpret_cast<int *>(take(sizeof(int) * P.N));
   P.nk = reinterpret_cast<int32_t *>(take(4));
   void *cws = take(psvd_complete_workspace_bytes(1, m, n));
-  const int mt = (m + kJs - 1) / kJs, nt = (flags & PSVD_NO_VECTORS) ? 0 : P.N / kJs;
+  const int mt = (m + kTileRows - 1) / kTileRows, nt = (flags & PSVD_NO_VECTORS) ? 0 : (P.N + kTileRows - 1) / kTileRows;
   auto blocks = [](int64_t items) {
     const int64_t b = (items + kJbThreads - 1) / kJbThreads;
     return (unsigned)(b < 4096 ? (b > 0 ? b : 1) : 4096);
@@ -570,10 +598,12 @@ static int run_bj(int m, int n, const T *a, T *u, T *s, T *vh, int32_t *info, do
     hipLaunchKernelGGL(bj_colmax_kernel<T>, dim3((n + 15) / 16), dim3(kJbThreads), 0, st, P);
     for (int r = 0; r < P.nb - 1; ++r) {
       // Gram partials of round r beside the V tiles of the previous round (none before the first)
-      hipLaunchKernelGGL(bj_round_kernel<T>, dim3((unsigned)(ngram + (vr >= 0 ? nvt : 0))), dim3(kJbThreads),
-                         bj_round_lds<T>(), st, P, r, ngram, vr, mt, nt);
-      hipLaunchKernelGGL(bj_inner_kernel<T>, dim3(npair), dim3(kJbInner), bj_inner_lds<T>(), st, P, r);
-      hipLaunchKernelGGL(bj_update_kernel<T>, dim3(npair, mt), dim3(kJbThreads), bj_update_lds<T>(), st, P, r);
+      hipLaunchKernelGGL((bj_round_kernel<T, KB>), dim3((unsigned)(ngram + (vr >= 0 ? nvt : 0))), dim3(kJbThreads),
+                         (bj_round_lds<T, KB>()), st, P, r, ngram, vr, mt, nt);
+      hipLaunchKernelGGL((bj_inner_kernel<T, KB>), dim3(npair), dim3(BjShape<KB>::INNER), (bj_inner_lds<T, KB>()), st,
+                         P, r);
+      hipLaunchKernelGGL((bj_update_kernel<T, KB>), dim3(npair, mt), dim3(kJbThreads), (bj_update_lds<T, KB>()), st,
+                         P, r);
       if (nt > 0) vr = r;
     }
     if (hipGetLastError() != hipSuccess) return PSVD_ELAUNCH;
@@ -587,8 +617,8 @@ static int run_bj(int m, int n, const T *a, T *u, T *s, T *vh, int32_t *info, do
   }
   // the last round's rotations on V
   if (vr >= 0 && nvt > 0)
-    hipLaunchKernelGGL(bj_round_kernel<T>, dim3((unsigned)nvt), dim3(kJbThreads), bj_round_lds<T>(), st, P, 0, 0, vr,
-                       mt, nt);
+    hipLaunchKernelGGL((bj_round_kernel<T, KB>), dim3((unsigned)nvt), dim3(kJbThreads), (bj_round_lds<T, KB>()), st, P,
+                       0, 0, vr, mt, nt);
   hipLaunchKernelGGL(bj_sigma_kernel<T>, dim3((n + 15) / 16), dim3(kJbThreads), 0, st, P);
   hipLaunchKernelGGL(bj_order_kernel<T>, dim3((n + kJbThreads - 1) / kJbThreads), dim3(kJbThreads), 0, st, P);
   hipLaunchKernelGGL(bj_out_kernel<T>, dim3(blocks((int64_t)m * n)), dim3(kJbThreads), 0, st, P, sweeps, converged);
@@ -600,6 +630,16 @@ static int run_bj(int m, int n, const T *a, T *u, T *s, T *vh, int32_t *info, do
     hipLaunchKernelGGL(bj_status_kernel, dim3(1), dim3(64), 0, st, info, &P.flag[2]);
   }
   return hipGetLastError() == hipSuccess ? PSVD_OK : PSVD_ELAUNCH;
+}
+
+// Block width: PSVD_BLOCK_NARROW / PSVD_BLOCK_WIDE choose it; by default blocks of 16.
+template <typename T>
+static int run_bj_any(int m, int n, const T *a, T *u, T *s, T *vh, int32_t *info, double tol, int max_sweeps,
+                      int flags, int inner_sweeps, void *ws, size_t ws_bytes, hipStream_t st) {
+  const int f = flags & ~(PSVD_BLOCK_NARROW | PSVD_BLOCK_WIDE);
+  if (flags & PSVD_BLOCK_WIDE)
+    return run_bj<T, 32>(m, n, a, u, s, vh, info, tol, max_sweeps, f, inner_sweeps, ws, ws_bytes, st);
+  return run_bj<T, 16>(m, n, a, u, s, vh, info, tol, max_sweeps, f, inner_sweeps, ws, ws_bytes, st);
 }
 
 }  // namespace psvd
@@ -617,16 +657,17 @@ int psvd_jacobi_blocked(int dtype, int m, int n, const void *a, void *u, void *s
                         int max_sweeps, int flags, int inner_sweeps, void *workspace, size_t workspace_bytes,
                         psvd_stream_t stream) {
   if (m < 1 || n < 1 || n > m || !a || !s || !info || !workspace || max_sweeps < 1 || inner_sweeps < 1 ||
-      (flags & ~(PSVD_SCALE | PSVD_CUT_FLOOR | PSVD_NO_VECTORS)) || n > 65536)
+      (flags & ~(PSVD_SCALE | PSVD_CUT_FLOOR | PSVD_NO_VECTORS | PSVD_BLOCK_NARROW | PSVD_BLOCK_WIDE)) ||
+      ((flags & PSVD_BLOCK_NARROW) && (flags & PSVD_BLOCK_WIDE)) || n > 65536)
     return PSVD_EINVAL;
   if (!(flags & PSVD_NO_VECTORS) && (!u || !vh)) return PSVD_EINVAL;
   hipStream_t st = static_cast<hipStream_t>(stream);
   if (dtype == PSVD_F32)
-    return psvd::run_bj<float>(m, n, static_cast<const float *>(a), static_cast<float *>(u), static_cast<float *>(s),
+    return psvd::run_bj_any<float>(m, n, static_cast<const float *>(a), static_cast<float *>(u), static_cast<float *>(s),
                                static_cast<float *>(vh), info, tol, max_sweeps, flags, inner_sweeps, workspace,
                                workspace_bytes, st);
   if (dtype == PSVD_F64)
-    return psvd::run_bj<double>(m, n, static_cast<const double *>(a), static_cast<double *>(u),
+    return psvd::run_bj_any<double>(m, n, static_cast<const double *>(a), static_cast<double *>(u),
                                 static_cast<double *>(s), static_cast<double *>(vh), info, tol, max_sweeps, flags,
                                 inner_sweeps, workspace, workspace_bytes, st);
   return PSVD_EINVAL;

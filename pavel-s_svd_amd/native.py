@@ -17,7 +17,7 @@ DTYPE_CODE = {torch.float32: F32, torch.float64: F64, torch.complex64: C64, torc
 OK, EINVAL, ELAUNCH, ETOOBIG = 0, 1, 2, 3
 INFO_OK, INFO_NOCONV, INFO_COMPLETION, INFO_NONFINITE, INFO_BADSHAPE = 0, 1, 2, 3, 4
 
-SCALE, CUT_FLOOR, NO_VECTORS, SLOT_SPREAD, ORDER_SORTED = 1, 2, 4, 8, 16
+SCALE, CUT_FLOOR, NO_VECTORS, SLOT_SPREAD, ORDER_SORTED, BLOCK_NARROW, BLOCK_WIDE = 1, 2, 4, 8, 16, 32, 64
 
 _STATUS = {EINVAL: 'invalid argument', ELAUNCH: 'kernel launch failed', ETOOBIG: 'problem exceeds the kernel budget'}
 

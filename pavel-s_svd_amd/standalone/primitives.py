@@ -82,15 +82,20 @@ def jacobi_grid(x, *, tol=None, max_sweeps=60, scale=False, cut='reference', vec
     return u, s, vh, info
 
 
-def jacobi_blocked(x, *, tol=None, max_sweeps=60, scale=False, cut='reference', vectors=True, inner_sweeps=1):
-    """Blocked one-sided Jacobi SVD of each tall matrix of ``x`` (b, m, n): 64 x 64 FP64 Gram
-    subproblems and MFMA column updates, one call per matrix.  Same outputs as ``jacobi_grid``."""
+def jacobi_blocked(x, *, tol=None, max_sweeps=60, scale=False, cut='reference', vectors=True, inner_sweeps=1,
+                   block_width=16):
+    """Blocked one-sided Jacobi SVD of each tall matrix of ``x`` (b, m, n): column blocks of
+    ``block_width`` (16 or 32), FP64 Gram subproblems of twice that size and MFMA column updates,
+    one call per matrix.  Same outputs as ``jacobi_grid``."""
     b, m, n = x.shape
     if m < n:
         raise ValueError('jacobi_blocked takes tall matrices (m >= n)')
+    if block_width not in (16, 32):
+        raise ValueError(f'block_width must be 16 or 32, not {block_width!r}')
     x = x.contiguous()
     code = N.dtype_code(x.dtype)
     flags = (N.SCALE if scale else 0) | (N.CUT_FLOOR if cut == 'floor' else 0) | (0 if vectors else N.NO_VECTORS)
+    flags |= N.BLOCK_WIDE if block_width == 32 else N.BLOCK_NARROW
     dev = x.device
     u = torch.empty_like(x) if vectors else None
     s = torch.empty(b, n, dtype=x.dtype, device=dev)
@@ -112,6 +117,7 @@ def jacobi_any(x, method='auto', **kw):
     (complex matrices beyond LDS: the complex cooperative kernel).  method 'grid' or 'blocked' forces
     a large-matrix kernel."""
     b, m, n = x.shape
+    block_width = kw.pop('block_width', None) or 16
     if method == 'auto' and lds_fits(x.dtype, m, n, kw.get('vectors', True)):
         return jacobi_lds_uniform(x, **kw)
     kw.pop('lanes', None)
@@ -121,7 +127,7 @@ def jacobi_any(x, method='auto', **kw):
             raise ValueError(f'complex matrices beyond LDS take the grid kernel, not {method!r}')
         return jacobi_grid(x, **kw)
     if method == 'blocked' or (method == 'auto' and n >= 64):
-        return jacobi_blocked(x, **kw)
+        return jacobi_blocked(x, block_width=block_width, **kw)
     if method not in ('auto', 'grid'):
         raise ValueError(f'unknown Jacobi method {method!r}')
     return jacobi_grid(x, **kw)

@@ -14,9 +14,10 @@ Extension options (no reference counterpart):
                                 reference contract, default) or only below eps*sigma_1, the floor
                                 under which the Jacobi test stops enforcing orthogonality
     lanes, threads              Jacobi launch shape (see ``include/pavelsvd.h``)
-    jacobi_method='auto'        matrices beyond LDS: 'blocked' (64 x 64 Gram subproblems on MFMA,
-                                the default for n >= 64) or 'grid' (one rotation per pair and
-                                round, the reference's order)
+    jacobi_method='auto'        matrices beyond LDS: 'blocked' (Gram subproblems of two column
+                                blocks on MFMA, the default for n >= 64) or 'grid' (one rotation per
+                                pair and round, the reference's order)
+    jacobi_block_width=16       'blocked': columns per block, 16 (32 x 32 subproblems) or 32
     gemm='f32' | 'bf16x3'       arithmetic of the FP32 QR / TSQR products and of the QR-branch QDWH
                                 iterations on MFMA (QDWH's Cholesky-branch steps stay FP32)
     polish=False                qr / tsqr, FP32 data: one FP64 Newton-Schulz step on U and on V.
@@ -65,7 +66,8 @@ def _jacobi(x, stats, opts, scale):
     max_sweeps = opts.get('max_sweeps', 60)
     u, s, vh, info = P.jacobi_any(x, method=opts.get('jacobi_method', 'auto'), tol=opts.get('tol'),
                                   max_sweeps=max_sweeps, scale=scale, cut=opts.get('cut', 'reference'),
-                                  vectors=vectors, lanes=opts.get('lanes'), threads=opts.get('threads'))
+                                  vectors=vectors, lanes=opts.get('lanes'), threads=opts.get('threads'),
+                                  block_width=opts.get('jacobi_block_width'))
     info = info.cpu()
     from .. import native as N
     N.raise_for_info(info, f'one-sided Jacobi did not converge in {max_sweeps} sweeps')

@@ -1,6 +1,6 @@
-"""GPU parity of the blocked one-sided Jacobi kernel (64 x 64 FP64 Gram subproblems, MFMA column
-updates) against FP64 LAPACK, the oracle and the cooperative kernel that keeps the reference's
-rotation order.
+"""GPU parity of the blocked one-sided Jacobi kernel (column blocks of 16 or 32: 32 x 32 or 64 x 64
+FP64 Gram subproblems, MFMA column updates) against FP64 LAPACK, the oracle and the cooperative
+kernel that keeps the reference's rotation order.
 
 Bars: sigma error over sigma_1, orthogonality of U and V, reconstruction: 1e-5 (FP32) / 1e-12 (FP64);
 sigma equal to the cooperative kernel's within the same bars; the reference cut zeroes the same
@@ -33,12 +33,13 @@ def _np(t):
     return t.detach().cpu().numpy()
 
 
+@pytest.mark.parametrize('width', [16, 32])
 @pytest.mark.parametrize('dtype', [np.float32, np.float64])
 @pytest.mark.parametrize('m,n', [(64, 64), (65, 64), (200, 100), (300, 256), (1000, 300), (700, 129)])
-def test_blocked_matches_lapack(P, dtype, m, n):
+def test_blocked_matches_lapack(P, dtype, m, n, width):
     rng = np.random.default_rng(m * 7 + n)
     a = rng.standard_normal((1, m, n)).astype(dtype)
-    u, s, vh, info = P.jacobi_blocked(torch.from_numpy(a).to(DEV), scale=True)
+    u, s, vh, info = P.jacobi_blocked(torch.from_numpy(a).to(DEV), scale=True, block_width=width)
     assert int(info[0, 1]) == 0 and 1 <= int(info[0, 0]) < 60
     met = M.validate(a[0], (_np(u[0]), _np(s[0]), _np(vh[0])), M.reference(a[0]))['worst']
     bar = 1e-5 if dtype == np.float32 else 1e-12
@@ -47,14 +48,15 @@ def test_blocked_matches_lapack(P, dtype, m, n):
     assert np.all(np.diff(_np(s[0])) <= 0)
 
 
-def test_blocked_equals_cooperative_kernel_and_cut(P):
+@pytest.mark.parametrize('width', [16, 32])
+def test_blocked_equals_cooperative_kernel_and_cut(P, width):
     a = X.workload_single_jacobi()
     x = torch.from_numpy(a[None]).to(DEV)
-    _, s1, _, i1 = P.jacobi_blocked(x, tol=1e-7, scale=True, cut='floor')
+    _, s1, _, i1 = P.jacobi_blocked(x, tol=1e-7, scale=True, cut='floor', block_width=width)
     _, s2, _, i2 = P.jacobi_grid(x, tol=1e-7, scale=True, cut='floor')
     assert np.max(np.abs(_np(s1) - _np(s2))) <= 2e-6
     # reference cut: the same sigma are zeroed as by the oracle
-    _, s3, _, _ = P.jacobi_blocked(x, scale=True)
+    _, s3, _, _ = P.jacobi_blocked(x, scale=True, block_width=width)
     _, rs, _ = R.svd(a, 'jacobi', max_sweeps=60)
     assert abs(int((_np(s3) == 0).sum()) - int((rs == 0).sum())) <= 2
 
