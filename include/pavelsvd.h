@@ -202,6 +202,11 @@ int psvd_qr(int dtype, int gemm_mode, int64_t batch, int m, int n, const void *a
  * ------------------------------------------------------------------------------------- */
 int psvd_cqr_lds_smem_bytes(int dtype, int m, int n, size_t *bytes);
 int psvd_cqr_lds(int dtype, int64_t batch, int m, int n, const void *a, void *q, void *r, psvd_stream_t stream);
+/* The same QR for any m >= n: shapes beyond LDS run the same kernel with the planes in the workspace
+ * (one region per matrix; psvd_cqr_workspace_bytes, 0 when the shape fits LDS). */
+size_t psvd_cqr_workspace_bytes(int dtype, int64_t batch, int m, int n);
+int psvd_cqr(int dtype, int64_t batch, int m, int n, const void *a, void *q, void *r, void *workspace,
+             size_t workspace_bytes, psvd_stream_t stream);
 int psvd_cmatmul(int dtype, int transa, int transb, int64_t batch, int m, int k, int n, const void *a, const void *b,
                  void *c, psvd_stream_t stream);
 int psvd_chermitize(int dtype, int64_t batch, int n, void *a, psvd_stream_t stream);
@@ -229,6 +234,18 @@ int psvd_ceigh_lds(int dtype, int64_t batch, int n, const void *h, void *w, void
 int psvd_ccholesky_lds(int dtype, int64_t batch, int n, const void *g, void *l, int32_t *fail, psvd_stream_t stream);
 int psvd_csolve_lds_smem_bytes(int dtype, int m, int n, size_t *bytes);
 int psvd_csolve_chol_lds(int dtype, int64_t batch, int m, int n, const void *l, void *x, psvd_stream_t stream);
+/* The same three for any size: shapes whose planes exceed LDS run the same kernels with the planes in
+ * the workspace (one region per matrix; the *_workspace_bytes queries give 0 when the shape fits
+ * LDS, where the workspace may be null). */
+size_t psvd_ceigh_workspace_bytes(int dtype, int64_t batch, int n);
+int psvd_ceigh(int dtype, int64_t batch, int n, const void *h, void *w, void *v, int32_t *info, double tol,
+               int max_sweeps, void *workspace, size_t workspace_bytes, psvd_stream_t stream);
+size_t psvd_ccholesky_workspace_bytes(int dtype, int64_t batch, int n);
+int psvd_ccholesky(int dtype, int64_t batch, int n, const void *g, void *l, int32_t *fail, void *workspace,
+                   size_t workspace_bytes, psvd_stream_t stream);
+size_t psvd_csolve_chol_workspace_bytes(int dtype, int64_t batch, int m, int n);
+int psvd_csolve_chol(int dtype, int64_t batch, int m, int n, const void *l, void *x, void *workspace,
+                     size_t workspace_bytes, psvd_stream_t stream);
 
 /* ---------------------------------------------------------------------------------------
  * Two-sided cyclic Jacobi eigensolver of a batch of symmetric n x n matrices (primitives.py:

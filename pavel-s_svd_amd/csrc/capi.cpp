@@ -459,6 +459,87 @@ static PyObject *py_csolve_chol_lds(PyObject *, PyObject *args) {
   return PyLong_FromLong(st);
 }
 
+static PyObject *py_cqr_workspace_bytes(PyObject *, PyObject *args) {
+  int dtype, m, n;
+  long long batch;
+  if (!PyArg_ParseTuple(args, "iLii", &dtype, &batch, &m, &n)) return nullptr;
+  return PyLong_FromSize_t(psvd_cqr_workspace_bytes(dtype, batch, m, n));
+}
+
+static PyObject *py_cqr(PyObject *, PyObject *args) {
+  int dtype, m, n;
+  long long batch;
+  unsigned long long a, q, r, ws, wsb, stream;
+  if (!PyArg_ParseTuple(args, "iLiiKKKKKK", &dtype, &batch, &m, &n, &a, &q, &r, &ws, &wsb, &stream)) return nullptr;
+  int st;
+  Py_BEGIN_ALLOW_THREADS
+  st = psvd_cqr(dtype, batch, m, n, PTR(a), PTR(q), PTR(r), PTR(ws), (size_t)wsb, PTR(stream));
+  Py_END_ALLOW_THREADS
+  return PyLong_FromLong(st);
+}
+
+static PyObject *py_ceigh_workspace_bytes(PyObject *, PyObject *args) {
+  int dtype, n;
+  long long batch;
+  if (!PyArg_ParseTuple(args, "iLi", &dtype, &batch, &n)) return nullptr;
+  return PyLong_FromSize_t(psvd_ceigh_workspace_bytes(dtype, batch, n));
+}
+
+static PyObject *py_ceigh(PyObject *, PyObject *args) {
+  int dtype, n, max_sweeps;
+  long long batch;
+  unsigned long long h, w, v, info, ws, wsb, stream;
+  double tol;
+  if (!PyArg_ParseTuple(args, "iLiKKKKdiKKK", &dtype, &batch, &n, &h, &w, &v, &info, &tol, &max_sweeps, &ws, &wsb,
+                        &stream))
+    return nullptr;
+  int st;
+  Py_BEGIN_ALLOW_THREADS
+  st = psvd_ceigh(dtype, batch, n, PTR(h), PTR(w), PTR(v), static_cast<int32_t *>(PTR(info)), tol, max_sweeps,
+                  PTR(ws), (size_t)wsb, PTR(stream));
+  Py_END_ALLOW_THREADS
+  return PyLong_FromLong(st);
+}
+
+static PyObject *py_ccholesky_workspace_bytes(PyObject *, PyObject *args) {
+  int dtype, n;
+  long long batch;
+  if (!PyArg_ParseTuple(args, "iLi", &dtype, &batch, &n)) return nullptr;
+  return PyLong_FromSize_t(psvd_ccholesky_workspace_bytes(dtype, batch, n));
+}
+
+static PyObject *py_ccholesky(PyObject *, PyObject *args) {
+  int dtype, n;
+  long long batch;
+  unsigned long long g, l, fail, ws, wsb, stream;
+  if (!PyArg_ParseTuple(args, "iLiKKKKKK", &dtype, &batch, &n, &g, &l, &fail, &ws, &wsb, &stream)) return nullptr;
+  int st;
+  Py_BEGIN_ALLOW_THREADS
+  st = psvd_ccholesky(dtype, batch, n, PTR(g), PTR(l), static_cast<int32_t *>(PTR(fail)), PTR(ws), (size_t)wsb,
+                      PTR(stream));
+  Py_END_ALLOW_THREADS
+  return PyLong_FromLong(st);
+}
+
+static PyObject *py_csolve_chol_workspace_bytes(PyObject *, PyObject *args) {
+  int dtype, m, n;
+  long long batch;
+  if (!PyArg_ParseTuple(args, "iLii", &dtype, &batch, &m, &n)) return nullptr;
+  return PyLong_FromSize_t(psvd_csolve_chol_workspace_bytes(dtype, batch, m, n));
+}
+
+static PyObject *py_csolve_chol(PyObject *, PyObject *args) {
+  int dtype, m, n;
+  long long batch;
+  unsigned long long l, x, ws, wsb, stream;
+  if (!PyArg_ParseTuple(args, "iLiiKKKKK", &dtype, &batch, &m, &n, &l, &x, &ws, &wsb, &stream)) return nullptr;
+  int st;
+  Py_BEGIN_ALLOW_THREADS
+  st = psvd_csolve_chol(dtype, batch, m, n, PTR(l), PTR(x), PTR(ws), (size_t)wsb, PTR(stream));
+  Py_END_ALLOW_THREADS
+  return PyLong_FromLong(st);
+}
+
 static PyMethodDef kMethods[] = {
     {"version", py_version, METH_NOARGS, "library version string"},
     {"jacobi_lds_workspace_bytes", py_jacobi_lds_workspace_bytes, METH_VARARGS, nullptr},
@@ -484,6 +565,14 @@ static PyMethodDef kMethods[] = {
     {"scale_columns", py_scale_columns, METH_VARARGS, nullptr},
     {"cqr_lds_smem_bytes", py_cqr_lds_smem_bytes, METH_VARARGS, nullptr},
     {"cqr_lds", py_cqr_lds, METH_VARARGS, nullptr},
+    {"cqr_workspace_bytes", py_cqr_workspace_bytes, METH_VARARGS, nullptr},
+    {"cqr", py_cqr, METH_VARARGS, nullptr},
+    {"ceigh_workspace_bytes", py_ceigh_workspace_bytes, METH_VARARGS, nullptr},
+    {"ceigh", py_ceigh, METH_VARARGS, nullptr},
+    {"ccholesky_workspace_bytes", py_ccholesky_workspace_bytes, METH_VARARGS, nullptr},
+    {"ccholesky", py_ccholesky, METH_VARARGS, nullptr},
+    {"csolve_chol_workspace_bytes", py_csolve_chol_workspace_bytes, METH_VARARGS, nullptr},
+    {"csolve_chol", py_csolve_chol, METH_VARARGS, nullptr},
     {"cmatmul", py_cmatmul, METH_VARARGS, nullptr},
     {"chermitize", py_chermitize, METH_VARARGS, nullptr},
     {"ccomplete_columns", py_ccomplete_columns, METH_VARARGS, nullptr},

@@ -18,6 +18,8 @@
 //
 // Complex values are interleaved (re, im) pairs in global memory, as torch stores them, and
 // separate real and imaginary planes in LDS.  Sums are accumulated in FP64 and rounded once.
+// Matrices whose planes exceed LDS run the same kernels (GL = true) with the planes in a global
+// workspace, one region per matrix, served by L2: the same arithmetic in the same order.
 #include <float.h>
 
 #include "common.hpp"
@@ -45,6 +47,8 @@ struct CeighParams {
   int32_t *info;
   int n, max_sweeps;
   double tol;
+  double *gws;       // GL: the per-matrix regions (gstride doubles each) that replace LDS
+  int64_t gstride;
 };
 
 // One 2x2 block (rows p1, q1; columns p2, q2) of G <- J1^H G J2: columns with pair 2's rotation, then
@@ -88,7 +92,7 @@ __device__ __forceinline__ double cblock_update(R *Gr, R *Gi, int ld, int p1, in
   return fmax(fmax(mod(n11r, n11i), mod(n12r, n12i)), fmax(mod(n21r, n21i), mod(n22r, n22i)));
 }
 
-template <typename R>
+template <typename R, bool GL>
 __global__ __launch_bounds__(kCsmThreads) void ceigh_lds_kernel(CeighParams<R> P) {
   constexpr int NT = kCsmThreads, NW = NT / kWave;
   extern __shared__ double smem_ce[];
@@ -97,7 +101,7 @@ __global__ __launch_bounds__(kCsmThreads) void ceigh_lds_kernel(CeighParams<R> P
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
   const int n = P.n, np = n + (n & 1), ld = ceigh_ld(np), npairs = np / 2;
   const int64_t b = blockIdx.x;
-  double *cc = smem_ce, *sr = cc + npairs, *si = sr + npairs, *dw = si + npairs;
+  double *cc = GL ? P.gws + b * P.gstride : smem_ce, *sr = cc + npairs, *si = sr + npairs, *dw = si + npairs;
   int *pp = reinterpret_cast<int *>(dw + np), *qq = pp + npairs, *ord = qq + npairs;
   R *Gr = reinterpret_cast<R *>(ord + np + (np & 1));
   R *Gi = Gr + np * ld, *Vr = Gi + np * ld, *Vi = Vr + np * ld;
@@ -222,15 +226,16 @@ __host__ __device__ inline size_t cchol_lds_bytes(int n) {
   return sizeof(R) * 2 * (size_t)n * (n + 1) + 16;
 }
 
-template <typename R>
-__global__ __launch_bounds__(kCsmThreads) void cchol_lds_kernel(const R *G, R *L, int32_t *fail, int n) {
+template <typename R, bool GL>
+__global__ __launch_bounds__(kCsmThreads) void cchol_lds_kernel(const R *G, R *L, int32_t *fail, int n, double *gws,
+                                                               int64_t gstride) {
   constexpr int NT = kCsmThreads, NW = NT / kWave;
   extern __shared__ double smem_cc[];
   __shared__ double s_red[NW];
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
   const int ld = n + 1;
   const int64_t b = blockIdx.x;
-  R *Lr = reinterpret_cast<R *>(smem_cc), *Li = Lr + n * ld;
+  R *Lr = reinterpret_cast<R *>(GL ? gws + b * gstride : smem_cc), *Li = Lr + (size_t)n * ld;
   const R *Gb = G + 2 * b * (int64_t)n * n;
   for (int e = tid; e < n * n; e += NT) {   // the lower triangle of G; L's columns overwrite it in place
     const int i = e / n, j = e - i * n;
@@ -285,13 +290,15 @@ __host__ __device__ inline size_t csolve_lds_bytes(int m, int n) {
   return sizeof(R) * 2 * ((size_t)n * (n + 1) + (size_t)m * (n + 1)) + 16;
 }
 
-template <typename R>
-__global__ __launch_bounds__(kCsmThreads) void csolve_kernel(const R *L, R *X, int m, int n) {
+template <typename R, bool GL>
+__global__ __launch_bounds__(kCsmThreads) void csolve_kernel(const R *L, R *X, int m, int n, double *gws,
+                                                            int64_t gstride) {
   constexpr int NT = kCsmThreads;
   extern __shared__ double smem_cs[];
   const int tid = threadIdx.x, ld = n + 1;
   const int64_t b = blockIdx.x;
-  R *Lr = reinterpret_cast<R *>(smem_cs), *Li = Lr + n * ld, *Xr = Li + n * ld, *Xi = Xr + m * ld;
+  R *Lr = reinterpret_cast<R *>(GL ? gws + b * gstride : smem_cs), *Li = Lr + (size_t)n * ld;
+  R *Xr = Li + (size_t)n * ld, *Xi = Xr + (size_t)m * ld;
   const R *Lb = L + 2 * b * (int64_t)n * n;
   R *Xb = X + 2 * b * (int64_t)m * n;
   for (int e = tid; e < n * n; e += NT) {
@@ -351,6 +358,63 @@ static int csm_attr(K kern, size_t bytes) {
   return PSVD_OK;
 }
 
+// Per-matrix region of the global-workspace variants, in doubles (256-byte aligned).
+static int64_t csm_gstride(size_t bytes) { return (int64_t)((bytes + 255) / 256 * 32); }
+
+// Workspace the kernels need for a batch: none when a matrix's planes fit in LDS.
+static size_t csm_ws(int64_t batch, size_t bytes) {
+  return bytes > kCsmLdsCap ? (size_t)(batch * csm_gstride(bytes)) * sizeof(double) : 0;
+}
+
+// The launch of kernel `lds` with `bytes` of dynamic LDS, or of `gl` over the workspace when the
+// planes exceed LDS (without a workspace that is PSVD_ETOOBIG).  `launch(kernel, smem, gws, gstride)`
+// issues it.
+template <typename KL, typename KG, typename F>
+static int csm_launch(KL lds, KG gl, size_t bytes, int64_t batch, void *ws, size_t ws_bytes, F launch) {
+  if (bytes > kCsmLdsCap) {
+    if (!ws) return PSVD_ETOOBIG;
+    if (ws_bytes < csm_ws(batch, bytes)) return PSVD_EINVAL;
+    launch(gl, size_t(0), static_cast<double *>(ws), csm_gstride(bytes));
+  } else {
+    const int rc = csm_attr(lds, bytes);
+    if (rc != PSVD_OK) return rc;
+    launch(lds, bytes, static_cast<double *>(nullptr), int64_t(0));
+  }
+  return hipGetLastError() == hipSuccess ? PSVD_OK : PSVD_ELAUNCH;
+}
+
+template <typename R>
+static int run_ceigh(int64_t batch, int n, const void *h, void *w, void *v, int32_t *info, double tol, int max_sweeps,
+                     void *ws, size_t ws_bytes, hipStream_t st) {
+  CeighParams<R> P{static_cast<const R *>(h), static_cast<R *>(w), static_cast<R *>(v), info, n, max_sweeps, tol,
+                   nullptr, 0};
+  return csm_launch(ceigh_lds_kernel<R, false>, ceigh_lds_kernel<R, true>, ceigh_lds_bytes<R>(n), batch, ws, ws_bytes,
+                    [&](auto kern, size_t smem, double *gws, int64_t gs) {
+                      P.gws = gws;
+                      P.gstride = gs;
+                      hipLaunchKernelGGL(kern, dim3((unsigned)batch), dim3(kCsmThreads), smem, st, P);
+                    });
+}
+
+template <typename R>
+static int run_cchol(int64_t batch, int n, const void *g, void *l, int32_t *fail, void *ws, size_t ws_bytes,
+                     hipStream_t st) {
+  return csm_launch(cchol_lds_kernel<R, false>, cchol_lds_kernel<R, true>, cchol_lds_bytes<R>(n), batch, ws, ws_bytes,
+                    [&](auto kern, size_t smem, double *gws, int64_t gs) {
+                      hipLaunchKernelGGL(kern, dim3((unsigned)batch), dim3(kCsmThreads), smem, st,
+                                         static_cast<const R *>(g), static_cast<R *>(l), fail, n, gws, gs);
+                    });
+}
+
+template <typename R>
+static int run_csolve(int64_t batch, int m, int n, const void *l, void *x, void *ws, size_t ws_bytes, hipStream_t st) {
+  return csm_launch(csolve_kernel<R, false>, csolve_kernel<R, true>, csolve_lds_bytes<R>(m, n), batch, ws, ws_bytes,
+                    [&](auto kern, size_t smem, double *gws, int64_t gs) {
+                      hipLaunchKernelGGL(kern, dim3((unsigned)batch), dim3(kCsmThreads), smem, st,
+                                         static_cast<const R *>(l), static_cast<R *>(x), m, n, gws, gs);
+                    });
+}
+
 }  // namespace psvd
 
 extern "C" {
@@ -363,51 +427,49 @@ int psvd_ceigh_lds_smem_bytes(int dtype, int n, size_t *bytes) {
   return *bytes > psvd::kCsmLdsCap ? PSVD_ETOOBIG : PSVD_OK;
 }
 
-int psvd_ceigh_lds(int dtype, int64_t batch, int n, const void *h, void *w, void *v, int32_t *info, double tol,
-                   int max_sweeps, psvd_stream_t stream) {
+size_t psvd_ceigh_workspace_bytes(int dtype, int64_t batch, int n) {
+  if (batch < 0 || n < 1) return 0;
+  if (dtype == PSVD_C64) return psvd::csm_ws(batch, psvd::ceigh_lds_bytes<float>(n));
+  if (dtype == PSVD_C128) return psvd::csm_ws(batch, psvd::ceigh_lds_bytes<double>(n));
+  return 0;
+}
+
+int psvd_ceigh(int dtype, int64_t batch, int n, const void *h, void *w, void *v, int32_t *info, double tol,
+               int max_sweeps, void *workspace, size_t workspace_bytes, psvd_stream_t stream) {
   if (batch < 0 || batch > 0x7fffffff || n < 1 || !h || !w || !v || !info || max_sweeps < 1) return PSVD_EINVAL;
   if (batch == 0) return PSVD_OK;
   hipStream_t st = static_cast<hipStream_t>(stream);
-  if (dtype == PSVD_C64) {
-    const size_t bytes = psvd::ceigh_lds_bytes<float>(n);
-    const int rc = psvd::csm_attr(psvd::ceigh_lds_kernel<float>, bytes);
-    if (rc != PSVD_OK) return rc;
-    psvd::CeighParams<float> P{static_cast<const float *>(h), static_cast<float *>(w), static_cast<float *>(v), info,
-                               n, max_sweeps, tol};
-    hipLaunchKernelGGL(psvd::ceigh_lds_kernel<float>, dim3((unsigned)batch), dim3(psvd::kCsmThreads), bytes, st, P);
-  } else if (dtype == PSVD_C128) {
-    const size_t bytes = psvd::ceigh_lds_bytes<double>(n);
-    const int rc = psvd::csm_attr(psvd::ceigh_lds_kernel<double>, bytes);
-    if (rc != PSVD_OK) return rc;
-    psvd::CeighParams<double> P{static_cast<const double *>(h), static_cast<double *>(w), static_cast<double *>(v),
-                                info, n, max_sweeps, tol};
-    hipLaunchKernelGGL(psvd::ceigh_lds_kernel<double>, dim3((unsigned)batch), dim3(psvd::kCsmThreads), bytes, st, P);
-  } else {
-    return PSVD_EINVAL;
-  }
-  return hipGetLastError() == hipSuccess ? PSVD_OK : PSVD_ELAUNCH;
+  if (dtype == PSVD_C64)
+    return psvd::run_ceigh<float>(batch, n, h, w, v, info, tol, max_sweeps, workspace, workspace_bytes, st);
+  if (dtype == PSVD_C128)
+    return psvd::run_ceigh<double>(batch, n, h, w, v, info, tol, max_sweeps, workspace, workspace_bytes, st);
+  return PSVD_EINVAL;
 }
 
-int psvd_ccholesky_lds(int dtype, int64_t batch, int n, const void *g, void *l, int32_t *fail, psvd_stream_t stream) {
+int psvd_ceigh_lds(int dtype, int64_t batch, int n, const void *h, void *w, void *v, int32_t *info, double tol,
+                   int max_sweeps, psvd_stream_t stream) {
+  return psvd_ceigh(dtype, batch, n, h, w, v, info, tol, max_sweeps, nullptr, 0, stream);
+}
+
+size_t psvd_ccholesky_workspace_bytes(int dtype, int64_t batch, int n) {
+  if (batch < 0 || n < 1) return 0;
+  if (dtype == PSVD_C64) return psvd::csm_ws(batch, psvd::cchol_lds_bytes<float>(n));
+  if (dtype == PSVD_C128) return psvd::csm_ws(batch, psvd::cchol_lds_bytes<double>(n));
+  return 0;
+}
+
+int psvd_ccholesky(int dtype, int64_t batch, int n, const void *g, void *l, int32_t *fail, void *workspace,
+                   size_t workspace_bytes, psvd_stream_t stream) {
   if (batch < 0 || batch > 0x7fffffff || n < 1 || !g || !l || !fail) return PSVD_EINVAL;
   if (batch == 0) return PSVD_OK;
   hipStream_t st = static_cast<hipStream_t>(stream);
-  if (dtype == PSVD_C64) {
-    const size_t bytes = psvd::cchol_lds_bytes<float>(n);
-    const int rc = psvd::csm_attr(psvd::cchol_lds_kernel<float>, bytes);
-    if (rc != PSVD_OK) return rc;
-    hipLaunchKernelGGL(psvd::cchol_lds_kernel<float>, dim3((unsigned)batch), dim3(psvd::kCsmThreads), bytes, st,
-                       static_cast<const float *>(g), static_cast<float *>(l), fail, n);
-  } else if (dtype == PSVD_C128) {
-    const size_t bytes = psvd::cchol_lds_bytes<double>(n);
-    const int rc = psvd::csm_attr(psvd::cchol_lds_kernel<double>, bytes);
-    if (rc != PSVD_OK) return rc;
-    hipLaunchKernelGGL(psvd::cchol_lds_kernel<double>, dim3((unsigned)batch), dim3(psvd::kCsmThreads), bytes, st,
-                       static_cast<const double *>(g), static_cast<double *>(l), fail, n);
-  } else {
-    return PSVD_EINVAL;
-  }
-  return hipGetLastError() == hipSuccess ? PSVD_OK : PSVD_ELAUNCH;
+  if (dtype == PSVD_C64) return psvd::run_cchol<float>(batch, n, g, l, fail, workspace, workspace_bytes, st);
+  if (dtype == PSVD_C128) return psvd::run_cchol<double>(batch, n, g, l, fail, workspace, workspace_bytes, st);
+  return PSVD_EINVAL;
+}
+
+int psvd_ccholesky_lds(int dtype, int64_t batch, int n, const void *g, void *l, int32_t *fail, psvd_stream_t stream) {
+  return psvd_ccholesky(dtype, batch, n, g, l, fail, nullptr, 0, stream);
 }
 
 int psvd_csolve_lds_smem_bytes(int dtype, int m, int n, size_t *bytes) {
@@ -418,26 +480,25 @@ int psvd_csolve_lds_smem_bytes(int dtype, int m, int n, size_t *bytes) {
   return *bytes > psvd::kCsmLdsCap ? PSVD_ETOOBIG : PSVD_OK;
 }
 
-int psvd_csolve_chol_lds(int dtype, int64_t batch, int m, int n, const void *l, void *x, psvd_stream_t stream) {
+size_t psvd_csolve_chol_workspace_bytes(int dtype, int64_t batch, int m, int n) {
+  if (batch < 0 || m < 1 || n < 1) return 0;
+  if (dtype == PSVD_C64) return psvd::csm_ws(batch, psvd::csolve_lds_bytes<float>(m, n));
+  if (dtype == PSVD_C128) return psvd::csm_ws(batch, psvd::csolve_lds_bytes<double>(m, n));
+  return 0;
+}
+
+int psvd_csolve_chol(int dtype, int64_t batch, int m, int n, const void *l, void *x, void *workspace,
+                     size_t workspace_bytes, psvd_stream_t stream) {
   if (batch < 0 || batch > 0x7fffffff || m < 1 || n < 1 || !l || !x) return PSVD_EINVAL;
   if (batch == 0) return PSVD_OK;
   hipStream_t st = static_cast<hipStream_t>(stream);
-  if (dtype == PSVD_C64) {
-    const size_t bytes = psvd::csolve_lds_bytes<float>(m, n);
-    const int rc = psvd::csm_attr(psvd::csolve_kernel<float>, bytes);
-    if (rc != PSVD_OK) return rc;
-    hipLaunchKernelGGL(psvd::csolve_kernel<float>, dim3((unsigned)batch), dim3(psvd::kCsmThreads), bytes, st,
-                       static_cast<const float *>(l), static_cast<float *>(x), m, n);
-  } else if (dtype == PSVD_C128) {
-    const size_t bytes = psvd::csolve_lds_bytes<double>(m, n);
-    const int rc = psvd::csm_attr(psvd::csolve_kernel<double>, bytes);
-    if (rc != PSVD_OK) return rc;
-    hipLaunchKernelGGL(psvd::csolve_kernel<double>, dim3((unsigned)batch), dim3(psvd::kCsmThreads), bytes, st,
-                       static_cast<const double *>(l), static_cast<double *>(x), m, n);
-  } else {
-    return PSVD_EINVAL;
-  }
-  return hipGetLastError() == hipSuccess ? PSVD_OK : PSVD_ELAUNCH;
+  if (dtype == PSVD_C64) return psvd::run_csolve<float>(batch, m, n, l, x, workspace, workspace_bytes, st);
+  if (dtype == PSVD_C128) return psvd::run_csolve<double>(batch, m, n, l, x, workspace, workspace_bytes, st);
+  return PSVD_EINVAL;
+}
+
+int psvd_csolve_chol_lds(int dtype, int64_t batch, int m, int n, const void *l, void *x, psvd_stream_t stream) {
+  return psvd_csolve_chol(dtype, batch, m, n, l, x, nullptr, 0, stream);
 }
 
 }  // extern "C"

@@ -10,7 +10,9 @@
 // trailing columns get A <- (I - conj(tau) v v^H) A (primitives.py:62-65, :79-83), one group of 16
 // lanes per column, with the v^H A sums in FP64.  The reflectors stay below the diagonal.  Q (thin,
 // m x n) is then accumulated backwards from [I; 0] with A <- (I - tau v v^H) A
-// (primitives.py:84-88); R is the upper triangle with the real diagonal beta.
+// (primitives.py:84-88); R is the upper triangle with the real diagonal beta.  Matrices whose
+// planes exceed LDS run the same kernel with the planes in a global workspace (one region per
+// matrix, served by L2): the same arithmetic and order, so the same bits.
 #include <float.h>
 
 #include "common.hpp"
@@ -60,12 +62,14 @@ __device__ __forceinline__ void cqr_dot(const R *xr, const R *xi, const R *vr, c
   wi = group_sum_dpp<kCqrGroup>(si);
 }
 
-template <typename R>
-__global__ __launch_bounds__(kCqrThreads) void cqr_lds_kernel(const R *a, R *q, R *r, int m, int n) {
+template <typename R, bool GL>
+__global__ __launch_bounds__(kCqrThreads) void cqr_lds_kernel(const R *a, R *q, R *r, int m, int n, double *gws,
+                                                             int64_t gstride) {
   extern __shared__ double smem_cq[];
   __shared__ double red[kCqrThreads / kWave];
   const int ld = cqr_ld(m);
-  double *taur = smem_cq, *taui = taur + n;
+  double *base = GL ? gws + (int64_t)blockIdx.x * gstride : smem_cq;
+  double *taur = base, *taui = taur + n;
   R *Ar = reinterpret_cast<R *>(taui + n), *Ai = Ar + (size_t)n * ld;
   R *Qr = Ai + (size_t)n * ld, *Qi = Qr + (size_t)n * ld;
   constexpr int NG = kCqrThreads / kCqrGroup;   // column groups in the workgroup
@@ -307,19 +311,33 @@ __global__ __launch_bounds__(kCcThreads) void ccomplete_kernel(R *u, int m, int 
   if (tid == 0) status[b] = st;
 }
 
+// Per-matrix region of the global-workspace variant (doubles, 256-byte aligned).
 template <typename R>
-static int run_cqr(int64_t batch, int m, int n, const void *a, void *q, void *r, hipStream_t st) {
+static int64_t cqr_gstride(int m, int n) {
+  return (int64_t)((cqr_lds_bytes<R>(m, n) + 255) / 256 * 32);
+}
+
+template <typename R>
+static int run_cqr(int64_t batch, int m, int n, const void *a, void *q, void *r, void *ws, size_t ws_bytes,
+                   hipStream_t st) {
   const size_t smem = cqr_lds_bytes<R>(m, n);
-  if (smem > kCqrLdsCap) return PSVD_ETOOBIG;
+  if (smem > kCqrLdsCap) {   // planes in the workspace
+    const int64_t gs = cqr_gstride<R>(m, n);
+    if (!ws || ws_bytes < (size_t)(batch * gs) * sizeof(double)) return ws ? PSVD_EINVAL : PSVD_ETOOBIG;
+    hipLaunchKernelGGL((cqr_lds_kernel<R, true>), dim3((unsigned)batch), dim3(kCqrThreads), 0, st,
+                       static_cast<const R *>(a), static_cast<R *>(q), static_cast<R *>(r), m, n,
+                       static_cast<double *>(ws), gs);
+    return hipGetLastError() == hipSuccess ? PSVD_OK : PSVD_ELAUNCH;
+  }
   static bool attr = false;
   if (!attr) {
-    if (hipFuncSetAttribute((const void *)cqr_lds_kernel<R>, hipFuncAttributeMaxDynamicSharedMemorySize,
+    if (hipFuncSetAttribute((const void *)cqr_lds_kernel<R, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)kCqrLdsCap) != hipSuccess)
       return PSVD_ELAUNCH;
     attr = true;
   }
-  hipLaunchKernelGGL(cqr_lds_kernel<R>, dim3((unsigned)batch), dim3(kCqrThreads), smem, st,
-                     static_cast<const R *>(a), static_cast<R *>(q), static_cast<R *>(r), m, n);
+  hipLaunchKernelGGL((cqr_lds_kernel<R, false>), dim3((unsigned)batch), dim3(kCqrThreads), smem, st,
+                     static_cast<const R *>(a), static_cast<R *>(q), static_cast<R *>(r), m, n, nullptr, int64_t(0));
   return hipGetLastError() == hipSuccess ? PSVD_OK : PSVD_ELAUNCH;
 }
 
@@ -341,8 +359,30 @@ int psvd_cqr_lds(int dtype, int64_t batch, int m, int n, const void *a, void *q,
   if (batch < 0 || batch > 0x7fffffff || m < 1 || n < 1 || n > m || !a || !q || !r) return PSVD_EINVAL;
   if (batch == 0) return PSVD_OK;
   hipStream_t st = static_cast<hipStream_t>(stream);
-  if (dtype == PSVD_C64) return psvd::run_cqr<float>(batch, m, n, a, q, r, st);
-  if (dtype == PSVD_C128) return psvd::run_cqr<double>(batch, m, n, a, q, r, st);
+  if (dtype == PSVD_C64) return psvd::run_cqr<float>(batch, m, n, a, q, r, nullptr, 0, st);
+  if (dtype == PSVD_C128) return psvd::run_cqr<double>(batch, m, n, a, q, r, nullptr, 0, st);
+  return PSVD_EINVAL;
+}
+
+size_t psvd_cqr_workspace_bytes(int dtype, int64_t batch, int m, int n) {
+  if (batch < 0 || m < 1 || n < 1 || n > m) return 0;
+  if (dtype == PSVD_C64)
+    return psvd::cqr_lds_bytes<float>(m, n) > psvd::kCqrLdsCap ? (size_t)(batch * psvd::cqr_gstride<float>(m, n)) * 8
+                                                               : 0;
+  if (dtype == PSVD_C128)
+    return psvd::cqr_lds_bytes<double>(m, n) > psvd::kCqrLdsCap
+               ? (size_t)(batch * psvd::cqr_gstride<double>(m, n)) * 8
+               : 0;
+  return 0;
+}
+
+int psvd_cqr(int dtype, int64_t batch, int m, int n, const void *a, void *q, void *r, void *workspace,
+             size_t workspace_bytes, psvd_stream_t stream) {
+  if (batch < 0 || batch > 0x7fffffff || m < 1 || n < 1 || n > m || !a || !q || !r) return PSVD_EINVAL;
+  if (batch == 0) return PSVD_OK;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (dtype == PSVD_C64) return psvd::run_cqr<float>(batch, m, n, a, q, r, workspace, workspace_bytes, st);
+  if (dtype == PSVD_C128) return psvd::run_cqr<double>(batch, m, n, a, q, r, workspace, workspace_bytes, st);
   return PSVD_EINVAL;
 }
 

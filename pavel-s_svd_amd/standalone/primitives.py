@@ -152,8 +152,8 @@ def cqr_fits(dtype, m, n):
 
 
 def cqr(A):
-    """Householder QR of the tall complex batch A (b, m, n) with each matrix in LDS: Q (b, m, n),
-    R (b, n, n) (primitives.py:41-89, complex branch)."""
+    """Householder QR of the tall complex batch A (b, m, n): Q (b, m, n), R (b, n, n)
+    (primitives.py:41-89, complex branch), each matrix in LDS or, beyond it, in a workspace region."""
     b, m, n = A.shape
     if m < n:
         raise ValueError('qr takes tall matrices (m >= n)')
@@ -161,7 +161,10 @@ def cqr(A):
     code = N.dtype_code(A.dtype)
     q = torch.empty(b, m, n, dtype=A.dtype, device=A.device)
     r = torch.empty(b, n, n, dtype=A.dtype, device=A.device)
-    N.check(N._hip.cqr_lds(code, b, m, n, N.ptr(A), N.ptr(q), N.ptr(r), N.stream_of(A)), 'psvd_cqr_lds')
+    wsb = N._hip.cqr_workspace_bytes(code, b, m, n)
+    ws = N.workspace(wsb, A.device) if wsb else None
+    N.check(N._hip.cqr(code, b, m, n, N.ptr(A), N.ptr(q), N.ptr(r), N.ptr(ws) if wsb else 0, wsb, N.stream_of(A)),
+            'psvd_cqr')
     return q, r
 
 
@@ -195,9 +198,6 @@ def qr(A, *, gemm='f32', want_q=True):
     if m < n:
         raise ValueError('qr takes tall matrices (m >= n)')
     if A.is_complex():
-        if not cqr_fits(A.dtype, m, n):
-            from .solvers import Unsupported
-            raise Unsupported('complex QR is built for matrices that fit in LDS only')
         q, r = cqr(A)
         return (q if want_q else None), r
     code = N.dtype_code(A.dtype)
@@ -301,7 +301,8 @@ def jacobi_eigh_raw(H, tol=None, max_sweeps=30, *, method='auto', inner_sweeps=1
     method 'cyclic' keeps the reference's rotation order (LDS kernel, or the per-round HBM path
     for large n); 'blocked' groups rotations into 64 x 64 subproblems applied with MFMA; 'auto'
     takes the LDS kernel when the matrix fits and the blocked path otherwise.  Complex H takes the
-    complex LDS kernel (the reference's complex rotation, primitives.py:398-417) and must fit it."""
+    complex two-sided kernel (the reference's complex rotation, primitives.py:398-417), in LDS or,
+    beyond it, in a workspace region."""
     b, n, _ = H.shape
     H = H.contiguous()
     code = N.dtype_code(H.dtype)
@@ -309,9 +310,11 @@ def jacobi_eigh_raw(H, tol=None, max_sweeps=30, *, method='auto', inner_sweeps=1
         w = torch.empty(b, n, dtype=H.real.dtype, device=H.device)
         v = torch.empty(b, n, n, dtype=H.dtype, device=H.device)
         info = torch.empty(b, 2, dtype=torch.int32, device=H.device)
-        st = N._hip.ceigh_lds(code, b, n, N.ptr(H), N.ptr(w), N.ptr(v), N.ptr(info), float(tol or 0.0),
-                              int(max_sweeps), N.stream_of(H))
-        N.check(st, 'psvd_ceigh_lds')
+        wsb = N._hip.ceigh_workspace_bytes(code, b, n)
+        ws = N.workspace(wsb, H.device) if wsb else None
+        st = N._hip.ceigh(code, b, n, N.ptr(H), N.ptr(w), N.ptr(v), N.ptr(info), float(tol or 0.0),
+                          int(max_sweeps), N.ptr(ws) if wsb else 0, wsb, N.stream_of(H))
+        N.check(st, 'psvd_ceigh')
         return w, v, info
     if method == 'auto':
         method = 'cyclic' if eigh_lds_fits(H.dtype, n) else 'blocked'
@@ -346,14 +349,18 @@ def jacobi_eigh(H, tol=None, max_sweeps=30, *, method='auto', inner_sweeps=1):
 
 def cholesky(G, *, gemm='f32'):
     """G (b, n, n) symmetric (complex: Hermitian) -> L lower (G = L L^H), fail (b,) bool
-    (primitives.py:145-159).  Complex G is factored in LDS (one workgroup per matrix)."""
+    (primitives.py:145-159).  Complex G is factored by one workgroup per matrix, in LDS or, beyond
+    it, in a workspace region."""
     b, n, _ = G.shape
     if G.is_complex():
         G = G.contiguous()
         L = torch.empty_like(G)
         fail = torch.empty(b, dtype=torch.int32, device=G.device)
-        N.check(N._hip.ccholesky_lds(N.dtype_code(G.dtype), b, n, N.ptr(G), N.ptr(L), N.ptr(fail),
-                                     N.stream_of(G)), 'psvd_ccholesky_lds')
+        code = N.dtype_code(G.dtype)
+        wsb = N._hip.ccholesky_workspace_bytes(code, b, n)
+        ws = N.workspace(wsb, G.device) if wsb else None
+        N.check(N._hip.ccholesky(code, b, n, N.ptr(G), N.ptr(L), N.ptr(fail), N.ptr(ws) if wsb else 0, wsb,
+                                 N.stream_of(G)), 'psvd_ccholesky')
         return L, fail != 0
     L = G.contiguous().clone()
     fail = torch.empty(b, dtype=torch.int32, device=G.device)
@@ -389,8 +396,11 @@ def csolve_chol_(X, L):
     b, m, n = X.shape
     if not (X.is_contiguous() and L.is_contiguous()):
         raise ValueError('csolve_chol_ takes contiguous operands')
-    N.check(N._hip.csolve_chol_lds(N.dtype_code(X.dtype), b, m, n, N.ptr(L), N.ptr(X), N.stream_of(X)),
-            'psvd_csolve_chol_lds')
+    code = N.dtype_code(X.dtype)
+    wsb = N._hip.csolve_chol_workspace_bytes(code, b, m, n)
+    ws = N.workspace(wsb, X.device) if wsb else None
+    N.check(N._hip.csolve_chol(code, b, m, n, N.ptr(L), N.ptr(X), N.ptr(ws) if wsb else 0, wsb, N.stream_of(X)),
+            'psvd_csolve_chol')
     return X
 
 

@@ -227,22 +227,18 @@ def _gram(x, stats, opts):
 
 
 def _gram_complex(x, stats, opts):
-    """Complex gram_jacobi (solvers.py:99-119) for n x n Gram matrices that fit in LDS: G = sym(A^H A),
-    its eigenpairs from the complex two-sided Jacobi (the reference's jacobi_eigh, flipped to
-    descending; beyond that kernel's LDS budget the one-sided Jacobi of G, whose singular pairs are its
-    eigenpairs), sigma = sqrt(w) above eps * n * w_1, U = A V / sigma with the cut columns completed."""
+    """Complex gram_jacobi (solvers.py:99-119): G = sym(A^H A), its eigenpairs from the complex
+    two-sided Jacobi (the reference's jacobi_eigh, flipped to descending), sigma = sqrt(w) above
+    eps * n * w_1, U = A V / sigma with the cut columns completed."""
     b, m, n = x.shape
     real = torch.float32 if x.dtype == torch.complex64 else torch.float64
     g = P.cmatmul(x, x, ha=True)
     P.hermitize_(g)
     from .. import native as N
     max_sweeps = opts.get('max_sweeps', 60)
-    if P.ceigh_fits(x.dtype, n):   # the reference's two-sided Jacobi (solvers.py:107), eigenvalues flipped
-        w, v, info = P.jacobi_eigh_raw(g, max_sweeps=max_sweeps)
-        w = w.flip(-1)
-        vh = v.flip(-1).mH.contiguous()
-    else:
-        _, w, vh, info = P.jacobi_lds_uniform(g, tol=None, max_sweeps=max_sweeps, cut='floor')
+    w, v, info = P.jacobi_eigh_raw(g, max_sweeps=max_sweeps)   # solvers.py:107, eigenvalues flipped
+    w = w.flip(-1)
+    vh = v.flip(-1).mH.contiguous()
     info = info.cpu()
     N.raise_for_info(info, f"Hermitian Jacobi did not converge in {max_sweeps} sweeps")
     stats['eigen_sweeps'] = int(info[:, 0].max())
@@ -329,16 +325,6 @@ def _partial(x, stats, opts, method):
     return mm(q, u)[:, :, :k], s[:, :k], vh[:, :k, :]
 
 
-def _complex_partial_fits(dtype, m, n, opts, method):
-    """The QRs of a complex range finder (m x l sketches and ranges, n x l power steps, the joined
-    block Krylov basis) and the Jacobi of Q^H A fit in LDS."""
-    k = opts.get('rank', min(m, n))
-    ell = min(k + opts.get('oversample', 8), min(m, n))
-    cols = min(m, ell * (opts.get('iterations', 2) + 1)) if method == 'block_krylov' else ell
-    return (P.cqr_fits(dtype, m, ell) and P.cqr_fits(dtype, n, ell) and P.cqr_fits(dtype, m, cols) and
-            P.lds_fits(dtype, max(cols, n), min(cols, n), True))
-
-
 def _gmode(x, opts):
     """GEMM arithmetic for FP32 data ('f32' or 'bf16x3' from opts['gemm']); FP64 data uses FP64."""
     return opts.get('gemm', 'f32') if x.dtype == torch.float32 else 'f64'
@@ -357,14 +343,6 @@ def tsqr_leaf_bounds(m, n, rows):
             start = bounds.pop()[0]
         bounds.append((start, end))
     return bounds
-
-
-def _complex_tsqr_fits(dtype, m, n, rows):
-    """Every QR of a complex TSQR (its leaves and the [R1; R2] merges) and the Jacobi of R fit in LDS."""
-    rows = 4 * n if rows is None else rows
-    bounds = tsqr_leaf_bounds(m, n, rows) or [(0, m)]
-    heights = {e - s for s, e in bounds} | ({2 * n} if len(bounds) > 1 else set())
-    return all(P.cqr_fits(dtype, h, n) for h in heights) and P.lds_fits(dtype, n, n, True)
 
 
 def _tsqr(x, rows, stats, opts):
@@ -603,16 +581,15 @@ def _polar(a, stats, opts):
 
 
 def _polar_complex(a, stats, opts):
-    """QDWH polar SVD of a complex tall batch held to LDS sizes (solvers.py:122-178 with the complex
-    arithmetic of primitives.py): X = A / ||A||_F and dynamically weighted Halley steps with the
+    """QDWH polar SVD of a complex tall batch (solvers.py:122-178 with the complex arithmetic of
+    primitives.py): X = A / ||A||_F and dynamically weighted Halley steps with the
     reference's branch rule -- while c > 100, X <- (b/c) X + (a - b/c) / sqrt(c) Q1 Q2^H from the QR
     of [sqrt(c) X; I]; afterwards Z = I + c X^H X, L = chol(sym Z) (RuntimeError when it loses positive
     definiteness) and X <- (b/c) X + (a - b/c) X Z^-1 by the two triangular solves -- stopping on
     max|X_new - X| <= tol max(max|X|, 1) over the batch; then H = sym(X^H A) and its eigenpairs from
     the complex two-sided Jacobi (primitives.py:370-427), sigma the flipped eigenvalues clamped at 0,
-    U = X V with the cut columns completed.  Shapes whose Cholesky step or eigensolve do not fit in
-    LDS take the QR form for every step and the one-sided Jacobi of H (H is positive semidefinite, so
-    its singular pairs are its eigenpairs)."""
+    U = X V with the cut columns completed.  The QR, Cholesky, solve and eigensolver kernels hold each
+    matrix in LDS or, beyond it, in a workspace region."""
     b, m, n = a.shape
     real = torch.float32 if a.dtype == torch.complex64 else torch.float64
     x = P.scale(a, P.reduce(a, 'fro'), invert=True)
@@ -623,11 +600,10 @@ def _polar_complex(a, stats, opts):
     max_iter = opts.get('max_iter', 40)
     ident = torch.zeros(b, n, n, dtype=a.dtype, device=a.device)
     ident.diagonal(dim1=-2, dim2=-1).fill_(1)
-    chol_form = P.csolve_fits(a.dtype, m, n)
     converged, nqr = False, 0
     for it in range(max_iter):
         aa, bb, cc, lower = qdwh_weights(lower)
-        if cc > 100 or not chol_form:
+        if cc > 100:
             nqr += 1
             q, _ = P.cqr(torch.cat([P.scale(x, None, s=math.sqrt(cc)), ident], -2))
             xn = P.cmatmul(q[:, :m], q[:, m:], hb=True)
@@ -651,16 +627,11 @@ def _polar_complex(a, stats, opts):
     P.hermitize_(h)
     from .. import native as N
     max_sweeps = opts.get('max_sweeps', 60)
-    if P.ceigh_fits(a.dtype, n):
-        w, v, info = P.jacobi_eigh_raw(h, max_sweeps=max_sweeps)
-        info = info.cpu()
-        N.raise_for_info(info, f'Hermitian Jacobi did not converge in {max_sweeps} sweeps')
-        w = w.flip(-1).clamp_min(0)
-        vh = v.flip(-1).mH.contiguous()
-    else:
-        _, w, vh, info = P.jacobi_lds_uniform(h, tol=None, max_sweeps=max_sweeps, cut='floor')
-        info = info.cpu()
-        N.raise_for_info(info, f'Hermitian Jacobi did not converge in {max_sweeps} sweeps')
+    w, v, info = P.jacobi_eigh_raw(h, max_sweeps=max_sweeps)
+    info = info.cpu()
+    N.raise_for_info(info, f'Hermitian Jacobi did not converge in {max_sweeps} sweeps')
+    w = w.flip(-1).clamp_min(0)
+    vh = v.flip(-1).mH.contiguous()
     keep = w > EPS[real] * max(m, n) * w[:, :1]
     u = P.complete_null_columns(P.cmatmul(x, vh, hb=True), keep)
     w = torch.where(keep, w, torch.zeros_like(w))
@@ -753,22 +724,6 @@ def svd(a, algorithm='jacobi', *, full_matrices=False, values_only=False, return
     if a.is_complex() and algorithm == 'block_jacobi' and not P.lds_fits(
             a.dtype, max(a.shape[-2:]), min(min(a.shape[-2:]), 2 * max(1, opts.get('block_size', 4))), True):
         raise Unsupported('complex block_jacobi is built for group pairs whose columns fit in LDS only')
-    if a.is_complex() and algorithm in PARTIAL and not _complex_partial_fits(a.dtype, a.shape[-2], a.shape[-1],
-                                                                              opts, algorithm):
-        raise Unsupported(f'complex {algorithm} is built for sketches and ranges that fit in LDS only')
-    if a.is_complex() and algorithm == 'gram_jacobi' and not P.lds_fits(a.dtype, min(a.shape[-2:]),
-                                                                        min(a.shape[-2:]), True):
-        raise Unsupported('complex gram_jacobi is built for Gram matrices that fit in LDS only')
-    if a.is_complex() and algorithm == 'polar_qdwh' and not (
-            P.cqr_fits(a.dtype, sum(a.shape[-2:]), min(a.shape[-2:])) and
-            P.lds_fits(a.dtype, min(a.shape[-2:]), min(a.shape[-2:]), True)):
-        raise Unsupported('complex polar_qdwh is built for matrices whose [sqrt(c) X; I] fits in LDS only')
-    if a.is_complex() and algorithm == 'tsqr' and not _complex_tsqr_fits(a.dtype, max(a.shape[-2:]),
-                                                                          min(a.shape[-2:]), opts.get('tile_rows')):
-        raise Unsupported('complex TSQR is built for leaves that fit in LDS only')
-    if a.is_complex() and algorithm == 'qr' and not (P.cqr_fits(a.dtype, max(a.shape[-2:]), min(a.shape[-2:])) and
-                                                     P.lds_fits(a.dtype, min(a.shape[-2:]), min(a.shape[-2:]), True)):
-        raise Unsupported('complex QR is built for matrices that fit in LDS only')
     if a.device.type != 'cuda':
         raise Unsupported('the MI355X solvers take ROCm device tensors; there is no CPU path')
     if algorithm != 'jacobi' and x.shape[0] > MAX_LAUNCH_BATCH:

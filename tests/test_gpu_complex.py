@@ -367,10 +367,6 @@ def test_complex_range_finders_match_oracle_on_the_same_sketch(pkg, expected, me
 
 def test_complex_unsupported_paths(pkg):
     x = torch.zeros(1, 20, 10, dtype=torch.complex64, device=DEV) + 1
-    with pytest.raises(pkg.Unsupported):   # complex QR beyond LDS
-        pkg.svd(torch.zeros(1, 600, 300, dtype=torch.complex128, device=DEV) + 1, 'qr')
-    with pytest.raises(pkg.Unsupported):   # complex TSQR leaves (256 x 64 complex128) beyond LDS
-        pkg.svd(torch.zeros(1, 600, 64, dtype=torch.complex128, device=DEV) + 1, 'tsqr')
     with pytest.raises(pkg.Unsupported):   # complex128 group pairs of 600 x 64 beyond LDS
         pkg.svd(torch.zeros(1, 600, 80, dtype=torch.complex128, device=DEV) + 1, 'block_jacobi', block_size=32)
     with pytest.raises(pkg.Unsupported):
@@ -399,13 +395,16 @@ def test_complex_full_matrices_and_rank_deficient(pkg, method, dtype):
 
 
 @pytest.mark.parametrize('dtype', [np.complex64, np.complex128])
-@pytest.mark.parametrize('b,n', [(3, 1), (2, 2), (4, 7), (2, 16), (1, 33), (2, 64)])
+@pytest.mark.parametrize('b,n', [(3, 1), (2, 2), (4, 7), (2, 16), (1, 33), (2, 64), (2, 120)])
 def test_complex_eigh_matches_lapack_and_oracle(dtype, b, n):
     """The complex two-sided Jacobi (primitives.py:370-427, complex branch): eigenvalues against
     LAPACK's zheevd, V unitary and V diag(w) V^H = H, sweep counts within one of the oracle's (the
-    reference's rotation order; FP64 sums change at most the last sweep)."""
+    reference's rotation order; FP64 sums change at most the last sweep).  n = 120 is beyond the LDS
+    budget (both dtypes): the same kernel on a workspace region."""
     from importlib import import_module
     P = import_module('pavel-s_svd_amd.standalone.primitives')
+    if n == 120:
+        assert not P.ceigh_fits(torch.complex64 if dtype == np.complex64 else torch.complex128, n)
     rng = np.random.default_rng(7 * n + b)
     h = rng.standard_normal((b, n, n)) + 1j * rng.standard_normal((b, n, n))
     h = ((h + np.conj(np.swapaxes(h, 1, 2))) / 2).astype(dtype)
@@ -425,14 +424,15 @@ def test_complex_eigh_matches_lapack_and_oracle(dtype, b, n):
 
 
 @pytest.mark.parametrize('dtype', [np.complex64, np.complex128])
-@pytest.mark.parametrize('b,m,n', [(3, 1, 1), (2, 20, 9), (2, 64, 48), (1, 96, 64)])
+@pytest.mark.parametrize('b,m,n', [(3, 1, 1), (2, 20, 9), (2, 64, 48), (1, 96, 64), (2, 300, 130)])
 def test_complex_cholesky_and_solve(dtype, b, m, n):
     """Complex Cholesky against LAPACK's zpotrf (lower), the reference's failure mask on an
-    indefinite matrix, and X (L L^H)^-1 against a direct solve (solvers.py:159-160)."""
+    indefinite matrix, and X (L L^H)^-1 against a direct solve (solvers.py:159-160); 300 x 130 is
+    beyond the LDS budget of both kernels (workspace regions)."""
     from importlib import import_module
     P = import_module('pavel-s_svd_amd.standalone.primitives')
-    if not P.csolve_fits(torch.complex64 if dtype == np.complex64 else torch.complex128, m, n):
-        pytest.skip('beyond the LDS budget of the complex solve')
+    if m == 300:
+        assert not P.csolve_fits(torch.complex64 if dtype == np.complex64 else torch.complex128, m, n)
     rng = np.random.default_rng(m + n)
     x = (rng.standard_normal((b, n, n)) + 1j * rng.standard_normal((b, n, n)))
     z = (np.eye(n) + 0.3 * np.conj(np.swapaxes(x, 1, 2)) @ x).astype(dtype)
@@ -489,3 +489,36 @@ def test_complex_jacobi_beyond_lds(pkg, dtype, shape, rank):
     assert np.abs(_np(s) - so).max() <= (bar + oracle_err) * so.max()
     if rank is None:
         assert abs(info['jacobi_sweeps'] - oinfo['jacobi_sweeps']) <= 1
+
+
+@pytest.mark.parametrize('method,opts', [('qr', {}), ('tsqr', dict(tile_rows=150)), ('polar_qdwh', {}),
+                                         ('gram_jacobi', {})])
+@pytest.mark.parametrize('dtype', [np.complex64, np.complex128])
+def test_complex_solvers_beyond_lds(pkg, method, opts, dtype):
+    """The complex solvers on a 400 x 140 matrix, whose QRs, QDWH stack, Cholesky factor, Gram and
+    eigen problems exceed LDS (the kernels run on workspace regions, the Jacobi of R on the complex
+    cooperative kernel): sigma, U, V against complex LAPACK at the complex bars (gram_jacobi: the kept
+    values above sqrt(eps n) sigma_1 at the Gram method's accuracy); QDWH iteration counts equal to the
+    oracle's restatement of the reference's solver on the same input."""
+    rng = np.random.default_rng(140)
+    m, n = 400, 140
+    a = (rng.standard_normal((1, m, n)) + 1j * rng.standard_normal((1, m, n))).astype(dtype)
+    from importlib import import_module
+    P = import_module('pavel-s_svd_amd.standalone.primitives')
+    tdt = torch.complex64 if dtype == np.complex64 else torch.complex128
+    assert not P.cqr_fits(tdt, m, n) and not P.ceigh_fits(tdt, n)
+    print(f'complex {method} beyond LDS {m}x{n} {np.dtype(dtype).name}', flush=True)
+    u, s, vh, info = pkg.svd(torch.from_numpy(a).to(DEV), method, return_info=True, **opts)
+    lap = np.linalg.svd(a[0].astype(np.complex128), compute_uv=False)
+    bar = 1e-5 if dtype == np.complex64 else 1e-12
+    if method == 'gram_jacobi':
+        keep = _np(s[0]) > 0
+        gbar = 2e-4 if dtype == np.complex64 else 1e-8
+        assert np.abs(_np(s[0])[keep] - lap[keep]).max() <= gbar * lap[0]
+        return
+    met = _metrics(a[0], (u[0], s[0], vh[0]))
+    for key in KEYS:
+        assert met[key] <= bar, (key, met[key])
+    if method == 'polar_qdwh':
+        _, _, _, oinfo = R.svd(a[0], 'polar_qdwh', return_info=True)
+        assert info['polar_iterations'] == oinfo['polar_iterations']
