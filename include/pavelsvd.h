@@ -55,6 +55,9 @@ typedef void *psvd_stream_t;
 #define PSVD_BLOCK_NARROW 32 /* psvd_jacobi_blocked: column blocks of 16 (32 x 32 subproblems; the
                                default when neither block flag is given) */
 #define PSVD_BLOCK_WIDE 64   /* psvd_jacobi_blocked: column blocks of 32 (64 x 64 subproblems) */
+#define PSVD_BLOCK_FULL_PAIRS 128 /* psvd_jacobi_blocked: every tournament round sweeps all pairs of its
+                               subproblems (by default only a sweep's first round does; the later
+                               rounds sweep the pairs across the two blocks) */
 
 /* GEMM arithmetic modes */
 #define PSVD_GEMM_F32 0     /* FP32 operands, FP32 MFMA, FP32 accumulation */

@@ -174,6 +174,21 @@ __device__ __forceinline__ void rr_pair(int r, int k, int np, int &p, int &q) {
   q = x < y ? y : x;
 }
 
+// Pair k of rotation round rr of a subproblem sweep over two blocks of KB indices (2 KB in all):
+// the full tournament (rr < 2 KB - 1), or a cross sweep -- index k of the first block against index
+// KB + (k + rr) mod KB of the second (rr < KB) -- which treats every pair between the blocks once and
+// none inside them.  A block-Jacobi sweep whose first tournament round sweeps its block pairs fully
+// and every later round only across them still treats every index pair once per sweep, as the
+// scalar cyclic method does, in about half the rotation rounds.
+__device__ __forceinline__ void sub_pair(bool cross, int rr, int k, int KB, int &a, int &b) {
+  if (cross) {
+    a = k;
+    b = KB + (k + rr) % KB;
+  } else {
+    rr_pair(rr, k, 2 * KB, a, b);
+  }
+}
+
 // Jacobi rotation for the 2x2 Gram block [[app, apq], [apq, aqq]] (primitives.py:340-347):
 // tau = (aqq - app) / (2|apq|), t = sign(tau) / (|tau| + hypot(tau, 1)) (t = 1 at tau = 0),
 // c = 1 / sqrt(1 + t^2), s = t c sign(apq).  Computed in FP64.

@@ -550,12 +550,16 @@ __device__ __forceinline__ void blk_inner_body(const BlkParams<T> &P, int p, int
   }
   constexpr int QSTEP = kBlkInner / kEs, QIT = (kEb + QSTEP - 1) / QSTEP;   // Qt row pairs per column
   const int qcol = tid % kEs, qk = tid / kEs;
+  // the sweep's first tournament round treats every pair of the subproblem, the others only the
+  // pairs across its two blocks (the pairs inside a block were treated in the first round)
+  const bool cross = r > 0;
+  const int nrr = cross ? kEb : kEs - 1;
   for (int sw = 0; live && sw < P.inner_sweeps; ++sw) {
     int act_sw = 0;
-    for (int rr = 0; rr < kEs - 1; ++rr) {
+    for (int rr = 0; rr < nrr; ++rr) {
       if (tid < kEb) {
         int a, b;
-        rr_pair(rr, tid, kEs, a, b);
+        sub_pair(cross, rr, tid, kEb, a, b);
         double c, s, ratio;
         bool act;
         pair_params(S, kEsLd, a, b, tol, scale, c, s, act, ratio, true);

@@ -42,6 +42,7 @@ struct BjParams {
   T *u, *s, *vh;
   int32_t *info;
   int m, n, N, nb, splits, inner_sweeps, flags;
+  int full_pairs;           // every round's pair sweeps over all pairs (else the first round's only)
   double tol;
   T *W, *V, *Q;             // W: m x N, V: N x N (row-major, ld N); Q: npair x 64 x 64 (W_IJ <- W_IJ Q)
   double *Sp;               // npair x splits x 64 x 64 Gram partials
@@ -241,12 +242,16 @@ __global__ __launch_bounds__(BjShape<KB>::INNER) void bj_inner_kernel(BjParams<T
   constexpr int QSTEP = NTI / KS, QIT = (KB + QSTEP - 1) / QSTEP;   // Qt row pairs per column
   static_assert(NTI % KS == 0, "the Qt items tile the columns exactly");
   const int qcol = tid % KS, qk = tid / KS;
+  // the sweep's first tournament round treats every pair of the subproblem, the others only the
+  // pairs across its two blocks (the pairs inside a block were treated in the first round)
+  const bool cross = r > 0 && !P.full_pairs;
+  const int nrr = cross ? KB : KS - 1;
   for (int sw = 0; live && sw < P.inner_sweeps; ++sw) {
     int act_sw = 0;
-    for (int rr = 0; rr < KS - 1; ++rr) {
+    for (int rr = 0; rr < nrr; ++rr) {
       if (tid < KB) {
         int a, b;
-        rr_pair(rr, tid, KS, a, b);
+        sub_pair(cross, rr, tid, KB, a, b);
         const double app = S[a * LD + a], aqq = S[b * LD + b], apq = S[a * LD + b];
         const bool act = apq * apq > tol * tol * app * aqq && fmin(app, aqq) > floor_;
         double c = 1.0, s = 0.0;
@@ -562,7 +567,8 @@ static int run_bj(int m, int n, const T *a, T *u, T *s, T *vh, int32_t *info, do
   const int npair = P.nb / 2;
   P.splits = bj_splits(m, npair);
   P.inner_sweeps = inner_sweeps;
-  P.flags = flags;
+  P.full_pairs = (flags & PSVD_BLOCK_FULL_PAIRS) != 0;
+  P.flags = flags & ~PSVD_BLOCK_FULL_PAIRS;
   P.tol = tol;
   char *cur = static_cast<char *>(ws);
   auto take = [&](size_t bytes) {
@@ -657,7 +663,8 @@ int psvd_jacobi_blocked(int dtype, int m, int n, const void *a, void *u, void *s
                         int max_sweeps, int flags, int inner_sweeps, void *workspace, size_t workspace_bytes,
                         psvd_stream_t stream) {
   if (m < 1 || n < 1 || n > m || !a || !s || !info || !workspace || max_sweeps < 1 || inner_sweeps < 1 ||
-      (flags & ~(PSVD_SCALE | PSVD_CUT_FLOOR | PSVD_NO_VECTORS | PSVD_BLOCK_NARROW | PSVD_BLOCK_WIDE)) ||
+      (flags & ~(PSVD_SCALE | PSVD_CUT_FLOOR | PSVD_NO_VECTORS | PSVD_BLOCK_NARROW | PSVD_BLOCK_WIDE |
+                 PSVD_BLOCK_FULL_PAIRS)) ||
       ((flags & PSVD_BLOCK_NARROW) && (flags & PSVD_BLOCK_WIDE)) || n > 65536)
     return PSVD_EINVAL;
   if (!(flags & PSVD_NO_VECTORS) && (!u || !vh)) return PSVD_EINVAL;

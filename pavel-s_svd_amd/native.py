@@ -18,6 +18,7 @@ OK, EINVAL, ELAUNCH, ETOOBIG = 0, 1, 2, 3
 INFO_OK, INFO_NOCONV, INFO_COMPLETION, INFO_NONFINITE, INFO_BADSHAPE = 0, 1, 2, 3, 4
 
 SCALE, CUT_FLOOR, NO_VECTORS, SLOT_SPREAD, ORDER_SORTED, BLOCK_NARROW, BLOCK_WIDE = 1, 2, 4, 8, 16, 32, 64
+BLOCK_FULL_PAIRS = 128
 
 _STATUS = {EINVAL: 'invalid argument', ELAUNCH: 'kernel launch failed', ETOOBIG: 'problem exceeds the kernel budget'}
 

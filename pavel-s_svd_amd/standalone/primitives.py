@@ -83,10 +83,12 @@ def jacobi_grid(x, *, tol=None, max_sweeps=60, scale=False, cut='reference', vec
 
 
 def jacobi_blocked(x, *, tol=None, max_sweeps=60, scale=False, cut='reference', vectors=True, inner_sweeps=1,
-                   block_width=16):
+                   block_width=16, full_pairs=False):
     """Blocked one-sided Jacobi SVD of each tall matrix of ``x`` (b, m, n): column blocks of
     ``block_width`` (16 or 32), FP64 Gram subproblems of twice that size and MFMA column updates,
-    one call per matrix.  Same outputs as ``jacobi_grid``."""
+    one call per matrix.  A sweep's first tournament round sweeps all pairs of its subproblems, the
+    later rounds the pairs across the two blocks (``full_pairs``: all pairs in every round).  Same
+    outputs as ``jacobi_grid``."""
     b, m, n = x.shape
     if m < n:
         raise ValueError('jacobi_blocked takes tall matrices (m >= n)')
@@ -96,6 +98,7 @@ def jacobi_blocked(x, *, tol=None, max_sweeps=60, scale=False, cut='reference', 
     code = N.dtype_code(x.dtype)
     flags = (N.SCALE if scale else 0) | (N.CUT_FLOOR if cut == 'floor' else 0) | (0 if vectors else N.NO_VECTORS)
     flags |= N.BLOCK_WIDE if block_width == 32 else N.BLOCK_NARROW
+    flags |= N.BLOCK_FULL_PAIRS if full_pairs else 0
     dev = x.device
     u = torch.empty_like(x) if vectors else None
     s = torch.empty(b, n, dtype=x.dtype, device=dev)
