@@ -28,9 +28,10 @@ namespace psvd {
 constexpr int kJbThreads = 256;
 
 // Block width KB: subproblem size KS, LDS row stride, threads of the pair-sweep kernel (one
-// upper-triangle 2 x 2 block of a rotation round per thread: 528 for KB = 32, 136 for KB = 16).
+// upper-triangle 2 x 2 block of a rotation round per thread: 528 for KB = 32, 136 for KB = 16; for
+// KB = 16 the 512 Qt entries of a round get threads of their own, 1024 in all).
 template <int KB> struct BjShape {
-  static constexpr int KS = 2 * KB, LD = KS + 1, INNER = KB == 32 ? 576 : 256;
+  static constexpr int KS = 2 * KB, LD = KS + 1, INNER = KB == 32 ? 576 : 1024;
 };
 
 typedef float j4f __attribute__((ext_vector_type(4)));
@@ -241,6 +242,9 @@ __global__ __launch_bounds__(BjShape<KB>::INNER) void bj_inner_kernel(BjParams<T
   }
   constexpr int QSTEP = NTI / KS, QIT = (KB + QSTEP - 1) / QSTEP;   // Qt row pairs per column
   static_assert(NTI % KS == 0, "the Qt items tile the columns exactly");
+  // blocks and Qt entries on separate threads when the workgroup holds both (one item per thread)
+  constexpr bool kSplitQt = NTI >= NTRI + KB * KS;
+  static_assert(!kSplitQt || NIT == 1, "one block per thread");
   const int qcol = tid % KS, qk = tid / KS;
   // the sweep's first tournament round treats every pair of the subproblem, the others only the
   // pairs across its two blocks (the pairs inside a block were treated in the first round)
@@ -257,19 +261,59 @@ __global__ __launch_bounds__(BjShape<KB>::INNER) void bj_inner_kernel(BjParams<T
         double c = 1.0, s = 0.0;
         if (act) jacobi_rotation_fast<T>(app, aqq, apq, c, s);
         cs[tid] = make_double2(c, act ? s : 0.0);
-        pq[tid] = make_int2(a, b);
+        if constexpr (!kSplitQt) pq[tid] = make_int2(a, b);
         act_sw |= act;
       }
       __syncthreads();
+      if constexpr (kSplitQt) {
+        // threads [0, NTRI): one 2 x 2 block each; threads [NTRI, NTRI + KB KS): one Qt entry pair
+        // each.  The pairs come from the closed form, so each thread's (c, s) and S or Qt loads are
+        // issued together: one LDS round trip before the arithmetic.
+        if (tid < NTRI) {
+          int p1, q1, p2, q2;
+          sub_pair(cross, rr, it1[0], KB, p1, q1);
+          sub_pair(cross, rr, it2[0], KB, p2, q2);
+          const double2 r1 = cs[it1[0]], r2 = cs[it2[0]];
+          const double b11 = S[p1 * LD + p2], b12 = S[p1 * LD + q2], b21 = S[q1 * LD + p2], b22 = S[q1 * LD + q2];
+          if (r1.y != 0.0 || r2.y != 0.0) {   // as block_update: columns (pair 2), then rows (pair 1)
+            const double c1 = r1.x, s1 = r1.y, c2 = r2.x, s2 = r2.y;
+            const double t11 = c2 * b11 - s2 * b12, t12 = s2 * b11 + c2 * b12;
+            const double t21 = c2 * b21 - s2 * b22, t22 = s2 * b21 + c2 * b22;
+            const double n11 = c1 * t11 - s1 * t21, n21 = s1 * t11 + c1 * t21;
+            const double n12 = c1 * t12 - s1 * t22, n22 = s1 * t12 + c1 * t22;
+            S[p1 * LD + p2] = n11;
+            S[p1 * LD + q2] = n12;
+            S[q1 * LD + p2] = n21;
+            S[q1 * LD + q2] = n22;
+            if (p1 != p2) {
+              S[p2 * LD + p1] = n11;
+              S[q2 * LD + p1] = n12;
+              S[p2 * LD + q1] = n21;
+              S[q2 * LD + q1] = n22;
+            }
+          }
+        } else if (tid - NTRI < KB * KS) {
+          const int e = tid - NTRI, k = e / KS, col = e - k * KS;
+          int pp, qq;
+          sub_pair(cross, rr, k, KB, pp, qq);
+          const double2 rq = cs[k];
+          const double x = Qt[pp * LD + col], y = Qt[qq * LD + col];
+          if (rq.y != 0.0) {
+            Qt[pp * LD + col] = rq.x * x - rq.y * y;
+            Qt[qq * LD + col] = rq.y * x + rq.x * y;
+          }
+        }
+      } else {
 #pragma unroll
-      for (int j = 0; j < NIT; ++j) {
-        if (it1[j] < 0) continue;
-        const double2 r1 = cs[it1[j]], r2 = cs[it2[j]];
-        if (r1.y == 0.0 && r2.y == 0.0) continue;
-        const int2 a1 = pq[it1[j]], a2 = pq[it2[j]];
-        block_update(S, LD, a1.x, a1.y, a2.x, a2.y, r1.x, r1.y, r2.x, r2.y, r1.y != 0.0, r2.y != 0.0);
+        for (int j = 0; j < NIT; ++j) {
+          if (it1[j] < 0) continue;
+          const double2 r1 = cs[it1[j]], r2 = cs[it2[j]];
+          if (r1.y == 0.0 && r2.y == 0.0) continue;
+          const int2 a1 = pq[it1[j]], a2 = pq[it2[j]];
+          block_update(S, LD, a1.x, a1.y, a2.x, a2.y, r1.x, r1.y, r2.x, r2.y, r1.y != 0.0, r2.y != 0.0);
+        }
+        rotate_rows_batched<double, KB, QSTEP, QIT>(Qt, LD, cs, pq, qk, qcol);
       }
-      rotate_rows_batched<double, KB, QSTEP, QIT>(Qt, LD, cs, pq, qk, qcol);
       __syncthreads();
     }
     act_sw = __syncthreads_or(act_sw);
