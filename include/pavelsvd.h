@@ -203,6 +203,15 @@ int psvd_qr(int dtype, int gemm_mode, int64_t batch, int m, int n, const void *a
  * order by the projected coordinate vector at the row of largest remaining leverage (two
  * Gram-Schmidt passes, FP64); status[b] = PSVD_INFO_COMPLETION when a residual falls below 10 eps.
  * ------------------------------------------------------------------------------------- */
+/* Batched complex one-sided Jacobi of a uniform batch at any size: psvd_jacobi_lds's complex path
+ * (same arithmetic, outputs and info) when a matrix fits LDS; beyond it the same kernel runs with
+ * each resident workgroup's planes in a region of the workspace (psvd_cjacobi_batched_workspace_bytes:
+ * 256 bytes for the work queue, plus the regions when the shape exceeds LDS). */
+size_t psvd_cjacobi_batched_workspace_bytes(int dtype, int64_t batch, int m, int n, int flags);
+int psvd_cjacobi_batched(int dtype, int64_t batch, int m, int n, const void *a, void *u, void *s, void *vh,
+                         int32_t *info, double tol, int max_sweeps, int flags, void *workspace, size_t workspace_bytes,
+                         psvd_stream_t stream);
+
 int psvd_cqr_lds_smem_bytes(int dtype, int m, int n, size_t *bytes);
 int psvd_cqr_lds(int dtype, int64_t batch, int m, int n, const void *a, void *q, void *r, psvd_stream_t stream);
 /* The same QR for any m >= n: shapes beyond LDS run the same kernel with the planes in the workspace

@@ -540,6 +540,29 @@ static PyObject *py_csolve_chol(PyObject *, PyObject *args) {
   return PyLong_FromLong(st);
 }
 
+static PyObject *py_cjacobi_batched_workspace_bytes(PyObject *, PyObject *args) {
+  int dtype, m, n, flags;
+  long long batch;
+  if (!PyArg_ParseTuple(args, "iLiii", &dtype, &batch, &m, &n, &flags)) return nullptr;
+  return PyLong_FromSize_t(psvd_cjacobi_batched_workspace_bytes(dtype, batch, m, n, flags));
+}
+
+static PyObject *py_cjacobi_batched(PyObject *, PyObject *args) {
+  int dtype, m, n, max_sweeps, flags;
+  long long batch;
+  unsigned long long a, u, s, vh, info, ws, wsb, stream;
+  double tol;
+  if (!PyArg_ParseTuple(args, "iLiiKKKKKdiiKKK", &dtype, &batch, &m, &n, &a, &u, &s, &vh, &info, &tol, &max_sweeps,
+                        &flags, &ws, &wsb, &stream))
+    return nullptr;
+  int st;
+  Py_BEGIN_ALLOW_THREADS
+  st = psvd_cjacobi_batched(dtype, batch, m, n, PTR(a), PTR(u), PTR(s), PTR(vh), static_cast<int32_t *>(PTR(info)),
+                            tol, max_sweeps, flags, PTR(ws), (size_t)wsb, PTR(stream));
+  Py_END_ALLOW_THREADS
+  return PyLong_FromLong(st);
+}
+
 static PyMethodDef kMethods[] = {
     {"version", py_version, METH_NOARGS, "library version string"},
     {"jacobi_lds_workspace_bytes", py_jacobi_lds_workspace_bytes, METH_VARARGS, nullptr},
@@ -565,6 +588,8 @@ static PyMethodDef kMethods[] = {
     {"scale_columns", py_scale_columns, METH_VARARGS, nullptr},
     {"cqr_lds_smem_bytes", py_cqr_lds_smem_bytes, METH_VARARGS, nullptr},
     {"cqr_lds", py_cqr_lds, METH_VARARGS, nullptr},
+    {"cjacobi_batched_workspace_bytes", py_cjacobi_batched_workspace_bytes, METH_VARARGS, nullptr},
+    {"cjacobi_batched", py_cjacobi_batched, METH_VARARGS, nullptr},
     {"cqr_workspace_bytes", py_cqr_workspace_bytes, METH_VARARGS, nullptr},
     {"cqr", py_cqr, METH_VARARGS, nullptr},
     {"ceigh_workspace_bytes", py_ceigh_workspace_bytes, METH_VARARGS, nullptr},

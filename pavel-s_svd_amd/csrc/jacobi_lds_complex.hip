@@ -31,6 +31,8 @@ struct CLdsParams {
   int m, n;
   double tol;
   int max_sweeps, flags;
+  double *gws;       // GL: one region of gstride doubles per workgroup in place of LDS
+  int64_t gstride;
 };
 
 template <int NW>
@@ -86,7 +88,7 @@ __device__ __forceinline__ void cplx_rotate(R *xr, R *xi, R *yr, R *yi, int rows
 
 // S > 0: every lane holds exactly S = ceil(m / L) row slots of a pair (L <= 16), kept in registers
 // from the sums to the rotation with no bounds tests; S = 0: rows streamed with a bounds test.
-template <typename R, int NT, int L, int S>
+template <typename R, int NT, int L, int S, bool GL = false>
 __global__ __launch_bounds__(NT) void jacobi_lds_cplx_kernel(CLdsParams<R> P) {
   extern __shared__ double smem_d[];
   __shared__ int s_idx;
@@ -106,7 +108,8 @@ __global__ __launch_bounds__(NT) void jacobi_lds_cplx_kernel(CLdsParams<R> P) {
   const int m = P.m, n = P.n;
   const int np = c_pad_even(n), ldw = c_ld(m), ldv = c_ld(np), npairs = np / 2;
   const int rcv = (np + L - 1) / L;
-  double *sig = smem_d;
+  // GL: the planes of matrices beyond LDS in the workgroup's region of a global workspace (L2)
+  double *sig = GL ? P.gws + (int64_t)blockIdx.x * P.gstride : smem_d;
   double *yr = sig + np, *yi = yr + np;
   double *vr = yi + np, *vi = vr + m;
   R *Wr = reinterpret_cast<R *>(vi + m);
@@ -499,6 +502,8 @@ static int run_clds_t(int64_t batch, int m, int n, const void *a, void *u, void 
   P.tol = tol;
   P.max_sweeps = max_sweeps;
   P.flags = flags;
+  P.gws = nullptr;
+  P.gstride = 0;
   // row slots per lane kept in registers: up to 8 (complex64) or 4 (complex128), L <= 16
   constexpr int SMAX = sizeof(R) == 4 ? 8 : 4;
   const int slots = (m + lanes - 1) / lanes;
@@ -526,6 +531,44 @@ static int run_clds_t(int64_t batch, int m, int n, const void *a, void *u, void 
 #undef PSVD_CLDS_L
 }
 
+// Matrices whose planes exceed LDS: the general (streamed-row) kernel with 64 lanes per pair on
+// regions of the workspace, one per resident workgroup (the work queue hands each its matrices).
+constexpr int kCglGrid = 512;   // workgroups: two per CU
+
+static int64_t cgl_stride(size_t bytes) { return (int64_t)((bytes + 255) / 256 * 32); }
+
+template <typename R>
+static size_t cgl_ws_bytes(int64_t batch, int m, int n, bool vectors) {
+  const int64_t grid = batch < kCglGrid ? batch : kCglGrid;
+  return 256 + (size_t)(grid * cgl_stride(clds_bytes<R>(m, n, vectors))) * sizeof(double);
+}
+
+template <typename R>
+static int run_cgl(int64_t batch, int m, int n, const void *a, void *u, void *s, void *vh, int32_t *info, double tol,
+                   int max_sweeps, int flags, void *ws, size_t ws_bytes, hipStream_t stream) {
+  const bool vectors = !(flags & PSVD_NO_VECTORS);
+  if (ws_bytes < cgl_ws_bytes<R>(batch, m, n, vectors)) return PSVD_EINVAL;
+  CLdsParams<R> P;
+  P.a = static_cast<const R *>(a);
+  P.u = static_cast<R *>(u);
+  P.s = static_cast<R *>(s);
+  P.vh = static_cast<R *>(vh);
+  P.info = info;
+  P.queue = static_cast<int32_t *>(ws);
+  P.batch = batch;
+  P.m = m;
+  P.n = n;
+  P.tol = tol;
+  P.max_sweeps = max_sweeps;
+  P.flags = flags;
+  P.gws = reinterpret_cast<double *>(static_cast<char *>(ws) + 256);
+  P.gstride = cgl_stride(clds_bytes<R>(m, n, vectors));
+  const int64_t grid = batch < kCglGrid ? batch : kCglGrid;
+  hipLaunchKernelGGL(zero_queue_kernel_c, dim3(1), dim3(64), 0, stream, P.queue);
+  hipLaunchKernelGGL((jacobi_lds_cplx_kernel<R, 256, 64, 0, true>), dim3((unsigned)grid), dim3(256), 0, stream, P);
+  return hipGetLastError() == hipSuccess ? PSVD_OK : PSVD_ELAUNCH;
+}
+
 size_t clds_smem_bytes(int dtype, int m, int n, bool vectors) {
   return dtype == PSVD_C64 ? clds_bytes<float>(m, n, vectors) : clds_bytes<double>(m, n, vectors);
 }
@@ -540,3 +583,43 @@ int run_clds(int dtype, int64_t batch, int m, int n, const void *a, void *u, voi
 }
 
 }  // namespace psvd
+
+extern "C" {
+
+size_t psvd_cjacobi_batched_workspace_bytes(int dtype, int64_t batch, int m, int n, int flags) {
+  if (batch < 0 || m < 1 || n < 1 || n > m) return 0;
+  const bool vectors = !(flags & PSVD_NO_VECTORS);
+  if (dtype == PSVD_C64)
+    return psvd::clds_bytes<float>(m, n, vectors) > psvd::kCLdsCap ? psvd::cgl_ws_bytes<float>(batch, m, n, vectors)
+                                                                   : 256;
+  if (dtype == PSVD_C128)
+    return psvd::clds_bytes<double>(m, n, vectors) > psvd::kCLdsCap ? psvd::cgl_ws_bytes<double>(batch, m, n, vectors)
+                                                                    : 256;
+  return 0;
+}
+
+int psvd_cjacobi_batched(int dtype, int64_t batch, int m, int n, const void *a, void *u, void *s, void *vh,
+                         int32_t *info, double tol, int max_sweeps, int flags, void *workspace, size_t workspace_bytes,
+                         psvd_stream_t stream) {
+  if (batch < 0 || batch > 0x7fffffff || m < 1 || n < 1 || n > m || max_sweeps < 1 || !a || !s || !info ||
+      !workspace || workspace_bytes < 256)
+    return PSVD_EINVAL;
+  if (!(flags & PSVD_NO_VECTORS) && (!u || !vh)) return PSVD_EINVAL;
+  if (batch == 0) return PSVD_OK;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const bool vectors = !(flags & PSVD_NO_VECTORS);
+  if (dtype == PSVD_C64) {
+    if (psvd::clds_bytes<float>(m, n, vectors) <= psvd::kCLdsCap)
+      return psvd::run_clds(dtype, batch, m, n, a, u, s, vh, info, tol, max_sweeps, flags, 0, workspace, st);
+    return psvd::run_cgl<float>(batch, m, n, a, u, s, vh, info, tol, max_sweeps, flags, workspace, workspace_bytes, st);
+  }
+  if (dtype == PSVD_C128) {
+    if (psvd::clds_bytes<double>(m, n, vectors) <= psvd::kCLdsCap)
+      return psvd::run_clds(dtype, batch, m, n, a, u, s, vh, info, tol, max_sweeps, flags, 0, workspace, st);
+    return psvd::run_cgl<double>(batch, m, n, a, u, s, vh, info, tol, max_sweeps, flags, workspace, workspace_bytes,
+                                 st);
+  }
+  return PSVD_EINVAL;
+}
+
+}  // extern "C"

@@ -29,6 +29,7 @@ def jacobi_lds_uniform(x, *, tol=None, max_sweeps=60, scale=False, cut='referenc
 
     Returns U (b, m, n), s (b, n) descending, Vh (b, n, n) (None, s, None without vectors) and
     the per-matrix info tensor (b, 2) = [sweeps, status] on the device; no host synchronisation.
+    Complex matrices beyond LDS run the same complex kernel on regions of a workspace.
     """
     b, m, n = x.shape
     if m < n:
@@ -41,6 +42,13 @@ def jacobi_lds_uniform(x, *, tol=None, max_sweeps=60, scale=False, cut='referenc
     s = torch.empty(b, n, dtype=x.real.dtype if x.is_complex() else x.dtype, device=dev)
     vh = torch.empty(b, n, n, dtype=x.dtype, device=dev) if vectors else None
     info = torch.empty(b, 2, dtype=torch.int32, device=dev)
+    if x.is_complex() and not lds_fits(x.dtype, m, n, vectors):
+        wsb = N._hip.cjacobi_batched_workspace_bytes(code, b, m, n, flags)
+        ws = N.workspace(wsb, dev)
+        st = N._hip.cjacobi_batched(code, b, m, n, N.ptr(x), N.ptr(u), N.ptr(s), N.ptr(vh), N.ptr(info),
+                                    float(tol or 0.0), int(max_sweeps), flags, N.ptr(ws), wsb, N.stream_of(x))
+        N.check(st, 'psvd_cjacobi_batched')
+        return u, s, vh, info
     wsb = N._hip.jacobi_lds_workspace_bytes(b)
     ws = N.workspace(wsb, dev)
     if x.is_complex():   # the complex kernel runs 256 threads, lanes from the row count
@@ -116,19 +124,22 @@ def jacobi_blocked(x, *, tol=None, max_sweeps=60, scale=False, cut='reference', 
 
 
 def jacobi_any(x, method='auto', **kw):
-    """K1 when the shape fits LDS; otherwise the blocked kernel (n >= 64) or the cooperative one
-    (complex matrices beyond LDS: the complex cooperative kernel).  method 'grid' or 'blocked' forces
-    a large-matrix kernel."""
+    """K1 when the shape fits LDS; otherwise the blocked kernel (n >= 64) or the cooperative one.
+    Complex matrices beyond LDS: one matrix takes the complex cooperative kernel, a batch the batched
+    complex kernel on workspace regions (both the reference's rotation order).  method 'grid' or
+    'blocked' forces a large-matrix kernel."""
     b, m, n = x.shape
     block_width = kw.pop('block_width', None) or 16
     if method == 'auto' and lds_fits(x.dtype, m, n, kw.get('vectors', True)):
         return jacobi_lds_uniform(x, **kw)
     kw.pop('lanes', None)
     kw.pop('threads', None)
-    if x.is_complex():   # the complex cooperative kernel (the reference's rotation order) at any size
+    if x.is_complex():   # the reference's rotation order at any size
         if method not in ('auto', 'grid'):
             raise ValueError(f'complex matrices beyond LDS take the grid kernel, not {method!r}')
-        return jacobi_grid(x, **kw)
+        if b > 1 and method == 'auto':   # a batch: the batched kernel on workspace regions
+            return jacobi_lds_uniform(x, **kw)
+        return jacobi_grid(x, **kw)       # one matrix: the cooperative kernel over the whole GPU
     if method == 'blocked' or (method == 'auto' and n >= 64):
         return jacobi_blocked(x, block_width=block_width, **kw)
     if method not in ('auto', 'grid'):

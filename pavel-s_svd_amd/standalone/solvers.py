@@ -721,9 +721,6 @@ def svd(a, algorithm='jacobi', *, full_matrices=False, values_only=False, return
         raise ValueError('input contains NaN or infinity')
     if algorithm not in SUPPORTED:
         raise Unsupported(f'{algorithm!r} is outside the MI355X hot path (built: {SUPPORTED})')
-    if a.is_complex() and algorithm == 'block_jacobi' and not P.lds_fits(
-            a.dtype, max(a.shape[-2:]), min(min(a.shape[-2:]), 2 * max(1, opts.get('block_size', 4))), True):
-        raise Unsupported('complex block_jacobi is built for group pairs whose columns fit in LDS only')
     if a.device.type != 'cuda':
         raise Unsupported('the MI355X solvers take ROCm device tensors; there is no CPU path')
     if algorithm != 'jacobi' and x.shape[0] > MAX_LAUNCH_BATCH:

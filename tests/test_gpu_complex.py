@@ -367,10 +367,38 @@ def test_complex_range_finders_match_oracle_on_the_same_sketch(pkg, expected, me
 
 def test_complex_unsupported_paths(pkg):
     x = torch.zeros(1, 20, 10, dtype=torch.complex64, device=DEV) + 1
-    with pytest.raises(pkg.Unsupported):   # complex128 group pairs of 600 x 64 beyond LDS
-        pkg.svd(torch.zeros(1, 600, 80, dtype=torch.complex128, device=DEV) + 1, 'block_jacobi', block_size=32)
     with pytest.raises(pkg.Unsupported):
         pkg.svd(x, 'lanczos')
+
+
+@pytest.mark.parametrize('dtype', [np.complex64, np.complex128])
+def test_complex_batched_jacobi_beyond_lds(pkg, dtype):
+    """A batch of complex matrices beyond LDS (the batched complex kernel on workspace regions, the
+    reference's rotation order): sigma, U, V against complex LAPACK, sweep counts within one of the
+    oracle's restatement; and complex block_jacobi whose 1200 x 16 group pairs exceed LDS against
+    LAPACK."""
+    from importlib import import_module
+    P = import_module('pavel-s_svd_amd.standalone.primitives')
+    tdt = torch.complex64 if dtype == np.complex64 else torch.complex128
+    rng = np.random.default_rng(9)
+    a = (rng.standard_normal((3, 260, 120)) + 1j * rng.standard_normal((3, 260, 120))).astype(dtype)
+    assert not P.lds_fits(tdt, 260, 120, True)
+    print(f'complex batched jacobi beyond LDS 3x260x120 {np.dtype(dtype).name}', flush=True)
+    u, s, vh, info = pkg.svd(torch.from_numpy(a).to(DEV), 'jacobi', return_info=True)
+    bar = 1e-5 if dtype == np.complex64 else 1e-12
+    for i in range(3):
+        met = _metrics(a[i], (u[i], s[i], vh[i]))
+        for key in KEYS:
+            assert met[key] <= bar, (i, key, met[key])
+    _, _, _, oinfo = R.svd(a[0], 'jacobi', return_info=True)
+    assert abs(info['jacobi_sweeps'] - oinfo['jacobi_sweeps']) <= 1
+    b = (rng.standard_normal((1, 1200, 40)) + 1j * rng.standard_normal((1, 1200, 40))).astype(dtype)
+    assert not P.lds_fits(tdt, 1200, 16, True)
+    print(f'complex block_jacobi with group pairs beyond LDS 1200x40 bs 8 {np.dtype(dtype).name}', flush=True)
+    u, s, vh = pkg.svd(torch.from_numpy(b).to(DEV), 'block_jacobi', block_size=8)
+    met = _metrics(b[0], (u[0], s[0], vh[0]))
+    for key in KEYS:
+        assert met[key] <= bar * 10, (key, met[key])
 
 
 @pytest.mark.parametrize('method', ['jacobi', 'qr', 'tsqr', 'polar_qdwh', 'block_jacobi'])
