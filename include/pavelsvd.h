@@ -41,6 +41,9 @@ typedef void *psvd_stream_t;
 #define PSVD_INFO_COMPLETION 2  /* maps to RuntimeError('null-space completion failed ...') */
 #define PSVD_INFO_NONFINITE 3   /* maps to ValueError('input contains NaN or infinity') */
 #define PSVD_INFO_BADSHAPE 4    /* a descriptor violates 1 <= n <= m, m <= max_m, n <= max_n */
+#define PSVD_INFO_BARRIER 5     /* a single-launch kernel (psvd_jacobi_grid) abandoned a grid-barrier wait:
+                                   its workgroups were not all resident (the GPU was shared with other
+                                   launches that held its CUs); maps to RuntimeError */
 
 /* flags for the Jacobi entry points */
 #define PSVD_SCALE 1        /* divide by max|a| first and multiply sigma back (solvers.py:294-299) */
@@ -177,8 +180,10 @@ int psvd_complete_columns(int dtype, int64_t batch, int m, int n, void *u, int64
  * so R matches primitives.qr (svdbench/standalone/primitives.py:41-89) up to rounding.  A is
  * read only (row stride lda, batch stride stride_a).  q or r may be NULL to skip that output.
  * gemm_mode: PSVD_GEMM_F32 or PSVD_GEMM_BF16X3 for dtype PSVD_F32 (ignored for PSVD_F64).
- * Panels taller than 512 rows are factored by several workgroups per matrix in one cooperative
- * launch (at most one workgroup per CU); such calls are not capturable in a hipGraph.
+ * Panels taller than 512 rows are factored by several workgroups per matrix in one launch of at
+ * most one workgroup per CU, its phases separated by the library's grid barrier (an ordinary
+ * launch, capturable in a hipGraph; a barrier wait abandoned because the GPU was shared with other
+ * launches holding its CUs leaves the panel NaN, so the caller's finiteness check reports it).
  * ------------------------------------------------------------------------------------- */
 size_t psvd_qr_workspace_bytes(int dtype, int64_t batch, int m, int n);
 int psvd_qr(int dtype, int gemm_mode, int64_t batch, int m, int n, const void *a, int64_t lda, int64_t stride_a,
@@ -297,14 +302,16 @@ int psvd_trsm_right(int dtype, int gemm_mode, int64_t batch, int trans, int n, i
 
 /* ---------------------------------------------------------------------------------------
  * One-sided Jacobi SVD of ONE tall matrix that does not fit LDS (e.g. 512x256, 2048x32, the
- * n x n R of TSQR), as a single cooperative launch: W and V in the workspace (HBM), each
- * workgroup owns a strided share of every round's column pairs, one grid barrier per round,
+ * n x n R of TSQR), as a single launch of at most one workgroup per CU: W and V in the workspace
+ * (HBM), each workgroup owns a strided share of every round's column pairs, one grid barrier per
+ * round (the library's own barrier on counters in the workspace; status PSVD_INFO_BARRIER if a wait
+ * is abandoned because other launches held the GPU's CUs),
  * FP64 dot products / convergence test / sigma, sort, cut and completion on the device.
  * Same algorithm and flags as psvd_jacobi_lds (primitives.py:316-367); info = [sweeps, status].
  * Complex dtypes (PSVD_C64 / PSVD_C128): interleaved a, u, vh, real s, the complex rotation of
  * primitives.py:340-351, vh = V^H; the cut columns are completed by psvd_ccomplete_columns in the
  * same stream (m + n up to about 9700).
- * Not capturable in a hipGraph (cooperative launch).
+ * Ordinary launches only: capturable in a hipGraph.
  * ------------------------------------------------------------------------------------- */
 size_t psvd_jacobi_grid_workspace_bytes(int dtype, int m, int n);
 int psvd_jacobi_grid(int dtype, int m, int n, const void *a, void *u, void *s, void *vh, int32_t *info,

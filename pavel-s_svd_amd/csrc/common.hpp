@@ -140,7 +140,7 @@ size_t clds_smem_bytes(int dtype, int m, int n, bool vectors);
 int run_clds(int dtype, int64_t batch, int m, int n, const void *a, void *u, void *s, void *vh, int32_t *info,
              double tol, int max_sweeps, int flags, int lanes, void *ws, hipStream_t stream);
 
-// Complex cooperative one-sided Jacobi (csrc/jacobi_grid_complex.hip), reached through
+// Complex single-launch one-sided Jacobi (csrc/jacobi_grid_complex.hip), reached through
 // psvd_jacobi_grid with dtype PSVD_C64 / PSVD_C128.
 size_t cgrid_workspace_bytes(int dtype, int m, int n);
 int cjacobi_grid(int dtype, int m, int n, const void *a, void *u, void *s, void *vh, int32_t *info, double tol,
@@ -350,6 +350,88 @@ __device__ __forceinline__ void rotate_rows_batched(T *Qt, int ld, const double2
       Qt[a[j].y * ld + qcol] = (T)(r[j].y * x[j] + r[j].x * y[j]);
     }
 }
+
+// ---- grid barrier of the package's single-launch kernels ---------------------------------------
+// K2, K2c and the tall QR panel run every workgroup of an ordinary launch of at most one workgroup
+// per CU (all of them resident on an otherwise idle GPU) and separate their phases with this
+// barrier instead of a cooperative launch: workgroups arrive on one of kBarGroups counters 256 bytes
+// apart (one counter when there are at most kBarFlat workgroups), the last arrival of a group bumps
+// the top counter, and every workgroup polls the top counter with agent-scope loads.  Counters only
+// grow during a launch (bar_reset_kernel clears them in-stream before it), so barrier e of the launch
+// waits for e x (number of groups).  Memory: every wave waits for its own stores before the
+// workgroup barrier; with FENCE the arriving thread then writes the XCD's L2 back (agent-scope
+// release) and, after the wait, invalidates its caches (agent-scope acquire), so plain global data
+// written before the barrier is read fresh after it; without FENCE the data that crosses the
+// barrier must be written and read with agent-scope atomic accesses (st_agent / ld_agent).  A wait
+// that exceeds kBarSpinLimit polls (workgroups that are not co-resident) raises the abort word, every
+// waiter then returns false, and the kernel ends early with its failure reported.
+constexpr int kBarGroups = 8, kBarFlat = 64;
+constexpr unsigned kBarSpinLimit = 1u << 23;
+constexpr int kBarWords = 64 * (2 + kBarGroups);   // top, abort, group counters: one 256-byte line each
+
+__global__ void bar_reset_kernel(unsigned *ctr, int words);
+
+template <typename T>
+__device__ __forceinline__ void st_agent(T *p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ T ld_agent(const T *p) {
+  return __hip_atomic_load(const_cast<T *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+struct GridBarrier {
+  unsigned *ctr;      // kBarWords counters of this barrier (zeroed before the launch)
+  unsigned nwg, idx;  // workgroups taking part and this workgroup's index among them
+  unsigned epoch;
+
+  __device__ GridBarrier(unsigned *c, unsigned n, unsigned i) : ctr(c), nwg(n), idx(i), epoch(0) {}
+
+  // Every thread of every participating workgroup calls sync() the same number of times.  Returns
+  // false (in every thread) when the wait was abandoned.
+  template <bool FENCE>
+  __device__ bool sync() {
+    __shared__ int s_ok;
+    ++epoch;
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");   // this wave's stores have reached L2
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      if (FENCE) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      const bool flat = nwg <= (unsigned)kBarFlat;
+      const unsigned ng = flat ? 1u : (unsigned)kBarGroups, g = flat ? 0u : idx % ng;
+      const unsigned gsz = flat ? nwg : nwg / ng + (g < nwg % ng ? 1u : 0u);
+      unsigned *top = ctr, *abort_ = ctr + 64;
+      if (flat) {
+        __hip_atomic_fetch_add(top, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      } else {
+        const unsigned got = __hip_atomic_fetch_add(ctr + 64 * (2 + g), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (got + 1 == epoch * gsz) __hip_atomic_fetch_add(top, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      const unsigned target = epoch * (flat ? nwg : ng);
+      int ok = 1;
+      for (unsigned spins = 0; ld_agent(top) < target; ++spins) {
+        if ((spins & 63) == 63 && ld_agent(abort_) != 0) {
+          ok = 0;
+          break;
+        }
+        if (spins > kBarSpinLimit) {
+          st_agent(abort_, 1u);
+          ok = 0;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+      if (FENCE) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      s_ok = ok;
+    }
+    __syncthreads();
+    return s_ok != 0;
+  }
+};
+
+// Workgroups the barrier kernels may launch: at most one per CU, and none when the kernel cannot
+// hold one workgroup per CU (its occupancy is below one).
+int barrier_grid_limit(const void *kernel, int threads, size_t lds);
 
 // MFMA GEMM with the psvd_gemm contract plus an optional split-K workspace (gemm.hip).
 int gemm_ws(int mode, int transa, int transb, int64_t m, int64_t n, int64_t k, double alpha, const void *a, int64_t lda,

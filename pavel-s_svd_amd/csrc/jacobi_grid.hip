@@ -1,9 +1,10 @@
-// K2: one-sided Jacobi SVD of one matrix too large for LDS, as a single cooperative launch.
+// K2: one-sided Jacobi SVD of one matrix too large for LDS, as a single launch.
 //
 // W (m x np, column-major) and V (np x np, column-major) live in a caller-provided workspace in
-// HBM.  Every workgroup of a co-resident grid owns a strided subset of the round's column pairs;
-// a pair's three dot products are a workgroup reduction (DPP within waves, LDS across waves)
-// accumulated in FP64; the rotation is applied in place and the round ends with a grid barrier.
+// HBM.  Every workgroup of a resident grid (at most one per CU) owns a strided subset of the round's
+// column pairs; a pair's three dot products are a workgroup reduction (DPP within waves, LDS across
+// waves) accumulated in FP64; the rotation is applied in place and the round ends with a grid barrier
+// (GridBarrier, with the agent-scope cache write-back and invalidation W and V need).
 // The whole solve -- load, max-abs scale, sweeps with the FP64 convergence test, FP64 sigma,
 // sort, cut, coordinate-vector completion, and the row-major outputs -- is one launch with no
 // host synchronisation.
@@ -11,11 +12,8 @@
 // Follows svdbench/standalone/primitives.py:316-367 (jacobi_svd), :309-313 (schedule),
 // :430-455 (complete_null_columns) and solvers.py:294-299 (max-abs scaling).
 #include <float.h>
-#include <hip/hip_cooperative_groups.h>
 
 #include "common.hpp"
-
-namespace cg = cooperative_groups;
 
 namespace psvd {
 
@@ -32,6 +30,7 @@ struct GridParams {
   int *ord, *parti;                  // np, grid
   unsigned long long *rmax;          // 3 (+1 for amax)
   int *flag;                         // 3 (+1 for nonfinite)
+  unsigned *bar;                     // kBarWords grid-barrier counters
   double tol;
   int max_sweeps, flags;
 };
@@ -53,6 +52,7 @@ __host__ __device__ inline size_t grid_ws_bytes(int m, int n, int grid) {
   take(sizeof(int) * grid);
   take(sizeof(unsigned long long) * 4);
   take(sizeof(int) * 4);
+  take(sizeof(unsigned) * kBarWords);
   return b;
 }
 
@@ -92,7 +92,18 @@ __device__ __forceinline__ double block_sum1(double a, double *red) {
 
 template <typename T>
 __global__ __launch_bounds__(kGridThreads) void jacobi_grid_kernel(GridParams<T> P) {
-  cg::grid_group grid = cg::this_grid();
+  GridBarrier bar(P.bar, gridDim.x, blockIdx.x);
+  // a barrier wait abandoned (workgroups not co-resident): report it and end the launch
+#define PSVD_GRID_SYNC()                       \
+  do {                                         \
+    if (!bar.sync<true>()) {                   \
+      if (gtid == 0) {                         \
+        P.info[0] = 0;                         \
+        P.info[1] = PSVD_INFO_BARRIER;         \
+      }                                        \
+      return;                                  \
+    }                                          \
+  } while (0)
   __shared__ double red[3 * (kGridThreads / kWave)];
   __shared__ int redi[kGridThreads / kWave];
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
@@ -110,7 +121,7 @@ __global__ __launch_bounds__(kGridThreads) void jacobi_grid_kernel(GridParams<T>
     rmax[0] = rmax[1] = rmax[2] = rmax[3] = 0ull;
     flag[0] = flag[1] = flag[2] = flag[3] = 0;
   }
-  grid.sync();
+  PSVD_GRID_SYNC();
   // ---- load (row-major A -> column-major W), finiteness, max |a|; V = I
   {
     double amax = 0.0;
@@ -133,7 +144,7 @@ __global__ __launch_bounds__(kGridThreads) void jacobi_grid_kernel(GridParams<T>
     if (lane == 0 && amax > 0.0) atomicMax(&rmax[3], (unsigned long long)__double_as_longlong(amax));
     if (bad) atomicOr(&flag[3], 1);
   }
-  grid.sync();
+  PSVD_GRID_SYNC();
   if (__atomic_load_n(&flag[3], __ATOMIC_RELAXED)) {
     if (gtid == 0) {
       P.info[0] = 0;
@@ -152,7 +163,7 @@ __global__ __launch_bounds__(kGridThreads) void jacobi_grid_kernel(GridParams<T>
         W[(int64_t)c * ldw + i] = W[(int64_t)c * ldw + i] / st;
       }
     }
-    grid.sync();
+    PSVD_GRID_SYNC();
   }
   // ---- initial floor: max_j |w_j|^2
   for (int c = b; c < n; c += G) {
@@ -164,7 +175,7 @@ __global__ __launch_bounds__(kGridThreads) void jacobi_grid_kernel(GridParams<T>
     acc = block_sum1(acc, red);
     if (tid == 0) atomicMax(&rmax[0], (unsigned long long)__double_as_longlong(acc));
   }
-  grid.sync();
+  PSVD_GRID_SYNC();
 
   // ---- sweeps: one grid barrier per round
   const double tol2 = tol * tol, eps2 = eps * eps;
@@ -213,7 +224,7 @@ __global__ __launch_bounds__(kGridThreads) void jacobi_grid_kernel(GridParams<T>
         }
         if (tid == 0) atomicMax(&rmax[(g + 1) % 3], (unsigned long long)__double_as_longlong(fmax(nap, naq)));
       }
-      grid.sync();
+      PSVD_GRID_SYNC();
     }
     if (__atomic_load_n(&flag[sw % 3], __ATOMIC_RELAXED) == 0) {
       converged = true;
@@ -232,7 +243,7 @@ __global__ __launch_bounds__(kGridThreads) void jacobi_grid_kernel(GridParams<T>
     acc = block_sum1(acc, red);
     if (tid == 0) P.sig[c] = sqrt(acc);
   }
-  grid.sync();
+  PSVD_GRID_SYNC();
   for (int64_t j = gtid; j < n; j += gsz) {
     const double sj = P.sig[j];
     int rank = 0;
@@ -242,7 +253,7 @@ __global__ __launch_bounds__(kGridThreads) void jacobi_grid_kernel(GridParams<T>
     }
     P.ord[rank] = (int)j;
   }
-  grid.sync();
+  PSVD_GRID_SYNC();
   const double s1 = P.sig[P.ord[0]];
   const double cut = (P.flags & PSVD_CUT_FLOOR) ? eps * s1 : eps * (double)(m > np ? m : np) * s1;
   int nk;
@@ -262,7 +273,7 @@ __global__ __launch_bounds__(kGridThreads) void jacobi_grid_kernel(GridParams<T>
       T *wc = W + (int64_t)c * ldw;
       wc[i] = r < nk ? (T)((double)wc[i] / P.sig[c]) : T(0);
     }
-    grid.sync();
+    PSVD_GRID_SYNC();
     for (int r = nk; r < n; ++r) {
       // leverage 1 - |U(i,:)|^2 and its first maximiser
       double best = -DBL_MAX;
@@ -302,7 +313,7 @@ __global__ __launch_bounds__(kGridThreads) void jacobi_grid_kernel(GridParams<T>
         P.part[b] = best;
         P.parti[b] = bi;
       }
-      grid.sync();
+      PSVD_GRID_SYNC();
       best = P.part[0];
       bi = P.parti[0];
       for (int w = 1; w < G; ++w)
@@ -316,14 +327,14 @@ __global__ __launch_bounds__(kGridThreads) void jacobi_grid_kernel(GridParams<T>
         for (int c = 0; c < n; ++c) acc -= (double)W[(int64_t)c * ldw + i] * (double)W[(int64_t)c * ldw + ix];
         P.vt[i] = acc;
       }
-      grid.sync();
+      PSVD_GRID_SYNC();
       for (int c = b; c < n; c += G) {
         double acc = 0.0;
         for (int i = tid; i < m; i += kGridThreads) acc = fma((double)W[(int64_t)c * ldw + i], P.vt[i], acc);
         acc = block_sum1(acc, red);
         if (tid == 0) P.y[c] = acc;
       }
-      grid.sync();
+      PSVD_GRID_SYNC();
       double l2 = 0.0;
       for (int64_t i = gtid; i < m; i += gsz) {
         double acc = P.vt[i];
@@ -333,7 +344,7 @@ __global__ __launch_bounds__(kGridThreads) void jacobi_grid_kernel(GridParams<T>
       }
       l2 = block_sum1(l2, red);
       if (tid == 0) P.part[G + b] = l2;
-      grid.sync();
+      PSVD_GRID_SYNC();
       double len2 = 0.0;
       for (int w = 0; w < G; ++w) len2 += P.part[G + w];
       const double len = sqrt(len2);
@@ -341,7 +352,7 @@ __global__ __launch_bounds__(kGridThreads) void jacobi_grid_kernel(GridParams<T>
       const double inv = 1.0 / fmax(len, DBL_MIN);
       T *wc = W + (int64_t)P.ord[r] * ldw;
       for (int64_t i = gtid; i < m; i += gsz) wc[i] = (T)(P.vt[i] * inv);
-      grid.sync();
+      PSVD_GRID_SYNC();
     }
   }
 
@@ -362,6 +373,7 @@ __global__ __launch_bounds__(kGridThreads) void jacobi_grid_kernel(GridParams<T>
     P.info[1] = status;
   }
 }
+#undef PSVD_GRID_SYNC
 
 template <typename T>
 static int grid_size(int n) {
@@ -375,8 +387,8 @@ static int grid_size(int n) {
   (void)hipGetDevice(&dev);
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   if (cus <= 0) cus = 1;
-  // at most one workgroup per CU, so that every workgroup of the cooperative grid is resident
-  // without relying on the occupancy estimate
+  // at most one workgroup per CU, so that every workgroup of the grid is resident without relying on
+  // the occupancy estimate
   (void)occ;
   const int np = n + (n & 1);
   int g = np / 2;
@@ -419,10 +431,12 @@ static int run_grid(int m, int n, const void *a, void *u, void *s, void *vh, int
   P.tol = tol;
   P.max_sweeps = max_sweeps;
   P.flags = flags;
-  void *args[] = {&P};
-  hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<const void *>(jacobi_grid_kernel<T>), dim3(G),
-                                            dim3(kGridThreads), args, 0, stream);
-  return e == hipSuccess ? PSVD_OK : PSVD_ELAUNCH;
+  P.bar = reinterpret_cast<unsigned *>(take(sizeof(unsigned) * kBarWords));
+  static const int limit = barrier_grid_limit(reinterpret_cast<const void *>(jacobi_grid_kernel<T>), kGridThreads, 0);
+  if (G > limit) return PSVD_ELAUNCH;
+  hipLaunchKernelGGL(bar_reset_kernel, dim3(1), dim3(256), 0, stream, P.bar, kBarWords);
+  hipLaunchKernelGGL(jacobi_grid_kernel<T>, dim3(G), dim3(kGridThreads), 0, stream, P);
+  return hipGetLastError() == hipSuccess ? PSVD_OK : PSVD_ELAUNCH;
 }
 
 }  // namespace psvd

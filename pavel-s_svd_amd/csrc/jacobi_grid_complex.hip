@@ -1,22 +1,19 @@
-// K2c: complex one-sided Jacobi SVD of one matrix too large for LDS, as a single cooperative launch
-// (the complex counterpart of K2, csrc/jacobi_grid.hip).
+// K2c: complex one-sided Jacobi SVD of one matrix too large for LDS, as a single launch (the complex
+// counterpart of K2, csrc/jacobi_grid.hip).
 //
 // W (m x np) and V (np x np) live in the workspace as column-major real and imaginary planes.  Every
-// workgroup of a co-resident grid (at most one per CU) owns a strided subset of each round's column
+// workgroup of a resident grid (at most one per CU) owns a strided subset of each round's column
 // pairs; a pair's sums |w_p|^2, |w_q|^2 and w_p^H w_q are workgroup reductions accumulated in FP64;
 // the complex rotation of svdbench/standalone/primitives.py:340-351 (c real, s = t c a_pq / |a_pq|,
 // w_p <- c w_p - conj(s) w_q, w_q <- s w_p + c w_q, the same for V) is applied in place, and each
-// round ends with a grid barrier.  The test is the reference's (:332-339): |a_pq| / sqrt(a_pp a_qq) >
+// round ends with a grid barrier (GridBarrier with cache write-back and invalidation).  The test is the reference's (:332-339): |a_pq| / sqrt(a_pp a_qq) >
 // tol and min(a_pp, a_qq) > eps^2 max_j |w_j|^2, evaluated on the FP64 sums.  The launch ends with
 // FP64 sigma, the descending order, the cut (:362) and the outputs -- U = W / sigma for the kept
 // columns (zero for the rest), Vh = V^H -- and the count of kept columns; the cut columns are then
 // completed by psvd_ccomplete_columns (:430-455) in the same stream.
 #include <float.h>
-#include <hip/hip_cooperative_groups.h>
 
 #include "common.hpp"
-
-namespace cg = cooperative_groups;
 
 namespace psvd {
 
@@ -34,6 +31,7 @@ struct CGridParams {
   int32_t *nk;         // 1
   unsigned long long *rmax;   // 4: three rotating floors, max modulus of A
   int *flag;                  // 4: three rotating sweep flags, non-finite input
+  unsigned *bar;              // kBarWords grid-barrier counters
   double tol;
   int max_sweeps, flags;
 };
@@ -52,6 +50,7 @@ __host__ __device__ inline size_t cgrid_ws_bytes(int m, int n) {
   take(sizeof(int32_t) * 2);
   take(sizeof(unsigned long long) * 4);
   take(sizeof(int) * 4);
+  take(sizeof(unsigned) * kBarWords);
   return b;
 }
 
@@ -77,7 +76,19 @@ __device__ __forceinline__ void cgrid_block_sum(double (&v)[K], double *red) {
 
 template <typename R>
 __global__ __launch_bounds__(kCGridThreads) void cjacobi_grid_kernel(CGridParams<R> P) {
-  cg::grid_group grid = cg::this_grid();
+  GridBarrier bar(P.bar, gridDim.x, blockIdx.x);
+  // a barrier wait abandoned (workgroups not co-resident): report it and end the launch
+#define PSVD_CGRID_SYNC()                      \
+  do {                                         \
+    if (!bar.sync<true>()) {                   \
+      if (blockIdx.x == 0 && threadIdx.x == 0) { \
+        P.info[0] = 0;                         \
+        P.info[1] = PSVD_INFO_BARRIER;         \
+        *P.nk = P.n;                           \
+      }                                        \
+      return;                                  \
+    }                                          \
+  } while (0)
   __shared__ double red[4 * (kCGridThreads / kWave)];
   const int tid = threadIdx.x, lane = tid & (kWave - 1);
   const int G = gridDim.x, b = blockIdx.x;
@@ -94,7 +105,7 @@ __global__ __launch_bounds__(kCGridThreads) void cjacobi_grid_kernel(CGridParams
     rmax[0] = rmax[1] = rmax[2] = rmax[3] = 0ull;
     flag[0] = flag[1] = flag[2] = flag[3] = 0;
   }
-  grid.sync();
+  PSVD_CGRID_SYNC();
   {  // load, finiteness, largest modulus; V = I
     double amax = 0.0;
     int bad = 0;
@@ -121,7 +132,7 @@ __global__ __launch_bounds__(kCGridThreads) void cjacobi_grid_kernel(CGridParams
     if (lane == 0 && amax > 0.0) atomicMax(&rmax[3], (unsigned long long)__double_as_longlong(amax));
     if (bad) atomicOr(&flag[3], 1);
   }
-  grid.sync();
+  PSVD_CGRID_SYNC();
   if (__atomic_load_n(&flag[3], __ATOMIC_RELAXED)) {
     if (gtid == 0) {
       P.info[0] = 0;
@@ -142,7 +153,7 @@ __global__ __launch_bounds__(kCGridThreads) void cjacobi_grid_kernel(CGridParams
         Wi[(int64_t)c * ldw + i] = Wi[(int64_t)c * ldw + i] / st;
       }
     }
-    grid.sync();
+    PSVD_CGRID_SYNC();
   }
   for (int c = b; c < n; c += G) {   // initial floor: max_j |w_j|^2
     double v[1] = {0.0};
@@ -153,7 +164,7 @@ __global__ __launch_bounds__(kCGridThreads) void cjacobi_grid_kernel(CGridParams
     cgrid_block_sum<1>(v, red);
     if (tid == 0) atomicMax(&rmax[0], (unsigned long long)__double_as_longlong(v[0]));
   }
-  grid.sync();
+  PSVD_CGRID_SYNC();
 
   const double tol2 = tol * tol, eps2 = eps * eps;
   int sweeps = 0;
@@ -212,7 +223,7 @@ __global__ __launch_bounds__(kCGridThreads) void cjacobi_grid_kernel(CGridParams
         }
         if (tid == 0) atomicMax(&rmax[(g + 1) % 3], (unsigned long long)__double_as_longlong(fmax(nap, naq)));
       }
-      grid.sync();
+      PSVD_CGRID_SYNC();
     }
     if (__atomic_load_n(&flag[sw % 3], __ATOMIC_RELAXED) == 0) {
       converged = true;
@@ -230,7 +241,7 @@ __global__ __launch_bounds__(kCGridThreads) void cjacobi_grid_kernel(CGridParams
     cgrid_block_sum<1>(v, red);
     if (tid == 0) P.sig[c] = sqrt(v[0]);
   }
-  grid.sync();
+  PSVD_CGRID_SYNC();
   for (int64_t j = gtid; j < n; j += gsz) {
     const double sj = P.sig[j];
     int rank = 0;
@@ -240,7 +251,7 @@ __global__ __launch_bounds__(kCGridThreads) void cjacobi_grid_kernel(CGridParams
     }
     P.ord[rank] = (int)j;
   }
-  grid.sync();
+  PSVD_CGRID_SYNC();
   const double s1 = P.sig[P.ord[0]];
   const double cut = (P.flags & PSVD_CUT_FLOOR) ? eps * s1 : eps * (double)(m > np ? m : np) * s1;
   int lo = 0, hi = n;   // the first column at or below the cut
@@ -270,6 +281,7 @@ __global__ __launch_bounds__(kCGridThreads) void cjacobi_grid_kernel(CGridParams
     P.info[1] = converged ? PSVD_INFO_OK : PSVD_INFO_NOCONV;
   }
 }
+#undef PSVD_CGRID_SYNC
 
 __global__ void cgrid_status_kernel(int32_t *info, const int32_t *cstatus) {
   if (threadIdx.x == 0 && info[1] == PSVD_INFO_OK && *cstatus != 0) info[1] = PSVD_INFO_COMPLETION;
@@ -280,7 +292,7 @@ static int cgrid_size(int n) {
   (void)hipGetDevice(&dev);
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   if (cus <= 0) cus = 1;
-  // at most one workgroup per CU, so every workgroup of the cooperative grid is resident
+  // at most one workgroup per CU, so every workgroup of the grid is resident
   const int np = n + (n & 1);
   int g = np / 2;
   if (g < 64) g = 64;
@@ -321,10 +333,12 @@ static int run_cgrid(int dtype, int m, int n, const void *a, void *u, void *s, v
   P.tol = tol;
   P.max_sweeps = max_sweeps;
   P.flags = flags;
-  void *args[] = {&P};
-  if (hipLaunchCooperativeKernel(reinterpret_cast<const void *>(cjacobi_grid_kernel<R>), dim3(G),
-                                 dim3(kCGridThreads), args, 0, stream) != hipSuccess)
-    return PSVD_ELAUNCH;
+  P.bar = reinterpret_cast<unsigned *>(take(sizeof(unsigned) * kBarWords));
+  static const int limit = barrier_grid_limit(reinterpret_cast<const void *>(cjacobi_grid_kernel<R>), kCGridThreads, 0);
+  if (G > limit) return PSVD_ELAUNCH;
+  hipLaunchKernelGGL(bar_reset_kernel, dim3(1), dim3(256), 0, stream, P.bar, kBarWords);
+  hipLaunchKernelGGL(cjacobi_grid_kernel<R>, dim3(G), dim3(kCGridThreads), 0, stream, P);
+  if (hipGetLastError() != hipSuccess) return PSVD_ELAUNCH;
   if (!(flags & PSVD_NO_VECTORS)) {
     int32_t *cstatus = P.nk + 1;
     const int rc = psvd_ccomplete_columns(dtype, 1, m, n, u, P.nk, cstatus, stream);

@@ -14,11 +14,7 @@
 // :74-89 (qr).
 #include <float.h>
 
-#include <hip/hip_cooperative_groups.h>
-
 #include "common.hpp"
-
-namespace cg = cooperative_groups;
 
 namespace psvd {
 
@@ -140,8 +136,10 @@ __global__ __launch_bounds__(kPanelThreads) void qr_panel_kernel(T *F, int64_t l
 // matrix (KR rows per thread, held in registers for the whole panel: one read, one write).  Each
 // column needs two reductions over all rows -- |x2|^2, then w = v^T F(:, later columns) -- done as
 // per-workgroup partial sums in FP64 that every workgroup then adds in the same fixed order after
-// a grid barrier (deterministic, no atomics).  Scratch buffers alternate with the column parity so
-// that a fast workgroup never overwrites partials a slow one is still reading.
+// a barrier over the matrix's workgroups (deterministic sums, no atomic adds).  The partials cross
+// workgroups through agent-scope stores and loads, so the barrier (GridBarrier without cache
+// maintenance) only orders them.  Scratch buffers alternate with the column parity so that a fast
+// workgroup never overwrites partials a slow one is still reading.
 // per workgroup and column: kPanel w partials and the five partial sums that carry the next
 // column's norm and alpha through the same grid barrier (kCoopW doubles)
 constexpr int kCoopW = kPanel + 8;
@@ -160,15 +158,16 @@ __host__ __device__ constexpr size_t coop_lds_bytes() {
 template <typename T, int KR>
 __global__ __launch_bounds__(256) void qr_panel_coop_kernel(T *F, int64_t ldf, int64_t sf, T *V, int64_t sv, T *tau,
                                                             int64_t stau, int m, int j0, int nbc, int nw,
-                                                            double *scr) {
+                                                            double *scr, unsigned *bar) {
   constexpr int NT = 256;
   extern __shared__ double smem_coop[];
   __shared__ double red[(NT / kWave) * kPanel];
   __shared__ double wsum[kPanel];
   __shared__ double seg[NT / kCoopW][kCoopW];
-  cg::grid_group grid = cg::this_grid();
   const int tid = threadIdx.x;
   const int b = blockIdx.x / nw, wi = blockIdx.x - b * nw;
+  GridBarrier gb(bar + (size_t)b * kBarWords, (unsigned)nw, (unsigned)wi);   // the matrix's workgroups
+  bool aborted = false;
   T *Fb = F + (int64_t)b * sf;
   T *Vb = V + (int64_t)b * sv;
   T *tb = tau + (int64_t)b * stau;
@@ -206,15 +205,18 @@ __global__ __launch_bounds__(256) void qr_panel_coop_kernel(T *F, int64_t ldf, i
       for (int q = 0; q < KR; ++q)
         if (row[q] > j && row[q] < m) part = fma((double)X(q, c), (double)X(q, c), part);
       part = block_sum_one<NT>(part, red);
-      if (tid == 0) np_[wi] = part;
+      if (tid == 0) st_agent(np_ + wi, part);
 #pragma unroll
       for (int q = 0; q < KR; ++q)
-        if (row[q] == j) *al = (double)X(q, c);
-      grid.sync();
+        if (row[q] == j) st_agent(al, (double)X(q, c));
+      if (!gb.sync<false>()) {
+        aborted = true;
+        break;
+      }
       double tot = 0.0;
-      for (int t = tid; t < nw; t += NT) tot += np_[t];
+      for (int t = tid; t < nw; t += NT) tot += ld_agent(np_ + t);
       xnorm = sqrt(block_sum_one<NT>(tot, red));
-      alpha = *al;
+      alpha = ld_agent(al);
     }
     carried = false;
     double beta, tj, scal;
@@ -282,14 +284,17 @@ __global__ __launch_bounds__(256) void qr_panel_coop_kernel(T *F, int64_t ldf, i
         }
       }
       block_sum_vec<NT>(w, c + 1, nbc, red, wsum);   // its barriers also publish red5
-      if (tid > c && tid < nbc) wp[(size_t)wi * kCoopW + tid] = wsum[tid];
+      if (tid > c && tid < nbc) st_agent(wp + (size_t)wi * kCoopW + tid, wsum[tid]);
       if (tid < 5) {
         double t = 0.0;
 #pragma unroll
         for (int x = 0; x < NT / kWave; ++x) t += red5[x * 8 + tid];
-        wp[(size_t)wi * kCoopW + kPanel + tid] = t;
+        st_agent(wp + (size_t)wi * kCoopW + kPanel + tid, t);
       }
-      grid.sync();
+      if (!gb.sync<false>()) {
+        aborted = true;
+        break;
+      }
       {  // thread (k = tid % kCoopW, s = tid / kCoopW), s < NSEG, adds the partials of workgroups
          // s, s + NSEG, ... in that order, four loads in flight at a time; then the NSEG segment
          // sums in order.  Every workgroup forms the same sums in the same order.
@@ -300,14 +305,15 @@ __global__ __launch_bounds__(256) void qr_panel_coop_kernel(T *F, int64_t ldf, i
           const double *src = wp + k;
           int t = s;
           for (; t + 3 * NSEG < nw; t += 4 * NSEG) {
-            const double a0 = src[(size_t)t * kCoopW], a1 = src[(size_t)(t + NSEG) * kCoopW];
-            const double a2 = src[(size_t)(t + 2 * NSEG) * kCoopW], a3 = src[(size_t)(t + 3 * NSEG) * kCoopW];
+            const double a0 = ld_agent(src + (size_t)t * kCoopW), a1 = ld_agent(src + (size_t)(t + NSEG) * kCoopW);
+            const double a2 = ld_agent(src + (size_t)(t + 2 * NSEG) * kCoopW);
+            const double a3 = ld_agent(src + (size_t)(t + 3 * NSEG) * kCoopW);
             acc += a0;
             acc += a1;
             acc += a2;
             acc += a3;
           }
-          for (; t < nw; t += NSEG) acc += src[(size_t)t * kCoopW];
+          for (; t < nw; t += NSEG) acc += ld_agent(src + (size_t)t * kCoopW);
         }
         if (s < NSEG) seg[s][k] = acc;
       }
@@ -337,12 +343,15 @@ __global__ __launch_bounds__(256) void qr_panel_coop_kernel(T *F, int64_t ldf, i
       __syncthreads();
     }
   }
+  // an abandoned barrier wait leaves the panel NaN, so the solve ends in the non-finite-output error
+  const T poison = T(__builtin_nan(""));
 #pragma unroll
   for (int q = 0; q < KR; ++q)
     if (row[q] < m)
 #pragma unroll
       for (int k = 0; k < kPanel; ++k)
-        if (k < nbc) Fb[(int64_t)row[q] * ldf + j0 + k] = X(q, k);
+        if (k < nbc) Fb[(int64_t)row[q] * ldf + j0 + k] = aborted ? poison : X(q, k);
+  if (aborted && wi == 0 && tid < nbc) tb[j0 + tid] = poison;
 }
 
 static int device_cus() {
@@ -355,9 +364,9 @@ static int device_cus() {
   return cus;
 }
 
-// Rows per thread for the cooperative panel of a `batch` of matrices with `rows` panel rows, or 0
-// when even the largest setting would need more workgroups than there are CUs (one resident
-// workgroup per CU keeps the cooperative grid co-resident without relying on occupancy).
+// Rows per thread for the multi-workgroup panel of a `batch` of matrices with `rows` panel rows, or 0
+// when even the largest setting would need more workgroups than there are CUs (one workgroup per
+// CU keeps the whole grid resident without relying on occupancy).
 template <typename T>
 static int coop_rows_per_thread(int64_t batch, int rows) {
   const int cus = device_cus();
@@ -372,7 +381,7 @@ static int coop_rows_per_thread(int64_t batch, int rows) {
 template <typename T>
 static size_t coop_scr_bytes(int64_t batch, int m) {
   const int64_t nw = (m + 255) / 256;   // the largest nw any panel of this matrix can use
-  return sizeof(double) * (size_t)batch * CoopScr::doubles((int)nw);
+  return sizeof(double) * (size_t)batch * CoopScr::doubles((int)nw) + sizeof(unsigned) * (size_t)batch * kBarWords;
 }
 
 template <typename T, int KR>
@@ -385,12 +394,17 @@ static int launch_coop(T *F, int64_t ldf, int64_t sf, T *V, int64_t sv, T *tau, 
       return PSVD_ELAUNCH;
     attr = true;
   }
-  int nw = (int)((m - j0 + 256LL * KR - 1) / (256LL * KR));
-  void *args[] = {&F, &ldf, &sf, &V, &sv, &tau, &stau, &m, &j0, &nbc, &nw, &scr};
-  const hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<const void *>(qr_panel_coop_kernel<T, KR>),
-                                                  dim3((unsigned)(nw * batch)), dim3(256), args,
-                                                  (unsigned)coop_lds_bytes<T, KR>(), st);
-  return e == hipSuccess ? PSVD_OK : PSVD_ELAUNCH;
+  static const int limit = barrier_grid_limit(reinterpret_cast<const void *>(qr_panel_coop_kernel<T, KR>), 256,
+                                              coop_lds_bytes<T, KR>());
+  const int nw = (int)((m - j0 + 256LL * KR - 1) / (256LL * KR));
+  if (nw * batch > limit) return PSVD_ELAUNCH;
+  // barrier counters after the partials of the largest nw (coop_scr_bytes)
+  unsigned *bar = reinterpret_cast<unsigned *>(scr + (size_t)batch * CoopScr::doubles((m + 255) / 256));
+  hipLaunchKernelGGL(bar_reset_kernel, dim3(1), dim3(256), 0, st, bar, (int)(batch * kBarWords));
+  const size_t lds = coop_lds_bytes<T, KR>();
+  hipLaunchKernelGGL((qr_panel_coop_kernel<T, KR>), dim3((unsigned)(nw * batch)), dim3(256), lds, st, F, ldf, sf, V,
+                     sv, tau, stau, m, j0, nbc, nw, scr, bar);
+  return hipGetLastError() == hipSuccess ? PSVD_OK : PSVD_ELAUNCH;
 }
 
 // Compact-WY factor of one panel: T (nbc x nbc upper triangular, stored kPanel x kPanel) from

@@ -21,6 +21,20 @@ __global__ void fill_u64_kernel(unsigned long long *p, int64_t n, unsigned long 
   if (i < n) p[i] = v;
 }
 
+// clears a GridBarrier's counters in-stream before the launch that uses them
+__global__ void bar_reset_kernel(unsigned *ctr, int words) {
+  for (int i = threadIdx.x; i < words; i += blockDim.x) ctr[i] = 0u;
+}
+
+int barrier_grid_limit(const void *kernel, int threads, size_t lds) {
+  int dev = 0, cus = 0, occ = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+    return 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel, threads, lds) != hipSuccess || occ < 1) return 0;
+  return cus;
+}
+
 static int fill_u64(unsigned long long *p, int64_t n, unsigned long long v, hipStream_t st) {
   if (n <= 0) return PSVD_OK;
   hipLaunchKernelGGL(fill_u64_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n, v);

@@ -15,7 +15,7 @@ F32, F64, C64, C128 = 0, 1, 2, 3
 DTYPE_CODE = {torch.float32: F32, torch.float64: F64, torch.complex64: C64, torch.complex128: C128}
 
 OK, EINVAL, ELAUNCH, ETOOBIG = 0, 1, 2, 3
-INFO_OK, INFO_NOCONV, INFO_COMPLETION, INFO_NONFINITE, INFO_BADSHAPE = 0, 1, 2, 3, 4
+INFO_OK, INFO_NOCONV, INFO_COMPLETION, INFO_NONFINITE, INFO_BADSHAPE, INFO_BARRIER = 0, 1, 2, 3, 4, 5
 
 SCALE, CUT_FLOOR, NO_VECTORS, SLOT_SPREAD, ORDER_SORTED, BLOCK_NARROW, BLOCK_WIDE = 1, 2, 4, 8, 16, 32, 64
 BLOCK_FULL_PAIRS = 128
@@ -64,6 +64,9 @@ def raise_for_info(info, algorithm_error):
         raise ValueError('input contains NaN or infinity')
     if bool((codes == INFO_BADSHAPE).any()):
         raise ValueError('matrix descriptor outside 1 <= n <= m <= max_m, n <= max_n')
+    if bool((codes == INFO_BARRIER).any()):
+        raise RuntimeError('grid barrier wait abandoned: the launch could not keep all its workgroups resident '
+                           '(the GPU was shared with other work)')
     if bool((codes == INFO_NOCONV).any()):
         raise RuntimeError(algorithm_error)
     if bool((codes == INFO_COMPLETION).any()):

@@ -4,9 +4,9 @@ sequence of them) through the C ABI of ``include/pavelsvd.h``.
 Mirrors the names and return conventions of svdbench/standalone/primitives.py for the hot path:
 
     jacobi_svd   primitives.py:316-367  -> K1 (LDS-resident batched one-sided Jacobi) or,
-                                           for matrices beyond LDS, K2 (cooperative launch)
+                                           for matrices beyond LDS, K2 (single launch)
     qr           primitives.py:41-89    -> K4 (blocked Householder QR, xLARFG reflectors)
-    jacobi_eigh  primitives.py:370-427  -> K6 (two-sided Jacobi, LDS-resident or cooperative)
+    jacobi_eigh  primitives.py:370-427  -> K6 (two-sided Jacobi, LDS-resident or in HBM)
     cholesky     primitives.py:145-159  -> K5 (blocked Cholesky with the failure mask)
     trsm_lower / trsm_upper  :162-179   -> K5 (right-sided blocked solves, transposed form)
 
@@ -67,7 +67,7 @@ def jacobi_lds_uniform(x, *, tol=None, max_sweeps=60, scale=False, cut='referenc
 
 def jacobi_grid(x, *, tol=None, max_sweeps=60, scale=False, cut='reference', vectors=True):
     """One-sided Jacobi SVD of each tall matrix of ``x`` (b, m, n) too large for LDS: one
-    cooperative launch per matrix.  Same outputs as ``jacobi_lds_uniform``."""
+    single launch per matrix (grid barrier per round).  Same outputs as ``jacobi_lds_uniform``."""
     b, m, n = x.shape
     if m < n:
         raise ValueError('jacobi_grid takes tall matrices (m >= n)')
@@ -124,8 +124,8 @@ def jacobi_blocked(x, *, tol=None, max_sweeps=60, scale=False, cut='reference', 
 
 
 def jacobi_any(x, method='auto', **kw):
-    """K1 when the shape fits LDS; otherwise the blocked kernel (n >= 64) or the cooperative one.
-    Complex matrices beyond LDS: one matrix takes the complex cooperative kernel, a batch the batched
+    """K1 when the shape fits LDS; otherwise the blocked kernel (n >= 64) or the single-launch one.
+    Complex matrices beyond LDS: one matrix takes the complex single-launch kernel, a batch the batched
     complex kernel on workspace regions (both the reference's rotation order).  method 'grid' or
     'blocked' forces a large-matrix kernel."""
     b, m, n = x.shape
@@ -139,7 +139,7 @@ def jacobi_any(x, method='auto', **kw):
             raise ValueError(f'complex matrices beyond LDS take the grid kernel, not {method!r}')
         if b > 1 and method == 'auto':   # a batch: the batched kernel on workspace regions
             return jacobi_lds_uniform(x, **kw)
-        return jacobi_grid(x, **kw)       # one matrix: the cooperative kernel over the whole GPU
+        return jacobi_grid(x, **kw)       # one matrix: the single-launch kernel over the whole GPU
     if method == 'blocked' or (method == 'auto' and n >= 64):
         return jacobi_blocked(x, block_width=block_width, **kw)
     if method not in ('auto', 'grid'):

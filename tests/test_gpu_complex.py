@@ -57,7 +57,7 @@ def test_complex_random_matches_lapack_and_oracle(pkg, dtype, shape):
     rng = np.random.default_rng(shape[1] * 7 + shape[2])
     a = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(dtype)
     print(f'complex random {shape} {np.dtype(dtype).name}', flush=True)
-    # 96 x 96 complex128 (300 KB) is beyond LDS: the complex cooperative kernel solves it
+    # 96 x 96 complex128 (300 KB) is beyond LDS: the complex single-launch kernel solves it
     u, s, vh, info = pkg.svd(torch.from_numpy(a).to(DEV), 'jacobi', return_info=True)
     met = _metrics(a, (u, s, vh))
     bar = 1e-5 if dtype == np.complex64 else 1e-12
@@ -487,7 +487,7 @@ def test_complex_cholesky_and_solve(dtype, b, m, n):
 @pytest.mark.parametrize('dtype,shape,rank', [(np.complex64, (300, 130), None), (np.complex128, (200, 100), None),
                                               (np.complex64, (100, 300), None), (np.complex128, (180, 120), 40)])
 def test_complex_jacobi_beyond_lds(pkg, dtype, shape, rank):
-    """Complex one-sided Jacobi on matrices larger than LDS (the complex cooperative kernel, the
+    """Complex one-sided Jacobi on matrices larger than LDS (the complex single-launch kernel, the
     reference's rotation order): sigma, U, V against complex LAPACK; sigma against the oracle's
     restatement of the reference's solver within the oracle's own error (its sums run in the input
     dtype) and sweep counts within one of it for full-rank inputs (for a rank-deficient one the
@@ -525,7 +525,7 @@ def test_complex_jacobi_beyond_lds(pkg, dtype, shape, rank):
 def test_complex_solvers_beyond_lds(pkg, method, opts, dtype):
     """The complex solvers on a 400 x 140 matrix, whose QRs, QDWH stack, Cholesky factor, Gram and
     eigen problems exceed LDS (the kernels run on workspace regions, the Jacobi of R on the complex
-    cooperative kernel): sigma, U, V against complex LAPACK at the complex bars (gram_jacobi: the kept
+    single-launch kernel): sigma, U, V against complex LAPACK at the complex bars (gram_jacobi: the kept
     values above sqrt(eps n) sigma_1 at the Gram method's accuracy); QDWH iteration counts equal to the
     oracle's restatement of the reference's solver on the same input."""
     rng = np.random.default_rng(140)

@@ -211,7 +211,7 @@ def test_factory_plugin_hook(pkg):
     assert run.diagnostics['jacobi_sweeps'] >= 1 and run.metadata['vendor_factorizations'] is False
 
 
-# --- K2: cooperative one-sided Jacobi for matrices beyond LDS ---------------------------------
+# --- K2: single-launch one-sided Jacobi for matrices beyond LDS ---------------------------------
 def _prims():
     from importlib import import_module
     return import_module('pavel-s_svd_amd.standalone.primitives')

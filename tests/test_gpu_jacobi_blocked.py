@@ -1,9 +1,9 @@
 """GPU parity of the blocked one-sided Jacobi kernel (column blocks of 16 or 32: 32 x 32 or 64 x 64
-FP64 Gram subproblems, MFMA column updates) against FP64 LAPACK, the oracle and the cooperative
+FP64 Gram subproblems, MFMA column updates) against FP64 LAPACK, the oracle and the single-launch
 kernel that keeps the reference's rotation order.
 
 Bars: sigma error over sigma_1, orthogonality of U and V, reconstruction: 1e-5 (FP32) / 1e-12 (FP64);
-sigma equal to the cooperative kernel's within the same bars; the reference cut zeroes the same
+sigma equal to the single-launch kernel's within the same bars; the reference cut zeroes the same
 singular values as the oracle (+-2 at the cut boundary).
 """
 from importlib import import_module
@@ -49,7 +49,7 @@ def test_blocked_matches_lapack(P, dtype, m, n, width):
 
 
 @pytest.mark.parametrize('width', [16, 32])
-def test_blocked_equals_cooperative_kernel_and_cut(P, width):
+def test_blocked_equals_single_launch_kernel_and_cut(P, width):
     a = X.workload_single_jacobi()
     x = torch.from_numpy(a[None]).to(DEV)
     _, s1, _, i1 = P.jacobi_blocked(x, tol=1e-7, scale=True, cut='floor', block_width=width)
