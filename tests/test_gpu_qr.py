@@ -94,3 +94,32 @@ def test_solvers_take_batches_beyond_one_launch(method):
     assert tuple(s.shape) == (70000, 3) and tuple(u.shape) == (70000, 6, 3) and tuple(vh.shape) == (70000, 3, 3)
     assert np.abs(s.cpu().numpy() - ref).max() <= 2e-5 * ref.max()
     assert info['algorithm'] == method
+
+
+def _graded(m, n, cond, seed, dtype):
+    """A = U diag(sigma) V^T, sigma geometric from 1 to 1/cond, Haar U and V (seeded)."""
+    rng = np.random.default_rng(seed)
+    u = np.linalg.qr(rng.standard_normal((m, n)))[0]
+    v = np.linalg.qr(rng.standard_normal((n, n)))[0]
+    return ((u * np.geomspace(1.0, 1.0 / cond, n)) @ v.T).astype(dtype)
+
+
+@pytest.mark.parametrize('dtype,cond', [(np.float64, 1e2), (np.float64, 1e6), (np.float64, 1e11),
+                                        (np.float32, 1e2), (np.float32, 1e5)])
+def test_qr_panel_paths_by_condition(prims, dtype, cond):
+    """Tall panels across the CholeskyQR panel's branches: well conditioned (second pass to first order),
+    moderately conditioned (second Cholesky factor) and, in FP64, beyond its reach (G2 = Q1^T Q1 far from
+    I: the column-by-column panel).  Every branch must give LAPACK's factorisation: Q R = A and
+    Q^T Q = I to the dtype's bar, R's diagonal signs equal LAPACK's (xLARFG convention), and R equal to
+    LAPACK's R within the condition number's amplification of the bar."""
+    m, n = 2000, 64
+    a = _graded(m, n, cond, 7, dtype)
+    q, r = prims.qr(torch.from_numpy(a[None]).to(DEV))
+    q, r = q.cpu().numpy()[0].astype(np.float64), r.cpu().numpy()[0].astype(np.float64)
+    ro = scipy.linalg.qr(a.astype(np.float64), mode='economic')[1]
+    bar = 1e-12 if dtype == np.float64 else 2e-5
+    scale = np.abs(a).max()
+    assert np.max(np.abs(q @ r - a)) / scale < bar * np.sqrt(m)
+    assert np.max(np.abs(q.T @ q - np.eye(n))) < bar * np.sqrt(m)
+    assert np.array_equal(np.sign(np.diag(r)), np.sign(np.diag(ro)))
+    assert np.max(np.abs(r - ro)) / np.abs(ro).max() < bar * np.sqrt(m) * cond
