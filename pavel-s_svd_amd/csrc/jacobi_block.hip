@@ -27,6 +27,9 @@ namespace psvd {
 
 constexpr int kJbThreads = 256;
 
+// the rotation flag of sweep sw (1-based): odd sweeps [0], even sweeps [3]
+__host__ __device__ inline int bj_flag_slot(int sw) { return (sw & 1) ? 0 : 3; }
+
 // Block width KB: subproblem size KS, LDS row stride, threads of the pair-sweep kernel (one
 // upper-triangle 2 x 2 block of a rotation round per thread: 528 for KB = 32, 136 for KB = 16; for
 // KB = 16 the 512 Qt entries of a round get threads of their own, 1024 in all).
@@ -46,10 +49,13 @@ struct BjParams {
   int full_pairs;           // every round's pair sweeps over all pairs (else the first round's only)
   double tol;
   T *W, *V, *Q;             // W: m x N, V: N x N (row-major, ld N); Q: npair x 64 x 64 (W_IJ <- W_IJ Q)
+  T *W2;                    // W's second buffer: round g's update reads one buffer and writes the other
+  unsigned long long *fmx;  // [2]: max column norm^2 bits at the start of the round, by round parity
   double *Sp;               // npair x splits x 64 x 64 Gram partials
   int *act;                 // npair
   unsigned long long *slot;  // [0] max|a| bits, [1] max column norm^2 bits (this sweep)
-  int *flag;                // [0] rotation in this sweep, [1] non-finite input, [2] completion status
+  int *flag;                // [0] / [3] rotation in an odd / even sweep, [1] non-finite input, [2] completion
+                            // status, [4] done: a sweep without a rotation came before (the kernels return)
   double *sig;              // N
   int *ord;                 // N
   int32_t *nk;              // 1
@@ -94,7 +100,7 @@ __global__ __launch_bounds__(kJbThreads) void bj_load_kernel(BjParams<T> P) {
   }
 }
 
-// max_j |w_j|^2 for the floor of this sweep (slot[1] cleared before the launch)
+// max_j |w_j|^2 of the loaded columns: the first round's floor (fmx[0], cleared before the launch)
 template <typename T>
 __global__ __launch_bounds__(kJbThreads) void bj_colmax_kernel(BjParams<T> P) {
   // 16 columns per workgroup, 16 row groups; thread (g, c) sums rows g, g + 16, ... with four
@@ -127,69 +133,11 @@ __global__ __launch_bounds__(kJbThreads) void bj_colmax_kernel(BjParams<T> P) {
     if (threadIdx.x < NC)
       for (int x = 0; x < NG; ++x) tot += part[x][threadIdx.x];
     tot = wave_max_fast(tot);
-    if (threadIdx.x == 0) atomicMax(&P.slot[1], (unsigned long long)__double_as_longlong(tot));
+    if (threadIdx.x == 0) atomicMax(&P.fmx[0], (unsigned long long)__double_as_longlong(tot));
   }
 }
 
-// Gram partial of pair p over the rows of slice z: KS x KS, FP64 MFMA (16x16x4), wave w owns the
-// KB x KB quadrant (w / 2, w % 2), TI x TI MFMA tiles.
 constexpr int kGramRows = 64;   // rows of W staged per step of the Gram partial
-
-template <typename T, int KB>
-__device__ __forceinline__ void bj_gram_body(const BjParams<T> &P, int p, int z, int r,
-                                             double (*Ws)[BjShape<KB>::LD]) {
-  constexpr int KR = kGramRows, KS = BjShape<KB>::KS, TI = KB / 16;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int lr = lane & 15, lk = lane >> 4, wm = (wave >> 1) * KB, wn = (wave & 1) * KB;
-  int I, J;
-  rr_pair(r, p, P.nb, I, J);
-  const int rows_per = (P.m + P.splits - 1) / P.splits;
-  const int r0 = z * rows_per, r1 = r0 + rows_per < P.m ? r0 + rows_per : P.m;
-  j4d acc[TI][TI];
-#pragma unroll
-  for (int i = 0; i < TI; ++i)
-#pragma unroll
-    for (int j = 0; j < TI; ++j)
-#pragma unroll
-      for (int q = 0; q < 4; ++q) acc[i][j][q] = 0.0;
-  using V16 = Vec16<T>;
-  constexpr int VE = V16::n, RV = KS / VE;   // 16-byte vectors: a row's two KB-column blocks
-  for (int k0 = r0; k0 < r1; k0 += KR) {
-    __syncthreads();
-    for (int e = tid; e < KR * RV; e += kJbThreads) {
-      const int rr = e / RV, c = (e - rr * RV) * VE, row = k0 + rr;
-      if (row < r1) {
-        const V16 x = *reinterpret_cast<const V16 *>(P.W + (int64_t)row * P.N + bj_col<KB>(I, J, c));
-#pragma unroll
-        for (int u = 0; u < VE; ++u) Ws[rr][c + u] = (double)x.v[u];
-      } else {
-#pragma unroll
-        for (int u = 0; u < VE; ++u) Ws[rr][c + u] = 0.0;
-      }
-    }
-    __syncthreads();
-#pragma unroll 4
-    for (int k = 0; k < KR; k += 4) {
-      double a[TI], b[TI];
-#pragma unroll
-      for (int i = 0; i < TI; ++i) a[i] = Ws[k + lk][wm + 16 * i + lr];   // A = W^T: A[col][row]
-#pragma unroll
-      for (int j = 0; j < TI; ++j) b[j] = Ws[k + lk][wn + 16 * j + lr];
-#pragma unroll
-      for (int i = 0; i < TI; ++i)
-#pragma unroll
-        for (int j = 0; j < TI; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
-    }
-  }
-  double *out = P.Sp + ((size_t)p * P.splits + z) * KS * KS;
-#pragma unroll
-  for (int i = 0; i < TI; ++i)
-#pragma unroll
-    for (int j = 0; j < TI; ++j)
-#pragma unroll
-      for (int q = 0; q < 4; ++q)   // FP64 MFMA D layout: register q of lane l is row 4 q + l / 16
-        out[(wm + 16 * i + 4 * q + lk) * KS + wn + 16 * j + lr] = acc[i][j][q];
-}
 
 template <typename T, int KB>
 __host__ __device__ inline size_t bj_inner_lds() {
@@ -201,15 +149,19 @@ __host__ __device__ inline size_t bj_inner_lds() {
 // per upper-triangle 2 x 2 block of a rotation round (KB (KB + 1) / 2 of them), so no thread
 // carries a second block on the round's critical path.
 template <typename T, int KB>
-__global__ __launch_bounds__(BjShape<KB>::INNER) void bj_inner_kernel(BjParams<T> P, int r) {
+__global__ __launch_bounds__(BjShape<KB>::INNER) void bj_inner_kernel(BjParams<T> P, int r, int g, int fslot) {
   constexpr int KS = BjShape<KB>::KS, LD = BjShape<KB>::LD, NTI = BjShape<KB>::INNER;
   extern __shared__ __align__(16) double smem_bj[];
   double2 *cs = reinterpret_cast<double2 *>(smem_bj);   // (c, s) of pair k of the round
   double *S = reinterpret_cast<double *>(cs + KB), *Qt = S + KS * LD;
   int2 *pq = reinterpret_cast<int2 *>(Qt + KS * LD);   // (p, q) of pair k
   const int p = blockIdx.x, tid = threadIdx.x;
+  if (P.flag[4]) return;   // a sweep launched after the one that rotated nowhere
   const double eps = Eps<T>::value, tol = P.tol > 0.0 ? P.tol : 10.0 * eps;
-  const double floor_ = eps * eps * __longlong_as_double((long long)P.slot[1]);
+  // the reference's floor eps^2 max_j |w_j|^2 at the start of this round (primitives.py:337): the largest
+  // diagonal the previous round's pair sweeps left (their column norms after its rotations), or for the
+  // first round the norms of the loaded columns
+  const double floor_ = eps * eps * __longlong_as_double((long long)P.fmx[g & 1]);
   for (int e = tid; e < KS * KS; e += NTI) {
     const int i = e / KS, j = e - i * KS;
     const double *sp = P.Sp + (size_t)p * P.splits * KS * KS + e;
@@ -325,9 +277,14 @@ __global__ __launch_bounds__(BjShape<KB>::INNER) void bj_inner_kernel(BjParams<T
       const int i = e / KS, j = e - i * KS;
       P.Q[(size_t)p * KS * KS + e] = (T)Qt[j * LD + i];   // Q = Qt^T
     }
+  __syncthreads();
+  if (tid < kWave) {   // the next round's floor: the largest column norm^2 after this round's rotations
+    const double d = wave_max_fast(tid < KS ? S[tid * LD + tid] : 0.0);
+    if (tid == 0) atomicMax(&P.fmx[(g + 1) & 1], (unsigned long long)__double_as_longlong(d));
+  }
   if (tid == 0) {
     P.act[p] = any;
-    if (any) atomicOr(&P.flag[0], 1);
+    if (any) atomicOr(&P.flag[fslot], 1);
   }
 }
 
@@ -425,35 +382,183 @@ __device__ __forceinline__ void bj_update_body(const BjParams<T> &P, int p, int 
   }
 }
 
-// W tiles of round r: blockIdx.x = pair, blockIdx.y = 64-row tile of W.
+// Row slice z of the two blocks of pair p of round r, as round rprev left them: each block X is rebuilt
+// from the row slice of its round-rprev pair (old buffer Wo) times that pair's rotation -- the half of Q
+// that makes X (the identity when the pair was inactive) -- on MFMA, written to the new buffer Wn and
+// kept in LDS in FP64, where the Gram partial of round r is formed from it (rprev < 0: no update, the
+// rows come straight from Wo).  Every block of round r is made by exactly one workgroup per slice, and
+// the old buffer is only read, so no workgroup reads a row another one writes.
 template <typename T, int KB>
-__global__ __launch_bounds__(kJbThreads) void bj_update_kernel(BjParams<T> P, int r) {
-  extern __shared__ double smem_bu2[];
-  bj_update_body<T, KB>(P, (int)blockIdx.x, (int)blockIdx.y, r, smem_bu2);
+__device__ __forceinline__ void bj_gram_update_body(const BjParams<T> &P, int p, int z, int r, int rprev, const T *Wo,
+                                                    T *Wn, bool gram, double *smem) {
+  constexpr int KR = kGramRows, KS = BjShape<KB>::KS, LD = BjShape<KB>::LD, TI = KB / 16;
+  double (*Ws)[LD] = reinterpret_cast<double (*)[LD]>(smem);               // KR x KS, FP64 new rows
+  T (*Xs)[KR][LD] = reinterpret_cast<T (*)[KR][LD]>(Ws + KR);              // [2][KR][KS] old pair rows per half
+  T (*Qs)[KS][KB + 1] = reinterpret_cast<T (*)[KS][KB + 1]>(Xs + 2);       // [2][KS][KB] rotation halves
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  int I, J;
+  rr_pair(r, p, P.nb, I, J);
+  // per half h (block X = I, J): its round-rprev pair (blocks A < B) and whether X is A
+  int pa[2], pb[2], pside[2], pact[2], pp[2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int X = h == 0 ? I : J;
+    if (rprev >= 0) {
+      // seat of player X in round rprev (closed form of rr_pair), then its pair
+      const int nm1 = P.nb - 1;
+      int seat = 0;
+      if (X > 0) {
+        seat = X - 1 + rprev;
+        if (seat >= nm1) seat -= nm1;
+        seat += 1;
+      }
+      pp[h] = seat < P.nb - 1 - seat ? seat : P.nb - 1 - seat;
+      rr_pair(rprev, pp[h], P.nb, pa[h], pb[h]);
+      pside[h] = X == pa[h] ? 0 : 1;
+      pact[h] = P.act[pp[h]];
+    } else {
+      pp[h] = 0;
+      pa[h] = pb[h] = X;
+      pside[h] = 0;
+      pact[h] = 0;
+    }
+  }
+  const int rows_per = (P.m + P.splits - 1) / P.splits;
+  const int r0 = z * rows_per, r1 = r0 + rows_per < P.m ? r0 + rows_per : P.m;
+  using V16 = Vec16<T>;
+  constexpr int VE = V16::n, RV = KS / VE, NPF = 2 * KR * RV / kJbThreads;
+  static_assert(2 * KR * RV % kJbThreads == 0, "the staged rows tile the workgroup");
+  // the old rows of a chunk (Xs[h] = [W_A W_B] of half h's previous pair), fetched into registers one
+  // chunk ahead so their latency overlaps the previous chunk's products
+  V16 pf[NPF];
+  auto fetch = [&](int k0) {
+#pragma unroll
+    for (int j = 0; j < NPF; ++j) {
+      const int e = tid + j * kJbThreads, h = e / (KR * RV), f = e - h * KR * RV, rr = f / RV, c = (f - rr * RV) * VE;
+      const int row = k0 + rr;
+      if (row < r1) {
+        pf[j] = *reinterpret_cast<const V16 *>(Wo + (int64_t)row * P.N + (c < KB ? pa[h] * KB + c : pb[h] * KB + c - KB));
+      } else {
+#pragma unroll
+        for (int u = 0; u < VE; ++u) pf[j].v[u] = T(0);
+      }
+    }
+  };
+  fetch(r0);
+  // the rotation halves (identity for an inactive or absent previous pair)
+  for (int e = tid; e < 2 * KS * KB; e += kJbThreads) {
+    const int h = e / (KS * KB), f = e - h * KS * KB, i = f / KB, j = f - i * KB;
+    Qs[h][i][j] = pact[h] ? P.Q[(size_t)pp[h] * KS * KS + (size_t)i * KS + pside[h] * KB + j]
+                          : (i == pside[h] * KB + j ? T(1) : T(0));
+  }
+  j4d acc[TI][TI];
+#pragma unroll
+  for (int i = 0; i < TI; ++i)
+#pragma unroll
+    for (int j = 0; j < TI; ++j)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) acc[i][j][q] = 0.0;
+  for (int k0 = r0; k0 < r1; k0 += KR) {
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < NPF; ++j) {
+      const int e = tid + j * kJbThreads, h = e / (KR * RV), f = e - h * KR * RV, rr = f / RV, c = (f - rr * RV) * VE;
+#pragma unroll
+      for (int u = 0; u < VE; ++u) Xs[h][rr][c + u] = pf[j].v[u];
+    }
+    if (k0 + KR < r1) fetch(k0 + KR);
+    __syncthreads();
+    // new rows: half h = Xs[h] (KR x KS) times Qs[h] (KS x KB); wave w: rows 16 (w + 4 t) .. of both halves
+    {
+      typename BjMfma<T>::V nacc[2][TI];
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int j = 0; j < TI; ++j)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) nacc[h][j][q] = 0;
+      const int rt = 16 * wave;   // each wave one 16-row tile of the 64-row chunk
+#pragma unroll 4
+      for (int k = 0; k < KS; k += 4) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const T a = Xs[h][rt + lr][k + lk];
+#pragma unroll
+          for (int j = 0; j < TI; ++j) nacc[h][j] = BjMfma<T>::op(a, Qs[h][k + lk][16 * j + lr], nacc[h][j]);
+        }
+      }
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int j = 0; j < TI; ++j)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int rr = rt + BjMfma<T>::row(q, lk), row = k0 + rr, c = 16 * j + lr;
+            const T v = (T)nacc[h][j][q];
+            Ws[rr][h * KB + c] = row < r1 ? (double)v : 0.0;
+            if (row < r1 && rprev >= 0) Wn[(int64_t)row * P.N + (h == 0 ? I : J) * KB + c] = v;
+          }
+    }
+    if (!gram) continue;   // the last round's update alone (the next loads wait at the loop's barrier)
+    __syncthreads();
+#pragma unroll 4
+    for (int k = 0; k < KR; k += 4) {
+      double a[TI], b[TI];
+#pragma unroll
+      for (int i = 0; i < TI; ++i) a[i] = Ws[k + lk][(wave >> 1) * KB + 16 * i + lr];
+#pragma unroll
+      for (int j = 0; j < TI; ++j) b[j] = Ws[k + lk][(wave & 1) * KB + 16 * j + lr];
+#pragma unroll
+      for (int i = 0; i < TI; ++i)
+#pragma unroll
+        for (int j = 0; j < TI; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+  }
+  if (!gram) return;
+  const int wm = (wave >> 1) * KB, wn = (wave & 1) * KB;
+  double *out = P.Sp + ((size_t)p * P.splits + z) * KS * KS;
+#pragma unroll
+  for (int i = 0; i < TI; ++i)
+#pragma unroll
+    for (int j = 0; j < TI; ++j)
+#pragma unroll
+      for (int q = 0; q < 4; ++q)   // FP64 MFMA D layout: register q of lane l is row 4 q + l / 16
+        out[(wm + 16 * i + 4 * q + lk) * KS + wn + 16 * j + lr] = acc[i][j][q];
 }
 
 template <typename T, int KB>
-__host__ __device__ inline size_t bj_round_lds() {
-  const size_t gram = sizeof(double) * kGramRows * BjShape<KB>::LD, upd = bj_update_lds<T, KB>();
-  return gram > upd ? gram : upd;
+__host__ __device__ inline size_t bj_fused_lds() {
+  constexpr int KR = kGramRows, KS = BjShape<KB>::KS, LD = BjShape<KB>::LD;
+  return sizeof(double) * KR * LD + sizeof(T) * (2 * (size_t)KR * LD + 2 * (size_t)KS * (KB + 1));
 }
 
-// First launch of round r: workgroups [0, ngram) form the Gram partials of round r (pair
-// b % npair, row slice b / npair); the rest, when vr >= 0, apply round vr's rotations to the V
-// tiles (pair, tile) -- V is read by no kernel of the sweep, and Q and the activity flags still
-// hold round vr until round r's subproblem sweeps run, so the V products of a round run beside
-// the next round's Gram partials.
+// One launch per round g (tournament round r): workgroups [0, ngram) rebuild their row slice of the two
+// blocks of pair b % npair from round rprev's rotations (old buffer Wo -> new buffer Wn) and, with gram,
+// form that slice's Gram partial of round r; the rest apply round vr's rotations to the V tiles (pair,
+// tile) -- V is read by no kernel of the sweep, and Q and the activity flags still hold the previous
+// round until this round's pair sweeps run.  Workgroup 0 clears the floor slot the round's pair sweeps
+// will publish into (clear >= 0).  Inside a sweep (`in_sweep`) nothing is done once the done flag is set.
 template <typename T, int KB>
-__global__ __launch_bounds__(kJbThreads) void bj_round_kernel(BjParams<T> P, int r, int ngram, int vr, int mt,
-                                                             int nt) {
-  extern __shared__ double smem_br2[];
+__global__ __launch_bounds__(kJbThreads) void bj_fused_kernel(BjParams<T> P, int r, int rprev, const T *Wo, T *Wn,
+                                                             int ngram, int gram, int vr, int mt, int nt, int clear,
+                                                             int in_sweep) {
+  extern __shared__ double smem_bf[];
   const int b = (int)blockIdx.x, npair = P.nb / 2;
+  if (in_sweep && P.flag[4]) return;
+  if (b == 0 && threadIdx.x == 0 && clear >= 0) P.fmx[clear] = 0ull;
   if (b < ngram) {
-    bj_gram_body<T, KB>(P, b % npair, b / npair, r, reinterpret_cast<double (*)[BjShape<KB>::LD]>(smem_br2));
+    bj_gram_update_body<T, KB>(P, b % npair, b / npair, r, rprev, Wo, Wn, gram != 0, smem_bf);
   } else {
     const int t = b - ngram;
-    bj_update_body<T, KB>(P, t / nt, mt + t % nt, vr, smem_br2);
+    bj_update_body<T, KB>(P, t / nt, mt + t % nt, vr, smem_bf);
   }
+}
+
+template <typename T, int KB>
+__host__ __device__ inline size_t bj_fused_launch_lds() {
+  const size_t a = bj_fused_lds<T, KB>(), b = bj_update_lds<T, KB>();
+  return a > b ? a : b;
 }
 
 template <typename T>
@@ -542,23 +647,26 @@ __global__ void bj_status_kernel(int32_t *info, const int *cstatus) {
   if (threadIdx.x == 0 && info[1] == PSVD_INFO_OK && *cstatus != 0) info[1] = PSVD_INFO_COMPLETION;
 }
 
-__global__ void bj_reset_kernel(unsigned long long *slot, int *flag, int what) {
+// what & 1: the solve's start (every slot and flag cleared); what & 2: the start of sweep `sw`, whose
+// rotation flag is cleared, and which is marked done when sweep sw - 1 rotated nowhere
+__global__ void bj_reset_kernel(unsigned long long *slot, int *flag, unsigned long long *fmx, int what, int sw) {
   if (threadIdx.x == 0) {
     if (what & 1) {
-      slot[0] = 0ull;
-      flag[1] = 0;
-      flag[2] = 0;
+      slot[0] = slot[1] = 0ull;
+      fmx[0] = fmx[1] = 0ull;
+      for (int i = 0; i < 5; ++i) flag[i] = 0;
     }
     if (what & 2) {
-      slot[1] = 0ull;
-      flag[0] = 0;
+      if (sw > 1 && flag[bj_flag_slot(sw - 1)] == 0) flag[4] = 1;
+      flag[bj_flag_slot(sw)] = 0;
     }
   }
 }
 
+// Row slices per pair: about 1024 workgroups per round, each over at least one 64-row chunk.
 static int bj_splits(int m, int npair) {
-  int s = (256 + npair - 1) / npair;
-  const int smax = (m + 127) / 128;   // at least 128 rows per slice
+  int s = (1024 + npair - 1) / npair;
+  const int smax = (m + kGramRows - 1) / kGramRows;
   if (s > smax) s = smax;
   if (s > 64) s = 64;
   return s < 1 ? 1 : s;
@@ -569,9 +677,9 @@ size_t bj_ws_bytes_kb(int m, int n) {
   constexpr int KS = BjShape<KB>::KS;
   const int N = (n + KS - 1) / KS * KS, npair = N / KS;
   auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
-  return al(sizeof(T) * (size_t)m * N) + al(sizeof(T) * (size_t)N * N) + al(sizeof(T) * (size_t)npair * KS * KS) +
+  return al(sizeof(T) * (size_t)m * N) * 2 + al(sizeof(T) * (size_t)N * N) + al(sizeof(T) * (size_t)npair * KS * KS) +
          al(sizeof(double) * (size_t)npair * bj_splits(m, npair) * KS * KS) + al(sizeof(int) * npair) + al(8 * 2) +
-         al(4 * 3) + al(sizeof(double) * N) + al(sizeof(int) * N) + al(4) +
+         al(4 * 5) + al(8 * 2) + al(sizeof(double) * N) + al(sizeof(int) * N) + al(4) +
          al(psvd_complete_workspace_bytes(1, m, n));
 }
 
@@ -591,10 +699,8 @@ static int run_bj(int m, int n, const T *a, T *u, T *s, T *vh, int32_t *info, do
   if (!attr) {
     if (hipFuncSetAttribute((const void *)bj_inner_kernel<T, KB>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)bj_inner_lds<T, KB>()) != hipSuccess ||
-        hipFuncSetAttribute((const void *)bj_update_kernel<T, KB>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)bj_update_lds<T, KB>()) != hipSuccess ||
-        hipFuncSetAttribute((const void *)bj_round_kernel<T, KB>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)bj_round_lds<T, KB>()) != hipSuccess)
+        hipFuncSetAttribute((const void *)bj_fused_kernel<T, KB>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)bj_fused_launch_lds<T, KB>()) != hipSuccess)
       return PSVD_ELAUNCH;
     attr = true;
   }
@@ -621,12 +727,14 @@ static int run_bj(int m, int n, const T *a, T *u, T *s, T *vh, int32_t *info, do
     return q;
   };
   P.W = reinterpret_cast<T *>(take(sizeof(T) * (size_t)m * P.N));
+  P.W2 = reinterpret_cast<T *>(take(sizeof(T) * (size_t)m * P.N));
   P.V = reinterpret_cast<T *>(take(sizeof(T) * (size_t)P.N * P.N));
   P.Q = reinterpret_cast<T *>(take(sizeof(T) * (size_t)npair * KS * KS));
   P.Sp = reinterpret_cast<double *>(take(sizeof(double) * (size_t)npair * P.splits * KS * KS));
   P.act = reinterpret_cast<int *>(take(sizeof(int) * npair));
   P.slot = reinterpret_cast<unsigned long long *>(take(8 * 2));
-  P.flag = reinterpret_cast<int *>(take(4 * 3));
+  P.flag = reinterpret_cast<int *>(take(4 * 5));
+  P.fmx = reinterpret_cast<unsigned long long *>(take(8 * 2));
   P.sig = reinterpret_cast<double *>(take(sizeof(double) * P.N));
   P.ord = reinterpret_cast<int *>(take(sizeof(int) * P.N));
   P.nk = reinterpret_cast<int32_t *>(take(4));
@@ -636,39 +744,94 @@ static int run_bj(int m, int n, const T *a, T *u, T *s, T *vh, int32_t *info, do
     const int64_t b = (items + kJbThreads - 1) / kJbThreads;
     return (unsigned)(b < 4096 ? (b > 0 ? b : 1) : 4096);
   };
-  hipLaunchKernelGGL(bj_reset_kernel, dim3(1), dim3(64), 0, st, P.slot, P.flag, 3);
+  hipLaunchKernelGGL(bj_reset_kernel, dim3(1), dim3(64), 0, st, P.slot, P.flag, P.fmx, 1, 0);
   hipLaunchKernelGGL(bj_absmax_kernel<T>, dim3(blocks((int64_t)m * n)), dim3(kJbThreads), 0, st, P);
   hipLaunchKernelGGL(bj_load_kernel<T>, dim3(blocks((int64_t)m * P.N + (int64_t)P.N * P.N)), dim3(kJbThreads), 0, st,
                      P);
-  int sweeps = 0, converged = 0, vr = -1;
+  // the first round's floor from the loaded columns; later rounds take it from the pair sweeps before them
+  hipLaunchKernelGGL(bj_colmax_kernel<T>, dim3((n + 15) / 16), dim3(kJbThreads), 0, st, P);
   const int ngram = npair * P.splits, nvt = npair * nt;
-  for (int sw = 1; sw <= max_sweeps; ++sw) {
-    sweeps = sw;
-    hipLaunchKernelGGL(bj_reset_kernel, dim3(1), dim3(64), 0, st, P.slot, P.flag, 2);
-    hipLaunchKernelGGL(bj_colmax_kernel<T>, dim3((n + 15) / 16), dim3(kJbThreads), 0, st, P);
-    for (int r = 0; r < P.nb - 1; ++r) {
-      // Gram partials of round r beside the V tiles of the previous round (none before the first)
-      hipLaunchKernelGGL((bj_round_kernel<T, KB>), dim3((unsigned)(ngram + (vr >= 0 ? nvt : 0))), dim3(kJbThreads),
-                         (bj_round_lds<T, KB>()), st, P, r, ngram, vr, mt, nt);
+  T *buf[2] = {P.W, P.W2};   // round g reads buf[(g - 1) & 1] (round g - 1's input) and writes buf[g & 1]
+  const size_t flds = bj_fused_launch_lds<T, KB>();
+  // Sweep sw + 1 is queued before the host reads whether sweep sw rotated, so the GPU does not idle
+  // through the host's wait; when sweep sw rotated nowhere, sweep sw + 1's kernels find the done flag
+  // set and return, and the solve continues from the state sweep sw left (its round count g).
+  int g = 0, vr = -1;
+  auto enqueue_sweep = [&](int sw, int *host_flag, hipEvent_t ev) {
+    hipLaunchKernelGGL(bj_reset_kernel, dim3(1), dim3(64), 0, st, P.slot, P.flag, P.fmx, 2, sw);
+    for (int r = 0; r < P.nb - 1; ++r, ++g) {
+      // round g - 1's rotations applied to W and this round's Gram partials, beside round g - 1's V tiles
+      const int rprev = g == 0 ? -1 : (r > 0 ? r - 1 : P.nb - 2);
+      const T *wo = g == 0 ? buf[0] : buf[(g - 1) & 1];
+      hipLaunchKernelGGL((bj_fused_kernel<T, KB>), dim3((unsigned)(ngram + (vr >= 0 ? nvt : 0))), dim3(kJbThreads), flds,
+                         st, P, r, rprev, wo, buf[g & 1], ngram, 1, vr, mt, nt, (g + 1) & 1, 1);
       hipLaunchKernelGGL((bj_inner_kernel<T, KB>), dim3(npair), dim3(BjShape<KB>::INNER), (bj_inner_lds<T, KB>()), st,
-                         P, r);
-      hipLaunchKernelGGL((bj_update_kernel<T, KB>), dim3(npair, mt), dim3(kJbThreads), (bj_update_lds<T, KB>()), st,
-                         P, r);
+                         P, r, g, bj_flag_slot(sw));
       if (nt > 0) vr = r;
     }
-    if (hipGetLastError() != hipSuccess) return PSVD_ELAUNCH;
-    int active = 0;
-    if (hipMemcpyAsync(&active, P.flag, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess) return PSVD_ELAUNCH;
-    if (hipStreamSynchronize(st) != hipSuccess) return PSVD_ELAUNCH;
-    if (!active) {
+    if (hipMemcpyAsync(host_flag, &P.flag[bj_flag_slot(sw)], sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipEventRecord(ev, st) != hipSuccess)
+      return false;
+    return hipGetLastError() == hipSuccess;
+  };
+  // pinned flag words for the asynchronous reads (one pair per host thread, kept for the process)
+  thread_local int *pinned = nullptr;
+  if (!pinned && hipHostMalloc((void **)&pinned, 2 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
+    pinned = nullptr;
+    return PSVD_ELAUNCH;
+  }
+  // the two sweep events; on leaving, the host waits for the last queued flag read (the pinned words are
+  // reused by this thread's next solve) and releases them
+  struct SweepEvents {
+    hipEvent_t e[2] = {nullptr, nullptr};
+    int wait = -1;
+    ~SweepEvents() {
+      if (wait >= 0) (void)hipEventSynchronize(e[wait]);
+      for (hipEvent_t x : e)
+        if (x) (void)hipEventDestroy(x);
+    }
+  } sev;
+  hipEvent_t *ev = sev.e;
+  if (hipEventCreateWithFlags(&ev[0], hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&ev[1], hipEventDisableTiming) != hipSuccess)
+    return PSVD_ELAUNCH;
+  int sweeps = 0, converged = 0, rc = PSVD_OK;
+  int g_end[2] = {0, 0}, vr_end[2] = {-1, -1};   // round count and V-tile round after sweep sw, by sw & 1
+  if (max_sweeps >= 1) {
+    sev.wait = 1;
+    if (!enqueue_sweep(1, &pinned[1], ev[1])) rc = PSVD_ELAUNCH;
+    g_end[1] = g;
+    vr_end[1] = vr;
+  }
+  for (int sw = 1; rc == PSVD_OK && sw <= max_sweeps; ++sw) {
+    if (sw < max_sweeps) {
+      sev.wait = (sw + 1) & 1;
+      if (!enqueue_sweep(sw + 1, &pinned[(sw + 1) & 1], ev[(sw + 1) & 1])) {
+        rc = PSVD_ELAUNCH;
+        break;
+      }
+      g_end[(sw + 1) & 1] = g;
+      vr_end[(sw + 1) & 1] = vr;
+    }
+    if (hipEventSynchronize(ev[sw & 1]) != hipSuccess) {
+      rc = PSVD_ELAUNCH;
+      break;
+    }
+    sweeps = sw;
+    if (!pinned[sw & 1]) {
       converged = 1;
+      g = g_end[sw & 1];
+      vr = vr_end[sw & 1];
       break;
     }
   }
-  // the last round's rotations on V
-  if (vr >= 0 && nvt > 0)
-    hipLaunchKernelGGL((bj_round_kernel<T, KB>), dim3((unsigned)nvt), dim3(kJbThreads), (bj_round_lds<T, KB>()), st, P,
-                       0, 0, vr, mt, nt);
+  if (rc != PSVD_OK) return rc;
+  // the last round's rotations on W (no Gram) and V; the tail reads W from the buffer they go to
+  if (g > 0) {
+    hipLaunchKernelGGL((bj_fused_kernel<T, KB>), dim3((unsigned)(ngram + (vr >= 0 ? nvt : 0))), dim3(kJbThreads), flds,
+                       st, P, 0, P.nb - 2, buf[(g - 1) & 1], buf[g & 1], ngram, 0, vr, mt, nt, -1, 0);
+    P.W = buf[g & 1];
+  }
   hipLaunchKernelGGL(bj_sigma_kernel<T>, dim3((n + 15) / 16), dim3(kJbThreads), 0, st, P);
   hipLaunchKernelGGL(bj_order_kernel<T>, dim3((n + kJbThreads - 1) / kJbThreads), dim3(kJbThreads), 0, st, P);
   hipLaunchKernelGGL(bj_out_kernel<T>, dim3(blocks((int64_t)m * n)), dim3(kJbThreads), 0, st, P, sweeps, converged);
