@@ -48,11 +48,11 @@ struct BjParams {
   int m, n, N, nb, splits, inner_sweeps, flags;
   int full_pairs;           // every round's pair sweeps over all pairs (else the first round's only)
   double tol;
-  T *W, *V, *Q;             // W: m x N, V: N x N (row-major, ld N); Q: npair x 64 x 64 (W_IJ <- W_IJ Q)
+  T *W, *V, *Q;             // W: m x N, V: N x N (row-major, ld N); Q: [2][npair][KS][KS] by round parity
   T *W2;                    // W's second buffer: round g's update reads one buffer and writes the other
   unsigned long long *fmx;  // [2]: max column norm^2 bits at the start of the round, by round parity
   double *Sp;               // npair x splits x 64 x 64 Gram partials
-  int *act;                 // npair
+  int *act;                 // [2][npair] by round parity: the pair rotated in its sweep
   unsigned long long *slot;  // [0] max|a| bits, [1] max column norm^2 bits (this sweep)
   int *flag;                // [0] / [3] rotation in an odd / even sweep, [1] non-finite input, [2] completion
                             // status, [4] done: a sweep without a rotation came before (the kernels return)
@@ -145,11 +145,16 @@ __host__ __device__ inline size_t bj_inner_lds() {
   return sizeof(double) * (2 * KB + 2 * (size_t)KS * LD) + sizeof(int) * 2 * KB;
 }
 
-// Sweeps of S_p (FP64, in LDS) with the one-sided test; Q_p = accumulated rotation.  One thread
-// per upper-triangle 2 x 2 block of a rotation round (KB (KB + 1) / 2 of them), so no thread
-// carries a second block on the round's critical path.
+template <typename T, int KB, int NTH>
+__device__ __forceinline__ void bj_update_body(const BjParams<T> &P, int p, int t0, int r, int par, double *smem);
+
+// Sweeps of S_p (FP64, in LDS) with the one-sided test; Q_p = accumulated rotation, written to the
+// round's parity slot.  One thread per upper-triangle 2 x 2 block of a rotation round (KB (KB + 1) / 2
+// of them), so no thread carries a second block on the round's critical path.  Blocks [npair, ...) of the launch (16-wide blocks only) apply round g - 1's rotations (tournament
+// round vr) to V, four 64-row tiles per workgroup, nvq workgroups per pair, beside the pair sweeps.
 template <typename T, int KB>
-__global__ __launch_bounds__(BjShape<KB>::INNER) void bj_inner_kernel(BjParams<T> P, int r, int g, int fslot) {
+__global__ __launch_bounds__(BjShape<KB>::INNER) void bj_inner_kernel(BjParams<T> P, int r, int g, int fslot, int vr,
+                                                                      int nvq) {
   constexpr int KS = BjShape<KB>::KS, LD = BjShape<KB>::LD, NTI = BjShape<KB>::INNER;
   extern __shared__ __align__(16) double smem_bj[];
   double2 *cs = reinterpret_cast<double2 *>(smem_bj);   // (c, s) of pair k of the round
@@ -157,6 +162,14 @@ __global__ __launch_bounds__(BjShape<KB>::INNER) void bj_inner_kernel(BjParams<T
   int2 *pq = reinterpret_cast<int2 *>(Qt + KS * LD);   // (p, q) of pair k
   const int p = blockIdx.x, tid = threadIdx.x;
   if (P.flag[4]) return;   // a sweep launched after the one that rotated nowhere
+  const int npair = P.nb / 2;
+  if constexpr (NTI % 256 == 0) {
+    if (p >= npair) {
+      const int b = p - npair;
+      bj_update_body<T, KB, NTI>(P, b / nvq, (b % nvq) * (NTI / 256), vr, (g - 1) & 1, smem_bj);
+      return;
+    }
+  }
   const double eps = Eps<T>::value, tol = P.tol > 0.0 ? P.tol : 10.0 * eps;
   // the reference's floor eps^2 max_j |w_j|^2 at the start of this round (primitives.py:337): the largest
   // diagonal the previous round's pair sweeps left (their column norms after its rotations), or for the
@@ -275,7 +288,7 @@ __global__ __launch_bounds__(BjShape<KB>::INNER) void bj_inner_kernel(BjParams<T
   if (any)
     for (int e = tid; e < KS * KS; e += NTI) {
       const int i = e / KS, j = e - i * KS;
-      P.Q[(size_t)p * KS * KS + e] = (T)Qt[j * LD + i];   // Q = Qt^T
+      P.Q[((size_t)(g & 1) * npair + p) * KS * KS + e] = (T)Qt[j * LD + i];   // Q = Qt^T
     }
   __syncthreads();
   if (tid < kWave) {   // the next round's floor: the largest column norm^2 after this round's rotations
@@ -283,7 +296,7 @@ __global__ __launch_bounds__(BjShape<KB>::INNER) void bj_inner_kernel(BjParams<T
     if (tid == 0) atomicMax(&P.fmx[(g + 1) & 1], (unsigned long long)__double_as_longlong(d));
   }
   if (tid == 0) {
-    P.act[p] = any;
+    P.act[(g & 1) * npair + p] = any;
     if (any) atomicOr(&P.flag[fslot], 1);
   }
 }
@@ -300,36 +313,51 @@ template <> struct BjMfma<double> {
   static __device__ __forceinline__ int row(int q, int lk) { return 4 * q + lk; }
 };
 
-// X[:, I u J] <- X[:, I u J] Q_p for a 64-row tile of W (tiles [0, mt)) or of V (tiles [mt, mt + nt)).
+// V[:, I u J] <- V[:, I u J] Q_p for NTH / 256 consecutive 64-row tiles t0, t0 + 1, ... of V (pair p of
+// tournament round r, rotation from parity slot par).
 constexpr int kTileRows = 64;
 
-template <typename T, int KB>
+template <typename T, int KB, int NTH>
 __host__ __device__ inline size_t bj_update_lds() {
-  return sizeof(T) * (size_t)(kTileRows + BjShape<KB>::KS) * BjShape<KB>::LD;
+  return sizeof(T) * (size_t)(kTileRows * (NTH / 256) + BjShape<KB>::KS) * BjShape<KB>::LD;
 }
 
-// The 64 x KS product on four waves: wave w owns rows 32 (w / 2) .. + 32 and columns KB (w % 2) .. + KB.
+// The pair-sweep launch: its V-tile workgroups (16-wide blocks) need more LDS than the sweeps.
 template <typename T, int KB>
-__device__ __forceinline__ void bj_update_body(const BjParams<T> &P, int p, int t, int r, double *smem) {
+__host__ __device__ inline size_t bj_inner_launch_lds() {
+  constexpr int NTI = BjShape<KB>::INNER;
+  const size_t a = bj_inner_lds<T, KB>();
+  if constexpr (NTI % 256 == 0) {
+    // more than half of a CU's 160 KiB: one workgroup per CU, so no V tiles share a CU with a pair sweep
+    const size_t b = bj_update_lds<T, KB, NTI>(), c = 81 * 1024;
+    return a > b ? (a > c ? a : c) : (b > c ? b : c);
+  }
+  return a;
+}
+
+// Each 64 x KS product on four waves of its 256-thread group: wave w owns rows 32 (w / 2) .. + 32 and
+// columns KB (w % 2) .. + KB.  Tiles past the last are zero rows that are not stored.
+template <typename T, int KB, int NTH>
+__device__ __forceinline__ void bj_update_body(const BjParams<T> &P, int p, int t0, int r, int par, double *smem) {
   constexpr int KS = BjShape<KB>::KS, LD = BjShape<KB>::LD, TJ = KB / 16;
-  T (*Xs)[LD] = reinterpret_cast<T (*)[LD]>(smem);
-  T (*Qs)[LD] = Xs + kTileRows;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  if (!P.act[p]) return;
+  static_assert(NTH % 256 == 0, "whole 256-thread groups");
+  const int npair = P.nb / 2;
+  if (!P.act[par * npair + p]) return;   // uniform over the workgroup
+  const int tid = threadIdx.x, qd = tid >> 8, qt = tid & 255, lane = tid & 63, wave = qt >> 6;
+  T (*Qs)[LD] = reinterpret_cast<T (*)[LD]>(smem);
+  T (*Xs)[LD] = Qs + KS + qd * kTileRows;
   const int lr = lane & 15, lk = lane >> 4, wm = (wave >> 1) * 32, wn = (wave & 1) * KB;
-  const int mt = (P.m + kTileRows - 1) / kTileRows;
-  T *X = t < mt ? P.W : P.V;
-  const int rows = t < mt ? P.m : P.N, r0 = (t < mt ? t : t - mt) * kTileRows;
+  const int r0 = (t0 + qd) * kTileRows;
   int I, J;
   rr_pair(r, p, P.nb, I, J);
   using V16 = Vec16<T>;
   constexpr int VE = V16::n, RV = KS / VE;   // 16-byte vectors: a row's two KB-column blocks
-  const V16 *Qp = reinterpret_cast<const V16 *>(P.Q + (size_t)p * KS * KS);
-  for (int e = tid; e < kTileRows * RV; e += kJbThreads) {
+  const V16 *Qp = reinterpret_cast<const V16 *>(P.Q + ((size_t)par * npair + p) * KS * KS);
+  for (int e = qt; e < kTileRows * RV; e += 256) {
     const int i = e / RV, j = (e - i * RV) * VE, row = r0 + i;
     V16 x;
-    if (row < rows) {
-      x = *reinterpret_cast<const V16 *>(X + (int64_t)row * P.N + bj_col<KB>(I, J, j));
+    if (row < P.N) {
+      x = *reinterpret_cast<const V16 *>(P.V + (int64_t)row * P.N + bj_col<KB>(I, J, j));
     } else {
 #pragma unroll
       for (int u = 0; u < VE; ++u) x.v[u] = T(0);
@@ -337,7 +365,7 @@ __device__ __forceinline__ void bj_update_body(const BjParams<T> &P, int p, int 
 #pragma unroll
     for (int u = 0; u < VE; ++u) Xs[i][j + u] = x.v[u];
   }
-  for (int e = tid; e < KS * RV; e += kJbThreads) {
+  for (int e = tid; e < KS * RV; e += NTH) {
     const int i = e / RV, j = (e - i * RV) * VE;
     const V16 y = Qp[e];
 #pragma unroll
@@ -371,13 +399,13 @@ __device__ __forceinline__ void bj_update_body(const BjParams<T> &P, int p, int 
 #pragma unroll
       for (int q = 0; q < 4; ++q) Xs[wm + 16 * i + BjMfma<T>::row(q, lk)][wn + 16 * j + lr] = (T)acc[i][j][q];
   __syncthreads();
-  for (int e = tid; e < kTileRows * RV; e += kJbThreads) {
+  for (int e = qt; e < kTileRows * RV; e += 256) {
     const int i = e / RV, j = (e - i * RV) * VE, row = r0 + i;
-    if (row < rows) {
+    if (row < P.N) {
       V16 x;
 #pragma unroll
       for (int u = 0; u < VE; ++u) x.v[u] = Xs[i][j + u];
-      *reinterpret_cast<V16 *>(X + (int64_t)row * P.N + bj_col<KB>(I, J, j)) = x;
+      *reinterpret_cast<V16 *>(P.V + (int64_t)row * P.N + bj_col<KB>(I, J, j)) = x;
     }
   }
 }
@@ -389,8 +417,8 @@ __device__ __forceinline__ void bj_update_body(const BjParams<T> &P, int p, int 
 // rows come straight from Wo).  Every block of round r is made by exactly one workgroup per slice, and
 // the old buffer is only read, so no workgroup reads a row another one writes.
 template <typename T, int KB>
-__device__ __forceinline__ void bj_gram_update_body(const BjParams<T> &P, int p, int z, int r, int rprev, const T *Wo,
-                                                    T *Wn, bool gram, double *smem) {
+__device__ __forceinline__ void bj_gram_update_body(const BjParams<T> &P, int p, int z, int r, int rprev, int qpar,
+                                                    const T *Wo, T *Wn, bool gram, double *smem) {
   constexpr int KR = kGramRows, KS = BjShape<KB>::KS, LD = BjShape<KB>::LD, TI = KB / 16;
   double (*Ws)[LD] = reinterpret_cast<double (*)[LD]>(smem);               // KR x KS, FP64 new rows
   T (*Xs)[KR][LD] = reinterpret_cast<T (*)[KR][LD]>(Ws + KR);              // [2][KR][KS] old pair rows per half
@@ -416,7 +444,7 @@ __device__ __forceinline__ void bj_gram_update_body(const BjParams<T> &P, int p,
       pp[h] = seat < P.nb - 1 - seat ? seat : P.nb - 1 - seat;
       rr_pair(rprev, pp[h], P.nb, pa[h], pb[h]);
       pside[h] = X == pa[h] ? 0 : 1;
-      pact[h] = P.act[pp[h]];
+      pact[h] = P.act[qpar * (P.nb / 2) + pp[h]];
     } else {
       pp[h] = 0;
       pa[h] = pb[h] = X;
@@ -436,9 +464,10 @@ __device__ __forceinline__ void bj_gram_update_body(const BjParams<T> &P, int p,
 #pragma unroll
     for (int j = 0; j < NPF; ++j) {
       const int e = tid + j * kJbThreads, h = e / (KR * RV), f = e - h * KR * RV, rr = f / RV, c = (f - rr * RV) * VE;
-      const int row = k0 + rr;
-      if (row < r1) {
-        pf[j] = *reinterpret_cast<const V16 *>(Wo + (int64_t)row * P.N + (c < KB ? pa[h] * KB + c : pb[h] * KB + c - KB));
+      const int row = k0 + rr, side = c < KB ? 0 : 1;
+      // an inactive (or absent) previous pair leaves X as it was: its partner's columns are not needed
+      if (row < r1 && (pact[h] || side == pside[h])) {
+        pf[j] = *reinterpret_cast<const V16 *>(Wo + (int64_t)row * P.N + (side == 0 ? pa[h] * KB + c : pb[h] * KB + c - KB));
       } else {
 #pragma unroll
         for (int u = 0; u < VE; ++u) pf[j].v[u] = T(0);
@@ -449,7 +478,7 @@ __device__ __forceinline__ void bj_gram_update_body(const BjParams<T> &P, int p,
   // the rotation halves (identity for an inactive or absent previous pair)
   for (int e = tid; e < 2 * KS * KB; e += kJbThreads) {
     const int h = e / (KS * KB), f = e - h * KS * KB, i = f / KB, j = f - i * KB;
-    Qs[h][i][j] = pact[h] ? P.Q[(size_t)pp[h] * KS * KS + (size_t)i * KS + pside[h] * KB + j]
+    Qs[h][i][j] = pact[h] ? P.Q[((size_t)qpar * (P.nb / 2) + pp[h]) * KS * KS + (size_t)i * KS + pside[h] * KB + j]
                           : (i == pside[h] * KB + j ? T(1) : T(0));
   }
   j4d acc[TI][TI];
@@ -534,30 +563,30 @@ __host__ __device__ inline size_t bj_fused_lds() {
 }
 
 // One launch per round g (tournament round r): workgroups [0, ngram) rebuild their row slice of the two
-// blocks of pair b % npair from round rprev's rotations (old buffer Wo -> new buffer Wn) and, with gram,
-// form that slice's Gram partial of round r; the rest apply round vr's rotations to the V tiles (pair,
-// tile) -- V is read by no kernel of the sweep, and Q and the activity flags still hold the previous
-// round until this round's pair sweeps run.  Workgroup 0 clears the floor slot the round's pair sweeps
-// will publish into (clear >= 0).  Inside a sweep (`in_sweep`) nothing is done once the done flag is set.
+// blocks of pair b % npair from round rprev's rotations (parity slot qpar; old buffer Wo -> new buffer
+// Wn) and, with gram, form that slice's Gram partial of round r; the rest apply round vr's rotations
+// (parity slot vpar) to the V tiles (pair, tile) -- V is read by no kernel of the sweep.  Workgroup 0
+// clears the floor slot the round's pair sweeps will publish into (clear >= 0).  Inside a sweep
+// (`in_sweep`) nothing is done once the done flag is set.
 template <typename T, int KB>
-__global__ __launch_bounds__(kJbThreads) void bj_fused_kernel(BjParams<T> P, int r, int rprev, const T *Wo, T *Wn,
-                                                             int ngram, int gram, int vr, int mt, int nt, int clear,
-                                                             int in_sweep) {
+__global__ __launch_bounds__(kJbThreads) void bj_fused_kernel(BjParams<T> P, int r, int rprev, int qpar, const T *Wo,
+                                                             T *Wn, int ngram, int gram, int vr, int vpar, int nt,
+                                                             int clear, int in_sweep) {
   extern __shared__ double smem_bf[];
   const int b = (int)blockIdx.x, npair = P.nb / 2;
   if (in_sweep && P.flag[4]) return;
   if (b == 0 && threadIdx.x == 0 && clear >= 0) P.fmx[clear] = 0ull;
   if (b < ngram) {
-    bj_gram_update_body<T, KB>(P, b % npair, b / npair, r, rprev, Wo, Wn, gram != 0, smem_bf);
+    bj_gram_update_body<T, KB>(P, b % npair, b / npair, r, rprev, qpar, Wo, Wn, gram != 0, smem_bf);
   } else {
     const int t = b - ngram;
-    bj_update_body<T, KB>(P, t / nt, mt + t % nt, vr, smem_bf);
+    bj_update_body<T, KB, kJbThreads>(P, t / nt, t % nt, vr, vpar, smem_bf);
   }
 }
 
 template <typename T, int KB>
 __host__ __device__ inline size_t bj_fused_launch_lds() {
-  const size_t a = bj_fused_lds<T, KB>(), b = bj_update_lds<T, KB>();
+  const size_t a = bj_fused_lds<T, KB>(), b = bj_update_lds<T, KB, kJbThreads>();
   return a > b ? a : b;
 }
 
@@ -677,8 +706,8 @@ size_t bj_ws_bytes_kb(int m, int n) {
   constexpr int KS = BjShape<KB>::KS;
   const int N = (n + KS - 1) / KS * KS, npair = N / KS;
   auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
-  return al(sizeof(T) * (size_t)m * N) * 2 + al(sizeof(T) * (size_t)N * N) + al(sizeof(T) * (size_t)npair * KS * KS) +
-         al(sizeof(double) * (size_t)npair * bj_splits(m, npair) * KS * KS) + al(sizeof(int) * npair) + al(8 * 2) +
+  return al(sizeof(T) * (size_t)m * N) * 2 + al(sizeof(T) * (size_t)N * N) + al(sizeof(T) * 2 * (size_t)npair * KS * KS) +
+         al(sizeof(double) * (size_t)npair * bj_splits(m, npair) * KS * KS) + al(sizeof(int) * 2 * npair) + al(8 * 2) +
          al(4 * 5) + al(8 * 2) + al(sizeof(double) * N) + al(sizeof(int) * N) + al(4) +
          al(psvd_complete_workspace_bytes(1, m, n));
 }
@@ -698,7 +727,7 @@ static int run_bj(int m, int n, const T *a, T *u, T *s, T *vh, int32_t *info, do
   static bool attr = false;
   if (!attr) {
     if (hipFuncSetAttribute((const void *)bj_inner_kernel<T, KB>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)bj_inner_lds<T, KB>()) != hipSuccess ||
+                            (int)bj_inner_launch_lds<T, KB>()) != hipSuccess ||
         hipFuncSetAttribute((const void *)bj_fused_kernel<T, KB>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)bj_fused_launch_lds<T, KB>()) != hipSuccess)
       return PSVD_ELAUNCH;
@@ -729,9 +758,9 @@ static int run_bj(int m, int n, const T *a, T *u, T *s, T *vh, int32_t *info, do
   P.W = reinterpret_cast<T *>(take(sizeof(T) * (size_t)m * P.N));
   P.W2 = reinterpret_cast<T *>(take(sizeof(T) * (size_t)m * P.N));
   P.V = reinterpret_cast<T *>(take(sizeof(T) * (size_t)P.N * P.N));
-  P.Q = reinterpret_cast<T *>(take(sizeof(T) * (size_t)npair * KS * KS));
+  P.Q = reinterpret_cast<T *>(take(sizeof(T) * 2 * (size_t)npair * KS * KS));
   P.Sp = reinterpret_cast<double *>(take(sizeof(double) * (size_t)npair * P.splits * KS * KS));
-  P.act = reinterpret_cast<int *>(take(sizeof(int) * npair));
+  P.act = reinterpret_cast<int *>(take(sizeof(int) * 2 * npair));
   P.slot = reinterpret_cast<unsigned long long *>(take(8 * 2));
   P.flag = reinterpret_cast<int *>(take(4 * 5));
   P.fmx = reinterpret_cast<unsigned long long *>(take(8 * 2));
@@ -739,7 +768,7 @@ static int run_bj(int m, int n, const T *a, T *u, T *s, T *vh, int32_t *info, do
   P.ord = reinterpret_cast<int *>(take(sizeof(int) * P.N));
   P.nk = reinterpret_cast<int32_t *>(take(4));
   void *cws = take(psvd_complete_workspace_bytes(1, m, n));
-  const int mt = (m + kTileRows - 1) / kTileRows, nt = (flags & PSVD_NO_VECTORS) ? 0 : (P.N + kTileRows - 1) / kTileRows;
+  const int nt = (flags & PSVD_NO_VECTORS) ? 0 : (P.N + kTileRows - 1) / kTileRows;   // 64-row tiles of V
   auto blocks = [](int64_t items) {
     const int64_t b = (items + kJbThreads - 1) / kJbThreads;
     return (unsigned)(b < 4096 ? (b > 0 ? b : 1) : 4096);
@@ -751,8 +780,13 @@ static int run_bj(int m, int n, const T *a, T *u, T *s, T *vh, int32_t *info, do
   // the first round's floor from the loaded columns; later rounds take it from the pair sweeps before them
   hipLaunchKernelGGL(bj_colmax_kernel<T>, dim3((n + 15) / 16), dim3(kJbThreads), 0, st, P);
   const int ngram = npair * P.splits, nvt = npair * nt;
+  // V tiles: with 16-wide blocks, four per workgroup of the pair-sweep launch (nvq per pair); else in the
+  // fused launch
+  constexpr bool kVInner = BjShape<KB>::INNER % 256 == 0;
+  constexpr int kTq = BjShape<KB>::INNER / 256;
+  const int nvq = kVInner ? (nt + kTq - 1) / kTq : 0;
   T *buf[2] = {P.W, P.W2};   // round g reads buf[(g - 1) & 1] (round g - 1's input) and writes buf[g & 1]
-  const size_t flds = bj_fused_launch_lds<T, KB>();
+  const size_t flds = bj_fused_launch_lds<T, KB>(), ilds = bj_inner_launch_lds<T, KB>();
   // Sweep sw + 1 is queued before the host reads whether sweep sw rotated, so the GPU does not idle
   // through the host's wait; when sweep sw rotated nowhere, sweep sw + 1's kernels find the done flag
   // set and return, and the solve continues from the state sweep sw left (its round count g).
@@ -763,10 +797,12 @@ static int run_bj(int m, int n, const T *a, T *u, T *s, T *vh, int32_t *info, do
       // round g - 1's rotations applied to W and this round's Gram partials, beside round g - 1's V tiles
       const int rprev = g == 0 ? -1 : (r > 0 ? r - 1 : P.nb - 2);
       const T *wo = g == 0 ? buf[0] : buf[(g - 1) & 1];
-      hipLaunchKernelGGL((bj_fused_kernel<T, KB>), dim3((unsigned)(ngram + (vr >= 0 ? nvt : 0))), dim3(kJbThreads), flds,
-                         st, P, r, rprev, wo, buf[g & 1], ngram, 1, vr, mt, nt, (g + 1) & 1, 1);
-      hipLaunchKernelGGL((bj_inner_kernel<T, KB>), dim3(npair), dim3(BjShape<KB>::INNER), (bj_inner_lds<T, KB>()), st,
-                         P, r, g, bj_flag_slot(sw));
+      const int par = (g - 1) & 1;   // round g - 1's rotations and activity flags
+      hipLaunchKernelGGL((bj_fused_kernel<T, KB>), dim3((unsigned)(ngram + (!kVInner && vr >= 0 ? nvt : 0))),
+                         dim3(kJbThreads), flds, st, P, r, rprev, par, wo, buf[g & 1], ngram, 1, vr, par, nt, (g + 1) & 1,
+                         1);
+      hipLaunchKernelGGL((bj_inner_kernel<T, KB>), dim3((unsigned)(npair + (kVInner && vr >= 0 ? npair * nvq : 0))),
+                         dim3(BjShape<KB>::INNER), ilds, st, P, r, g, bj_flag_slot(sw), vr, nvq);
       if (nt > 0) vr = r;
     }
     if (hipMemcpyAsync(host_flag, &P.flag[bj_flag_slot(sw)], sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
@@ -828,8 +864,9 @@ static int run_bj(int m, int n, const T *a, T *u, T *s, T *vh, int32_t *info, do
   if (rc != PSVD_OK) return rc;
   // the last round's rotations on W (no Gram) and V; the tail reads W from the buffer they go to
   if (g > 0) {
+    const int par = (g - 1) & 1;
     hipLaunchKernelGGL((bj_fused_kernel<T, KB>), dim3((unsigned)(ngram + (vr >= 0 ? nvt : 0))), dim3(kJbThreads), flds,
-                       st, P, 0, P.nb - 2, buf[(g - 1) & 1], buf[g & 1], ngram, 0, vr, mt, nt, -1, 0);
+                       st, P, 0, P.nb - 2, par, buf[par], buf[g & 1], ngram, 0, vr, par, nt, -1, 0);
     P.W = buf[g & 1];
   }
   hipLaunchKernelGGL(bj_sigma_kernel<T>, dim3((n + 15) / 16), dim3(kJbThreads), 0, st, P);
