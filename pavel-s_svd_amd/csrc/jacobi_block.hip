@@ -215,6 +215,16 @@ __global__ __launch_bounds__(BjShape<KB>::INNER) void bj_inner_kernel(BjParams<T
   // pairs across its two blocks (the pairs inside a block were treated in the first round)
   const bool cross = r > 0 && !P.full_pairs;
   const int nrr = cross ? KB : KS - 1;
+  // Qt threads start at a 16-lane row.  In the cross rounds pair k is (k, KB + (k + rr) mod KB), so
+  // row k of Qt stays with pair k and row KB + j passes from pair k to pair k - 1 each round: the
+  // thread of (pair k, column col) -- 16 consecutive lanes per column -- keeps Qt[k][col] and the
+  // Qt row its pair meets in registers and hands the latter to the next lower lane of its row with one
+  // DPP rotate; after a whole cross sweep (KB rounds) lane k holds row KB + k again.
+  constexpr int QB = (NTRI + 15) / 16 * 16;
+  static_assert(!kSplitQt || (KB == 16 && NTI >= QB + KB * KS), "register Qt needs 16-lane rows");
+  const bool qthr = kSplitQt && tid >= QB && tid < QB + KB * KS;
+  const int qe = tid - QB, tk = cross ? (qe & (KB - 1)) : qe / KS, tc = cross ? (qe >> 4) : qe - (qe / KS) * KS;
+  double qp = tk == tc ? 1.0 : 0.0, qq = KB + tk == tc ? 1.0 : 0.0;
   for (int sw = 0; live && sw < P.inner_sweeps; ++sw) {
     int act_sw = 0;
     for (int rr = 0; rr < nrr; ++rr) {
@@ -257,15 +267,23 @@ __global__ __launch_bounds__(BjShape<KB>::INNER) void bj_inner_kernel(BjParams<T
               S[q2 * LD + q1] = n22;
             }
           }
-        } else if (tid - NTRI < KB * KS) {
-          const int e = tid - NTRI, k = e / KS, col = e - k * KS;
-          int pp, qq;
-          sub_pair(cross, rr, k, KB, pp, qq);
-          const double2 rq = cs[k];
-          const double x = Qt[pp * LD + col], y = Qt[qq * LD + col];
-          if (rq.y != 0.0) {
-            Qt[pp * LD + col] = rq.x * x - rq.y * y;
-            Qt[qq * LD + col] = rq.y * x + rq.x * y;
+        } else if (qthr) {
+          const double2 rq = cs[tk];
+          if (cross) {
+            if (rq.y != 0.0) {
+              const double x = qp, y = qq;
+              qp = rq.x * x - rq.y * y;
+              qq = rq.y * x + rq.x * y;
+            }
+            qq = dpp_double<0x12F>(qq);   // row_ror:15 -- lane i takes lane i + 1 of its row
+          } else {
+            int pp, q2;
+            sub_pair(false, rr, tk, KB, pp, q2);
+            const double x = Qt[pp * LD + tc], y = Qt[q2 * LD + tc];
+            if (rq.y != 0.0) {
+              Qt[pp * LD + tc] = rq.x * x - rq.y * y;
+              Qt[q2 * LD + tc] = rq.y * x + rq.x * y;
+            }
           }
         }
       } else {
@@ -285,6 +303,11 @@ __global__ __launch_bounds__(BjShape<KB>::INNER) void bj_inner_kernel(BjParams<T
     any |= act_sw;
     if (!act_sw) break;
   }
+  if (cross && qthr && any) {   // the register rows back into Qt
+    Qt[tk * LD + tc] = qp;
+    Qt[(KB + tk) * LD + tc] = qq;
+  }
+  if (kSplitQt && cross) __syncthreads();
   if (any)
     for (int e = tid; e < KS * KS; e += NTI) {
       const int i = e / KS, j = e - i * KS;
