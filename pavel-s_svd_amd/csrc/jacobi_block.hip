@@ -148,10 +148,24 @@ __host__ __device__ inline size_t bj_inner_lds() {
 template <typename T, int KB, int NTH>
 __device__ __forceinline__ void bj_update_body(const BjParams<T> &P, int p, int t0, int r, int par, double *smem);
 
+// One 2 x 2 block of S under a rotation round: columns by pair 2's (c2, s2), then rows by pair 1's
+// (c1, s1), with the fused multiply-adds spelled out.
+__device__ __forceinline__ void bj_rot2x2(double c1, double s1, double c2, double s2, double b11, double b12,
+                                          double b21, double b22, double &n11, double &n12, double &n21,
+                                          double &n22) {
+  const double t11 = fma(c2, b11, -(s2 * b12)), t12 = fma(s2, b11, c2 * b12);
+  const double t21 = fma(c2, b21, -(s2 * b22)), t22 = fma(s2, b21, c2 * b22);
+  n11 = fma(c1, t11, -(s1 * t21));
+  n21 = fma(s1, t11, c1 * t21);
+  n12 = fma(c1, t12, -(s1 * t22));
+  n22 = fma(s1, t12, c1 * t22);
+}
+
 // Sweeps of S_p (FP64, in LDS) with the one-sided test; Q_p = accumulated rotation, written to the
 // round's parity slot.  One thread per upper-triangle 2 x 2 block of a rotation round (KB (KB + 1) / 2
-// of them), so no thread carries a second block on the round's critical path.  Blocks [npair, ...) of the launch (16-wide blocks only) apply round g - 1's rotations (tournament
-// round vr) to V, four 64-row tiles per workgroup, nvq workgroups per pair, beside the pair sweeps.
+// of them), so no thread carries a second block on the round's critical path.  Blocks [npair, ...) of
+// the launch (16-wide blocks only) apply round g - 1's rotations (tournament round vr) to V, four
+// 64-row tiles per workgroup, nvq workgroups per pair, beside the pair sweeps.
 template <typename T, int KB>
 __global__ __launch_bounds__(BjShape<KB>::INNER) void bj_inner_kernel(BjParams<T> P, int r, int g, int fslot, int vr,
                                                                       int nvq) {
@@ -231,7 +245,8 @@ __global__ __launch_bounds__(BjShape<KB>::INNER) void bj_inner_kernel(BjParams<T
       if (tid < KB) {
         int a, b;
         sub_pair(cross, rr, tid, KB, a, b);
-        const double app = S[a * LD + a], aqq = S[b * LD + b], apq = S[a * LD + b];
+        const int ra = __mul24(a, LD), rb = __mul24(b, LD);   // 24-bit products: full-rate multiplies
+        const double app = S[ra + a], aqq = S[rb + b], apq = S[ra + b];
         const bool act = apq * apq > tol * tol * app * aqq && fmin(app, aqq) > floor_;
         double c = 1.0, s = 0.0;
         if (act) jacobi_rotation_fast<T>(app, aqq, apq, c, s);
@@ -248,23 +263,23 @@ __global__ __launch_bounds__(BjShape<KB>::INNER) void bj_inner_kernel(BjParams<T
           int p1, q1, p2, q2;
           sub_pair(cross, rr, it1[0], KB, p1, q1);
           sub_pair(cross, rr, it2[0], KB, p2, q2);
+          // every load of the item goes out before the activity test, so the item costs one LDS round trip
           const double2 r1 = cs[it1[0]], r2 = cs[it2[0]];
-          const double b11 = S[p1 * LD + p2], b12 = S[p1 * LD + q2], b21 = S[q1 * LD + p2], b22 = S[q1 * LD + q2];
-          if (r1.y != 0.0 || r2.y != 0.0) {   // as block_update: columns (pair 2), then rows (pair 1)
-            const double c1 = r1.x, s1 = r1.y, c2 = r2.x, s2 = r2.y;
-            const double t11 = c2 * b11 - s2 * b12, t12 = s2 * b11 + c2 * b12;
-            const double t21 = c2 * b21 - s2 * b22, t22 = s2 * b21 + c2 * b22;
-            const double n11 = c1 * t11 - s1 * t21, n21 = s1 * t11 + c1 * t21;
-            const double n12 = c1 * t12 - s1 * t22, n22 = s1 * t12 + c1 * t22;
-            S[p1 * LD + p2] = n11;
-            S[p1 * LD + q2] = n12;
-            S[q1 * LD + p2] = n21;
-            S[q1 * LD + q2] = n22;
+          const int rp1 = __mul24(p1, LD), rq1 = __mul24(q1, LD), rp2 = __mul24(p2, LD), rq2 = __mul24(q2, LD);
+          const double b11 = S[rp1 + p2], b12 = S[rp1 + q2], b21 = S[rq1 + p2], b22 = S[rq1 + q2];
+          asm volatile("" ::"v"(b11), "v"(b12), "v"(b21), "v"(b22));   // keeps the loads ahead of the test
+          double n11, n12, n21, n22;
+          bj_rot2x2(r1.x, r1.y, r2.x, r2.y, b11, b12, b21, b22, n11, n12, n21, n22);
+          if (r1.y != 0.0 || r2.y != 0.0) {
+            S[rp1 + p2] = n11;
+            S[rp1 + q2] = n12;
+            S[rq1 + p2] = n21;
+            S[rq1 + q2] = n22;
             if (p1 != p2) {
-              S[p2 * LD + p1] = n11;
-              S[q2 * LD + p1] = n12;
-              S[p2 * LD + q1] = n21;
-              S[q2 * LD + q1] = n22;
+              S[rp2 + p1] = n11;
+              S[rq2 + p1] = n12;
+              S[rp2 + q1] = n21;
+              S[rq2 + q1] = n22;
             }
           }
         } else if (qthr) {
