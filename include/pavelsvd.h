@@ -325,7 +325,9 @@ int psvd_jacobi_grid(int dtype, int m, int n, const void *a, void *u, void *s, v
  * cyclic sweeps of the reference's one-sided rotations and test, the accumulated rotation applied
  * to W and V on MFMA; stops after a sweep without rotations.  Same flags as psvd_jacobi_grid plus
  * the block-width flags, same outputs and info (info[0] counts block sweeps).  One host read per
- * sweep (not capturable).
+ * sweep, with the next sweep already queued (when a sweep rotated nowhere, the queued one finds a
+ * done flag and does nothing); the flag words are 8 bytes of pinned host memory allocated on a host
+ * thread's first call and kept for the process.  Not capturable.
  * ------------------------------------------------------------------------------------- */
 size_t psvd_jacobi_blocked_workspace_bytes(int dtype, int m, int n);
 int psvd_jacobi_blocked(int dtype, int m, int n, const void *a, void *u, void *s, void *vh, int32_t *info, double tol,
