@@ -757,6 +757,14 @@ size_t bj_ws_bytes(int m, int n) {
   return a > b ? a : b;
 }
 
+// Pinned flag words for the asynchronous per-sweep reads: one pair per host thread, allocated on its
+// first call and kept for the process (null if the allocation fails).
+static int *bj_pinned_words() {
+  thread_local int *pinned = nullptr;
+  if (!pinned && hipHostMalloc((void **)&pinned, 2 * sizeof(int), hipHostMallocDefault) != hipSuccess) pinned = nullptr;
+  return pinned;
+}
+
 template <typename T, int KB>
 static int run_bj(int m, int n, const T *a, T *u, T *s, T *vh, int32_t *info, double tol, int max_sweeps, int flags,
                   int inner_sweeps, void *ws, size_t ws_bytes, hipStream_t st) {
@@ -848,12 +856,8 @@ static int run_bj(int m, int n, const T *a, T *u, T *s, T *vh, int32_t *info, do
       return false;
     return hipGetLastError() == hipSuccess;
   };
-  // pinned flag words for the asynchronous reads (one pair per host thread, kept for the process)
-  thread_local int *pinned = nullptr;
-  if (!pinned && hipHostMalloc((void **)&pinned, 2 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
-    pinned = nullptr;
-    return PSVD_ELAUNCH;
-  }
+  int *pinned = bj_pinned_words();
+  if (!pinned) return PSVD_ELAUNCH;
   // the two sweep events; on leaving, the host waits for the last queued flag read (the pinned words are
   // reused by this thread's next solve) and releases them
   struct SweepEvents {

@@ -89,3 +89,34 @@ def test_svd_dispatches_large_matrices_to_blocked(P):
     assert np.max(np.abs(_np(s) - _np(s2))) / float(s[0]) < 2e-6
     met = M.validate(a, (_np(u), _np(s), _np(vh)), M.reference(a))['worst']
     assert all(met[k] <= 1e-5 for k in KEYS), met
+
+
+def test_blocked_sweep_limits_queued_sweep_and_inner_sweeps(P):
+    """The next sweep is queued before the host reads the convergence flag: a solve capped at its own
+    sweep count runs the same rounds as the uncapped one (whose extra sweep turns into no-ops), so the
+    outputs are bit-identical; a solve stopped early still has A V = W exactly (U diag(s) V^T = A
+    to rounding, V orthogonal); two inner sweeps per pair and the full pair sweeps stay at LAPACK's
+    accuracy."""
+    rng = np.random.default_rng(11)
+    a = (rng.standard_normal((300, 128)) * np.logspace(0, -3, 128)).astype(np.float64)
+    x = torch.from_numpy(a[None]).to(DEV)
+    u, s, vh, info = P.jacobi_blocked(x, scale=True)
+    sw = int(info[0, 0])
+    assert int(info[0, 1]) == 0 and 2 < sw < 60
+    u2, s2, vh2, info2 = P.jacobi_blocked(x, scale=True, max_sweeps=sw)
+    assert _np(info2).tolist() == _np(info).tolist()
+    for t1, t2 in ((u, u2), (s, s2), (vh, vh2)):
+        assert torch.equal(t1, t2)
+    for cap in (1, 2):
+        u3, s3, vh3, info3 = P.jacobi_blocked(x, scale=True, max_sweeps=cap)
+        assert _np(info3[0]).tolist() == [cap, 1]   # PSVD_INFO_NOCONV
+        v3 = _np(vh3[0]).T
+        assert np.max(np.abs(v3.T @ v3 - np.eye(128))) < 1e-12
+        rec = (_np(u3[0]) * _np(s3[0])) @ _np(vh3[0])
+        assert np.linalg.norm(rec - a) / np.linalg.norm(a) < 1e-13
+    ref = M.reference(a)
+    for kw in (dict(inner_sweeps=2), dict(full_pairs=True), dict(inner_sweeps=2, full_pairs=True)):
+        u4, s4, vh4, info4 = P.jacobi_blocked(x, scale=True, **kw)
+        assert int(info4[0, 1]) == 0, kw
+        met = M.validate(a, (_np(u4[0]), _np(s4[0]), _np(vh4[0])), ref)['worst']
+        assert all(met[k] <= 1e-12 for k in KEYS), (kw, met)
