@@ -730,9 +730,10 @@ __global__ void bj_reset_kernel(unsigned long long *slot, int *flag, unsigned lo
   }
 }
 
-// Row slices per pair: about 1024 workgroups per round, each over at least one 64-row chunk.
+// Row slices per pair: about 512 workgroups per round (fewer partials for the pair sweeps to add than
+// with 1024, more overlap of chunks than with 256), each over at least one 64-row chunk.
 static int bj_splits(int m, int npair) {
-  int s = (1024 + npair - 1) / npair;
+  int s = (512 + npair - 1) / npair;
   const int smax = (m + kGramRows - 1) / kGramRows;
   if (s > smax) s = smax;
   if (s > 64) s = 64;
