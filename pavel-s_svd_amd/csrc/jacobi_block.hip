@@ -192,8 +192,16 @@ __global__ __launch_bounds__(BjShape<KB>::INNER) void bj_inner_kernel(BjParams<T
   for (int e = tid; e < KS * KS; e += NTI) {
     const int i = e / KS, j = e - i * KS;
     const double *sp = P.Sp + (size_t)p * P.splits * KS * KS + e;
-    double acc = sp[0];
-    for (int z = 1; z < P.splits; ++z) acc += sp[(size_t)z * KS * KS];
+    // the slices' partials added in slice order, eight loads in flight at a time
+    double acc = 0.0;
+    for (int z0 = 0; z0 < P.splits; z0 += 8) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = z0 + u < P.splits ? sp[(size_t)(z0 + u) * KS * KS] : 0.0;
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (z0 + u < P.splits) acc = z0 + u == 0 ? v[u] : acc + v[u];
+    }
     S[i * LD + j] = acc;
     Qt[i * LD + j] = i == j ? 1.0 : 0.0;
   }
@@ -502,10 +510,11 @@ __device__ __forceinline__ void bj_gram_update_body(const BjParams<T> &P, int p,
 #pragma unroll
     for (int j = 0; j < NPF; ++j) {
       const int e = tid + j * kJbThreads, h = e / (KR * RV), f = e - h * KR * RV, rr = f / RV, c = (f - rr * RV) * VE;
-      const int row = k0 + rr, side = c < KB ? 0 : 1;
-      // an inactive (or absent) previous pair leaves X as it was: its partner's columns are not needed
-      if (row < r1 && (pact[h] || side == pside[h])) {
-        pf[j] = *reinterpret_cast<const V16 *>(Wo + (int64_t)row * P.N + (side == 0 ? pa[h] * KB + c : pb[h] * KB + c - KB));
+      const int row = k0 + rr;
+      // read whether or not the previous pair rotated (its rotation half is then the identity): no load
+      // waits for the activity flags
+      if (row < r1) {
+        pf[j] = *reinterpret_cast<const V16 *>(Wo + (int64_t)row * P.N + (c < KB ? pa[h] * KB + c : pb[h] * KB + c - KB));
       } else {
 #pragma unroll
         for (int u = 0; u < VE; ++u) pf[j].v[u] = T(0);
@@ -513,11 +522,28 @@ __device__ __forceinline__ void bj_gram_update_body(const BjParams<T> &P, int p,
     }
   };
   fetch(r0);
-  // the rotation halves (identity for an inactive or absent previous pair)
-  for (int e = tid; e < 2 * KS * KB; e += kJbThreads) {
-    const int h = e / (KS * KB), f = e - h * KS * KB, i = f / KB, j = f - i * KB;
-    Qs[h][i][j] = pact[h] ? P.Q[((size_t)qpar * (P.nb / 2) + pp[h]) * KS * KS + (size_t)i * KS + pside[h] * KB + j]
-                          : (i == pside[h] * KB + j ? T(1) : T(0));
+  // the rotation halves (identity for an inactive or absent previous pair): every thread's vectors are
+  // loaded unconditionally and together (an inactive pair's slot holds an older round's rotation, which
+  // the select discards), then stored
+  {
+    constexpr int QV = KB / VE, NQ = 2 * KS * QV, NQT = (NQ + kJbThreads - 1) / kJbThreads;
+    V16 qv[NQT];
+#pragma unroll
+    for (int j = 0; j < NQT; ++j) {
+      const int e = tid + j * kJbThreads, h = e / (KS * QV), f = e - h * KS * QV, i = f / QV, jv = (f - i * QV) * VE;
+      if (e < NQ && rprev >= 0)
+        qv[j] = *reinterpret_cast<const V16 *>(P.Q + ((size_t)qpar * (P.nb / 2) + pp[h]) * KS * KS + (size_t)i * KS +
+                                               pside[h] * KB + jv);
+    }
+#pragma unroll
+    for (int j = 0; j < NQT; ++j) {
+      const int e = tid + j * kJbThreads, h = e / (KS * QV), f = e - h * KS * QV, i = f / QV, jv = (f - i * QV) * VE;
+      if (e < NQ) {
+#pragma unroll
+        for (int u = 0; u < VE; ++u)
+          Qs[h][i][jv + u] = pact[h] ? qv[j].v[u] : (i == pside[h] * KB + jv + u ? T(1) : T(0));
+      }
+    }
   }
   j4d acc[TI][TI];
 #pragma unroll
