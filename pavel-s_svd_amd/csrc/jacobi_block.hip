@@ -27,8 +27,11 @@ namespace psvd {
 
 constexpr int kJbThreads = 256;
 
-// the rotation flag of sweep sw (1-based): odd sweeps [0], even sweeps [3]
-__host__ __device__ inline int bj_flag_slot(int sw) { return (sw & 1) ? 0 : 3; }
+// the rotation flag of sweep sw (1-based), by sw mod 3: [3], [4], [0] for remainders 1, 2, 0
+__host__ __device__ inline int bj_flag_slot(int sw) {
+  const int k = sw % 3;
+  return k == 1 ? 3 : (k == 2 ? 4 : 0);
+}
 
 // Block width KB: subproblem size KS, LDS row stride, threads of the pair-sweep kernel (one
 // upper-triangle 2 x 2 block of a rotation round per thread: 528 for KB = 32, 136 for KB = 16; for
@@ -54,8 +57,8 @@ struct BjParams {
   double *Sp;               // npair x splits x 64 x 64 Gram partials
   int *act;                 // [2][npair] by round parity: the pair rotated in its sweep
   unsigned long long *slot;  // [0] max|a| bits, [1] max column norm^2 bits (this sweep)
-  int *flag;                // [0] / [3] rotation in an odd / even sweep, [1] non-finite input, [2] completion
-                            // status, [4] done: a sweep without a rotation came before (the kernels return)
+  int *flag;                // [3] / [4] / [0] rotation in sweep sw with sw mod 3 = 1 / 2 / 0, [1] non-finite
+                            // input, [2] completion status
   double *sig;              // N
   int *ord;                 // N
   int32_t *nk;              // 1
@@ -168,14 +171,14 @@ __device__ __forceinline__ void bj_rot2x2(double c1, double s1, double c2, doubl
 // 64-row tiles per workgroup, nvq workgroups per pair, beside the pair sweeps.
 template <typename T, int KB>
 __global__ __launch_bounds__(BjShape<KB>::INNER) void bj_inner_kernel(BjParams<T> P, int r, int g, int fslot, int vr,
-                                                                      int nvq) {
+                                                                      int nvq, int done_slot) {
   constexpr int KS = BjShape<KB>::KS, LD = BjShape<KB>::LD, NTI = BjShape<KB>::INNER;
   extern __shared__ __align__(16) double smem_bj[];
   double2 *cs = reinterpret_cast<double2 *>(smem_bj);   // (c, s) of pair k of the round
   double *S = reinterpret_cast<double *>(cs + KB), *Qt = S + KS * LD;
   int2 *pq = reinterpret_cast<int2 *>(Qt + KS * LD);   // (p, q) of pair k
   const int p = blockIdx.x, tid = threadIdx.x;
-  if (P.flag[4]) return;   // a sweep launched after the one that rotated nowhere
+  if (done_slot >= 0 && P.flag[done_slot] == 0) return;   // the previous sweep rotated nowhere
   const int npair = P.nb / 2;
   if constexpr (NTI % 256 == 0) {
     if (p >= npair) {
@@ -630,16 +633,20 @@ __host__ __device__ inline size_t bj_fused_lds() {
 // blocks of pair b % npair from round rprev's rotations (parity slot qpar; old buffer Wo -> new buffer
 // Wn) and, with gram, form that slice's Gram partial of round r; the rest apply round vr's rotations
 // (parity slot vpar) to the V tiles (pair, tile) -- V is read by no kernel of the sweep.  Workgroup 0
-// clears the floor slot the round's pair sweeps will publish into (clear >= 0).  Inside a sweep
-// (`in_sweep`) nothing is done once the done flag is set.
+// clears the floor slot the round's pair sweeps will publish into (clear >= 0) and the rotation flag
+// slot clr_flag (>= 0: the next sweep's, in a sweep's first round).  Nothing is done when the previous
+// sweep's flag (done_slot >= 0) says it rotated nowhere.
 template <typename T, int KB>
 __global__ __launch_bounds__(kJbThreads) void bj_fused_kernel(BjParams<T> P, int r, int rprev, int qpar, const T *Wo,
                                                              T *Wn, int ngram, int gram, int vr, int vpar, int nt,
-                                                             int clear, int in_sweep) {
+                                                             int clear, int done_slot, int clr_flag) {
   extern __shared__ double smem_bf[];
   const int b = (int)blockIdx.x, npair = P.nb / 2;
-  if (in_sweep && P.flag[4]) return;
-  if (b == 0 && threadIdx.x == 0 && clear >= 0) P.fmx[clear] = 0ull;
+  if (done_slot >= 0 && P.flag[done_slot] == 0) return;
+  if (b == 0 && threadIdx.x == 0) {
+    if (clear >= 0) P.fmx[clear] = 0ull;
+    if (clr_flag >= 0) P.flag[clr_flag] = 0;
+  }
   if (b < ngram) {
     bj_gram_update_body<T, KB>(P, b % npair, b / npair, r, rprev, qpar, Wo, Wn, gram != 0, smem_bf);
   } else {
@@ -740,19 +747,12 @@ __global__ void bj_status_kernel(int32_t *info, const int *cstatus) {
   if (threadIdx.x == 0 && info[1] == PSVD_INFO_OK && *cstatus != 0) info[1] = PSVD_INFO_COMPLETION;
 }
 
-// what & 1: the solve's start (every slot and flag cleared); what & 2: the start of sweep `sw`, whose
-// rotation flag is cleared, and which is marked done when sweep sw - 1 rotated nowhere
-__global__ void bj_reset_kernel(unsigned long long *slot, int *flag, unsigned long long *fmx, int what, int sw) {
+// the solve's start: every slot and flag cleared
+__global__ void bj_reset_kernel(unsigned long long *slot, int *flag, unsigned long long *fmx) {
   if (threadIdx.x == 0) {
-    if (what & 1) {
-      slot[0] = slot[1] = 0ull;
-      fmx[0] = fmx[1] = 0ull;
-      for (int i = 0; i < 5; ++i) flag[i] = 0;
-    }
-    if (what & 2) {
-      if (sw > 1 && flag[bj_flag_slot(sw - 1)] == 0) flag[4] = 1;
-      flag[bj_flag_slot(sw)] = 0;
-    }
+    slot[0] = slot[1] = 0ull;
+    fmx[0] = fmx[1] = 0ull;
+    for (int i = 0; i < 5; ++i) flag[i] = 0;
   }
 }
 
@@ -846,7 +846,7 @@ static int run_bj(int m, int n, const T *a, T *u, T *s, T *vh, int32_t *info, do
     const int64_t b = (items + kJbThreads - 1) / kJbThreads;
     return (unsigned)(b < 4096 ? (b > 0 ? b : 1) : 4096);
   };
-  hipLaunchKernelGGL(bj_reset_kernel, dim3(1), dim3(64), 0, st, P.slot, P.flag, P.fmx, 1, 0);
+  hipLaunchKernelGGL(bj_reset_kernel, dim3(1), dim3(64), 0, st, P.slot, P.flag, P.fmx);
   hipLaunchKernelGGL(bj_absmax_kernel<T>, dim3(blocks((int64_t)m * n)), dim3(kJbThreads), 0, st, P);
   hipLaunchKernelGGL(bj_load_kernel<T>, dim3(blocks((int64_t)m * P.N + (int64_t)P.N * P.N)), dim3(kJbThreads), 0, st,
                      P);
@@ -861,11 +861,13 @@ static int run_bj(int m, int n, const T *a, T *u, T *s, T *vh, int32_t *info, do
   T *buf[2] = {P.W, P.W2};   // round g reads buf[(g - 1) & 1] (round g - 1's input) and writes buf[g & 1]
   const size_t flds = bj_fused_launch_lds<T, KB>(), ilds = bj_inner_launch_lds<T, KB>();
   // Sweep sw + 1 is queued before the host reads whether sweep sw rotated, so the GPU does not idle
-  // through the host's wait; when sweep sw rotated nowhere, sweep sw + 1's kernels find the done flag
-  // set and return, and the solve continues from the state sweep sw left (its round count g).
+  // through the host's wait; when sweep sw rotated nowhere, sweep sw + 1's kernels find its flag clear
+  // and return, and the solve continues from the state sweep sw left (its round count g).  The flags
+  // rotate over three slots: sweep sw's first launch clears sweep sw + 1's slot, which sweep sw - 2
+  // used and whose read by the host is queued before.
   int g = 0, vr = -1;
   auto enqueue_sweep = [&](int sw, int *host_flag, hipEvent_t ev) {
-    hipLaunchKernelGGL(bj_reset_kernel, dim3(1), dim3(64), 0, st, P.slot, P.flag, P.fmx, 2, sw);
+    const int done_slot = sw > 1 ? bj_flag_slot(sw - 1) : -1;
     for (int r = 0; r < P.nb - 1; ++r, ++g) {
       // round g - 1's rotations applied to W and this round's Gram partials, beside round g - 1's V tiles
       const int rprev = g == 0 ? -1 : (r > 0 ? r - 1 : P.nb - 2);
@@ -873,9 +875,9 @@ static int run_bj(int m, int n, const T *a, T *u, T *s, T *vh, int32_t *info, do
       const int par = (g - 1) & 1;   // round g - 1's rotations and activity flags
       hipLaunchKernelGGL((bj_fused_kernel<T, KB>), dim3((unsigned)(ngram + (!kVInner && vr >= 0 ? nvt : 0))),
                          dim3(kJbThreads), flds, st, P, r, rprev, par, wo, buf[g & 1], ngram, 1, vr, par, nt, (g + 1) & 1,
-                         1);
+                         done_slot, r == 0 ? bj_flag_slot(sw + 1) : -1);
       hipLaunchKernelGGL((bj_inner_kernel<T, KB>), dim3((unsigned)(npair + (kVInner && vr >= 0 ? npair * nvq : 0))),
-                         dim3(BjShape<KB>::INNER), ilds, st, P, r, g, bj_flag_slot(sw), vr, nvq);
+                         dim3(BjShape<KB>::INNER), ilds, st, P, r, g, bj_flag_slot(sw), vr, nvq, done_slot);
       if (nt > 0) vr = r;
     }
     if (hipMemcpyAsync(host_flag, &P.flag[bj_flag_slot(sw)], sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
@@ -935,7 +937,7 @@ static int run_bj(int m, int n, const T *a, T *u, T *s, T *vh, int32_t *info, do
   if (g > 0) {
     const int par = (g - 1) & 1;
     hipLaunchKernelGGL((bj_fused_kernel<T, KB>), dim3((unsigned)(ngram + (vr >= 0 ? nvt : 0))), dim3(kJbThreads), flds,
-                       st, P, 0, P.nb - 2, par, buf[par], buf[g & 1], ngram, 0, vr, par, nt, -1, 0);
+                       st, P, 0, P.nb - 2, par, buf[par], buf[g & 1], ngram, 0, vr, par, nt, -1, -1, -1);
     P.W = buf[g & 1];
   }
   hipLaunchKernelGGL(bj_sigma_kernel<T>, dim3((n + 15) / 16), dim3(kJbThreads), 0, st, P);
