@@ -181,9 +181,9 @@ __device__ __forceinline__ void rr_pair(int r, int k, int np, int &p, int &q) {
 // and every later round only across them still treats every index pair once per sweep, as the
 // scalar cyclic method does, in about half the rotation rounds.
 __device__ __forceinline__ void sub_pair(bool cross, int rr, int k, int KB, int &a, int &b) {
-  if (cross) {
+  if (cross) {   // KB is a power of two and k + rr >= 0: the remainder is a mask
     a = k;
-    b = KB + (k + rr) % KB;
+    b = KB + ((k + rr) & (KB - 1));
   } else {
     rr_pair(rr, k, 2 * KB, a, b);
   }
