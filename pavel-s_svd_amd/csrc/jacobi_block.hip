@@ -145,11 +145,38 @@ constexpr int kGramRows = 64;   // rows of W staged per step of the Gram partial
 template <typename T, int KB>
 __host__ __device__ inline size_t bj_inner_lds() {
   constexpr int KS = BjShape<KB>::KS, LD = BjShape<KB>::LD;
-  return sizeof(double) * (2 * KB + 2 * (size_t)KS * LD) + sizeof(int) * 2 * KB;
+  return sizeof(double) * (4 * KB + 3 * (size_t)KS * LD) + sizeof(int) * 2 * KB;
 }
 
 template <typename T, int KB, int NTH>
 __device__ __forceinline__ void bj_update_body(const BjParams<T> &P, int p, int t0, int r, int par, double *smem);
+
+// The pair rotation of the sweeps (primitives.py:340-347) with the shortest dependent chain: for FP32
+// data tau and t in FP32 (as the LDS kernel forms them: native rcp/sqrt, FP64 quotient outside the FP32
+// range), c = 1/sqrt(1 + t^2) and s = t c in FP64 (rsq plus one Newton step); FP64 data takes the
+// FP64 rotation.
+template <typename T>
+__device__ __forceinline__ void bj_rotation(double app, double aqq, double apq, double &c, double &s) {
+  if constexpr (sizeof(T) == 4) {
+    const double a2 = 2.0 * fabs(apq), dd = aqq - app;
+    float tau;
+    if (a2 > 1e-30 && a2 < 1e30 && fabs(dd) < 1e37)
+      tau = (float)dd * __builtin_amdgcn_rcpf((float)a2);
+    else
+      tau = (float)(dd / a2);
+    const float at = fabsf(tau);
+    float t = at > 1e18f ? 0.5f * __builtin_amdgcn_rcpf(at)
+                         : __builtin_amdgcn_rcpf(at + __builtin_amdgcn_sqrtf(fmaf(at, at, 1.0f)));
+    t = copysignf(t, tau);
+    const double td = (double)t, x = fma(td, td, 1.0);
+    double y = __builtin_amdgcn_rsq(x);
+    y = y * fma(-0.5 * x * y, y, 1.5);
+    c = y;
+    s = apq < 0.0 ? -td * y : td * y;
+  } else {
+    jacobi_rotation_fast<T>(app, aqq, apq, c, s);
+  }
+}
 
 // One 2 x 2 block of S under a rotation round: columns by pair 2's (c2, s2), then rows by pair 1's
 // (c1, s1), with the fused multiply-adds spelled out.
@@ -177,6 +204,8 @@ __global__ __launch_bounds__(BjShape<KB>::INNER) void bj_inner_kernel(BjParams<T
   double2 *cs = reinterpret_cast<double2 *>(smem_bj);   // (c, s) of pair k of the round
   double *S = reinterpret_cast<double *>(cs + KB), *Qt = S + KS * LD;
   int2 *pq = reinterpret_cast<int2 *>(Qt + KS * LD);   // (p, q) of pair k
+  double2 *cs2 = reinterpret_cast<double2 *>(pq + KB);   // cross rounds: the other (c, s) buffer
+  double *S2 = reinterpret_cast<double *>(cs2 + KB);     // cross rounds: the other S buffer
   const int p = blockIdx.x, tid = threadIdx.x;
   if (done_slot >= 0 && P.flag[done_slot] == 0) return;   // the previous sweep rotated nowhere
   const int npair = P.nb / 2;
@@ -240,17 +269,112 @@ __global__ __launch_bounds__(BjShape<KB>::INNER) void bj_inner_kernel(BjParams<T
   // pairs across its two blocks (the pairs inside a block were treated in the first round)
   const bool cross = r > 0 && !P.full_pairs;
   const int nrr = cross ? KB : KS - 1;
-  // Qt threads start at a 16-lane row.  In the cross rounds pair k is (k, KB + (k + rr) mod KB), so
+  // Qt threads start at a wave.  In the cross rounds pair k is (k, KB + (k + rr) mod KB), so
   // row k of Qt stays with pair k and row KB + j passes from pair k to pair k - 1 each round: the
   // thread of (pair k, column col) -- 16 consecutive lanes per column -- keeps Qt[k][col] and the
   // Qt row its pair meets in registers and hands the latter to the next lower lane of its row with one
   // DPP rotate; after a whole cross sweep (KB rounds) lane k holds row KB + k again.
-  constexpr int QB = (NTRI + 15) / 16 * 16;
+  constexpr int QB = (NTRI + kWave - 1) / kWave * kWave;
   static_assert(!kSplitQt || (KB == 16 && NTI >= QB + KB * KS), "register Qt needs 16-lane rows");
   const bool qthr = kSplitQt && tid >= QB && tid < QB + KB * KS;
   const int qe = tid - QB, tk = cross ? (qe & (KB - 1)) : qe / KS, tc = cross ? (qe >> 4) : qe - (qe / KS) * KS;
   double qp = tk == tc ? 1.0 : 0.0, qq = KB + tk == tc ? 1.0 : 0.0;
-  for (int sw = 0; live && sw < P.inner_sweeps; ++sw) {
+  bool swept = false;   // the cross rounds below ran the sweeps
+  if constexpr (kSplitQt) {
+    if (cross) {
+      // Cross rounds, one barrier per rotation round.  While the blocks update (from S into the other S
+      // buffer, every entry written) and Qt turns in registers, the rotation threads form the next
+      // round's rotations: pair k of round rr + 1 meets the q column of pair k + 1 of round rr, so its
+      // three inputs are entries of round rr's blocks (k, k), (k + 1, k + 1) and (k, k + 1) (for
+      // k = KB - 1 the mirror entry of block (0, KB - 1)), formed from the same loads by the same
+      // arithmetic as the block threads form them.  Threads DB .. DB + KB - 2 (one wave) serve pairs
+      // 0 .. KB - 2 and the first thread of the next wave pair KB - 1, whose entry is the mirror one.
+      constexpr int DB = (QB + KB * KS + kWave - 1) / kWave * kWave;
+      static_assert(NTI >= DB + kWave + 1, "the rotation threads fit in the workgroup");
+      const int rk_ = tid == DB + kWave ? KB - 1 : (tid >= DB && tid < DB + KB - 1 ? tid - DB : -1);
+      int act_sw = 0, act_next = 0;
+      if (live && tid < KB) {   // round 0's rotations
+        const int a = tid, b = KB + tid, ra = __mul24(a, LD), rb = __mul24(b, LD);
+        const double app = S[ra + a], aqq = S[rb + b], apq = S[ra + b];
+        const bool act = apq * apq > tol * tol * app * aqq && fmin(app, aqq) > floor_;
+        double c = 1.0, sn = 0.0;
+        if (act) bj_rotation<T>(app, aqq, apq, c, sn);
+        cs[tid] = make_double2(c, act ? sn : 0.0);
+        act_sw = act;
+      }
+      __syncthreads();
+      for (int sw = 0; live && sw < P.inner_sweeps; ++sw) {
+        if (sw > 0) {
+          act_sw = act_next;
+          act_next = 0;
+        }
+        for (int rr = 0; rr < KB; ++rr) {
+          const double2 *csc = (rr & 1) ? cs2 : cs;
+          double2 *csn = (rr & 1) ? cs : cs2;
+          const double *Sc = (rr & 1) ? S2 : S;
+          double *Sn = (rr & 1) ? S : S2;
+          if (tid < NTRI) {
+            const int k1 = it1[0], k2 = it2[0];
+            const int p1 = k1, q1 = KB + ((k1 + rr) & (KB - 1)), p2 = k2, q2 = KB + ((k2 + rr) & (KB - 1));
+            const int rp1 = __mul24(p1, LD), rq1 = __mul24(q1, LD), rp2 = __mul24(p2, LD), rq2 = __mul24(q2, LD);
+            const double2 r1 = csc[k1], r2 = csc[k2];
+            const double b11 = Sc[rp1 + p2], b12 = Sc[rp1 + q2], b21 = Sc[rq1 + p2], b22 = Sc[rq1 + q2];
+            double n11, n12, n21, n22;
+            bj_rot2x2(r1.x, r1.y, r2.x, r2.y, b11, b12, b21, b22, n11, n12, n21, n22);
+            Sn[rp1 + p2] = n11;
+            Sn[rp1 + q2] = n12;
+            Sn[rq1 + p2] = n21;
+            Sn[rq1 + q2] = n22;
+            if (p1 != p2) {
+              Sn[rp2 + p1] = n11;
+              Sn[rq2 + p1] = n12;
+              Sn[rp2 + q1] = n21;
+              Sn[rq2 + q1] = n22;
+            }
+          } else if (qthr) {
+            const double2 rq = csc[tk];
+            if (rq.y != 0.0) {
+              const double x = qp, y = qq;
+              qp = rq.x * x - rq.y * y;
+              qq = rq.y * x + rq.x * y;
+            }
+            qq = dpp_double<0x12F>(qq);   // row_ror:15 -- lane i takes lane i + 1 of its row
+          } else if (rk_ >= 0) {
+            const int k = rk_, kn = (k + 1) & (KB - 1);
+            const int pk = k, qk = KB + ((k + rr) & (KB - 1)), pn = kn, qn = KB + ((kn + rr) & (KB - 1));
+            const int rpk = __mul24(pk, LD), rqk = __mul24(qk, LD), rpn = __mul24(pn, LD), rqn = __mul24(qn, LD);
+            const double2 rk = csc[k], rn = csc[kn];
+            const double a11 = Sc[rpk + pk], a12 = Sc[rpk + qk], a21 = Sc[rqk + pk], a22 = Sc[rqk + qk];
+            const double b11 = Sc[rpn + pn], b12 = Sc[rpn + qn], b21 = Sc[rqn + pn], b22 = Sc[rqn + qn];
+            double app, aqq, apq, x12, x21, x22;
+            bj_rot2x2(rk.x, rk.y, rk.x, rk.y, a11, a12, a21, a22, app, x12, x21, x22);
+            bj_rot2x2(rn.x, rn.y, rn.x, rn.y, b11, b12, b21, b22, x12, x21, x22, aqq);
+            if (k < KB - 1) {   // block (k, k + 1): its n12
+              const double c11 = Sc[rpk + pn], c12 = Sc[rpk + qn], c21 = Sc[rqk + pn], c22 = Sc[rqk + qn];
+              bj_rot2x2(rk.x, rk.y, rn.x, rn.y, c11, c12, c21, c22, x22, apq, x12, x21);
+            } else {            // block (0, KB - 1): the n21 its mirror stores
+              const double c11 = Sc[rpn + pk], c12 = Sc[rpn + qk], c21 = Sc[rqn + pk], c22 = Sc[rqn + qk];
+              bj_rot2x2(rn.x, rn.y, rk.x, rk.y, c11, c12, c21, c22, x22, x12, apq, x21);
+            }
+            const bool act = apq * apq > tol * tol * app * aqq && fmin(app, aqq) > floor_;
+            double c = 1.0, sn = 0.0;
+            if (act) bj_rotation<T>(app, aqq, apq, c, sn);
+            csn[k] = make_double2(c, act ? sn : 0.0);
+            if (rr + 1 < KB)
+              act_sw |= act;
+            else
+              act_next |= act;   // round 0 of the next sweep
+          }
+          __syncthreads();
+        }
+        act_sw = __syncthreads_or(act_sw);
+        any |= act_sw;
+        if (!act_sw) break;
+      }
+      swept = true;
+    }
+  }
+  for (int sw = 0; !swept && live && sw < P.inner_sweeps; ++sw) {
     int act_sw = 0;
     for (int rr = 0; rr < nrr; ++rr) {
       if (tid < KB) {
@@ -260,7 +384,7 @@ __global__ __launch_bounds__(BjShape<KB>::INNER) void bj_inner_kernel(BjParams<T
         const double app = S[ra + a], aqq = S[rb + b], apq = S[ra + b];
         const bool act = apq * apq > tol * tol * app * aqq && fmin(app, aqq) > floor_;
         double c = 1.0, s = 0.0;
-        if (act) jacobi_rotation_fast<T>(app, aqq, apq, c, s);
+        if (act) bj_rotation<T>(app, aqq, apq, c, s);
         cs[tid] = make_double2(c, act ? s : 0.0);
         if constexpr (!kSplitQt) pq[tid] = make_int2(a, b);
         act_sw |= act;
