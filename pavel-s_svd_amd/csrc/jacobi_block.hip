@@ -313,7 +313,11 @@ __global__ __launch_bounds__(BjShape<KB>::INNER) void bj_inner_kernel(BjParams<T
           double2 *csn = (rr & 1) ? cs : cs2;
           const double *Sc = (rr & 1) ? S2 : S;
           double *Sn = (rr & 1) ? S : S2;
+          // the rotation threads set the round's length (about 1280 cycles against 680 for a block item
+          // and 410 for a Qt item): the block and Qt threads start 128 and 192 cycles later, so the
+          // rotation threads' LDS loads are served first (512x256: 8.8 -> 8.5 ms)
           if (tid < NTRI) {
+            __builtin_amdgcn_s_sleep(2);
             const int k1 = it1[0], k2 = it2[0];
             const int p1 = k1, q1 = KB + ((k1 + rr) & (KB - 1)), p2 = k2, q2 = KB + ((k2 + rr) & (KB - 1));
             const int rp1 = __mul24(p1, LD), rq1 = __mul24(q1, LD), rp2 = __mul24(p2, LD), rq2 = __mul24(q2, LD);
@@ -332,6 +336,7 @@ __global__ __launch_bounds__(BjShape<KB>::INNER) void bj_inner_kernel(BjParams<T
               Sn[rq2 + q1] = n22;
             }
           } else if (qthr) {
+            __builtin_amdgcn_s_sleep(3);
             const double2 rq = csc[tk];
             if (rq.y != 0.0) {
               const double x = qp, y = qq;
