@@ -759,7 +759,10 @@ def svd(a, algorithm='jacobi', *, full_matrices=False, values_only=False, return
             u, vh = u[:, :, :rank], vh[:, :rank, :]
         s = s[:, :rank]
         outs = (s,) if values_only else (u, s, vh)
-        if not all(bool(torch.isfinite(z).all()) for z in outs):
+        finite = torch.isfinite(outs[0]).all()
+        for z in outs[1:]:
+            finite = finite & torch.isfinite(z).all()
+        if not bool(finite):   # one host read for all outputs
             raise RuntimeError('standalone solver produced nonfinite output')
         result = (None if values_only else u.reshape(*shape, *u.shape[-2:]),
                   s.reshape(*shape, s.shape[-1]),
