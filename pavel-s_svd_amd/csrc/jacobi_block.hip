@@ -3,16 +3,20 @@
 // The columns of W (m x N, N = n padded to a multiple of 2 KB with zero columns that never rotate)
 // are grouped in blocks of KB = 16 or 32; round r of a tournament over the nb blocks pairs block I
 // with block J.  For each pair p the KS x KS (KS = 2 KB) Gram matrix S_p = [W_I W_J]^T [W_I W_J] is
-// formed in FP64 on MFMA
-// (row slices of W, per-slice partials added in a fixed order), and S_p is swept by `inner_sweeps`
-// cyclic sweeps of two-sided plane rotations in LDS -- the rotation of pair (a, b) of S is exactly
-// the one-sided rotation of columns a and b of W (app = s_aa, aqq = s_bb, apq = s_ab), with the
-// reference's test apq^2 > tol^2 app aqq and min(app, aqq) > eps^2 max_j |w_j|^2
-// (primitives.py:337-338; the column-norm maximum is taken at the start of each sweep).  The
-// accumulated KS x KS rotation Q_p then updates W[:, I u J] and V[:, I u J] on MFMA.  A sweep without
-// a rotation ends the iteration (primitives.py:352).  The tail -- FP64 sigma, descending order, cut
-// at eps max(m, n) sigma_1 (or eps sigma_1), U = W / sigma, coordinate-vector completion, row-major
-// outputs -- follows primitives.py:356-366 and :430-455.
+// formed in FP64 on MFMA (row slices of W, per-slice partials added in a fixed order), and S_p is
+// swept by `inner_sweeps` cyclic sweeps of two-sided plane rotations in LDS -- the rotation of pair
+// (a, b) of S is exactly the one-sided rotation of columns a and b of W (app = s_aa, aqq = s_bb,
+// apq = s_ab), with the reference's test apq^2 > tol^2 app aqq and min(app, aqq) > eps^2 max_j |w_j|^2
+// (primitives.py:337-338; the column-norm maximum of the round's start, published by the previous
+// round's pair sweeps).  The accumulated KS x KS rotation Q_p then updates W[:, I u J] and V[:, I u J]
+// on MFMA.  A sweep without a rotation ends the iteration (primitives.py:352).  The tail -- FP64
+// sigma, descending order, cut at eps max(m, n) sigma_1 (or eps sigma_1), U = W / sigma,
+// coordinate-vector completion, row-major outputs -- follows primitives.py:356-366 and :430-455.
+//
+// Two launches per round g: bj_fused_kernel applies round g - 1's rotations to W (double-buffered)
+// and forms round g's Gram partials from the rotated rows in the same workgroup; bj_inner_kernel runs
+// the pair sweeps (one 1024-thread workgroup per pair; one barrier per rotation round in the cross
+// rounds) beside round g - 1's V tiles.  The host queues sweep s + 1 before it reads sweep s's flag.
 //
 // Work per round: 2 m KS^2 FP64 MFMA flops per pair for the Gram, 2 (m + N) KS^2 for the updates;
 // nb - 1 rounds per sweep, so the work per sweep does not depend on KB.  What KB sets is the
