@@ -365,10 +365,13 @@ __global__ __launch_bounds__(BjShape<KB>::INNER) void bj_inner_kernel(BjParams<T
               const double c11 = Sc[rpn + pk], c12 = Sc[rpn + qk], c21 = Sc[rqn + pk], c22 = Sc[rqn + qk];
               bj_rot2x2(rn.x, rn.y, rk.x, rk.y, c11, c12, c21, c22, x22, x12, apq, x21);
             }
+            // the rotation is formed whether or not the pair passes the test (an inactive pair's
+            // possibly non-finite values are discarded by the select), so the test and the rotation's
+            // chain overlap
             const bool act = apq * apq > tol * tol * app * aqq && fmin(app, aqq) > floor_;
-            double c = 1.0, sn = 0.0;
-            if (act) bj_rotation<T>(app, aqq, apq, c, sn);
-            csn[k] = make_double2(c, act ? sn : 0.0);
+            double c, sn;
+            bj_rotation<T>(app, aqq, apq, c, sn);
+            csn[k] = act ? make_double2(c, sn) : make_double2(1.0, 0.0);
             if (rr + 1 < KB)
               act_sw |= act;
             else
