@@ -81,11 +81,25 @@ __global__ __launch_bounds__(kJbThreads) void bj_absmax_kernel(BjParams<T> P) {
     bad |= !isfinite(x);
     mx = fmax(mx, fabs(x));
   }
+  // the workgroup's maximum (one atomic per workgroup: the launch has at most 256 of them)
+  __shared__ double wmx[kJbThreads / kWave];
+  __shared__ int wbad[kJbThreads / kWave];
   mx = wave_max_fast(mx);
   const bool wave_bad = __any(bad);
   if ((threadIdx.x & (kWave - 1)) == 0) {
-    atomicMax(&P.slot[0], (unsigned long long)__double_as_longlong(mx));
-    if (wave_bad) atomicOr(&P.flag[1], 1);
+    wmx[threadIdx.x / kWave] = mx;
+    wbad[threadIdx.x / kWave] = wave_bad;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m2 = wmx[0];
+    int b2 = wbad[0];
+    for (int w = 1; w < kJbThreads / kWave; ++w) {
+      m2 = fmax(m2, wmx[w]);
+      b2 |= wbad[w];
+    }
+    atomicMax(&P.slot[0], (unsigned long long)__double_as_longlong(m2));
+    if (b2) atomicOr(&P.flag[1], 1);
   }
 }
 
@@ -832,15 +846,23 @@ __global__ __launch_bounds__(kJbThreads) void bj_sigma_kernel(BjParams<T> P) {
 
 template <typename T>
 __global__ __launch_bounds__(kJbThreads) void bj_order_kernel(BjParams<T> P) {
+  // rank of sigma_j among all n (ties by index), the sigma read in LDS chunks of kChunk
+  constexpr int kChunk = 2048;
+  __shared__ double sc[kChunk];
   const int j = blockIdx.x * kJbThreads + threadIdx.x;
-  if (j >= P.n) return;
-  const double sj = P.sig[j];
+  const double sj = j < P.n ? P.sig[j] : 0.0;
   int rank = 0;
-  for (int k = 0; k < P.n; ++k) {
-    const double sk = P.sig[k];
-    rank += (sk > sj) || (sk == sj && k < j);
+  for (int k0 = 0; k0 < P.n; k0 += kChunk) {
+    const int len = P.n - k0 < kChunk ? P.n - k0 : kChunk;
+    __syncthreads();
+    for (int e = threadIdx.x; e < len; e += kJbThreads) sc[e] = P.sig[k0 + e];
+    __syncthreads();
+    for (int k = 0; k < len; ++k) {
+      const double sk = sc[k];
+      rank += (sk > sj) || (sk == sj && k0 + k < j);
+    }
   }
-  P.ord[rank] = j;
+  if (j < P.n) P.ord[rank] = j;
 }
 
 // s, U (kept columns W / sigma, others zero for the completion), Vh, nk, info
@@ -983,7 +1005,8 @@ static int run_bj(int m, int n, const T *a, T *u, T *s, T *vh, int32_t *info, do
     return (unsigned)(b < 4096 ? (b > 0 ? b : 1) : 4096);
   };
   hipLaunchKernelGGL(bj_reset_kernel, dim3(1), dim3(64), 0, st, P.slot, P.flag, P.fmx);
-  hipLaunchKernelGGL(bj_absmax_kernel<T>, dim3(blocks((int64_t)m * n)), dim3(kJbThreads), 0, st, P);
+  hipLaunchKernelGGL(bj_absmax_kernel<T>, dim3(blocks((int64_t)m * n) < 256 ? blocks((int64_t)m * n) : 256u),
+                     dim3(kJbThreads), 0, st, P);
   hipLaunchKernelGGL(bj_load_kernel<T>, dim3(blocks((int64_t)m * P.N + (int64_t)P.N * P.N)), dim3(kJbThreads), 0, st,
                      P);
   // the first round's floor from the loaded columns; later rounds take it from the pair sweeps before them
