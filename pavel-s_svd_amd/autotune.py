@@ -111,11 +111,12 @@ def _tune_classes(batch, policy, rounds, verbose, key, gain=0.97):
 SCHEDULES = [dict(ordering=o, slots=sl) for o in ('tournament', 'sorted') for sl in ('identity', 'spread')]
 
 
-def _tune_schedule(batch, params, rounds, verbose, key, gain=0.98):
+def _tune_schedule(batch, params, rounds, verbose, key, gain=0.98, slot_gain=0.995):
     """Third stage: with the launch shape fixed, the pair ordering (the reference's tournament or
     the norm-sorted one) and the slot order of a round's pairs, each timed in alternation with the
-    current best over `rounds` rounds, with the batch's sweep counts recorded.  A schedule replaces
-    the reference's (tournament, identity) only when its median beats it by more than 1 - gain."""
+    current best over `rounds` rounds, with the batch's sweep counts recorded.  Another ordering
+    replaces the reference's tournament only when its median beats it by more than 1 - gain; within
+    the chosen ordering the slot order is taken at a margin of 1 - slot_gain."""
     rows, medians = [], {}
     samples = {i: [] for i in range(len(SCHEDULES))}
     for _ in range(rounds):
@@ -131,9 +132,15 @@ def _tune_schedule(batch, params, rounds, verbose, key, gain=0.98):
         if verbose:
             print(f'autotune {key} schedule {sched}: {medians[i]:.3f} ms, sweeps {rows[-1]["sweeps_mean"]:.2f}',
                   file=sys.stderr, flush=True)
+    # the ordering changes the rotations (and the sweep counts): it replaces the reference's only by
+    # more than 1 - gain; the slot order only changes which lanes serve a pair (bit-identical
+    # results), so within the chosen ordering the faster slot order is taken above a 0.5 % margin
     best = 0
     for i in range(1, len(SCHEDULES)):
-        if medians[i] < gain * medians[best]:
+        if SCHEDULES[i]['ordering'] != SCHEDULES[best]['ordering'] and medians[i] < gain * medians[best]:
+            best = i
+    for i in range(len(SCHEDULES)):
+        if SCHEDULES[i]['ordering'] == SCHEDULES[best]['ordering'] and medians[i] < slot_gain * medians[best]:
             best = i
     return dict(SCHEDULES[best]), medians[best], rows
 
