@@ -120,3 +120,16 @@ def test_blocked_sweep_limits_queued_sweep_and_inner_sweeps(P):
         assert int(info4[0, 1]) == 0, kw
         met = M.validate(a, (_np(u4[0]), _np(s4[0]), _np(vh4[0])), ref)['worst']
         assert all(met[k] <= 1e-12 for k in KEYS), (kw, met)
+
+
+def test_blocked_wide_rank_sort_chunks(P):
+    """n above the 2048-value LDS chunk of the sigma rank sort (two chunks) and m * n above what the
+    max|a| launch covers in one pass per thread: sigma against LAPACK, sorted descending."""
+    rng = np.random.default_rng(21)
+    a = rng.standard_normal((1, 2200, 2100)).astype(np.float32)
+    _, s, _, info = P.jacobi_blocked(torch.from_numpy(a).to(DEV), vectors=False, scale=True)
+    assert int(info[0, 1]) == 0
+    sv = _np(s[0])
+    ref = np.linalg.svd(a[0].astype(np.float64), compute_uv=False)
+    assert np.all(np.diff(sv) <= 0)
+    assert np.max(np.abs(sv - ref)) / ref[0] < 1e-5
