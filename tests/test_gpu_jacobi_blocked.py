@@ -133,3 +133,22 @@ def test_blocked_wide_rank_sort_chunks(P):
     ref = np.linalg.svd(a[0].astype(np.float64), compute_uv=False)
     assert np.all(np.diff(sv) <= 0)
     assert np.max(np.abs(sv - ref)) / ref[0] < 1e-5
+
+
+@pytest.mark.parametrize('width', [16, 32])
+def test_blocked_unscaled_fp32_extremes(P, width):
+    """Unscaled FP32 data near 1e-25 and 1e20 (jacobi_blocked does not scale by default): the FP64
+    Gram entries leave the FP32 range, the rotation angle then takes its FP64 quotient, and U, sigma
+    and V stay finite and equal to the unit-scale problem's."""
+    rng = np.random.default_rng(33)
+    a = rng.standard_normal((1, 300, 144)).astype(np.float32)
+    ref = np.linalg.svd(a[0].astype(np.float64), compute_uv=False)
+    for factor in (1e-25, 1e20):
+        x = torch.from_numpy(a * np.float32(factor)).to(DEV)
+        u, s, vh, info = P.jacobi_blocked(x, block_width=width)
+        assert int(info[0, 1]) == 0, factor
+        un, sv, vn = _np(u[0]).astype(np.float64), _np(s[0]).astype(np.float64), _np(vh[0]).astype(np.float64)
+        assert np.isfinite(un).all() and np.isfinite(sv).all() and np.isfinite(vn).all(), factor
+        assert np.max(np.abs(sv / factor - ref)) / ref[0] < 1e-5, factor
+        assert np.linalg.norm(un.T @ un - np.eye(144)) / np.sqrt(144) < 1e-5, factor
+        assert np.linalg.norm(vn @ vn.T - np.eye(144)) / np.sqrt(144) < 1e-5, factor
